@@ -1,0 +1,154 @@
+"""Recipe for oracle/_ref: the reference's own brick leaf kernels, compiled from
+their sources where they lie under the reference tree, plus the test harness
+oracle/ref/harness_brick.F, linked into oracle/_ref/libref_brick.so.
+
+Test infrastructure only.  Runs in the build container (amdflang from ROCm);
+outputs go to oracle/_ref/ only.  Module dependencies are resolved by compiling
+each missing module's defining file from the reference tree, recursively.
+
+Flags follow the reference's LLVM-flang double-precision recipe
+(engine/CMake_Compilers/cmake_linuxa64.txt:83-103): -DMYREAL8, fixed-form with
+unlimited line length, no FMA contraction, no fast-math.
+"""
+import glob
+import os
+import re
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "_ref")
+REF = os.environ.get("ORAD_REFERENCE_ROOT", "/root/reference")
+
+LEAF = [
+    "engine/source/elements/solid/solide/srcoor3.F",
+    "engine/source/elements/solid/solide/srepiso3.F",
+    "engine/source/elements/solid/solide/sortho3.F",
+    "engine/source/elements/solid/solide/srrota3.F",
+    "engine/source/elements/solid/solide/sderi3.F",
+    "engine/source/elements/solid/solide4/schkjabt3.F",
+    "engine/source/elements/solid/solide/sdlen3.F",
+    "engine/source/elements/solid/solide/slen.F",
+    "engine/source/elements/solid/solide/sdefo3.F",
+    "engine/source/elements/solid/solide/srho3.F",
+    "engine/source/elements/solid/solide/srmall3.F",
+    "engine/source/materials/mat/mat002/m2law.F",
+    "engine/source/materials/mat/mat002/m2iter_imp.F",
+    "engine/source/materials/mat_share/mqviscb.F",
+    "engine/source/materials/mat_share/mstrain_rate.F",
+    "engine/source/elements/solid/solide/shvis3.F",
+    "engine/source/elements/solid/solide/sfint3.F",
+    "engine/source/elements/solid/solide/scumu3p.F",
+]
+
+
+def flags():
+    inc = [
+        f"-I{REF}/engine/share/includes", f"-I{REF}/engine/share/spe_inc",
+        f"-I{REF}/engine/share/r8", f"-I{REF}/common_source/includes",
+        f"-I{REF}/engine/share/message_module", f"-J{OUT}", f"-I{OUT}",
+    ]
+    return ["amdflang", "-c", "-fPIC", "-cpp", "-ffixed-line-length-none",
+            "-DMYREAL8", "-DCPP_mach=CPP_p4linux964", "-O2",
+            "-ffp-contract=off", "-fno-fast-math"] + inc
+
+
+def module_index():
+    idx = {}
+    for root in ("common_source", "engine/share", "engine/source"):
+        for f in glob.glob(f"{REF}/{root}/**/*.F*", recursive=True):
+            try:
+                txt = open(f, errors="ignore").read()
+            except OSError:
+                continue
+            for m in re.findall(r"^\s*module\s+(\w+)\s*$", txt, re.I | re.M):
+                idx.setdefault(m.lower(), f)
+    return idx
+
+
+def main():
+    if not os.path.isdir(REF):
+        print(f"reference tree {REF} absent: oracle/_ref not built")
+        return 0
+    os.makedirs(OUT, exist_ok=True)
+    # Start clean: a stale .mod would hide its object from the link list.
+    for f in glob.glob(os.path.join(OUT, "*.o")) + glob.glob(os.path.join(OUT, "*.mod")):
+        os.remove(f)
+    fl = flags()
+    idx = module_index()
+    objs, done = [], set()
+
+    def comp(src):
+        if src in done:
+            return
+        obj = os.path.join(OUT, re.sub(r"[^A-Za-z0-9_]", "_",
+                                       os.path.relpath(src, REF)) + ".o")
+        for _ in range(64):
+            p = subprocess.run(fl + [src, "-o", obj], capture_output=True, text=True)
+            if p.returncode == 0:
+                done.add(src)
+                objs.append(obj)
+                return
+            missing = re.findall(r"Cannot parse module file for module '(\w+)'", p.stderr)
+            if not missing:
+                raise RuntimeError(f"compile failed: {src}\n{p.stderr[:3000]}")
+            for m in missing:
+                if m.lower() not in idx:
+                    raise RuntimeError(f"module {m} has no source in the reference tree")
+                comp(idx[m.lower()])
+        raise RuntimeError(f"dependency loop on {src}")
+
+    for rel in LEAF:
+        comp(os.path.join(REF, rel))
+    comp(os.path.join(HERE, "ref", "harness_brick.F"))
+    lib = os.path.join(OUT, "libref_brick.so")
+    # Routines referenced on branches the path does not take (error messages,
+    # SPH, orthotropic frames) are compiled from their own reference sources too,
+    # until the library has no unresolved Fortran symbol.
+    # ANCMSG (message catalogue) and ARRET (abort, restart flush) are only reached
+    # on error branches and pull in the whole I/O layer, including C++ sources that
+    # need headers absent from the image; their references are left weak, so a
+    # call would fault loudly rather than run a substitute.
+    subs = subroutine_index()
+    for _ in range(32):
+        subprocess.run(["amdflang", "-shared", "-o", lib] + objs, check=True)
+        und = subprocess.run(["nm", "-u", lib], capture_output=True, text=True).stdout.split()
+        need = sorted({s[:-1] for s in und if re.fullmatch(r"[a-z][a-z0-9_]*_", s)} - {""})
+        need = [s for s in need if s in subs and s not in ERROR_PATH]
+        if not need:
+            break
+        for s in need:
+            comp(subs[s])
+    weak = []
+    for s in ERROR_PATH:
+        weak += ["--weaken-symbol", s + "_"]
+    for o in objs:
+        subprocess.run(["objcopy"] + weak + [o], check=True)
+    subprocess.run(["amdflang", "-shared", "-o", lib] + objs, check=True)
+    und = subprocess.run(["nm", "-u", lib], capture_output=True, text=True).stdout.splitlines()
+    left = [ln.split()[-1] for ln in und
+            if ln.split()[0] == "U" and re.fullmatch(r"[a-z][a-z0-9_]*_", ln.split()[-1])]
+    if left:
+        raise RuntimeError(f"unresolved reference routines: {left}")
+    print(f"built {lib} from {len(objs)} objects")
+    return 0
+
+
+ERROR_PATH = ("ancmsg", "arret")
+
+
+def subroutine_index():
+    idx = {}
+    for root in ("engine/source", "common_source", "engine/share"):
+        for f in glob.glob(f"{REF}/{root}/**/*.F*", recursive=True):
+            try:
+                txt = open(f, errors="ignore").read()
+            except OSError:
+                continue
+            for m in re.findall(r"^\s+(?:recursive\s+)?subroutine\s+(\w+)", txt, re.I | re.M):
+                idx.setdefault(m.lower(), f)
+    return idx
+
+
+if __name__ == "__main__":
+    sys.exit(main())

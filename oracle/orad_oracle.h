@@ -1,0 +1,109 @@
+/*
+ * orad_oracle.h -- CPU restatement of the OpenRadioss explicit brick path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the HIP product
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it.  Nothing in openradioss_amd/ links or calls it.
+ *
+ * Semantics follow the reference's double-precision build (my_real = double,
+ * IRESP = 0) for the default solid:  Isolid=1 (JHBE=1), Iframe automatic
+ * (JCVT=1, co-rotational, starter/source/elements/solid/solide/sgrtails.F:857-859),
+ * Ismstr=4, law 2 Johnson-Cook with viscous hourglass, PARITH/ON assembly.
+ * Every function cites the reference routine it restates.
+ */
+#ifndef ORAD_ORACLE_H
+#define ORAD_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Law-2 parameters (starter/source/materials/mat/mat002/hm_read_mat02_jc.F90:146-192,319-339). */
+typedef struct {
+  double rho0, shear, bulk;       /* mat_param%rho, %shear, %bulk           */
+  double ca, cb, cn;              /* uparam(1..3)  A, B, n                   */
+  double epmx, sigmx;             /* uparam(4,5)   EPS_max, SIG_max          */
+  double cc, epdr;                /* uparam(6,7)   C, EPS0                   */
+  double fisokin;                 /* uparam(8)     (kinematic hardening, 0)  */
+  double mexp;                    /* uparam(10)    thermal exponent m        */
+  int    icc, vp, israte, pad;    /* iparam(2..4)                            */
+} orad_law2_t;
+
+/* Solid property + time-step control (hm_read_prop14.F:348-362, lectur.F:410-416). */
+typedef struct {
+  double qa, qb;                  /* GEO(14), GEO(15) bulk viscosity          */
+  double hvis;                    /* GEO(13) hourglass coefficient h          */
+  double dtfac;                   /* DTFAC1(1)                                */
+} orad_sprop_t;
+
+/* Element state of one brick (ELBUF GBUF for NPT=1, elbufdef_mod.F90:728-1069). */
+typedef struct {
+  double sig[6];                  /* co-rotational-frame stress xx yy zz xy yz zx */
+  double eint;                    /* internal energy per unit initial volume      */
+  double rho;                     /* current density                              */
+  double vol0;                    /* GBUF%VOL (initial volume)                    */
+  double pla;                     /* equivalent plastic strain                    */
+  double qvis;                    /* bulk-viscosity pressure                      */
+  double offg;                    /* GBUF%OFF                                     */
+  double epsd;                    /* equivalent strain rate (output)              */
+  double dmg;                     /* EPS_max reached flag                         */
+} orad_bstate_t;
+
+/* Per-element outputs of the element chain. */
+typedef struct {
+  double f[3][8];                 /* global corner forces F1j,F2j,F3j (j=node)    */
+  double sti;                     /* STI after SCUMU3P's FOURTH scaling           */
+  double dtx;                     /* DTFAC1 * DTX (element time-step candidate)   */
+  double ehou;                    /* hourglass energy increment                   */
+  double voln;                    /* current volume                               */
+  int    mstop;                   /* 1 if a non-positive volume was found         */
+  int    pad;
+} orad_bout_t;
+
+/* SFORC3 chain for one element (engine/source/elements/solid/solide/sforc3.F:125). */
+void oracle_brick_element(const double xg[3][8], const double vg[3][8],
+                          orad_bstate_t *st, double dt1,
+                          const orad_law2_t *mat, const orad_sprop_t *prop,
+                          orad_bout_t *out);
+
+/* Initial volume as the starter computes it (same determinant as SDERI3, global frame). */
+double oracle_brick_volume(const double xg[3][8]);
+
+/* ---- PARITH/ON element-corner CSR (FILLCNE + W_PON + GET_FSKY_ADDRESS) ---- */
+/* ixs: 8*numels 0-based node ids, element-major ([e*8+k]); uid: user ids.
+ * adsky: numnod+1 (0-based offsets); iads: 8*numels slot of (e,k).
+ * Returns 0 on success. */
+int oracle_build_pon(int numnod, int numels, const int32_t *ixs,
+                     const int32_t *uid, int32_t *adsky, int32_t *iads);
+
+/* ---- Whole-model explicit cycle (RESOL order, resol.F:2586-9595) ---- */
+typedef struct {
+  int numnod, numels;
+  double *x, *v, *d;              /* 3*numnod, node-major (Fortran X(3,NUMNOD))   */
+  const double *ms;               /* numnod                                       */
+  const int32_t *icodt;           /* numnod, BCS1V code (x=4,y=2,z=1)             */
+  const int32_t *ixs;             /* 8*numels, 0-based                            */
+  const int32_t *adsky, *iads;    /* PARITH/ON CSR                                */
+  orad_bstate_t *st;              /* numels                                       */
+  double *fsky;                   /* 4*8*numels: fx fy fz sti per slot            */
+  double *a, *stifn;              /* 3*numnod, numnod (assembly outputs)          */
+  orad_law2_t mat;
+  orad_sprop_t prop;
+  double tt, dt1, dt2, dt12, dt2old, dtmx;
+  double ehour;                   /* cumulative hourglass energy                  */
+  int    ncycle, mstop;
+  int    nthreads;                /* OpenMP threads (0 = runtime default)         */
+  int    pad;
+} orad_omodel_t;
+
+/* One full cycle: element forces -> assembly -> dt -> accel -> BC -> vel -> disp. */
+void oracle_cycle(orad_omodel_t *m);
+
+/* Energies: internal sum(EINT*V0), kinetic 0.5*sum(m v^2). */
+void oracle_energies(const orad_omodel_t *m, double *eint, double *ekin);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
