@@ -1,0 +1,149 @@
+"""Benchmark: explicit brick cycles on one MI355X.
+
+Workload: the large brick deck -- 200x200x200 under-integrated hexahedra
+(8,000,000 elements, 8,120,601 nodes), 1 mm spacing with +-5% seeded jitter,
+law-2 Johnson-Cook steel, Isolid=1 co-rotational, bottom face fixed, top node
+layer at -10 m/s.  A step is one full explicit cycle (element forces, PARITH/ON
+assembly, time step, accelerations, boundary conditions, velocities,
+displacements) over the whole deck, replayed from a captured HIP graph with the
+deck resident in HBM.
+
+The JSON line reports the mixed form (FP32 element arithmetic, FP64 nodes) as
+`value`, the all-FP64 form beside it, the element kernel's roofline position
+(algorithmic bytes per launch / mean launch time from HIP events on the
+library stream), and the oracle timed on the host cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from openradioss_amd import deck as dk  # noqa: E402
+from openradioss_amd import engine as en  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def elem_bytes(ne, nn, tsize):
+    """Algorithmic HBM bytes of one element-kernel launch: every array element it
+    must touch, once, at its stored width; nodes once per node (ideal reuse)."""
+    reads = 8 * 4 + 8 * 4 + 13 * tsize     # conn, IADS, sig(6) eint rho vol0 pla qvis offg epsd
+    writes = 11 * tsize + 8 * 4 * tsize    # sig(6) eint rho pla qvis epsd, 8 FSKY slots
+    return ne * (reads + writes) + nn * (32 + 32)   # X4, V4 records
+
+
+def node_bytes(ne, nn, tsize):
+    """Algorithmic bytes of one nodal-kernel launch."""
+    slots = 8 * ne * 4 * tsize                                        # every FSKY slot read once
+    per_node = 8 + 32 + 32 + 24 + 32 + 32 + 24                        # adsky, X4/V4/D read + write
+    return slots + nn * per_node
+
+
+def run(prec, deck, steps, warmup):
+    s = en.BrickSolver(deck, prec)
+    s.run_cycles(warmup)
+    t0 = time.perf_counter()
+    t = s.run_cycles(steps)                    # synchronises on return
+    t1 = time.perf_counter()
+    el_ms, nd_ms = s.profile_cycles(20)
+    return s, (t1 - t0), t, el_ms, nd_ms
+
+
+def cpu_baseline(deck, cycles):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    cores = len(os.sched_getaffinity(0))
+    m = po.Model(deck, nthreads=cores)
+    m.cycle(1)
+    t0 = time.perf_counter()
+    m.cycle(cycles)
+    dt = time.perf_counter() - t0
+    ne = deck["ixs"].shape[0]
+    return dict(value=ne * cycles / dt, unit="element-cycles/s", cores=cores, kind="port",
+                sample=f"{cycles} cycles of the same {ne}-brick deck, FP64 C oracle, OpenMP")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=200, help="bricks per edge of the block")
+    ap.add_argument("--cpu-cycles", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the slab decomposition is described in DESIGN.md)", file=sys.stderr)
+        return 2
+    n = args.n
+    deck = dk.brick_block(n, n, n)
+    ne, nn = deck["ixs"].shape[0], deck["x"].shape[0]
+    print(f"deck {n}^3: {ne} bricks, {nn} nodes", file=sys.stderr, flush=True)
+
+    s32, dt32, t32, el32, nd32 = run(en.MIXED, deck, args.steps, args.warmup)
+    s32.close()
+    print(f"mixed: {ne * args.steps / dt32:.4e} el-cyc/s, elem {el32:.3f} ms, node {nd32:.3f} ms",
+          file=sys.stderr, flush=True)
+    fp64 = None
+    if not args.no_fp64:
+        s64, dt64, t64, el64, nd64 = run(en.FP64, deck, args.steps, args.warmup)
+        s64.close()
+        fp64 = dict(value=ne * args.steps / dt64, ms_per_step=1e3 * dt64 / args.steps,
+                    elem_kernel_ms=el64, node_kernel_ms=nd64,
+                    elem_kernel_gbs=elem_bytes(ne, nn, 8) / (el64 * 1e-3) / 1e9)
+        print(f"fp64: {fp64['value']:.4e} el-cyc/s, elem {el64:.3f} ms, node {nd64:.3f} ms",
+              file=sys.stderr, flush=True)
+
+    eb = elem_bytes(ne, nn, 4)
+    nb = node_bytes(ne, nn, 4)
+    ach = eb / (el32 * 1e-3) / 1e9
+    value = ne * args.steps / dt32
+    cycle_gbs = (eb + nb) * args.steps / dt32 / 1e9
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(deck, args.cpu_cycles)
+        print(f"cpu: {cpu['value']:.4e} el-cyc/s on {cpu['cores']} cores", file=sys.stderr,
+              flush=True)
+    out = {
+        "metric": "element-cycles/s & % HBM peak; mixed-FP32 vs FP64 GPU at stated force error",
+        "value": value,
+        "unit": "element-cycles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt32 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "precision": "element arithmetic FP32, nodal X/V/D and all sums FP64",
+        "data": "synthetic",
+        "config": {"workload": f"brick block {n}x{n}x{n} ({ne} hexas, {nn} nodes), law-2 "
+                               "Johnson-Cook steel, Isolid=1 co-rotational, PARITH/ON, "
+                               "hipGraph-replayed cycle",
+                   "elements": ne, "nodes": nn, "cycles_timed": args.steps},
+        "roofline": {"bound": "hbm", "kernel": "brick_forces_kernel<float>",
+                     "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": eb, "mean_launch_ms": el32},
+        "cycle_hbm_frac": cycle_gbs / HBM_PEAK_GBS,
+        "node_kernel": {"mean_launch_ms": nd32,
+                        "achieved_gbs": nb / (nd32 * 1e-3) / 1e9, "algorithmic_bytes": nb},
+        "fp64": fp64,
+        "mixed_over_fp64": (value / fp64["value"]) if fp64 else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,116 @@
+// orad_brick_kernel.hpp -- element-force kernel: one thread per brick.
+//
+// Per element it gathers the eight corners' X and V (FP64, 32-byte records),
+// runs the SFORC3 chain (orad_brick_chain.hpp), writes the element state back
+// (structure of arrays, coalesced) and the eight PARITH/ON corner slots at
+// IADS (SCUMU3P).  The group's stable time step is a wavefront shuffle min,
+// then one 64-bit atomicMin per workgroup on the bits of a positive double
+// (order independent, so deterministic).  Hourglass energy is summed per
+// workgroup in a fixed tree order into ehou_part[block].
+#pragma once
+#include "orad_brick_chain.hpp"
+
+namespace orad {
+
+constexpr int BRICK_BLOCK = 256;
+
+template <typename T> struct Store4;
+template <> struct Store4<double> {
+  static __device__ __forceinline__ double4 make(double a, double b, double c, double d) {
+    return make_double4(a, b, c, d);
+  }
+};
+template <> struct Store4<float> {
+  static __device__ __forceinline__ float4 make(float a, float b, float c, float d) {
+    return make_float4(a, b, c, d);
+  }
+};
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_xor(v, o, 64);
+    v = (w < v) ? w : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// REL: coordinates/velocities relative to corner 1 (mixed form).
+template <typename T, bool REL>
+__global__ __launch_bounds__(BRICK_BLOCK) void brick_forces_kernel(
+    BrickArgs<T> a, int e_begin, int e_end, const double *dt1p,
+    unsigned long long *dt2t_out, double *ehou_part, int xcd_remap) {
+  __shared__ double s_dt[BRICK_BLOCK / 64];
+  __shared__ double s_eh[BRICK_BLOCK / 64];
+  const int nb = gridDim.x;
+  const int b = xcd_remap ? xcd_block(blockIdx.x, nb) : (int)blockIdx.x;
+  const int e = e_begin + b * BRICK_BLOCK + (int)threadIdx.x;
+  const T dt1 = T(*dt1p);
+  double my_dt = 1.0e300, my_eh = 0.0;
+  if (e < e_end) {
+    const int ne = a.numels;
+    int nd[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) nd[k] = a.conn[k * ne + e];
+    T xd[8], yd[8], zd[8], vx[8], vy[8], vz[8];
+    if (REL) {
+      const double4 x0 = a.X4[nd[0]], v0 = a.V4[nd[0]];
+      xd[0] = T(0); yd[0] = T(0); zd[0] = T(0); vx[0] = T(0); vy[0] = T(0); vz[0] = T(0);
+#pragma unroll
+      for (int k = 1; k < 8; ++k) {
+        const double4 xk = a.X4[nd[k]], vk = a.V4[nd[k]];
+        xd[k] = T(xk.x - x0.x); yd[k] = T(xk.y - x0.y); zd[k] = T(xk.z - x0.z);
+        vx[k] = T(vk.x - v0.x); vy[k] = T(vk.y - v0.y); vz[k] = T(vk.z - v0.z);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const double4 xk = a.X4[nd[k]], vk = a.V4[nd[k]];
+        xd[k] = T(xk.x); yd[k] = T(xk.y); zd[k] = T(xk.z);
+        vx[k] = T(vk.x); vy[k] = T(vk.y); vz[k] = T(vk.z);
+      }
+    }
+    T sig[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sig[c] = a.sig[c * ne + e];
+    T eint = a.eint[e], rho = a.rho[e], pla = a.pla[e], qvis = a.qvis[e];
+    T offg = a.offg[e], epsd = a.epsd[e];
+    const T vol0 = a.vol0[e];
+    const T offg_in = offg;
+    BrickOut<T> o;
+    brick_chain<T>(xd, yd, zd, vx, vy, vz, sig, eint, rho, vol0, pla, qvis, offg, epsd, dt1,
+                   a.mat, a.prop, o);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a.sig[c * ne + e] = sig[c];
+    a.eint[e] = eint; a.rho[e] = rho; a.pla[e] = pla; a.qvis[e] = qvis; a.epsd[e] = epsd;
+    if (offg != offg_in) a.offg[e] = offg;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int slot = a.iads[k * ne + e];
+      a.fsky[slot] = Store4<T>::make(o.f[0][k], o.f[1][k], o.f[2][k], o.sti);
+    }
+    if (o.mstop) atomicOr(&a.ctl->mstop, 1);
+    my_dt = double(o.dtx);
+    my_eh = double(o.ehou);
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double wd = wave_min(my_dt);
+  double we = wave_sum(my_eh);
+  if (lane == 0) { s_dt[wv] = wd; s_eh[wv] = we; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double d = s_dt[0], h = s_eh[0];
+#pragma unroll
+    for (int w = 1; w < BRICK_BLOCK / 64; ++w) { d = (s_dt[w] < d) ? s_dt[w] : d; h += s_eh[w]; }
+    if (d < 1.0e300) atomicMin(dt2t_out, (unsigned long long)__double_as_longlong(d));
+    if (ehou_part) ehou_part[b] = h;
+  }
+}
+
+}  // namespace orad

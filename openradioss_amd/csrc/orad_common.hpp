@@ -1,0 +1,84 @@
+// orad_common.hpp -- device data layout shared by the kernels and the C ABI.
+//
+// HBM layout (one context, one GPU):
+//   nodes (FP64 always):  X4[n] = (x, y, z, MS), V4[n] = (vx, vy, vz, ICODT as double),
+//                         D[3n] displacement, A[3n] / STIFN[n] assembled outputs.
+//   bricks (T = double for the FP64 form, float for the mixed form), structure of
+//   arrays indexed by element:  conn[k*ne+e], iads[k*ne+e] (int32, k = corner),
+//   sig[c*ne+e] (c = 0..5), eint, rho, vol0, pla, qvis, offg, epsd.
+//   PARITH/ON slots: fsky[s] = (fx, fy, fz, sti) as a T4, slots in node-major
+//   CSR order (adsky[n]..adsky[n+1]-1).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace orad {
+
+struct Law2 {
+  double rho0, shear, bulk, ca, cb, cn, epmx, sigmx, cc, epdr, mexp;
+  int icc, vp, israte;
+};
+
+struct SProp {
+  double qa, qb, hvis, dtfac;
+};
+
+// Cycle scalars, double-buffered by cycle parity so that one kernel can read the
+// current cycle's values while the next cycle's are written.
+struct Scal {
+  double tt, dt1, dt2, dt12, dt2old, dtmx;
+  int ncycle, mstop;
+};
+
+struct CycleCtl {
+  Scal s[2];
+  unsigned long long dt2t[2];   // bits of the positive element dt minimum
+  int mstop;                    // sticky: non-positive volume seen
+  int pad;
+  double ehour;                 // cumulative hourglass energy
+};
+
+template <typename T> struct Vec4;
+template <> struct Vec4<double> { typedef double4 type; };
+template <> struct Vec4<float> { typedef float4 type; };
+
+template <typename T>
+struct BrickArgs {
+  int numels;
+  const int32_t *conn;      // [8][ne]
+  const int32_t *iads;      // [8][ne]
+  const double4 *X4;
+  const double4 *V4;
+  T *sig;                   // [6][ne]
+  T *eint, *rho, *pla, *qvis, *offg, *epsd;
+  const T *vol0;
+  typename Vec4<T>::type *fsky;
+  double *ehou_part;        // per-block hourglass energy partials
+  CycleCtl *ctl;
+  Law2 mat;
+  SProp prop;
+};
+
+struct NodeArgs {
+  int numnod;
+  const int32_t *adsky;     // [numnod+1]
+  double4 *X4;
+  double4 *V4;
+  double *D;
+  double *A;
+  double *STIFN;
+  const void *fsky;         // float4 or double4 slots
+  const double *ehou_part;
+  int n_ehou_part;
+  CycleCtl *ctl;
+};
+
+// Logical block index that gives each XCD a contiguous run of blocks (the
+// hardware deals workgroups round-robin over the 8 XCDs, each with its own L2).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int NX = 8;
+  int xcd = b % NX, local = b / NX, per = nb / NX, rem = nb % NX;
+  return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+}  // namespace orad

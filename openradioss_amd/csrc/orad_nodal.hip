@@ -1,0 +1,198 @@
+// orad_nodal.hip -- node-side kernels, FP64, built without FMA contraction.
+//
+// node_kernel fuses, per node n and in the reference's order:
+//   ASSPAR4  (assembly/asspar4.F:157-175)  a = sum of FSKY slots adsky[n]..adsky[n+1]-1
+//            in slot order (PARITH/ON: bitwise independent of launch shape);
+//   DT2      (resol.F:2640,5857-5861,6080,6224-6226) from the element minimum;
+//   ACCELE   (assembly/accele.F:64-82)       a = a / MS, 0 where MS <= 0;
+//   BCS10    (constraints/general/bcs/bcs1.F) zero fixed components of a and v;
+//   VELOCITY (assembly/velocity.F:56-64)     v = v + DT12 a;
+//   DEPLA    (assembly/displacement.F:88-101) d = d + DT2 v, x = x + DT2 v.
+// Workgroup 0 also publishes the next cycle's time scalars (double-buffered by
+// cycle parity), resets the next cycle's dt minimum and adds this cycle's
+// hourglass-energy partials to the running total in a fixed order.
+#include "orad_common.hpp"
+
+namespace orad {
+
+constexpr int NODE_BLOCK = 256;
+
+template <typename S>
+__global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity, int mode,
+                                                          int write_a, int xcd_remap) {
+  const int nb = gridDim.x;
+  const int b = xcd_remap ? xcd_block(blockIdx.x, nb) : (int)blockIdx.x;
+  const int n = b * NODE_BLOCK + (int)threadIdx.x;
+  const S *fsky = reinterpret_cast<const S *>(a.fsky);
+  CycleCtl *ctl = a.ctl;
+  const Scal sc = ctl->s[parity];
+
+  double dt2 = 0.0, dt12 = 0.0;
+  if (mode == 1) {
+    const double dt1 = sc.dt2;
+    const double dt2t = __longlong_as_double((long long)ctl->dt2t[parity]);
+    dt2 = 1.0e6;                                  // DT2 = EP06
+    if (dt2t < dt2) dt2 = dt2t;
+    const double lim = (double)1.1f * sc.dt2old;  // 1.1 is a default-real literal
+    dt2 = (lim < dt2) ? lim : dt2;
+    dt2 = (sc.dtmx < dt2) ? sc.dtmx : dt2;
+    dt12 = 0.5 * (dt1 + dt2);
+    if (b == 0) {
+      if (threadIdx.x == 0) {
+        Scal nx = sc;
+        nx.dt1 = dt1;
+        nx.dt2 = dt2;
+        nx.dt12 = dt12;
+        nx.dt2old = dt2;
+        nx.tt = sc.tt + dt2;
+        nx.ncycle = sc.ncycle + 1;
+        nx.mstop = ctl->mstop;
+        ctl->s[parity ^ 1] = nx;
+        ctl->dt2t[parity ^ 1] = 0x7FF0000000000000ull;
+      }
+      if (a.ehou_part) {
+        __shared__ double s_part[NODE_BLOCK];
+        double acc = 0.0;
+        for (int i = threadIdx.x; i < a.n_ehou_part; i += NODE_BLOCK) acc += a.ehou_part[i];
+        s_part[threadIdx.x] = acc;
+        __syncthreads();
+        for (int w = NODE_BLOCK / 2; w > 0; w >>= 1) {
+          if ((int)threadIdx.x < w) s_part[threadIdx.x] += s_part[threadIdx.x + w];
+          __syncthreads();
+        }
+        if (threadIdx.x == 0) ctl->ehour += s_part[0];
+      }
+    }
+  }
+  if (n >= a.numnod) return;
+
+  double ax = 0.0, ay = 0.0, az = 0.0, sf = 0.0;
+  const int k0 = a.adsky[n], k1 = a.adsky[n + 1];
+  for (int k = k0; k < k1; ++k) {
+    const S f = fsky[k];
+    ax = ax + double(f.x);
+    ay = ay + double(f.y);
+    az = az + double(f.z);
+    sf = sf + double(f.w);
+  }
+  if (mode == 0) {
+    a.A[3 * n] = ax; a.A[3 * n + 1] = ay; a.A[3 * n + 2] = az;
+    a.STIFN[n] = sf;
+    return;
+  }
+  double4 X = a.X4[n];
+  double4 V = a.V4[n];
+  const double ms = X.w;
+  if (ms > 0.0) {
+    const double RTMP = 1.0 / ms;
+    ax = ax * RTMP; ay = ay * RTMP; az = az * RTMP;
+  } else {
+    ax = 0.0; ay = 0.0; az = 0.0;
+  }
+  const int code = (int)V.w;
+  if (code & 4) { V.x = 0.0; ax = 0.0; }
+  if (code & 2) { V.y = 0.0; ay = 0.0; }
+  if (code & 1) { V.z = 0.0; az = 0.0; }
+  if (write_a) {
+    a.A[3 * n] = ax; a.A[3 * n + 1] = ay; a.A[3 * n + 2] = az;
+    a.STIFN[n] = sf;
+  }
+  V.x = V.x + dt12 * ax;
+  V.y = V.y + dt12 * ay;
+  V.z = V.z + dt12 * az;
+  const double vdx = dt2 * V.x, vdy = dt2 * V.y, vdz = dt2 * V.z;
+  double *D = a.D + 3 * n;
+  D[0] = D[0] + vdx; D[1] = D[1] + vdy; D[2] = D[2] + vdz;
+  X.x = X.x + vdx; X.y = X.y + vdy; X.z = X.z + vdz;
+  a.X4[n] = X;
+  a.V4[n] = V;
+}
+
+// Energy partial sums: block b sums its range in a fixed tree order.
+template <typename T>
+__global__ __launch_bounds__(NODE_BLOCK) void energy_partial_kernel(
+    int numels, const T *eint, const T *vol0, int numnod, const double4 *X4, const double4 *V4,
+    double *part_e, double *part_k, int nbe) {
+  __shared__ double s[NODE_BLOCK];
+  const int b = blockIdx.x;
+  double v = 0.0;
+  double *out;
+  if (b < nbe) {
+    const int e = b * NODE_BLOCK + threadIdx.x;
+    if (e < numels) v = double(eint[e]) * double(vol0[e]);
+    out = part_e + b;
+  } else {
+    const int n = (b - nbe) * NODE_BLOCK + threadIdx.x;
+    if (n < numnod) {
+      const double4 V = V4[n];
+      v = 0.5 * X4[n].w * (V.x * V.x + V.y * V.y + V.z * V.z);
+    }
+    out = part_k + (b - nbe);
+  }
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = NODE_BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+__global__ __launch_bounds__(NODE_BLOCK) void energy_final_kernel(const double *part_e, int ne,
+                                                                  const double *part_k, int nk,
+                                                                  double *res) {
+  __shared__ double s[2][NODE_BLOCK];
+  double ae = 0.0, ak = 0.0;
+  for (int i = threadIdx.x; i < ne; i += NODE_BLOCK) ae += part_e[i];
+  for (int i = threadIdx.x; i < nk; i += NODE_BLOCK) ak += part_k[i];
+  s[0][threadIdx.x] = ae;
+  s[1][threadIdx.x] = ak;
+  __syncthreads();
+  for (int w = NODE_BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s[0][threadIdx.x] += s[0][threadIdx.x + w];
+      s[1][threadIdx.x] += s[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { res[0] = s[0][0]; res[1] = s[1][0]; }
+}
+
+hipError_t launch_node(const NodeArgs &a, bool slot_fp32, int parity, int mode, int write_a,
+                       int xcd_remap, hipStream_t s) {
+  const int nb = (a.numnod + NODE_BLOCK - 1) / NODE_BLOCK;
+  if (nb <= 0) return hipSuccess;
+  if (slot_fp32)
+    hipLaunchKernelGGL(node_kernel<float4>, dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity, mode,
+                       write_a, xcd_remap);
+  else
+    hipLaunchKernelGGL(node_kernel<double4>, dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity, mode,
+                       write_a, xcd_remap);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_energy_t(int numels, const T *eint, const T *vol0, int numnod,
+                                  const double4 *X4, const double4 *V4, double *part,
+                                  double *res, hipStream_t s) {
+  const int nbe = (numels + NODE_BLOCK - 1) / NODE_BLOCK;
+  const int nbk = (numnod + NODE_BLOCK - 1) / NODE_BLOCK;
+  if (nbe + nbk > 0)
+    hipLaunchKernelGGL(energy_partial_kernel<T>, dim3(nbe + nbk), dim3(NODE_BLOCK), 0, s, numels,
+                       eint, vol0, numnod, X4, V4, part, part + nbe, nbe);
+  hipLaunchKernelGGL(energy_final_kernel, dim3(1), dim3(NODE_BLOCK), 0, s, part, nbe, part + nbe,
+                     nbk, res);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy(bool fp32, int numels, const void *eint, const void *vol0, int numnod,
+                         const double4 *X4, const double4 *V4, double *part, double *res,
+                         hipStream_t s) {
+  if (fp32)
+    return launch_energy_t<float>(numels, (const float *)eint, (const float *)vol0, numnod, X4, V4,
+                                  part, res, s);
+  return launch_energy_t<double>(numels, (const double *)eint, (const double *)vol0, numnod, X4,
+                                 V4, part, res, s);
+}
+
+}  // namespace orad

@@ -1,0 +1,221 @@
+"""Host-side mirror of the reference's brick-path interface over the C ABI
+(include/orad.h), through ctypes.  No CPU fallback: every compute call goes to
+liborad.so, and constructing a solver fails loudly if it is missing.
+
+Names follow the reference: sforc3 (per-group element forces, SFORC3),
+asspar4 (PARITH/ON assembly, ASSPAR4), run_cycles (RESOL's explicit cycle).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborad.so")
+
+ORAD_OK, ORAD_EINVAL, ORAD_EHIP, ORAD_ESTATE, ORAD_ENEGVOL = range(5)
+FP64, MIXED = 0, 1
+NIXS = 11
+
+EXPORTS = (
+    "orad_create", "orad_destroy", "orad_last_error", "orad_set_nodes", "orad_set_bricks",
+    "orad_set_law2", "orad_set_solid_prop", "orad_get_pon", "orad_set_time", "orad_get_time",
+    "orad_sforc3", "orad_asspar4", "orad_run_cycles", "orad_profile_cycles", "orad_get_nodes",
+    "orad_get_brick_state", "orad_set_brick_state", "orad_get_fsky", "orad_energies",
+)
+
+
+class TimeState(C.Structure):
+    """orad_time_t: RESOL's time scalars."""
+    _fields_ = [("tt", C.c_double), ("dt1", C.c_double), ("dt2", C.c_double),
+                ("dt12", C.c_double), ("dt2old", C.c_double), ("dtmx", C.c_double),
+                ("ncycle", C.c_int), ("mstop", C.c_int)]
+
+
+class OradError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"orad error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load liborad.so (raises if absent) and declare the C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library {path} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    sig = {
+        "orad_create": ([C.POINTER(P), I, I], I),
+        "orad_destroy": ([P], I),
+        "orad_last_error": ([P], C.c_char_p),
+        "orad_set_nodes": ([P, I, P, P, P, P], I),
+        "orad_set_bricks": ([P, I, P, P], I),
+        "orad_set_law2": ([P, P, P], I),
+        "orad_set_solid_prop": ([P, D, D, D, D], I),
+        "orad_get_pon": ([P, P, P], I),
+        "orad_set_time": ([P, C.POINTER(TimeState)], I),
+        "orad_get_time": ([P, C.POINTER(TimeState)], I),
+        "orad_sforc3": ([P, I, I, D, C.POINTER(D)], I),
+        "orad_asspar4": ([P, P, P], I),
+        "orad_run_cycles": ([P, I, C.POINTER(TimeState)], I),
+        "orad_profile_cycles": ([P, I, C.POINTER(D), C.POINTER(D)], I),
+        "orad_get_nodes": ([P, P, P, P, P, P], I),
+        "orad_get_brick_state": ([P, P, P, P, P, P, P, P], I),
+        "orad_set_brick_state": ([P, P, P, P, P, P, P], I),
+        "orad_get_fsky": ([P, P], I),
+        "orad_energies": ([P, C.POINTER(D), C.POINTER(D), C.POINTER(D)], I),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def ixs_from_deck(deck):
+    """IXS(NIXS=11, NUMELS) in the engine's layout: material, 8 nodes (1-based),
+    property, user id (scoor3.F:106-121)."""
+    ne = deck["ixs"].shape[0]
+    ixs = np.zeros((ne, NIXS), np.int32)
+    ixs[:, 0] = 1
+    ixs[:, 1:9] = deck["ixs"] + 1
+    ixs[:, 9] = 1
+    ixs[:, 10] = deck["uid"]
+    return ixs
+
+
+class BrickSolver:
+    """One model (bricks, law 2) resident on one GPU."""
+
+    def __init__(self, deck, precision=FP64, device=0):
+        self.lib = load_library()
+        self.precision = precision
+        h = C.c_void_p()
+        rc = self.lib.orad_create(C.byref(h), device, precision)
+        if rc != ORAD_OK:
+            raise OradError(rc, "orad_create failed (no HIP device?)")
+        self.h = h
+        self.numnod = deck["x"].shape[0]
+        self.numels = deck["ixs"].shape[0]
+        x = _f64(deck["x"], (self.numnod, 3))
+        v = _f64(deck["v"], (self.numnod, 3))
+        ms = _f64(deck["ms"], (self.numnod,))
+        icodt = np.ascontiguousarray(deck["icodt"], dtype=np.int32)
+        self._chk(self.lib.orad_set_nodes(h, self.numnod, _p(x), _p(v), _p(ms), _p(icodt)))
+        mat, prop = deck["mat"], deck["prop"]
+        p = np.array([mat["rho0"], mat["shear"], mat["bulk"], mat["ca"], mat["cb"], mat["cn"],
+                      mat["epmx"], mat["sigmx"], mat["cc"], mat["epdr"], mat["mexp"]], np.float64)
+        ip = np.array([mat["icc"], mat["vp"], mat["israte"]], np.int32)
+        self._chk(self.lib.orad_set_law2(h, _p(p), _p(ip)))
+        self._chk(self.lib.orad_set_solid_prop(h, prop["qa"], prop["qb"], prop["hvis"],
+                                               prop["dtfac"]))
+        ixs = ixs_from_deck(deck)
+        vol0 = _f64(deck["vol0"], (self.numels,))
+        self._chk(self.lib.orad_set_bricks(h, self.numels, _p(ixs), _p(vol0)))
+
+    def _chk(self, rc):
+        if rc != ORAD_OK:
+            raise OradError(rc, self.lib.orad_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.orad_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- reference-shaped operations ----
+    def pon(self):
+        """(ADSKY(NUMNOD+1), IADS(8,NUMELS)) 1-based, as the starter writes them."""
+        adsky = np.zeros(self.numnod + 1, np.int32)
+        iads = np.zeros((self.numels, 8), np.int32)
+        self._chk(self.lib.orad_get_pon(self.h, _p(adsky), _p(iads)))
+        return adsky, iads
+
+    def sforc3(self, nft, nel, dt1, dt2t=1.0e30):
+        """Element forces of elements nft+1..nft+nel (SFORC3); returns the group DT2T."""
+        d = C.c_double(dt2t)
+        self._chk(self.lib.orad_sforc3(self.h, nft, nel, dt1, C.byref(d)))
+        return d.value
+
+    def asspar4(self):
+        """Ordered nodal assembly of the FSKY slots: (A(NUMNOD,3), STIFN(NUMNOD))."""
+        a = np.zeros((self.numnod, 3))
+        s = np.zeros(self.numnod)
+        self._chk(self.lib.orad_asspar4(self.h, _p(a), _p(s)))
+        return a, s
+
+    def run_cycles(self, n):
+        t = TimeState()
+        self._chk(self.lib.orad_run_cycles(self.h, n, C.byref(t)))
+        return t
+
+    def profile_cycles(self, n):
+        e, nd = C.c_double(), C.c_double()
+        self._chk(self.lib.orad_profile_cycles(self.h, n, C.byref(e), C.byref(nd)))
+        return e.value, nd.value
+
+    def time(self):
+        t = TimeState()
+        self._chk(self.lib.orad_get_time(self.h, C.byref(t)))
+        return t
+
+    def set_time(self, **kw):
+        t = self.time()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._chk(self.lib.orad_set_time(self.h, C.byref(t)))
+
+    def nodes(self):
+        x = np.zeros((self.numnod, 3)); v = np.zeros_like(x); d = np.zeros_like(x)
+        a = np.zeros_like(x); s = np.zeros(self.numnod)
+        self._chk(self.lib.orad_get_nodes(self.h, _p(x), _p(v), _p(d), _p(a), _p(s)))
+        return dict(x=x, v=v, d=d, a=a, stifn=s)
+
+    def state(self):
+        ne = self.numels
+        sig = np.zeros((6, ne))
+        cols = {k: np.zeros(ne) for k in ("eint", "rho", "pla", "qvis", "offg", "epsd")}
+        self._chk(self.lib.orad_get_brick_state(self.h, _p(sig), _p(cols["eint"]), _p(cols["rho"]),
+                                                _p(cols["pla"]), _p(cols["qvis"]),
+                                                _p(cols["offg"]), _p(cols["epsd"])))
+        cols["sig"] = sig.T.copy()
+        return cols
+
+    def set_state(self, sig=None, eint=None, rho=None, pla=None, qvis=None, offg=None):
+        ne = self.numels
+        sg = None if sig is None else _f64(np.asarray(sig).T, (6, ne))
+        args = [None if a is None else _f64(a, (ne,)) for a in (eint, rho, pla, qvis, offg)]
+        self._chk(self.lib.orad_set_brick_state(self.h, _p(sg), *[_p(a) for a in args]))
+
+    def fsky(self):
+        f = np.zeros((8 * self.numels, 4))
+        self._chk(self.lib.orad_get_fsky(self.h, _p(f)))
+        return f
+
+    def energies(self):
+        ei, ek, eh = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self.lib.orad_energies(self.h, C.byref(ei), C.byref(ek), C.byref(eh)))
+        return ei.value, ek.value, eh.value

@@ -517,7 +517,7 @@ void oracle_cycle(orad_omodel_t *m) {
   const int ne = m->numels, nn = m->numnod;
   double dt2t = 1.0e300;
   int mstop = 0;
-  double ehour = 0.0;
+  double *ehou = malloc(sizeof(double)*(size_t)(ne > 0 ? ne : 1));
   /* resol.F:2639-2640 */
   m->dt1 = m->dt2;
   double dt1 = m->dt1;
@@ -541,10 +541,11 @@ void oracle_cycle(orad_omodel_t *m) {
     }
     if (o.dtx < dt2t) dt2t = o.dtx;
     mstop |= o.mstop;
+    ehou[e] = o.ehou;
   }
-  /* Hourglass energy in element order (deterministic, PARTSAV(8,·) analogue). */
-  /* Recomputed sums are not needed for the force path; kept as a running total. */
-  (void)ehour;
+  /* Hourglass energy (SHVIS3 -> PARTSAV(8,.)), summed in element order. */
+  for (int e = 0; e < ne; ++e) m->ehour += ehou[e];
+  free(ehou);
   /* ASSPAR4 (asspar4.F:157-175) ordered per-node gather */
 #pragma omp parallel for num_threads(nth) schedule(static)
   for (int n = 0; n < nn; ++n) {

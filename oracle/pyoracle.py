@@ -207,9 +207,10 @@ class Model:
             lib().oracle_cycle(C.byref(self.m))
 
     def energies(self):
+        """(internal, kinetic, cumulative hourglass) energy."""
         ei, ek = C.c_double(), C.c_double()
         lib().oracle_energies(C.byref(self.m), C.byref(ei), C.byref(ek))
-        return ei.value, ek.value
+        return ei.value, ek.value, self.m.ehour
 
     @property
     def dt2(self):

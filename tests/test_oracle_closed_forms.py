@@ -1,0 +1,229 @@
+"""Closed-form and brute-force pins of the oracle, independent of the reference build.
+
+Each check is chosen so that a plausible slip (sign, index, transposed frame,
+missing term) breaks it.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from openradioss_amd import deck as dk
+
+CUBE = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                 [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float).T
+MAT = dk.law2_params()
+PROP = dk.solid_params()
+# Co-rotational frame of the axis-aligned unit cube (SREPISO3/SORTHO3):
+# e1 = y, e2 = z, e3 = x; columns of R.  Stress is kept in this frame.
+R_CUBE = np.array([[0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+
+
+def sym(s):
+    return np.array([[s[0], s[3], s[5]], [s[3], s[1], s[4]], [s[5], s[4], s[2]]])
+
+
+def vec(S):
+    return np.array([S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[1, 2], S[0, 2]])
+
+
+def rot(a, b, c):
+    ca, sa, cb, sb, cc, sc = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(c), np.sin(c)
+    rz = np.array([[ca, -sa, 0], [sa, ca, 0], [0, 0, 1]])
+    ry = np.array([[cb, 0, sb], [0, 1, 0], [-sb, 0, cb]])
+    rx = np.array([[1, 0, 0], [0, cc, -sc], [0, sc, cc]])
+    return rz @ ry @ rx
+
+
+def fresh_state(vol0, sig=None, pla=0.0, rho=None):
+    st = np.zeros(po.NST)
+    st[po.ST_VOL0] = vol0
+    st[po.ST_RHO] = MAT["rho0"] if rho is None else rho
+    st[po.ST_OFFG] = 1.0
+    st[po.ST_PLA] = pla
+    if sig is not None:
+        st[0:6] = sig
+    return st
+
+
+def test_unit_cube_volume_and_rotated_volume():
+    assert po.brick_volume(CUBE) == 1.0
+    q = rot(0.3, -0.7, 1.1)
+    assert abs(po.brick_volume(q @ CUBE * 2.0) - 8.0) < 1e-13
+
+
+def test_rigid_translation_gives_no_force():
+    v = np.tile(np.array([[1e4], [-3e3], [2e3]]), (1, 8))
+    st = fresh_state(1.0)
+    o = po.brick_element(CUBE, v, st, 1.3e-7, MAT, PROP)
+    assert np.abs(o["f"]).max() == 0.0
+    assert o["ehou"] == 0.0
+    assert np.all(st[0:6] == 0.0) and st[po.ST_PLA] == 0.0
+
+
+def test_rigid_rotation_first_cycle_gives_no_force():
+    w = np.array([0.0, 0.0, 5e3])                   # rad/s about z
+    x = rot(0.2, 0.4, 0.1) @ CUBE + 3.0
+    v = np.cross(w, (x - x.mean(1, keepdims=True)).T).T
+    st = fresh_state(po.brick_volume(x))
+    o = po.brick_element(x, v, st, 0.0, MAT, PROP)
+    assert np.abs(o["f"]).max() < 1e-9 * MAT["bulk"]
+
+
+def test_stable_time_step_closed_form():
+    # Unit cube at rest: DELTAX = 4V/sqrt(max face E*G-F^2) = 1, no compression, so
+    # SSP_EQ = c (qb + sqrt(qb^2 + 1)), c = sqrt((1.333 G + K)/rho0) (m2law.F:196).
+    st = fresh_state(1.0)
+    o = po.brick_element(CUBE, np.zeros((3, 8)), st, 0.0, MAT, PROP)
+    c = np.sqrt((1.333 * MAT["shear"] + MAT["bulk"]) / MAT["rho0"])
+    qb = PROP["qb"]
+    dt = PROP["dtfac"] * 1.0 / (qb * c + np.sqrt((qb * c) ** 2 + c * c))
+    assert o["dtx"] == pytest.approx(dt, rel=1e-14)
+    # STI = rho V / DTX^2 (before DTFAC), quartered per corner
+    dtx = dt / PROP["dtfac"]
+    assert o["sti"] == pytest.approx(0.25 * MAT["rho0"] / dtx ** 2, rel=1e-13)
+
+
+def test_mass_conservation_rho_v():
+    rng = np.random.default_rng(3)
+    x = CUBE + rng.uniform(-0.1, 0.1, (3, 8))
+    v = rng.uniform(-2e3, 2e3, (3, 8))
+    v0 = po.brick_volume(x) * 1.01
+    st = fresh_state(v0)
+    o = po.brick_element(x, v, st, 1.2e-7, MAT, PROP)
+    assert st[po.ST_RHO] * o["voln"] == pytest.approx(MAT["rho0"] * v0, rel=1e-15)
+
+
+def test_constant_stress_patch_forces():
+    # Axis-aligned unit cube, no motion, DT1 = 0: SFINT3 gives F = -sigma_dev . P V,
+    # P(node 1) = (-1/4,-1/4,-1/4) ...; the hydrostatic part is replaced by K*mu = 0.
+    sig = np.array([120.0, -40.0, 10.0, 30.0, -20.0, 15.0])
+    st = fresh_state(1.0, sig=sig)
+    o = po.brick_element(CUBE, np.zeros((3, 8)), st, 0.0, MAT, PROP)
+    p = -(sig[0] + sig[1] + sig[2]) / 3.0
+    s = sig.copy()
+    s[0:3] += p
+    S = R_CUBE @ sym(s) @ R_CUBE.T                   # local-frame stress in global axes
+    grad = (CUBE - 0.5) * 0.5                        # dN_k/dx at the centre, unit cube
+    expect = -(S @ grad)
+    np.testing.assert_allclose(o["f"], expect, rtol=0, atol=1e-12)
+    assert abs(o["f"].sum(axis=1)).max() < 1e-12     # equilibrium
+    np.testing.assert_allclose(st[0:6], s, rtol=0, atol=1e-12)
+
+
+def test_affine_velocity_deviatoric_increment():
+    # v = L x on an axis-aligned cube: D = R^T sym(L) R + O(dt |L|^2);
+    # elastic step: s_new = 2 G dt dev(D); no hourglass for a linear field.
+    L = np.array([[3.0, 1.0, -2.0], [0.5, -1.0, 4.0], [1.5, 2.0, 0.5]])
+    v = L @ CUBE
+    dt = 1.0e-7
+    st = fresh_state(1.0)
+    o = po.brick_element(CUBE, v, st, dt, MAT, PROP)
+    D = R_CUBE.T @ (0.5 * (L + L.T)) @ R_CUBE        # strain rate in the element frame
+    dev = D - np.trace(D) / 3.0 * np.eye(3)
+    G = MAT["shear"]
+    s = st[0:6].copy()
+    p = -(s[0] + s[1] + s[2]) / 3.0
+    s[0:3] += p
+    np.testing.assert_allclose(s, 2 * G * dt * vec(dev), rtol=1e-5)  # + O(dt |L|) correction
+    assert abs(o["ehou"]) < 1e-20
+
+
+@pytest.mark.parametrize("n", [1.0, 0.4])
+def test_radial_return_lands_on_yield_surface(n):
+    # Trial stress far outside the surface: with n = 1 the one-step return is exact,
+    # so J2(dev sigma_new) = A + B * pla_new; for n < 1 the second rescale still puts
+    # the stress on the linearised surface A + B pla^n + dpla*H.
+    mat = dk.law2_params(n=n)
+    sig = np.array([900.0, -300.0, 100.0, 200.0, 0.0, -150.0])
+    st = fresh_state(1.0, sig=sig, pla=0.05)
+    po.brick_element(CUBE, np.zeros((3, 8)), st, 1e-7, mat, PROP)
+    s = st[0:6].copy()
+    p = -(s[0] + s[1] + s[2]) / 3.0
+    s[0:3] += p
+    j2 = np.sqrt(3 * (0.5 * (s[0] ** 2 + s[1] ** 2 + s[2] ** 2) + s[3] ** 2 + s[4] ** 2 + s[5] ** 2))
+    pla = st[po.ST_PLA]
+    assert pla > 0.05
+    if n == 1.0:
+        assert j2 == pytest.approx(mat["ca"] + mat["cb"] * pla, rel=1e-13)
+    else:
+        h = mat["cb"] * n * 0.05 ** (n - 1)
+        assert j2 == pytest.approx(mat["ca"] + mat["cb"] * 0.05 ** n + (pla - 0.05) * h, rel=1e-13)
+
+
+def test_objectivity_of_corotational_forces():
+    rng = np.random.default_rng(11)
+    x = CUBE * 1.3 + rng.uniform(-0.1, 0.1, (3, 8))
+    v = rng.uniform(-2e3, 2e3, (3, 8))
+    sig = rng.uniform(-200, 200, 6)
+    q = rot(0.9, -0.4, 2.0)
+    st1 = fresh_state(po.brick_volume(x), sig=sig, pla=0.02)
+    st2 = st1.copy()
+    o1 = po.brick_element(x, v, st1, 1.1e-7, MAT, PROP)
+    o2 = po.brick_element(q @ x + 5.0, q @ v, st2, 1.1e-7, MAT, PROP)
+    np.testing.assert_allclose(o2["f"], q @ o1["f"], rtol=0, atol=1e-9 * np.abs(o1["f"]).max())
+    np.testing.assert_allclose(st2, st1, rtol=1e-9, atol=1e-12)
+
+
+def brute_force_pon(numnod, ixs, uid):
+    entries = sorted((ixs[e, k], uid[e], k, e) for e in range(ixs.shape[0]) for k in range(8))
+    adsky = np.zeros(numnod + 1, np.int64)
+    iads = np.zeros(ixs.shape, np.int64)
+    for s, (n, _, k, e) in enumerate(entries):
+        adsky[n + 1] += 1
+        iads[e, k] = s
+    return np.cumsum(adsky), iads
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_pon_csr_matches_sorted_node_uid_corner(seed):
+    d = dk.brick_block(3, 4, 2, seed=seed)
+    rng = np.random.default_rng(seed)
+    ixs = d["ixs"][rng.permutation(d["ixs"].shape[0])]       # elements out of id order
+    uid = rng.permutation(np.arange(10, 10 + ixs.shape[0])).astype(np.int32)
+    if seed == 1:                                              # a degenerate brick (7 nodes)
+        ixs = ixs.copy()
+        ixs[0, 7] = ixs[0, 6]
+    adsky, iads = po.build_pon(d["x"].shape[0], ixs, uid)
+    a2, i2 = brute_force_pon(d["x"].shape[0], ixs, uid)
+    np.testing.assert_array_equal(adsky, a2)
+    np.testing.assert_array_equal(iads, i2)
+
+
+def test_block_deck_shapes_and_mass():
+    d = dk.brick_block(4, 3, 2)
+    assert d["x"].shape == (5 * 4 * 3, 3) and d["ixs"].shape == (24, 8)
+    assert np.all(d["vol0"] > 0.8)
+    assert d["ms"].sum() == pytest.approx(MAT["rho0"] * d["vol0"].sum(), rel=1e-13)
+    vols = np.array([po.brick_volume(d["x"][e].T) for e in d["ixs"]])
+    np.testing.assert_array_equal(vols, d["vol0"])           # deck uses the SDERI3 determinant
+    assert np.all(d["icodt"][d["x"][:, 2] < 0.5] == 7)
+
+
+def test_oracle_cycle_free_body_momentum_is_exact():
+    # No boundary conditions: the element forces of every brick sum to zero, so the
+    # PARITH/ON assembly must conserve total momentum to rounding.
+    d = dk.brick_block(4, 3, 5, v_top=0.0)
+    rng = np.random.default_rng(5)
+    d["v"] = rng.uniform(-500.0, 500.0, d["v"].shape)
+    d["icodt"][:] = 0
+    m = po.Model(d, nthreads=2)
+    p0 = (d["ms"][:, None] * d["v"]).sum(0)
+    m.cycle(40)
+    p1 = (d["ms"][:, None] * m.v).sum(0)
+    scale = (d["ms"][:, None] * np.abs(d["v"])).sum(0)
+    assert np.abs(p1 - p0).max() < 1e-12 * scale.max()
+    assert m.st[:, po.ST_PLA].max() > 0.0 or np.abs(m.st[:, 0:6]).max() > 1.0
+
+
+def test_oracle_cycle_smooth_vibration_energy():
+    # Elastic standing wave: EINT + EKIN + EHOUR stays near the initial EKIN (the
+    # half-step kinetic energy of the central-difference scheme oscillates a few %).
+    d = dk.brick_block(4, 4, 8, v_top=0.0, jitter=0.0)
+    z = d["x"][:, 2]
+    d["v"][:, 2] = -100.0 * np.sin(0.5 * np.pi * z / z.max())
+    m = po.Model(d, nthreads=2)
+    ek0 = 0.5 * np.sum(d["ms"] * (d["v"] ** 2).sum(1))
+    for _ in range(8):
+        m.cycle(40)
+        ei, ek, eh = m.energies()
+        assert abs(ei + ek + eh - ek0) < 0.12 * ek0
