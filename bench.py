@@ -19,8 +19,6 @@ import os
 import sys
 import time
 
-import numpy as np
-
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -69,6 +67,32 @@ def cpu_baseline(deck, cycles):
                 sample=f"{cycles} cycles of the same {ne}-brick deck, FP64 C oracle, OpenMP")
 
 
+def copy_calibration():
+    """A plain device copy of known size (3 x 1 GiB), for scaling the rocprofv3
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"copied {4 * n} bytes 3 times", flush=True)
+    return 0
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/counters_200cube.json), or None."""
+    path = os.path.join(ROOT, "profiles", "counters_200cube.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,7 +102,10 @@ def main():
     ap.add_argument("--cpu-cycles", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the slab decomposition is described in DESIGN.md)", file=sys.stderr)
@@ -132,7 +159,9 @@ def main():
                    "elements": ne, "nodes": nn, "cycles_timed": args.steps},
         "roofline": {"bound": "hbm", "kernel": "brick_forces_kernel<float>",
                      "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach / HBM_PEAK_GBS,
+                     "traffic": (measured_traffic("brick_forces_kernel<float,true>")
+                                 if n == 200 else None),
                      "algorithmic_bytes_per_launch": eb, "mean_launch_ms": el32},
         "cycle_hbm_frac": cycle_gbs / HBM_PEAK_GBS,
         "node_kernel": {"mean_launch_ms": nd32,

@@ -41,7 +41,25 @@ def build(verbose=False):
     subprocess.run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs,
                    check=True)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
+
+
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN_DIR, "resol_brick")
+
+
+def build_fortran(verbose=False):
+    """ISO_C_BINDING module + RESOL-style driver, linked against liborad.so."""
+    fc = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+    cmd = [fc, "-O2", "-ffp-contract=off", "-J", BUILD,
+           os.path.join(FORTRAN_DIR, "orad_bind.F90"), os.path.join(FORTRAN_DIR, "resol_brick.F90"),
+           "-L", HERE, "-lorad", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

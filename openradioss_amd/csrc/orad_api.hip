@@ -23,7 +23,7 @@ hipError_t launch_brick_fp64(const BrickArgs<double> &, int, int, const double *
 hipError_t launch_brick_mixed(const BrickArgs<float> &, int, int, const double *,
                               unsigned long long *, double *, int, hipStream_t);
 hipError_t launch_node(const NodeArgs &, bool, int, int, int, int, hipStream_t);
-hipError_t launch_energy(bool, int, const void *, const void *, int, const double4 *,
+hipError_t launch_energy(bool, int, const void *, const double *, int, const double4 *,
                          const double4 *, double *, double *, hipStream_t);
 }  // namespace orad
 
@@ -47,7 +47,8 @@ struct orad_ctx {
   double4 *X4 = nullptr, *V4 = nullptr;
   double *D = nullptr, *A = nullptr, *STIFN = nullptr;
   int32_t *conn = nullptr, *iads = nullptr, *adsky = nullptr;
-  void *state = nullptr;  // SoA element state of the storage type
+  void *state = nullptr;  // SoA element state of the storage type (NSTATE columns)
+  double *vol0 = nullptr; // initial volumes, FP64
   void *fsky = nullptr;
   double *ehou_part = nullptr;
   int n_ehou_part = 0;
@@ -107,18 +108,19 @@ static BrickArgs<T> brick_args(orad_ctx *c) {
   a.sig = base;
   a.eint = base + 6 * ne;
   a.rho = base + 7 * ne;
-  a.vol0 = base + 8 * ne;
-  a.pla = base + 9 * ne;
-  a.qvis = base + 10 * ne;
-  a.offg = base + 11 * ne;
-  a.epsd = base + 12 * ne;
+  a.pla = base + 8 * ne;
+  a.qvis = base + 9 * ne;
+  a.offg = base + 10 * ne;
+  a.epsd = base + 11 * ne;
+  a.vol0 = c->vol0;
   a.fsky = (typename Vec4<T>::type *)c->fsky;
   a.ctl = c->ctl;
   a.mat = c->mat;
   a.prop = c->prop;
   return a;
 }
-static constexpr int NSTATE = 13;  // sig(6) eint rho vol0 pla qvis offg epsd
+static constexpr int NSTATE = 12;  // sig(6) eint rho pla qvis offg epsd
+enum { C_EINT = 6, C_RHO = 7, C_PLA = 8, C_QVIS = 9, C_OFFG = 10, C_EPSD = 11 };
 
 static NodeArgs node_args(orad_ctx *c) {
   NodeArgs a{};
@@ -156,11 +158,7 @@ static hipError_t enqueue_cycle(orad_ctx *c, int p) {
 static int init_state(orad_ctx *c) {
   if (!(c->have_bricks && c->have_mat) || c->state_init) return ORAD_OK;
   const size_t ne = (size_t)c->numels;
-  // Keep the vol0 already uploaded, set the rest: SIG=0 EINT=0 RHO=rho0 PLA=0 QVIS=0 OFF=1.
-  std::vector<double> vol(ne);
-  std::vector<char> buf(ne * c->tsize());
-  HIPCHK(c, hipMemcpy(buf.data(), (char *)c->state + 8 * ne * c->tsize(), ne * c->tsize(),
-                      hipMemcpyDeviceToHost));
+  // SIG=0 EINT=0 RHO=rho0 PLA=0 QVIS=0 OFF=1 EPSD=0 (starter initial state).
   std::vector<char> all(NSTATE * ne * c->tsize(), 0);
   auto put = [&](int col, double v) {
     for (size_t e = 0; e < ne; ++e) {
@@ -170,9 +168,8 @@ static int init_state(orad_ctx *c) {
         ((double *)all.data())[col * ne + e] = v;
     }
   };
-  put(7, c->mat.rho0);
-  put(11, 1.0);
-  std::memcpy(all.data() + 8 * ne * c->tsize(), buf.data(), ne * c->tsize());
+  put(C_RHO, c->mat.rho0);
+  put(C_OFFG, 1.0);
   HIPCHK(c, hipMemcpy(c->state, all.data(), all.size(), hipMemcpyHostToDevice));
   c->state_init = true;
   return ORAD_OK;
@@ -221,7 +218,8 @@ int orad_destroy(orad_ctx *c) {
   drop_graphs(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void *p : {(void *)c->X4, (void *)c->V4, (void *)c->D, (void *)c->A, (void *)c->STIFN,
-                  (void *)c->conn, (void *)c->iads, (void *)c->adsky, c->state, c->fsky,
+                  (void *)c->conn, (void *)c->iads, (void *)c->adsky, c->state, (void *)c->vol0,
+                  c->fsky,
                   (void *)c->ehou_part, (void *)c->ctl, (void *)c->scratch_dt1,
                   (void *)c->scratch_dt2t})
     dfree(p);
@@ -324,14 +322,8 @@ int orad_set_bricks(orad_ctx *c, int numels, const int *ixs, const double *vol0)
   c->n_ehou_part = (int)((ne + 255) / 256);
   HIPCHK(c, dalloc(&c->ehou_part, (size_t)c->n_ehou_part * sizeof(double)));
   HIPCHK(c, hipMemset(c->ehou_part, 0, (size_t)c->n_ehou_part * sizeof(double)));
-  {
-    std::vector<char> v0(ne * ts);
-    for (size_t e = 0; e < ne; ++e) {
-      if (c->precision == ORAD_MIXED) ((float *)v0.data())[e] = (float)vol0[e];
-      else ((double *)v0.data())[e] = vol0[e];
-    }
-    HIPCHK(c, hipMemcpy((char *)c->state + 8 * ne * ts, v0.data(), ne * ts, hipMemcpyHostToDevice));
-  }
+  HIPCHK(c, dalloc(&c->vol0, ne * sizeof(double)));
+  HIPCHK(c, hipMemcpy(c->vol0, vol0, ne * sizeof(double), hipMemcpyHostToDevice));
   c->numels = numels;
   c->have_bricks = true;
   drop_graphs(c);
@@ -553,12 +545,12 @@ int orad_get_brick_state(orad_ctx *c, double *sig, double *eint, double *rho, do
   for (size_t e = 0; e < ne; ++e) {
     if (sig)
       for (int k = 0; k < 6; ++k) sig[k * ne + e] = col(k, e);
-    if (eint) eint[e] = col(6, e);
-    if (rho) rho[e] = col(7, e);
-    if (pla) pla[e] = col(9, e);
-    if (qvis) qvis[e] = col(10, e);
-    if (offg) offg[e] = col(11, e);
-    if (epsd) epsd[e] = col(12, e);
+    if (eint) eint[e] = col(C_EINT, e);
+    if (rho) rho[e] = col(C_RHO, e);
+    if (pla) pla[e] = col(C_PLA, e);
+    if (qvis) qvis[e] = col(C_QVIS, e);
+    if (offg) offg[e] = col(C_OFFG, e);
+    if (epsd) epsd[e] = col(C_EPSD, e);
   }
   return ORAD_OK;
 }
@@ -579,11 +571,11 @@ int orad_set_brick_state(orad_ctx *c, const double *sig, const double *eint, con
   for (size_t e = 0; e < ne; ++e) {
     if (sig)
       for (int k = 0; k < 6; ++k) put(k, e, sig[k * ne + e]);
-    if (eint) put(6, e, eint[e]);
-    if (rho) put(7, e, rho[e]);
-    if (pla) put(9, e, pla[e]);
-    if (qvis) put(10, e, qvis[e]);
-    if (offg) put(11, e, offg[e]);
+    if (eint) put(C_EINT, e, eint[e]);
+    if (rho) put(C_RHO, e, rho[e]);
+    if (pla) put(C_PLA, e, pla[e]);
+    if (qvis) put(C_QVIS, e, qvis[e]);
+    if (offg) put(C_OFFG, e, offg[e]);
   }
   HIPCHK(c, hipMemcpy(c->state, h.data(), h.size(), hipMemcpyHostToDevice));
   c->state_init = true;
@@ -615,9 +607,9 @@ int orad_energies(orad_ctx *c, double *eint, double *ekin, double *ehour) {
   const size_t nbe = (ne + 255) / 256, nbk = ((size_t)c->numnod + 255) / 256;
   double *part = nullptr;
   HIPCHK(c, dalloc(&part, (nbe + nbk + 2) * sizeof(double)));
-  hipError_t e = launch_energy(c->precision == ORAD_MIXED, c->numels, (char *)c->state + 6 * ne * ts,
-                               (char *)c->state + 8 * ne * ts, c->numnod, c->X4, c->V4, part,
-                               part + nbe + nbk, c->stream);
+  hipError_t e = launch_energy(c->precision == ORAD_MIXED, c->numels,
+                               (char *)c->state + C_EINT * ne * ts, c->vol0, c->numnod, c->X4,
+                               c->V4, part, part + nbe + nbk, c->stream);
   double res[2] = {0, 0}, eh = 0;
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) e = hipMemcpy(res, part + nbe + nbk, 2 * sizeof(double), hipMemcpyDeviceToHost);

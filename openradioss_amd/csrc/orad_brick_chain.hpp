@@ -61,10 +61,14 @@ template <typename T> struct K {
 
 // st_* are the element's state values (in/out).  xd/yd/zd, vx/vy/vz: the eight
 // corners' coordinates and velocities (absolute for T=double, relative to corner
-// 1 for the mixed form).
+// 1 for the mixed form).  vol0d: initial volume (FP64).  voldp: the mixed form's
+// FP64 Jacobian determinant (VOLDP), used, as the reference's single-precision
+// build does (srho3.F, IRESP=1 branch), for density, volume change and the
+// volumetric strain mu, where FP32 would cancel; ignored when T = double.
 template <typename T>
 __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], T vy[8], T vz[8],
-                                            T sig[6], T &st_eint, T &st_rho, T vol0, T &st_pla,
+                                            T sig[6], T &st_eint, T &st_rho, double vol0d,
+                                            double voldp, T &st_pla,
                                             T &st_qvis, T &st_offg, T &st_epsd, T dt1,
                                             const Law2 &mat, const SProp &prop,
                                             BrickOut<T> &out) {
@@ -160,7 +164,9 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
     T JAC_59_68 = JAC5 * JAC9 - JAC6 * JAC8;
     T JAC_67_49 = JAC6 * JAC7 - JAC4 * JAC9;
     T JAC_48_57 = JAC4 * JAC8 - JAC5 * JAC7;
-    T det = K<T>::ONE_OVER_64 * (JAC1 * JAC_59_68 + JAC2 * JAC_67_49 + JAC3 * JAC_48_57);
+    T det = (sizeof(T) == sizeof(double))
+                ? K<T>::ONE_OVER_64 * (JAC1 * JAC_59_68 + JAC2 * JAC_67_49 + JAC3 * JAC_48_57)
+                : T(voldp);
     if (off == ZERO) {
       det = ONE;
     } else if (offg > ONE) {
@@ -255,12 +261,22 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
 
   // ---- SRHO3 (Lagrangian), MMAIN prologue (mmain.F90:678-737) ----
   const T rho0 = T(mat.rho0);
-  if (offg == ZERO && voln == ONE) voln = vol0;
-  T dvol = voln - (rho0 / st_rho) * vol0;
-  T rho = rho0 * (vol0 / voln);
+  const T vol0 = T(vol0d);
+  T dvol, rho, amu;
+  if (sizeof(T) == sizeof(double)) {
+    if (offg == ZERO && voln == ONE) voln = vol0;
+    dvol = voln - (rho0 / st_rho) * vol0;
+    rho = rho0 * (vol0 / voln);
+    amu = rho / rho0 - ONE;
+  } else {
+    double vdp = (off == ZERO || mstop) ? 1.0 : voldp;   // SCHKJABT3 set DET = 1
+    if (offg == ZERO) { vdp = vol0d; voln = vol0; }
+    dvol = T(vdp - double(rho0 / st_rho) * vol0d);
+    rho = T(mat.rho0 * (vol0d / vdp));
+    amu = T(vol0d / vdp - 1.0);
+  }
   T eint = st_eint * vol0;
   T vol_avg = voln - HALF * dvol;
-  T amu = rho / rho0 - ONE;
 
   // ---- M2LAW (m2law.F:162-353) ----
   T sold[6];

@@ -41,6 +41,20 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// SDERI3's determinant (sderi3.F:104-155) of FP64 corner coordinates: the mixed
+// form's VOLDP.  Rotation-invariant, so it is taken in the global frame.
+__device__ __forceinline__ double jacobian_det(const double x[8], const double y[8],
+                                               const double z[8]) {
+  double X17 = x[6] - x[0], X28 = x[7] - x[1], X35 = x[4] - x[2], X46 = x[5] - x[3];
+  double Y17 = y[6] - y[0], Y28 = y[7] - y[1], Y35 = y[4] - y[2], Y46 = y[5] - y[3];
+  double Z17 = z[6] - z[0], Z28 = z[7] - z[1], Z35 = z[4] - z[2], Z46 = z[5] - z[3];
+  double J1 = X17 + X28 - X35 - X46, J2 = Y17 + Y28 - Y35 - Y46, J3 = Z17 + Z28 - Z35 - Z46;
+  double a1 = X17 + X46, a2 = X28 + X35, b1 = Y17 + Y46, b2 = Y28 + Y35;
+  double c1 = Z17 + Z46, c2 = Z28 + Z35;
+  double J4 = a1 + a2, J5 = b1 + b2, J6 = c1 + c2, J7 = a1 - a2, J8 = b1 - b2, J9 = c1 - c2;
+  return 0.015625 * (J1 * (J5 * J9 - J6 * J8) + J2 * (J6 * J7 - J4 * J9) + J3 * (J4 * J8 - J5 * J7));
+}
+
 // REL: coordinates/velocities relative to corner 1 (mixed form).
 template <typename T, bool REL>
 __global__ __launch_bounds__(BRICK_BLOCK) void brick_forces_kernel(
@@ -59,15 +73,21 @@ __global__ __launch_bounds__(BRICK_BLOCK) void brick_forces_kernel(
 #pragma unroll
     for (int k = 0; k < 8; ++k) nd[k] = a.conn[k * ne + e];
     T xd[8], yd[8], zd[8], vx[8], vy[8], vz[8];
+    double voldp = 0.0;
     if (REL) {
       const double4 x0 = a.X4[nd[0]], v0 = a.V4[nd[0]];
-      xd[0] = T(0); yd[0] = T(0); zd[0] = T(0); vx[0] = T(0); vy[0] = T(0); vz[0] = T(0);
+      double rx[8], ry[8], rz[8];
+      rx[0] = 0.0; ry[0] = 0.0; rz[0] = 0.0;
+      vx[0] = T(0); vy[0] = T(0); vz[0] = T(0);
 #pragma unroll
       for (int k = 1; k < 8; ++k) {
         const double4 xk = a.X4[nd[k]], vk = a.V4[nd[k]];
-        xd[k] = T(xk.x - x0.x); yd[k] = T(xk.y - x0.y); zd[k] = T(xk.z - x0.z);
+        rx[k] = xk.x - x0.x; ry[k] = xk.y - x0.y; rz[k] = xk.z - x0.z;
         vx[k] = T(vk.x - v0.x); vy[k] = T(vk.y - v0.y); vz[k] = T(vk.z - v0.z);
       }
+      voldp = jacobian_det(rx, ry, rz);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) { xd[k] = T(rx[k]); yd[k] = T(ry[k]); zd[k] = T(rz[k]); }
     } else {
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
@@ -81,11 +101,11 @@ __global__ __launch_bounds__(BRICK_BLOCK) void brick_forces_kernel(
     for (int c = 0; c < 6; ++c) sig[c] = a.sig[c * ne + e];
     T eint = a.eint[e], rho = a.rho[e], pla = a.pla[e], qvis = a.qvis[e];
     T offg = a.offg[e], epsd = a.epsd[e];
-    const T vol0 = a.vol0[e];
+    const double vol0 = a.vol0[e];
     const T offg_in = offg;
     BrickOut<T> o;
-    brick_chain<T>(xd, yd, zd, vx, vy, vz, sig, eint, rho, vol0, pla, qvis, offg, epsd, dt1,
-                   a.mat, a.prop, o);
+    brick_chain<T>(xd, yd, zd, vx, vy, vz, sig, eint, rho, vol0, voldp, pla, qvis, offg, epsd,
+                   dt1, a.mat, a.prop, o);
 #pragma unroll
     for (int c = 0; c < 6; ++c) a.sig[c * ne + e] = sig[c];
     a.eint[e] = eint; a.rho[e] = rho; a.pla[e] = pla; a.qvis[e] = qvis; a.epsd[e] = epsd;

@@ -5,7 +5,8 @@
 //                         D[3n] displacement, A[3n] / STIFN[n] assembled outputs.
 //   bricks (T = double for the FP64 form, float for the mixed form), structure of
 //   arrays indexed by element:  conn[k*ne+e], iads[k*ne+e] (int32, k = corner),
-//   sig[c*ne+e] (c = 0..5), eint, rho, vol0, pla, qvis, offg, epsd.
+//   sig[c*ne+e] (c = 0..5), eint, rho, pla, qvis, offg, epsd; vol0[e] is FP64
+//   in both forms (the reference's single-precision build keeps VOL0DP too).
 //   PARITH/ON slots: fsky[s] = (fx, fy, fz, sti) as a T4, slots in node-major
 //   CSR order (adsky[n]..adsky[n+1]-1).
 #pragma once
@@ -51,7 +52,7 @@ struct BrickArgs {
   const double4 *V4;
   T *sig;                   // [6][ne]
   T *eint, *rho, *pla, *qvis, *offg, *epsd;
-  const T *vol0;
+  const double *vol0;       // initial volumes, FP64 in both forms (VOL0DP)
   typename Vec4<T>::type *fsky;
   double *ehou_part;        // per-block hourglass energy partials
   CycleCtl *ctl;

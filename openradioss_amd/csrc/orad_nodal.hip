@@ -17,12 +17,24 @@ namespace orad {
 
 constexpr int NODE_BLOCK = 256;
 
+template <typename S> struct SlotScalar;
+template <> struct SlotScalar<float4> { typedef float type; static constexpr int TILE = 2048; };
+template <> struct SlotScalar<double4> { typedef double type; static constexpr int TILE = 1024; };
+
+// LDS index swizzle: permutes each aligned run of 32 words, so the per-node
+// strided reads below (about 8 words apart) spread over the banks.
+__device__ __forceinline__ int lds_sw(int i) { return i ^ ((i >> 5) & 31); }
+
 template <typename S>
 __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity, int mode,
                                                           int write_a, int xcd_remap) {
+  typedef typename SlotScalar<S>::type R;
+  constexpr int TILE = SlotScalar<S>::TILE;
+  __shared__ R t_x[TILE], t_y[TILE], t_z[TILE], t_w[TILE];
   const int nb = gridDim.x;
   const int b = xcd_remap ? xcd_block(blockIdx.x, nb) : (int)blockIdx.x;
-  const int n = b * NODE_BLOCK + (int)threadIdx.x;
+  const int n0 = b * NODE_BLOCK;
+  const int n = n0 + (int)threadIdx.x;
   const S *fsky = reinterpret_cast<const S *>(a.fsky);
   CycleCtl *ctl = a.ctl;
   const Scal sc = ctl->s[parity];
@@ -64,17 +76,34 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
       }
     }
   }
-  if (n >= a.numnod) return;
 
+  // ASSPAR4: the workgroup's nodes own the contiguous slot range [s0, s1); it is
+  // staged through LDS in tiles with coalesced loads, and each thread folds its
+  // own node's slots left to right, in slot order, across tile boundaries.
+  const int n1 = (n0 + NODE_BLOCK < a.numnod) ? n0 + NODE_BLOCK : a.numnod;
+  const int s0 = a.adsky[n0], s1 = a.adsky[n1];
+  const bool live = n < a.numnod;
+  const int k0 = live ? a.adsky[n] : 0, k1 = live ? a.adsky[n + 1] : 0;
   double ax = 0.0, ay = 0.0, az = 0.0, sf = 0.0;
-  const int k0 = a.adsky[n], k1 = a.adsky[n + 1];
-  for (int k = k0; k < k1; ++k) {
-    const S f = fsky[k];
-    ax = ax + double(f.x);
-    ay = ay + double(f.y);
-    az = az + double(f.z);
-    sf = sf + double(f.w);
+  for (int ts = s0; ts < s1; ts += TILE) {
+    const int te = (ts + TILE < s1) ? ts + TILE : s1;
+    __syncthreads();
+    for (int i = threadIdx.x; i < te - ts; i += NODE_BLOCK) {
+      const S f = fsky[ts + i];
+      const int j = lds_sw(i);
+      t_x[j] = f.x; t_y[j] = f.y; t_z[j] = f.z; t_w[j] = f.w;
+    }
+    __syncthreads();
+    const int lo = (k0 > ts) ? k0 : ts, hi = (k1 < te) ? k1 : te;
+    for (int k = lo; k < hi; ++k) {
+      const int j = lds_sw(k - ts);
+      ax = ax + double(t_x[j]);
+      ay = ay + double(t_y[j]);
+      az = az + double(t_z[j]);
+      sf = sf + double(t_w[j]);
+    }
   }
+  if (!live) return;
   if (mode == 0) {
     a.A[3 * n] = ax; a.A[3 * n + 1] = ay; a.A[3 * n + 2] = az;
     a.STIFN[n] = sf;
@@ -111,7 +140,7 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
 // Energy partial sums: block b sums its range in a fixed tree order.
 template <typename T>
 __global__ __launch_bounds__(NODE_BLOCK) void energy_partial_kernel(
-    int numels, const T *eint, const T *vol0, int numnod, const double4 *X4, const double4 *V4,
+    int numels, const T *eint, const double *vol0, int numnod, const double4 *X4, const double4 *V4,
     double *part_e, double *part_k, int nbe) {
   __shared__ double s[NODE_BLOCK];
   const int b = blockIdx.x;
@@ -172,7 +201,7 @@ hipError_t launch_node(const NodeArgs &a, bool slot_fp32, int parity, int mode, 
 }
 
 template <typename T>
-static hipError_t launch_energy_t(int numels, const T *eint, const T *vol0, int numnod,
+static hipError_t launch_energy_t(int numels, const T *eint, const double *vol0, int numnod,
                                   const double4 *X4, const double4 *V4, double *part,
                                   double *res, hipStream_t s) {
   const int nbe = (numels + NODE_BLOCK - 1) / NODE_BLOCK;
@@ -185,14 +214,12 @@ static hipError_t launch_energy_t(int numels, const T *eint, const T *vol0, int 
   return hipGetLastError();
 }
 
-hipError_t launch_energy(bool fp32, int numels, const void *eint, const void *vol0, int numnod,
+hipError_t launch_energy(bool fp32, int numels, const void *eint, const double *vol0, int numnod,
                          const double4 *X4, const double4 *V4, double *part, double *res,
                          hipStream_t s) {
   if (fp32)
-    return launch_energy_t<float>(numels, (const float *)eint, (const float *)vol0, numnod, X4, V4,
-                                  part, res, s);
-  return launch_energy_t<double>(numels, (const double *)eint, (const double *)vol0, numnod, X4,
-                                 V4, part, res, s);
+    return launch_energy_t<float>(numels, (const float *)eint, vol0, numnod, X4, V4, part, res, s);
+  return launch_energy_t<double>(numels, (const double *)eint, vol0, numnod, X4, V4, part, res, s);
 }
 
 }  // namespace orad

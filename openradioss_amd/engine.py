@@ -219,3 +219,30 @@ class BrickSolver:
         ei, ek, eh = C.c_double(), C.c_double(), C.c_double()
         self._chk(self.lib.orad_energies(self.h, C.byref(ei), C.byref(ek), C.byref(eh)))
         return ei.value, ek.value, eh.value
+
+
+DECK_MAGIC = 0x4F524144   # "ORAD"
+
+
+def write_deck(deck, path):
+    """Binary deck for the Fortran driver (openradioss_amd/fortran/resol_brick)."""
+    nn, ne = deck["x"].shape[0], deck["ixs"].shape[0]
+    mat, prop = deck["mat"], deck["prop"]
+    with open(path, "wb") as f:
+        np.array([DECK_MAGIC, 1, nn, ne], np.int32).tofile(f)
+        np.ascontiguousarray(deck["x"], np.float64).tofile(f)
+        np.ascontiguousarray(deck["v"], np.float64).tofile(f)
+        np.ascontiguousarray(deck["ms"], np.float64).tofile(f)
+        np.ascontiguousarray(deck["icodt"], np.int32).tofile(f)
+        ixs_from_deck(deck).tofile(f)
+        np.ascontiguousarray(deck["vol0"], np.float64).tofile(f)
+        np.array([mat[k] for k in ("rho0", "shear", "bulk", "ca", "cb", "cn", "epmx", "sigmx",
+                                   "cc", "epdr", "mexp")], np.float64).tofile(f)
+        np.array([mat["icc"], mat["vp"], mat["israte"], 0], np.int32).tofile(f)
+        np.array([prop[k] for k in ("qa", "qb", "hvis", "dtfac")], np.float64).tofile(f)
+
+
+def read_driver_output(path, numnod):
+    a = np.fromfile(path, np.float64)
+    x, v, d = a[:3 * numnod], a[3 * numnod:6 * numnod], a[6 * numnod:9 * numnod]
+    return x.reshape(numnod, 3), v.reshape(numnod, 3), d.reshape(numnod, 3)

@@ -1,0 +1,123 @@
+!  orad_bind.F90 -- ISO_C_BINDING interface to the C ABI in include/orad.h.
+!
+!  Style follows the reference's coupling adapters (engine/source/coupling/
+!  coupling_adapter.F90:83-131): type(c_ptr), value handles, scalars by value,
+!  arrays by reference in the engine's own layouts (X(3,NUMNOD), IXS(NIXS,NUMELS)).
+module orad_bind
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: ORAD_OK = 0, ORAD_EINVAL = 1, ORAD_EHIP = 2, &
+                               ORAD_ESTATE = 3, ORAD_ENEGVOL = 4
+  integer(c_int), parameter :: ORAD_FP64 = 0, ORAD_MIXED = 1
+
+  type, bind(c) :: orad_time_t
+    real(c_double) :: tt, dt1, dt2, dt12, dt2old, dtmx
+    integer(c_int) :: ncycle, mstop
+  end type orad_time_t
+
+  interface
+    integer(c_int) function orad_create(ctx, device, precision) bind(c, name='orad_create')
+      import :: c_ptr, c_int
+      type(c_ptr) :: ctx
+      integer(c_int), value :: device, precision
+    end function
+    integer(c_int) function orad_destroy(ctx) bind(c, name='orad_destroy')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    type(c_ptr) function orad_last_error(ctx) bind(c, name='orad_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function orad_set_nodes(ctx, numnod, x, v, ms, icodt) bind(c, name='orad_set_nodes')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: numnod
+      real(c_double) :: x(3,*), v(3,*), ms(*)
+      integer(c_int) :: icodt(*)
+    end function
+    integer(c_int) function orad_set_bricks(ctx, numels, ixs, vol0) bind(c, name='orad_set_bricks')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: numels
+      integer(c_int) :: ixs(11,*)
+      real(c_double) :: vol0(*)
+    end function
+    integer(c_int) function orad_set_law2(ctx, p, ip) bind(c, name='orad_set_law2')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: p(*)
+      integer(c_int) :: ip(*)
+    end function
+    integer(c_int) function orad_set_solid_prop(ctx, qa, qb, hvis, dtfac) bind(c, name='orad_set_solid_prop')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: qa, qb, hvis, dtfac
+    end function
+    integer(c_int) function orad_get_pon(ctx, adsky, iads) bind(c, name='orad_get_pon')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: adsky(*), iads(8,*)
+    end function
+    integer(c_int) function orad_set_time(ctx, t) bind(c, name='orad_set_time')
+      import :: c_ptr, c_int, orad_time_t
+      type(c_ptr), value :: ctx
+      type(orad_time_t) :: t
+    end function
+    integer(c_int) function orad_get_time(ctx, t) bind(c, name='orad_get_time')
+      import :: c_ptr, c_int, orad_time_t
+      type(c_ptr), value :: ctx
+      type(orad_time_t) :: t
+    end function
+    integer(c_int) function orad_sforc3(ctx, nft, nel, dt1, dt2t) bind(c, name='orad_sforc3')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nft, nel
+      real(c_double), value :: dt1
+      real(c_double) :: dt2t
+    end function
+    integer(c_int) function orad_asspar4(ctx, a, stifn) bind(c, name='orad_asspar4')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: a(3,*), stifn(*)
+    end function
+    integer(c_int) function orad_run_cycles(ctx, ncycle, t) bind(c, name='orad_run_cycles')
+      import :: c_ptr, c_int, orad_time_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ncycle
+      type(orad_time_t) :: t
+    end function
+    integer(c_int) function orad_get_nodes(ctx, x, v, d, a, stifn) bind(c, name='orad_get_nodes')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: x(3,*), v(3,*), d(3,*), a(3,*), stifn(*)
+    end function
+    integer(c_int) function orad_energies(ctx, eint, ekin, ehour) bind(c, name='orad_energies')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: eint, ekin, ehour
+    end function
+  end interface
+
+contains
+
+  ! Status check in place of the reference's ANCMSG/ARRET exit.
+  subroutine orad_check(ctx, rc, what)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (rc == ORAD_OK) return
+    call c_f_pointer(orad_last_error(ctx), msg, [512])
+    n = 0
+    do while (n < 512)
+      if (msg(n+1) == c_null_char) exit
+      n = n + 1
+    end do
+    write(*,'(a,a,a,i0,a,512a)') 'orad: ', what, ' failed, status ', rc, ': ', msg(1:n)
+    stop 2
+  end subroutine orad_check
+
+end module orad_bind

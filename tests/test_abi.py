@@ -41,3 +41,21 @@ def test_ixs_layout_is_the_engines():
     assert ixs.shape == (8, 11)
     assert ixs[:, 1:9].min() == 1 and ixs[:, 1:9].max() == d["x"].shape[0]
     assert list(ixs[:, 10]) == list(range(1, 9))
+
+
+def test_fortran_binding_declares_every_abi_entry():
+    """Every C entry point the engine needs has an ISO_C_BINDING interface."""
+    txt = open(os.path.join(ROOT, "openradioss_amd", "fortran", "orad_bind.F90")).read()
+    bound = set(re.findall(r"bind\(c, name='(orad_\w+)'\)", txt))
+    needed = set(declared()) - {"orad_profile_cycles", "orad_get_brick_state",
+                                "orad_set_brick_state", "orad_get_fsky"}
+    assert needed <= bound, needed - bound
+
+
+def test_deck_file_roundtrip(tmp_path):
+    from openradioss_amd import deck as dk
+    import numpy as np
+    d = dk.brick_block(2, 3, 2)
+    engine.write_deck(d, tmp_path / "d.bin")
+    raw = np.fromfile(tmp_path / "d.bin", np.int32, count=4)
+    assert list(raw) == [engine.DECK_MAGIC, 1, d["x"].shape[0], d["ixs"].shape[0]]
