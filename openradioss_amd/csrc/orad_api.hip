@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <numeric>
@@ -50,6 +51,8 @@ struct orad_ctx {
   void *state = nullptr;  // SoA element state of the storage type (NSTATE columns)
   double *vol0 = nullptr; // initial volumes, FP64
   void *fsky = nullptr;
+  void *sti = nullptr;              // per-element STI/4
+  int32_t *slot_elem = nullptr;     // element owning each slot
   double *ehou_part = nullptr;
   int n_ehou_part = 0;
   CycleCtl *ctl = nullptr;
@@ -113,7 +116,8 @@ static BrickArgs<T> brick_args(orad_ctx *c) {
   a.offg = base + 10 * ne;
   a.epsd = base + 11 * ne;
   a.vol0 = c->vol0;
-  a.fsky = (typename Vec4<T>::type *)c->fsky;
+  a.sti = (T *)c->sti;
+  a.fsky = (Slot3<T> *)c->fsky;
   a.ctl = c->ctl;
   a.mat = c->mat;
   a.prop = c->prop;
@@ -132,6 +136,8 @@ static NodeArgs node_args(orad_ctx *c) {
   a.A = c->A;
   a.STIFN = c->STIFN;
   a.fsky = c->fsky;
+  a.sti = c->sti;
+  a.slot_elem = c->slot_elem;
   a.ehou_part = c->ehou_part;
   a.n_ehou_part = c->n_ehou_part;
   a.ctl = c->ctl;
@@ -185,6 +191,7 @@ int orad_create(orad_ctx **out, int device, int precision) {
   if (!c) return ORAD_EINVAL;
   c->device = device;
   c->precision = precision;
+  if (const char *xr = std::getenv("ORAD_XCD_REMAP")) c->xcd_remap = std::atoi(xr) != 0;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = dalloc(&c->ctl, sizeof(CycleCtl));
@@ -219,7 +226,7 @@ int orad_destroy(orad_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void *p : {(void *)c->X4, (void *)c->V4, (void *)c->D, (void *)c->A, (void *)c->STIFN,
                   (void *)c->conn, (void *)c->iads, (void *)c->adsky, c->state, (void *)c->vol0,
-                  c->fsky,
+                  c->fsky, c->sti, (void *)c->slot_elem,
                   (void *)c->ehou_part, (void *)c->ctl, (void *)c->scratch_dt1,
                   (void *)c->scratch_dt2t})
     dfree(p);
@@ -317,8 +324,17 @@ int orad_set_bricks(orad_ctx *c, int numels, const int *ixs, const double *vol0)
   const size_t ts = c->tsize();
   HIPCHK(c, dalloc(&c->state, NSTATE * ne * ts));
   HIPCHK(c, hipMemset(c->state, 0, NSTATE * ne * ts));
-  HIPCHK(c, dalloc(&c->fsky, 8 * ne * 4 * ts));
-  HIPCHK(c, hipMemset(c->fsky, 0, 8 * ne * 4 * ts));
+  HIPCHK(c, dalloc(&c->fsky, 8 * ne * 3 * ts));
+  HIPCHK(c, hipMemset(c->fsky, 0, 8 * ne * 3 * ts));
+  HIPCHK(c, dalloc(&c->sti, ne * ts));
+  HIPCHK(c, hipMemset(c->sti, 0, ne * ts));
+  {
+    std::vector<int32_t> se(8 * ne);
+    for (size_t e = 0; e < ne; ++e)
+      for (int k = 0; k < 8; ++k) se[c->h_iads[8 * e + k]] = (int32_t)e;
+    HIPCHK(c, dalloc(&c->slot_elem, 8 * ne * sizeof(int32_t)));
+    HIPCHK(c, hipMemcpy(c->slot_elem, se.data(), 8 * ne * 4, hipMemcpyHostToDevice));
+  }
   c->n_ehou_part = (int)((ne + 255) / 256);
   HIPCHK(c, dalloc(&c->ehou_part, (size_t)c->n_ehou_part * sizeof(double)));
   HIPCHK(c, hipMemset(c->ehou_part, 0, (size_t)c->n_ehou_part * sizeof(double)));
@@ -587,14 +603,26 @@ int orad_get_fsky(orad_ctx *c, double *fsky) {
   if (!c->have_bricks) return fail(c, ORAD_ESTATE, "get_fsky: no mesh");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  const size_t ns = 8 * (size_t)c->numels;
+  const size_t ne = (size_t)c->numels, ns = 8 * ne;
+  std::vector<double> f(3 * ns), st(ne);
   if (c->precision == ORAD_MIXED) {
-    std::vector<float> h(4 * ns);
+    std::vector<float> h(3 * ns), hs(ne);
     HIPCHK(c, hipMemcpy(h.data(), c->fsky, h.size() * 4, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < h.size(); ++i) fsky[i] = h[i];
+    HIPCHK(c, hipMemcpy(hs.data(), c->sti, ne * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < h.size(); ++i) f[i] = h[i];
+    for (size_t i = 0; i < ne; ++i) st[i] = hs[i];
   } else {
-    HIPCHK(c, hipMemcpy(fsky, c->fsky, 4 * ns * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(f.data(), c->fsky, 3 * ns * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(st.data(), c->sti, ne * 8, hipMemcpyDeviceToHost));
   }
+  for (size_t e = 0; e < ne; ++e)
+    for (int k = 0; k < 8; ++k) {
+      const size_t sl = (size_t)c->h_iads[8 * e + k];
+      fsky[4 * sl] = f[3 * sl];
+      fsky[4 * sl + 1] = f[3 * sl + 1];
+      fsky[4 * sl + 2] = f[3 * sl + 2];
+      fsky[4 * sl + 3] = st[e];
+    }
   return ORAD_OK;
 }
 

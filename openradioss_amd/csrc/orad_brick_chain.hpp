@@ -21,9 +21,15 @@ template <> struct M<double> {
   static __device__ __forceinline__ double two_third_pow(double x) { return pow(x, 2.0 / 3.0); }
   static __device__ __forceinline__ double log_(double x) { return log(x); }
   static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+  // aa = SQRT(x); IF (aa /= 0) aa = 1/aa  (sortho3.F)
+  static __device__ __forceinline__ double rnorm(double x) {
+    double a = __builtin_sqrt(x);
+    return (a != 0.0) ? 1.0 / a : a;
+  }
 };
 template <> struct M<float> {
-  static __device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+  static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float pow_(float x, float y) {
     return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
   }
@@ -35,6 +41,11 @@ template <> struct M<float> {
   }
   static __device__ __forceinline__ float log_(float x) { return __logf(x); }
   static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  // FP32 form: hardware reciprocal / square root (1 ulp) instead of the IEEE sequences.
+  static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+  static __device__ __forceinline__ float rnorm(float x) {
+    return (x > 0.0f) ? __builtin_amdgcn_rsqf(x) : 0.0f;
+  }
 };
 
 template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return (a < b) ? a : b; }
@@ -92,11 +103,11 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
   T R[3][3];  // R[i][j] = R(i+1,j+1); column j is e_(j+1)
   {
     T aa, bb, U[3], V[3], W[3], e1[3], e2[3], e3[3];
-    aa = M<T>::sqrt_(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]); if (aa != ZERO) aa = ONE / aa;
+    aa = M<T>::rnorm(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
     U[0] = r[0] * aa; U[1] = r[1] * aa; U[2] = r[2] * aa;
-    aa = M<T>::sqrt_(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]); if (aa != ZERO) aa = ONE / aa;
+    aa = M<T>::rnorm(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
     V[0] = s[0] * aa; V[1] = s[1] * aa; V[2] = s[2] * aa;
-    aa = M<T>::sqrt_(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]); if (aa != ZERO) aa = ONE / aa;
+    aa = M<T>::rnorm(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
     W[0] = t[0] * aa; W[1] = t[1] * aa; W[2] = t[2] * aa;
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
@@ -109,18 +120,18 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
       e3[0] = U[1] * V[2] - U[2] * V[1] + W[0];
       e3[1] = U[2] * V[0] - U[0] * V[2] + W[1];
       e3[2] = U[0] * V[1] - U[1] * V[0] + W[2];
-      bb = M<T>::sqrt_(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]); if (bb != ZERO) bb = ONE / bb;
+      bb = M<T>::rnorm(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
       U[0] = e1[0] * bb; U[1] = e1[1] * bb; U[2] = e1[2] * bb;
-      bb = M<T>::sqrt_(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]); if (bb != ZERO) bb = ONE / bb;
+      bb = M<T>::rnorm(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
       V[0] = e2[0] * bb; V[1] = e2[1] * bb; V[2] = e2[2] * bb;
-      bb = M<T>::sqrt_(e3[0] * e3[0] + e3[1] * e3[1] + e3[2] * e3[2]); if (bb != ZERO) bb = ONE / bb;
+      bb = M<T>::rnorm(e3[0] * e3[0] + e3[1] * e3[1] + e3[2] * e3[2]);
       W[0] = e3[0] * bb; W[1] = e3[1] * bb; W[2] = e3[2] * bb;
     }
     e1[0] = U[0]; e1[1] = U[1]; e1[2] = U[2];
     e3[0] = e1[1] * V[2] - e1[2] * V[1];
     e3[1] = e1[2] * V[0] - e1[0] * V[2];
     e3[2] = e1[0] * V[1] - e1[1] * V[0];
-    aa = M<T>::sqrt_(e3[0] * e3[0] + e3[1] * e3[1] + e3[2] * e3[2]); if (aa != ZERO) aa = ONE / aa;
+    aa = M<T>::rnorm(e3[0] * e3[0] + e3[1] * e3[1] + e3[2] * e3[2]);
     e3[0] = e3[0] * aa; e3[1] = e3[1] * aa; e3[2] = e3[2] * aa;
     e2[0] = e3[1] * e1[2] - e3[2] * e1[1];
     e2[1] = e3[2] * e1[0] - e3[0] * e1[2];
@@ -175,7 +186,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
       det = ONE; off = ZERO; offg = ZERO; mstop = 1;
     }
     voln = det;
-    T DETT = K<T>::ONE_OVER_64 / det;
+    T DETT = M<T>::div(K<T>::ONE_OVER_64, det);
     T JACI1 = DETT * JAC_59_68, JACI4 = DETT * JAC_67_49, JACI7 = DETT * JAC_48_57;
     T JACI2 = DETT * (JAC3 * JAC8 - JAC2 * JAC9);
     T JACI5 = DETT * (JAC1 * JAC9 - JAC3 * JAC7);
@@ -227,7 +238,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
       T ar = E * G - F * F;
       aream = tmax(ar, aream);
     }
-    deltax = FOUR * voln * ONE / M<T>::sqrt_(aream);
+    deltax = M<T>::div(FOUR * voln * ONE, M<T>::sqrt_(aream));
   }
 
   // ---- SDEFO3, JCVT/=0 branch (sdefo3.F:114-138,164-194) ----
@@ -271,7 +282,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
   } else {
     double vdp = (off == ZERO || mstop) ? 1.0 : voldp;   // SCHKJABT3 set DET = 1
     if (offg == ZERO) { vdp = vol0d; voln = vol0; }
-    dvol = T(vdp - double(rho0 / st_rho) * vol0d);
+    dvol = T(vdp - double(M<T>::div(rho0, st_rho)) * vol0d);
     rho = T(mat.rho0 * (vol0d / vdp));
     amu = T(vol0d / vdp - 1.0);
   }
@@ -288,7 +299,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
   T P = -K<T>::THIRD * (sig[0] + sig[1] + sig[2]);
   T DAV = -K<T>::THIRD * (DXX + DYY + DZZ);
   T G1 = dt1 * G, G2 = TWO * G1;
-  T SSP = M<T>::sqrt_((K<T>::ONEP333 * G + bulk) / rho0);
+  T SSP = M<T>::sqrt_(M<T>::div(K<T>::ONEP333 * G + bulk, rho0));
   sig[0] = sig[0] + P + G2 * (DXX + DAV);
   sig[1] = sig[1] + P + G2 * (DYY + DAV);
   sig[2] = sig[2] + P + G2 * (DZZ + DAV);
@@ -308,7 +319,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
     T E1 = DXX - dav, E2 = DYY - dav, E3 = DZZ - dav;
     T E4 = HALF * D4, E5 = HALF * D5, E6 = HALF * D6;
     T EPSP = HALF * (E1 * E1 + E2 * E2 + E3 * E3) + E4 * E4 + E5 * E5 + E6 * E6;
-    EPSDOT = M<T>::sqrt_(THREE * EPSP) / T(1.5);
+    EPSDOT = M<T>::div(M<T>::sqrt_(THREE * EPSP), T(1.5));
   }
   T epsd = EPSDOT;
   if (mat.israte > 0) {
@@ -319,7 +330,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
   T EPD = ONE;
   if (mat.cc != 0.0) {
     EPD = (mat.vp == 1) ? tmax(epsd, T(mat.epdr)) : tmax(epsd, K<T>::EM15);
-    EPD = M<T>::log_(EPD / T(mat.epdr));
+    EPD = M<T>::log_(M<T>::div(EPD, T(mat.epdr)));
     EPD = tmax(ZERO, EPD);
     EPD = (ONE + T(mat.cc) * EPD) * ONE;
     if (mat.icc == 1) SIGMX = T(mat.sigmx) * EPD;
@@ -334,17 +345,17 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
   } else if (EPXE > ZERO) {
     AK = CA + CB * M<T>::pow_(EPXE, CN);
     if (CN > ONE) QH = (CB * CN * M<T>::pow_(EPXE, CN - ONE)) * EPD;
-    else          QH = (CB * CN / M<T>::pow_(EPXE, ONE - CN)) * EPD;
+    else          QH = M<T>::div(CB * CN, M<T>::pow_(EPXE, ONE - CN)) * EPD;
   } else {
     AK = CA; QH = ZERO;
   }
   AK = AK * EPD;
   if (SIGMX < AK) { AK = SIGMX; QH = ZERO; }
   if (EPXE > T(mat.epmx)) { AK = ZERO; QH = ZERO; }
-  T SCALE = tmin(ONE, AK / tmax(AJ2, K<T>::EM15));
-  T DPLA = (ONE - SCALE) * AJ2 / tmax(THREE * G + QH, K<T>::EM15);
+  T SCALE = tmin(ONE, M<T>::div(AK, tmax(AJ2, K<T>::EM15)));
+  T DPLA = M<T>::div((ONE - SCALE) * AJ2, tmax(THREE * G + QH, K<T>::EM15));
   AK = AK + ONE * DPLA * QH;
-  SCALE = tmin(ONE, AK / tmax(AJ2, K<T>::EM15));
+  SCALE = tmin(ONE, M<T>::div(AK, tmax(AJ2, K<T>::EM15)));
 #pragma unroll
   for (int i = 0; i < 6; ++i) sig[i] = SCALE * sig[i];
   EPXE = EPXE + DPLA;
@@ -362,10 +373,10 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
   T QX = QB * SSP + AL * QAA;
   T qnew = rho * AD * AL * (QAA * AL + QB * SSP);
   T SSP_EQ = tmax(K<T>::EM20, QX + M<T>::sqrt_(QX * QX + CX * CX));
-  T DTX = deltax / SSP_EQ;
+  T DTX = M<T>::div(deltax, SSP_EQ);
   T STI = ZERO;
   if (!(off == ZERO || offg < ZERO)) {
-    T TIDT = ONE / DTX;
+    T TIDT = M<T>::div(ONE, DTX);
     T TRHO = rho * TIDT;
     T TVOL = voln * TIDT;
     STI = TRHO * TVOL;
@@ -388,7 +399,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
     T E5 = D5 * (sold[4] + sig[4]);
     T E6 = D6 * (sold[5] + sig[5]);
     T EINC = vol_avg * (E1 + E2 + E3 + E4 + E5 + E6) * DTA - HALF * dvol * (st_qvis + qnew);
-    eint = (eint + EINC * off) / tmax(K<T>::EM15, vol0);
+    eint = M<T>::div(eint + EINC * off, tmax(K<T>::EM15, vol0));
   }
 
   // ---- SHVIS3 (shvis3.F:163-238,317-428) ----

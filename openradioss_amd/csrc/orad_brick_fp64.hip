@@ -2,6 +2,11 @@
 // arithmetic follows the reference's double-precision build operation by operation.
 #include "orad_brick_kernel.hpp"
 
+// Minimum waves per SIMD requested from the register allocator.
+#ifndef ORAD_MINW
+#define ORAD_MINW 1
+#endif
+
 namespace orad {
 
 hipError_t launch_brick_fp64(const BrickArgs<double> &a, int e_begin, int e_end,
@@ -10,7 +15,7 @@ hipError_t launch_brick_fp64(const BrickArgs<double> &a, int e_begin, int e_end,
   const int n = e_end - e_begin;
   if (n <= 0) return hipSuccess;
   const int nb = (n + BRICK_BLOCK - 1) / BRICK_BLOCK;
-  hipLaunchKernelGGL((brick_forces_kernel<double, false>), dim3(nb), dim3(BRICK_BLOCK), 0, s,
+  hipLaunchKernelGGL((brick_forces_kernel<double, false, ORAD_MINW>), dim3(nb), dim3(BRICK_BLOCK), 0, s,
                      a, e_begin, e_end, dt1p, dt2t, ehou_part, xcd_remap);
   return hipGetLastError();
 }

@@ -2,8 +2,8 @@
 //
 // Per element it gathers the eight corners' X and V (FP64, 32-byte records),
 // runs the SFORC3 chain (orad_brick_chain.hpp), writes the element state back
-// (structure of arrays, coalesced) and the eight PARITH/ON corner slots at
-// IADS (SCUMU3P).  The group's stable time step is a wavefront shuffle min,
+// (structure of arrays, coalesced, non-temporal), STI once per element, and the
+// eight PARITH/ON corner force slots at IADS (SCUMU3P).  The group's stable time step is a wavefront shuffle min,
 // then one 64-bit atomicMin per workgroup on the bits of a positive double
 // (order independent, so deterministic).  Hourglass energy is summed per
 // workgroup in a fixed tree order into ehou_part[block].
@@ -14,17 +14,10 @@ namespace orad {
 
 constexpr int BRICK_BLOCK = 256;
 
-template <typename T> struct Store4;
-template <> struct Store4<double> {
-  static __device__ __forceinline__ double4 make(double a, double b, double c, double d) {
-    return make_double4(a, b, c, d);
-  }
-};
-template <> struct Store4<float> {
-  static __device__ __forceinline__ float4 make(float a, float b, float c, float d) {
-    return make_float4(a, b, c, d);
-  }
-};
+// Element state is read once and written once per cycle: non-temporal accesses
+// keep it from displacing the corner-node records that neighbouring elements reuse.
+template <typename T> __device__ __forceinline__ T ldnt(const T *p) { return __builtin_nontemporal_load(p); }
+template <typename T> __device__ __forceinline__ void stnt(T *p, T v) { __builtin_nontemporal_store(v, p); }
 
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
@@ -56,8 +49,9 @@ __device__ __forceinline__ double jacobian_det(const double x[8], const double y
 }
 
 // REL: coordinates/velocities relative to corner 1 (mixed form).
-template <typename T, bool REL>
-__global__ __launch_bounds__(BRICK_BLOCK) void brick_forces_kernel(
+template <typename T, bool REL, int MINW>
+__global__ __launch_bounds__(BRICK_BLOCK) __attribute__((amdgpu_waves_per_eu(MINW)))
+void brick_forces_kernel(
     BrickArgs<T> a, int e_begin, int e_end, const double *dt1p,
     unsigned long long *dt2t_out, double *ehou_part, int xcd_remap) {
   __shared__ double s_dt[BRICK_BLOCK / 64];
@@ -71,7 +65,7 @@ __global__ __launch_bounds__(BRICK_BLOCK) void brick_forces_kernel(
     const int ne = a.numels;
     int nd[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) nd[k] = a.conn[k * ne + e];
+    for (int k = 0; k < 8; ++k) nd[k] = ldnt(&a.conn[k * ne + e]);
     T xd[8], yd[8], zd[8], vx[8], vy[8], vz[8];
     double voldp = 0.0;
     if (REL) {
@@ -98,22 +92,25 @@ __global__ __launch_bounds__(BRICK_BLOCK) void brick_forces_kernel(
     }
     T sig[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) sig[c] = a.sig[c * ne + e];
-    T eint = a.eint[e], rho = a.rho[e], pla = a.pla[e], qvis = a.qvis[e];
-    T offg = a.offg[e], epsd = a.epsd[e];
-    const double vol0 = a.vol0[e];
+    for (int c = 0; c < 6; ++c) sig[c] = ldnt(&a.sig[c * ne + e]);
+    T eint = ldnt(&a.eint[e]), rho = ldnt(&a.rho[e]), pla = ldnt(&a.pla[e]);
+    T qvis = ldnt(&a.qvis[e]), offg = ldnt(&a.offg[e]), epsd = ldnt(&a.epsd[e]);
+    const double vol0 = ldnt(&a.vol0[e]);
     const T offg_in = offg;
     BrickOut<T> o;
     brick_chain<T>(xd, yd, zd, vx, vy, vz, sig, eint, rho, vol0, voldp, pla, qvis, offg, epsd,
                    dt1, a.mat, a.prop, o);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) a.sig[c * ne + e] = sig[c];
-    a.eint[e] = eint; a.rho[e] = rho; a.pla[e] = pla; a.qvis[e] = qvis; a.epsd[e] = epsd;
+    for (int c = 0; c < 6; ++c) stnt(&a.sig[c * ne + e], sig[c]);
+    stnt(&a.eint[e], eint); stnt(&a.rho[e], rho); stnt(&a.pla[e], pla);
+    stnt(&a.qvis[e], qvis); stnt(&a.epsd[e], epsd); stnt(&a.sti[e], o.sti);
     if (offg != offg_in) a.offg[e] = offg;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const int slot = a.iads[k * ne + e];
-      a.fsky[slot] = Store4<T>::make(o.f[0][k], o.f[1][k], o.f[2][k], o.sti);
+      const int slot = ldnt(&a.iads[k * ne + e]);
+      Slot3<T> f;
+      f.x = o.f[0][k]; f.y = o.f[1][k]; f.z = o.f[2][k];
+      a.fsky[slot] = f;
     }
     if (o.mstop) atomicOr(&a.ctl->mstop, 1);
     my_dt = double(o.dtx);

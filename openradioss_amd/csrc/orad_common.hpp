@@ -7,8 +7,8 @@
 //   arrays indexed by element:  conn[k*ne+e], iads[k*ne+e] (int32, k = corner),
 //   sig[c*ne+e] (c = 0..5), eint, rho, pla, qvis, offg, epsd; vol0[e] is FP64
 //   in both forms (the reference's single-precision build keeps VOL0DP too).
-//   PARITH/ON slots: fsky[s] = (fx, fy, fz, sti) as a T4, slots in node-major
-//   CSR order (adsky[n]..adsky[n+1]-1).
+//   PARITH/ON slots: fsky[s] = (fx, fy, fz) in precision T, slots in node-major
+//   CSR order (adsky[n]..adsky[n+1]-1); sti[e] = STI/4 once per element.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -39,9 +39,9 @@ struct CycleCtl {
   double ehour;                 // cumulative hourglass energy
 };
 
-template <typename T> struct Vec4;
-template <> struct Vec4<double> { typedef double4 type; };
-template <> struct Vec4<float> { typedef float4 type; };
+// One PARITH/ON slot: the corner force (FSKY(1:3,·)).  STI (FSKY(7,·)) is the same
+// for the eight corners of a brick and is kept once per element.
+template <typename T> struct Slot3 { T x, y, z; };
 
 template <typename T>
 struct BrickArgs {
@@ -53,7 +53,8 @@ struct BrickArgs {
   T *sig;                   // [6][ne]
   T *eint, *rho, *pla, *qvis, *offg, *epsd;
   const double *vol0;       // initial volumes, FP64 in both forms (VOL0DP)
-  typename Vec4<T>::type *fsky;
+  T *sti;                   // STI/4 per element (SCUMU3P's FSKY(7,·) value)
+  Slot3<T> *fsky;
   double *ehou_part;        // per-block hourglass energy partials
   CycleCtl *ctl;
   Law2 mat;
@@ -68,7 +69,9 @@ struct NodeArgs {
   double *D;
   double *A;
   double *STIFN;
-  const void *fsky;         // float4 or double4 slots
+  const void *fsky;         // Slot3<float> or Slot3<double> slots
+  const void *sti;          // per-element STI (float or double), read by ASSPAR4 only
+  const int32_t *slot_elem; // element of each slot, for STIFN in ASSPAR4
   const double *ehou_part;
   int n_ehou_part;
   CycleCtl *ctl;

@@ -2,7 +2,8 @@
 //
 // node_kernel fuses, per node n and in the reference's order:
 //   ASSPAR4  (assembly/asspar4.F:157-175)  a = sum of FSKY slots adsky[n]..adsky[n+1]-1
-//            in slot order (PARITH/ON: bitwise independent of launch shape);
+//            in slot order (PARITH/ON: bitwise independent of launch shape); STIFN
+//            only when ASSPAR4 is called on its own (mode 0);
 //   DT2      (resol.F:2640,5857-5861,6080,6224-6226) from the element minimum;
 //   ACCELE   (assembly/accele.F:64-82)       a = a / MS, 0 where MS <= 0;
 //   BCS10    (constraints/general/bcs/bcs1.F) zero fixed components of a and v;
@@ -17,20 +18,20 @@ namespace orad {
 
 constexpr int NODE_BLOCK = 256;
 
-template <typename S> struct SlotScalar;
-template <> struct SlotScalar<float4> { typedef float type; static constexpr int TILE = 2048; };
-template <> struct SlotScalar<double4> { typedef double type; static constexpr int TILE = 1024; };
+template <typename R> struct SlotTile;
+template <> struct SlotTile<float> { static constexpr int TILE = 2048; };
+template <> struct SlotTile<double> { static constexpr int TILE = 1024; };
 
 // LDS index swizzle: permutes each aligned run of 32 words, so the per-node
 // strided reads below (about 8 words apart) spread over the banks.
 __device__ __forceinline__ int lds_sw(int i) { return i ^ ((i >> 5) & 31); }
 
-template <typename S>
+template <typename R>
 __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity, int mode,
                                                           int write_a, int xcd_remap) {
-  typedef typename SlotScalar<S>::type R;
-  constexpr int TILE = SlotScalar<S>::TILE;
-  __shared__ R t_x[TILE], t_y[TILE], t_z[TILE], t_w[TILE];
+  typedef Slot3<R> S;
+  constexpr int TILE = SlotTile<R>::TILE;
+  __shared__ R t_x[TILE], t_y[TILE], t_z[TILE];
   const int nb = gridDim.x;
   const int b = xcd_remap ? xcd_block(blockIdx.x, nb) : (int)blockIdx.x;
   const int n0 = b * NODE_BLOCK;
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
     for (int i = threadIdx.x; i < te - ts; i += NODE_BLOCK) {
       const S f = fsky[ts + i];
       const int j = lds_sw(i);
-      t_x[j] = f.x; t_y[j] = f.y; t_z[j] = f.z; t_w[j] = f.w;
+      t_x[j] = f.x; t_y[j] = f.y; t_z[j] = f.z;
     }
     __syncthreads();
     const int lo = (k0 > ts) ? k0 : ts, hi = (k1 < te) ? k1 : te;
@@ -100,11 +101,13 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
       ax = ax + double(t_x[j]);
       ay = ay + double(t_y[j]);
       az = az + double(t_z[j]);
-      sf = sf + double(t_w[j]);
     }
   }
   if (!live) return;
   if (mode == 0) {
+    // STIFN (FSKY(7,·) summed in slot order); not consumed by the explicit cycle.
+    const R *sti = reinterpret_cast<const R *>(a.sti);
+    for (int k = k0; k < k1; ++k) sf = sf + double(sti[a.slot_elem[k]]);
     a.A[3 * n] = ax; a.A[3 * n + 1] = ay; a.A[3 * n + 2] = az;
     a.STIFN[n] = sf;
     return;
@@ -124,7 +127,6 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
   if (code & 1) { V.z = 0.0; az = 0.0; }
   if (write_a) {
     a.A[3 * n] = ax; a.A[3 * n + 1] = ay; a.A[3 * n + 2] = az;
-    a.STIFN[n] = sf;
   }
   V.x = V.x + dt12 * ax;
   V.y = V.y + dt12 * ay;
@@ -192,10 +194,10 @@ hipError_t launch_node(const NodeArgs &a, bool slot_fp32, int parity, int mode, 
   const int nb = (a.numnod + NODE_BLOCK - 1) / NODE_BLOCK;
   if (nb <= 0) return hipSuccess;
   if (slot_fp32)
-    hipLaunchKernelGGL(node_kernel<float4>, dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity, mode,
+    hipLaunchKernelGGL(node_kernel<float>, dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity, mode,
                        write_a, xcd_remap);
   else
-    hipLaunchKernelGGL(node_kernel<double4>, dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity, mode,
+    hipLaunchKernelGGL(node_kernel<double>, dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity, mode,
                        write_a, xcd_remap);
   return hipGetLastError();
 }
