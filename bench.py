@@ -31,16 +31,15 @@ HBM_PEAK_GBS = 8000.0
 def elem_bytes(ne, nn, tsize):
     """Algorithmic HBM bytes of one element-kernel launch: every array element it
     must touch, once, at its stored width; nodes once per node (ideal reuse)."""
-    reads = 8 * 4 + 8 * 4 + 13 * tsize     # conn, IADS, sig(6) eint rho vol0 pla qvis offg epsd
-    writes = 11 * tsize + 8 * 4 * tsize    # sig(6) eint rho pla qvis epsd, 8 FSKY slots
+    reads = 8 * 4 + 8 * 4 + 12 * tsize + 8   # conn, IADS, sig(6) eint rho pla qvis offg epsd, vol0
+    writes = 12 * tsize + 8 * 3 * tsize      # sig(6) eint rho pla qvis epsd sti, 8 force slots
     return ne * (reads + writes) + nn * (32 + 32)   # X4, V4 records
 
 
 def node_bytes(ne, nn, tsize):
-    """Algorithmic bytes of one nodal-kernel launch."""
-    slots = 8 * ne * 4 * tsize                                        # every FSKY slot read once
-    per_node = 8 + 32 + 32 + 24 + 32 + 32 + 24                        # adsky, X4/V4/D read + write
-    return slots + nn * per_node
+    """Algorithmic bytes of one nodal-kernel launch: every force slot once, and per node
+    ADSKY (8) + X4, V4 (32 + 32) + D (24) read, X4, V4, D written."""
+    return 8 * ne * 3 * tsize + nn * (8 + 32 + 32 + 24 + 32 + 32 + 24)
 
 
 def run(prec, deck, steps, warmup):
@@ -160,7 +159,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "brick_forces_kernel<float>",
                      "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS,
-                     "traffic": (measured_traffic("brick_forces_kernel<float,true>")
+                     "traffic": (measured_traffic("brick_forces_kernel<float, true, 2>")
                                  if n == 200 else None),
                      "algorithmic_bytes_per_launch": eb, "mean_launch_ms": el32},
         "cycle_hbm_frac": cycle_gbs / HBM_PEAK_GBS,
