@@ -81,15 +81,19 @@ def copy_calibration():
     return 0
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary
-    (profiles/counters_200cube.json), or None."""
+def measured_traffic(prefix):
+    """HBM bytes per launch of the kernel whose name starts with `prefix`, from the
+    committed counter summary (profiles/counters_200cube.json), or None."""
     path = os.path.join(ROOT, "profiles", "counters_200cube.json")
     try:
         with open(path) as f:
-            return json.load(f)["kernels"][kernel]["hbm_bytes_per_launch"]
+            kernels = json.load(f)["kernels"]
     except (OSError, KeyError, ValueError):
         return None
+    for name, rec in kernels.items():
+        if name.startswith(prefix):
+            return rec["hbm_bytes_per_launch"]
+    return None
 
 
 def main():
@@ -159,7 +163,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "brick_forces_kernel<float>",
                      "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS,
-                     "traffic": (measured_traffic("brick_forces_kernel<float, true, 2>")
+                     "traffic": (measured_traffic("brick_forces_kernel<float")
                                  if n == 200 else None),
                      "algorithmic_bytes_per_launch": eb, "mean_launch_ms": el32},
         "cycle_hbm_frac": cycle_gbs / HBM_PEAK_GBS,

@@ -20,9 +20,11 @@
 
 namespace orad {
 hipError_t launch_brick_fp64(const BrickArgs<double> &, int, int, const double *,
-                             unsigned long long *, double *, int, hipStream_t);
+                             unsigned long long *, double *, int, int, hipStream_t);
 hipError_t launch_brick_mixed(const BrickArgs<float> &, int, int, const double *,
-                              unsigned long long *, double *, int, hipStream_t);
+                              unsigned long long *, double *, int, int, hipStream_t);
+int brick_block_fp64();
+int brick_block_mixed();
 hipError_t launch_node(const NodeArgs &, bool, int, int, int, int, hipStream_t);
 hipError_t launch_energy(bool, int, const void *, const double *, int, const double4 *,
                          const double4 *, double *, double *, hipStream_t);
@@ -62,6 +64,7 @@ struct orad_ctx {
   int host_ncycle = 0;  // cycle counter mirrored on the host (its parity selects buffers)
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // two cycles starting at parity 0 / 1
   int xcd_remap = 1;
+  int elem_lds = -1;        // dynamic LDS bytes per element workgroup; < 0: the form's default
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
   size_t tsize() const { return precision == ORAD_MIXED ? sizeof(float) : sizeof(double); }
@@ -98,6 +101,9 @@ static void drop_graphs(orad_ctx *c) {
     }
 }
 
+static constexpr int NSTATE = 12;  // sig(6) eint rho pla qvis offg epsd
+enum { C_EINT = 6, C_RHO = 7, C_PLA = 8, C_QVIS = 9, C_OFFG = 10, C_EPSD = 11 };
+
 template <typename T>
 static BrickArgs<T> brick_args(orad_ctx *c) {
   BrickArgs<T> a{};
@@ -123,8 +129,6 @@ static BrickArgs<T> brick_args(orad_ctx *c) {
   a.prop = c->prop;
   return a;
 }
-static constexpr int NSTATE = 12;  // sig(6) eint rho pla qvis offg epsd
-enum { C_EINT = 6, C_RHO = 7, C_PLA = 8, C_QVIS = 9, C_OFFG = 10, C_EPSD = 11 };
 
 static NodeArgs node_args(orad_ctx *c) {
   NodeArgs a{};
@@ -148,9 +152,9 @@ static hipError_t launch_elements(orad_ctx *c, int e0, int e1, const double *dt1
                                   unsigned long long *dt2t, double *ehou_part) {
   if (c->precision == ORAD_MIXED)
     return launch_brick_mixed(brick_args<float>(c), e0, e1, dt1p, dt2t, ehou_part, c->xcd_remap,
-                              c->stream);
+                              c->elem_lds, c->stream);
   return launch_brick_fp64(brick_args<double>(c), e0, e1, dt1p, dt2t, ehou_part, c->xcd_remap,
-                           c->stream);
+                           c->elem_lds, c->stream);
 }
 
 // One cycle's launches at parity p (no host synchronisation, graph-capturable).
@@ -192,6 +196,7 @@ int orad_create(orad_ctx **out, int device, int precision) {
   c->device = device;
   c->precision = precision;
   if (const char *xr = std::getenv("ORAD_XCD_REMAP")) c->xcd_remap = std::atoi(xr) != 0;
+  if (const char *el = std::getenv("ORAD_ELEM_LDS_KB")) c->elem_lds = 1024 * std::atoi(el);
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = dalloc(&c->ctl, sizeof(CycleCtl));
@@ -335,7 +340,8 @@ int orad_set_bricks(orad_ctx *c, int numels, const int *ixs, const double *vol0)
     HIPCHK(c, dalloc(&c->slot_elem, 8 * ne * sizeof(int32_t)));
     HIPCHK(c, hipMemcpy(c->slot_elem, se.data(), 8 * ne * 4, hipMemcpyHostToDevice));
   }
-  c->n_ehou_part = (int)((ne + 255) / 256);
+  const int bb = c->precision == ORAD_MIXED ? brick_block_mixed() : brick_block_fp64();
+  c->n_ehou_part = (int)((ne + bb - 1) / bb);
   HIPCHK(c, dalloc(&c->ehou_part, (size_t)c->n_ehou_part * sizeof(double)));
   HIPCHK(c, hipMemset(c->ehou_part, 0, (size_t)c->n_ehou_part * sizeof(double)));
   HIPCHK(c, dalloc(&c->vol0, ne * sizeof(double)));

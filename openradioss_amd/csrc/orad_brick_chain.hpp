@@ -51,9 +51,10 @@ template <> struct M<float> {
 template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return (a < b) ? a : b; }
 template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return (a > b) ? a : b; }
 
+// Element outputs besides the corner forces, which go out through `emit(k, fx, fy, fz)`
+// (global frame) as soon as each corner is complete.
 template <typename T>
 struct BrickOut {
-  T f[3][8];       // global-frame corner forces
   T sti;           // STI/4 (SCUMU3P)
   T dtx;           // DTFAC*DTX, or a huge value when the element gives no candidate
   T ehou;          // hourglass energy increment
@@ -70,19 +71,46 @@ template <typename T> struct K {
   static constexpr T ONE_OVER_64 = T(0x1.0p-6);
 };
 
+// Hourglass base vectors Gamma_m (shvis3.F:317-353 without the PxHy terms).
+__device__ __forceinline__ int gamma_sign(int m, int k) {
+  const int g[3][8] = {{1, -1, 1, -1, 1, -1, 1, -1},
+                       {1, 1, -1, -1, -1, -1, 1, 1},
+                       {1, -1, -1, 1, -1, 1, 1, -1}};
+  return g[m][k];
+}
+__device__ __forceinline__ float gamma_of(int m, int k) { return float(gamma_sign(m, k)); }
+
+// Reference gamma vector G(k, m) = Gamma_m[k] -/+ PX(a)H(m) (shvis3.F:317-353):
+// corners 1-4 subtract PX1..PX4 H, corners 5..8 add PX3, PX4, PX1, PX2 H.
+template <typename T>
+__device__ __forceinline__ T G_of(int m, int k, const T PH[3][4]) {
+  const int ak[8] = {0, 1, 2, 3, 2, 3, 0, 1};
+  const T one = T(1);
+  const T base = (gamma_sign(m, k) > 0) ? one : -one;
+  return (k < 4) ? base - PH[m][ak[k]] : base + PH[m][ak[k]];
+}
+
 // st_* are the element's state values (in/out).  xd/yd/zd, vx/vy/vz: the eight
 // corners' coordinates and velocities (absolute for T=double, relative to corner
 // 1 for the mixed form).  vol0d: initial volume (FP64).  voldp: the mixed form's
 // FP64 Jacobian determinant (VOLDP), used, as the reference's single-precision
 // build does (srho3.F, IRESP=1 branch), for density, volume change and the
 // volumetric strain mu, where FP32 would cancel; ignored when T = double.
-template <typename T>
-__device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], T vy[8], T vz[8],
+// Corner values of one direction: plain registers, or a column of an LDS tile
+// (thread-major, stride = workgroup size) to keep them out of registers.
+template <typename T, int STRIDE>
+struct Col {
+  T *p;
+  __device__ __forceinline__ T &operator[](int k) const { return p[k * STRIDE]; }
+};
+
+template <typename T, typename VA, typename Emit>
+__device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA vy, VA vz,
                                             T sig[6], T &st_eint, T &st_rho, double vol0d,
                                             double voldp, T &st_pla,
                                             T &st_qvis, T &st_offg, T &st_epsd, T dt1,
                                             const Law2 &mat, const SProp &prop,
-                                            BrickOut<T> &out) {
+                                            BrickOut<T> &out, Emit emit) {
   const T ZERO = T(0), ONE = T(1), TWO = T(2), THREE = T(3), FOUR = T(4), HALF = T(0.5);
   T offg = st_offg;
   int mstop = 0;
@@ -407,68 +435,80 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], T vx[8], 
   T FCL = CAQ * rho * M<T>::two_third_pow(voln);
   T FCQ = FCL * CAQ * T(100);
   FCL = FCL * SSP;
-  T Gm[3][8];
-  Gm[0][0] =  ONE - PH[0][0]; Gm[0][1] = -ONE - PH[0][1]; Gm[0][2] =  ONE - PH[0][2]; Gm[0][3] = -ONE - PH[0][3];
-  Gm[0][4] =  ONE + PH[0][2]; Gm[0][5] = -ONE + PH[0][3]; Gm[0][6] =  ONE + PH[0][0]; Gm[0][7] = -ONE + PH[0][1];
-  Gm[1][0] =  ONE - PH[1][0]; Gm[1][1] =  ONE - PH[1][1]; Gm[1][2] = -ONE - PH[1][2]; Gm[1][3] = -ONE - PH[1][3];
-  Gm[1][4] = -ONE + PH[1][2]; Gm[1][5] = -ONE + PH[1][3]; Gm[1][6] =  ONE + PH[1][0]; Gm[1][7] =  ONE + PH[1][1];
-  Gm[2][0] =  ONE - PH[2][0]; Gm[2][1] = -ONE - PH[2][1]; Gm[2][2] = -ONE - PH[2][2]; Gm[2][3] =  ONE - PH[2][3];
-  Gm[2][4] = -ONE + PH[2][2]; Gm[2][5] =  ONE + PH[2][3]; Gm[2][6] =  ONE + PH[2][0]; Gm[2][7] = -ONE + PH[2][1];
-  T HG[3][4], HH[3][4];
+  // Hourglass rates HG and viscous hourglass "forces" HH per direction and mode;
+  // the corner forces are formed corner by corner below.
+  T HG[3][4], HH[3][4], Q[3][4];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const T *w = (d == 0) ? vx : (d == 1) ? vy : vz;
+    const VA w = (d == 0) ? vx : (d == 1) ? vy : vz;
+    if (sizeof(T) == sizeof(double)) {
+      // FP64 form: the reference's G = Gamma -/+ PxHy vectors and operation order.
 #pragma unroll
-    for (int m = 0; m < 3; ++m)
-      HG[d][m] = Gm[m][0] * w[0] + Gm[m][1] * w[1] + Gm[m][2] * w[2] + Gm[m][3] * w[3] +
-                 Gm[m][4] * w[4] + Gm[m][5] * w[5] + Gm[m][6] * w[6] + Gm[m][7] * w[7];
+      for (int m = 0; m < 3; ++m)
+        HG[d][m] = G_of<T>(m, 0, PH) * w[0] + G_of<T>(m, 1, PH) * w[1] + G_of<T>(m, 2, PH) * w[2] +
+                   G_of<T>(m, 3, PH) * w[3] + G_of<T>(m, 4, PH) * w[4] + G_of<T>(m, 5, PH) * w[5] +
+                   G_of<T>(m, 6, PH) * w[6] + G_of<T>(m, 7, PH) * w[7];
+    } else {
+      // FP32 form, factored: G_m = Gamma_m + c_k PH[m][a(k)], a = (0,1,2,3,2,3,0,1),
+      // c = (-,-,-,-,+,+,+,+), so Gamma_m.v is adds and PH acts on v1-v7, v2-v8, v3-v5, v4-v6.
+      const T p0 = w[0] - w[6], p1 = w[1] - w[7], p2 = w[2] - w[4], p3 = w[3] - w[5];
+      const T g[3] = {w[0] - w[1] + w[2] - w[3] + w[4] - w[5] + w[6] - w[7],
+                      w[0] + w[1] - w[2] - w[3] - w[4] - w[5] + w[6] + w[7],
+                      w[0] - w[1] - w[2] + w[3] - w[4] + w[5] + w[6] - w[7]};
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        HG[d][m] = g[m] - PH[m][0] * p0 - PH[m][1] * p1 - PH[m][2] * p2 - PH[m][3] * p3;
+    }
     HG[d][3] = w[0] - w[1] + w[2] - w[3] - w[4] + w[5] - w[6] + w[7];
 #pragma unroll
     for (int m = 0; m < 4; ++m) HH[d][m] = HG[d][m] * (FCL + ((HG[d][m] < ZERO) ? -HG[d][m] : HG[d][m]) * FCQ);
-  }
-  T F[3][8];
-  const int s4[8] = {-1, 1, -1, 1, 1, -1, 1, -1};
+    if (sizeof(T) != sizeof(double)) {
 #pragma unroll
-  for (int d = 0; d < 3; ++d)
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      T h4 = (s4[k] > 0) ? HH[d][3] : -HH[d][3];
-      F[d][k] = -Gm[0][k] * HH[d][0] - Gm[1][k] * HH[d][1] - Gm[2][k] * HH[d][2] + h4;
+      for (int a = 0; a < 4; ++a) Q[d][a] = PH[0][a] * HH[d][0] + PH[1][a] * HH[d][1] + PH[2][a] * HH[d][2];
     }
+  }
   out.ehou = dt1 * (HH[2][0] * HG[2][0] + HH[2][1] * HG[2][1] + HH[2][2] * HG[2][2] + HH[2][3] * HG[2][3] +
                     HH[0][0] * HG[0][0] + HH[0][1] * HG[0][1] + HH[0][2] * HG[0][2] + HH[0][3] * HG[0][3] +
                     HH[1][0] * HG[1][0] + HH[1][1] * HG[1][1] + HH[1][2] * HG[1][2] + HH[1][3] * HG[1][3]);
 
-  // ---- SFINT3 (sfint3.F:257-322), no viscous stress for law 2 ----
+  // SFINT3 (sfint3.F:257-322), no viscous stress for law 2: the pair term FINT(d,a)
+  // is subtracted at corner a and added at its opposite corner (1-7, 2-8, 3-5, 4-6).
+  T FI[3][4];
   {
-    T S1 = (sig[0] - qnew) * voln;
-    T S2 = (sig[1] - qnew) * voln;
-    T S3 = (sig[2] - qnew) * voln;
-    T S4 = sig[3] * voln;
-    T S5 = sig[4] * voln;
-    T S6 = sig[5] * voln;
-    const int pa[4] = {0, 1, 2, 3}, pb[4] = {6, 7, 4, 5};
+    const T S1 = (sig[0] - qnew) * voln, S2 = (sig[1] - qnew) * voln, S3 = (sig[2] - qnew) * voln;
+    const T S4 = sig[3] * voln, S5 = sig[4] * voln, S6 = sig[5] * voln;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      T FINT;
-      FINT = S1 * PX[a] + S4 * PY[a] + S6 * PZ[a];
-      F[0][pa[a]] = F[0][pa[a]] - FINT; F[0][pb[a]] = F[0][pb[a]] + FINT;
-      FINT = S2 * PY[a] + S4 * PX[a] + S5 * PZ[a];
-      F[1][pa[a]] = F[1][pa[a]] - FINT; F[1][pb[a]] = F[1][pb[a]] + FINT;
-      FINT = S3 * PZ[a] + S6 * PX[a] + S5 * PY[a];
-      F[2][pa[a]] = F[2][pa[a]] - FINT; F[2][pb[a]] = F[2][pb[a]] + FINT;
+      FI[0][a] = S1 * PX[a] + S4 * PY[a] + S6 * PZ[a];
+      FI[1][a] = S2 * PY[a] + S4 * PX[a] + S5 * PZ[a];
+      FI[2][a] = S3 * PZ[a] + S6 * PX[a] + S5 * PY[a];
     }
   }
-  // ---- SRROTA3 back to the global frame (sforc3.F:1604-1614), SCUMU3P ----
+  // Corner by corner: SHVIS3 force, +/- SFINT3 term, SRROTA3 back to the global
+  // frame (sforc3.F:1604-1614), SCUMU3P's zeroing of deleted elements, emit.
   const bool zero_f = (offg < ZERO);
+  const int s4[8] = {-1, 1, -1, 1, 1, -1, 1, -1};
+  const int ak[8] = {0, 1, 2, 3, 2, 3, 0, 1};
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    T X = R[0][0] * F[0][k] + R[0][1] * F[1][k] + R[0][2] * F[2][k];
-    T Y = R[1][0] * F[0][k] + R[1][1] * F[1][k] + R[1][2] * F[2][k];
-    T Z = R[2][0] * F[0][k] + R[2][1] * F[1][k] + R[2][2] * F[2][k];
-    out.f[0][k] = zero_f ? ZERO : X;
-    out.f[1][k] = zero_f ? ZERO : Y;
-    out.f[2][k] = zero_f ? ZERO : Z;
+    T Fl[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T h0 = HH[d][0], h1 = HH[d][1], h2 = HH[d][2];
+      const T h4 = (s4[k] > 0) ? HH[d][3] : -HH[d][3];
+      T hg;
+      if (sizeof(T) == sizeof(double)) {
+        hg = -G_of<T>(0, k, PH) * h0 - G_of<T>(1, k, PH) * h1 - G_of<T>(2, k, PH) * h2 + h4;
+      } else {
+        const T gh = gamma_of(0, k) * h0 + gamma_of(1, k) * h1 + gamma_of(2, k) * h2;
+        hg = -gh + ((k < 4) ? Q[d][ak[k]] : -Q[d][ak[k]]) + h4;
+      }
+      Fl[d] = (k < 4) ? hg - FI[d][ak[k]] : hg + FI[d][ak[k]];
+    }
+    const T X = R[0][0] * Fl[0] + R[0][1] * Fl[1] + R[0][2] * Fl[2];
+    const T Y = R[1][0] * Fl[0] + R[1][1] * Fl[1] + R[1][2] * Fl[2];
+    const T Z = R[2][0] * Fl[0] + R[2][1] * Fl[1] + R[2][2] * Fl[2];
+    emit(k, zero_f ? ZERO : X, zero_f ? ZERO : Y, zero_f ? ZERO : Z);
   }
   out.sti = T(0.25) * STI;
   out.mstop = mstop;

@@ -8,16 +8,12 @@
 // (order independent, so deterministic).  Hourglass energy is summed per
 // workgroup in a fixed tree order into ehou_part[block].
 #pragma once
+#include <type_traits>
+
 #include "orad_brick_chain.hpp"
+#include "orad_brick_io.hpp"
 
 namespace orad {
-
-constexpr int BRICK_BLOCK = 256;
-
-// Element state is read once and written once per cycle: non-temporal accesses
-// keep it from displacing the corner-node records that neighbouring elements reuse.
-template <typename T> __device__ __forceinline__ T ldnt(const T *p) { return __builtin_nontemporal_load(p); }
-template <typename T> __device__ __forceinline__ void stnt(T *p, T v) { __builtin_nontemporal_store(v, p); }
 
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
@@ -48,70 +44,71 @@ __device__ __forceinline__ double jacobian_det(const double x[8], const double y
   return 0.015625 * (J1 * (J5 * J9 - J6 * J8) + J2 * (J6 * J7 - J4 * J9) + J3 * (J4 * J8 - J5 * J7));
 }
 
-// REL: coordinates/velocities relative to corner 1 (mixed form).
-template <typename T, bool REL, int MINW>
-__global__ __launch_bounds__(BRICK_BLOCK) __attribute__((amdgpu_waves_per_eu(MINW)))
+// REL: coordinates and velocities relative to corner 1 (mixed form).
+// MINW: minimum waves per SIMD asked of the register allocator; BLOCK: threads per group.
+template <typename T, bool REL, int MINW, int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW)))
 void brick_forces_kernel(
     BrickArgs<T> a, int e_begin, int e_end, const double *dt1p,
     unsigned long long *dt2t_out, double *ehou_part, int xcd_remap) {
-  __shared__ double s_dt[BRICK_BLOCK / 64];
-  __shared__ double s_eh[BRICK_BLOCK / 64];
+  __shared__ double s_dt[BLOCK / 64];
+  __shared__ double s_eh[BLOCK / 64];
   const int nb = gridDim.x;
   const int b = xcd_remap ? xcd_block(blockIdx.x, nb) : (int)blockIdx.x;
-  const int e = e_begin + b * BRICK_BLOCK + (int)threadIdx.x;
+  const int e = e_begin + b * BLOCK + (int)threadIdx.x;
   const T dt1 = T(*dt1p);
   double my_dt = 1.0e300, my_eh = 0.0;
   if (e < e_end) {
-    const int ne = a.numels;
+    const BrickIO<T> io(a, e);
     int nd[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) nd[k] = ldnt(&a.conn[k * ne + e]);
+    for (int k = 0; k < 8; ++k) nd[k] = io.conn(k);
     T xd[8], yd[8], zd[8], vx[8], vy[8], vz[8];
     double voldp = 0.0;
     if (REL) {
-      const double4 x0 = a.X4[nd[0]], v0 = a.V4[nd[0]];
+      // coordinates first: FP64 differences, the FP64 determinant, then FP32 copies
+      const D3 x0 = io.x(nd[0]);
       double rx[8], ry[8], rz[8];
       rx[0] = 0.0; ry[0] = 0.0; rz[0] = 0.0;
-      vx[0] = T(0); vy[0] = T(0); vz[0] = T(0);
 #pragma unroll
       for (int k = 1; k < 8; ++k) {
-        const double4 xk = a.X4[nd[k]], vk = a.V4[nd[k]];
+        const D3 xk = io.x(nd[k]);
         rx[k] = xk.x - x0.x; ry[k] = xk.y - x0.y; rz[k] = xk.z - x0.z;
-        vx[k] = T(vk.x - v0.x); vy[k] = T(vk.y - v0.y); vz[k] = T(vk.z - v0.z);
       }
       voldp = jacobian_det(rx, ry, rz);
 #pragma unroll
       for (int k = 0; k < 8; ++k) { xd[k] = T(rx[k]); yd[k] = T(ry[k]); zd[k] = T(rz[k]); }
+      const D3 v0 = io.v(nd[0]);
+      vx[0] = T(0); vy[0] = T(0); vz[0] = T(0);
+#pragma unroll
+      for (int k = 1; k < 8; ++k) {
+        const D3 vk = io.v(nd[k]);
+        vx[k] = T(vk.x - v0.x); vy[k] = T(vk.y - v0.y); vz[k] = T(vk.z - v0.z);
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const double4 xk = a.X4[nd[k]], vk = a.V4[nd[k]];
+        const D3 xk = io.x(nd[k]), vk = io.v(nd[k]);
         xd[k] = T(xk.x); yd[k] = T(xk.y); zd[k] = T(xk.z);
         vx[k] = T(vk.x); vy[k] = T(vk.y); vz[k] = T(vk.z);
       }
     }
     T sig[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) sig[c] = ldnt(&a.sig[c * ne + e]);
-    T eint = ldnt(&a.eint[e]), rho = ldnt(&a.rho[e]), pla = ldnt(&a.pla[e]);
-    T qvis = ldnt(&a.qvis[e]), offg = ldnt(&a.offg[e]), epsd = ldnt(&a.epsd[e]);
-    const double vol0 = ldnt(&a.vol0[e]);
+    for (int c = 0; c < 6; ++c) sig[c] = io.sig(c);
+    T eint = io.get(F_EINT), rho = io.get(F_RHO), pla = io.get(F_PLA);
+    T qvis = io.get(F_QVIS), offg = io.get(F_OFFG), epsd = io.get(F_EPSD);
+    const double vol0 = io.vol0();
     const T offg_in = offg;
     BrickOut<T> o;
-    brick_chain<T>(xd, yd, zd, vx, vy, vz, sig, eint, rho, vol0, voldp, pla, qvis, offg, epsd,
-                   dt1, a.mat, a.prop, o);
+    brick_chain<T, T *>(xd, yd, zd, vx, vy, vz, sig, eint, rho, vol0, voldp, pla, qvis, offg, epsd,
+                        dt1, a.mat, a.prop, o,
+                        [&](int k, T fx, T fy, T fz) { io.slot(k, fx, fy, fz); });
 #pragma unroll
-    for (int c = 0; c < 6; ++c) stnt(&a.sig[c * ne + e], sig[c]);
-    stnt(&a.eint[e], eint); stnt(&a.rho[e], rho); stnt(&a.pla[e], pla);
-    stnt(&a.qvis[e], qvis); stnt(&a.epsd[e], epsd); stnt(&a.sti[e], o.sti);
-    if (offg != offg_in) a.offg[e] = offg;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int slot = ldnt(&a.iads[k * ne + e]);
-      Slot3<T> f;
-      f.x = o.f[0][k]; f.y = o.f[1][k]; f.z = o.f[2][k];
-      a.fsky[slot] = f;
-    }
+    for (int c = 0; c < 6; ++c) io.set_sig(c, sig[c]);
+    io.put(F_EINT, eint); io.put(F_RHO, rho); io.put(F_PLA, pla);
+    io.put(F_QVIS, qvis); io.put(F_EPSD, epsd); io.set_sti(o.sti);
+    if (offg != offg_in) io.put(F_OFFG, offg);
     if (o.mstop) atomicOr(&a.ctl->mstop, 1);
     my_dt = double(o.dtx);
     my_eh = double(o.ehou);
@@ -124,7 +121,7 @@ void brick_forces_kernel(
   if (threadIdx.x == 0) {
     double d = s_dt[0], h = s_eh[0];
 #pragma unroll
-    for (int w = 1; w < BRICK_BLOCK / 64; ++w) { d = (s_dt[w] < d) ? s_dt[w] : d; h += s_eh[w]; }
+    for (int w = 1; w < BLOCK / 64; ++w) { d = (s_dt[w] < d) ? s_dt[w] : d; h += s_eh[w]; }
     if (d < 1.0e300) atomicMin(dt2t_out, (unsigned long long)__double_as_longlong(d));
     if (ehou_part) ehou_part[b] = h;
   }
