@@ -11,6 +11,8 @@ Material/property defaults follow the reference starter:
   /PROP/SOLID Isolid=1 (starter/source/properties/solid/hm_read_prop14.F:348-362):
     qa = 1.1, qb = 0.05, h = 0.1;  DTFAC = 0.9 for the element time step.
 """
+import math
+
 import numpy as np
 
 SEED_BASE = 20261203
@@ -110,3 +112,90 @@ def brick_block(nx, ny, nz, h=1.0, jitter=0.05, seed=SEED_BASE, v_top=-1.0e4,
     v[kk == nz, 2] = v_top
     return dict(x=x, v=v, ms=ms, icodt=icodt, ixs=ixs, uid=uid, vol0=vol0, mat=mat,
                 prop=prop, dims=(nx, ny, nz), seed=seed)
+
+
+# ============================ shells ============================
+
+def shell_mat_params(young=2.1e5, nu=0.3, rho0=7.85e-9, ca=250.0, cb=600.0, cn=0.4,
+                     epmx=None, sigmx=None, cc=0.0, eps0=0.0, mexp=1.0, fcut=0.0, icc=1,
+                     vp=2, ipla=0):
+    """Law 2 for shells with the derived PM values of the starter
+    (starter/source/materials/mat/mat002/hm_read_mat02_jc.F90:166-239): G, A11,
+    A12, SSP = sqrt(E/rho), the EP20 defaults, ISRATE and PM(9)."""
+    ep20 = 1.0e20
+    israte = 1
+    if cc == 0.0:
+        eps0 = 1.0
+        israte = 0
+    if vp == 1 and fcut == 0.0:
+        fcut = 10000.0
+    asrate = ep20 if fcut == 0.0 else 2.0 * math.pi * fcut
+    return dict(rho0=rho0, young=young, nu=nu,
+                shear=young / (2.0 * (1.0 + nu)),
+                a11=young / (1.0 - nu * nu), a12=nu * (young / (1.0 - nu * nu)),
+                ssp=math.sqrt(young / max(rho0, 1.0e-20)),
+                ca=ca, cb=cb, cn=cn, epmx=ep20 if epmx is None else epmx,
+                ymax=ep20 if sigmx is None else sigmx, cc=cc, epdr=eps0, mexp=mexp,
+                asrate_pm9=asrate, icc=icc, vp=vp, israte=israte, ipla=ipla)
+
+
+def shell_prop_params(npt=5, hm=0.01, hf=0.01, hr=0.01, dm=0.0, dtfac=0.9, dtmin=0.0):
+    """/PROP/SHELL TYPE1 with Ishell=1 defaults (hm_read_prop01.F:205-244): shear
+    factor 5/6 (ISHEAR=0), hourglass coefficients 0.01."""
+    return dict(hm=hm, hf=hf, hr=hr, dm=dm, fsh=5.0 / 6.0, dtfac=dtfac, dtmin=dtmin,
+                ish=0, npt=npt)
+
+
+def shell_masses(x, ixc, thk, rho0):
+    """Nodal mass and rotational inertia as CINMAS distributes them for an isotropic
+    shell (starter/source/elements/shell/coque/cinmas.F:1294-1296,1322):
+    EMS = rho*t*A/4 and XI = EMS*(A/9 + t^2/12) to each corner."""
+    p = x[ixc]
+    d13 = p[:, 2] - p[:, 0]
+    d24 = p[:, 3] - p[:, 1]
+    area = 0.5 * np.linalg.norm(np.cross(d13, d24), axis=1)
+    ems = rho0 * thk * area * 0.25
+    xi = ems * (area / 9.0 + thk * thk * (1.0 / 12.0))
+    nn = x.shape[0]
+    ms = np.zeros(nn)
+    inr = np.zeros(nn)
+    for k in range(4):
+        np.add.at(ms, ixc[:, k], ems)
+        np.add.at(inr, ixc[:, k], xi)
+    return ms, inr, area
+
+
+def shell_plate(n=512, h=1.0, thk=1.0, patch=32, v0=1.5e4, noise=0.1, thk_noise=0.0,
+                seed=20261203, npt=5, ipla=0):
+    """Square BT shell plate: n x n elements of size h in the XY plane, clamped edges
+    (all six DOF), the nodes of the central patch x patch elements starting at
+    Vz = v0 * (1 + U(-noise, noise)); element thickness thk * (1 + U(-thk_noise,
+    thk_noise)).  Counter-based (Philox) streams keyed on the seed."""
+    ni = n + 1
+    ii, jj = np.meshgrid(np.arange(ni), np.arange(ni), indexing="ij")
+    x = np.zeros((ni * ni, 3))
+    nid = (ii + ni * jj).ravel(order="F")
+    x[nid, 0] = (ii * h).ravel(order="F")
+    x[nid, 1] = (jj * h).ravel(order="F")
+    ei, ej = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    ei, ej = ei.ravel(order="F"), ej.ravel(order="F")
+    n1 = ei + ni * ej
+    ixc = np.stack([n1, n1 + 1, n1 + 1 + ni, n1 + ni], axis=1).astype(np.int32)
+    ne = n * n
+    rs = np.random.Generator(np.random.Philox(key=seed))
+    t = thk * (1.0 + rs.uniform(-thk_noise, thk_noise, ne)) if thk_noise > 0 else np.full(ne, thk)
+    v = np.zeros_like(x)
+    lo = (n - patch) // 2
+    inpatch = (ii >= lo) & (ii <= lo + patch) & (jj >= lo) & (jj <= lo + patch)
+    pn = (ii + ni * jj)[inpatch]
+    rv = np.random.Generator(np.random.Philox(key=seed + 1))
+    v[np.sort(pn), 2] = v0 * (1.0 + rv.uniform(-noise, noise, pn.size))
+    edge = (ii == 0) | (jj == 0) | (ii == n) | (jj == n)
+    icodt = np.zeros(ni * ni, np.int32)
+    icodt[(ii + ni * jj)[edge]] = 7
+    icodr = icodt.copy()
+    mat = shell_mat_params(ipla=ipla)
+    ms, inr, area = shell_masses(x, ixc, t, mat["rho0"])
+    return dict(x=x, v=v, vr=np.zeros_like(x), ms=ms, **{"in": inr}, icodt=icodt,
+                icodr=icodr, ixc=ixc, uid=np.arange(1, ne + 1, dtype=np.int32), thk=t,
+                mat=mat, prop=shell_prop_params(npt=npt))

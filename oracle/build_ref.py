@@ -1,6 +1,7 @@
-"""Recipe for oracle/_ref: the reference's own brick leaf kernels, compiled from
-their sources where they lie under the reference tree, plus the test harness
-oracle/ref/harness_brick.F, linked into oracle/_ref/libref_brick.so.
+"""Recipe for oracle/_ref: the reference's own brick and shell leaf kernels,
+compiled from their sources where they lie under the reference tree, plus the
+test harnesses oracle/ref/harness_brick.F and harness_shell.F, linked into
+oracle/_ref/libref.so.
 
 Test infrastructure only.  Runs in the build container (amdflang from ROCm);
 outputs go to oracle/_ref/ only.  Module dependencies are resolved by compiling
@@ -39,6 +40,22 @@ LEAF = [
     "engine/source/elements/solid/solide/shvis3.F",
     "engine/source/elements/solid/solide/sfint3.F",
     "engine/source/elements/solid/solide/scumu3p.F",
+    "engine/source/elements/shell/coque/ccoor3.F",
+    "engine/source/elements/shell/coque/cnvec3.F",
+    "engine/source/elements/shell/coque/cortdir3.F",
+    "engine/source/elements/shell/coque/cderi3.F",
+    "engine/source/elements/shell/coque/ccoef3.F",
+    "engine/source/elements/shell/coque/cdlen3.F",
+    "engine/source/elements/shell/coque/cdefo3.F",
+    "engine/source/elements/shell/coque/ccurv3.F",
+    "engine/source/elements/shell/coque/cstra3.F",
+    "engine/source/elements/shell/coque/chvis3.F",
+    "engine/source/elements/shell/coque/cdt3.F",
+    "engine/source/elements/shell/coque/cfint3.F",
+    "engine/source/elements/shell/coque/cupdt3.F",
+    "engine/source/elements/shell/coqini.F",
+    "engine/source/materials/mat/mat002/sigeps02c.F",
+    "engine/source/materials/mat/mat002/m2cplr.F",
 ]
 
 
@@ -101,7 +118,8 @@ def main():
     for rel in LEAF:
         comp(os.path.join(REF, rel))
     comp(os.path.join(HERE, "ref", "harness_brick.F"))
-    lib = os.path.join(OUT, "libref_brick.so")
+    comp(os.path.join(HERE, "ref", "harness_shell.F"))
+    lib = os.path.join(OUT, "libref.so")
     # Routines referenced on branches the path does not take (error messages,
     # SPH, orthotropic frames) are compiled from their own reference sources too,
     # until the library has no unresolved Fortran symbol.

@@ -103,6 +103,94 @@ void oracle_cycle(orad_omodel_t *m);
 /* Energies: internal sum(EINT*V0), kinetic 0.5*sum(m v^2). */
 void oracle_energies(const orad_omodel_t *m, double *eint, double *ekin);
 
+/* ======================= Belytschko-Tsay shells (CFORC3) ======================= */
+
+/* Law 2 for shells: the PM slots and MAT_PARAM values the starter derives
+ * (hm_read_mat02_jc.F90:234-339). */
+typedef struct {
+  double rho0, young, nu, shear;  /* PM(1), PM(20), PM(21), PM(22)            */
+  double a11, a12, ssp;           /* PM(24), PM(25), PM(27) = sqrt(E/rho)     */
+  double ca, cb, cn;              /* uparam(1..3)                              */
+  double epmx, ymax;              /* uparam(4,5)  EPS_max, SIG_max             */
+  double cc, epdr, mexp;          /* uparam(6,7,10)                            */
+  double asrate_pm9;              /* PM(9): 2*pi*Fcut, EP20 when Fcut = 0      */
+  int    icc, vp, israte, ipla;   /* iparam(2..4); IPARG(29) plasticity algo. */
+} orad_shmat_t;
+
+/* /PROP/SHELL (TYPE1) values used by CFORC3 (hm_read_prop01.F:155-244). */
+typedef struct {
+  double hm, hf, hr;              /* GEO(13..15) hourglass coefficients       */
+  double dm;                      /* membrane damping GROUP_PARAM%VISC_DM     */
+  double fsh;                     /* GEO(38) shear factor (5/6)               */
+  double dtfac, dtmin;            /* DTFAC1(3), DTMIN1(3)                     */
+  int    ish;                     /* GEO(37) ISHEAR                           */
+  int    npt;                     /* through-thickness points, 2..5           */
+} orad_shprop_t;
+
+#define ORAD_SH_NPT_MAX 5
+/* Element state (GBUF + LBUF of one BT shell, elbufdef_mod.F90). */
+typedef struct {
+  double forc[5], mom[3];         /* GBUF%FOR, GBUF%MOM (generalised stresses) */
+  double thk;                     /* GBUF%THK                                   */
+  double eint[2];                 /* GBUF%EINT membrane, bending (energy)       */
+  double epsd;                    /* GBUF%EPSD element strain rate              */
+  double hour[5];                 /* GBUF%HOURG                                 */
+  double offg;                    /* GBUF%OFF                                   */
+  double sig[ORAD_SH_NPT_MAX][5]; /* LBUF%SIG per point                         */
+  double pla[ORAD_SH_NPT_MAX];    /* LBUF%PLA                                   */
+  double epsdp[ORAD_SH_NPT_MAX];  /* LBUF%EPSD                                  */
+} orad_shstate_t;
+
+typedef struct {
+  double f[3][4], m[3][4];        /* FSKY(1:6,corner) = -F, -M (CUPDT3P)        */
+  double sti, stir;               /* FSKY(7), FSKY(8)                           */
+  double dt;                      /* element time step (CDT3)                   */
+  double ehou;                    /* hourglass energy increment                 */
+  int    alive;                   /* dt enters DT2T (OFFG > 0 and OFF != 0)     */
+  int    pad;
+} orad_shout_t;
+
+/* CFORC3 chain for one element (engine/source/elements/shell/coque/cforc3.F:398-746).
+ * tri: nodes 3 and 4 coincide (IXC(4)==IXC(5)). */
+void oracle_shell_element(const double xg[3][4], const double vg[3][4], const double vrg[3][4],
+                          double thke, int tri, orad_shstate_t *st, double dt1,
+                          const orad_shmat_t *mat, const orad_shprop_t *prop,
+                          orad_shout_t *out);
+
+/* COQINI tables for npt points (single-precision DATA values widened). */
+void oracle_shell_tables(int npt, double *z0, double *wf, double *wm);
+/* Derived constants used on the shell path, in a fixed order (for pinning). */
+void oracle_shell_constants(double *out16);
+
+/* PARITH/ON CSR for nc corners per element (FILLCNE + W_PON). */
+int oracle_build_pon_nc(int numnod, int numel, int nc, const int32_t *conn,
+                        const int32_t *uid, int32_t *adsky, int32_t *iads);
+
+/* Shell model: 6 DOF nodes, FSKY slots of 8 values (F, M, STI, STIR). */
+typedef struct {
+  int numnod, numelc;
+  double *x, *v, *vr, *d;         /* 3*numnod each                               */
+  const double *ms, *inr;         /* nodal mass, rotational inertia             */
+  const int32_t *icodt, *icodr;   /* BCS codes (x=4,y=2,z=1) on V and VR         */
+  const int32_t *fv_dir;          /* FIXVEL direction per node (1..3), 0 = none  */
+  const double *fv_val;           /* FIXVEL imposed velocity per node            */
+  const int32_t *ixc;             /* 4*numelc, 0-based                           */
+  const double *thke;             /* element thickness                           */
+  const int32_t *adsky, *iads;    /* PARITH/ON CSR                               */
+  orad_shstate_t *st;
+  double *fsky;                   /* 8 per slot                                  */
+  double *a, *ar, *stifn, *stifr;
+  orad_shmat_t mat;
+  orad_shprop_t prop;
+  double tt, dt1, dt2, dt12, dt2old, dtmx;
+  double ehour;
+  int    ncycle, nthreads;
+} orad_oshell_t;
+
+void oracle_shell_cycle(orad_oshell_t *m);
+/* Energies: internal sum(EINT(1)+EINT(2)) (cbilan.F:182), kinetic with rotations. */
+void oracle_shell_energies(const orad_oshell_t *m, double *eint, double *ekin);
+
 #ifdef __cplusplus
 }
 #endif

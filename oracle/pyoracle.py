@@ -1,5 +1,5 @@
 """ctypes bindings for the oracle (liborad_oracle.so) and the reference-kernel
-library (oracle/_ref/libref_brick.so).
+library (oracle/_ref/libref.so).
 
 Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, never by openradioss_amd/.
@@ -11,7 +11,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborad_oracle.so")
-REF_PATH = os.path.join(HERE, "_ref", "libref_brick.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref.so")
 
 # Column layout of one element's state row (orad_bstate_t).
 ST_SIG = slice(0, 6)
@@ -49,6 +49,49 @@ class OModel(C.Structure):
                 ("nthreads", C.c_int), ("pad", C.c_int)]
 
 
+# ---- shells: one element's state row (orad_shstate_t), npt <= 5 ----
+SH_FOR = slice(0, 5)
+SH_MOM = slice(5, 8)
+SH_THK = 8
+SH_EINT = slice(9, 11)
+SH_EPSD = 11
+SH_HOUR = slice(12, 17)
+SH_OFFG = 17
+SH_SIG0 = 18          # sig[pt][c] at 18 + 5*pt + c
+SH_PLA0 = 43          # pla[pt]
+SH_EPSDP0 = 48        # epsdp[pt]
+SH_NST = 53
+
+
+class ShMat(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("rho0", "young", "nu", "shear", "a11", "a12", "ssp", "ca", "cb", "cn",
+                 "epmx", "ymax", "cc", "epdr", "mexp", "asrate_pm9")] + \
+               [(n, C.c_int) for n in ("icc", "vp", "israte", "ipla")]
+
+
+class ShProp(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("hm", "hf", "hr", "dm", "fsh", "dtfac", "dtmin")] + \
+               [(n, C.c_int) for n in ("ish", "npt")]
+
+
+class ShOut(C.Structure):
+    _fields_ = [("f", C.c_double * 12), ("m", C.c_double * 12), ("sti", C.c_double),
+                ("stir", C.c_double), ("dt", C.c_double), ("ehou", C.c_double),
+                ("alive", C.c_int), ("pad", C.c_int)]
+
+
+class OShell(C.Structure):
+    _fields_ = [("numnod", C.c_int), ("numelc", C.c_int)] + \
+               [(n, C.c_void_p) for n in ("x", "v", "vr", "d", "ms", "inr", "icodt", "icodr",
+                                          "fv_dir", "fv_val", "ixc", "thke", "adsky", "iads",
+                                          "st", "fsky", "a", "ar", "stifn", "stifr")] + \
+               [("mat", ShMat), ("prop", ShProp)] + \
+               [(n, C.c_double) for n in ("tt", "dt1", "dt2", "dt12", "dt2old", "dtmx",
+                                          "ehour")] + \
+               [("ncycle", C.c_int), ("nthreads", C.c_int)]
+
+
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
@@ -72,6 +115,17 @@ def lib():
         _lib.oracle_cycle.argtypes = [C.POINTER(OModel)]
         _lib.oracle_energies.argtypes = [C.POINTER(OModel), C.POINTER(C.c_double),
                                          C.POINTER(C.c_double)]
+        _lib.oracle_shell_element.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                              C.c_int, C.c_void_p, C.c_double,
+                                              C.POINTER(ShMat), C.POINTER(ShProp),
+                                              C.POINTER(ShOut)]
+        _lib.oracle_shell_tables.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.oracle_shell_constants.argtypes = [C.c_void_p]
+        _lib.oracle_build_pon_nc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.oracle_shell_cycle.argtypes = [C.POINTER(OShell)]
+        _lib.oracle_shell_energies.argtypes = [C.POINTER(OShell), C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double)]
     return _lib
 
 
@@ -87,6 +141,10 @@ def ref():
             [C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
              C.c_void_p, C.c_void_p, C.c_void_p]
         _ref.ref_constants.argtypes = [C.c_void_p]
+        _ref.ref_shell_group.argtypes = [C.c_int] + [C.c_void_p] * 15 + \
+            [C.c_double] + [C.c_void_p] * 6
+        _ref.ref_shell_constants.argtypes = [C.c_void_p]
+        _ref.ref_shell_tables.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     return _ref
 
 
@@ -210,6 +268,186 @@ class Model:
         """(internal, kinetic, cumulative hourglass) energy."""
         ei, ek = C.c_double(), C.c_double()
         lib().oracle_energies(C.byref(self.m), C.byref(ei), C.byref(ek))
+        return ei.value, ek.value, self.m.ehour
+
+    @property
+    def dt2(self):
+        return self.m.dt2
+
+    @property
+    def tt(self):
+        return self.m.tt
+
+
+# ============================== shells ==============================
+
+def shmat(p):
+    """ShMat from a shell material dict (openradioss_amd.deck.shell_mat_params)."""
+    return ShMat(*[p[k] for k in ("rho0", "young", "nu", "shear", "a11", "a12", "ssp", "ca",
+                                  "cb", "cn", "epmx", "ymax", "cc", "epdr", "mexp",
+                                  "asrate_pm9")],
+                 p["icc"], p["vp"], p["israte"], p["ipla"])
+
+
+def shprop(p):
+    return ShProp(*[p[k] for k in ("hm", "hf", "hr", "dm", "fsh", "dtfac", "dtmin")],
+                  p["ish"], p["npt"])
+
+
+def shell_state(ne, thk, offg=1.0):
+    st = np.zeros((ne, SH_NST))
+    st[:, SH_THK] = thk
+    st[:, SH_OFFG] = offg
+    return st
+
+
+def shell_element(xg, vg, vrg, thke, state, dt1, mat, prop, tri=False):
+    """One shell through the oracle chain. xg, vg, vrg: (3,4); state (53,) in place."""
+    xg, vg, vrg = (np.ascontiguousarray(a, dtype=np.float64) for a in (xg, vg, vrg))
+    assert xg.shape == (3, 4) and state.shape == (SH_NST,) and state.flags.c_contiguous
+    out = ShOut()
+    lib().oracle_shell_element(_ptr(xg), _ptr(vg), _ptr(vrg), float(thke), int(tri),
+                               _ptr(state), dt1, C.byref(shmat(mat)), C.byref(shprop(prop)),
+                               C.byref(out))
+    return dict(f=np.array(out.f).reshape(3, 4), m=np.array(out.m).reshape(3, 4),
+                sti=out.sti, stir=out.stir, dt=out.dt, ehou=out.ehou, alive=out.alive)
+
+
+def shell_tables(npt):
+    z, w, m = np.zeros(npt), np.zeros(npt), np.zeros(npt)
+    lib().oracle_shell_tables(npt, _ptr(z), _ptr(w), _ptr(m))
+    return z, w, m
+
+
+def shell_constants():
+    c = np.zeros(16)
+    lib().oracle_shell_constants(_ptr(c))
+    return c
+
+
+def ref_shell_constants():
+    c = np.zeros(16)
+    ref().ref_shell_constants(_ptr(c))
+    return c
+
+
+def ref_shell_tables(npt):
+    z, w, m = np.zeros(npt), np.zeros(npt), np.zeros(npt)
+    ref().ref_shell_tables(npt, _ptr(z), _ptr(w), _ptr(m))
+    return z, w, m
+
+
+def build_pon_nc(numnod, conn, uid):
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    uid = np.ascontiguousarray(uid, dtype=np.int32)
+    ne, nc = conn.shape
+    adsky = np.zeros(numnod + 1, np.int32)
+    iads = np.zeros((ne, nc), np.int32)
+    rc = lib().oracle_build_pon_nc(numnod, ne, nc, _ptr(conn), _ptr(uid), _ptr(adsky),
+                                   _ptr(iads))
+    if rc != 0:
+        raise ValueError(f"oracle_build_pon_nc failed ({rc})")
+    return adsky, iads
+
+
+def ref_shell_group(xg, vg, vrg, thke, state, dt1, mat, prop, tri=None):
+    """Reference routines on a group: xg, vg, vrg (nel,3,4); state (nel,53) updated in
+    place.  Returns fsky (nel,4,8), per-element dt, ehou."""
+    nel = xg.shape[0]
+    assert nel <= 128
+    npt = prop["npt"]
+    fx = lambda a: np.ascontiguousarray(np.transpose(a, (0, 1, 2)), dtype=np.float64)
+    xg, vg, vrg = fx(xg), fx(vg), fx(vrg)          # Fortran XG(4,3,NEL) == C [nel][3][4]
+    thke = np.ascontiguousarray(thke, dtype=np.float64)
+    itri = np.zeros(nel, np.int32) if tri is None else np.ascontiguousarray(tri, np.int32)
+    forc = np.asfortranarray(state[:, SH_FOR])
+    mom = np.asfortranarray(state[:, SH_MOM])
+    thk = np.ascontiguousarray(state[:, SH_THK])
+    eint = np.asfortranarray(state[:, SH_EINT])
+    epsd = np.ascontiguousarray(state[:, SH_EPSD])
+    hour = np.asfortranarray(state[:, SH_HOUR])
+    offg = np.ascontiguousarray(state[:, SH_OFFG])
+    sig = np.zeros((nel, 5, npt), order="F")
+    pla = np.zeros((nel, npt), order="F")
+    epsdp = np.zeros((nel, npt), order="F")
+    for p_ in range(npt):
+        sig[:, :, p_] = state[:, SH_SIG0 + 5 * p_: SH_SIG0 + 5 * p_ + 5]
+        pla[:, p_] = state[:, SH_PLA0 + p_]
+        epsdp[:, p_] = state[:, SH_EPSDP0 + p_]
+    mpar = np.array([mat[k] for k in ("rho0", "young", "nu", "shear", "a11", "a12", "ssp",
+                                      "ca", "cb", "cn", "epmx", "ymax", "cc", "epdr", "mexp",
+                                      "asrate_pm9")], np.float64)
+    ipar = np.array([mat["icc"], mat["vp"], mat["israte"], mat["ipla"], npt, prop["ish"]],
+                    np.int32)
+    ppar = np.array([prop[k] for k in ("hm", "hf", "hr", "dm", "fsh", "dtfac", "dtmin")],
+                    np.float64)
+    fsky = np.zeros((4 * nel, 8))
+    dte = np.zeros(nel)
+    ehou = np.zeros(nel)
+    ref().ref_shell_group(nel, _ptr(xg), _ptr(vg), _ptr(vrg), _ptr(thke), _ptr(itri),
+                          _ptr(forc), _ptr(mom), _ptr(thk), _ptr(eint), _ptr(epsd), _ptr(hour),
+                          _ptr(offg), _ptr(sig), _ptr(pla), _ptr(epsdp), dt1, _ptr(mpar),
+                          _ptr(ipar), _ptr(ppar), _ptr(fsky), _ptr(dte), _ptr(ehou))
+    state[:, SH_FOR] = forc
+    state[:, SH_MOM] = mom
+    state[:, SH_THK] = thk
+    state[:, SH_EINT] = eint
+    state[:, SH_EPSD] = epsd
+    state[:, SH_HOUR] = hour
+    state[:, SH_OFFG] = offg
+    for p_ in range(npt):
+        state[:, SH_SIG0 + 5 * p_: SH_SIG0 + 5 * p_ + 5] = sig[:, :, p_]
+        state[:, SH_PLA0 + p_] = pla[:, p_]
+        state[:, SH_EPSDP0 + p_] = epsdp[:, p_]
+    return dict(fsky=fsky.reshape(nel, 4, 8), dt=dte, ehou=ehou)
+
+
+class ShellModel:
+    """Whole-model shell oracle (FP64, RESOL order, 6 DOF)."""
+
+    def __init__(self, deck, nthreads=0):
+        self.deck = deck
+        nn = deck["x"].shape[0]
+        self.x = np.ascontiguousarray(deck["x"], dtype=np.float64).copy()
+        self.v = np.ascontiguousarray(deck["v"], dtype=np.float64).copy()
+        self.vr = np.ascontiguousarray(deck.get("vr", np.zeros((nn, 3))), np.float64).copy()
+        self.d = np.zeros_like(self.x)
+        self.ms = np.ascontiguousarray(deck["ms"], dtype=np.float64)
+        self.inr = np.ascontiguousarray(deck["in"], dtype=np.float64)
+        self.icodt = np.ascontiguousarray(deck["icodt"], dtype=np.int32)
+        self.icodr = np.ascontiguousarray(deck["icodr"], dtype=np.int32)
+        self.fv_dir = np.ascontiguousarray(deck.get("fv_dir", np.zeros(nn, np.int32)), np.int32)
+        self.fv_val = np.ascontiguousarray(deck.get("fv_val", np.zeros(nn)), np.float64)
+        self.ixc = np.ascontiguousarray(deck["ixc"], dtype=np.int32)
+        self.thke = np.ascontiguousarray(deck["thk"], dtype=np.float64)
+        self.adsky, self.iads = build_pon_nc(nn, self.ixc, deck["uid"])
+        ne = self.ixc.shape[0]
+        self.st = shell_state(ne, self.thke)
+        self.fsky = np.zeros((4 * ne, 8))
+        self.a = np.zeros_like(self.x)
+        self.ar = np.zeros_like(self.x)
+        self.stifn = np.zeros(nn)
+        self.stifr = np.zeros(nn)
+        m = OShell()
+        m.numnod, m.numelc = nn, ne
+        for name in ("x", "v", "vr", "d", "ms", "inr", "icodt", "icodr", "fv_dir", "fv_val",
+                     "ixc", "thke", "adsky", "iads", "st", "fsky", "a", "ar", "stifn",
+                     "stifr"):
+            setattr(m, name, _ptr(getattr(self, name)))
+        m.mat = shmat(deck["mat"])
+        m.prop = shprop(deck["prop"])
+        m.dt2old = 1.0e6 / 1.1
+        m.dtmx = 1.0e20
+        m.nthreads = nthreads
+        self.m = m
+
+    def cycle(self, n=1):
+        for _ in range(n):
+            lib().oracle_shell_cycle(C.byref(self.m))
+
+    def energies(self):
+        ei, ek = C.c_double(), C.c_double()
+        lib().oracle_shell_energies(C.byref(self.m), C.byref(ei), C.byref(ek))
         return ei.value, ek.value, self.m.ehour
 
     @property

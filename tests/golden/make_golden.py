@@ -55,7 +55,7 @@ PKEYS = ("qa", "qb", "hvis", "dtfac")
 
 def main():
     if not po.ref_available():
-        raise SystemExit("oracle/_ref/libref_brick.so not built (python oracle/build_ref.py)")
+        raise SystemExit("oracle/_ref/libref.so not built (python oracle/build_ref.py)")
     rng = np.random.Generator(np.random.Philox(dk.SEED_BASE + 100))
     steel = dk.law2_params()
     prop = dk.solid_params()

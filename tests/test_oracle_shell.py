@@ -1,0 +1,184 @@
+"""The shell oracle against the reference's own shell routines, plus closed forms.
+
+tests/golden/shell_groups.npz holds inputs and outputs of the reference routines
+CCOOR3, CNVEC3, CDERI3, CCOEF3, CDLEN3, CDEFO3, CCURV3, CSTRA3, SIGEPS02C/M2CPLR,
+CHVIS3, CDT3, CFINT3 and CUPDT3P (compiled from source, driven in CFORC3 order by
+oracle/ref/harness_shell.F; written by tests/golden/make_golden_shell.py).  The
+oracle restates them in C and must reproduce them bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from openradioss_amd import deck as dk
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+MKEYS = ("rho0", "young", "nu", "shear", "a11", "a12", "ssp", "ca", "cb", "cn", "epmx",
+         "ymax", "cc", "epdr", "mexp", "asrate_pm9")
+IKEYS = ("icc", "vp", "israte", "ipla")
+PKEYS = ("hm", "hf", "hr", "dm", "fsh", "dtfac", "dtmin")
+PIKEYS = ("ish", "npt")
+
+
+def load_cases():
+    z = np.load(os.path.join(GOLD, "shell_groups.npz"))
+    names = sorted({k.split("/")[0] for k in z.files})
+    out = {}
+    for n in names:
+        c = {k.split("/")[1]: z[k] for k in z.files if k.startswith(n + "/")}
+        c["mat"] = dict(zip(MKEYS, c["mat"].tolist()))
+        c["mat"].update({k: int(v) for k, v in zip(IKEYS, c["imat"].tolist())})
+        c["prop"] = dict(zip(PKEYS, c["prop"].tolist()))
+        c["prop"].update({k: int(v) for k, v in zip(PIKEYS, c["iprop"].tolist())})
+        out[n] = c
+    return out
+
+
+CASES = load_cases()
+
+
+def run_oracle(c):
+    st = c["state"].copy()
+    nel = st.shape[0]
+    fsky = np.zeros((nel, 4, 8))
+    dt = np.zeros(nel)
+    eh = np.zeros(nel)
+    for e in range(nel):
+        row = st[e].copy()
+        o = po.shell_element(c["xg"][e], c["vg"][e], c["vrg"][e], c["thk"][e], row,
+                             float(c["dt1"]), c["mat"], c["prop"], tri=bool(c["tri"][e]))
+        st[e] = row
+        fsky[e, :, 0:3] = o["f"].T
+        fsky[e, :, 3:6] = o["m"].T
+        fsky[e, :, 6] = o["sti"]
+        fsky[e, :, 7] = o["stir"]
+        dt[e] = o["dt"]
+        eh[e] = o["ehou"]
+    return fsky, st, dt, eh
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_shell_oracle_bitwise_vs_reference_routines(name):
+    c = CASES[name]
+    fsky, st, dt, eh = run_oracle(c)
+    assert np.array_equal(fsky, c["fsky"]), np.abs(fsky - c["fsky"]).max()
+    assert np.array_equal(st, c["state_out"]), np.abs(st - c["state_out"]).max()
+    assert np.array_equal(dt, c["dt"])
+    assert np.array_equal(eh, c["ehou"])
+
+
+def test_golden_cases_exercise_plasticity_and_deletion():
+    assert CASES["plastic_ipla0"]["state_out"][:, po.SH_PLA0:po.SH_PLA0 + 5].max() > 1e-3
+    assert CASES["plastic_ipla1"]["state_out"][:, po.SH_PLA0:po.SH_PLA0 + 5].max() > 1e-3
+    d = CASES["deleted"]
+    dead = d["state"][:, po.SH_OFFG] < 0
+    assert dead.any() and np.all(d["fsky"][dead] == 0.0)
+
+
+def test_shell_constants_and_tables_match_reference():
+    assert np.array_equal(po.shell_constants(), np.load(os.path.join(GOLD, "shell_constants.npy")))
+    t = np.load(os.path.join(GOLD, "shell_tables.npz"))
+    for npt in range(1, 6):
+        z, w, m = po.shell_tables(npt)
+        assert np.array_equal(z, t[f"z{npt}"]) and np.array_equal(w, t[f"w{npt}"])
+        assert np.array_equal(m, t[f"m{npt}"])
+
+
+def test_force_weights_integrate_unity_and_moment_weights_odd():
+    for npt in range(2, 6):
+        z, w, m = po.shell_tables(npt)
+        assert abs(w.sum() - 1.0) < 1e-6          # FOR = sum w*sig integrates a constant
+        assert np.allclose(m, -m[::-1])            # antisymmetric through the thickness
+        assert np.allclose(z, -z[::-1])
+
+
+SQ = np.array([[0, 0, 0], [2, 0, 0], [2, 2, 0], [0, 2, 0]], float).T
+
+
+def _elem(v, vr, state=None, dt1=1e-7, mat=None, prop=None, xg=SQ, thk=1.0):
+    mat = mat or dk.shell_mat_params()
+    prop = prop or dk.shell_prop_params()
+    st = po.shell_state(1, thk)[0] if state is None else state
+    o = po.shell_element(xg, v, vr, thk, st, dt1, mat, prop)
+    return o, st
+
+
+def test_rigid_translation_gives_no_force_or_stress():
+    v = np.tile(np.array([[3e3], [-2e3], [5e3]]), (1, 4))
+    o, st = _elem(v, np.zeros((3, 4)))
+    assert np.all(o["f"] == 0.0) and np.all(o["m"] == 0.0)
+    assert np.all(st[po.SH_SIG0:po.SH_SIG0 + 25] == 0.0)
+
+
+def test_rigid_spin_gives_no_force():
+    w = 4e3                                       # spin about z through the centre
+    xc = SQ - SQ.mean(axis=1, keepdims=True)
+    v = np.stack([-w * xc[1], w * xc[0], 0 * xc[0]])
+    vr = np.tile(np.array([[0.0], [0.0], [w]]), (1, 4))
+    o, st = _elem(v, vr)
+    scale = 2.1e5 * 1e-7 * w
+    assert np.abs(o["f"]).max() < 1e-9 * scale and np.abs(o["m"]).max() < 1e-9 * scale
+
+
+def test_uniform_stretch_gives_plane_stress_resultant():
+    rate = 50.0                                   # 1/s, elastic
+    v = np.stack([rate * SQ[0], 0 * SQ[0], 0 * SQ[0]])
+    dt1 = 1e-7
+    mat = dk.shell_mat_params()
+    o, st = _elem(v, np.zeros((3, 4)), dt1=dt1, mat=mat)
+    de = rate * dt1
+    forc = st[po.SH_FOR]
+    assert forc[0] == pytest.approx(mat["a11"] * de, rel=1e-12)
+    assert forc[1] == pytest.approx(mat["a12"] * de, rel=1e-12)
+    assert abs(forc[2]) < 1e-12 * mat["a11"] * de
+    # nodal x-forces: -F on the x = 2 edge balances +F on x = 0, total area*thk*sigma
+    fx = o["f"][0]
+    assert fx.sum() == pytest.approx(0.0, abs=1e-9 * abs(fx).max())
+    assert fx[1] + fx[2] == pytest.approx(-mat["a11"] * de * 2.0 * 1.0, rel=1e-12)
+
+
+def test_pure_bending_moment_matches_through_thickness_sum():
+    kdot = 0.05                                   # 1/(mm s): rotation rate about y growing with x
+    vr = np.stack([0 * SQ[0], kdot * SQ[0], 0 * SQ[0]])
+    dt1 = 1e-7
+    mat = dk.shell_mat_params()
+    prop = dk.shell_prop_params(npt=5)
+    o, st = _elem(np.zeros((3, 4)), vr, dt1=dt1, mat=mat, prop=prop)
+    kxx = kdot * dt1
+    z, w, m = po.shell_tables(5)
+    expect = sum(mi * mat["a11"] * zi * 1.0 * kxx for zi, mi in zip(z, m))
+    assert st[po.SH_MOM][0] == pytest.approx(expect, rel=1e-12)
+    assert st[po.SH_MOM][0] > 0
+
+
+def test_plate_vibration_energy_balance():
+    # First bending mode of a clamped 16 x 16 plate, elastic: internal + kinetic +
+    # hourglass energy stays within 3% of the initial kinetic energy over a period
+    # (an impulsive start would excite the highest modes, whose half-step kinetic
+    # energy the leapfrog scheme does not resolve).
+    d = dk.shell_plate(n=16, patch=4, v0=0.0)
+    L = 16.0
+    d["v"][:, 2] = 200.0 * np.sin(np.pi * d["x"][:, 0] / L) * np.sin(np.pi * d["x"][:, 1] / L)
+    m = po.ShellModel(d, nthreads=2)
+    ek0 = 0.5 * np.sum(d["ms"] * np.sum(d["v"] ** 2, axis=1))
+    eimax = 0.0
+    for _ in range(8):
+        m.cycle(250)
+        ei, ek, eh = m.energies()
+        eimax = max(eimax, ei)
+        assert abs(ei + ek + eh - ek0) < 0.03 * ek0, (ei, ek, eh, ek0)
+    assert eimax > 0.8 * ek0
+    edge = d["icodt"] == 7
+    assert np.all(m.d[edge] == 0.0) and np.all(m.vr[edge] == 0.0)
+
+
+def test_plate_impact_is_bounded_and_plastic():
+    d = dk.shell_plate(n=16, patch=4, v0=1.5e4)
+    m = po.ShellModel(d, nthreads=2)
+    ek0 = 0.5 * np.sum(d["ms"] * np.sum(d["v"] ** 2, axis=1))
+    m.cycle(300)
+    ei, ek, eh = m.energies()
+    assert 0.85 * ek0 < ei + ek + eh < 1.15 * ek0
+    assert m.st[:, po.SH_PLA0:po.SH_PLA0 + 5].max() > 1e-3
