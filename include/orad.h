@@ -1,5 +1,6 @@
 /*
- * orad.h -- C ABI of the MI355X explicit brick element loop.
+ * orad.h -- C ABI of the MI355X explicit element loop (8-node bricks and
+ * 4-node Belytschko-Tsay shells).
  *
  * The Fortran engine binds these through ISO_C_BINDING (see
  * openradioss_amd/fortran/orad_bind.f90); Python binds them through ctypes
@@ -65,8 +66,8 @@ int orad_set_law2(orad_ctx *ctx, const double *p, const int *ip);
  * hm_read_prop14.F:348-362) and DTFAC1(1) (input/lectur.F:410-416). */
 int orad_set_solid_prop(orad_ctx *ctx, double qa, double qb, double hvis, double dtfac);
 
-/* PARITH/ON arrays as the starter writes them: ADSKY(NUMNOD+1) and IADS(8,NUMELS),
- * both 1-based (parith_on_mod.F90:38-73). */
+/* PARITH/ON arrays as the starter writes them: ADSKY(NUMNOD+1) and IADS(8,NUMELS)
+ * for bricks or IADC(4,NUMELC) for shells, both 1-based (parith_on_mod.F90:38-73). */
 int orad_get_pon(orad_ctx *ctx, int *adsky, int *iads);
 
 /* Time state (RESOL scalars).  Defaults: TT=DT1=DT2=0, DT2OLD=DTINI/1.1 with
@@ -105,13 +106,67 @@ int orad_get_brick_state(orad_ctx *ctx, double *sig, double *eint, double *rho,
 int orad_set_brick_state(orad_ctx *ctx, const double *sig, const double *eint,
                          const double *rho, const double *pla, const double *qvis,
                          const double *offg);
-/* FSKY slots: fsky(4, 8*NUMELS) = (fx, fy, fz, stifn) in slot order. */
+/* FSKY slots in slot order: bricks fsky(4, 8*NUMELS) = (fx, fy, fz, STI);
+ * shells fsky(8, 4*NUMELC) = (fx, fy, fz, mx, my, mz, STI, STIR) as CUPDT3P
+ * writes them (cupdt3.F:1060-1067). */
 int orad_get_fsky(orad_ctx *ctx, double *fsky);
 
-/* Energy balance terms (ECRIT, output/ecrit.F:500-508): internal sum(EINT*V0),
- * kinetic 0.5*sum(MS*V^2), cumulative hourglass energy (PARTSAV(8,:), shvis3.F:431-469).
- * Sums use a fixed reduction order, so they are run-to-run reproducible. */
+/* Energy balance terms (ECRIT, output/ecrit.F:500-508): internal energy (bricks
+ * sum(EINT*V0); shells sum(EINT(1)+EINT(2)), cbilan.F:182), kinetic energy
+ * 0.5*sum(MS*V^2) (+ 0.5*sum(IN*VR^2) with rotations), cumulative hourglass energy
+ * (PARTSAV(8,:), shvis3.F:431-469, chvis3.F:366-393).  Sums use a fixed reduction
+ * order, so they are run-to-run reproducible. */
 int orad_energies(orad_ctx *ctx, double *eint, double *ekin, double *ehour);
+
+/* Imposed velocities, FIXVEL semantics for a constant velocity curve
+ * (constraints/general/impvel/fixvel.F:362-375): after BCS10, the imposed
+ * component's acceleration becomes (VEL - V)/DT12.  nodes 1-based, dir 1..3
+ * (x, y, z).  Replaces the previous set; n = 0 clears it.  Both element families. */
+int orad_set_fixvel(orad_ctx *ctx, int n, const int *nodes, const int *dir, const double *vel);
+
+/* ------------------------------ shells ------------------------------ */
+
+/* Rotational DOF of the nodes (call after orad_set_nodes): VR(3,NUMNOD),
+ * nodal inertia IN(NUMNOD) and ICODR(NUMNOD) (same bit meaning as ICODT on
+ * VR/AR).  Replaces NODES%VR, NODES%IN, NODES%ICODR read by RESOL
+ * (resol.F:6639-6641 ACCELE, 7022 BCS10). */
+int orad_set_nodes_rot(orad_ctx *ctx, const double *vr, const double *in, const int *icodr);
+
+/* Shells: IXC(NIXC=7,NUMELC) as the engine stores it (ccoor3.F:69-78):
+ * IXC(1)=material, IXC(2..5)=nodes (1-based, IXC(5)=IXC(4) for a triangle),
+ * IXC(6)=property, IXC(7)=user id; thke = element thickness THKE.  Builds the
+ * PARITH/ON CSR like orad_set_bricks and initialises the state (THK=THKE, OFF=1). */
+int orad_set_shells(orad_ctx *ctx, int numelc, const int *ixc, const double *thke);
+
+/* Law 2 for shells: the starter's derived values (hm_read_mat02_jc.F90:234-339)
+ * p[0]=rho0 p[1]=E p[2]=nu p[3]=G p[4]=A11 p[5]=A12 p[6]=SSP p[7]=A p[8]=B
+ * p[9]=n p[10]=EPS_max p[11]=SIG_max p[12]=C p[13]=EPS0 p[14]=m p[15]=PM(9);
+ * ip[0]=ICC ip[1]=VP ip[2]=ISRATE ip[3]=IPLA (0 radial return, 1 Newton,
+ * 2 approximate normal: m2cplr.F:174-377, IPARG(29) from cgrtails.F:823-829). */
+int orad_set_shell_law2(orad_ctx *ctx, const double *p, const int *ip);
+
+/* /PROP/SHELL TYPE1: g[0..2]=hm, hf, hr (GEO(13..15)), g[3]=DM membrane damping
+ * (GROUP_PARAM%VISC_DM), g[4]=shear factor GEO(38), g[5]=DTFAC1(3),
+ * g[6]=DTMIN1(3); ish=GEO(37); npt = 2..5 points through the thickness
+ * (hm_read_prop01.F:155-244).  Ishell=1, Ismstr=2, Ithick=0 are implied. */
+int orad_set_shell_prop(orad_ctx *ctx, const double *g, int ish, int npt);
+
+/* Drop-in for one shell group, CFORC3 semantics (shell/coque/cforc3.F:398-746,
+ * called from forintc.F:420): elements NFT+1..NFT+NEL; writes their FSKY slots
+ * (CUPDT3P), updates element state, lowers *dt2t to the group minimum (CDT3). */
+int orad_cforc3(orad_ctx *ctx, int nft, int nel, double dt1, double *dt2t);
+
+/* ASSPAR4 with rotations: A(3,NUMNOD), AR(3,NUMNOD), STIFN, STIFR. */
+int orad_asspar4_rot(orad_ctx *ctx, double *a, double *ar, double *stifn, double *stifr);
+
+/* Downloads of VR(3,NUMNOD) and AR(3,NUMNOD) (any pointer may be null). */
+int orad_get_nodes_rot(orad_ctx *ctx, double *vr, double *ar);
+
+/* Shell element state as rows of 53 FP64 values per element:
+ * FOR(5) MOM(3) THK EINT(2) EPSD HOUR(5) OFF, then SIG(5) for points 1..5,
+ * PLA(1..5), point EPSD(1..5) (GBUF/LBUF of elbufdef_mod.F90). */
+int orad_get_shell_state(orad_ctx *ctx, double *rows);
+int orad_set_shell_state(orad_ctx *ctx, const double *rows);
 
 #ifdef __cplusplus
 }

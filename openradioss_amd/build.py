@@ -1,8 +1,8 @@
 """Builds openradioss_amd/liborad.so (HIP, gfx950) in-tree with hipcc.
 
-The FP64 element kernel and the nodal kernels are compiled without FMA
+The FP64 element kernels and the nodal kernels are compiled without FMA
 contraction so their arithmetic follows the reference's double-precision
-build operation for operation; the mixed (FP32) element kernel keeps
+build operation for operation; the mixed (FP32) element kernels keep
 contraction.  Packed-FP32 instructions are disabled for every HIP file.
 """
 import os
@@ -22,6 +22,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
 SOURCES = [
     ("orad_brick_fp64.hip", ["-ffp-contract=off"]),
     ("orad_brick_mixed.hip", ["-ffp-contract=fast"]),
+    ("orad_shell_fp64.hip", ["-ffp-contract=off"]),
+    ("orad_shell_mixed.hip", ["-ffp-contract=fast"]),
     ("orad_nodal.hip", ["-ffp-contract=off"]),
     ("orad_api.hip", []),
 ]

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -18,91 +19,7 @@
 #include "../../include/orad.h"
 #include "orad_common.hpp"
 
-namespace orad {
-hipError_t launch_brick_fp64(const BrickArgs<double> &, int, int, const double *,
-                             unsigned long long *, double *, int, int, hipStream_t);
-hipError_t launch_brick_mixed(const BrickArgs<float> &, int, int, const double *,
-                              unsigned long long *, double *, int, int, hipStream_t);
-int brick_block_fp64();
-int brick_block_mixed();
-hipError_t launch_node(const NodeArgs &, bool, int, int, int, int, hipStream_t);
-hipError_t launch_energy(bool, int, const void *, const double *, int, const double4 *,
-                         const double4 *, double *, double *, hipStream_t);
-}  // namespace orad
-
-using namespace orad;
-
-struct orad_ctx {
-  int device = 0;
-  int precision = ORAD_FP64;
-  hipStream_t stream = nullptr;
-  std::string err;
-
-  int numnod = 0, numels = 0;
-  bool have_nodes = false, have_bricks = false, have_mat = false, have_prop = false;
-  bool state_init = false;
-  Law2 mat{};
-  SProp prop{};
-
-  std::vector<int32_t> h_adsky, h_iads;  // 0-based, element-major iads [e*8+k]
-
-  // device buffers
-  double4 *X4 = nullptr, *V4 = nullptr;
-  double *D = nullptr, *A = nullptr, *STIFN = nullptr;
-  int32_t *conn = nullptr, *iads = nullptr, *adsky = nullptr;
-  void *state = nullptr;  // SoA element state of the storage type (NSTATE columns)
-  double *vol0 = nullptr; // initial volumes, FP64
-  void *fsky = nullptr;
-  void *sti = nullptr;              // per-element STI/4
-  int32_t *slot_elem = nullptr;     // element owning each slot
-  double *ehou_part = nullptr;
-  int n_ehou_part = 0;
-  CycleCtl *ctl = nullptr;
-  double *scratch_dt1 = nullptr;
-  unsigned long long *scratch_dt2t = nullptr;
-
-  int host_ncycle = 0;  // cycle counter mirrored on the host (its parity selects buffers)
-  hipGraphExec_t graph[2] = {nullptr, nullptr};  // two cycles starting at parity 0 / 1
-  int xcd_remap = 1;
-  int elem_lds = -1;        // dynamic LDS bytes per element workgroup; < 0: the form's default
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-
-  size_t tsize() const { return precision == ORAD_MIXED ? sizeof(float) : sizeof(double); }
-};
-
-static int fail(orad_ctx *c, int code, const std::string &m) {
-  if (c) c->err = m;
-  return code;
-}
-
-#define HIPCHK(c, call)                                                              \
-  do {                                                                               \
-    hipError_t e_ = (call);                                                          \
-    if (e_ != hipSuccess)                                                            \
-      return fail((c), ORAD_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-template <typename P>
-static hipError_t dalloc(P **p, size_t bytes) {
-  *p = nullptr;
-  if (bytes == 0) bytes = 16;
-  return hipMalloc((void **)p, bytes);
-}
-
-static void dfree(void *p) {
-  if (p) (void)hipFree(p);
-}
-
-static void drop_graphs(orad_ctx *c) {
-  for (auto &g : c->graph)
-    if (g) {
-      (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
-}
-
-static constexpr int NSTATE = 12;  // sig(6) eint rho pla qvis offg epsd
-enum { C_EINT = 6, C_RHO = 7, C_PLA = 8, C_QVIS = 9, C_OFFG = 10, C_EPSD = 11 };
+#include "orad_ctx.hpp"
 
 template <typename T>
 static BrickArgs<T> brick_args(orad_ctx *c) {
@@ -145,11 +62,42 @@ static NodeArgs node_args(orad_ctx *c) {
   a.ehou_part = c->ehou_part;
   a.n_ehou_part = c->n_ehou_part;
   a.ctl = c->ctl;
+  a.VR4 = c->family == FAM_SHELL ? c->VR4 : nullptr;
+  a.AR = c->AR;
+  a.STIFR = c->STIFR;
+  a.fv_val = c->fv_val;
+  return a;
+}
+
+template <typename T>
+static ShellArgs<T> shell_args(orad_ctx *c) {
+  ShellArgs<T> a{};
+  a.numelc = c->numelc;
+  a.conn = c->conn;
+  a.iadc = c->iads;
+  a.X4 = c->X4;
+  a.V4 = c->V4;
+  a.VR4 = c->VR4;
+  a.thke = c->thke;
+  a.st = (T *)c->state;
+  a.sti = (T *)c->sti;
+  a.fsky = (Slot6<T> *)c->fsky;
+  a.ctl = c->ctl;
+  a.mat = c->shmat;
+  a.prop = c->shprop;
+  a.k = c->shk;
   return a;
 }
 
 static hipError_t launch_elements(orad_ctx *c, int e0, int e1, const double *dt1p,
                                   unsigned long long *dt2t, double *ehou_part) {
+  if (c->family == FAM_SHELL) {
+    if (c->precision == ORAD_MIXED)
+      return launch_shell_mixed(shell_args<float>(c), e0, e1, dt1p, dt2t, ehou_part, c->xcd_remap,
+                                c->stream);
+    return launch_shell_fp64(shell_args<double>(c), e0, e1, dt1p, dt2t, ehou_part, c->xcd_remap,
+                             c->stream);
+  }
   if (c->precision == ORAD_MIXED)
     return launch_brick_mixed(brick_args<float>(c), e0, e1, dt1p, dt2t, ehou_part, c->xcd_remap,
                               c->elem_lds, c->stream);
@@ -158,8 +106,10 @@ static hipError_t launch_elements(orad_ctx *c, int e0, int e1, const double *dt1
 }
 
 // One cycle's launches at parity p (no host synchronisation, graph-capturable).
+static int nelem(const orad_ctx *c) { return c->family == FAM_SHELL ? c->numelc : c->numels; }
+
 static hipError_t enqueue_cycle(orad_ctx *c, int p) {
-  hipError_t e = launch_elements(c, 0, c->numels, &c->ctl->s[p].dt2, &c->ctl->dt2t[p],
+  hipError_t e = launch_elements(c, 0, nelem(c), &c->ctl->s[p].dt2, &c->ctl->dt2t[p],
                                  c->ehou_part);
   if (e != hipSuccess) return e;
   return launch_node(node_args(c), c->precision == ORAD_MIXED, p, 1, 0, c->xcd_remap, c->stream);
@@ -184,6 +134,8 @@ static int init_state(orad_ctx *c) {
   c->state_init = true;
   return ORAD_OK;
 }
+
+static int get_fsky_shell(orad_ctx *c, double *fsky);
 
 extern "C" {
 
@@ -233,7 +185,8 @@ int orad_destroy(orad_ctx *c) {
                   (void *)c->conn, (void *)c->iads, (void *)c->adsky, c->state, (void *)c->vol0,
                   c->fsky, c->sti, (void *)c->slot_elem,
                   (void *)c->ehou_part, (void *)c->ctl, (void *)c->scratch_dt1,
-                  (void *)c->scratch_dt2t})
+                  (void *)c->scratch_dt2t, (void *)c->VR4, (void *)c->AR, (void *)c->STIFR,
+                  (void *)c->thke, (void *)c->fv_val})
     dfree(p);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -266,7 +219,8 @@ int orad_set_nodes(orad_ctx *c, int numnod, const double *x, const double *v, co
     hx[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], ms[i]);
     const int code = icodt ? icodt[i] : 0;
     if (code < 0 || code > 7) return fail(c, ORAD_EINVAL, "set_nodes: ICODT outside 0..7");
-    hv[i] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], (double)code);
+    const int hi = c->h_code_hi.empty() ? 0 : c->h_code_hi[i];
+    hv[i] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], (double)(code | hi));
   }
   HIPCHK(c, hipMemcpy(c->X4, hx.data(), n * sizeof(double4), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->V4, hv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
@@ -281,7 +235,7 @@ int orad_set_nodes(orad_ctx *c, int numnod, const double *x, const double *v, co
 int orad_set_bricks(orad_ctx *c, int numels, const int *ixs, const double *vol0) {
   if (!c) return ORAD_EINVAL;
   if (!c->have_nodes) return fail(c, ORAD_ESTATE, "set_bricks: call orad_set_nodes first");
-  if (c->have_bricks) return fail(c, ORAD_ESTATE, "set_bricks: mesh already set");
+  if (c->family != FAM_NONE) return fail(c, ORAD_ESTATE, "set_bricks: mesh already set");
   if (numels <= 0 || !ixs || !vol0) return fail(c, ORAD_EINVAL, "set_bricks: bad arguments");
   HIPCHK(c, hipSetDevice(c->device));
   const size_t ne = (size_t)numels;
@@ -348,6 +302,8 @@ int orad_set_bricks(orad_ctx *c, int numels, const int *ixs, const double *vol0)
   HIPCHK(c, hipMemcpy(c->vol0, vol0, ne * sizeof(double), hipMemcpyHostToDevice));
   c->numels = numels;
   c->have_bricks = true;
+  c->family = FAM_BRICK;
+  c->nc = 8;
   drop_graphs(c);
   return init_state(c);
 }
@@ -381,7 +337,7 @@ int orad_set_solid_prop(orad_ctx *c, double qa, double qb, double hvis, double d
 
 int orad_get_pon(orad_ctx *c, int *adsky, int *iads) {
   if (!c) return ORAD_EINVAL;
-  if (!c->have_bricks) return fail(c, ORAD_ESTATE, "get_pon: no mesh");
+  if (c->family == FAM_NONE) return fail(c, ORAD_ESTATE, "get_pon: no mesh");
   if (adsky)
     for (int n = 0; n <= c->numnod; ++n) adsky[n] = c->h_adsky[n] + 1;
   if (iads)
@@ -417,6 +373,12 @@ int orad_get_time(orad_ctx *c, orad_time_t *t) {
 }
 
 static int ready(orad_ctx *c) {
+  if (c->family == FAM_SHELL) {
+    if (!(c->have_nodes && c->have_rot && c->have_shells && c->have_shmat && c->have_shprop))
+      return fail(c, ORAD_ESTATE,
+                  "shell mesh, nodes with rotations, material and property must be set first");
+    return ORAD_OK;
+  }
   if (!(c->have_nodes && c->have_bricks && c->have_mat && c->have_prop))
     return fail(c, ORAD_ESTATE, "mesh, nodes, material and property must be set first");
   return ORAD_OK;
@@ -424,6 +386,7 @@ static int ready(orad_ctx *c) {
 
 int orad_sforc3(orad_ctx *c, int nft, int nel, double dt1, double *dt2t) {
   if (!c || !dt2t) return ORAD_EINVAL;
+  if (c->family != FAM_BRICK) return fail(c, ORAD_ESTATE, "sforc3: the mesh is not of bricks");
   int r = ready(c);
   if (r) return r;
   if (nft < 0 || nel < 0 || nft + nel > c->numels)
@@ -511,7 +474,7 @@ int orad_profile_cycles(orad_ctx *c, int ncycle, double *elem_ms, double *node_m
   for (int i = 0; i < ncycle; ++i) {
     const int p = c->host_ncycle & 1;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(c, launch_elements(c, 0, c->numels, &c->ctl->s[p].dt2, &c->ctl->dt2t[p], c->ehou_part));
+    HIPCHK(c, launch_elements(c, 0, nelem(c), &c->ctl->s[p].dt2, &c->ctl->dt2t[p], c->ehou_part));
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHK(c, launch_node(node_args(c), c->precision == ORAD_MIXED, p, 1, 0, c->xcd_remap, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
@@ -606,6 +569,7 @@ int orad_set_brick_state(orad_ctx *c, const double *sig, const double *eint, con
 
 int orad_get_fsky(orad_ctx *c, double *fsky) {
   if (!c || !fsky) return ORAD_EINVAL;
+  if (c->family == FAM_SHELL) return get_fsky_shell(c, fsky);
   if (!c->have_bricks) return fail(c, ORAD_ESTATE, "get_fsky: no mesh");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -637,13 +601,17 @@ int orad_energies(orad_ctx *c, double *eint, double *ekin, double *ehour) {
   int r = ready(c);
   if (r) return r;
   HIPCHK(c, hipSetDevice(c->device));
-  const size_t ne = (size_t)c->numels, ts = c->tsize();
+  const bool shell = c->family == FAM_SHELL;
+  const size_t ne = (size_t)nelem(c), ts = c->tsize();
   const size_t nbe = (ne + 255) / 256, nbk = ((size_t)c->numnod + 255) / 256;
   double *part = nullptr;
   HIPCHK(c, dalloc(&part, (nbe + nbk + 2) * sizeof(double)));
-  hipError_t e = launch_energy(c->precision == ORAD_MIXED, c->numels,
-                               (char *)c->state + C_EINT * ne * ts, c->vol0, c->numnod, c->X4,
-                               c->V4, part, part + nbe + nbk, c->stream);
+  const char *st = (const char *)c->state;
+  hipError_t e = launch_energy(c->precision == ORAD_MIXED, (int)ne,
+                               shell ? st + SC_EINT * ne * ts : st + C_EINT * ne * ts,
+                               shell ? st + (SC_EINT + 1) * ne * ts : nullptr,
+                               shell ? nullptr : c->vol0, c->numnod, c->X4, c->V4,
+                               shell ? c->VR4 : nullptr, part, part + nbe + nbk, c->stream);
   double res[2] = {0, 0}, eh = 0;
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) e = hipMemcpy(res, part + nbe + nbk, 2 * sizeof(double), hipMemcpyDeviceToHost);
@@ -657,3 +625,374 @@ int orad_energies(orad_ctx *c, double *eint, double *ekin, double *ehour) {
 }
 
 }  // extern "C"
+
+// ================================================================ shells ===
+
+// constant_mod values of the shell path, derived from their definitions
+// (common_source/modules/constant_mod.F): each a double operation on exact or
+// previously rounded values, as the compiler folds the reference's parameters.
+static ShConst shell_constants() {
+  volatile double ten = 10.0;
+  double ep[31];
+  ep[2] = 100.0;
+  for (int i = 3; i <= 20; ++i) ep[i] = ep[i - 1] * ten;
+  const double ep10 = ep[10], ep20 = ep[20], ep30 = ep20 * ep10;
+  const double em01 = 1.0 / ten, em02 = 1.0 / ep[2], em04 = 1.0 / ep[4], em05 = 1.0 / ep[5];
+  const double em06 = 1.0 / ep[6];
+  const double zep1 = em01, zep01 = em02, zep03 = 3.0 / ep[2], zep07 = zep1 - zep03;
+  const double zep09 = zep1 - zep01, zep19 = zep1 + zep09;
+  const double twoem3 = 2.0 / ep[3], fourem3 = 4.0 / ep[3], fourem5 = 4.0 / ep[5];
+  const double sixem5 = em04 - fourem5, nineem6 = em05 - em06;
+  const double zep4 = 4.0 / ten, onep4 = 1.0 + zep4, onep41 = onep4 + zep01;
+  ShConst k{};
+  k.zep81 = 1.0 - zep19;
+  k.zep072169 = zep07 + twoem3 + em04 + sixem5 + nineem6;
+  k.onep414 = onep41 + fourem3;
+  k.four_over_3 = 4.0 / 3.0;
+  k.one_over_9 = 1.0 / 9.0;
+  k.four_over_5 = 4.0 / 5.0;
+  k.em30 = 1.0 / ep30;
+  k.sr2d2 = std::sqrt(2.0) * 0.5;
+  k.em15 = 1.0 / ep[15];
+  k.em20 = 1.0 / ep20;
+  k.em07 = 1.0 / ep[7];
+  k.three_over_4 = 3.0 / 4.0;
+  k.third = 1.0 / 3.0;
+  k.three_half = 3.0 / 2.0;
+  return k;
+}
+
+// COQINI tables (engine/source/elements/shell/coqini.F:45-121): default-REAL
+// DATA literals, widened from single precision as the reference's double build holds them.
+static void shell_tables(int npt, ShProp &p) {
+  static const float z0[5][5] = {{0.f, 0.f, 0.f, 0.f, 0.f},
+                                 {-.5f, 0.5f, 0.f, 0.f, 0.f},
+                                 {-.5f, 0.f, 0.5f, 0.f, 0.f},
+                                 {-.5f, -.1666667f, 0.1666667f, 0.5f, 0.f},
+                                 {-.5f, -.25f, 0.f, 0.25f, 0.5f}};
+  static const float wf[5][5] = {{1.f, 0.f, 0.f, 0.f, 0.f},
+                                 {0.5f, 0.5f, 0.f, 0.f, 0.f},
+                                 {0.25f, 0.5f, 0.25f, 0.f, 0.f},
+                                 {0.1666667f, 0.3333333f, 0.3333333f, 0.1666667f, 0.f},
+                                 {0.125f, 0.25f, 0.25f, 0.25f, 0.125f}};
+  static const float wm[5][5] = {{0.f, 0.f, 0.f, 0.f, 0.f},
+                                 {-.0833333f, 0.0833333f, 0.f, 0.f, 0.f},
+                                 {-.0833333f, 0.f, 0.0833333f, 0.f, 0.f},
+                                 {-.0648148f, -.0555556f, 0.0555556f, 0.0648148f, 0.f},
+                                 {-.0520833f, -.0625f, 0.f, 0.0625f, 0.0520833f}};
+  for (int i = 0; i < 5; ++i) {
+    p.z0[i] = i < npt ? (double)z0[npt - 1][i] : 0.0;
+    p.wf[i] = i < npt ? (double)wf[npt - 1][i] : 0.0;
+    p.wm[i] = i < npt ? (double)wm[npt - 1][i] : 0.0;
+  }
+}
+
+static int upload_codes(orad_ctx *c) {
+  // Re-pack V4.w after a change of ICODR or FIXVEL: keep ICODT, replace the high bits.
+  if (!c->have_nodes) return ORAD_OK;
+  const size_t n = (size_t)c->numnod;
+  std::vector<double4> hv(n);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(hv.data(), c->V4, n * sizeof(double4), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) {
+    const int code = (int)hv[i].w;
+    const int hi = c->h_code_hi.empty() ? 0 : c->h_code_hi[i];
+    hv[i].w = (double)(code_icodt(code) | hi);
+  }
+  HIPCHK(c, hipMemcpy(c->V4, hv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+  return ORAD_OK;
+}
+
+static int init_shell_state(orad_ctx *c) {
+  if (!c->have_shells || c->state_init) return ORAD_OK;
+  const size_t ne = (size_t)c->numelc, ts = c->tsize();
+  std::vector<char> all(SC_NCOL * ne * ts, 0);
+  std::vector<double> thk(ne);
+  HIPCHK(c, hipMemcpy(thk.data(), c->thke, ne * 8, hipMemcpyDeviceToHost));
+  for (size_t e = 0; e < ne; ++e) {
+    if (c->precision == ORAD_MIXED) {
+      ((float *)all.data())[SC_THK * ne + e] = (float)thk[e];
+      ((float *)all.data())[SC_OFFG * ne + e] = 1.0f;
+    } else {
+      ((double *)all.data())[SC_THK * ne + e] = thk[e];
+      ((double *)all.data())[SC_OFFG * ne + e] = 1.0;
+    }
+  }
+  HIPCHK(c, hipMemcpy(c->state, all.data(), all.size(), hipMemcpyHostToDevice));
+  c->state_init = true;
+  return ORAD_OK;
+}
+
+extern "C" {
+
+int orad_set_nodes_rot(orad_ctx *c, const double *vr, const double *in, const int *icodr) {
+  if (!c) return ORAD_EINVAL;
+  if (!c->have_nodes) return fail(c, ORAD_ESTATE, "set_nodes_rot: call orad_set_nodes first");
+  if (!in) return fail(c, ORAD_EINVAL, "set_nodes_rot: nodal inertia IN is required");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  const size_t n = (size_t)c->numnod;
+  if (!c->have_rot) {
+    HIPCHK(c, dalloc(&c->VR4, n * sizeof(double4)));
+    HIPCHK(c, dalloc(&c->AR, 3 * n * sizeof(double)));
+    HIPCHK(c, dalloc(&c->STIFR, n * sizeof(double)));
+    HIPCHK(c, hipMemset(c->AR, 0, 3 * n * sizeof(double)));
+    HIPCHK(c, hipMemset(c->STIFR, 0, n * sizeof(double)));
+  }
+  std::vector<double4> hw(n);
+  if (c->h_code_hi.empty()) c->h_code_hi.assign(n, 0);
+  for (size_t i = 0; i < n; ++i) {
+    hw[i] = make_double4(vr ? vr[3 * i] : 0.0, vr ? vr[3 * i + 1] : 0.0, vr ? vr[3 * i + 2] : 0.0,
+                         in[i]);
+    const int r = icodr ? icodr[i] : 0;
+    if (r < 0 || r > 7) return fail(c, ORAD_EINVAL, "set_nodes_rot: ICODR outside 0..7");
+    c->h_code_hi[i] = (c->h_code_hi[i] & ~(7 << 3)) | (r << 3);
+  }
+  HIPCHK(c, hipMemcpy(c->VR4, hw.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+  c->have_rot = true;
+  return upload_codes(c);
+}
+
+int orad_set_fixvel(orad_ctx *c, int n, const int *nodes, const int *dir, const double *vel) {
+  if (!c) return ORAD_EINVAL;
+  if (!c->have_nodes) return fail(c, ORAD_ESTATE, "set_fixvel: call orad_set_nodes first");
+  if (n < 0 || (n > 0 && (!nodes || !dir || !vel)))
+    return fail(c, ORAD_EINVAL, "set_fixvel: bad arguments");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  const size_t nn = (size_t)c->numnod;
+  if (c->h_code_hi.empty()) c->h_code_hi.assign(nn, 0);
+  c->h_fv_val.assign(nn, 0.0);
+  for (size_t i = 0; i < nn; ++i) c->h_code_hi[i] &= ~(3 << 6);
+  for (int j = 0; j < n; ++j) {
+    const int nd = nodes[j];
+    if (nd < 1 || nd > c->numnod) return fail(c, ORAD_EINVAL, "set_fixvel: node id out of range");
+    if (dir[j] < 1 || dir[j] > 3) return fail(c, ORAD_EINVAL, "set_fixvel: direction not 1..3");
+    c->h_code_hi[nd - 1] = (c->h_code_hi[nd - 1] & ~(3 << 6)) | (dir[j] << 6);
+    c->h_fv_val[nd - 1] = vel[j];
+  }
+  if (!c->fv_val) HIPCHK(c, dalloc(&c->fv_val, nn * sizeof(double)));
+  HIPCHK(c, hipMemcpy(c->fv_val, c->h_fv_val.data(), nn * 8, hipMemcpyHostToDevice));
+  return upload_codes(c);
+}
+
+int orad_set_shells(orad_ctx *c, int numelc, const int *ixc, const double *thke) {
+  if (!c) return ORAD_EINVAL;
+  if (!c->have_nodes) return fail(c, ORAD_ESTATE, "set_shells: call orad_set_nodes first");
+  if (c->family != FAM_NONE) return fail(c, ORAD_ESTATE, "set_shells: mesh already set");
+  if (numelc <= 0 || !ixc || !thke) return fail(c, ORAD_EINVAL, "set_shells: bad arguments");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t ne = (size_t)numelc;
+  const int nn = c->numnod;
+  std::vector<int32_t> conn_em(4 * ne), uid(ne);
+  for (size_t e = 0; e < ne; ++e) {
+    for (int k = 0; k < 4; ++k) {
+      const int nd = ixc[7 * e + 1 + k];
+      if (nd < 1 || nd > nn) return fail(c, ORAD_EINVAL, "set_shells: node id out of range");
+      conn_em[4 * e + k] = nd - 1;
+    }
+    if (!(thke[e] > 0.0)) return fail(c, ORAD_EINVAL, "set_shells: thickness must be positive");
+    uid[e] = ixc[7 * e + 6];
+  }
+  // PARITH/ON CSR with the starter's ordering (FILLCNE + W_PON), 4 corners.
+  std::vector<int32_t> cnt(nn + 1, 0);
+  for (size_t i = 0; i < 4 * ne; ++i) cnt[conn_em[i]]++;
+  c->h_adsky.assign(nn + 1, 0);
+  for (int n = 0; n < nn; ++n) c->h_adsky[n + 1] = c->h_adsky[n] + cnt[n];
+  for (int n = 0; n < nn; ++n) cnt[n] = c->h_adsky[n];
+  std::vector<int32_t> order(ne);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return uid[a] < uid[b]; });
+  c->h_iads.assign(4 * ne, 0);
+  for (size_t j = 0; j < ne; ++j) {
+    const int32_t e = order[j];
+    for (int k = 0; k < 4; ++k) c->h_iads[4 * (size_t)e + k] = cnt[conn_em[4 * (size_t)e + k]]++;
+  }
+  std::vector<int32_t> conn_cm(4 * ne), iads_cm(4 * ne), se(4 * ne);
+  for (size_t e = 0; e < ne; ++e)
+    for (int k = 0; k < 4; ++k) {
+      conn_cm[k * ne + e] = conn_em[4 * e + k];
+      iads_cm[k * ne + e] = c->h_iads[4 * e + k];
+      se[c->h_iads[4 * e + k]] = (int32_t)e;
+    }
+  const size_t ts = c->tsize();
+  HIPCHK(c, dalloc(&c->conn, 4 * ne * 4));
+  HIPCHK(c, dalloc(&c->iads, 4 * ne * 4));
+  HIPCHK(c, dalloc(&c->adsky, (nn + 1) * 4));
+  HIPCHK(c, dalloc(&c->slot_elem, 4 * ne * 4));
+  HIPCHK(c, hipMemcpy(c->conn, conn_cm.data(), 4 * ne * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->iads, iads_cm.data(), 4 * ne * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->adsky, c->h_adsky.data(), (nn + 1) * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->slot_elem, se.data(), 4 * ne * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, dalloc(&c->state, SC_NCOL * ne * ts));
+  HIPCHK(c, hipMemset(c->state, 0, SC_NCOL * ne * ts));
+  HIPCHK(c, dalloc(&c->fsky, 4 * ne * 6 * ts));
+  HIPCHK(c, hipMemset(c->fsky, 0, 4 * ne * 6 * ts));
+  HIPCHK(c, dalloc(&c->sti, ne * ts));
+  HIPCHK(c, hipMemset(c->sti, 0, ne * ts));
+  HIPCHK(c, dalloc(&c->thke, ne * 8));
+  HIPCHK(c, hipMemcpy(c->thke, thke, ne * 8, hipMemcpyHostToDevice));
+  const int bb = c->precision == ORAD_MIXED ? shell_block_mixed() : shell_block_fp64();
+  c->n_ehou_part = (int)((ne + bb - 1) / bb);
+  HIPCHK(c, dalloc(&c->ehou_part, (size_t)c->n_ehou_part * sizeof(double)));
+  HIPCHK(c, hipMemset(c->ehou_part, 0, (size_t)c->n_ehou_part * sizeof(double)));
+  c->numelc = numelc;
+  c->have_shells = true;
+  c->family = FAM_SHELL;
+  c->nc = 4;
+  c->shk = shell_constants();
+  drop_graphs(c);
+  return init_shell_state(c);
+}
+
+int orad_set_shell_law2(orad_ctx *c, const double *p, const int *ip) {
+  if (!c) return ORAD_EINVAL;
+  if (!p || !ip) return fail(c, ORAD_EINVAL, "set_shell_law2: null parameters");
+  ShMat m{};
+  m.rho0 = p[0]; m.young = p[1]; m.nu = p[2]; m.shear = p[3]; m.a11 = p[4]; m.a12 = p[5];
+  m.ssp = p[6]; m.ca = p[7]; m.cb = p[8]; m.cn = p[9]; m.epmx = p[10]; m.ymax = p[11];
+  m.cc = p[12]; m.epdr = p[13]; m.mexp = p[14]; m.asrate_pm9 = p[15];
+  m.icc = ip[0]; m.vp = ip[1]; m.israte = ip[2]; m.ipla = ip[3];
+  if (!(m.rho0 > 0.0) || !(m.young > 0.0) || !(m.shear > 0.0) || !(m.ssp > 0.0))
+    return fail(c, ORAD_EINVAL, "set_shell_law2: rho0, E, G and SSP must be positive");
+  if (!(m.ca > 0.0)) return fail(c, ORAD_EINVAL, "set_shell_law2: A must be positive");
+  if (m.vp < 1 || m.vp > 3 || m.ipla < 0 || m.ipla > 2)
+    return fail(c, ORAD_EINVAL, "set_shell_law2: VP must be 1..3 and IPLA 0..2");
+  if (m.cc != 0.0 && !(m.epdr > 0.0)) return fail(c, ORAD_EINVAL, "set_shell_law2: EPS0 must be > 0");
+  c->shmat = m;
+  c->have_shmat = true;
+  drop_graphs(c);
+  return ORAD_OK;
+}
+
+int orad_set_shell_prop(orad_ctx *c, const double *g, int ish, int npt) {
+  if (!c) return ORAD_EINVAL;
+  if (!g) return fail(c, ORAD_EINVAL, "set_shell_prop: null parameters");
+  if (npt < 2 || npt > 5) return fail(c, ORAD_EINVAL, "set_shell_prop: NPT must be 2..5");
+  if (ish < 0 || ish > 1) return fail(c, ORAD_EINVAL, "set_shell_prop: ISHEAR must be 0 or 1");
+  ShProp p{};
+  p.hm = g[0]; p.hf = g[1]; p.hr = g[2]; p.dm = g[3]; p.fsh = g[4]; p.dtfac = g[5]; p.dtmin = g[6];
+  p.ish = ish;
+  p.npt = npt;
+  if (!(p.hm > 0.0 || p.hf > 0.0) || !(p.dtfac > 0.0) || p.dm < 0.0 || !(p.fsh > 0.0))
+    return fail(c, ORAD_EINVAL, "set_shell_prop: hm/hf, DTFAC and the shear factor must be positive");
+  shell_tables(npt, p);
+  c->shprop = p;
+  c->have_shprop = true;
+  drop_graphs(c);
+  return ORAD_OK;
+}
+
+int orad_cforc3(orad_ctx *c, int nft, int nel, double dt1, double *dt2t) {
+  if (!c || !dt2t) return ORAD_EINVAL;
+  if (c->family != FAM_SHELL) return fail(c, ORAD_ESTATE, "cforc3: the mesh is not of shells");
+  int r = ready(c);
+  if (r) return r;
+  if (nft < 0 || nel < 0 || nft + nel > c->numelc)
+    return fail(c, ORAD_EINVAL, "cforc3: group outside the element range");
+  HIPCHK(c, hipSetDevice(c->device));
+  unsigned long long bits = 0x7FF0000000000000ull;
+  HIPCHK(c, hipMemcpyAsync(c->scratch_dt1, &dt1, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->scratch_dt2t, &bits, sizeof(bits), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch_elements(c, nft, nft + nel, c->scratch_dt1, c->scratch_dt2t, nullptr));
+  HIPCHK(c, hipMemcpyAsync(&bits, c->scratch_dt2t, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double g;
+  std::memcpy(&g, &bits, sizeof(g));
+  if (g < *dt2t) *dt2t = g;
+  return ORAD_OK;
+}
+
+int orad_asspar4_rot(orad_ctx *c, double *a, double *ar, double *stifn, double *stifr) {
+  if (!c) return ORAD_EINVAL;
+  if (c->family != FAM_SHELL) return fail(c, ORAD_ESTATE, "asspar4_rot: the mesh is not of shells");
+  int r = ready(c);
+  if (r) return r;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, launch_node(node_args(c), c->precision == ORAD_MIXED, c->host_ncycle & 1, 0, 1,
+                        c->xcd_remap, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t n = (size_t)c->numnod;
+  if (a) HIPCHK(c, hipMemcpy(a, c->A, 3 * n * 8, hipMemcpyDeviceToHost));
+  if (ar) HIPCHK(c, hipMemcpy(ar, c->AR, 3 * n * 8, hipMemcpyDeviceToHost));
+  if (stifn) HIPCHK(c, hipMemcpy(stifn, c->STIFN, n * 8, hipMemcpyDeviceToHost));
+  if (stifr) HIPCHK(c, hipMemcpy(stifr, c->STIFR, n * 8, hipMemcpyDeviceToHost));
+  return ORAD_OK;
+}
+
+int orad_get_nodes_rot(orad_ctx *c, double *vr, double *ar) {
+  if (!c) return ORAD_EINVAL;
+  if (!c->have_rot) return fail(c, ORAD_ESTATE, "get_nodes_rot: no rotational DOF");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t n = (size_t)c->numnod;
+  if (vr) {
+    std::vector<double4> h(n);
+    HIPCHK(c, hipMemcpy(h.data(), c->VR4, n * sizeof(double4), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) { vr[3 * i] = h[i].x; vr[3 * i + 1] = h[i].y; vr[3 * i + 2] = h[i].z; }
+  }
+  if (ar) HIPCHK(c, hipMemcpy(ar, c->AR, 3 * n * 8, hipMemcpyDeviceToHost));
+  return ORAD_OK;
+}
+
+int orad_get_shell_state(orad_ctx *c, double *rows) {
+  if (!c || !rows) return ORAD_EINVAL;
+  if (c->family != FAM_SHELL) return fail(c, ORAD_ESTATE, "get_shell_state: no shell mesh");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t ne = (size_t)c->numelc, ts = c->tsize();
+  std::vector<char> h(SC_NCOL * ne * ts);
+  HIPCHK(c, hipMemcpy(h.data(), c->state, h.size(), hipMemcpyDeviceToHost));
+  for (size_t e = 0; e < ne; ++e)
+    for (int k = 0; k < SC_NCOL; ++k)
+      rows[e * SC_NCOL + k] = c->precision == ORAD_MIXED ? (double)((float *)h.data())[k * ne + e]
+                                                         : ((double *)h.data())[k * ne + e];
+  return ORAD_OK;
+}
+
+int orad_set_shell_state(orad_ctx *c, const double *rows) {
+  if (!c || !rows) return ORAD_EINVAL;
+  if (c->family != FAM_SHELL) return fail(c, ORAD_ESTATE, "set_shell_state: no shell mesh");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t ne = (size_t)c->numelc, ts = c->tsize();
+  std::vector<char> h(SC_NCOL * ne * ts);
+  for (size_t e = 0; e < ne; ++e)
+    for (int k = 0; k < SC_NCOL; ++k) {
+      if (c->precision == ORAD_MIXED) ((float *)h.data())[k * ne + e] = (float)rows[e * SC_NCOL + k];
+      else ((double *)h.data())[k * ne + e] = rows[e * SC_NCOL + k];
+    }
+  HIPCHK(c, hipMemcpy(c->state, h.data(), h.size(), hipMemcpyHostToDevice));
+  c->state_init = true;
+  return ORAD_OK;
+}
+
+}  // extern "C"
+
+static int get_fsky_shell(orad_ctx *c, double *fsky) {
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t ne = (size_t)c->numelc, ns = 4 * ne;
+  std::vector<double> f(6 * ns), st(ne);
+  if (c->precision == ORAD_MIXED) {
+    std::vector<float> h(6 * ns), hs(ne);
+    HIPCHK(c, hipMemcpy(h.data(), c->fsky, h.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(hs.data(), c->sti, ne * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < h.size(); ++i) f[i] = h[i];
+    for (size_t i = 0; i < ne; ++i) st[i] = hs[i];
+  } else {
+    HIPCHK(c, hipMemcpy(f.data(), c->fsky, 6 * ns * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(st.data(), c->sti, ne * 8, hipMemcpyDeviceToHost));
+  }
+  for (size_t e = 0; e < ne; ++e)
+    for (int k = 0; k < 4; ++k) {
+      const size_t sl = (size_t)c->h_iads[4 * e + k];
+      for (int j = 0; j < 6; ++j) fsky[8 * sl + j] = f[6 * sl + j];
+      fsky[8 * sl + 6] = st[e];
+      fsky[8 * sl + 7] = 0.0;
+    }
+  return ORAD_OK;
+}

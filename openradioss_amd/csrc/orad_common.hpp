@@ -61,6 +61,67 @@ struct BrickArgs {
   SProp prop;
 };
 
+// ---------------------------------------------------------------- shells --
+// Law 2 as the shell chain reads it: the starter's PM values and MAT_PARAM
+// (hm_read_mat02_jc.F90:234-339) and the plasticity algorithm IPLA (cgrtails.F:823-829).
+struct ShMat {
+  double rho0, young, nu, shear, a11, a12, ssp;
+  double ca, cb, cn, epmx, ymax, cc, epdr, mexp, asrate_pm9;
+  int icc, vp, israte, ipla;
+};
+
+// /PROP/SHELL values (hm_read_prop01.F:155-244) with the COQINI through-thickness
+// tables for npt points (positions z0, force weights wf, moment weights wm).
+struct ShProp {
+  double hm, hf, hr, dm, fsh, dtfac, dtmin;
+  int ish, npt;
+  double z0[5], wf[5], wm[5];
+};
+
+// constant_mod values used on the shell path, derived on the host from their
+// definitions (the same double operations as the reference's parameters).
+struct ShConst {
+  double zep81, zep072169, onep414, four_over_3, one_over_9, four_over_5, em30, sr2d2;
+  double em15, em20, em07, three_over_4, third, three_half;
+};
+
+// One shell PARITH/ON slot: corner force and moment (FSKY(1:6,·)).  STI
+// (FSKY(7,·)) is kept once per element; STIR (FSKY(8,·)) is zero on this path.
+template <typename T> struct Slot6 { T fx, fy, fz, mx, my, mz; };
+
+// Shell element state columns (structure of arrays, column c at st[c*ne + e]).
+enum {
+  SC_FOR = 0, SC_MOM = 5, SC_THK = 8, SC_EINT = 9, SC_EPSD = 11, SC_HOUR = 12, SC_OFFG = 17,
+  SC_SIG = 18,     // + 5*pt + component
+  SC_PLA = 43,     // + pt
+  SC_EPSDP = 48,   // + pt
+  SC_NCOL = 53
+};
+
+template <typename T>
+struct ShellArgs {
+  int numelc;
+  const int32_t *conn;      // [4][ne]
+  const int32_t *iadc;      // [4][ne]
+  const double4 *X4;        // (x, y, z, MS)
+  const double4 *V4;        // (vx, vy, vz, codes)
+  const double4 *VR4;       // (vrx, vry, vrz, IN)
+  const double *thke;       // element thickness THKE (FP64)
+  T *st;                    // [SC_NCOL][ne]
+  T *sti;                   // STI per element (FSKY(7,·))
+  Slot6<T> *fsky;
+  CycleCtl *ctl;
+  ShMat mat;
+  ShProp prop;
+  ShConst k;
+};
+
+// Node record codes (V4.w): ICODT bits 0-2, ICODR bits 3-5, FIXVEL direction
+// (1..3, 0 = none) bits 6-7.
+__host__ __device__ __forceinline__ int code_icodt(int c) { return c & 7; }
+__host__ __device__ __forceinline__ int code_icodr(int c) { return (c >> 3) & 7; }
+__host__ __device__ __forceinline__ int code_fvdir(int c) { return (c >> 6) & 3; }
+
 struct NodeArgs {
   int numnod;
   const int32_t *adsky;     // [numnod+1]
@@ -75,6 +136,11 @@ struct NodeArgs {
   const double *ehou_part;
   int n_ehou_part;
   CycleCtl *ctl;
+  // rotational DOF (shells) and imposed velocities
+  double4 *VR4;             // (vrx, vry, vrz, IN), null for bricks
+  double *AR;               // angular accelerations (assembled moments in ASSPAR4)
+  double *STIFR;
+  const double *fv_val;     // FIXVEL value per node, null when none is imposed
 };
 
 // Logical block index that gives each XCD a contiguous run of blocks (the

@@ -1,0 +1,124 @@
+// orad_ctx.hpp -- the context behind the C ABI (internal to liborad).
+//
+// One context models one mesh of one element family (8-node bricks or 4-node
+// shells) on one GPU: device arrays, the PARITH/ON CSR built on the host with
+// the starter's ordering, and the captured cycle graphs.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/orad.h"
+#include "orad_common.hpp"
+
+namespace orad {
+hipError_t launch_brick_fp64(const BrickArgs<double> &, int, int, const double *,
+                             unsigned long long *, double *, int, int, hipStream_t);
+hipError_t launch_brick_mixed(const BrickArgs<float> &, int, int, const double *,
+                              unsigned long long *, double *, int, int, hipStream_t);
+int brick_block_fp64();
+int brick_block_mixed();
+hipError_t launch_node(const NodeArgs &, bool, int, int, int, int, hipStream_t);
+hipError_t launch_energy(bool, int, const void *, const void *, const double *, int,
+                         const double4 *, const double4 *, const double4 *, double *, double *,
+                         hipStream_t);
+hipError_t launch_shell_fp64(const ShellArgs<double> &, int, int, const double *,
+                             unsigned long long *, double *, int, hipStream_t);
+hipError_t launch_shell_mixed(const ShellArgs<float> &, int, int, const double *,
+                              unsigned long long *, double *, int, hipStream_t);
+int shell_block_fp64();
+int shell_block_mixed();
+}  // namespace orad
+
+using namespace orad;
+
+enum { FAM_NONE = 0, FAM_BRICK = 1, FAM_SHELL = 2 };
+
+struct orad_ctx {
+  int device = 0;
+  int family = FAM_NONE;
+  int precision = ORAD_FP64;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  int numnod = 0, numels = 0;
+  bool have_nodes = false, have_bricks = false, have_mat = false, have_prop = false;
+  bool state_init = false;
+  Law2 mat{};
+  SProp prop{};
+
+  std::vector<int32_t> h_adsky, h_iads;  // 0-based, element-major iads [e*nc+k]
+  int nc = 8;                             // corners per element of the family
+
+  // shells
+  int numelc = 0;
+  bool have_shells = false, have_shmat = false, have_shprop = false, have_rot = false;
+  ShMat shmat{};
+  ShProp shprop{};
+  ShConst shk{};
+  double4 *VR4 = nullptr;
+  double *AR = nullptr, *STIFR = nullptr, *thke = nullptr;
+  // imposed velocities (FIXVEL) and rotational codes, folded into V4.w on upload
+  std::vector<int> h_code_hi;   // per node: ICODR << 3 | FIXVEL direction << 6
+  std::vector<double> h_fv_val;
+  double *fv_val = nullptr;
+
+  // device buffers
+  double4 *X4 = nullptr, *V4 = nullptr;
+  double *D = nullptr, *A = nullptr, *STIFN = nullptr;
+  int32_t *conn = nullptr, *iads = nullptr, *adsky = nullptr;
+  void *state = nullptr;  // SoA element state of the storage type (NSTATE columns)
+  double *vol0 = nullptr; // initial volumes, FP64
+  void *fsky = nullptr;
+  void *sti = nullptr;              // per-element STI/4
+  int32_t *slot_elem = nullptr;     // element owning each slot
+  double *ehou_part = nullptr;
+  int n_ehou_part = 0;
+  CycleCtl *ctl = nullptr;
+  double *scratch_dt1 = nullptr;
+  unsigned long long *scratch_dt2t = nullptr;
+
+  int host_ncycle = 0;  // cycle counter mirrored on the host (its parity selects buffers)
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // two cycles starting at parity 0 / 1
+  int xcd_remap = 1;
+  int elem_lds = -1;        // dynamic LDS bytes per element workgroup; < 0: the form's default
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+
+  size_t tsize() const { return precision == ORAD_MIXED ? sizeof(float) : sizeof(double); }
+};
+
+static inline int fail(orad_ctx *c, int code, const std::string &m) {
+  if (c) c->err = m;
+  return code;
+}
+
+#define HIPCHK(c, call)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail((c), ORAD_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename P>
+static inline hipError_t dalloc(P **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  return hipMalloc((void **)p, bytes);
+}
+
+static inline void dfree(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+static inline void drop_graphs(orad_ctx *c) {
+  for (auto &g : c->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+static constexpr int NSTATE = 12;  // sig(6) eint rho pla qvis offg epsd
+enum { C_EINT = 6, C_RHO = 7, C_PLA = 8, C_QVIS = 9, C_OFFG = 10, C_EPSD = 11 };
+

@@ -7,8 +7,12 @@
 //   DT2      (resol.F:2640,5857-5861,6080,6224-6226) from the element minimum;
 //   ACCELE   (assembly/accele.F:64-82)       a = a / MS, 0 where MS <= 0;
 //   BCS10    (constraints/general/bcs/bcs1.F) zero fixed components of a and v;
-//   VELOCITY (assembly/velocity.F:56-64)     v = v + DT12 a;
+//   FIXVEL   (constraints/general/impvel/fixvel.F:362-375) a = (v_imp - v)/DT12
+//            on the imposed component;
+//   VELOCITY (assembly/velocity.F:56-64,77-83) v = v + DT12 a (and vr = vr + DT12 ar);
 //   DEPLA    (assembly/displacement.F:88-101) d = d + DT2 v, x = x + DT2 v.
+// ROT = true adds the rotational DOF of shells: the slots carry moments, ACCELE
+// divides by the nodal inertia IN (accele.F:95-109) and BCS10 applies ICODR.
 // Workgroup 0 also publishes the next cycle's time scalars (double-buffered by
 // cycle parity), resets the next cycle's dt minimum and adds this cycle's
 // hourglass-energy partials to the running total in a fixed order.
@@ -26,12 +30,16 @@ template <> struct SlotTile<double> { static constexpr int TILE = 1024; };
 // strided reads below (about 8 words apart) spread over the banks.
 __device__ __forceinline__ int lds_sw(int i) { return i ^ ((i >> 5) & 31); }
 
-template <typename R>
+template <typename R, bool ROT> struct SlotOf { typedef Slot3<R> type; };
+template <typename R> struct SlotOf<R, true> { typedef Slot6<R> type; };
+
+template <typename R, bool ROT>
 __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity, int mode,
                                                           int write_a, int xcd_remap) {
-  typedef Slot3<R> S;
+  typedef typename SlotOf<R, ROT>::type S;
+  constexpr int NC = ROT ? 6 : 3;
   constexpr int TILE = SlotTile<R>::TILE;
-  __shared__ R t_x[TILE], t_y[TILE], t_z[TILE];
+  __shared__ R t[NC][TILE];
   const int nb = gridDim.x;
   const int b = xcd_remap ? xcd_block(blockIdx.x, nb) : (int)blockIdx.x;
   const int n0 = b * NODE_BLOCK;
@@ -85,31 +93,39 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
   const int s0 = a.adsky[n0], s1 = a.adsky[n1];
   const bool live = n < a.numnod;
   const int k0 = live ? a.adsky[n] : 0, k1 = live ? a.adsky[n + 1] : 0;
-  double ax = 0.0, ay = 0.0, az = 0.0, sf = 0.0;
+  double acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
   for (int ts = s0; ts < s1; ts += TILE) {
     const int te = (ts + TILE < s1) ? ts + TILE : s1;
     __syncthreads();
     for (int i = threadIdx.x; i < te - ts; i += NODE_BLOCK) {
-      const S f = fsky[ts + i];
       const int j = lds_sw(i);
-      t_x[j] = f.x; t_y[j] = f.y; t_z[j] = f.z;
+      const R *f = reinterpret_cast<const R *>(&fsky[ts + i]);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) t[c][j] = f[c];
     }
     __syncthreads();
     const int lo = (k0 > ts) ? k0 : ts, hi = (k1 < te) ? k1 : te;
     for (int k = lo; k < hi; ++k) {
       const int j = lds_sw(k - ts);
-      ax = ax + double(t_x[j]);
-      ay = ay + double(t_y[j]);
-      az = az + double(t_z[j]);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] = acc[c] + double(t[c][j]);
     }
   }
   if (!live) return;
+  double ax = acc[0], ay = acc[1], az = acc[2];
   if (mode == 0) {
     // STIFN (FSKY(7,·) summed in slot order); not consumed by the explicit cycle.
+    double sf = 0.0;
     const R *sti = reinterpret_cast<const R *>(a.sti);
     for (int k = k0; k < k1; ++k) sf = sf + double(sti[a.slot_elem[k]]);
     a.A[3 * n] = ax; a.A[3 * n + 1] = ay; a.A[3 * n + 2] = az;
     a.STIFN[n] = sf;
+    if (ROT) {
+      a.AR[3 * n] = acc[NC - 3]; a.AR[3 * n + 1] = acc[NC - 2]; a.AR[3 * n + 2] = acc[NC - 1];
+      a.STIFR[n] = 0.0;   // FSKY(8,·) = STIR is zero on this path (cdt3.F:219-227)
+    }
     return;
   }
   double4 X = a.X4[n];
@@ -122,15 +138,46 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
     ax = 0.0; ay = 0.0; az = 0.0;
   }
   const int code = (int)V.w;
-  if (code & 4) { V.x = 0.0; ax = 0.0; }
-  if (code & 2) { V.y = 0.0; ay = 0.0; }
-  if (code & 1) { V.z = 0.0; az = 0.0; }
+  double rx = 0.0, ry = 0.0, rz = 0.0;
+  double4 W = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (ROT) {
+    W = a.VR4[n];
+    rx = acc[NC - 3]; ry = acc[NC - 2]; rz = acc[NC - 1];
+    const double in = W.w;
+    if (in > 0.0) {
+      const double RTMP = 1.0 / in;
+      rx = rx * RTMP; ry = ry * RTMP; rz = rz * RTMP;
+    } else {
+      rx = 0.0; ry = 0.0; rz = 0.0;
+    }
+  }
+  const int ct = code_icodt(code);
+  if (ct & 4) { V.x = 0.0; ax = 0.0; }
+  if (ct & 2) { V.y = 0.0; ay = 0.0; }
+  if (ct & 1) { V.z = 0.0; az = 0.0; }
+  if (ROT) {
+    const int cr = code_icodr(code);
+    if (cr & 4) { W.x = 0.0; rx = 0.0; }
+    if (cr & 2) { W.y = 0.0; ry = 0.0; }
+    if (cr & 1) { W.z = 0.0; rz = 0.0; }
+  }
+  const int fd = code_fvdir(code);
+  if (fd == 1) ax = (a.fv_val[n] - V.x) / dt12;
+  if (fd == 2) ay = (a.fv_val[n] - V.y) / dt12;
+  if (fd == 3) az = (a.fv_val[n] - V.z) / dt12;
   if (write_a) {
     a.A[3 * n] = ax; a.A[3 * n + 1] = ay; a.A[3 * n + 2] = az;
+    if (ROT) { a.AR[3 * n] = rx; a.AR[3 * n + 1] = ry; a.AR[3 * n + 2] = rz; }
   }
   V.x = V.x + dt12 * ax;
   V.y = V.y + dt12 * ay;
   V.z = V.z + dt12 * az;
+  if (ROT) {
+    W.x = W.x + dt12 * rx;
+    W.y = W.y + dt12 * ry;
+    W.z = W.z + dt12 * rz;
+    a.VR4[n] = W;
+  }
   const double vdx = dt2 * V.x, vdy = dt2 * V.y, vdz = dt2 * V.z;
   double *D = a.D + 3 * n;
   D[0] = D[0] + vdx; D[1] = D[1] + vdy; D[2] = D[2] + vdz;
@@ -140,23 +187,30 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
 }
 
 // Energy partial sums: block b sums its range in a fixed tree order.
+// Bricks: eint per unit initial volume times vol0.  Shells (vol0 null): membrane
+// plus bending energy eint[e] + eint2[e]; kinetic energy adds 0.5*IN*|vr|^2.
 template <typename T>
 __global__ __launch_bounds__(NODE_BLOCK) void energy_partial_kernel(
-    int numels, const T *eint, const double *vol0, int numnod, const double4 *X4, const double4 *V4,
-    double *part_e, double *part_k, int nbe) {
+    int numels, const T *eint, const T *eint2, const double *vol0, int numnod, const double4 *X4,
+    const double4 *V4, const double4 *VR4, double *part_e, double *part_k, int nbe) {
   __shared__ double s[NODE_BLOCK];
   const int b = blockIdx.x;
   double v = 0.0;
   double *out;
   if (b < nbe) {
     const int e = b * NODE_BLOCK + threadIdx.x;
-    if (e < numels) v = double(eint[e]) * double(vol0[e]);
+    if (e < numels)
+      v = vol0 ? double(eint[e]) * double(vol0[e]) : double(eint[e]) + double(eint2[e]);
     out = part_e + b;
   } else {
     const int n = (b - nbe) * NODE_BLOCK + threadIdx.x;
     if (n < numnod) {
       const double4 V = V4[n];
       v = 0.5 * X4[n].w * (V.x * V.x + V.y * V.y + V.z * V.z);
+      if (VR4) {
+        const double4 W = VR4[n];
+        v += 0.5 * W.w * (W.x * W.x + W.y * W.y + W.z * W.z);
+      }
     }
     out = part_k + (b - nbe);
   }
@@ -193,35 +247,44 @@ hipError_t launch_node(const NodeArgs &a, bool slot_fp32, int parity, int mode, 
                        int xcd_remap, hipStream_t s) {
   const int nb = (a.numnod + NODE_BLOCK - 1) / NODE_BLOCK;
   if (nb <= 0) return hipSuccess;
-  if (slot_fp32)
-    hipLaunchKernelGGL(node_kernel<float>, dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity, mode,
-                       write_a, xcd_remap);
+  const bool rot = a.VR4 != nullptr;
+  if (slot_fp32 && rot)
+    hipLaunchKernelGGL((node_kernel<float, true>), dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity,
+                       mode, write_a, xcd_remap);
+  else if (slot_fp32)
+    hipLaunchKernelGGL((node_kernel<float, false>), dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity,
+                       mode, write_a, xcd_remap);
+  else if (rot)
+    hipLaunchKernelGGL((node_kernel<double, true>), dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity,
+                       mode, write_a, xcd_remap);
   else
-    hipLaunchKernelGGL(node_kernel<double>, dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity, mode,
-                       write_a, xcd_remap);
+    hipLaunchKernelGGL((node_kernel<double, false>), dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity,
+                       mode, write_a, xcd_remap);
   return hipGetLastError();
 }
 
 template <typename T>
-static hipError_t launch_energy_t(int numels, const T *eint, const double *vol0, int numnod,
-                                  const double4 *X4, const double4 *V4, double *part,
-                                  double *res, hipStream_t s) {
+static hipError_t launch_energy_t(int numels, const T *eint, const T *eint2, const double *vol0,
+                                  int numnod, const double4 *X4, const double4 *V4,
+                                  const double4 *VR4, double *part, double *res, hipStream_t s) {
   const int nbe = (numels + NODE_BLOCK - 1) / NODE_BLOCK;
   const int nbk = (numnod + NODE_BLOCK - 1) / NODE_BLOCK;
   if (nbe + nbk > 0)
     hipLaunchKernelGGL(energy_partial_kernel<T>, dim3(nbe + nbk), dim3(NODE_BLOCK), 0, s, numels,
-                       eint, vol0, numnod, X4, V4, part, part + nbe, nbe);
+                       eint, eint2, vol0, numnod, X4, V4, VR4, part, part + nbe, nbe);
   hipLaunchKernelGGL(energy_final_kernel, dim3(1), dim3(NODE_BLOCK), 0, s, part, nbe, part + nbe,
                      nbk, res);
   return hipGetLastError();
 }
 
-hipError_t launch_energy(bool fp32, int numels, const void *eint, const double *vol0, int numnod,
-                         const double4 *X4, const double4 *V4, double *part, double *res,
-                         hipStream_t s) {
+hipError_t launch_energy(bool fp32, int numels, const void *eint, const void *eint2,
+                         const double *vol0, int numnod, const double4 *X4, const double4 *V4,
+                         const double4 *VR4, double *part, double *res, hipStream_t s) {
   if (fp32)
-    return launch_energy_t<float>(numels, (const float *)eint, vol0, numnod, X4, V4, part, res, s);
-  return launch_energy_t<double>(numels, (const double *)eint, vol0, numnod, X4, V4, part, res, s);
+    return launch_energy_t<float>(numels, (const float *)eint, (const float *)eint2, vol0, numnod,
+                                  X4, V4, VR4, part, res, s);
+  return launch_energy_t<double>(numels, (const double *)eint, (const double *)eint2, vol0,
+                                 numnod, X4, V4, VR4, part, res, s);
 }
 
 }  // namespace orad

@@ -22,7 +22,12 @@ EXPORTS = (
     "orad_set_law2", "orad_set_solid_prop", "orad_get_pon", "orad_set_time", "orad_get_time",
     "orad_sforc3", "orad_asspar4", "orad_run_cycles", "orad_profile_cycles", "orad_get_nodes",
     "orad_get_brick_state", "orad_set_brick_state", "orad_get_fsky", "orad_energies",
+    "orad_set_fixvel", "orad_set_nodes_rot", "orad_set_shells", "orad_set_shell_law2",
+    "orad_set_shell_prop", "orad_cforc3", "orad_asspar4_rot", "orad_get_nodes_rot",
+    "orad_get_shell_state", "orad_set_shell_state",
 )
+NIXC = 7
+SH_NCOL = 53
 
 
 class TimeState(C.Structure):
@@ -70,6 +75,16 @@ def load_library(path=LIB_PATH):
         "orad_set_brick_state": ([P, P, P, P, P, P, P], I),
         "orad_get_fsky": ([P, P], I),
         "orad_energies": ([P, C.POINTER(D), C.POINTER(D), C.POINTER(D)], I),
+        "orad_set_fixvel": ([P, I, P, P, P], I),
+        "orad_set_nodes_rot": ([P, P, P, P], I),
+        "orad_set_shells": ([P, I, P, P], I),
+        "orad_set_shell_law2": ([P, P, P], I),
+        "orad_set_shell_prop": ([P, P, I, I], I),
+        "orad_cforc3": ([P, I, I, D, C.POINTER(D)], I),
+        "orad_asspar4_rot": ([P, P, P, P, P], I),
+        "orad_get_nodes_rot": ([P, P, P], I),
+        "orad_get_shell_state": ([P, P], I),
+        "orad_set_shell_state": ([P, P], I),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -102,6 +117,19 @@ def ixs_from_deck(deck):
     return ixs
 
 
+def _set_fixvel(solver, deck):
+    fv = deck.get("fv_dir")
+    if fv is None:
+        return
+    nodes = np.flatnonzero(np.asarray(fv) > 0)
+    if nodes.size == 0:
+        return
+    ids = np.ascontiguousarray(nodes + 1, dtype=np.int32)
+    dirs = np.ascontiguousarray(np.asarray(fv)[nodes], dtype=np.int32)
+    vals = np.ascontiguousarray(np.asarray(deck["fv_val"])[nodes], dtype=np.float64)
+    solver._chk(solver.lib.orad_set_fixvel(solver.h, nodes.size, _p(ids), _p(dirs), _p(vals)))
+
+
 class BrickSolver:
     """One model (bricks, law 2) resident on one GPU."""
 
@@ -130,6 +158,7 @@ class BrickSolver:
         ixs = ixs_from_deck(deck)
         vol0 = _f64(deck["vol0"], (self.numels,))
         self._chk(self.lib.orad_set_bricks(h, self.numels, _p(ixs), _p(vol0)))
+        _set_fixvel(self, deck)
 
     def _chk(self, rc):
         if rc != ORAD_OK:
@@ -219,6 +248,108 @@ class BrickSolver:
         ei, ek, eh = C.c_double(), C.c_double(), C.c_double()
         self._chk(self.lib.orad_energies(self.h, C.byref(ei), C.byref(ek), C.byref(eh)))
         return ei.value, ek.value, eh.value
+
+
+def ixc_from_deck(deck):
+    """IXC(NIXC=7, NUMELC) in the engine's layout (ccoor3.F:69-78): material,
+    4 nodes (1-based), property, user id."""
+    ne = deck["ixc"].shape[0]
+    ixc = np.zeros((ne, NIXC), np.int32)
+    ixc[:, 0] = 1
+    ixc[:, 1:5] = deck["ixc"] + 1
+    ixc[:, 5] = 1
+    ixc[:, 6] = deck["uid"]
+    return ixc
+
+
+SHMAT_KEYS = ("rho0", "young", "nu", "shear", "a11", "a12", "ssp", "ca", "cb", "cn", "epmx",
+              "ymax", "cc", "epdr", "mexp", "asrate_pm9")
+
+
+class ShellSolver:
+    """One Belytschko-Tsay shell model (law 2) resident on one GPU: the CFORC3
+    path with 6-DOF nodes."""
+
+    def __init__(self, deck, precision=FP64, device=0):
+        self.lib = load_library()
+        self.precision = precision
+        h = C.c_void_p()
+        rc = self.lib.orad_create(C.byref(h), device, precision)
+        if rc != ORAD_OK:
+            raise OradError(rc, "orad_create failed (no HIP device?)")
+        self.h = h
+        nn = deck["x"].shape[0]
+        self.numnod = nn
+        self.numelc = deck["ixc"].shape[0]
+        x = _f64(deck["x"], (nn, 3))
+        v = _f64(deck["v"], (nn, 3))
+        ms = _f64(deck["ms"], (nn,))
+        icodt = np.ascontiguousarray(deck["icodt"], dtype=np.int32)
+        self._chk(self.lib.orad_set_nodes(h, nn, _p(x), _p(v), _p(ms), _p(icodt)))
+        vr = _f64(deck.get("vr", np.zeros((nn, 3))), (nn, 3))
+        inr = _f64(deck["in"], (nn,))
+        icodr = np.ascontiguousarray(deck["icodr"], dtype=np.int32)
+        self._chk(self.lib.orad_set_nodes_rot(h, _p(vr), _p(inr), _p(icodr)))
+        _set_fixvel(self, deck)
+        mat, prop = deck["mat"], deck["prop"]
+        p = np.array([mat[k] for k in SHMAT_KEYS], np.float64)
+        ip = np.array([mat["icc"], mat["vp"], mat["israte"], mat["ipla"]], np.int32)
+        self._chk(self.lib.orad_set_shell_law2(h, _p(p), _p(ip)))
+        g = np.array([prop[k] for k in ("hm", "hf", "hr", "dm", "fsh", "dtfac", "dtmin")],
+                     np.float64)
+        self._chk(self.lib.orad_set_shell_prop(h, _p(g), int(prop["ish"]), int(prop["npt"])))
+        ixc = ixc_from_deck(deck)
+        thk = _f64(deck["thk"], (self.numelc,))
+        self._chk(self.lib.orad_set_shells(h, self.numelc, _p(ixc), _p(thk)))
+
+    _chk = BrickSolver._chk
+    close = BrickSolver.close
+    __del__ = BrickSolver.__del__
+    run_cycles = BrickSolver.run_cycles
+    profile_cycles = BrickSolver.profile_cycles
+    time = BrickSolver.time
+    set_time = BrickSolver.set_time
+    energies = BrickSolver.energies
+
+    def pon(self):
+        adsky = np.zeros(self.numnod + 1, np.int32)
+        iadc = np.zeros((self.numelc, 4), np.int32)
+        self._chk(self.lib.orad_get_pon(self.h, _p(adsky), _p(iadc)))
+        return adsky, iadc
+
+    def cforc3(self, nft, nel, dt1, dt2t=1.0e30):
+        """Forces of shells nft+1..nft+nel (CFORC3); returns the group DT2T."""
+        d = C.c_double(dt2t)
+        self._chk(self.lib.orad_cforc3(self.h, nft, nel, dt1, C.byref(d)))
+        return d.value
+
+    def asspar4(self):
+        a = np.zeros((self.numnod, 3)); ar = np.zeros_like(a)
+        s = np.zeros(self.numnod); sr = np.zeros(self.numnod)
+        self._chk(self.lib.orad_asspar4_rot(self.h, _p(a), _p(ar), _p(s), _p(sr)))
+        return a, ar, s, sr
+
+    def nodes(self):
+        x = np.zeros((self.numnod, 3)); v = np.zeros_like(x); d = np.zeros_like(x)
+        a = np.zeros_like(x); s = np.zeros(self.numnod)
+        vr = np.zeros_like(x); ar = np.zeros_like(x)
+        self._chk(self.lib.orad_get_nodes(self.h, _p(x), _p(v), _p(d), _p(a), _p(s)))
+        self._chk(self.lib.orad_get_nodes_rot(self.h, _p(vr), _p(ar)))
+        return dict(x=x, v=v, d=d, a=a, stifn=s, vr=vr, ar=ar)
+
+    def state(self):
+        rows = np.zeros((self.numelc, SH_NCOL))
+        self._chk(self.lib.orad_get_shell_state(self.h, _p(rows)))
+        return rows
+
+    def set_state(self, rows):
+        rows = _f64(rows, (self.numelc, SH_NCOL))
+        self._chk(self.lib.orad_set_shell_state(self.h, _p(rows)))
+
+    def fsky(self):
+        f = np.zeros((4 * self.numelc, 8))
+        self._chk(self.lib.orad_get_fsky(self.h, _p(f)))
+        return f
 
 
 DECK_MAGIC = 0x4F524144   # "ORAD"

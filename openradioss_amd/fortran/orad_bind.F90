@@ -2,7 +2,8 @@
 !
 !  Style follows the reference's coupling adapters (engine/source/coupling/
 !  coupling_adapter.F90:83-131): type(c_ptr), value handles, scalars by value,
-!  arrays by reference in the engine's own layouts (X(3,NUMNOD), IXS(NIXS,NUMELS)).
+!  arrays by reference in the engine's own layouts (X(3,NUMNOD), IXS(NIXS,NUMELS),
+!  IXC(NIXC,NUMELC)).
 module orad_bind
   use iso_c_binding
   implicit none
@@ -97,6 +98,66 @@ module orad_bind
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
       real(c_double) :: eint, ekin, ehour
+    end function
+    integer(c_int) function orad_set_fixvel(ctx, n, nodes, dir, vel) bind(c, name='orad_set_fixvel')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n
+      integer(c_int) :: nodes(*), dir(*)
+      real(c_double) :: vel(*)
+    end function
+    ! ---- shells (CFORC3 path) ----
+    integer(c_int) function orad_set_nodes_rot(ctx, vr, in, icodr) bind(c, name='orad_set_nodes_rot')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: vr(3,*), in(*)
+      integer(c_int) :: icodr(*)
+    end function
+    integer(c_int) function orad_set_shells(ctx, numelc, ixc, thke) bind(c, name='orad_set_shells')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: numelc
+      integer(c_int) :: ixc(7,*)
+      real(c_double) :: thke(*)
+    end function
+    integer(c_int) function orad_set_shell_law2(ctx, p, ip) bind(c, name='orad_set_shell_law2')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: p(*)
+      integer(c_int) :: ip(*)
+    end function
+    integer(c_int) function orad_set_shell_prop(ctx, g, ish, npt) bind(c, name='orad_set_shell_prop')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: g(*)
+      integer(c_int), value :: ish, npt
+    end function
+    integer(c_int) function orad_cforc3(ctx, nft, nel, dt1, dt2t) bind(c, name='orad_cforc3')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nft, nel
+      real(c_double), value :: dt1
+      real(c_double) :: dt2t
+    end function
+    integer(c_int) function orad_asspar4_rot(ctx, a, ar, stifn, stifr) bind(c, name='orad_asspar4_rot')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: a(3,*), ar(3,*), stifn(*), stifr(*)
+    end function
+    integer(c_int) function orad_get_nodes_rot(ctx, vr, ar) bind(c, name='orad_get_nodes_rot')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: vr(3,*), ar(3,*)
+    end function
+    integer(c_int) function orad_get_shell_state(ctx, rows) bind(c, name='orad_get_shell_state')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: rows(53,*)
+    end function
+    integer(c_int) function orad_set_shell_state(ctx, rows) bind(c, name='orad_set_shell_state')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: rows(53,*)
     end function
   end interface
 

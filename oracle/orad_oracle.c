@@ -559,12 +559,13 @@ void oracle_cycle(orad_omodel_t *m) {
   /* dt (resol.F:2640, 5857-5861, 6080, 6224-6226) */
   double dt2 = 1.0e6;                         /* DT2 = EP06 */
   if (dt2t < dt2) dt2 = dt2t;
-  dt2 = dmin(dmin(dt2, ONEP1*m->dt2old), m->dtmx);
+  /* resol.F:6080 writes 1.1*DT2OLD with a default-REAL literal */
+  dt2 = dmin(dmin(dt2, (double)1.1f*m->dt2old), m->dtmx);
   m->dt2old = dt2;
   m->dt2 = dt2;
   m->dt12 = HALF*(dt1 + dt2);
   const double dt12 = m->dt12;
-  /* ACCELE -> BCS10 -> VELOCITY -> DEPLA (accele.F:64-82, bcs1.F, velocity.F:56-64,
+  /* ACCELE -> BCS10 -> FIXVEL -> VELOCITY -> DEPLA (accele.F:64-82, bcs1.F, fixvel.F:362-375,
    * displacement.F:88-101) */
 #pragma omp parallel for num_threads(nth) schedule(static)
   for (int n = 0; n < nn; ++n) {
@@ -580,6 +581,11 @@ void oracle_cycle(orad_omodel_t *m) {
     if (c & 4) { v[0] = ZERO; a[0] = ZERO; }
     if (c & 2) { v[1] = ZERO; a[1] = ZERO; }
     if (c & 1) { v[2] = ZERO; a[2] = ZERO; }
+    if (m->fv_dir && m->fv_dir[n] > 0) {
+      /* FIXVEL (constraints/general/impvel/fixvel.F:362-375), constant curve */
+      int d = m->fv_dir[n] - 1;
+      a[d] = (m->fv_val[n] - v[d]) / dt12;
+    }
     for (int d = 0; d < 3; ++d) {
       m->a[3*n + d] = a[d];
       v[d] = v[d] + dt12*a[d];

@@ -95,6 +95,8 @@ typedef struct {
   int    ncycle, mstop;
   int    nthreads;                /* OpenMP threads (0 = runtime default)         */
   int    pad;
+  const int32_t *fv_dir;          /* FIXVEL direction per node (1..3), 0 = none, or null */
+  const double *fv_val;           /* FIXVEL imposed velocity per node              */
 } orad_omodel_t;
 
 /* One full cycle: element forces -> assembly -> dt -> accel -> BC -> vel -> disp. */

@@ -743,8 +743,8 @@ void oracle_shell_cycle(orad_oshell_t *m) {
   }
   double dt2 = 1.0e6;
   if (dt2t < dt2) dt2 = dt2t;
-  double onep1 = 0x1.199999999999ap+0;
-  dt2 = dmin(dmin(dt2, onep1 * m->dt2old), m->dtmx);
+  /* resol.F:6080 writes 1.1*DT2OLD with a default-REAL literal */
+  dt2 = dmin(dmin(dt2, (double)1.1f * m->dt2old), m->dtmx);
   m->dt2old = dt2;
   m->dt2 = dt2;
   m->dt12 = HALF * (dt1 + dt2);

@@ -46,7 +46,8 @@ class OModel(C.Structure):
                 ("tt", C.c_double), ("dt1", C.c_double), ("dt2", C.c_double),
                 ("dt12", C.c_double), ("dt2old", C.c_double), ("dtmx", C.c_double),
                 ("ehour", C.c_double), ("ncycle", C.c_int), ("mstop", C.c_int),
-                ("nthreads", C.c_int), ("pad", C.c_int)]
+                ("nthreads", C.c_int), ("pad", C.c_int),
+                ("fv_dir", C.c_void_p), ("fv_val", C.c_void_p)]
 
 
 # ---- shells: one element's state row (orad_shstate_t), npt <= 5 ----
@@ -254,6 +255,10 @@ class Model:
             setattr(m, name, _ptr(getattr(self, name)))
         m.mat = law2(deck["mat"])
         m.prop = sprop(deck["prop"])
+        if deck.get("fv_dir") is not None:
+            self.fv_dir = np.ascontiguousarray(deck["fv_dir"], dtype=np.int32)
+            self.fv_val = np.ascontiguousarray(deck["fv_val"], dtype=np.float64)
+            m.fv_dir, m.fv_val = _ptr(self.fv_dir), _ptr(self.fv_val)
         m.tt, m.dt1, m.dt2, m.dt12 = 0.0, 0.0, 0.0, 0.0
         m.dt2old = 1.0e6 / 1.1          # starter contrl.F:1108-1110: DTINI/ONEP1
         m.dtmx = 1.0e20

@@ -227,3 +227,14 @@ def test_oracle_cycle_smooth_vibration_energy():
         m.cycle(40)
         ei, ek, eh = m.energies()
         assert abs(ei + ek + eh - ek0) < 0.12 * ek0
+
+
+def test_dt_growth_limit_uses_the_real_literal():
+    # resol.F:6080: DT2 = MIN(DT2, 1.1*DT2OLD, DTMX) with 1.1 a default-REAL literal,
+    # while the starting DT2OLD = DTINI/ONEP1 uses the double constant ONEP1.
+    d = dk.brick_block(3, 3, 3, v_top=-1e3)
+    m = po.Model(d, nthreads=1)
+    m.m.dt2old = 1.0e-9                      # far below the element time step
+    m.cycle(1)
+    assert m.dt2 == float(np.float32(1.1)) * 1.0e-9
+    assert m.dt2 != 1.1 * 1.0e-9
