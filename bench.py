@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -40,6 +42,80 @@ def node_bytes(ne, nn, tsize):
     """Algorithmic bytes of one nodal-kernel launch: every force slot once, and per node
     ADSKY (8) + X4, V4 (32 + 32) + D (24) read, X4, V4, D written."""
     return 8 * ne * 3 * tsize + nn * (8 + 32 + 32 + 24 + 32 + 32 + 24)
+
+
+def shell_elem_bytes(ne, nn, tsize):
+    """Algorithmic bytes of one shell element-kernel launch (5 points): connectivity
+    and slot addresses (4+4 int32), THKE (FP64), the 53 state columns read
+    (element: FOR(5) MOM(3) THK EINT(2) EPSD HOUR(5) OFFG; points: SIG(5) PLA EPSD)
+    and all but OFFG written, STI, 4 slots of 6 values, and the X4/V4/VR4 node
+    records once per node."""
+    reads = 16 + 16 + 8 + 53 * tsize
+    writes = 52 * tsize + tsize + 4 * 6 * tsize
+    return ne * (reads + writes) + nn * 96
+
+
+def shell_node_bytes(ne, nn, tsize):
+    """Nodal kernel with rotations: every slot once, ADSKY, X4/V4/VR4/D read and
+    X4/V4/VR4/D written."""
+    return 4 * ne * 6 * tsize + nn * (8 + 96 + 24 + 96 + 24)
+
+
+def _shell_forces(s, dt1):
+    """Assembled nodal forces and moments of the current state (CFORC3 + ASSPAR4),
+    leaving the element state as it was."""
+    st = s.state()
+    s.cforc3(0, s.numelc, dt1)
+    a, ar, _, _ = s.asspar4()
+    s.set_state(st)
+    return a, ar
+
+
+def shell_plate_leg(n, steps, err_cycles):
+    """The 512 x 512 BT plate (Ishell=1, 5 points, law 2, clamped edges, central
+    32 x 32 patch at 15 m/s +-10%): element-cycles/s of both forms and the mixed
+    form's error against the FP64 form after `steps` and after `err_cycles` cycles."""
+    deck = dk.shell_plate(n=n)
+    ne, nn = deck["ixc"].shape[0], deck["x"].shape[0]
+    res = {"workload": f"BT shell plate {n}x{n} ({ne} shells, {nn} nodes), 5 points, law 2",
+           "elements": ne, "nodes": nn, "cycles_timed": steps - 2}
+    snaps = {}
+    for prec, name in ((en.MIXED, "mixed"), (en.FP64, "fp64")):
+        s = en.ShellSolver(deck, prec)
+        s.run_cycles(2)                          # graph capture outside the timed region
+        t0 = time.perf_counter()
+        s.run_cycles(steps - 2)
+        dt = time.perf_counter() - t0
+        snap = []
+        for target in (steps, err_cycles):
+            t = s.time()
+            if t.ncycle < target:
+                s.run_cycles(target - t.ncycle)
+                t = s.time()
+            a, ar = _shell_forces(s, t.dt2)
+            snap.append((s.nodes()["d"], a, ar, t.tt))
+            if target == steps:
+                el, nd = s.profile_cycles(20)
+        snaps[name] = snap
+        s.close()
+        ts = 4 if prec == en.MIXED else 8
+        eb = shell_elem_bytes(ne, nn, ts)
+        res[name] = dict(value=ne * (steps - 2) / dt, ms_per_step=1e3 * dt / (steps - 2),
+                         elem_kernel_ms=el, node_kernel_ms=nd,
+                         elem_kernel_gbs=eb / (el * 1e-3) / 1e9,
+                         elem_algorithmic_bytes=eb,
+                         node_kernel_gbs=shell_node_bytes(ne, nn, ts) / (nd * 1e-3) / 1e9)
+        print(f"shell plate {name}: {res[name]['value']:.4e} el-cyc/s, elem {el:.3f} ms, "
+              f"node {nd:.3f} ms", file=sys.stderr, flush=True)
+    l2 = lambda x, y: float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+    res["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+    res["mixed_vs_fp64_error"] = {}
+    for i, cyc in enumerate((steps, err_cycles)):
+        (d32, a32, r32, t32), (d64, a64, r64, t64) = snaps["mixed"][i], snaps["fp64"][i]
+        res["mixed_vs_fp64_error"][f"after_{cyc}_cycles"] = dict(
+            nodal_force_l2=l2(a32, a64), nodal_moment_l2=l2(r32, r64),
+            displacement_l2=l2(d32, d64), time=abs(t32 - t64) / t64)
+    return res
 
 
 def run(prec, deck, steps, warmup):
@@ -106,12 +182,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-shell", action="store_true", help="skip the 512^2 shell plate leg")
+    ap.add_argument("--shell-n", type=int, default=512)
+    ap.add_argument("--shell-steps", type=int, default=1000)
+    ap.add_argument("--shell-err-cycles", type=int, default=5000)
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the slab decomposition is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path in this library; "
+              "run it with --gpus 1", file=sys.stderr)
         return 2
     n = args.n
     deck = dk.brick_block(n, n, n)
@@ -137,6 +217,9 @@ def main():
     ach = eb / (el32 * 1e-3) / 1e9
     value = ne * args.steps / dt32
     cycle_gbs = (eb + nb) * args.steps / dt32 / 1e9
+    shell = None
+    if not args.no_shell:
+        shell = shell_plate_leg(args.shell_n, args.shell_steps, args.shell_err_cycles)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(deck, args.cpu_cycles)
@@ -171,6 +254,7 @@ def main():
                         "achieved_gbs": nb / (nd32 * 1e-3) / 1e9, "algorithmic_bytes": nb},
         "fp64": fp64,
         "mixed_over_fp64": (value / fp64["value"]) if fp64 else None,
+        "shell_plate": shell,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

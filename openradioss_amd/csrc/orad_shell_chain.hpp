@@ -181,63 +181,163 @@ __device__ __forceinline__ void shell_law2_point(const ShMat &m, const ShConst &
   thkn = thkn + ezz * thklyl * off;
 }
 
-// IO: get(col) / put(col, v) on this element's state columns.
-// emit(k, fx, fy, fz, mx, my, mz): slot values (FSKY(1:6), already negated) of corner k.
-template <typename T, typename IO, typename Emit>
-__device__ __forceinline__ void shell_chain(const T xg[3][4], const T vg[3][4], const T vrg[3][4],
-                                            T thk0, bool tri, const IO &io, T dt1,
-                                            const ShMat &m, const ShProp &p, const ShConst &k,
-                                            ShellOut<T> &out, Emit emit) {
-  const T ZERO = T(0), ONE = T(1), TWO = T(2), THREE = T(3), HALF = T(0.5), FOURTH = T(0.25);
-  const T EM20 = T(k.em20);
-  // ---- CCOOR3 ----
-  T offg = io.get(SC_OFFG);
-  T off = tmin(ONE, tabs(offg));
-  const bool dead = offg < ZERO;
+// Local frame and kinematics of one element in precision TG: CNVEC3 (ISHFRAM = 0),
+// CDERI3, CDEFO3 (IHBE <= 1) and CCURV3, up to the unscaled strain and curvature
+// rates.  The mixed form runs this stage in FP64 from FP64 relative coordinates:
+// an FP32 frame is off by about 1e-7 rad, which leaks that fraction of the large
+// out-of-plane relative velocity of a bending element into its membrane rates,
+// with a sign that persists from cycle to cycle.
+template <typename TG>
+struct ShellKin {
+  TG E1X, E1Y, E1Z, E2X, E2Y, E2Z, E3X, E3Y, E3Z;
+  TG X2, Y2, X3, Y3, X4, Y4, PX1, PY1, PX2, PY2, AREA, VHX, VHY;
+  TG VX[4], VY[4], VZ[4], RX[4], RY[4];
+  TG EXX, EYY, EXY, EYZ, EXZ, KXX, KYY, KXY;
+};
+
+template <typename TG>
+__device__ __forceinline__ void shell_kinematics(const TG xg[3][4], const TG vg[3][4],
+                                                 const TG vrg[3][4], bool dead, TG dt1, TG EM20,
+                                                 ShellKin<TG> &o) {
+  const TG ZERO = TG(0), TWO = TG(2), HALF = TG(0.5), FOURTH = TG(0.25);
+  TG &E1X = o.E1X, &E1Y = o.E1Y, &E1Z = o.E1Z, &E2X = o.E2X, &E2Y = o.E2Y, &E2Z = o.E2Z;
+  TG &E3X = o.E3X, &E3Y = o.E3Y, &E3Z = o.E3Z;
+  TG &X2 = o.X2, &Y2 = o.Y2, &X3 = o.X3, &Y3 = o.Y3, &X4 = o.X4, &Y4 = o.Y4;
+  TG &PX1 = o.PX1, &PY1 = o.PY1, &PX2 = o.PX2, &PY2 = o.PY2, &AREA = o.AREA;
+  TG &VHX = o.VHX, &VHY = o.VHY;
+  TG *VX = o.VX, *VY = o.VY, *VZ = o.VZ, *RX = o.RX, *RY = o.RY;
+  TG &EXX = o.EXX, &EYY = o.EYY, &EXY = o.EXY, &EYZ = o.EYZ, &EXZ = o.EXZ;
+  TG &KXX = o.KXX, &KYY = o.KYY, &KXY = o.KXY;
+  const TG ONE = TG(1);
   // ---- CNVEC3, ISHFRAM = 0 ----
-  const T X21 = xg[0][1] - xg[0][0], X32 = xg[0][2] - xg[0][1], X34 = xg[0][2] - xg[0][3],
+  const TG X21 = xg[0][1] - xg[0][0], X32 = xg[0][2] - xg[0][1], X34 = xg[0][2] - xg[0][3],
           X41 = xg[0][3] - xg[0][0];
-  const T Y21 = xg[1][1] - xg[1][0], Y32 = xg[1][2] - xg[1][1], Y34 = xg[1][2] - xg[1][3],
+  const TG Y21 = xg[1][1] - xg[1][0], Y32 = xg[1][2] - xg[1][1], Y34 = xg[1][2] - xg[1][3],
           Y41 = xg[1][3] - xg[1][0];
-  const T Z21 = xg[2][1] - xg[2][0], Z32 = xg[2][2] - xg[2][1], Z34 = xg[2][2] - xg[2][3],
+  const TG Z21 = xg[2][1] - xg[2][0], Z32 = xg[2][2] - xg[2][1], Z34 = xg[2][2] - xg[2][3],
           Z41 = xg[2][3] - xg[2][0];
-  T E1X = (X21 + X34), E1Y = (Y21 + Y34), E1Z = (Z21 + Z34);
-  T E2X = (X32 + X41), E2Y = (Y32 + Y41), E2Z = (Z32 + Z41);
-  T E3X = E1Y * E2Z - E1Z * E2Y;
-  T E3Y = E1Z * E2X - E1X * E2Z;
-  T E3Z = E1X * E2Y - E1Y * E2X;
-  T SUMA = E3X * E3X + E3Y * E3Y + E3Z * E3Z;
-  SUMA = M<T>::div(ONE, tmax(M<T>::sqrt_(SUMA), EM20));
+  E1X = (X21 + X34); E1Y = (Y21 + Y34); E1Z = (Z21 + Z34);
+  E2X = (X32 + X41); E2Y = (Y32 + Y41); E2Z = (Z32 + Z41);
+  E3X = E1Y * E2Z - E1Z * E2Y;
+  E3Y = E1Z * E2X - E1X * E2Z;
+  E3Z = E1X * E2Y - E1Y * E2X;
+  TG SUMA = E3X * E3X + E3Y * E3Y + E3Z * E3Z;
+  SUMA = M<TG>::div(ONE, tmax(M<TG>::sqrt_(SUMA), EM20));
   E3X = E3X * SUMA; E3Y = E3Y * SUMA; E3Z = E3Z * SUMA;
   {
-    const T S1 = E1X * E1X + E1Y * E1Y + E1Z * E1Z;
-    const T S2 = E2X * E2X + E2Y * E2Y + E2Z * E2Z;
-    SUMA = M<T>::sqrt_(M<T>::div(S1, S2));
+    const TG S1 = E1X * E1X + E1Y * E1Y + E1Z * E1Z;
+    const TG S2 = E2X * E2X + E2Y * E2Y + E2Z * E2Z;
+    SUMA = M<TG>::sqrt_(M<TG>::div(S1, S2));
   }
   E1X = E1X + (E2Y * E3Z - E2Z * E3Y) * SUMA;
   E1Y = E1Y + (E2Z * E3X - E2X * E3Z) * SUMA;
   E1Z = E1Z + (E2X * E3Y - E2Y * E3X) * SUMA;
   SUMA = E1X * E1X + E1Y * E1Y + E1Z * E1Z;
-  SUMA = M<T>::div(ONE, tmax(M<T>::sqrt_(SUMA), EM20));
+  SUMA = M<TG>::div(ONE, tmax(M<TG>::sqrt_(SUMA), EM20));
   E1X = E1X * SUMA; E1Y = E1Y * SUMA; E1Z = E1Z * SUMA;
   E2X = E3Y * E1Z - E3Z * E1Y;
   E2Y = E3Z * E1X - E3X * E1Z;
   E2Z = E3X * E1Y - E3Y * E1X;
   // ---- CDERI3 ----
-  const T X31GA = xg[0][2] - xg[0][0], Y31GA = xg[1][2] - xg[1][0], Z31GA = xg[2][2] - xg[2][0];
-  const T X2 = E1X * X21 + E1Y * Y21 + E1Z * Z21;
-  const T Y2 = E2X * X21 + E2Y * Y21 + E2Z * Z21;
-  const T Y3 = E2X * X31GA + E2Y * Y31GA + E2Z * Z31GA;
-  const T X3 = E1X * X31GA + E1Y * Y31GA + E1Z * Z31GA;
-  const T X4 = E1X * X41 + E1Y * Y41 + E1Z * Z41;
-  const T Y4 = E2X * X41 + E2Y * Y41 + E2Z * Z41;
-  const T PX1 = HALF * (Y2 - Y4);
-  const T PY1 = HALF * (X4 - X2);
-  const T PX2 = HALF * Y3;
-  const T PY2 = -HALF * X3;
-  const T AREA = tmax(TWO * (PY2 * PX1 - PY1 * PX2), EM20);
-  const T VHX = M<T>::div(-X2 + X3 - X4, AREA);
-  const T VHY = M<T>::div(-Y2 + Y3 - Y4, AREA);
+  const TG X31GA = xg[0][2] - xg[0][0], Y31GA = xg[1][2] - xg[1][0], Z31GA = xg[2][2] - xg[2][0];
+  X2 = E1X * X21 + E1Y * Y21 + E1Z * Z21;
+  Y2 = E2X * X21 + E2Y * Y21 + E2Z * Z21;
+  Y3 = E2X * X31GA + E2Y * Y31GA + E2Z * Z31GA;
+  X3 = E1X * X31GA + E1Y * Y31GA + E1Z * Z31GA;
+  X4 = E1X * X41 + E1Y * Y41 + E1Z * Z41;
+  Y4 = E2X * X41 + E2Y * Y41 + E2Z * Z41;
+  PX1 = HALF * (Y2 - Y4);
+  PY1 = HALF * (X4 - X2);
+  PX2 = HALF * Y3;
+  PY2 = -HALF * X3;
+  AREA = tmax(TWO * (PY2 * PX1 - PY1 * PX2), EM20);
+  VHX = M<TG>::div(-X2 + X3 - X4, AREA);
+  VHY = M<TG>::div(-Y2 + Y3 - Y4, AREA);
+  // ---- CDEFO3, IHBE <= 1 ----
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const TG a = dead ? ZERO : vg[0][c], b = dead ? ZERO : vg[1][c], d = dead ? ZERO : vg[2][c];
+    VX[c] = E1X * a + E1Y * b + E1Z * d;
+    VY[c] = E2X * a + E2Y * b + E2Z * d;
+    VZ[c] = E3X * a + E3Y * b + E3Z * d;
+  }
+  const TG VZ13 = VZ[0] - VZ[2], VZ24 = VZ[1] - VZ[3];
+  EYZ = PY1 * VZ13 + PY2 * VZ24;
+  EXZ = PX1 * VZ13 + PX2 * VZ24;
+  {
+    const TG DT1V4 = FOURTH * dt1;
+    const TG TMP2A = PY2 + PY1;
+    const TG TMP3A = tsign(tmax(tabs(TMP2A), EM20), TMP2A);
+    const TG TMP1A = M<TG>::div(DT1V4 * (VZ13 - VZ24) * (VZ13 - VZ24), TMP3A);
+    TG VX13 = VX[0] - VX[2], VX24 = VX[1] - VX[3];
+    VX13 = VX13 - TMP1A;
+    VX24 = VX24 + TMP1A;
+    EXX = PX1 * VX13 + PX2 * VX24;
+    EXY = PY1 * VX13 + PY2 * VX24;
+    const TG TMP1B = PX2 - PX1;
+    const TG TMP3B = tsign(tmax(tabs(TMP1B), EM20), TMP1B);
+    const TG TMP2B = M<TG>::div(DT1V4 * (VZ13 + VZ24) * (VZ13 + VZ24), TMP3B);
+    TG VY13 = VY[0] - VY[2], VY24 = VY[1] - VY[3];
+    VY13 = VY13 + TMP2B;
+    VY24 = VY24 + TMP2B;
+    EXY = EXY + PX1 * VY13 + PX2 * VY24;
+    EYY = PY1 * VY13 + PY2 * VY24;
+  }
+  // ---- CCURV3 ----
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const TG a = dead ? ZERO : vrg[0][c], b = dead ? ZERO : vrg[1][c], d = dead ? ZERO : vrg[2][c];
+    RX[c] = E1X * a + E1Y * b + E1Z * d;
+    RY[c] = E2X * a + E2Y * b + E2Z * d;
+  }
+  {
+    const TG RX13TA = RX[0] - RX[2];
+    const TG RXAVTA = RX[0] + RX[1] + RX[2] + RX[3];
+    const TG RX24TA = RX[1] - RX[3];
+    KYY = -PY1 * RX13TA - PY2 * RX24TA;
+    KXY = PX1 * RX13TA + PX2 * RX24TA;
+    const TG RY13TA = RY[0] - RY[2];
+    const TG RYAVTA = RY[0] + RY[1] + RY[2] + RY[3];
+    const TG RY24TA = RY[1] - RY[3];
+    KXX = PX1 * RY13TA + PX2 * RY24TA;
+    KXY = PY1 * RY13TA + PY2 * RY24TA - KXY;
+    EXZ = EXZ + RYAVTA * (FOURTH * AREA);
+    EYZ = EYZ - RXAVTA * (FOURTH * AREA);
+  }
+  (void)VHX; (void)VHY; (void)ZERO; (void)TWO;
+}
+
+// IO: get(col) / put(col, v) on this element's state columns.
+// emit(k, fx, fy, fz, mx, my, mz): slot values (FSKY(1:6), already negated) of corner k.
+template <typename T, typename TG, typename IO, typename Emit>
+__device__ __forceinline__ void shell_chain(const TG xg[3][4], const TG vg[3][4],
+                                            const TG vrg[3][4], T thk0, bool tri, const IO &io,
+                                            double dt1d,
+                                            const ShMat &m, const ShProp &p, const ShConst &k,
+                                            ShellOut<T> &out, Emit emit) {
+  const T ZERO = T(0), ONE = T(1), TWO = T(2), THREE = T(3), HALF = T(0.5), FOURTH = T(0.25);
+  const T EM20 = T(k.em20);
+  const T dt1 = T(dt1d);
+  // ---- CCOOR3 ----
+  T offg = io.get(SC_OFFG);
+  T off = tmin(ONE, tabs(offg));
+  const bool dead = offg < ZERO;
+  ShellKin<TG> g;
+  shell_kinematics<TG>(xg, vg, vrg, dead, TG(dt1d), TG(k.em20), g);
+  const T E1X = T(g.E1X), E1Y = T(g.E1Y), E1Z = T(g.E1Z);
+  const T E2X = T(g.E2X), E2Y = T(g.E2Y), E2Z = T(g.E2Z);
+  const T E3X = T(g.E3X), E3Y = T(g.E3Y), E3Z = T(g.E3Z);
+  const T X2 = T(g.X2), Y2 = T(g.Y2), X3 = T(g.X3), Y3 = T(g.Y3), X4 = T(g.X4), Y4 = T(g.Y4);
+  const T PX1 = T(g.PX1), PY1 = T(g.PY1), PX2 = T(g.PX2), PY2 = T(g.PY2);
+  const T AREA = T(g.AREA), VHX = T(g.VHX), VHY = T(g.VHY);
+  T VX[4], VY[4], VZ[4], RX[4], RY[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    VX[c] = T(g.VX[c]); VY[c] = T(g.VY[c]); VZ[c] = T(g.VZ[c]);
+    RX[c] = T(g.RX[c]); RY[c] = T(g.RY[c]);
+  }
+  T EXX = T(g.EXX), EYY = T(g.EYY), EXY = T(g.EXY), EYZ = T(g.EYZ), EXZ = T(g.EXZ);
+  T KXX = T(g.KXX), KYY = T(g.KYY), KXY = T(g.KXY);
   // ---- CCOEF3, ITHK = 0 ----
   const T VOL0 = thk0 * AREA;
   const T THK02 = thk0 * thk0;
@@ -267,61 +367,6 @@ __device__ __forceinline__ void shell_chain(const T xg[3][4], const T vg[3][4], 
     const T DTHOUR = M<T>::div(HALF * (ALMIN + ALDT), tmax(H1, H2));
     if (DTHOUR < ALDT) ALDT = DTHOUR;
     ALDT = M<T>::sqrt_(ALDT);
-  }
-  // ---- CDEFO3, IHBE <= 1 ----
-  T VX[4], VY[4], VZ[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const T a = dead ? ZERO : vg[0][c], b = dead ? ZERO : vg[1][c], d = dead ? ZERO : vg[2][c];
-    VX[c] = E1X * a + E1Y * b + E1Z * d;
-    VY[c] = E2X * a + E2Y * b + E2Z * d;
-    VZ[c] = E3X * a + E3Y * b + E3Z * d;
-  }
-  const T VZ13 = VZ[0] - VZ[2], VZ24 = VZ[1] - VZ[3];
-  T EYZ = PY1 * VZ13 + PY2 * VZ24;
-  T EXZ = PX1 * VZ13 + PX2 * VZ24;
-  T EXX, EYY, EXY;
-  {
-    const T DT1V4 = FOURTH * dt1;
-    const T TMP2A = PY2 + PY1;
-    const T TMP3A = tsign(tmax(tabs(TMP2A), EM20), TMP2A);
-    const T TMP1A = M<T>::div(DT1V4 * (VZ13 - VZ24) * (VZ13 - VZ24), TMP3A);
-    T VX13 = VX[0] - VX[2], VX24 = VX[1] - VX[3];
-    VX13 = VX13 - TMP1A;
-    VX24 = VX24 + TMP1A;
-    EXX = PX1 * VX13 + PX2 * VX24;
-    EXY = PY1 * VX13 + PY2 * VX24;
-    const T TMP1B = PX2 - PX1;
-    const T TMP3B = tsign(tmax(tabs(TMP1B), EM20), TMP1B);
-    const T TMP2B = M<T>::div(DT1V4 * (VZ13 + VZ24) * (VZ13 + VZ24), TMP3B);
-    T VY13 = VY[0] - VY[2], VY24 = VY[1] - VY[3];
-    VY13 = VY13 + TMP2B;
-    VY24 = VY24 + TMP2B;
-    EXY = EXY + PX1 * VY13 + PX2 * VY24;
-    EYY = PY1 * VY13 + PY2 * VY24;
-  }
-  // ---- CCURV3 ----
-  T RX[4], RY[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const T a = dead ? ZERO : vrg[0][c], b = dead ? ZERO : vrg[1][c], d = dead ? ZERO : vrg[2][c];
-    RX[c] = E1X * a + E1Y * b + E1Z * d;
-    RY[c] = E2X * a + E2Y * b + E2Z * d;
-  }
-  T KXX, KYY, KXY;
-  {
-    const T RX13TA = RX[0] - RX[2];
-    const T RXAVTA = RX[0] + RX[1] + RX[2] + RX[3];
-    const T RX24TA = RX[1] - RX[3];
-    KYY = -PY1 * RX13TA - PY2 * RX24TA;
-    KXY = PX1 * RX13TA + PX2 * RX24TA;
-    const T RY13TA = RY[0] - RY[2];
-    const T RYAVTA = RY[0] + RY[1] + RY[2] + RY[3];
-    const T RY24TA = RY[1] - RY[3];
-    KXX = PX1 * RY13TA + PX2 * RY24TA;
-    KXY = PY1 * RY13TA + PY2 * RY24TA - KXY;
-    EXZ = EXZ + RYAVTA * (FOURTH * AREA);
-    EYZ = EYZ - RXAVTA * (FOURTH * AREA);
   }
   // ---- CSTRA3 ----
   {

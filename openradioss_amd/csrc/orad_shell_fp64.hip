@@ -16,7 +16,7 @@ hipError_t launch_shell_fp64(const ShellArgs<double> &a, int e_begin, int e_end,
   const int n = e_end - e_begin;
   if (n <= 0) return hipSuccess;
   const int nb = (n + ORAD_SHELL_BLOCK - 1) / ORAD_SHELL_BLOCK;
-  hipLaunchKernelGGL((shell_forces_kernel<double, false, ORAD_SHELL_BLOCK>), dim3(nb),
+  hipLaunchKernelGGL((shell_forces_kernel<double, double, false, ORAD_SHELL_BLOCK>), dim3(nb),
                      dim3(ORAD_SHELL_BLOCK), 0, s, a, e_begin, e_end, dt1p, dt2t, ehou_part,
                      xcd_remap);
   return hipGetLastError();

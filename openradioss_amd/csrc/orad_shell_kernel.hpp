@@ -36,7 +36,8 @@ struct ShellIO {
 };
 
 // REL: coordinates and translational velocities relative to corner 1 (mixed form).
-template <typename T, bool REL, int BLOCK>
+// TG: precision of the frame and kinematics stage (shell_kinematics).
+template <typename T, typename TG, bool REL, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void shell_forces_kernel(ShellArgs<T> a, int e_begin, int e_end,
                                                              const double *dt1p,
                                                              unsigned long long *dt2t_out,
@@ -46,27 +47,27 @@ __global__ __launch_bounds__(BLOCK) void shell_forces_kernel(ShellArgs<T> a, int
   const int nb = gridDim.x;
   const int b = xcd_remap ? xcd_block(blockIdx.x, nb) : (int)blockIdx.x;
   const int e = e_begin + b * BLOCK + (int)threadIdx.x;
-  const T dt1 = T(*dt1p);
+  const double dt1 = *dt1p;
   double my_dt = 1.0e300, my_eh = 0.0;
   if (e < e_end) {
     const int ne = a.numelc;
     int nd[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) nd[c] = ldnt(&a.conn[(size_t)c * ne + e]);
-    T xg[3][4], vg[3][4], vrg[3][4];
+    TG xg[3][4], vg[3][4], vrg[3][4];
     {
       const double4 x0 = a.X4[nd[0]], v0 = a.V4[nd[0]];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const double4 xc = a.X4[nd[c]], vc = a.V4[nd[c]], wc = a.VR4[nd[c]];
         if (REL) {
-          xg[0][c] = T(xc.x - x0.x); xg[1][c] = T(xc.y - x0.y); xg[2][c] = T(xc.z - x0.z);
-          vg[0][c] = T(vc.x - v0.x); vg[1][c] = T(vc.y - v0.y); vg[2][c] = T(vc.z - v0.z);
+          xg[0][c] = TG(xc.x - x0.x); xg[1][c] = TG(xc.y - x0.y); xg[2][c] = TG(xc.z - x0.z);
+          vg[0][c] = TG(vc.x - v0.x); vg[1][c] = TG(vc.y - v0.y); vg[2][c] = TG(vc.z - v0.z);
         } else {
-          xg[0][c] = T(xc.x); xg[1][c] = T(xc.y); xg[2][c] = T(xc.z);
-          vg[0][c] = T(vc.x); vg[1][c] = T(vc.y); vg[2][c] = T(vc.z);
+          xg[0][c] = TG(xc.x); xg[1][c] = TG(xc.y); xg[2][c] = TG(xc.z);
+          vg[0][c] = TG(vc.x); vg[1][c] = TG(vc.y); vg[2][c] = TG(vc.z);
         }
-        vrg[0][c] = T(wc.x); vrg[1][c] = T(wc.y); vrg[2][c] = T(wc.z);
+        vrg[0][c] = TG(wc.x); vrg[1][c] = TG(wc.y); vrg[2][c] = TG(wc.z);
       }
     }
     const ShellIO<T> io{a.st, e, ne};
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(BLOCK) void shell_forces_kernel(ShellArgs<T> a, int
     ShellOut<T> o;
     Slot6<T> *fsky = a.fsky;
     const int32_t *iadc = a.iadc;
-    shell_chain<T>(xg, vg, vrg, thk0, nd[2] == nd[3], io, dt1, a.mat, a.prop, a.k, o,
+    shell_chain<T, TG>(xg, vg, vrg, thk0, nd[2] == nd[3], io, dt1, a.mat, a.prop, a.k, o,
                    [&](int c, T fx, T fy, T fz, T mx, T my, T mz) {
                      Slot6<T> f;
                      f.fx = fx; f.fy = fy; f.fz = fz; f.mx = mx; f.my = my; f.mz = mz;

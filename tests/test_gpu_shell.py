@@ -9,7 +9,7 @@ Tolerances:
   * FP64 whole cycles vs the FP64 oracle: 1e-8 relative on nodes and state after
     hundreds of cycles, time within 1e-12;
   * mixed form vs FP64 form on a 64 x 64 plate impact after 1000 cycles: nodal force
-    and moment relative L2 error < 2e-4, displacement < 1e-5;
+    and moment relative L2 error < 1e-4, displacement < 1e-6;
   * run-to-run: bitwise identical.
 """
 import os
@@ -199,8 +199,8 @@ def test_shell_mixed_vs_fp64_plate_impact():
     ef, em = rel_l2(a32, a64), rel_l2(ar32, ar64)
     ed = rel_l2(n32["d"], n64["d"])
     print(f"mixed vs FP64 after 1000 cycles: force {ef:.2e} moment {em:.2e} displacement {ed:.2e}")
-    # measured on MI355X: force 7.1e-5, moment 8.5e-5, displacement 1.7e-7
-    assert ef < 2e-4 and em < 2e-4 and ed < 1e-5
+    # measured on MI355X: force 2.2e-5, moment 2.0e-5, displacement 6.6e-8
+    assert ef < 1e-4 and em < 1e-4 and ed < 1e-6
 
 
 @pytest.mark.parametrize("prec", [en.FP64, en.MIXED])
