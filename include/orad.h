@@ -36,6 +36,16 @@ typedef struct {
   int ncycle, mstop;
 } orad_time_t;
 
+/* Launch configuration of the element kernels (orad_set_launch, orad_autotune).
+ * persistent: batched shells only -- a resident grid takes 256-element tiles from
+ * a counter instead of one workgroup per tile; lds_kb: dynamic LDS reserved per
+ * element workgroup, which caps resident workgroups per CU (-1 = the form's
+ * default; bricks and batched shells); xcd_remap: give each of the 8 XCDs a
+ * contiguous run of workgroups; grid: the persistent grid size (output). */
+typedef struct {
+  int persistent, lds_kb, xcd_remap, grid;
+} orad_launch_t;
+
 /* Create/destroy a context on one HIP device.  No reference counterpart: the
  * reference keeps these arrays in Fortran modules (RESOL locals, ELBUF_TAB). */
 int orad_create(orad_ctx **ctx, int device, int precision);
@@ -167,6 +177,30 @@ int orad_get_nodes_rot(orad_ctx *ctx, double *vr, double *ar);
  * PLA(1..5), point EPSD(1..5) (GBUF/LBUF of elbufdef_mod.F90). */
 int orad_get_shell_state(orad_ctx *ctx, double *rows);
 int orad_set_shell_state(orad_ctx *ctx, const double *rows);
+
+/* Batched independent shell models in one context (no reference counterpart: a
+ * reference run is one model; the nearest analogue is its dynamic schedule over
+ * uneven element groups, forint.F:325, forintc.F:140-160).  Model m owns nodes
+ * node_off[m]..node_off[m+1]-1 and shells elem_off[m]..elem_off[m+1]-1 (0-based,
+ * offsets span the whole mesh; a shell may only use nodes of its own model) and
+ * has its own law-2 record p[16*m..], ip[4*m..] (layout of orad_set_shell_law2;
+ * null = the context's material for all) and its own time scalars and time step.
+ * Every cycle runs all models in one element launch over 256-element tiles and one
+ * nodal launch; results per model are bitwise those of the model run alone. */
+int orad_set_shell_models(orad_ctx *ctx, int nmodel, const int *node_off, const int *elem_off,
+                          const double *p, const int *ip);
+int orad_get_model_time(orad_ctx *ctx, int model, orad_time_t *t);
+/* Energies of one model; orad_energies sums all models. */
+int orad_model_energies(orad_ctx *ctx, int model, double *eint, double *ekin, double *ehour);
+
+/* Launch configuration (no reference counterpart; the reference fixes its group
+ * size NVSIZ per architecture, archloops.inc:32-64).  Results do not depend on it. */
+int orad_set_launch(orad_ctx *ctx, const orad_launch_t *l);
+int orad_get_launch(orad_ctx *ctx, orad_launch_t *l);
+/* Times `cycles` whole cycles under each candidate configuration (LDS residency
+ * cap, XCD mapping, persistent tiles) from the current state, restores the state
+ * after every trial, and keeps the fastest; *best_ms = its time per cycle. */
+int orad_autotune(orad_ctx *ctx, int cycles, orad_launch_t *best, double *best_ms);
 
 #ifdef __cplusplus
 }

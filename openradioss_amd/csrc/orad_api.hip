@@ -66,7 +66,28 @@ static NodeArgs node_args(orad_ctx *c) {
   a.AR = c->AR;
   a.STIFR = c->STIFR;
   a.fv_val = c->fv_val;
+  if (c->batched) {
+    a.node_model = c->node_model;
+    a.model_node0 = c->model_node0;
+    a.tile_queue = c->queue;
+    a.ehou_part = nullptr;       // the batched element kernel keeps the per-model totals
+    a.n_ehou_part = 0;
+  }
   return a;
+}
+
+static BatchArgs batch_args(orad_ctx *c) {
+  BatchArgs b{};
+  b.tiles = c->tiles;
+  b.ntiles = c->ntiles;
+  b.mats = c->mats;
+  b.ctl = c->ctl;
+  b.queue = c->queue;
+  b.done = c->done;
+  b.model_tiles = c->model_tiles;
+  b.model_chunks = c->model_tiles;
+  b.chunk_ehou = c->chunk_ehou;
+  return b;
 }
 
 template <typename T>
@@ -108,9 +129,21 @@ static hipError_t launch_elements(orad_ctx *c, int e0, int e1, const double *dt1
 // One cycle's launches at parity p (no host synchronisation, graph-capturable).
 static int nelem(const orad_ctx *c) { return c->family == FAM_SHELL ? c->numelc : c->numels; }
 
+// The element launch of a whole cycle at parity p.
+static hipError_t launch_cycle_elements(orad_ctx *c, int p) {
+  if (c->batched) {
+    const int grid = c->persist ? std::min(c->batch_grid, c->ntiles) : c->ntiles;
+    if (c->precision == ORAD_MIXED)
+      return launch_shell_batch_mixed(shell_args<float>(c), batch_args(c), p, c->persist, grid,
+                                      c->batch_lds, c->xcd_remap, c->stream);
+    return launch_shell_batch_fp64(shell_args<double>(c), batch_args(c), p, c->persist, grid,
+                                   c->batch_lds, c->xcd_remap, c->stream);
+  }
+  return launch_elements(c, 0, nelem(c), &c->ctl->s[p].dt2, &c->ctl->dt2t[p], c->ehou_part);
+}
+
 static hipError_t enqueue_cycle(orad_ctx *c, int p) {
-  hipError_t e = launch_elements(c, 0, nelem(c), &c->ctl->s[p].dt2, &c->ctl->dt2t[p],
-                                 c->ehou_part);
+  hipError_t e = launch_cycle_elements(c, p);
   if (e != hipSuccess) return e;
   return launch_node(node_args(c), c->precision == ORAD_MIXED, p, 1, 0, c->xcd_remap, c->stream);
 }
@@ -186,7 +219,9 @@ int orad_destroy(orad_ctx *c) {
                   c->fsky, c->sti, (void *)c->slot_elem,
                   (void *)c->ehou_part, (void *)c->ctl, (void *)c->scratch_dt1,
                   (void *)c->scratch_dt2t, (void *)c->VR4, (void *)c->AR, (void *)c->STIFR,
-                  (void *)c->thke, (void *)c->fv_val})
+                  (void *)c->thke, (void *)c->fv_val, (void *)c->node_model,
+                  (void *)c->model_node0, (void *)c->model_tiles, (void *)c->tiles,
+                  (void *)c->mats, (void *)c->queue, (void *)c->done, (void *)c->chunk_ehou})
     dfree(p);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -353,23 +388,29 @@ int orad_set_time(orad_ctx *c, const orad_time_t *t) {
   const int p = c->host_ncycle & 1;
   Scal s{t->tt, t->dt1, t->dt2, t->dt12, t->dt2old, t->dtmx, t->ncycle, t->mstop};
   unsigned long long inf = 0x7FF0000000000000ull;
-  HIPCHK(c, hipMemcpy(&c->ctl->s[p], &s, sizeof(s), hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(&c->ctl->dt2t[p], &inf, sizeof(inf), hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(&c->ctl->mstop, &t->mstop, sizeof(int), hipMemcpyHostToDevice));
+  for (int m = 0; m < c->nmodel; ++m) {
+    HIPCHK(c, hipMemcpy(&c->ctl[m].s[p], &s, sizeof(s), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(&c->ctl[m].dt2t[p], &inf, sizeof(inf), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(&c->ctl[m].mstop, &t->mstop, sizeof(int), hipMemcpyHostToDevice));
+  }
+  return ORAD_OK;
+}
+
+static int model_time(orad_ctx *c, int m, orad_time_t *t) {
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  Scal s;
+  int ms = 0;
+  HIPCHK(c, hipMemcpy(&s, &c->ctl[m].s[c->host_ncycle & 1], sizeof(s), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(&ms, &c->ctl[m].mstop, sizeof(int), hipMemcpyDeviceToHost));
+  t->tt = s.tt; t->dt1 = s.dt1; t->dt2 = s.dt2; t->dt12 = s.dt12; t->dt2old = s.dt2old;
+  t->dtmx = s.dtmx; t->ncycle = s.ncycle; t->mstop = ms;
   return ORAD_OK;
 }
 
 int orad_get_time(orad_ctx *c, orad_time_t *t) {
   if (!c || !t) return ORAD_EINVAL;
-  HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  Scal s;
-  int ms = 0;
-  HIPCHK(c, hipMemcpy(&s, &c->ctl->s[c->host_ncycle & 1], sizeof(s), hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(&ms, &c->ctl->mstop, sizeof(int), hipMemcpyDeviceToHost));
-  t->tt = s.tt; t->dt1 = s.dt1; t->dt2 = s.dt2; t->dt12 = s.dt12; t->dt2old = s.dt2old;
-  t->dtmx = s.dtmx; t->ncycle = s.ncycle; t->mstop = ms;
-  return ORAD_OK;
+  return model_time(c, 0, t);
 }
 
 static int ready(orad_ctx *c) {
@@ -474,7 +515,7 @@ int orad_profile_cycles(orad_ctx *c, int ncycle, double *elem_ms, double *node_m
   for (int i = 0; i < ncycle; ++i) {
     const int p = c->host_ncycle & 1;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(c, launch_elements(c, 0, nelem(c), &c->ctl->s[p].dt2, &c->ctl->dt2t[p], c->ehou_part));
+    HIPCHK(c, launch_cycle_elements(c, p));
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHK(c, launch_node(node_args(c), c->precision == ORAD_MIXED, p, 1, 0, c->xcd_remap, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
@@ -596,32 +637,44 @@ int orad_get_fsky(orad_ctx *c, double *fsky) {
   return ORAD_OK;
 }
 
-int orad_energies(orad_ctx *c, double *eint, double *ekin, double *ehour) {
-  if (!c) return ORAD_EINVAL;
-  int r = ready(c);
-  if (r) return r;
+// Internal energy of elements [e0, e1), kinetic energy of nodes [n0, n1) and the
+// hourglass energy totals of models [m0, m1).
+static int energies_range(orad_ctx *c, size_t e0, size_t e1, size_t n0, size_t n1, int m0,
+                          int m1, double *eint, double *ekin, double *ehour) {
   HIPCHK(c, hipSetDevice(c->device));
   const bool shell = c->family == FAM_SHELL;
-  const size_t ne = (size_t)nelem(c), ts = c->tsize();
-  const size_t nbe = (ne + 255) / 256, nbk = ((size_t)c->numnod + 255) / 256;
+  const size_t ne = (size_t)nelem(c), ts = c->tsize(), me = e1 - e0, mn = n1 - n0;
+  const size_t nbe = (me + 255) / 256, nbk = (mn + 255) / 256;
   double *part = nullptr;
   HIPCHK(c, dalloc(&part, (nbe + nbk + 2) * sizeof(double)));
   const char *st = (const char *)c->state;
-  hipError_t e = launch_energy(c->precision == ORAD_MIXED, (int)ne,
-                               shell ? st + SC_EINT * ne * ts : st + C_EINT * ne * ts,
-                               shell ? st + (SC_EINT + 1) * ne * ts : nullptr,
-                               shell ? nullptr : c->vol0, c->numnod, c->X4, c->V4,
-                               shell ? c->VR4 : nullptr, part, part + nbe + nbk, c->stream);
+  hipError_t e = launch_energy(c->precision == ORAD_MIXED, (int)me,
+                               shell ? st + (SC_EINT * ne + e0) * ts : st + (C_EINT * ne + e0) * ts,
+                               shell ? st + ((SC_EINT + 1) * ne + e0) * ts : nullptr,
+                               shell ? nullptr : c->vol0 + e0, (int)mn, c->X4 + n0, c->V4 + n0,
+                               shell ? c->VR4 + n0 : nullptr, part, part + nbe + nbk, c->stream);
   double res[2] = {0, 0}, eh = 0;
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) e = hipMemcpy(res, part + nbe + nbk, 2 * sizeof(double), hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(&eh, &c->ctl->ehour, sizeof(double), hipMemcpyDeviceToHost);
+  for (int m = m0; m < m1 && e == hipSuccess; ++m) {
+    double h = 0.0;
+    e = hipMemcpy(&h, &c->ctl[m].ehour, sizeof(double), hipMemcpyDeviceToHost);
+    eh += h;
+  }
   dfree(part);
   if (e != hipSuccess) return fail(c, ORAD_EHIP, std::string("energies: ") + hipGetErrorString(e));
   if (eint) *eint = res[0];
   if (ekin) *ekin = res[1];
   if (ehour) *ehour = eh;
   return ORAD_OK;
+}
+
+int orad_energies(orad_ctx *c, double *eint, double *ekin, double *ehour) {
+  if (!c) return ORAD_EINVAL;
+  int r = ready(c);
+  if (r) return r;
+  return energies_range(c, 0, (size_t)nelem(c), 0, (size_t)c->numnod, 0, c->nmodel, eint, ekin,
+                        ehour);
 }
 
 }  // extern "C"
@@ -848,22 +901,40 @@ int orad_set_shells(orad_ctx *c, int numelc, const int *ixc, const double *thke)
   return init_shell_state(c);
 }
 
-int orad_set_shell_law2(orad_ctx *c, const double *p, const int *ip) {
-  if (!c) return ORAD_EINVAL;
-  if (!p || !ip) return fail(c, ORAD_EINVAL, "set_shell_law2: null parameters");
-  ShMat m{};
+}  // extern "C"
+
+// One law-2 parameter record (the layout orad_set_shell_law2 documents); returns
+// an empty string when valid, else the reason.
+static std::string parse_shell_law2(const double *p, const int *ip, ShMat &m) {
+  m = ShMat{};
   m.rho0 = p[0]; m.young = p[1]; m.nu = p[2]; m.shear = p[3]; m.a11 = p[4]; m.a12 = p[5];
   m.ssp = p[6]; m.ca = p[7]; m.cb = p[8]; m.cn = p[9]; m.epmx = p[10]; m.ymax = p[11];
   m.cc = p[12]; m.epdr = p[13]; m.mexp = p[14]; m.asrate_pm9 = p[15];
   m.icc = ip[0]; m.vp = ip[1]; m.israte = ip[2]; m.ipla = ip[3];
   if (!(m.rho0 > 0.0) || !(m.young > 0.0) || !(m.shear > 0.0) || !(m.ssp > 0.0))
-    return fail(c, ORAD_EINVAL, "set_shell_law2: rho0, E, G and SSP must be positive");
-  if (!(m.ca > 0.0)) return fail(c, ORAD_EINVAL, "set_shell_law2: A must be positive");
-  if (m.vp < 1 || m.vp > 3 || m.ipla < 0 || m.ipla > 2)
-    return fail(c, ORAD_EINVAL, "set_shell_law2: VP must be 1..3 and IPLA 0..2");
-  if (m.cc != 0.0 && !(m.epdr > 0.0)) return fail(c, ORAD_EINVAL, "set_shell_law2: EPS0 must be > 0");
+    return "rho0, E, G and SSP must be positive";
+  if (!(m.ca > 0.0)) return "A must be positive";
+  if (m.vp < 1 || m.vp > 3 || m.ipla < 0 || m.ipla > 2) return "VP must be 1..3 and IPLA 0..2";
+  if (m.cc != 0.0 && !(m.epdr > 0.0)) return "EPS0 must be > 0";
+  return std::string();
+}
+
+extern "C" {
+
+int orad_set_shell_law2(orad_ctx *c, const double *p, const int *ip) {
+  if (!c) return ORAD_EINVAL;
+  if (!p || !ip) return fail(c, ORAD_EINVAL, "set_shell_law2: null parameters");
+  ShMat m;
+  const std::string why = parse_shell_law2(p, ip, m);
+  if (!why.empty()) return fail(c, ORAD_EINVAL, "set_shell_law2: " + why);
   c->shmat = m;
   c->have_shmat = true;
+  if (c->batched) {
+    // one material for every model
+    for (auto &x : c->h_mats) x = m;
+    HIPCHK(c, hipMemcpy(c->mats, c->h_mats.data(), c->h_mats.size() * sizeof(ShMat),
+                        hipMemcpyHostToDevice));
+  }
   drop_graphs(c);
   return ORAD_OK;
 }
@@ -897,7 +968,18 @@ int orad_cforc3(orad_ctx *c, int nft, int nel, double dt1, double *dt2t) {
   unsigned long long bits = 0x7FF0000000000000ull;
   HIPCHK(c, hipMemcpyAsync(c->scratch_dt1, &dt1, sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->scratch_dt2t, &bits, sizeof(bits), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, launch_elements(c, nft, nft + nel, c->scratch_dt1, c->scratch_dt2t, nullptr));
+  const ShMat keep = c->shmat;
+  if (c->batched && nel > 0) {
+    // a group of batched models lies within one model and takes that model's material
+    const int m = (int)(std::upper_bound(c->h_elem_off.begin(), c->h_elem_off.end(), nft) -
+                        c->h_elem_off.begin()) - 1;
+    if (nft + nel > c->h_elem_off[m + 1])
+      return fail(c, ORAD_EINVAL, "cforc3: a group of batched models must lie in one model");
+    c->shmat = c->h_mats[m];
+  }
+  hipError_t le = launch_elements(c, nft, nft + nel, c->scratch_dt1, c->scratch_dt2t, nullptr);
+  c->shmat = keep;
+  HIPCHK(c, le);
   HIPCHK(c, hipMemcpyAsync(&bits, c->scratch_dt2t, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double g;
@@ -967,6 +1049,258 @@ int orad_set_shell_state(orad_ctx *c, const double *rows) {
     }
   HIPCHK(c, hipMemcpy(c->state, h.data(), h.size(), hipMemcpyHostToDevice));
   c->state_init = true;
+  return ORAD_OK;
+}
+
+}  // extern "C"
+
+// ======================================================= batched models ===
+
+static int batch_occupancy(orad_ctx *c) {
+  return c->precision == ORAD_MIXED ? shell_batch_occupancy_mixed(c->batch_lds)
+                                    : shell_batch_occupancy_fp64(c->batch_lds);
+}
+
+static int update_batch_grid(orad_ctx *c) {
+  if (!c->batched) return ORAD_OK;
+  if (c->num_cu <= 0)
+    HIPCHK(c, hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, c->device));
+  const int occ = batch_occupancy(c);
+  if (occ <= 0)
+    return fail(c, ORAD_EINVAL, "launch: the batched kernel does not fit at this LDS reservation");
+  c->batch_grid = occ * c->num_cu;
+  return ORAD_OK;
+}
+
+extern "C" {
+
+int orad_set_shell_models(orad_ctx *c, int nmodel, const int *node_off, const int *elem_off,
+                          const double *p, const int *ip) {
+  if (!c) return ORAD_EINVAL;
+  if (c->family != FAM_SHELL || !c->have_shmat)
+    return fail(c, ORAD_ESTATE, "set_shell_models: set the shells and a law-2 material first");
+  if (c->batched) return fail(c, ORAD_ESTATE, "set_shell_models: models already set");
+  if (nmodel < 1 || !node_off || !elem_off || (p == nullptr) != (ip == nullptr))
+    return fail(c, ORAD_EINVAL, "set_shell_models: bad arguments");
+  if (node_off[0] != 0 || elem_off[0] != 0 || node_off[nmodel] != c->numnod ||
+      elem_off[nmodel] != c->numelc)
+    return fail(c, ORAD_EINVAL, "set_shell_models: offsets must span all nodes and shells");
+  for (int m = 0; m < nmodel; ++m)
+    if (node_off[m + 1] <= node_off[m] || elem_off[m + 1] < elem_off[m])
+      return fail(c, ORAD_EINVAL, "set_shell_models: offsets must increase (a model has nodes)");
+  std::vector<ShMat> mats(nmodel, c->shmat);
+  if (p)
+    for (int m = 0; m < nmodel; ++m) {
+      const std::string why = parse_shell_law2(p + 16 * (size_t)m, ip + 4 * (size_t)m, mats[m]);
+      if (!why.empty())
+        return fail(c, ORAD_EINVAL, "set_shell_models: model " + std::to_string(m) + ": " + why);
+    }
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t ne = (size_t)c->numelc, nn = (size_t)c->numnod;
+  // every shell's corners belong to its own model
+  std::vector<int32_t> conn(4 * ne);
+  HIPCHK(c, hipMemcpy(conn.data(), c->conn, 4 * ne * 4, hipMemcpyDeviceToHost));
+  std::vector<int32_t> nmod(nn), node0(nmodel), mt(nmodel + 1, 0);
+  std::vector<int4> tiles;
+  for (int m = 0; m < nmodel; ++m) {
+    for (int n = node_off[m]; n < node_off[m + 1]; ++n) nmod[n] = m;
+    node0[m] = node_off[m];
+  }
+  for (int m = 0; m < nmodel; ++m) {
+    for (int e = elem_off[m]; e < elem_off[m + 1]; ++e)
+      for (int k = 0; k < 4; ++k)
+        if (nmod[conn[(size_t)k * ne + e]] != m)
+          return fail(c, ORAD_EINVAL, "set_shell_models: shell " + std::to_string(e + 1) +
+                                          " uses a node of another model");
+    mt[m] = (int)tiles.size();
+    for (int e = elem_off[m]; e < elem_off[m + 1]; e += 256)
+      tiles.push_back(make_int4(m, e, std::min(e + 256, elem_off[m + 1]), (int)tiles.size()));
+  }
+  mt[nmodel] = (int)tiles.size();
+  // per-model time control, each starting from the context's current scalars
+  CycleCtl h0{};
+  HIPCHK(c, hipMemcpy(&h0, c->ctl, sizeof(CycleCtl), hipMemcpyDeviceToHost));
+  std::vector<CycleCtl> hc(nmodel, h0);
+  CycleCtl *ctl = nullptr;
+  HIPCHK(c, dalloc(&ctl, nmodel * sizeof(CycleCtl)));
+  HIPCHK(c, hipMemcpy(ctl, hc.data(), nmodel * sizeof(CycleCtl), hipMemcpyHostToDevice));
+  dfree(c->ctl);
+  c->ctl = ctl;
+  HIPCHK(c, dalloc(&c->node_model, nn * 4));
+  HIPCHK(c, dalloc(&c->model_node0, nmodel * 4));
+  HIPCHK(c, dalloc(&c->model_tiles, (nmodel + 1) * 4));
+  HIPCHK(c, dalloc(&c->tiles, tiles.size() * sizeof(int4)));
+  HIPCHK(c, dalloc(&c->mats, nmodel * sizeof(ShMat)));
+  HIPCHK(c, dalloc(&c->queue, sizeof(unsigned)));
+  HIPCHK(c, dalloc(&c->done, nmodel * sizeof(unsigned)));
+  HIPCHK(c, dalloc(&c->chunk_ehou, tiles.size() * sizeof(double)));
+  HIPCHK(c, hipMemcpy(c->node_model, nmod.data(), nn * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->model_node0, node0.data(), nmodel * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->model_tiles, mt.data(), (nmodel + 1) * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->mats, mats.data(), nmodel * sizeof(ShMat), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemset(c->queue, 0, sizeof(unsigned)));
+  HIPCHK(c, hipMemset(c->done, 0, nmodel * sizeof(unsigned)));
+  HIPCHK(c, hipMemset(c->chunk_ehou, 0, tiles.size() * sizeof(double)));
+  c->nmodel = nmodel;
+  c->h_node_off.assign(node_off, node_off + nmodel + 1);
+  c->h_elem_off.assign(elem_off, elem_off + nmodel + 1);
+  c->h_model_tiles = mt;
+  c->h_mats = mats;
+  c->ntiles = (int)tiles.size();
+  c->batched = true;
+  drop_graphs(c);
+  return update_batch_grid(c);
+}
+
+int orad_get_model_time(orad_ctx *c, int model, orad_time_t *t) {
+  if (!c || !t) return ORAD_EINVAL;
+  if (model < 0 || model >= c->nmodel) return fail(c, ORAD_EINVAL, "get_model_time: no such model");
+  return model_time(c, model, t);
+}
+
+int orad_model_energies(orad_ctx *c, int model, double *eint, double *ekin, double *ehour) {
+  if (!c) return ORAD_EINVAL;
+  int r = ready(c);
+  if (r) return r;
+  if (model < 0 || model >= c->nmodel) return fail(c, ORAD_EINVAL, "model_energies: no such model");
+  if (!c->batched) return orad_energies(c, eint, ekin, ehour);
+  return energies_range(c, c->h_elem_off[model], c->h_elem_off[model + 1], c->h_node_off[model],
+                        c->h_node_off[model + 1], model, model + 1, eint, ekin, ehour);
+}
+
+int orad_get_launch(orad_ctx *c, orad_launch_t *l) {
+  if (!c || !l) return ORAD_EINVAL;
+  l->persistent = c->persist;
+  l->lds_kb = c->batched ? c->batch_lds / 1024 : (c->elem_lds < 0 ? -1 : c->elem_lds / 1024);
+  l->xcd_remap = c->xcd_remap;
+  l->grid = c->batched && c->persist ? std::min(c->batch_grid, c->ntiles) : 0;
+  return ORAD_OK;
+}
+
+int orad_set_launch(orad_ctx *c, const orad_launch_t *l) {
+  if (!c || !l) return ORAD_EINVAL;
+  if (l->lds_kb > 160 || l->lds_kb < -1 || (l->persistent && !c->batched))
+    return fail(c, ORAD_EINVAL, "set_launch: LDS in -1..160 KB; persistent needs batched models");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int keep_p = c->persist, keep_b = c->batch_lds, keep_e = c->elem_lds;
+  c->xcd_remap = l->xcd_remap != 0;
+  if (c->batched) {
+    c->persist = l->persistent != 0;
+    c->batch_lds = l->lds_kb > 0 ? 1024 * l->lds_kb : 0;
+    const int r = update_batch_grid(c);
+    if (r) {
+      c->persist = keep_p;
+      c->batch_lds = keep_b;
+      (void)update_batch_grid(c);
+      return r;
+    }
+  } else {
+    c->elem_lds = l->lds_kb < 0 ? -1 : 1024 * l->lds_kb;
+  }
+  (void)keep_e;
+  drop_graphs(c);
+  return ORAD_OK;
+}
+
+}  // extern "C"
+
+// Device state a cycle changes, saved and restored around autotuner trials.
+struct Snapshot {
+  std::vector<std::pair<void *, void *>> bufs;  // (live, copy)
+  std::vector<size_t> sizes;
+  int host_ncycle = 0;
+  hipError_t take(orad_ctx *c) {
+    host_ncycle = c->host_ncycle;
+    const size_t nn = (size_t)c->numnod, ne = (size_t)nelem(c);
+    const size_t ncol = c->family == FAM_SHELL ? SC_NCOL : NSTATE;
+    std::vector<std::pair<void *, size_t>> live = {
+        {c->X4, nn * 32}, {c->V4, nn * 32}, {c->D, nn * 24}, {c->state, ncol * ne * c->tsize()},
+        {c->sti, ne * c->tsize()}, {c->ctl, c->ctl_bytes()}};
+    if (c->family == FAM_SHELL) live.push_back({c->VR4, nn * 32});
+    for (auto &b : live) {
+      void *p = nullptr;
+      hipError_t e = dalloc(&p, b.second);
+      if (e == hipSuccess) e = hipMemcpy(p, b.first, b.second, hipMemcpyDeviceToDevice);
+      if (e != hipSuccess) {
+        dfree(p);
+        return e;
+      }
+      bufs.push_back({b.first, p});
+      sizes.push_back(b.second);
+    }
+    return hipSuccess;
+  }
+  hipError_t restore(orad_ctx *c) {
+    for (size_t i = 0; i < bufs.size(); ++i) {
+      hipError_t e = hipMemcpy(bufs[i].first, bufs[i].second, sizes[i], hipMemcpyDeviceToDevice);
+      if (e != hipSuccess) return e;
+    }
+    c->host_ncycle = host_ncycle;
+    return hipSuccess;
+  }
+  ~Snapshot() {
+    for (auto &b : bufs) dfree(b.second);
+  }
+};
+
+extern "C" {
+
+int orad_autotune(orad_ctx *c, int cycles, orad_launch_t *best, double *best_ms) {
+  if (!c) return ORAD_EINVAL;
+  int r = ready(c);
+  if (r) return r;
+  if (cycles < 2) return fail(c, ORAD_EINVAL, "autotune: at least 2 timed cycles");
+  cycles += cycles & 1;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<orad_launch_t> cand;
+  auto add = [&](int pers, int lds, int xcd) { cand.push_back(orad_launch_t{pers, lds, xcd, 0}); };
+  if (c->family == FAM_BRICK) {
+    for (int lds : {-1, 0, 40, 54, 80})
+      for (int xcd : {0, 1}) add(0, lds, xcd);
+  } else if (c->batched) {
+    for (int lds : {0, 40, 80}) {
+      for (int xcd : {0, 1}) add(0, lds, xcd);
+      add(1, lds, 0);
+    }
+  } else {
+    for (int xcd : {0, 1}) add(0, -1, xcd);
+  }
+  orad_launch_t keep{};
+  orad_get_launch(c, &keep);
+  Snapshot snap;
+  HIPCHK(c, snap.take(c));
+  double tbest = 1e300;
+  orad_launch_t lbest = keep;
+  for (const orad_launch_t &l : cand) {
+    if (orad_set_launch(c, &l) != ORAD_OK) continue;      // does not fit: skip
+    HIPCHK(c, snap.restore(c));
+    r = orad_run_cycles(c, 2, nullptr);                     // captures the graphs
+    if (r == ORAD_OK) {
+      HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+      r = orad_run_cycles(c, cycles, nullptr);
+      HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+      HIPCHK(c, hipEventSynchronize(c->ev[1]));
+    }
+    if (r == ORAD_ENEGVOL) continue;
+    if (r != ORAD_OK) return r;
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    const double per = ms / cycles;
+    if (per < tbest) {
+      tbest = per;
+      lbest = l;
+    }
+  }
+  HIPCHK(c, snap.restore(c));
+  r = orad_set_launch(c, tbest < 1e300 ? &lbest : &keep);
+  if (r) return r;
+  if (best) orad_get_launch(c, best);
+  if (best_ms) *best_ms = tbest;
+  c->err.clear();
   return ORAD_OK;
 }
 

@@ -141,6 +141,28 @@ struct NodeArgs {
   double *AR;               // angular accelerations (assembled moments in ASSPAR4)
   double *STIFR;
   const double *fv_val;     // FIXVEL value per node, null when none is imposed
+  // batched independent models: ctl[model], node -> model, first node of each model
+  const int32_t *node_model;   // null: one model (ctl[0])
+  const int32_t *model_node0;
+  unsigned *tile_queue;        // persistent element-tile counter, reset here for the next cycle
+};
+
+// Batched models (one launch over uneven models): elements of model m are
+// [elem_off[m], elem_off[m+1]), cut into 256-element tiles from the model's first
+// element (the last one ragged); tiles[t] = (model, first element, end element, t).  Each model has its own material, time scalars and
+// time step; the hourglass energy is summed per chunk in a fixed order (four
+// 64-lane wave sums, left to right) and per model over its chunks, so results do
+// not depend on the tile size, the workgroup size or the launch form.
+struct BatchArgs {
+  const int4 *tiles;
+  int ntiles;
+  const ShMat *mats;            // [nmodel]
+  CycleCtl *ctl;                // [nmodel]
+  unsigned *queue;              // next tile (persistent launch)
+  unsigned *done;               // [nmodel] tiles finished this cycle; reset by the last one
+  const int32_t *model_tiles;   // [nmodel+1]
+  const int32_t *model_chunks;  // [nmodel+1] (tiles are the 256-element chunks)
+  double *chunk_ehou;           // [ntiles]
 };
 
 // Logical block index that gives each XCD a contiguous run of blocks (the

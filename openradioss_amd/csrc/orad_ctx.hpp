@@ -29,6 +29,12 @@ hipError_t launch_shell_mixed(const ShellArgs<float> &, int, int, const double *
                               unsigned long long *, double *, int, hipStream_t);
 int shell_block_fp64();
 int shell_block_mixed();
+hipError_t launch_shell_batch_fp64(const ShellArgs<double> &, const BatchArgs &, int, int, int,
+                                   int, int, hipStream_t);
+hipError_t launch_shell_batch_mixed(const ShellArgs<float> &, const BatchArgs &, int, int, int,
+                                    int, int, hipStream_t);
+int shell_batch_occupancy_fp64(int);
+int shell_batch_occupancy_mixed(int);
 }  // namespace orad
 
 using namespace orad;
@@ -79,12 +85,31 @@ struct orad_ctx {
   double *scratch_dt1 = nullptr;
   unsigned long long *scratch_dt2t = nullptr;
 
+  // batched independent shell models (orad_set_shell_models): per-model material,
+  // time scalars and time step; element tiles of 256 elements within one model
+  int nmodel = 1;
+  bool batched = false;
+  std::vector<int32_t> h_node_off, h_elem_off, h_model_tiles;
+  std::vector<ShMat> h_mats;
+  int32_t *node_model = nullptr, *model_node0 = nullptr, *model_tiles = nullptr;
+  int4 *tiles = nullptr;
+  int ntiles = 0;
+  ShMat *mats = nullptr;
+  unsigned *queue = nullptr, *done = nullptr;
+  double *chunk_ehou = nullptr;
+  // launch configuration of the element kernels (orad_set_launch / orad_autotune)
+  int persist = 0;          // batched shells: persistent grid taking tiles from a counter
+  int batch_lds = 0;        // batched shells: dynamic LDS bytes per workgroup (residency cap)
+  int batch_grid = 0;       // persistent grid size (resident capacity at batch_lds)
+  int num_cu = 0;
+
   int host_ncycle = 0;  // cycle counter mirrored on the host (its parity selects buffers)
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // two cycles starting at parity 0 / 1
   int xcd_remap = 1;
   int elem_lds = -1;        // dynamic LDS bytes per element workgroup; < 0: the form's default
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
+  size_t ctl_bytes() const { return (size_t)nmodel * sizeof(CycleCtl); }
   size_t tsize() const { return precision == ORAD_MIXED ? sizeof(float) : sizeof(double); }
 };
 

@@ -15,7 +15,10 @@
 // divides by the nodal inertia IN (accele.F:95-109) and BCS10 applies ICODR.
 // Workgroup 0 also publishes the next cycle's time scalars (double-buffered by
 // cycle parity), resets the next cycle's dt minimum and adds this cycle's
-// hourglass-energy partials to the running total in a fixed order.
+// hourglass-energy partials to the running total in a fixed order.  With batched
+// models (node_model set) every node reads its own model's scalars, the thread of
+// each model's first node publishes that model's next scalars, and the element
+// kernel accumulates the hourglass energy itself.
 #include "orad_common.hpp"
 
 namespace orad {
@@ -45,7 +48,10 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
   const int n0 = b * NODE_BLOCK;
   const int n = n0 + (int)threadIdx.x;
   const S *fsky = reinterpret_cast<const S *>(a.fsky);
-  CycleCtl *ctl = a.ctl;
+  const bool live = n < a.numnod;
+  // batched models: each node steps with its own model's time scalars
+  const int m = a.node_model ? a.node_model[live ? n : a.numnod - 1] : 0;
+  CycleCtl *ctl = a.ctl + m;
   const Scal sc = ctl->s[parity];
 
   double dt2 = 0.0, dt12 = 0.0;
@@ -58,31 +64,32 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
     dt2 = (lim < dt2) ? lim : dt2;
     dt2 = (sc.dtmx < dt2) ? sc.dtmx : dt2;
     dt12 = 0.5 * (dt1 + dt2);
-    if (b == 0) {
-      if (threadIdx.x == 0) {
-        Scal nx = sc;
-        nx.dt1 = dt1;
-        nx.dt2 = dt2;
-        nx.dt12 = dt12;
-        nx.dt2old = dt2;
-        nx.tt = sc.tt + dt2;
-        nx.ncycle = sc.ncycle + 1;
-        nx.mstop = ctl->mstop;
-        ctl->s[parity ^ 1] = nx;
-        ctl->dt2t[parity ^ 1] = 0x7FF0000000000000ull;
-      }
-      if (a.ehou_part) {
-        __shared__ double s_part[NODE_BLOCK];
-        double acc = 0.0;
-        for (int i = threadIdx.x; i < a.n_ehou_part; i += NODE_BLOCK) acc += a.ehou_part[i];
-        s_part[threadIdx.x] = acc;
+    const bool publisher = a.node_model ? (live && a.model_node0[m] == n)
+                                        : (b == 0 && threadIdx.x == 0);
+    if (publisher) {
+      Scal nx = sc;
+      nx.dt1 = dt1;
+      nx.dt2 = dt2;
+      nx.dt12 = dt12;
+      nx.dt2old = dt2;
+      nx.tt = sc.tt + dt2;
+      nx.ncycle = sc.ncycle + 1;
+      nx.mstop = ctl->mstop;
+      ctl->s[parity ^ 1] = nx;
+      ctl->dt2t[parity ^ 1] = 0x7FF0000000000000ull;
+    }
+    if (a.tile_queue && b == 0 && threadIdx.x == 0) *a.tile_queue = 0u;
+    if (b == 0 && a.ehou_part) {
+      __shared__ double s_part[NODE_BLOCK];
+      double acc = 0.0;
+      for (int i = threadIdx.x; i < a.n_ehou_part; i += NODE_BLOCK) acc += a.ehou_part[i];
+      s_part[threadIdx.x] = acc;
+      __syncthreads();
+      for (int w = NODE_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s_part[threadIdx.x] += s_part[threadIdx.x + w];
         __syncthreads();
-        for (int w = NODE_BLOCK / 2; w > 0; w >>= 1) {
-          if ((int)threadIdx.x < w) s_part[threadIdx.x] += s_part[threadIdx.x + w];
-          __syncthreads();
-        }
-        if (threadIdx.x == 0) ctl->ehour += s_part[0];
       }
+      if (threadIdx.x == 0) ctl->ehour += s_part[0];
     }
   }
 
@@ -91,7 +98,6 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
   // own node's slots left to right, in slot order, across tile boundaries.
   const int n1 = (n0 + NODE_BLOCK < a.numnod) ? n0 + NODE_BLOCK : a.numnod;
   const int s0 = a.adsky[n0], s1 = a.adsky[n1];
-  const bool live = n < a.numnod;
   const int k0 = live ? a.adsky[n] : 0, k1 = live ? a.adsky[n + 1] : 0;
   double acc[NC];
 #pragma unroll

@@ -35,6 +35,26 @@ struct ShellIO {
   __device__ __forceinline__ void put(int c, T v) const { stnt(&st[(size_t)c * ne + e], v); }
 };
 
+// Gather of one element's corners in the kinematics precision TG (REL: relative
+// to corner 1, formed in FP64).
+template <typename T, typename TG, bool REL>
+__device__ __forceinline__ void shell_gather(const ShellArgs<T> &a, const int nd[4], TG xg[3][4],
+                                             TG vg[3][4], TG vrg[3][4]) {
+  const double4 x0 = a.X4[nd[0]], v0 = a.V4[nd[0]];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double4 xc = a.X4[nd[c]], vc = a.V4[nd[c]], wc = a.VR4[nd[c]];
+    if (REL) {
+      xg[0][c] = TG(xc.x - x0.x); xg[1][c] = TG(xc.y - x0.y); xg[2][c] = TG(xc.z - x0.z);
+      vg[0][c] = TG(vc.x - v0.x); vg[1][c] = TG(vc.y - v0.y); vg[2][c] = TG(vc.z - v0.z);
+    } else {
+      xg[0][c] = TG(xc.x); xg[1][c] = TG(xc.y); xg[2][c] = TG(xc.z);
+      vg[0][c] = TG(vc.x); vg[1][c] = TG(vc.y); vg[2][c] = TG(vc.z);
+    }
+    vrg[0][c] = TG(wc.x); vrg[1][c] = TG(wc.y); vrg[2][c] = TG(wc.z);
+  }
+}
+
 // REL: coordinates and translational velocities relative to corner 1 (mixed form).
 // TG: precision of the frame and kinematics stage (shell_kinematics).
 template <typename T, typename TG, bool REL, int BLOCK>
@@ -55,21 +75,7 @@ __global__ __launch_bounds__(BLOCK) void shell_forces_kernel(ShellArgs<T> a, int
 #pragma unroll
     for (int c = 0; c < 4; ++c) nd[c] = ldnt(&a.conn[(size_t)c * ne + e]);
     TG xg[3][4], vg[3][4], vrg[3][4];
-    {
-      const double4 x0 = a.X4[nd[0]], v0 = a.V4[nd[0]];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const double4 xc = a.X4[nd[c]], vc = a.V4[nd[c]], wc = a.VR4[nd[c]];
-        if (REL) {
-          xg[0][c] = TG(xc.x - x0.x); xg[1][c] = TG(xc.y - x0.y); xg[2][c] = TG(xc.z - x0.z);
-          vg[0][c] = TG(vc.x - v0.x); vg[1][c] = TG(vc.y - v0.y); vg[2][c] = TG(vc.z - v0.z);
-        } else {
-          xg[0][c] = TG(xc.x); xg[1][c] = TG(xc.y); xg[2][c] = TG(xc.z);
-          vg[0][c] = TG(vc.x); vg[1][c] = TG(vc.y); vg[2][c] = TG(vc.z);
-        }
-        vrg[0][c] = TG(wc.x); vrg[1][c] = TG(wc.y); vrg[2][c] = TG(wc.z);
-      }
-    }
+    shell_gather<T, TG, REL>(a, nd, xg, vg, vrg);
     const ShellIO<T> io{a.st, e, ne};
     const T thk0 = T(ldnt(&a.thke[e]));
     ShellOut<T> o;
@@ -96,6 +102,125 @@ __global__ __launch_bounds__(BLOCK) void shell_forces_kernel(ShellArgs<T> a, int
     for (int w = 1; w < BLOCK / 64; ++w) { d = (s_dt[w] < d) ? s_dt[w] : d; h += s_eh[w]; }
     if (d < 1.0e300) atomicMin(dt2t_out, (unsigned long long)__double_as_longlong(d));
     if (ehou_part) ehou_part[b] = h;
+  }
+}
+
+// Batched models, one 256-element chunk per tile (BatchArgs); a tile loop over
+// several chunks made the compiler keep the chain's invariants live across
+// iterations (256 registers and one wave per SIMD instead of 225 and two).
+// PERSIST: a grid sized
+// to the resident capacity takes tiles from an atomic counter until none is left
+// (the reference's dynamic group schedule, forint.F:325); otherwise one workgroup
+// per tile.  Per tile: each element's CFORC3 chain with its model's material and
+// DT1, slots written at IADC as in the one-model kernel, the model's DT2T by an
+// atomic minimum, and each chunk's hourglass energy as four 64-lane wave sums
+// added left to right.  The workgroup that finishes a model's last tile adds the
+// model's chunk sums (fixed order, one wave) to that model's running total.
+template <typename T, typename TG, bool REL, int BLOCK, bool PERSIST>
+__global__ __launch_bounds__(BLOCK) void shell_batch_kernel(ShellArgs<T> a, BatchArgs bt,
+                                                            int parity, int xcd_remap) {
+  static_assert(BLOCK == 64 || BLOCK == 128 || BLOCK == 256, "BLOCK divides a 256-element chunk");
+  constexpr int NW = BLOCK / 64, SUB = 256 / BLOCK;
+  __shared__ double s_dt[NW], s_eh[NW];
+  __shared__ int s_tile, s_last;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ne = a.numelc;
+  int pass = 0;
+  for (;;) {
+    int t;
+    if (PERSIST) {
+      if (threadIdx.x == 0) s_tile = (int)atomicAdd(bt.queue, 1u);
+      __syncthreads();
+      t = s_tile;
+      __syncthreads();
+    } else {
+      if (pass > 0) break;
+      t = xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    }
+    ++pass;
+    if (t >= bt.ntiles) break;
+    t = __builtin_amdgcn_readfirstlane(t);
+    const int4 td = bt.tiles[t];
+    const int m = td.x;
+    const ShMat &mat = bt.mats[m];
+    CycleCtl *ctl = bt.ctl + m;
+    const double dt1 = ctl->s[parity].dt2;
+    double my_dt = 1.0e300, ch_eh = 0.0;
+    const int chunk = td.w;
+    {
+      const int c0 = td.y;
+#pragma unroll 1
+      for (int sub = 0; sub < SUB; ++sub) {
+        const int e = c0 + sub * BLOCK + (int)threadIdx.x;
+        double my_eh = 0.0;
+        if (e < td.z) {
+          int nd[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) nd[c] = ldnt(&a.conn[(size_t)c * ne + e]);
+          TG xg[3][4], vg[3][4], vrg[3][4];
+          shell_gather<T, TG, REL>(a, nd, xg, vg, vrg);
+          const ShellIO<T> io{a.st, e, ne};
+          const T thk0 = T(ldnt(&a.thke[e]));
+          ShellOut<T> o;
+          Slot6<T> *fsky = a.fsky;
+          const int32_t *iadc = a.iadc;
+          shell_chain<T, TG>(xg, vg, vrg, thk0, nd[2] == nd[3], io, dt1, mat, a.prop,
+                             a.k, o, [&](int c, T fx, T fy, T fz, T mx, T my, T mz) {
+                               Slot6<T> f;
+                               f.fx = fx; f.fy = fy; f.fz = fz; f.mx = mx; f.my = my; f.mz = mz;
+                               fsky[ldnt(&iadc[(size_t)c * ne + e])] = f;
+                             });
+          stnt(&a.sti[e], o.sti);
+          if (o.alive) my_dt = (double(o.dt) < my_dt) ? double(o.dt) : my_dt;
+          my_eh = double(o.ehou);
+        }
+        const double we = shell_wave_sum(my_eh);   // identical in every lane
+        if (NW == 1) {
+          ch_eh = (sub == 0) ? we : ch_eh + we;
+        } else {
+          if (lane == 0) s_eh[wv] = we;
+          __syncthreads();
+          if (threadIdx.x == 0) {
+            double h = (sub == 0) ? s_eh[0] : ch_eh + s_eh[0];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) h += s_eh[w];
+            ch_eh = h;
+          }
+          __syncthreads();
+        }
+      }
+      if (threadIdx.x == 0)
+        __hip_atomic_store(&bt.chunk_ehou[chunk], ch_eh, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const double wd = shell_wave_min(my_dt);
+    if (lane == 0) s_dt[wv] = wd;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double d = s_dt[0];
+#pragma unroll
+      for (int w = 1; w < NW; ++w) d = (s_dt[w] < d) ? s_dt[w] : d;
+      if (d < 1.0e300) atomicMin(&ctl->dt2t[parity], (unsigned long long)__double_as_longlong(d));
+      __threadfence();
+      const unsigned ntm = (unsigned)(bt.model_tiles[m + 1] - bt.model_tiles[m]);
+      s_last = (atomicAdd(&bt.done[m], 1u) == ntm - 1u);
+    }
+    __syncthreads();
+    if (s_last) {
+      __threadfence();
+      if (wv == 0) {
+        const int k0 = bt.model_chunks[m], k1 = bt.model_chunks[m + 1];
+        double h = 0.0;
+        for (int k = k0 + lane; k < k1; k += 64)
+          h += __hip_atomic_load(&bt.chunk_ehou[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        h = shell_wave_sum(h);
+        if (lane == 0) {
+          ctl->ehour += h;
+          __hip_atomic_store(&bt.done[m], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 

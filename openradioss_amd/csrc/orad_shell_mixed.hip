@@ -34,4 +34,30 @@ hipError_t launch_shell_mixed(const ShellArgs<float> &a, int e_begin, int e_end,
   return hipGetLastError();
 }
 
+// Batched models: one workgroup per tile, or (persist) a resident grid taking tiles
+// from the counter; dyn_lds reserves LDS per workgroup to cap residency per CU.
+hipError_t launch_shell_batch_mixed(const ShellArgs<float> &a, const BatchArgs &b, int parity,
+                                    int persist, int grid, int dyn_lds, int xcd_remap,
+                                    hipStream_t s) {
+  if (b.ntiles <= 0) return hipSuccess;
+  const size_t lds = dyn_lds > 0 ? (size_t)dyn_lds : 0;
+  if (persist)
+    hipLaunchKernelGGL((shell_batch_kernel<float, ShellKinT, true, ORAD_SHELL_BLOCK, true>), dim3(grid),
+                       dim3(ORAD_SHELL_BLOCK), lds, s, a, b, parity, xcd_remap);
+  else
+    hipLaunchKernelGGL((shell_batch_kernel<float, ShellKinT, true, ORAD_SHELL_BLOCK, false>),
+                       dim3(b.ntiles), dim3(ORAD_SHELL_BLOCK), lds, s, a, b, parity, xcd_remap);
+  return hipGetLastError();
+}
+
+// Resident workgroups per CU of the persistent batch kernel at dyn_lds bytes.
+int shell_batch_occupancy_mixed(int dyn_lds) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &nb, shell_batch_kernel<float, ShellKinT, true, ORAD_SHELL_BLOCK, true>, ORAD_SHELL_BLOCK,
+          dyn_lds > 0 ? (size_t)dyn_lds : 0) != hipSuccess)
+    return 0;
+  return nb;
+}
+
 }  // namespace orad

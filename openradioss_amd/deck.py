@@ -199,3 +199,80 @@ def shell_plate(n=512, h=1.0, thk=1.0, patch=32, v0=1.5e4, noise=0.1, thk_noise=
     return dict(x=x, v=v, vr=np.zeros_like(x), ms=ms, **{"in": inr}, icodt=icodt,
                 icodr=icodr, ixc=ixc, uid=np.arange(1, ne + 1, dtype=np.int32), thk=t,
                 mat=mat, prop=shell_prop_params(npt=npt))
+
+
+def square_tube(side, length, h, thk, mat, v_imp=-1.0e4, uid0=1):
+    """One square thin-walled tube: cross-section side x side centred on the z axis,
+    z from 0 to length; ceil(side/h) shells per face and ceil(length/h) along the
+    axis.  The z = 0 ring is clamped (six DOF); the z = length ring has an imposed
+    z velocity v_imp (FIXVEL) and starts at it."""
+    ns = int(np.ceil(side / h))
+    nl = int(np.ceil(length / h))
+    npr = 4 * ns                                   # nodes per ring (closed perimeter)
+    i = np.arange(npr)
+    face, t = i // ns, (i % ns) / ns
+    a = 0.5 * side
+    px = np.select([face == 0, face == 1, face == 2, face == 3],
+                   [-a + t * side, np.full(npr, a), a - t * side, np.full(npr, -a)])
+    py = np.select([face == 0, face == 1, face == 2, face == 3],
+                   [np.full(npr, -a), -a + t * side, np.full(npr, a), a - t * side])
+    z = np.arange(nl + 1) * (length / nl)
+    x = np.zeros(((nl + 1) * npr, 3))
+    x[:, 0] = np.tile(px, nl + 1)
+    x[:, 1] = np.tile(py, nl + 1)
+    x[:, 2] = np.repeat(z, npr)
+    jj, ii = np.meshgrid(np.arange(nl), np.arange(npr), indexing="ij")
+    jj, ii = jj.ravel(), ii.ravel()
+    n1 = jj * npr + ii
+    n2 = jj * npr + (ii + 1) % npr
+    ixc = np.stack([n1, n2, n2 + npr, n1 + npr], axis=1).astype(np.int32)
+    ne, nn = ixc.shape[0], x.shape[0]
+    tk = np.full(ne, float(thk))
+    ms, inr, _ = shell_masses(x, ixc, tk, mat["rho0"])
+    icodt = np.zeros(nn, np.int32)
+    icodt[:npr] = 7
+    icodr = icodt.copy()
+    fv_dir = np.zeros(nn, np.int32)
+    fv_val = np.zeros(nn)
+    fv_dir[nl * npr:] = 3
+    fv_val[nl * npr:] = v_imp
+    v = np.zeros_like(x)
+    v[nl * npr:, 2] = v_imp
+    return dict(x=x, v=v, vr=np.zeros_like(x), ms=ms, **{"in": inr}, icodt=icodt, icodr=icodr,
+                ixc=ixc, uid=np.arange(uid0, uid0 + ne, dtype=np.int32), thk=tk,
+                fv_dir=fv_dir, fv_val=fv_val)
+
+
+def shell_tubes(ntube=256, side=(30.0, 80.0), length=(100.0, 400.0), h=(2.0, 5.0),
+                thk=(0.8, 2.5), ca=(200.0, 400.0), v_imp=-1.0e4, seed=20261203, npt=5, ipla=0):
+    """Batch of independent square tubes crushed axially (one model each): side,
+    length, element size, thickness and Johnson-Cook A drawn per tube from seeded
+    uniform distributions (Philox stream keyed on the seed), the other law-2 values
+    as shell_mat_params.  Returns one concatenated deck with node_off / elem_off
+    (model ranges) and mats (one law-2 dict per tube)."""
+    rs = np.random.Generator(np.random.Philox(key=seed))
+    draws = rs.uniform(size=(ntube, 5))
+    lo = np.array([side[0], length[0], h[0], thk[0], ca[0]])
+    hi = np.array([side[1], length[1], h[1], thk[1], ca[1]])
+    par = lo + draws * (hi - lo)
+    parts, mats = [], []
+    node_off, elem_off = [0], [0]
+    uid0 = 1
+    for k in range(ntube):
+        s_, l_, h_, t_, a_ = par[k]
+        mat = shell_mat_params(ca=float(a_), ipla=ipla)
+        d = square_tube(s_, l_, h_, t_, mat, v_imp=v_imp, uid0=uid0)
+        d["ixc"] = d["ixc"] + node_off[-1]
+        uid0 += d["ixc"].shape[0]
+        node_off.append(node_off[-1] + d["x"].shape[0])
+        elem_off.append(elem_off[-1] + d["ixc"].shape[0])
+        parts.append(d)
+        mats.append(mat)
+    cat = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+    cat["node_off"] = np.array(node_off, np.int32)
+    cat["elem_off"] = np.array(elem_off, np.int32)
+    cat["mats"] = mats
+    cat["mat"] = mats[0]
+    cat["prop"] = shell_prop_params(npt=npt)
+    cat["tubes"] = par
+    return cat

@@ -24,7 +24,9 @@ EXPORTS = (
     "orad_get_brick_state", "orad_set_brick_state", "orad_get_fsky", "orad_energies",
     "orad_set_fixvel", "orad_set_nodes_rot", "orad_set_shells", "orad_set_shell_law2",
     "orad_set_shell_prop", "orad_cforc3", "orad_asspar4_rot", "orad_get_nodes_rot",
-    "orad_get_shell_state", "orad_set_shell_state",
+    "orad_get_shell_state", "orad_set_shell_state", "orad_set_shell_models",
+    "orad_get_model_time", "orad_model_energies", "orad_set_launch", "orad_get_launch",
+    "orad_autotune",
 )
 NIXC = 7
 SH_NCOL = 53
@@ -35,6 +37,15 @@ class TimeState(C.Structure):
     _fields_ = [("tt", C.c_double), ("dt1", C.c_double), ("dt2", C.c_double),
                 ("dt12", C.c_double), ("dt2old", C.c_double), ("dtmx", C.c_double),
                 ("ncycle", C.c_int), ("mstop", C.c_int)]
+
+
+class Launch(C.Structure):
+    """orad_launch_t: element-kernel launch configuration."""
+    _fields_ = [("persistent", C.c_int), ("lds_kb", C.c_int), ("xcd_remap", C.c_int),
+                ("grid", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class OradError(RuntimeError):
@@ -85,6 +96,12 @@ def load_library(path=LIB_PATH):
         "orad_get_nodes_rot": ([P, P, P], I),
         "orad_get_shell_state": ([P, P], I),
         "orad_set_shell_state": ([P, P], I),
+        "orad_set_shell_models": ([P, I, P, P, P, P], I),
+        "orad_get_model_time": ([P, I, C.POINTER(TimeState)], I),
+        "orad_model_energies": ([P, I, C.POINTER(D), C.POINTER(D), C.POINTER(D)], I),
+        "orad_set_launch": ([P, C.POINTER(Launch)], I),
+        "orad_get_launch": ([P, C.POINTER(Launch)], I),
+        "orad_autotune": ([P, I, C.POINTER(Launch), C.POINTER(D)], I),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -176,6 +193,22 @@ class BrickSolver:
             pass
 
     # ---- reference-shaped operations ----
+    def set_launch(self, persistent=0, lds_kb=-1, xcd_remap=1):
+        l = Launch(persistent, lds_kb, xcd_remap, 0)
+        self._chk(self.lib.orad_set_launch(self.h, C.byref(l)))
+
+    def launch(self):
+        l = Launch()
+        self._chk(self.lib.orad_get_launch(self.h, C.byref(l)))
+        return l.as_dict()
+
+    def autotune(self, cycles=20):
+        """Time each candidate launch configuration over `cycles` cycles from the
+        current state (restored after each trial); keeps the fastest."""
+        l, ms = Launch(), C.c_double()
+        self._chk(self.lib.orad_autotune(self.h, cycles, C.byref(l), C.byref(ms)))
+        return l.as_dict(), ms.value
+
     def pon(self):
         """(ADSKY(NUMNOD+1), IADS(8,NUMELS)) 1-based, as the starter writes them."""
         adsky = np.zeros(self.numnod + 1, np.int32)
@@ -301,8 +334,40 @@ class ShellSolver:
         ixc = ixc_from_deck(deck)
         thk = _f64(deck["thk"], (self.numelc,))
         self._chk(self.lib.orad_set_shells(h, self.numelc, _p(ixc), _p(thk)))
+        self.nmodel = 1
+        if "node_off" in deck:
+            self.set_models(deck["node_off"], deck["elem_off"], deck.get("mats"))
+
+    def set_models(self, node_off, elem_off, mats=None):
+        """Declare batched independent models (orad_set_shell_models); mats: one
+        law-2 parameter dict per model, or None for the deck's material."""
+        no = np.ascontiguousarray(node_off, dtype=np.int32)
+        eo = np.ascontiguousarray(elem_off, dtype=np.int32)
+        nm = no.size - 1
+        p = ip = None
+        if mats is not None:
+            p = np.ascontiguousarray([[m[k] for k in SHMAT_KEYS] for m in mats], np.float64)
+            ip = np.ascontiguousarray([[m["icc"], m["vp"], m["israte"], m["ipla"]] for m in mats],
+                                      np.int32)
+        self._chk(self.lib.orad_set_shell_models(self.h, nm, _p(no), _p(eo),
+                                                 _p(p) if p is not None else None,
+                                                 _p(ip) if ip is not None else None))
+        self.nmodel = nm
+
+    def model_time(self, m):
+        t = TimeState()
+        self._chk(self.lib.orad_get_model_time(self.h, m, C.byref(t)))
+        return t
+
+    def model_energies(self, m):
+        ei, ek, eh = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self.lib.orad_model_energies(self.h, m, C.byref(ei), C.byref(ek), C.byref(eh)))
+        return ei.value, ek.value, eh.value
 
     _chk = BrickSolver._chk
+    set_launch = BrickSolver.set_launch
+    launch = BrickSolver.launch
+    autotune = BrickSolver.autotune
     close = BrickSolver.close
     __del__ = BrickSolver.__del__
     run_cycles = BrickSolver.run_cycles

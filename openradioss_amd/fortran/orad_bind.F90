@@ -17,6 +17,10 @@ module orad_bind
     integer(c_int) :: ncycle, mstop
   end type orad_time_t
 
+  type, bind(c) :: orad_launch_t
+    integer(c_int) :: persistent, lds_kb, xcd_remap, grid
+  end type orad_launch_t
+
   interface
     integer(c_int) function orad_create(ctx, device, precision) bind(c, name='orad_create')
       import :: c_ptr, c_int
@@ -158,6 +162,45 @@ module orad_bind
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
       real(c_double) :: rows(53,*)
+    end function
+    ! p and ip: c_loc of the per-model law-2 records, or c_null_ptr
+    integer(c_int) function orad_set_shell_models(ctx, nmodel, node_off, elem_off, p, ip) &
+        bind(c, name='orad_set_shell_models')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nmodel
+      integer(c_int) :: node_off(*), elem_off(*)
+      type(c_ptr), value :: p, ip
+    end function
+    integer(c_int) function orad_get_model_time(ctx, model, t) bind(c, name='orad_get_model_time')
+      import :: c_ptr, c_int, orad_time_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: model
+      type(orad_time_t) :: t
+    end function
+    integer(c_int) function orad_model_energies(ctx, model, eint, ekin, ehour) &
+        bind(c, name='orad_model_energies')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: model
+      real(c_double) :: eint, ekin, ehour
+    end function
+    integer(c_int) function orad_set_launch(ctx, l) bind(c, name='orad_set_launch')
+      import :: c_ptr, c_int, orad_launch_t
+      type(c_ptr), value :: ctx
+      type(orad_launch_t) :: l
+    end function
+    integer(c_int) function orad_get_launch(ctx, l) bind(c, name='orad_get_launch')
+      import :: c_ptr, c_int, orad_launch_t
+      type(c_ptr), value :: ctx
+      type(orad_launch_t) :: l
+    end function
+    integer(c_int) function orad_autotune(ctx, cycles, best, best_ms) bind(c, name='orad_autotune')
+      import :: c_ptr, c_int, c_double, orad_launch_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: cycles
+      type(orad_launch_t) :: best
+      real(c_double) :: best_ms
     end function
   end interface
 
