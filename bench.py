@@ -118,6 +118,56 @@ def shell_plate_leg(n, steps, err_cycles):
     return res
 
 
+def tube_batch_leg(ntube, steps, solo_cycles):
+    """256 independent square tubes (one model each, per-tube A, thickness, size and
+    time step), crushed axially at 10 m/s by FIXVEL: all tubes in one batched cycle
+    (autotuned launch), both forms; and, for comparison, the same tubes run one
+    model at a time (one context each, graph-replayed) for solo_cycles cycles."""
+    deck = dk.shell_tubes(ntube=ntube)
+    ne, nn = deck["ixc"].shape[0], deck["x"].shape[0]
+    res = {"workload": f"{ntube} square tubes ({ne} shells, {nn} nodes, "
+                       f"{int(np.diff(deck['elem_off']).min())}-"
+                       f"{int(np.diff(deck['elem_off']).max())} shells each), FIXVEL crush, law 2",
+           "elements": ne, "nodes": nn, "cycles_timed": steps}
+    for prec, name in ((en.MIXED, "mixed"), (en.FP64, "fp64")):
+        s = en.ShellSolver(deck, prec)
+        s.run_cycles(2)
+        best, tune_ms = s.autotune(cycles=20)
+        s.run_cycles(2)
+        t0 = time.perf_counter()
+        s.run_cycles(steps)
+        dt = time.perf_counter() - t0
+        el, nd = s.profile_cycles(20)
+        ts = 4 if prec == en.MIXED else 8
+        eb = shell_elem_bytes(ne, nn, ts)
+        res[name] = dict(value=ne * steps / dt, ms_per_step=1e3 * dt / steps, launch=best,
+                         elem_kernel_ms=el, node_kernel_ms=nd,
+                         elem_kernel_gbs=eb / (el * 1e-3) / 1e9)
+        s.close()
+        print(f"tubes {name}: {res[name]['value']:.4e} el-cyc/s, elem {el:.3f} ms, node {nd:.3f} ms,"
+              f" launch {best}", file=sys.stderr, flush=True)
+    # one model at a time, mixed form
+    no, eo = deck["node_off"], deck["elem_off"]
+    keys = ("x", "v", "vr", "ms", "in", "icodt", "icodr", "fv_dir", "fv_val")
+    total = 0.0
+    for m in range(ntube):
+        md = {k: deck[k][no[m]:no[m + 1]] for k in keys}
+        md.update(ixc=deck["ixc"][eo[m]:eo[m + 1]] - no[m], uid=deck["uid"][eo[m]:eo[m + 1]],
+                  thk=deck["thk"][eo[m]:eo[m + 1]], mat=deck["mats"][m], prop=deck["prop"])
+        s = en.ShellSolver(md, en.MIXED)
+        s.run_cycles(2)
+        t0 = time.perf_counter()
+        s.run_cycles(solo_cycles)
+        total += time.perf_counter() - t0
+        s.close()
+    res["one_model_at_a_time_mixed"] = dict(value=ne * solo_cycles / total,
+                                            cycles=solo_cycles)
+    res["batched_over_one_at_a_time"] = res["mixed"]["value"] / (ne * solo_cycles / total)
+    print(f"tubes one at a time: {ne * solo_cycles / total:.4e} el-cyc/s", file=sys.stderr,
+          flush=True)
+    return res
+
+
 def run(prec, deck, steps, warmup):
     s = en.BrickSolver(deck, prec)
     s.run_cycles(warmup)
@@ -186,6 +236,10 @@ def main():
     ap.add_argument("--shell-n", type=int, default=512)
     ap.add_argument("--shell-steps", type=int, default=1000)
     ap.add_argument("--shell-err-cycles", type=int, default=5000)
+    ap.add_argument("--no-tubes", action="store_true", help="skip the 256-tube batch leg")
+    ap.add_argument("--tubes", type=int, default=256)
+    ap.add_argument("--tube-steps", type=int, default=3000)
+    ap.add_argument("--tube-solo-cycles", type=int, default=200)
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -220,6 +274,9 @@ def main():
     shell = None
     if not args.no_shell:
         shell = shell_plate_leg(args.shell_n, args.shell_steps, args.shell_err_cycles)
+    tubes = None
+    if not args.no_tubes:
+        tubes = tube_batch_leg(args.tubes, args.tube_steps, args.tube_solo_cycles)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(deck, args.cpu_cycles)
@@ -255,6 +312,7 @@ def main():
         "fp64": fp64,
         "mixed_over_fp64": (value / fp64["value"]) if fp64 else None,
         "shell_plate": shell,
+        "tube_batch": tubes,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -70,8 +70,11 @@ static NodeArgs node_args(orad_ctx *c) {
     a.node_model = c->node_model;
     a.model_node0 = c->model_node0;
     a.tile_queue = c->queue;
-    a.ehou_part = nullptr;       // the batched element kernel keeps the per-model totals
+    a.ehou_part = nullptr;       // per-model totals from the chunk sums instead
     a.n_ehou_part = 0;
+    a.nmodel = c->nmodel;
+    a.model_chunks = c->model_tiles;
+    a.chunk_ehou = c->chunk_ehou;
   }
   return a;
 }
@@ -83,9 +86,6 @@ static BatchArgs batch_args(orad_ctx *c) {
   b.mats = c->mats;
   b.ctl = c->ctl;
   b.queue = c->queue;
-  b.done = c->done;
-  b.model_tiles = c->model_tiles;
-  b.model_chunks = c->model_tiles;
   b.chunk_ehou = c->chunk_ehou;
   return b;
 }
@@ -221,7 +221,7 @@ int orad_destroy(orad_ctx *c) {
                   (void *)c->scratch_dt2t, (void *)c->VR4, (void *)c->AR, (void *)c->STIFR,
                   (void *)c->thke, (void *)c->fv_val, (void *)c->node_model,
                   (void *)c->model_node0, (void *)c->model_tiles, (void *)c->tiles,
-                  (void *)c->mats, (void *)c->queue, (void *)c->done, (void *)c->chunk_ehou})
+                  (void *)c->mats, (void *)c->queue, (void *)c->chunk_ehou})
     dfree(p);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1133,7 +1133,6 @@ int orad_set_shell_models(orad_ctx *c, int nmodel, const int *node_off, const in
   HIPCHK(c, dalloc(&c->tiles, tiles.size() * sizeof(int4)));
   HIPCHK(c, dalloc(&c->mats, nmodel * sizeof(ShMat)));
   HIPCHK(c, dalloc(&c->queue, sizeof(unsigned)));
-  HIPCHK(c, dalloc(&c->done, nmodel * sizeof(unsigned)));
   HIPCHK(c, dalloc(&c->chunk_ehou, tiles.size() * sizeof(double)));
   HIPCHK(c, hipMemcpy(c->node_model, nmod.data(), nn * 4, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->model_node0, node0.data(), nmodel * 4, hipMemcpyHostToDevice));
@@ -1141,7 +1140,6 @@ int orad_set_shell_models(orad_ctx *c, int nmodel, const int *node_off, const in
   HIPCHK(c, hipMemcpy(c->tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->mats, mats.data(), nmodel * sizeof(ShMat), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemset(c->queue, 0, sizeof(unsigned)));
-  HIPCHK(c, hipMemset(c->done, 0, nmodel * sizeof(unsigned)));
   HIPCHK(c, hipMemset(c->chunk_ehou, 0, tiles.size() * sizeof(double)));
   c->nmodel = nmodel;
   c->h_node_off.assign(node_off, node_off + nmodel + 1);

@@ -145,23 +145,23 @@ struct NodeArgs {
   const int32_t *node_model;   // null: one model (ctl[0])
   const int32_t *model_node0;
   unsigned *tile_queue;        // persistent element-tile counter, reset here for the next cycle
+  int nmodel;                  // batched: one extra workgroup per model sums its chunk energies
+  const int32_t *model_chunks; // [nmodel+1]
+  const double *chunk_ehou;
 };
 
 // Batched models (one launch over uneven models): elements of model m are
 // [elem_off[m], elem_off[m+1]), cut into 256-element tiles from the model's first
 // element (the last one ragged); tiles[t] = (model, first element, end element, t).  Each model has its own material, time scalars and
 // time step; the hourglass energy is summed per chunk in a fixed order (four
-// 64-lane wave sums, left to right) and per model over its chunks, so results do
-// not depend on the tile size, the workgroup size or the launch form.
+// 64-lane wave sums, left to right) and, by the nodal kernel, per model over its
+// chunks, so results do not depend on the workgroup size or the launch form.
 struct BatchArgs {
   const int4 *tiles;
   int ntiles;
   const ShMat *mats;            // [nmodel]
   CycleCtl *ctl;                // [nmodel]
   unsigned *queue;              // next tile (persistent launch)
-  unsigned *done;               // [nmodel] tiles finished this cycle; reset by the last one
-  const int32_t *model_tiles;   // [nmodel+1]
-  const int32_t *model_chunks;  // [nmodel+1] (tiles are the 256-element chunks)
   double *chunk_ehou;           // [ntiles]
 };
 

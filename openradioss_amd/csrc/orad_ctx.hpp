@@ -95,7 +95,7 @@ struct orad_ctx {
   int4 *tiles = nullptr;
   int ntiles = 0;
   ShMat *mats = nullptr;
-  unsigned *queue = nullptr, *done = nullptr;
+  unsigned *queue = nullptr;
   double *chunk_ehou = nullptr;
   // launch configuration of the element kernels (orad_set_launch / orad_autotune)
   int persist = 0;          // batched shells: persistent grid taking tiles from a counter

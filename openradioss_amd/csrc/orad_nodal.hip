@@ -17,8 +17,8 @@
 // cycle parity), resets the next cycle's dt minimum and adds this cycle's
 // hourglass-energy partials to the running total in a fixed order.  With batched
 // models (node_model set) every node reads its own model's scalars, the thread of
-// each model's first node publishes that model's next scalars, and the element
-// kernel accumulates the hourglass energy itself.
+// each model's first node publishes that model's next scalars, and one extra
+// workgroup per model adds that model's per-chunk hourglass energies to its total.
 #include "orad_common.hpp"
 
 namespace orad {
@@ -45,6 +45,21 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
   __shared__ R t[NC][TILE];
   const int nb = gridDim.x;
   const int b = xcd_remap ? xcd_block(blockIdx.x, nb) : (int)blockIdx.x;
+  const int nbn = (a.numnod + NODE_BLOCK - 1) / NODE_BLOCK;
+  if (b >= nbn) {
+    // batched models: workgroup nbn + m adds model m's chunk hourglass energies to
+    // its total, lanes striding the chunks then a butterfly sum (fixed order)
+    const int m = b - nbn;
+    if (mode == 1 && m < a.nmodel && threadIdx.x < 64) {
+      const int k0 = a.model_chunks[m], k1 = a.model_chunks[m + 1];
+      double h = 0.0;
+      for (int k = k0 + (int)threadIdx.x; k < k1; k += 64) h += a.chunk_ehou[k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o, 64);
+      if (threadIdx.x == 0) a.ctl[m].ehour += h;
+    }
+    return;
+  }
   const int n0 = b * NODE_BLOCK;
   const int n = n0 + (int)threadIdx.x;
   const S *fsky = reinterpret_cast<const S *>(a.fsky);
@@ -251,7 +266,7 @@ __global__ __launch_bounds__(NODE_BLOCK) void energy_final_kernel(const double *
 
 hipError_t launch_node(const NodeArgs &a, bool slot_fp32, int parity, int mode, int write_a,
                        int xcd_remap, hipStream_t s) {
-  const int nb = (a.numnod + NODE_BLOCK - 1) / NODE_BLOCK;
+  const int nb = (a.numnod + NODE_BLOCK - 1) / NODE_BLOCK + (mode == 1 ? a.nmodel : 0);
   if (nb <= 0) return hipSuccess;
   const bool rot = a.VR4 != nullptr;
   if (slot_fp32 && rot)

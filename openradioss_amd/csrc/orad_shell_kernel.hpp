@@ -113,16 +113,18 @@ __global__ __launch_bounds__(BLOCK) void shell_forces_kernel(ShellArgs<T> a, int
 // (the reference's dynamic group schedule, forint.F:325); otherwise one workgroup
 // per tile.  Per tile: each element's CFORC3 chain with its model's material and
 // DT1, slots written at IADC as in the one-model kernel, the model's DT2T by an
-// atomic minimum, and each chunk's hourglass energy as four 64-lane wave sums
-// added left to right.  The workgroup that finishes a model's last tile adds the
-// model's chunk sums (fixed order, one wave) to that model's running total.
+// atomic minimum, and the chunk's hourglass energy as four 64-lane wave sums added
+// left to right (the nodal kernel adds each model's chunks to its total).  No
+// workgroup waits for or reads another's results within the launch, so no
+// device-scope fence is needed (a fence per tile, which writes back the XCD's L2,
+// made the kernel three times slower).
 template <typename T, typename TG, bool REL, int BLOCK, bool PERSIST>
 __global__ __launch_bounds__(BLOCK) void shell_batch_kernel(ShellArgs<T> a, BatchArgs bt,
                                                             int parity, int xcd_remap) {
   static_assert(BLOCK == 64 || BLOCK == 128 || BLOCK == 256, "BLOCK divides a 256-element chunk");
   constexpr int NW = BLOCK / 64, SUB = 256 / BLOCK;
   __shared__ double s_dt[NW], s_eh[NW];
-  __shared__ int s_tile, s_last;
+  __shared__ int s_tile;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ne = a.numelc;
   int pass = 0;
@@ -189,9 +191,7 @@ __global__ __launch_bounds__(BLOCK) void shell_batch_kernel(ShellArgs<T> a, Batc
           __syncthreads();
         }
       }
-      if (threadIdx.x == 0)
-        __hip_atomic_store(&bt.chunk_ehou[chunk], ch_eh, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+      if (threadIdx.x == 0) bt.chunk_ehou[chunk] = ch_eh;
     }
     const double wd = shell_wave_min(my_dt);
     if (lane == 0) s_dt[wv] = wd;
@@ -201,24 +201,6 @@ __global__ __launch_bounds__(BLOCK) void shell_batch_kernel(ShellArgs<T> a, Batc
 #pragma unroll
       for (int w = 1; w < NW; ++w) d = (s_dt[w] < d) ? s_dt[w] : d;
       if (d < 1.0e300) atomicMin(&ctl->dt2t[parity], (unsigned long long)__double_as_longlong(d));
-      __threadfence();
-      const unsigned ntm = (unsigned)(bt.model_tiles[m + 1] - bt.model_tiles[m]);
-      s_last = (atomicAdd(&bt.done[m], 1u) == ntm - 1u);
-    }
-    __syncthreads();
-    if (s_last) {
-      __threadfence();
-      if (wv == 0) {
-        const int k0 = bt.model_chunks[m], k1 = bt.model_chunks[m + 1];
-        double h = 0.0;
-        for (int k = k0 + lane; k < k1; k += 64)
-          h += __hip_atomic_load(&bt.chunk_ehou[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        h = shell_wave_sum(h);
-        if (lane == 0) {
-          ctl->ehour += h;
-          __hip_atomic_store(&bt.done[m], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-      }
     }
     __syncthreads();
   }
