@@ -202,6 +202,26 @@ int orad_get_launch(orad_ctx *ctx, orad_launch_t *l);
  * after every trial, and keeps the fastest; *best_ms = its time per cycle. */
 int orad_autotune(orad_ctx *ctx, int cycles, orad_launch_t *best, double *best_ms);
 
+/* Adjoint of a run of one shell model (no reference counterpart: the reference's
+ * only derivative is the implicit consistent tangent of law 2, m2law.F:487-531,
+ * m2cplr.F:448-469).  orad_adjoint_forward runs ncycle cycles from the current
+ * state as orad_run_cycles does, keeping a checkpoint every `every` cycles.
+ * orad_adjoint_reverse takes the adjoints of the final state -- lam_x, lam_v,
+ * lam_vr, lam_d as X(3,NUMNOD) arrays, lam_state as NUMELC rows of 33
+ * (SIG(5) of points 1..5, PLA(1..5), HOUR(1:3)); null = zero -- and, recomputing
+ * each segment from its checkpoint, returns the gradients with respect to the
+ * law-2 A, B, n (g_jc[3]), the element thicknesses THKE (g_thk[NUMELC]), the
+ * nodal masses and rotational inertias (g_ms, g_in [NUMNOD]), and the adjoints of
+ * the initial X, V, VR (any output may be null).  The time-step sequence is taken
+ * as fixed (exact when DTMX pins it).  The element derivatives are those of the
+ * FP64 chain evaluated on the forward states, in FP64 for either form; the context
+ * is left at the end of the forward run. */
+int orad_adjoint_forward(orad_ctx *ctx, int ncycle, int every);
+int orad_adjoint_reverse(orad_ctx *ctx, const double *lam_x, const double *lam_v,
+                         const double *lam_vr, const double *lam_d, const double *lam_state,
+                         double *g_jc, double *g_thk, double *g_ms, double *g_in,
+                         double *lam_x0, double *lam_v0, double *lam_vr0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,7 @@ SOURCES = [
     ("orad_shell_fp64.hip", ["-ffp-contract=off"]),
     ("orad_shell_mixed.hip", ["-ffp-contract=fast"]),
     ("orad_nodal.hip", ["-ffp-contract=off"]),
+    ("orad_adjoint.hip", ["-ffp-contract=off"]),
     ("orad_api.hip", []),
 ]
 

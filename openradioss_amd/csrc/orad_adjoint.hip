@@ -1,0 +1,244 @@
+// orad_adjoint.hip -- reverse (adjoint) sweep of the explicit shell cycle, FP64,
+// built without FMA contraction.
+//
+// One forward cycle k maps (nodes x_k, v_k, vr_k, d_k; element state s_k) to
+// cycle k+1: the element kernel turns (x_k, v_k, vr_k, s_k, thickness, A, B, n)
+// into PARITH/ON slots and s_{k+1}; the nodal kernel sums the slots (S, R per
+// node), divides by MS / IN, applies BCS10 / FIXVEL and integrates
+// v_{k+1} = v_k + DT12 a, x_{k+1} = x_k + DT2 v_{k+1}, d_{k+1} = d_k + DT2 v_{k+1}.
+// The time-step sequence is taken as given (exact when DTMX pins it).
+//
+// Reverse of cycle k, three launches:
+//   adj_node_kernel    adjoints of v_{k+1}, x, d -> adjoint of S and R per node,
+//                      of v_k through the update, and the MS / IN sensitivities;
+//   adj_elem_kernel    vector-Jacobian product of the element chain: the CFORC3
+//                      chain (orad_shell_chain.hpp) instantiated on Dual<K> seeds K
+//                      of the element's 73 inputs per thread (4 corners x (x, v, vr),
+//                      25 stresses, 5 plastic strains, 3 incremental hourglass
+//                      forces, thickness, A, B, n) and contracts the tangents of
+//                      its outputs (slots, new state) with their adjoints;
+//   adj_gather_kernel  per node, the corner contributions in slot order (the
+//                      PARITH/ON CSR), so the sums are deterministic.
+#include "orad_dual.hpp"
+#include "orad_shell_chain.hpp"
+#include "orad_brick_io.hpp"
+#include "orad_adjoint_args.hpp"
+
+namespace orad {
+
+constexpr int ADJ_BLOCK = 64;
+constexpr int ADJ_K = 4;         // tangents per thread
+
+template <typename S, int K>
+struct AdjIO {
+  const S *st;
+  int e, ne, i0;
+  const double *lst_in;
+  double *acc;
+  __device__ __forceinline__ Dual<K> get(int c) const {
+    Dual<K> r{double(st[(size_t)c * ne + e])};
+    const int j = adj_state_index(c);
+    if (j >= 0) {
+      const int q = 36 + j - i0;
+#pragma unroll
+      for (int t = 0; t < K; ++t) r.d[t] = (q == t) ? 1.0 : 0.0;
+    }
+    return r;
+  }
+  __device__ __forceinline__ void put(int c, const Dual<K> &v) const {
+    const int j = adj_state_index(c);
+    if (j < 0) return;
+    const double l = lst_in[(size_t)j * ne + e];
+#pragma unroll
+    for (int t = 0; t < K; ++t) acc[t] += l * v.d[t];
+  }
+};
+
+template <typename S, int K>
+__global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_kernel(AdjArgs<S> a, int nchunk) {
+  typedef Dual<K> D;
+  const int ne = a.numelc;
+  const long idx = (long)blockIdx.x * ADJ_BLOCK + threadIdx.x;
+  if (idx >= (long)ne * nchunk) return;
+  const int chunk = (int)(idx / ne), e = (int)(idx - (long)chunk * ne);
+  const int i0 = chunk * K;
+  int nd[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) nd[c] = a.conn[(size_t)c * ne + e];
+  D xg[3][4], vg[3][4], vrg[3][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double4 x = a.X4[nd[c]], v = a.V4[nd[c]], w = a.VR4[nd[c]];
+    const double xv[3] = {x.x, x.y, x.z}, vv[3] = {v.x, v.y, v.z}, wv[3] = {w.x, w.y, w.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      xg[i][c] = D(xv[i]);
+      vg[i][c] = D(vv[i]);
+      vrg[i][c] = D(wv[i]);
+#pragma unroll
+      for (int t = 0; t < K; ++t) {
+        xg[i][c].d[t] = (9 * c + i - i0 == t) ? 1.0 : 0.0;
+        vg[i][c].d[t] = (9 * c + 3 + i - i0 == t) ? 1.0 : 0.0;
+        vrg[i][c].d[t] = (9 * c + 6 + i - i0 == t) ? 1.0 : 0.0;
+      }
+    }
+  }
+  D thk0(a.thke[e]);
+  D jc[3] = {D(a.mat.ca), D(a.mat.cb), D(a.mat.cn)};
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    thk0.d[t] = (69 - i0 == t) ? 1.0 : 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) jc[j].d[t] = (70 + j - i0 == t) ? 1.0 : 0.0;
+  }
+  double acc[K];
+#pragma unroll
+  for (int t = 0; t < K; ++t) acc[t] = 0.0;
+  const AdjIO<S, K> io{a.st, e, ne, i0, a.lst_in, acc};
+  ShellOut<D> o;
+  const double *lamS = a.lamS, *lamR = a.lamR;
+  shell_chain<D, D>(xg, vg, vrg, thk0, nd[2] == nd[3], io, a.dt1, a.mat, jc, a.prop, a.k, o,
+                    [&](int c, const D &fx, const D &fy, const D &fz, const D &mx, const D &my,
+                        const D &mz) {
+                      const size_t n3 = 3 * (size_t)nd[c];
+                      const double l0 = lamS[n3], l1 = lamS[n3 + 1], l2 = lamS[n3 + 2];
+                      const double r0 = lamR[n3], r1 = lamR[n3 + 1], r2 = lamR[n3 + 2];
+#pragma unroll
+                      for (int t = 0; t < K; ++t)
+                        acc[t] += l0 * fx.d[t] + l1 * fy.d[t] + l2 * fz.d[t] + r0 * mx.d[t] +
+                                  r1 * my.d[t] + r2 * mz.d[t];
+                    });
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    const int i = i0 + t;
+    if (i < 36) a.lcorner[(size_t)i * ne + e] = acc[t];
+    else if (i < 36 + ADJ_NST) a.lst_out[(size_t)(i - 36) * ne + e] = acc[t];
+    else if (i == 69) a.g_thk[e] += acc[t];
+    else if (i < ADJ_NIN) a.g_jc[(size_t)(i - 70) * ne + e] += acc[t];
+  }
+}
+
+// Reverse of the nodal update of one cycle.  In: lx, lv, ld, lvr = adjoints of
+// x, v, d, vr after the cycle; A, AR = the cycle's accelerations (after BCS10).
+// Out: lv, lvr = adjoints of v_k, vr_k through the update (element terms are added
+// by the gather), lamS / lamR = adjoints of the slot sums, MS / IN sensitivities
+// accumulated into g_ms / g_in.
+__global__ __launch_bounds__(256) void adj_node_kernel(int numnod, const double4 *X4,
+                                                       const double4 *V4, const double4 *VR4,
+                                                       const double *A, const double *AR,
+                                                       double dt2, double dt12, const double *lx,
+                                                       double *lv, const double *ld, double *lvr,
+                                                       double *lamS, double *lamR, double *g_ms,
+                                                       double *g_in) {
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= numnod) return;
+  const int code = (int)V4[n].w;
+  const int ct = code_icodt(code), cr = code_icodr(code), fd = code_fvdir(code);
+  const double ms = X4[n].w, in = VR4[n].w;
+  double gm = 0.0, gi = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const size_t j = 3 * (size_t)n + i;
+    const double u = lv[j] + dt2 * (lx[j] + ld[j]);
+    const bool fixed = (ct & (4 >> i)) != 0 || fd == i + 1;
+    double ls = 0.0, lvn = 0.0;
+    if (!fixed) {
+      lvn = u;
+      if (ms > 0.0) {
+        ls = dt12 * u / ms;
+        gm = gm - dt12 * u * A[j] / ms;
+      }
+    }
+    lv[j] = lvn;
+    lamS[j] = ls;
+    const double ur = lvr[j];
+    const bool rfixed = (cr & (4 >> i)) != 0;
+    double lr = 0.0, lvrn = 0.0;
+    if (!rfixed) {
+      lvrn = ur;
+      if (in > 0.0) {
+        lr = dt12 * ur / in;
+        gi = gi - dt12 * ur * AR[j] / in;
+      }
+    }
+    lvr[j] = lvrn;
+    lamR[j] = lr;
+  }
+  g_ms[n] += gm;
+  g_in[n] += gi;
+}
+
+// Element corner contributions to the node adjoints, per node in slot order.
+__global__ __launch_bounds__(256) void adj_gather_kernel(int numnod, int numelc,
+                                                         const int32_t *adsky,
+                                                         const int32_t *slot_elem,
+                                                         const int32_t *slot_corner,
+                                                         const double *lcorner, double *lx,
+                                                         double *lv, double *lvr) {
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= numnod) return;
+  double ax[3] = {0, 0, 0}, av[3] = {0, 0, 0}, ar[3] = {0, 0, 0};
+  for (int s = adsky[n]; s < adsky[n + 1]; ++s) {
+    const int e = slot_elem[s], c = slot_corner[s];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      ax[i] += lcorner[(size_t)(9 * c + i) * numelc + e];
+      av[i] += lcorner[(size_t)(9 * c + 3 + i) * numelc + e];
+      ar[i] += lcorner[(size_t)(9 * c + 6 + i) * numelc + e];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    lx[3 * (size_t)n + i] += ax[i];
+    lv[3 * (size_t)n + i] += av[i];
+    lvr[3 * (size_t)n + i] += ar[i];
+  }
+}
+
+int adj_nchunk() { return (ADJ_NIN + ADJ_K - 1) / ADJ_K; }
+
+hipError_t launch_adj_elem(bool fp32_state, const AdjArgs<double> &a64, hipStream_t s) {
+  const int nchunk = adj_nchunk();
+  const long nt = (long)a64.numelc * nchunk;
+  const int nb = (int)((nt + ADJ_BLOCK - 1) / ADJ_BLOCK);
+  if (nb <= 0) return hipSuccess;
+  if (fp32_state) {
+    AdjArgs<float> a;
+    a.numelc = a64.numelc; a.conn = a64.conn; a.X4 = a64.X4; a.V4 = a64.V4; a.VR4 = a64.VR4;
+    a.st = reinterpret_cast<const float *>(a64.st);
+    a.thke = a64.thke; a.lamS = a64.lamS; a.lamR = a64.lamR; a.lst_in = a64.lst_in;
+    a.lst_out = a64.lst_out; a.lcorner = a64.lcorner; a.g_thk = a64.g_thk; a.g_jc = a64.g_jc;
+    a.mat = a64.mat; a.prop = a64.prop; a.k = a64.k; a.dt1 = a64.dt1;
+    hipLaunchKernelGGL((adj_elem_kernel<float, ADJ_K>), dim3(nb), dim3(ADJ_BLOCK), 0, s, a,
+                       nchunk);
+  } else {
+    hipLaunchKernelGGL((adj_elem_kernel<double, ADJ_K>), dim3(nb), dim3(ADJ_BLOCK), 0, s, a64,
+                       nchunk);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_node(int numnod, const double4 *X4, const double4 *V4, const double4 *VR4,
+                           const double *A, const double *AR, double dt2, double dt12,
+                           const double *lx, double *lv, const double *ld, double *lvr,
+                           double *lamS, double *lamR, double *g_ms, double *g_in,
+                           hipStream_t s) {
+  const int nb = (numnod + 255) / 256;
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(adj_node_kernel, dim3(nb), dim3(256), 0, s, numnod, X4, V4, VR4, A, AR,
+                     dt2, dt12, lx, lv, ld, lvr, lamS, lamR, g_ms, g_in);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_gather(int numnod, int numelc, const int32_t *adsky,
+                             const int32_t *slot_elem, const int32_t *slot_corner,
+                             const double *lcorner, double *lx, double *lv, double *lvr,
+                             hipStream_t s) {
+  const int nb = (numnod + 255) / 256;
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(adj_gather_kernel, dim3(nb), dim3(256), 0, s, numnod, numelc, adsky,
+                     slot_elem, slot_corner, lcorner, lx, lv, lvr);
+  return hipGetLastError();
+}
+
+}  // namespace orad

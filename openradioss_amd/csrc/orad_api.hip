@@ -20,6 +20,7 @@
 #include "orad_common.hpp"
 
 #include "orad_ctx.hpp"
+#include "orad_adjoint_args.hpp"
 
 template <typename T>
 static BrickArgs<T> brick_args(orad_ctx *c) {
@@ -145,7 +146,8 @@ static hipError_t launch_cycle_elements(orad_ctx *c, int p) {
 static hipError_t enqueue_cycle(orad_ctx *c, int p) {
   hipError_t e = launch_cycle_elements(c, p);
   if (e != hipSuccess) return e;
-  return launch_node(node_args(c), c->precision == ORAD_MIXED, p, 1, 0, c->xcd_remap, c->stream);
+  return launch_node(node_args(c), c->precision == ORAD_MIXED, p, 1, c->cycle_write_a,
+                     c->xcd_remap, c->stream);
 }
 
 static int init_state(orad_ctx *c) {
@@ -169,6 +171,7 @@ static int init_state(orad_ctx *c) {
 }
 
 static int get_fsky_shell(orad_ctx *c, double *fsky);
+static void free_adjoint(orad_ctx *c);
 
 extern "C" {
 
@@ -214,6 +217,7 @@ int orad_destroy(orad_ctx *c) {
   (void)hipSetDevice(c->device);
   drop_graphs(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_adjoint(c);
   for (void *p : {(void *)c->X4, (void *)c->V4, (void *)c->D, (void *)c->A, (void *)c->STIFN,
                   (void *)c->conn, (void *)c->iads, (void *)c->adsky, c->state, (void *)c->vol0,
                   c->fsky, c->sti, (void *)c->slot_elem,
@@ -1221,7 +1225,8 @@ struct Snapshot {
     for (auto &b : live) {
       void *p = nullptr;
       hipError_t e = dalloc(&p, b.second);
-      if (e == hipSuccess) e = hipMemcpy(p, b.first, b.second, hipMemcpyDeviceToDevice);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(p, b.first, b.second, hipMemcpyDeviceToDevice, c->stream);
       if (e != hipSuccess) {
         dfree(p);
         return e;
@@ -1229,15 +1234,17 @@ struct Snapshot {
       bufs.push_back({b.first, p});
       sizes.push_back(b.second);
     }
-    return hipSuccess;
+    return hipStreamSynchronize(c->stream);
   }
+  // copies on the context's stream, ordered with its kernels
   hipError_t restore(orad_ctx *c) {
     for (size_t i = 0; i < bufs.size(); ++i) {
-      hipError_t e = hipMemcpy(bufs[i].first, bufs[i].second, sizes[i], hipMemcpyDeviceToDevice);
+      hipError_t e = hipMemcpyAsync(bufs[i].first, bufs[i].second, sizes[i],
+                                    hipMemcpyDeviceToDevice, c->stream);
       if (e != hipSuccess) return e;
     }
     c->host_ncycle = host_ncycle;
-    return hipSuccess;
+    return hipStreamSynchronize(c->stream);
   }
   ~Snapshot() {
     for (auto &b : bufs) dfree(b.second);
@@ -1299,6 +1306,219 @@ int orad_autotune(orad_ctx *c, int cycles, orad_launch_t *best, double *best_ms)
   if (best) orad_get_launch(c, best);
   if (best_ms) *best_ms = tbest;
   c->err.clear();
+  return ORAD_OK;
+}
+
+}  // extern "C"
+
+// ============================================================== adjoint ===
+
+// Checkpoints of an adjoint forward run: checkpoint s holds the state before cycle
+// s * every; fin the state after the last cycle.
+struct AdjStore {
+  int ncycle = 0, every = 0;
+  std::vector<Snapshot *> ck;
+  Snapshot *fin = nullptr;
+  ~AdjStore() {
+    for (Snapshot *p : ck) delete p;
+    delete fin;
+  }
+};
+
+static void free_adjoint(orad_ctx *c) {
+  delete c->adj;
+  c->adj = nullptr;
+}
+
+// Device buffers of one reverse sweep (freed on scope exit).
+struct DevBufs {
+  std::vector<void *> ptrs;
+  template <typename P>
+  hipError_t get(P **p, size_t bytes, bool zero = true) {
+    hipError_t e = dalloc(p, bytes);
+    if (e != hipSuccess) return e;
+    ptrs.push_back((void *)*p);
+    return zero ? hipMemset(*p, 0, bytes) : hipSuccess;
+  }
+  ~DevBufs() {
+    for (void *p : ptrs) dfree(p);
+  }
+};
+
+extern "C" {
+
+int orad_adjoint_forward(orad_ctx *c, int ncycle, int every) {
+  if (!c) return ORAD_EINVAL;
+  if (c->family != FAM_SHELL || c->batched)
+    return fail(c, ORAD_ESTATE, "adjoint_forward: one shell model only");
+  int r = ready(c);
+  if (r) return r;
+  if (ncycle < 1 || every < 1) return fail(c, ORAD_EINVAL, "adjoint_forward: bad cycle counts");
+  HIPCHK(c, hipSetDevice(c->device));
+  free_adjoint(c);
+  AdjStore *a = new (std::nothrow) AdjStore();
+  if (!a) return fail(c, ORAD_EINVAL, "adjoint_forward: out of host memory");
+  c->adj = a;
+  a->ncycle = ncycle;
+  a->every = every;
+  for (int k = 0; k < ncycle; ++k) {
+    if (k % every == 0) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      Snapshot *sp = new (std::nothrow) Snapshot();
+      if (!sp) return fail(c, ORAD_EINVAL, "adjoint_forward: out of host memory");
+      a->ck.push_back(sp);
+      HIPCHK(c, sp->take(c));
+    }
+    HIPCHK(c, enqueue_cycle(c, c->host_ncycle & 1));
+    c->host_ncycle += 1;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  a->fin = new (std::nothrow) Snapshot();
+  if (!a->fin) return fail(c, ORAD_EINVAL, "adjoint_forward: out of host memory");
+  HIPCHK(c, a->fin->take(c));
+  return ORAD_OK;
+}
+
+int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
+                         const double *lam_vr, const double *lam_d, const double *lam_state,
+                         double *g_jc, double *g_thk, double *g_ms, double *g_in,
+                         double *lam_x0, double *lam_v0, double *lam_vr0) {
+  if (!c) return ORAD_EINVAL;
+  if (!c->adj || !c->adj->fin) return fail(c, ORAD_ESTATE, "adjoint_reverse: no forward run");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  AdjStore *A = c->adj;
+  const size_t nn = (size_t)c->numnod, ne = (size_t)c->numelc, ts = c->tsize();
+  const size_t stb = SC_NCOL * ne * ts;
+  const int L = A->every;
+  DevBufs B;
+  double *lx, *lv, *ld, *lvr, *lamS, *lamR, *gms, *gin, *lst[2], *lcorner, *gthk, *gjc;
+  int32_t *scorner;
+  double4 *sx, *sv, *svr;
+  double *sa, *sar;
+  char *sst;
+  Scal *sscal;
+  HIPCHK(c, B.get(&lx, 3 * nn * 8));
+  HIPCHK(c, B.get(&lv, 3 * nn * 8));
+  HIPCHK(c, B.get(&ld, 3 * nn * 8));
+  HIPCHK(c, B.get(&lvr, 3 * nn * 8));
+  HIPCHK(c, B.get(&lamS, 3 * nn * 8));
+  HIPCHK(c, B.get(&lamR, 3 * nn * 8));
+  HIPCHK(c, B.get(&gms, nn * 8));
+  HIPCHK(c, B.get(&gin, nn * 8));
+  HIPCHK(c, B.get(&lst[0], 33 * ne * 8));
+  HIPCHK(c, B.get(&lst[1], 33 * ne * 8));
+  HIPCHK(c, B.get(&lcorner, 36 * ne * 8));
+  HIPCHK(c, B.get(&gthk, ne * 8));
+  HIPCHK(c, B.get(&gjc, 3 * ne * 8));
+  HIPCHK(c, B.get(&scorner, 4 * ne * 4, false));
+  HIPCHK(c, B.get(&sx, (size_t)L * nn * 32, false));
+  HIPCHK(c, B.get(&sv, (size_t)L * nn * 32, false));
+  HIPCHK(c, B.get(&svr, (size_t)L * nn * 32, false));
+  HIPCHK(c, B.get(&sa, (size_t)L * 3 * nn * 8, false));
+  HIPCHK(c, B.get(&sar, (size_t)L * 3 * nn * 8, false));
+  HIPCHK(c, B.get(&sst, (size_t)L * stb, false));
+  HIPCHK(c, B.get(&sscal, (size_t)L * sizeof(Scal), false));
+  {
+    std::vector<int32_t> sc(4 * ne);
+    for (size_t e = 0; e < ne; ++e)
+      for (int k = 0; k < 4; ++k) sc[c->h_iads[4 * e + k]] = k;
+    HIPCHK(c, hipMemcpy(scorner, sc.data(), 4 * ne * 4, hipMemcpyHostToDevice));
+  }
+  if (lam_x) HIPCHK(c, hipMemcpy(lx, lam_x, 3 * nn * 8, hipMemcpyHostToDevice));
+  if (lam_v) HIPCHK(c, hipMemcpy(lv, lam_v, 3 * nn * 8, hipMemcpyHostToDevice));
+  if (lam_vr) HIPCHK(c, hipMemcpy(lvr, lam_vr, 3 * nn * 8, hipMemcpyHostToDevice));
+  if (lam_d) HIPCHK(c, hipMemcpy(ld, lam_d, 3 * nn * 8, hipMemcpyHostToDevice));
+  if (lam_state) {
+    // rows of 33 (SIG, PLA, HOUR(1:3)) -> columns
+    std::vector<double> col(33 * ne);
+    for (size_t e = 0; e < ne; ++e)
+      for (int j = 0; j < 33; ++j) col[j * ne + e] = lam_state[33 * e + j];
+    HIPCHK(c, hipMemcpy(lst[0], col.data(), 33 * ne * 8, hipMemcpyHostToDevice));
+  }
+  int cur = 0;
+  AdjArgs<double> ag{};
+  ag.numelc = c->numelc;
+  ag.conn = c->conn;
+  ag.thke = c->thke;
+  ag.lamS = lamS;
+  ag.lamR = lamR;
+  ag.lcorner = lcorner;
+  ag.g_thk = gthk;
+  ag.g_jc = gjc;
+  ag.mat = c->shmat;
+  ag.prop = c->shprop;
+  ag.k = c->shk;
+  const int nseg = (int)A->ck.size();
+  std::vector<Scal> hs(L);
+  for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+    const int k0 = sgi * A->every, len = std::min(A->every, A->ncycle - k0);
+    // recompute the segment, storing each cycle's inputs and accelerations
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, A->ck[sgi]->restore(c));
+    drop_graphs(c);
+    c->cycle_write_a = 1;
+    hipError_t e = hipSuccess;
+    for (int j = 0; j < len && e == hipSuccess; ++j) {
+      const int p = c->host_ncycle & 1;
+      e = hipMemcpyAsync(sx + (size_t)j * nn, c->X4, nn * 32, hipMemcpyDeviceToDevice, c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(sv + (size_t)j * nn, c->V4, nn * 32, hipMemcpyDeviceToDevice, c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(svr + (size_t)j * nn, c->VR4, nn * 32, hipMemcpyDeviceToDevice, c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(sst + (size_t)j * stb, c->state, stb, hipMemcpyDeviceToDevice, c->stream);
+      if (e == hipSuccess) e = enqueue_cycle(c, p);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(sa + (size_t)j * 3 * nn, c->A, 3 * nn * 8, hipMemcpyDeviceToDevice, c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(sar + (size_t)j * 3 * nn, c->AR, 3 * nn * 8, hipMemcpyDeviceToDevice, c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(sscal + j, &c->ctl->s[p ^ 1], sizeof(Scal), hipMemcpyDeviceToDevice, c->stream);
+      c->host_ncycle += 1;
+    }
+    c->cycle_write_a = 0;
+    HIPCHK(c, e);
+    HIPCHK(c, hipMemcpyAsync(hs.data(), sscal, len * sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // reverse through the segment
+    for (int j = len - 1; j >= 0; --j) {
+      const double4 *X = sx + (size_t)j * nn, *V = sv + (size_t)j * nn, *W = svr + (size_t)j * nn;
+      HIPCHK(c, launch_adj_node(c->numnod, X, V, W, sa + (size_t)j * 3 * nn,
+                                sar + (size_t)j * 3 * nn, hs[j].dt2, hs[j].dt12, lx, lv, ld, lvr,
+                                lamS, lamR, gms, gin, c->stream));
+      ag.X4 = X;
+      ag.V4 = V;
+      ag.VR4 = W;
+      ag.st = reinterpret_cast<const double *>(sst + (size_t)j * stb);
+      ag.lst_in = lst[cur];
+      ag.lst_out = lst[cur ^ 1];
+      ag.dt1 = hs[j].dt1;
+      HIPCHK(c, launch_adj_elem(c->precision == ORAD_MIXED, ag, c->stream));
+      HIPCHK(c, launch_adj_gather(c->numnod, c->numelc, c->adsky, c->slot_elem, scorner, lcorner,
+                                  lx, lv, lvr, c->stream));
+      cur ^= 1;
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (g_jc) {
+    std::vector<double> h(3 * ne);
+    HIPCHK(c, hipMemcpy(h.data(), gjc, 3 * ne * 8, hipMemcpyDeviceToHost));
+    for (int j = 0; j < 3; ++j) {
+      double acc = 0.0;
+      for (size_t e2 = 0; e2 < ne; ++e2) acc += h[j * ne + e2];
+      g_jc[j] = acc;
+    }
+  }
+  if (g_thk) HIPCHK(c, hipMemcpy(g_thk, gthk, ne * 8, hipMemcpyDeviceToHost));
+  if (g_ms) HIPCHK(c, hipMemcpy(g_ms, gms, nn * 8, hipMemcpyDeviceToHost));
+  if (g_in) HIPCHK(c, hipMemcpy(g_in, gin, nn * 8, hipMemcpyDeviceToHost));
+  if (lam_x0) HIPCHK(c, hipMemcpy(lam_x0, lx, 3 * nn * 8, hipMemcpyDeviceToHost));
+  if (lam_v0) HIPCHK(c, hipMemcpy(lam_v0, lv, 3 * nn * 8, hipMemcpyDeviceToHost));
+  if (lam_vr0) HIPCHK(c, hipMemcpy(lam_vr0, lvr, 3 * nn * 8, hipMemcpyDeviceToHost));
+  // leave the context at the end of the forward run
+  HIPCHK(c, A->fin->restore(c));
+  drop_graphs(c);
   return ORAD_OK;
 }
 

@@ -35,7 +35,17 @@ hipError_t launch_shell_batch_mixed(const ShellArgs<float> &, const BatchArgs &,
                                     int, int, hipStream_t);
 int shell_batch_occupancy_fp64(int);
 int shell_batch_occupancy_mixed(int);
+template <typename S> struct AdjArgs;
+hipError_t launch_adj_elem(bool, const AdjArgs<double> &, hipStream_t);
+hipError_t launch_adj_node(int, const double4 *, const double4 *, const double4 *,
+                           const double *, const double *, double, double, const double *,
+                           double *, const double *, double *, double *, double *, double *,
+                           double *, hipStream_t);
+hipError_t launch_adj_gather(int, int, const int32_t *, const int32_t *, const int32_t *,
+                             const double *, double *, double *, double *, hipStream_t);
 }  // namespace orad
+
+struct AdjStore;   // checkpoints of an adjoint forward run (orad_api.hip)
 
 using namespace orad;
 
@@ -102,6 +112,9 @@ struct orad_ctx {
   int batch_lds = 0;        // batched shells: dynamic LDS bytes per workgroup (residency cap)
   int batch_grid = 0;       // persistent grid size (resident capacity at batch_lds)
   int num_cu = 0;
+
+  AdjStore *adj = nullptr;
+  int cycle_write_a = 0;    // cycles also store accelerations A / AR (adjoint recomputation)
 
   int host_ncycle = 0;  // cycle counter mirrored on the host (its parity selects buffers)
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // two cycles starting at parity 0 / 1

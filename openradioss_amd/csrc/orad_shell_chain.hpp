@@ -44,7 +44,8 @@ template <typename T> __device__ __forceinline__ T tabs(T a) { return a < T(0) ?
 // TSTAR = 0).  off: element OFF, lowered by the ductile-rupture test; thkn: the
 // element thickness updated point by point.
 template <typename T>
-__device__ __forceinline__ void shell_law2_point(const ShMat &m, const ShConst &k, T gs, T dt1,
+__device__ __forceinline__ void shell_law2_point(const ShMat &m, const T jc[3], const ShConst &k,
+                                                 T gs, T dt1,
                                                  T asrate, T epsd_pg, T off_old, T &off, T &pla,
                                                  T &epsdp, T &epchk, T &thkn, T thklyl,
                                                  const T deps[5], T s[5]) {
@@ -52,7 +53,7 @@ __device__ __forceinline__ void shell_law2_point(const ShMat &m, const ShConst &
   const T young = T(m.young), g = T(m.shear), nu = T(m.nu);
   const T a11 = M<T>::div(young, ONE - nu * nu);
   const T a12 = a11 * nu;
-  const T ca0 = T(m.ca), cb0 = T(m.cb), cn = T(m.cn), ymax0 = T(m.ymax);
+  const T ca0 = jc[0], cb0 = jc[1], cn = jc[2], ymax0 = T(m.ymax);
   const T cc = T(m.cc);
   const T EM20 = T(k.em20);
   const T epdr = tmax(T(m.epdr) * dt1, EM20);
@@ -312,8 +313,8 @@ __device__ __forceinline__ void shell_kinematics(const TG xg[3][4], const TG vg[
 template <typename T, typename TG, typename IO, typename Emit>
 __device__ __forceinline__ void shell_chain(const TG xg[3][4], const TG vg[3][4],
                                             const TG vrg[3][4], T thk0, bool tri, const IO &io,
-                                            double dt1d,
-                                            const ShMat &m, const ShProp &p, const ShConst &k,
+                                            double dt1d, const ShMat &m, const T jc[3],
+                                            const ShProp &p, const ShConst &k,
                                             ShellOut<T> &out, Emit emit) {
   const T ZERO = T(0), ONE = T(1), TWO = T(2), THREE = T(3), HALF = T(0.5), FOURTH = T(0.25);
   const T EM20 = T(k.em20);
@@ -418,7 +419,7 @@ __device__ __forceinline__ void shell_chain(const TG xg[3][4], const TG vg[3][4]
 #pragma unroll
     for (int c = 0; c < 5; ++c) s[c] = io.get(SC_SIG + 5 * it + c);
     T pla = io.get(SC_PLA + it), epsdp = io.get(SC_EPSDP + it);
-    shell_law2_point<T>(m, k, GS, dt1, asrate, epsd_pg, off_old, off, pla, epsdp, epchk, thkn,
+    shell_law2_point<T>(m, jc, k, GS, dt1, asrate, epsd_pg, off_old, off, pla, epsdp, epchk, thkn,
                         thklyl, deps, s);
     viscmx = tmax(T(p.dm), viscmx);
 #pragma unroll

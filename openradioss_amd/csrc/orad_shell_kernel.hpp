@@ -81,7 +81,8 @@ __global__ __launch_bounds__(BLOCK) void shell_forces_kernel(ShellArgs<T> a, int
     ShellOut<T> o;
     Slot6<T> *fsky = a.fsky;
     const int32_t *iadc = a.iadc;
-    shell_chain<T, TG>(xg, vg, vrg, thk0, nd[2] == nd[3], io, dt1, a.mat, a.prop, a.k, o,
+    const T jc[3] = {T(a.mat.ca), T(a.mat.cb), T(a.mat.cn)};
+    shell_chain<T, TG>(xg, vg, vrg, thk0, nd[2] == nd[3], io, dt1, a.mat, jc, a.prop, a.k, o,
                    [&](int c, T fx, T fy, T fz, T mx, T my, T mz) {
                      Slot6<T> f;
                      f.fx = fx; f.fy = fy; f.fz = fz; f.mx = mx; f.my = my; f.mz = mz;
@@ -166,7 +167,8 @@ __global__ __launch_bounds__(BLOCK) void shell_batch_kernel(ShellArgs<T> a, Batc
           ShellOut<T> o;
           Slot6<T> *fsky = a.fsky;
           const int32_t *iadc = a.iadc;
-          shell_chain<T, TG>(xg, vg, vrg, thk0, nd[2] == nd[3], io, dt1, mat, a.prop,
+          const T jc[3] = {T(mat.ca), T(mat.cb), T(mat.cn)};
+          shell_chain<T, TG>(xg, vg, vrg, thk0, nd[2] == nd[3], io, dt1, mat, jc, a.prop,
                              a.k, o, [&](int c, T fx, T fy, T fz, T mx, T my, T mz) {
                                Slot6<T> f;
                                f.fx = fx; f.fy = fy; f.fz = fz; f.mx = mx; f.my = my; f.mz = mz;

@@ -26,7 +26,7 @@ EXPORTS = (
     "orad_set_shell_prop", "orad_cforc3", "orad_asspar4_rot", "orad_get_nodes_rot",
     "orad_get_shell_state", "orad_set_shell_state", "orad_set_shell_models",
     "orad_get_model_time", "orad_model_energies", "orad_set_launch", "orad_get_launch",
-    "orad_autotune",
+    "orad_autotune", "orad_adjoint_forward", "orad_adjoint_reverse",
 )
 NIXC = 7
 SH_NCOL = 53
@@ -102,6 +102,8 @@ def load_library(path=LIB_PATH):
         "orad_set_launch": ([P, C.POINTER(Launch)], I),
         "orad_get_launch": ([P, C.POINTER(Launch)], I),
         "orad_autotune": ([P, I, C.POINTER(Launch), C.POINTER(D)], I),
+        "orad_adjoint_forward": ([P, I, I], I),
+        "orad_adjoint_reverse": ([P] + [P] * 12, I),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -353,6 +355,24 @@ class ShellSolver:
                                                  _p(p) if p is not None else None,
                                                  _p(ip) if ip is not None else None))
         self.nmodel = nm
+
+    def adjoint_forward(self, ncycle, every):
+        """Run ncycle cycles keeping a checkpoint every `every` cycles."""
+        self._chk(self.lib.orad_adjoint_forward(self.h, ncycle, every))
+
+    def adjoint_reverse(self, lam_x=None, lam_v=None, lam_vr=None, lam_d=None, lam_state=None):
+        """Gradients from the adjoints of the final state: dict with g_jc (dA, dB, dn),
+        g_thk, g_ms, g_in and the adjoints of the initial x, v, vr."""
+        nn, ne = self.numnod, self.numelc
+        f = lambda a, shape: None if a is None else _f64(a, shape)
+        ins = [f(lam_x, (nn, 3)), f(lam_v, (nn, 3)), f(lam_vr, (nn, 3)), f(lam_d, (nn, 3)),
+               f(lam_state, (ne, 33))]
+        out = dict(g_jc=np.zeros(3), g_thk=np.zeros(ne), g_ms=np.zeros(nn), g_in=np.zeros(nn),
+                   lam_x0=np.zeros((nn, 3)), lam_v0=np.zeros((nn, 3)), lam_vr0=np.zeros((nn, 3)))
+        args = [None if a is None else _p(a) for a in ins] + [_p(out[k]) for k in
+                ("g_jc", "g_thk", "g_ms", "g_in", "lam_x0", "lam_v0", "lam_vr0")]
+        self._chk(self.lib.orad_adjoint_reverse(self.h, *args))
+        return out
 
     def model_time(self, m):
         t = TimeState()

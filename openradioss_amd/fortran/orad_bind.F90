@@ -202,6 +202,20 @@ module orad_bind
       type(orad_launch_t) :: best
       real(c_double) :: best_ms
     end function
+    integer(c_int) function orad_adjoint_forward(ctx, ncycle, every) &
+        bind(c, name='orad_adjoint_forward')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ncycle, every
+    end function
+    ! every argument: c_loc of an array, or c_null_ptr
+    integer(c_int) function orad_adjoint_reverse(ctx, lam_x, lam_v, lam_vr, lam_d, lam_state, &
+        g_jc, g_thk, g_ms, g_in, lam_x0, lam_v0, lam_vr0) bind(c, name='orad_adjoint_reverse')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: lam_x, lam_v, lam_vr, lam_d, lam_state
+      type(c_ptr), value :: g_jc, g_thk, g_ms, g_in, lam_x0, lam_v0, lam_vr0
+    end function
   end interface
 
 contains

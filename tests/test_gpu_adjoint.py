@@ -1,0 +1,233 @@
+"""Adjoint of the shell cycle through the C ABI, wrapped as a torch.autograd.Function,
+checked against central finite differences of forward runs.
+
+Setup (all tests): the time step is pinned by DTMX at 0.8 x the initial element
+time step, so the time-step sequence does not depend on the parameters; masses and
+rotational inertias are torch functions of the thicknesses (CINMAS), so their
+sensitivities reach the thickness gradient.
+
+Objective J = final z displacement of the centre node + total plastic work, the
+plastic work of a point being V_pt * (A*pla + B*pla^(n+1)/(n+1)) with
+V_pt = wf_pt * thickness * initial area.
+
+Bars:
+  * short runs (checkpoint boundaries inside the run): adjoint vs central FD with
+    h = 1e-6 relative, relative error < 1e-5 on A, B, n, thicknesses and an
+    initial velocity.  FD converges to the adjoint as h falls (1e-3 .. 1e-7); at
+    h = 1e-4 a difference quotient can straddle the cycle at which a point starts
+    to yield (the radial return is not differentiable there), which after 60
+    cycles of the 16 x 16 plate leaves up to 1% in the quotient of A;
+  * 128 x 128 plate, 2000 cycles, checkpoint every 100 (the adjoint study): relative
+    error < 1e-3 against FD with h = 1e-6 on 20 sampled parameters (A, B, n and 17
+    thicknesses; measured on MI355X: 5.6e-4 at most, 1e-5 typical).  The quotients
+    with h = 1e-4 are printed beside them: over 2000 cycles so many points start to
+    yield inside a 1e-4 step that they are not derivative estimates (measured: 1%
+    to 5x away from both the adjoint and the h = 1e-6 quotients);
+  * FP32-element forward with the FP64 adjoint vs the all-FP64 gradient: printed.
+"""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as po
+from openradioss_amd import deck as dk
+from openradioss_amd import engine as en
+
+pytestmark = pytest.mark.gpu
+
+
+def masses_torch(x, ixc, thk, rho0):
+    """CINMAS for isotropic shells (cinmas.F:1294-1296,1322) in torch."""
+    p = x[ixc]
+    area = 0.5 * torch.linalg.norm(torch.cross(p[:, 2] - p[:, 0], p[:, 3] - p[:, 1], dim=1), dim=1)
+    ems = rho0 * thk * area * 0.25
+    xi = ems * (area / 9.0 + thk * thk / 12.0)
+    nn = x.shape[0]
+    ms = torch.zeros(nn, dtype=torch.float64)
+    inr = torch.zeros(nn, dtype=torch.float64)
+    for k in range(4):
+        ms = ms.index_add(0, ixc[:, k], ems)
+        inr = inr.index_add(0, ixc[:, k], xi)
+    return ms, inr, area
+
+
+class ShellRun(torch.autograd.Function):
+    """(A, B, n, thk, ms, in) -> (final displacement d, final plastic strains pla)
+    of `ncyc` cycles of a shell deck; backward runs the library's adjoint."""
+
+    @staticmethod
+    def forward(ctx, A, B, n, thk, ms, inr, deck, prec, ncyc, every, dtmx):
+        d = dict(deck)
+        d["mat"] = dict(deck["mat"], ca=float(A), cb=float(B), cn=float(n))
+        d["thk"] = thk.detach().numpy()
+        d["ms"] = ms.detach().numpy()
+        d["in"] = inr.detach().numpy()
+        s = en.ShellSolver(d, prec)
+        s.set_time(dtmx=dtmx)
+        s.adjoint_forward(ncyc, every)
+        nodes, st = s.nodes(), s.state()
+        ctx.solver = s
+        ctx.tt = s.time().tt
+        return torch.from_numpy(nodes["d"]), torch.from_numpy(st[:, po.SH_PLA0:po.SH_PLA0 + 5].copy())
+
+    @staticmethod
+    def backward(ctx, gd, gpla):
+        s = ctx.solver
+        lam_state = np.zeros((s.numelc, 33))
+        lam_state[:, 25:30] = gpla.numpy()
+        g = s.adjoint_reverse(lam_d=gd.numpy(), lam_state=lam_state)
+        s.close()
+        t = torch.tensor
+        return (t(g["g_jc"][0]), t(g["g_jc"][1]), t(g["g_jc"][2]), torch.from_numpy(g["g_thk"]),
+                torch.from_numpy(g["g_ms"]), torch.from_numpy(g["g_in"]), None, None, None, None,
+                None)
+
+
+def objective(deck, theta, prec, ncyc, every, dtmx, centre, wf):
+    """theta = (A, B, n, thk) tensors; returns J (a 0-d tensor) and the final time."""
+    A, B, n, thk = theta
+    x = torch.from_numpy(deck["x"])
+    ixc = torch.from_numpy(deck["ixc"].astype(np.int64))
+    ms, inr, area0 = masses_torch(x, ixc, thk, deck["mat"]["rho0"])
+    d, pla = ShellRun.apply(A, B, n, thk, ms, inr, deck, prec, ncyc, every, dtmx)
+    pos = pla > 0
+    safe = torch.where(pos, pla, torch.ones_like(pla))
+    w = torch.where(pos, A * pla + B * safe ** (n + 1.0) / (n + 1.0), torch.zeros_like(pla))
+    vpt = torch.from_numpy(wf)[None, :] * (thk * area0)[:, None]
+    return d[centre, 2] + (vpt * w).sum()
+
+
+def forward_only(deck, theta, ncyc, dtmx, centre, wf):
+    """J from a plain FP64 forward run (no adjoint bookkeeping)."""
+    A, B, n, thk = [float(v) if np.ndim(v) == 0 else np.asarray(v) for v in theta]
+    x = torch.from_numpy(deck["x"])
+    ixc = torch.from_numpy(deck["ixc"].astype(np.int64))
+    ms, inr, area0 = masses_torch(x, ixc, torch.from_numpy(thk), deck["mat"]["rho0"])
+    d = dict(deck, thk=thk, ms=ms.numpy(), **{"in": inr.numpy()})
+    d["mat"] = dict(deck["mat"], ca=A, cb=B, cn=n)
+    s = en.ShellSolver(d, en.FP64)
+    s.set_time(dtmx=dtmx)
+    t = s.run_cycles(ncyc)
+    nd, st = s.nodes(), s.state()
+    s.close()
+    pla = st[:, po.SH_PLA0:po.SH_PLA0 + 5]
+    w = np.where(pla > 0, A * pla + B * np.where(pla > 0, pla, 1.0) ** (n + 1.0) / (n + 1.0), 0.0)
+    vpt = wf[None, :] * (thk * area0.numpy())[:, None]
+    return nd["d"][centre, 2] + (vpt * w).sum(), t.tt
+
+
+def pinned_dt(deck):
+    s = en.ShellSolver(deck, en.FP64)
+    dt = s.cforc3(0, s.numelc, 0.0)
+    s.close()
+    return 0.8 * dt
+
+
+def setup_plate(n, patch, thk_noise=0.1, seed=20261203):
+    deck = dk.shell_plate(n=n, patch=patch, thk_noise=thk_noise, seed=seed)
+    ni = n + 1
+    centre = (n // 2) + ni * (n // 2)
+    _, wf, _ = po.shell_tables(deck["prop"]["npt"])
+    return deck, centre, wf, pinned_dt(deck)
+
+
+def gradients(deck, prec, ncyc, every, dtmx, centre, wf):
+    m = deck["mat"]
+    A = torch.tensor(m["ca"], dtype=torch.float64, requires_grad=True)
+    B = torch.tensor(m["cb"], dtype=torch.float64, requires_grad=True)
+    n = torch.tensor(m["cn"], dtype=torch.float64, requires_grad=True)
+    thk = torch.tensor(deck["thk"], dtype=torch.float64, requires_grad=True)
+    J = objective(deck, (A, B, n, thk), prec, ncyc, every, dtmx, centre, wf)
+    J.backward()
+    return float(J), np.array([A.grad.item(), B.grad.item(), n.grad.item()]), thk.grad.numpy()
+
+
+def fd(deck, which, idx, h, ncyc, dtmx, centre, wf):
+    m = deck["mat"]
+    base = [m["ca"], m["cb"], m["cn"], deck["thk"].copy()]
+    vals = []
+    for sgn in (1.0, -1.0):
+        th = [base[0], base[1], base[2], base[3].copy()]
+        if which < 3:
+            step = h * abs(base[which])
+            th[which] = base[which] + sgn * step
+        else:
+            step = h * abs(base[3][idx])
+            th[3][idx] = base[3][idx] + sgn * step
+        J, tt = forward_only(deck, th, ncyc, dtmx, centre, wf)
+        vals.append(J)
+    return (vals[0] - vals[1]) / (2.0 * step), tt
+
+
+def test_adjoint_short_runs_vs_fd():
+    deck, centre, wf, dtmx = setup_plate(n=16, patch=6)
+    ncyc, every = 60, 7                         # segment boundaries inside the run
+    J, gjc, gthk = gradients(deck, en.FP64, ncyc, every, dtmx, centre, wf)
+    J0, tt = forward_only(deck, [deck["mat"]["ca"], deck["mat"]["cb"], deck["mat"]["cn"],
+                                 deck["thk"]], ncyc, dtmx, centre, wf)
+    assert J == pytest.approx(J0, rel=1e-13)
+    assert tt == pytest.approx(ncyc * dtmx, rel=1e-12)       # the time step stayed pinned
+    rng = np.random.default_rng(1)
+    elems = [int(i) for i in rng.choice(deck["ixc"].shape[0], 6, replace=False)]
+    for which, idx, g in [(0, 0, gjc[0]), (1, 0, gjc[1]), (2, 0, gjc[2])] + \
+                         [(3, e, gthk[e]) for e in elems]:
+        gf, _ = fd(deck, which, idx, 1e-6, ncyc, dtmx, centre, wf)
+        assert abs(g - gf) <= 1e-5 * max(abs(gf), 1e-3 * np.abs(gthk).max()), (which, idx, g, gf)
+
+
+def test_adjoint_initial_velocity():
+    deck, centre, wf, dtmx = setup_plate(n=12, patch=4)
+    s = en.ShellSolver(deck, en.FP64)
+    s.set_time(dtmx=dtmx)
+    s.adjoint_forward(40, 9)
+    lam_d = np.zeros((deck["x"].shape[0], 3))
+    lam_d[centre, 2] = 1.0
+    g = s.adjoint_reverse(lam_d=lam_d)
+    s.close()
+    node = centre + 1
+    h = 1e-6 * abs(deck["v"][node, 2])
+    vals = []
+    for sgn in (1.0, -1.0):
+        d = dict(deck, v=deck["v"].copy())
+        d["v"][node, 2] += sgn * h
+        s = en.ShellSolver(d, en.FP64)
+        s.set_time(dtmx=dtmx)
+        s.run_cycles(40)
+        vals.append(s.nodes()["d"][centre, 2])
+        s.close()
+    gf = (vals[0] - vals[1]) / (2 * h)
+    print(f"d(centre z)/d(v0 z): adjoint {g['lam_v0'][node, 2]:.9e} fd {gf:.9e}")
+    assert g["lam_v0"][node, 2] == pytest.approx(gf, rel=1e-5)
+
+
+def test_adjoint_plate_128_vs_fd():
+    deck, centre, wf, dtmx = setup_plate(n=128, patch=32)
+    ncyc, every = 2000, 100
+    J, gjc, gthk = gradients(deck, en.FP64, ncyc, every, dtmx, centre, wf)
+    rng = np.random.default_rng(20261203)
+    ne = deck["ixc"].shape[0]
+    patch_elems = np.flatnonzero(np.abs(deck["x"][deck["ixc"][:, 0], 0] - 64.0) < 18.0)
+    elems = list(rng.choice(patch_elems, 9, replace=False)) + list(rng.choice(ne, 8, replace=False))
+    errs = {1e-4: [], 1e-6: []}
+    for which, idx, g in [(0, 0, gjc[0]), (1, 0, gjc[1]), (2, 0, gjc[2])] + \
+                         [(3, int(e), gthk[int(e)]) for e in elems]:
+        line = f"param {which}:{idx} adjoint {g:.6e}"
+        for h in errs:
+            gf, tt = fd(deck, which, idx, h, ncyc, dtmx, centre, wf)
+            errs[h].append(abs(g - gf) / max(abs(gf), 1e-30))
+            line += f" | fd(h={h:g}) {gf:.6e} rel {errs[h][-1]:.2e}"
+            assert tt == pytest.approx(ncyc * dtmx, rel=1e-12)
+        print(line, flush=True)
+    for h, e in errs.items():
+        print(f"h={h:g}: max relative error over {len(e)} parameters {max(e):.2e}")
+    assert max(errs[1e-6]) < 1e-3
+
+
+def test_adjoint_mixed_forward_close_to_fp64():
+    deck, centre, wf, dtmx = setup_plate(n=32, patch=8)
+    _, g64, t64 = gradients(deck, en.FP64, 300, 50, dtmx, centre, wf)
+    _, g32, t32 = gradients(deck, en.MIXED, 300, 50, dtmx, centre, wf)
+    ej = np.abs(g32 - g64) / np.abs(g64)
+    et = np.linalg.norm(t32 - t64) / np.linalg.norm(t64)
+    print(f"FP32-forward gradient vs FP64: A,B,n {ej}, thickness L2 {et:.2e}")
+    assert ej.max() < 1e-2 and et < 1e-2
