@@ -118,7 +118,7 @@ def shell_plate_leg(n, steps, err_cycles):
     return res
 
 
-def tube_batch_leg(ntube, steps, solo_cycles):
+def tube_batch_leg(ntube, steps, solo_cycles, autotune=True):
     """256 independent square tubes (one model each, per-tube A, thickness, size and
     time step), crushed axially at 10 m/s by FIXVEL: all tubes in one batched cycle
     (autotuned launch), both forms; and, for comparison, the same tubes run one
@@ -132,7 +132,10 @@ def tube_batch_leg(ntube, steps, solo_cycles):
     for prec, name in ((en.MIXED, "mixed"), (en.FP64, "fp64")):
         s = en.ShellSolver(deck, prec)
         s.run_cycles(2)
-        best, tune_ms = s.autotune(cycles=20)
+        if autotune:
+            best, _ = s.autotune(cycles=20)
+        else:
+            best = s.launch()
         s.run_cycles(2)
         t0 = time.perf_counter()
         s.run_cycles(steps)
@@ -240,6 +243,8 @@ def main():
     ap.add_argument("--tubes", type=int, default=256)
     ap.add_argument("--tube-steps", type=int, default=3000)
     ap.add_argument("--tube-solo-cycles", type=int, default=200)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="tube leg: keep the default launch configuration")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -276,7 +281,8 @@ def main():
         shell = shell_plate_leg(args.shell_n, args.shell_steps, args.shell_err_cycles)
     tubes = None
     if not args.no_tubes:
-        tubes = tube_batch_leg(args.tubes, args.tube_steps, args.tube_solo_cycles)
+        tubes = tube_batch_leg(args.tubes, args.tube_steps, args.tube_solo_cycles,
+                               autotune=not args.no_autotune)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(deck, args.cpu_cycles)
