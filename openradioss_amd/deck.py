@@ -139,11 +139,14 @@ def shell_mat_params(young=2.1e5, nu=0.3, rho0=7.85e-9, ca=250.0, cb=600.0, cn=0
                 asrate_pm9=asrate, icc=icc, vp=vp, israte=israte, ipla=ipla)
 
 
-def shell_prop_params(npt=5, hm=0.01, hf=0.01, hr=0.01, dm=0.0, dtfac=0.9, dtmin=0.0):
-    """/PROP/SHELL TYPE1 with Ishell=1 defaults (hm_read_prop01.F:205-244): shear
-    factor 5/6 (ISHEAR=0), hourglass coefficients 0.01."""
+def shell_prop_params(npt=5, hm=0.01, hf=0.01, hr=0.01, dm=0.0, dtfac=0.9, dtmin=0.0,
+                      ishell=1, dn=0.015, cvis=1.0):
+    """/PROP/SHELL TYPE1 defaults (hm_read_prop01.F:155-244): shear factor 5/6
+    (ISHEAR=0), hourglass coefficients 0.01 for Ishell=1.  Ishell=24 (QEPH) uses
+    instead Dn (GEO(17) on input, 0.015 by default, moved to GEO(13)) and the
+    hourglass stiffness factor CVIS = 1 (GEO(17)) (hm_read_prop01.F:196-229)."""
     return dict(hm=hm, hf=hf, hr=hr, dm=dm, fsh=5.0 / 6.0, dtfac=dtfac, dtmin=dtmin,
-                ish=0, npt=npt)
+                ish=0, npt=npt, ishell=ishell, dn=dn, cvis=cvis)
 
 
 def shell_masses(x, ixc, thk, rho0):

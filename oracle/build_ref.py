@@ -1,6 +1,7 @@
 """Recipe for oracle/_ref: the reference's own brick and shell leaf kernels,
 compiled from their sources where they lie under the reference tree, plus the
-test harnesses oracle/ref/harness_brick.F and harness_shell.F, linked into
+test harnesses oracle/ref/harness_brick.F, harness_shell.F and harness_qeph.F,
+linked into
 oracle/_ref/libref.so.
 
 Test infrastructure only.  Runs in the build container (amdflang from ROCm);
@@ -56,6 +57,18 @@ LEAF = [
     "engine/source/elements/shell/coqini.F",
     "engine/source/materials/mat/mat002/sigeps02c.F",
     "engine/source/materials/mat/mat002/m2cplr.F",
+    "engine/source/elements/shell/coquez/czcorc.F",
+    "engine/source/elements/shell/coquez/czcorp5.F",
+    "engine/source/elements/shell/coquez/a3invdp_v.F",
+    "engine/source/elements/sh3n/coquedk/cdkcoor3.F",
+    "engine/source/elements/sh3n/coquedk/cncoef3.F",
+    "engine/source/elements/shell/coquez/czdef.F",
+    "engine/source/elements/shell/coquez/czstra3.F",
+    "engine/source/elements/sh3n/coquedk/cndt3.F",
+    "engine/source/elements/shell/coquez/czfintce.F",
+    "engine/source/elements/shell/coquez/czfintn.F",
+    "engine/source/elements/shell/coquez/czproj.F",
+    "engine/source/elements/shell/coque/cupdtn3.F",
 ]
 
 
@@ -119,14 +132,16 @@ def main():
         comp(os.path.join(REF, rel))
     comp(os.path.join(HERE, "ref", "harness_brick.F"))
     comp(os.path.join(HERE, "ref", "harness_shell.F"))
+    comp(os.path.join(HERE, "ref", "harness_qeph.F"))
     lib = os.path.join(OUT, "libref.so")
     # Routines referenced on branches the path does not take (error messages,
     # SPH, orthotropic frames) are compiled from their own reference sources too,
     # until the library has no unresolved Fortran symbol.
     # ANCMSG (message catalogue) and ARRET (abort, restart flush) are only reached
     # on error branches and pull in the whole I/O layer, including C++ sources that
-    # need headers absent from the image; their references are left weak, so a
-    # call would fault loudly rather than run a substitute.
+    # need headers absent from the image; they and the routines of ERROR_PATH below
+    # that no harness reaches are left weak, so a call would fault loudly rather
+    # than run a substitute.
     subs = subroutine_index()
     for _ in range(32):
         subprocess.run(["amdflang", "-shared", "-o", lib] + objs, check=True)
@@ -152,7 +167,12 @@ def main():
     return 0
 
 
-ERROR_PATH = ("ancmsg", "arret")
+ERROR_PATH = ("ancmsg", "arret",
+              # Called only by CNCOEF3 (sh3n/coquedk/cncoef3.F:610-939), the
+              # coefficient routine of the DKT/QBAT shells, which shares its source
+              # file with CNCOEF3B; their own file (coqueba/cmatc3.F) pulls in the
+              # implicit solver's initialisation and from there the whole engine.
+              "gepm_lc", "cctoglob")
 
 
 def subroutine_index():

@@ -127,6 +127,10 @@ typedef struct {
   double dtfac, dtmin;            /* DTFAC1(3), DTMIN1(3)                     */
   int    ish;                     /* GEO(37) ISHEAR                           */
   int    npt;                     /* through-thickness points, 2..5           */
+  int    ishell;                  /* 1: Belytschko-Tsay (CFORC3), 24: QEPH (CZFORC3) */
+  int    pad;
+  double dn;                      /* QEPH: GEO(13) = Dn numerical damping (0.015) */
+  double cvis;                    /* QEPH: GEO(17) = hourglass stiffness factor (1) */
 } orad_shprop_t;
 
 #define ORAD_SH_NPT_MAX 5
@@ -141,6 +145,7 @@ typedef struct {
   double sig[ORAD_SH_NPT_MAX][5]; /* LBUF%SIG per point                         */
   double pla[ORAD_SH_NPT_MAX];    /* LBUF%PLA                                   */
   double epsdp[ORAD_SH_NPT_MAX];  /* LBUF%EPSD                                  */
+  double hourx[7];                /* QEPH: GBUF%HOURG(6:12) (HOURG(1:5) in hour) */
 } orad_shstate_t;
 
 typedef struct {
@@ -150,6 +155,7 @@ typedef struct {
   double ehou;                    /* hourglass energy increment                 */
   int    alive;                   /* dt enters DT2T (OFFG > 0 and OFF != 0)     */
   int    pad;
+  double stic[4];                 /* FSKY(7) per corner (QEPH: STI*FACN)         */
 } orad_shout_t;
 
 /* CFORC3 chain for one element (engine/source/elements/shell/coque/cforc3.F:398-746).
@@ -158,6 +164,13 @@ void oracle_shell_element(const double xg[3][4], const double vg[3][4], const do
                           double thke, int tri, orad_shstate_t *st, double dt1,
                           const orad_shmat_t *mat, const orad_shprop_t *prop,
                           orad_shout_t *out);
+
+/* CZFORC3 chain (QEPH, Ishell=24) for one quadrilateral
+ * (engine/source/elements/shell/coquez/czforc3.F:436-808). */
+void oracle_qeph_element(const double xg[3][4], const double vg[3][4], const double vrg[3][4],
+                         double thke, orad_shstate_t *st, double dt1,
+                         const orad_shmat_t *mat, const orad_shprop_t *prop,
+                         orad_shout_t *out);
 
 /* COQINI tables for npt points (single-precision DATA values widened). */
 void oracle_shell_tables(int npt, double *z0, double *wf, double *wm);

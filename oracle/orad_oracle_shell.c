@@ -49,6 +49,9 @@ typedef struct {
   double ONEP4, ONEP41, ONEP414;
   double ONE_OVER_9, ONE_OVER_12, FOUR_OVER_3, FOUR_OVER_5, FIVE_OVER_6, THIRD;
   double THREE_OVER_4, THREE_HALF, SR2D2;
+  /* QEPH (CZFORC3 chain) */
+  double EM08, EM10, EM18, FIVE_OVER_4, ONE_OVER_16, ONE_OVER_64, ZEP85, ZEP999, FIVEP333;
+  double THREEP464, ZEP9EP30;
 } shconst_t;
 
 static shconst_t K;
@@ -100,6 +103,24 @@ static void init_constants(void) {
   K.THREE_OVER_4 = THREE / 4.0;
   K.THREE_HALF = THREE / TWO;
   K.SR2D2 = sqrt(TWO) * HALF;               /* chvis3.F:118 */
+  {
+    double EP29 = EP20 * EP09;
+    K.EM08 = ONE / EP08;
+    K.EM10 = ONE / EP10;
+    K.EM18 = ONE / EP18;
+    K.FIVE_OVER_4 = 5.0 / 4.0;
+    K.ONE_OVER_16 = ONE / 16.0;
+    K.ONE_OVER_64 = ONE / 64.0;
+    double ZEP05 = 5.0 / EP02;
+    double ZEP15 = K.ZEP1 + ZEP05;
+    K.ZEP85 = ONE - ZEP15;
+    K.ZEP999 = ONE - K.EM03;
+    K.FIVEP333 = 16.0 / THREE;
+    double ZEP04 = 4.0 / EP02;
+    double ZEP06 = K.ZEP1 - ZEP04;
+    K.THREEP464 = THREE + K.ZEP4 + ZEP06 + K.FOUREM3;
+    K.ZEP9EP30 = 9.0 * EP29;
+  }
   k_ready = 1;
 }
 
@@ -155,7 +176,7 @@ static void sigeps02c(const orad_shmat_t *m, double gs, double dt1, double asrat
                       double epsd_pg, double off_old, double *off, double *pla,
                       double *epsdp, double *epchk, double *sigy, double *thkn,
                       double thklyl, int npt, const double deps[5], const double sigo[5],
-                      double sign[5]) {
+                      double sign[5], double *etse_out) {
   const double young = m->young, g = m->shear, nu = m->nu;
   const double a11 = young / (ONE - nu * nu);
   const double a12 = a11 * nu;
@@ -286,7 +307,7 @@ static void sigeps02c(const orad_shmat_t *m, double gs, double dt1, double asrat
       ezz = dpla * HALF * (sxx + syy) / yld;
     }
   }
-  (void)etse;
+  if (etse_out) *etse_out = etse;
   /* ---- back in SIGEPS02C (sigeps02c.F:176-213) ---- */
   if (m->vp == 1) {
     double epsdot_p = dpla / dmax(K.EM20, dt1);
@@ -495,7 +516,7 @@ void oracle_shell_element(const double xg[3][4], const double vg[3][4], const do
     deps[4] = EXZ;
     double sign[5];
     sigeps02c(m, GS, dt1, asrate, epsd_pg, off_old, &off, &st->pla[it], &st->epsdp[it],
-              &epchk, &sigy, &thkn, thklyl, npt, deps, st->sig[it], sign);
+              &epchk, &sigy, &thkn, thklyl, npt, deps, st->sig[it], sign, NULL);
     viscmx = dmax(p->dm, viscmx);
     for (int c = 0; c < 5; ++c) st->sig[it][c] = sign[c] * ONE;
     for (int c = 0; c < 5; ++c) FOR[c] = FOR[c] + wf[it] * sign[c];
@@ -655,7 +676,726 @@ void oracle_shell_element(const double xg[3][4], const double vg[3][4], const do
     for (int k = 0; k < 4; ++k) { out->f[d][k] = -F[d][k]; out->m[d][k] = -M[d][k]; }
   out->sti = sti;
   out->stir = stir;
+  for (int c = 0; c < 4; ++c) out->stic[c] = sti;
   out->ehou = EHOU;
+}
+
+/* ====================================================================== */
+/* QEPH shell (Ishell=24): CZFORC3 chain for one quadrilateral            */
+/* (engine/source/elements/shell/coquez/czforc3.F:436-808) with IGTYP=1,  */
+/* law 2, IORTH=0, ISROT=0, ISMSTR=2, IMPL_S=0, IVECTOR=0, IRESP=0,       */
+/* ISIGI=0, ISTRAIN=0, NPT>1.                                             */
+/* ====================================================================== */
+void oracle_qeph_element(const double xg[3][4], const double vg[3][4], const double vrg[3][4],
+                         double thke, orad_shstate_t *st, double dt1,
+                         const orad_shmat_t *m, const orad_shprop_t *p, orad_shout_t *out) {
+  init_constants();
+  const double EM20 = K.EM20, FOUR = 4.0;
+  /* ---- CZCORC1 (czcorc.F:37-618), IXFEM = 0 ---- */
+  const double TOL = K.EM08;                       /* IRESP = 0 (czcorc.F:133-136) */
+  double RX = xg[0][1] + xg[0][2] - xg[0][0] - xg[0][3];
+  double SX = xg[0][2] + xg[0][3] - xg[0][0] - xg[0][1];
+  double RY = xg[1][1] + xg[1][2] - xg[1][0] - xg[1][3];
+  double SY = xg[1][2] + xg[1][3] - xg[1][0] - xg[1][1];
+  double RZ = xg[2][1] + xg[2][2] - xg[2][0] - xg[2][3];
+  double SZ = xg[2][2] + xg[2][3] - xg[2][0] - xg[2][1];
+  /* CLSKEW3, IREP = 0 (cdkcoor3.F:303-368) */
+  double E3X = RY * SZ - RZ * SY, E3Y = RZ * SX - RX * SZ, E3Z = RX * SY - RY * SX;
+  double DET = sqrt(E3X * E3X + E3Y * E3Y + E3Z * E3Z);
+  if (DET < EM20 && st->offg != ZERO) st->offg = ZERO;
+  {
+    double off_loc = (fabs(st->offg) != ZERO) ? ONE : ZERO;
+    DET = dmax(EM20, DET);
+    double CC = off_loc / DET;
+    CC = dmax(CC, EM20);
+    E3X = E3X * CC; E3Y = E3Y * CC; E3Z = E3Z * CC;
+  }
+  double E1X, E1Y, E1Z;
+  {
+    double C1C1 = RX * RX + RY * RY + RZ * RZ;
+    double C2C2 = SX * SX + SY * SY + SZ * SZ;
+    double c2_1 = 0.0, c1_1 = 0.0;
+    if (C1C1 != ZERO) {
+      c2_1 = sqrt(C2C2 / dmax(EM20, C1C1));
+      c1_1 = ONE;
+    } else if (C2C2 != ZERO) {
+      c2_1 = ONE;
+      c1_1 = sqrt(C1C1 / dmax(EM20, C2C2));
+    }
+    E1X = RX * c2_1 + (SY * E3Z - SZ * E3Y) * c1_1;
+    E1Y = RY * c2_1 + (SZ * E3X - SX * E3Z) * c1_1;
+    E1Z = RZ * c2_1 + (SX * E3Y - SY * E3X) * c1_1;
+    double C1 = sqrt(E1X * E1X + E1Y * E1Y + E1Z * E1Z);
+    if (C1 != ZERO) C1 = ONE / dmax(EM20, C1);
+    E1X = E1X * C1; E1Y = E1Y * C1; E1Z = E1Z * C1;
+  }
+  double E2X = E3Y * E1Z - E3Z * E1Y, E2Y = E3Z * E1X - E3X * E1Z, E2Z = E3X * E1Y - E3Y * E1X;
+  /* VQ(:,1) = E1, VQ(:,2) = E2, VQ(:,3) = E3 */
+  const double e1[3] = {E1X, E1Y, E1Z}, e2[3] = {E2X, E2Y, E2Z}, e3[3] = {E3X, E3Y, E3Z};
+  double AREA = FOURTH * DET;
+  double AREA_I;
+  {
+    double off_loc = (fabs(st->offg) != ZERO) ? ONE : ZERO;
+    AREA_I = off_loc / AREA;
+    AREA_I = dmax(AREA_I, EM20);
+  }
+  double XL2, YL2, XL3, YL3, XL4, YL4, Z1;
+  {
+    double L0[3];
+    for (int d = 0; d < 3; ++d) L0[d] = FOURTH * (xg[d][2] + xg[d][3] + xg[d][0] + xg[d][1]);
+    double XX = xg[0][1] - xg[0][0], YY = xg[1][1] - xg[1][0], ZZ = xg[2][1] - xg[2][0];
+    XL2 = E1X * XX + E1Y * YY + E1Z * ZZ;
+    YL2 = E2X * XX + E2Y * YY + E2Z * ZZ;
+    XX = xg[0][0] - L0[0]; YY = xg[1][0] - L0[1]; ZZ = xg[2][0] - L0[2];
+    Z1 = E3X * XX + E3Y * YY + E3Z * ZZ;
+    XX = xg[0][2] - xg[0][0]; YY = xg[1][2] - xg[1][0]; ZZ = xg[2][2] - xg[2][0];
+    XL3 = E1X * XX + E1Y * YY + E1Z * ZZ;
+    YL3 = E2X * XX + E2Y * YY + E2Z * ZZ;
+    XX = xg[0][3] - xg[0][0]; YY = xg[1][3] - xg[1][0]; ZZ = xg[2][3] - xg[2][0];
+    XL4 = E1X * XX + E1Y * YY + E1Z * ZZ;
+    YL4 = E2X * XX + E2Y * YY + E2Z * ZZ;
+  }
+  /* ISMSTR = 2: the small-strain reference (SMSTR) is stored but only read
+   * after a switch to small strain (OFFG = 2), which needs /DT/SHELL/CST. */
+  double COREL[2][4];
+  {
+    double lx = FOURTH * (XL2 + XL3 + XL4), ly = FOURTH * (YL2 + YL3 + YL4);
+    COREL[0][0] = -lx; COREL[0][1] = XL2 - lx; COREL[0][2] = XL3 - lx; COREL[0][3] = XL4 - lx;
+    COREL[1][0] = -ly; COREL[1][1] = YL2 - ly; COREL[1][2] = YL3 - ly; COREL[1][3] = YL4 - ly;
+  }
+  const double X13 = (COREL[0][0] - COREL[0][2]) * HALF, X24 = (COREL[0][1] - COREL[0][3]) * HALF;
+  const double Y13 = (COREL[1][0] - COREL[1][2]) * HALF, Y24 = (COREL[1][1] - COREL[1][3]) * HALF;
+  const double MX13 = (COREL[0][0] + COREL[0][2]) * HALF, MX23 = (COREL[0][1] + COREL[0][2]) * HALF;
+  const double MX34 = (COREL[0][2] + COREL[0][3]) * HALF;
+  const double MY13 = (COREL[1][0] + COREL[1][2]) * HALF, MY23 = (COREL[1][1] + COREL[1][2]) * HALF;
+  const double MY34 = (COREL[1][2] + COREL[1][3]) * HALF;
+  const double L13 = X13 * X13 + Y13 * Y13, L24 = X24 * X24 + Y24 * Y24;
+  double HS;
+  {
+    double C1 = COREL[0][1] * COREL[1][3] - COREL[1][1] * COREL[0][3];
+    double C2 = COREL[0][0] * COREL[1][2] - COREL[1][0] * COREL[0][2];
+    HS = dmax(fabs(C1), fabs(C2)) * AREA_I;
+  }
+  double LL, LM, FACN[2];
+  {
+    const double FACDT = K.FIVE_OVER_4;             /* IDT1SH = 0 */
+    double rx = XL2 + XL3 - XL4, ry = YL2 + YL3 - YL4;
+    double sx = -XL2 + XL3 + XL4, sy = -YL2 + YL3 + YL4;
+    double C1 = sqrt(rx * rx + ry * ry), C2 = sqrt(sx * sx + sy * sy);
+    double S1 = FOURTH * (dmax(C1, C2) / dmin(C1, C2) - ONE);
+    double FAC1 = dmin(HALF, S1) + ONE;
+    double FAC2 = FOUR * AREA / (C1 * C2);
+    /* default-real literals 3.413, 0.7071, 0.78, 0.22 (czcorc.F:393-394) */
+    FAC2 = (double)3.413f * dmax(ZERO, FAC2 - (double)0.7071f);
+    FAC2 = (double)0.78f + (double)0.22f * FAC2 * FAC2 * FAC2;
+    double FACI = TWO * FAC1 * FAC2;
+    LL = dmax(L13, L24);
+    LM = HALF * (L13 + L24);
+    FACN[0] = sqrt(L24 / LL);
+    FACN[1] = sqrt(L13 / LL);
+    S1 = sqrt(FACI * (FACDT + HS) * LL);
+    S1 = dmax(S1, K.EM10);
+    LL = AREA / S1;
+  }
+  double RL[2][4];                                  /* RLXYZ(1:2, corner) */
+  for (int c = 0; c < 4; ++c) {
+    RL[0][c] = e1[0] * vrg[0][c] + e1[1] * vrg[1][c] + e1[2] * vrg[2][c];
+    RL[1][c] = e2[0] * vrg[0][c] + e2[1] * vrg[1][c] + e2[2] * vrg[2][c];
+  }
+  double V13[3], V24[3], VHI[3];
+  {
+    double G13[3], G24[3], GHI[3];
+    for (int d = 0; d < 3; ++d) {
+      G13[d] = vg[d][0] - vg[d][2];
+      G24[d] = vg[d][1] - vg[d][3];
+      GHI[d] = vg[d][0] - vg[d][1] + vg[d][2] - vg[d][3];
+    }
+    const double *E[3] = {e1, e2, e3};
+    for (int k = 0; k < 3; ++k) {
+      V13[k] = (E[k][0] * G13[0] + E[k][1] * G13[1] + E[k][2] * G13[2]);
+      V24[k] = (E[k][0] * G24[0] + E[k][1] * G24[1] + E[k][2] * G24[2]);
+      VHI[k] = (E[k][0] * GHI[0] + E[k][1] * GHI[1] + E[k][2] * GHI[2]);
+    }
+  }
+  {                                                 /* IMPL_S = 0 (czcorc.F:452-478) */
+    double DT05 = HALF * dt1, DT025 = FOURTH * dt1;
+    double EXZ = Y24 * V13[2] - Y13 * V24[2];
+    double EYZ = -X24 * V13[2] + X13 * V24[2];
+    double DDRY = DT05 * EXZ * AREA_I;
+    double DDRX = DT05 * EYZ * AREA_I;
+    double V13X = V13[0], V24X = V24[0], VHIX = VHI[0];
+    double DDRZ1, DDRZ2;
+    if (fabs(X13 - X24) < K.EM10) DDRZ1 = ZERO;
+    else DDRZ1 = DT025 * (V13[1] - V24[1]) / (X13 - X24);
+    V13[0] = V13[0] - DDRY * V13[2] - DDRZ1 * V13[1];
+    V24[0] = V24[0] - DDRY * V24[2] - DDRZ1 * V24[1];
+    VHI[0] = VHI[0] - DDRY * VHI[2] - DDRZ1 * VHI[1];
+    if (fabs(Y13 + Y24) < K.EM10) DDRZ2 = ZERO;
+    else DDRZ2 = DT025 * (V13X + V24X) / (Y13 + Y24);
+    V13[1] = V13[1] - DDRX * V13[2] - DDRZ2 * V13X;
+    V24[1] = V24[1] - DDRX * V24[2] - DDRZ2 * V24X;
+    VHI[1] = VHI[1] - DDRX * VHI[2] - DDRZ2 * VHIX;
+  }
+  /* CZCORP5 (czcorp5.F:32-396), warped element projection.  Its flatness test
+   * compares Z1^2 with LM = (L13+L24)/2, which CZCORC1 passes in the LL slot
+   * (czcorc.F:370,565-571). */
+  int plat;
+  double VQN[3][4] = {{0}}, DI[6] = {0}, DB[3][4] = {{0}};
+  {
+    double Z2 = Z1 * Z1;
+    if (Z2 < LM * TOL || p->npt == 1) {
+      Z1 = ZERO;
+      plat = 1;
+    } else {
+      plat = 0;
+      double A_4 = AREA * FOURTH;
+      double SZ1 = MX13 * Y24 - MY13 * X24;
+      double SZ2 = A_4 + SZ1;
+      double SZs = Z2 * L24;
+      double SL = ONE / sqrt(SZs + SZ2 * SZ2);
+      VQN[0][0] = -Z1 * Y24;
+      VQN[1][0] = Z1 * X24;
+      VQN[2][0] = SZ2 * SL;
+      VQN[0][2] = -VQN[0][0];
+      VQN[1][2] = -VQN[1][0];
+      VQN[0][0] = VQN[0][0] * SL;
+      VQN[1][0] = VQN[1][0] * SL;
+      SZ2 = A_4 - SZ1;
+      SL = ONE / sqrt(SZs + SZ2 * SZ2);
+      VQN[0][2] = VQN[0][2] * SL;
+      VQN[1][2] = VQN[1][2] * SL;
+      VQN[2][2] = SZ2 * SL;
+      SZ1 = MX13 * Y13 - MY13 * X13;
+      SZ2 = A_4 + SZ1;
+      SZs = Z2 * L13;
+      SL = ONE / sqrt(SZs + SZ2 * SZ2);
+      VQN[0][1] = -Z1 * Y13;
+      VQN[1][1] = Z1 * X13;
+      VQN[2][1] = SZ2 * SL;
+      VQN[0][3] = -VQN[0][1];
+      VQN[1][3] = -VQN[1][1];
+      VQN[0][1] = VQN[0][1] * SL;
+      VQN[1][1] = VQN[1][1] * SL;
+      SZ2 = A_4 - SZ1;
+      SL = ONE / sqrt(SZs + SZ2 * SZ2);
+      VQN[0][3] = VQN[0][3] * SL;
+      VQN[1][3] = VQN[1][3] * SL;
+      VQN[2][3] = SZ2 * SL;
+      double RR[3][4];
+      for (int c = 0; c < 4; ++c) {
+        RR[0][c] = RL[0][c];
+        RR[1][c] = RL[1][c];
+        RR[2][c] = e3[0] * vrg[0][c] + e3[1] * vrg[1][c] + e3[2] * vrg[2][c];
+      }
+      double AR[3], AD[4];
+      AR[0] = -Z1 * VHI[1] + Y13 * V13[2] + Y24 * V24[2] + MY13 * VHI[2] + RR[0][0] + RR[0][1] +
+              RR[0][2] + RR[0][3];
+      AR[1] = Z1 * VHI[0] - X13 * V13[2] - X24 * V24[2] - MX13 * VHI[2] + RR[1][0] + RR[1][1] +
+              RR[1][2] + RR[1][3];
+      AR[2] = X13 * V13[1] + X24 * V24[1] + MX13 * VHI[1] - Y13 * V13[0] - Y24 * V24[0] -
+              MY13 * VHI[0] + RR[2][0] + RR[2][1] + RR[2][2] + RR[2][3];
+      for (int c = 0; c < 4; ++c)
+        AD[c] = VQN[0][c] * RR[0][c] + VQN[1][c] * RR[1][c] + VQN[2][c] * RR[2][c];
+      double XX = COREL[0][0] * COREL[0][0] + COREL[0][1] * COREL[0][1] +
+                  COREL[0][2] * COREL[0][2] + COREL[0][3] * COREL[0][3];
+      double YY = COREL[1][0] * COREL[1][0] + COREL[1][1] * COREL[1][1] +
+                  COREL[1][2] * COREL[1][2] + COREL[1][3] * COREL[1][3];
+      double XY = COREL[0][0] * COREL[1][0] + COREL[0][1] * COREL[1][1] +
+                  COREL[0][2] * COREL[1][2] + COREL[0][3] * COREL[1][3];
+      double XZ = (COREL[0][0] - COREL[0][1] + COREL[0][2] - COREL[0][3]) * Z1;
+      double YZ = (COREL[1][0] - COREL[1][1] + COREL[1][2] - COREL[1][3]) * Z1;
+      double ZZ = FOUR * Z2;
+      double BTB[6];
+      BTB[0] = VQN[0][0] * VQN[0][0] + VQN[0][1] * VQN[0][1] + VQN[0][2] * VQN[0][2] + VQN[0][3] * VQN[0][3];
+      BTB[1] = VQN[1][0] * VQN[1][0] + VQN[1][1] * VQN[1][1] + VQN[1][2] * VQN[1][2] + VQN[1][3] * VQN[1][3];
+      BTB[2] = VQN[2][0] * VQN[2][0] + VQN[2][1] * VQN[2][1] + VQN[2][2] * VQN[2][2] + VQN[2][3] * VQN[2][3];
+      BTB[3] = VQN[0][0] * VQN[1][0] + VQN[0][1] * VQN[1][1] + VQN[0][2] * VQN[1][2] + VQN[0][3] * VQN[1][3];
+      BTB[4] = VQN[0][0] * VQN[2][0] + VQN[0][1] * VQN[2][1] + VQN[0][2] * VQN[2][2] + VQN[0][3] * VQN[2][3];
+      BTB[5] = VQN[1][0] * VQN[2][0] + VQN[1][1] * VQN[2][1] + VQN[1][2] * VQN[2][2] + VQN[1][3] * VQN[2][3];
+      double D[6];
+      D[0] = YY + ZZ + FOUR - BTB[0];
+      D[1] = XX + ZZ + FOUR - BTB[1];
+      D[2] = XX + YY + FOUR - BTB[2];
+      D[3] = -XY - BTB[3];
+      D[4] = -XZ - BTB[4];
+      D[5] = -YZ - BTB[5];
+      {                                              /* IRESP = 0 inverse */
+        double ABC = D[0] * D[1] * D[2];
+        double XXYZ2 = D[0] * D[5] * D[5];
+        double YYXZ2 = D[1] * D[4] * D[4];
+        double ZZXY2 = D[2] * D[3] * D[3];
+        double DETA = fabs(ABC + TWO * D[3] * D[4] * D[5] - XXYZ2 - YYXZ2 - ZZXY2);
+        DETA = ONE / dmax(DETA, EM20);
+        DI[0] = (ABC - XXYZ2) * DETA / dmax(D[0], EM20);
+        DI[1] = (ABC - YYXZ2) * DETA / dmax(D[1], EM20);
+        DI[2] = (ABC - ZZXY2) * DETA / dmax(D[2], EM20);
+        DI[3] = (D[4] * D[5] - D[3] * D[2]) * DETA;
+        DI[4] = (D[5] * D[3] - D[4] * D[1]) * DETA;
+        DI[5] = (D[3] * D[4] - D[5] * D[0]) * DETA;
+      }
+      for (int c = 0; c < 4; ++c) {
+        DB[0][c] = DI[0] * VQN[0][c] + DI[3] * VQN[1][c] + DI[4] * VQN[2][c];
+        DB[1][c] = DI[3] * VQN[0][c] + DI[1] * VQN[1][c] + DI[5] * VQN[2][c];
+        DB[2][c] = DI[4] * VQN[0][c] + DI[5] * VQN[1][c] + DI[2] * VQN[2][c];
+      }
+      double DBAD[3], ALR[3], ALD[4];
+      for (int k = 0; k < 3; ++k)
+        DBAD[k] = DB[k][0] * AD[0] + DB[k][1] * AD[1] + DB[k][2] * AD[2] + DB[k][3] * AD[3];
+      ALR[0] = DI[0] * AR[0] + DI[3] * AR[1] + DI[4] * AR[2] - DBAD[0];
+      ALR[1] = DI[3] * AR[0] + DI[1] * AR[1] + DI[5] * AR[2] - DBAD[1];
+      ALR[2] = DI[4] * AR[0] + DI[5] * AR[1] + DI[2] * AR[2] - DBAD[2];
+      for (int c = 0; c < 4; ++c)
+        ALD[c] = AD[c] + VQN[0][c] * DBAD[0] + VQN[1][c] * DBAD[1] + VQN[2][c] * DBAD[2] -
+                 DB[0][c] * AR[0] - DB[1][c] * AR[1] - DB[2][c] * AR[2];
+      double C1 = TWO * ALR[2];
+      V13[0] = V13[0] + C1 * Y13;
+      V24[0] = V24[0] + C1 * Y24;
+      VHI[0] = VHI[0] + FOUR * (ALR[2] * MY13 - Z1 * ALR[1]);
+      V13[1] = V13[1] - C1 * X13;
+      V24[1] = V24[1] - C1 * X24;
+      VHI[1] = VHI[1] - FOUR * (ALR[2] * MX13 - Z1 * ALR[0]);
+      V13[2] = V13[2] - TWO * (Y13 * ALR[0] - X13 * ALR[1]);
+      V24[2] = V24[2] - TWO * (Y24 * ALR[0] - X24 * ALR[1]);
+      VHI[2] = VHI[2] + FOUR * (MX13 * ALR[1] - MY13 * ALR[0]);
+      for (int c = 0; c < 4; ++c) {
+        RL[0][c] = RR[0][c] - ALR[0] - VQN[0][c] * ALD[c];
+        RL[1][c] = RR[1][c] - ALR[1] - VQN[1][c] * ALD[c];
+      }
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    V13[k] = V13[k] * AREA_I;
+    V24[k] = V24[k] * AREA_I;
+    VHI[k] = VHI[k] * FOURTH;
+  }
+  /* ---- CNCOEF3B (cncoef3.F:29-293), IGTYP = 1, law 2, ITHK = 0 ---- */
+  const double thk0 = thke, thk02 = thk0 * thk0, VOL0 = thk0 * AREA, dt1c = dt1;
+  const double RHO = m->rho0, YM = m->young, NU = m->nu, G = m->shear;
+  const double A11 = m->a11, A12 = m->a12, SSP = m->ssp;
+  const double GSR = sqrt(fabs(G)), A11SR = sqrt(fabs(A11)), A12SR = sqrt(fabs(A12));
+  (void)NU;                                         /* resol_init.F:689-696 */
+  const double SHF = p->fsh, SHFSR = sqrt(p->fsh);
+  const double GS = G * SHF;
+  /* AMU = Dn = GEO(13), 0.015 when unset (cncoef3.F:259-264; FIVEEM3 = 5/EP03) */
+  const double AMU = (p->dn == ZERO) ? K.ZEP01 + 5.0 / 1.0e3 : p->dn;
+  /* ---- CZDEF (czdef.F:29-209) ---- */
+  double VDEF[8], DHG[6];
+  double off = dmin(ONE, fabs(st->offg));
+  {
+    const double AA = AREA_I;
+    double R13[2], R24[2], RSOM[2], RHI[2];
+    for (int k = 0; k < 2; ++k) {
+      R13[k] = (RL[k][0] - RL[k][2]) * AA;
+      R24[k] = (RL[k][1] - RL[k][3]) * AA;
+      RSOM[k] = (RL[k][3] + RL[k][2] + RL[k][0] + RL[k][1]) * AA;
+      RHI[k] = (RL[k][0] - RL[k][1] + RL[k][2] - RL[k][3]) * FOURTH;
+    }
+    VDEF[0] = Y24 * V13[0] - Y13 * V24[0];
+    VDEF[1] = -X24 * V13[1] + X13 * V24[1];
+    double BXV2 = Y24 * V13[1] - Y13 * V24[1];
+    double BYV1 = -X24 * V13[0] + X13 * V24[0];
+    VDEF[2] = BXV2 + BYV1;
+    VDEF[5] = Y24 * R13[1] - Y13 * R24[1];
+    VDEF[6] = X24 * R13[0] - X13 * R24[0];
+    double BXR1 = Y13 * R24[0] - Y24 * R13[0];
+    double BYR2 = -X24 * R13[1] + X13 * R24[1];
+    VDEF[7] = BXR1 + BYR2;
+    double BCXY = AREA * FOURTH;
+    double BCX = V13[2] - MY13 * R13[0] + MX13 * R13[1];
+    double BCY = V24[2] + MY13 * R24[0] - MX13 * R24[1];
+    VDEF[3] = Y24 * BCX - Y13 * BCY + BCXY * RSOM[1];
+    VDEF[4] = X13 * BCY - X24 * BCX - BCXY * RSOM[0];
+    DHG[0] = VHI[0] - MX13 * VDEF[0] - MY13 * BYV1;
+    DHG[1] = VHI[1] - MX13 * BXV2 - MY13 * VDEF[1];
+    DHG[2] = RHI[1] - MX13 * VDEF[5] - MY13 * BYR2;
+    DHG[3] = -RHI[0] - MX13 * BXR1 - MY13 * VDEF[6];
+    DHG[4] = (VHI[2] * 4. - (MY13 * RSOM[0] - MY23 * (R13[0] + R24[0]) + MX23 * (R13[1] + R24[1]) -
+                             MX13 * RSOM[1]) * AREA) * FOUR;
+    DHG[5] = (VHI[2] * 4. - (MY13 * RSOM[0] - MY34 * (R13[0] - R24[0]) + MX34 * (R13[1] - R24[1]) -
+                             MX13 * RSOM[1]) * AREA) * FOUR;
+    DHG[0] = DHG[0] + (Y24 * V13[2] - Y13 * V24[2]) * Z1;
+    DHG[1] = DHG[1] + (-X24 * V13[2] + X13 * V24[2]) * Z1;
+    double DETA1 = Z1 * FOUR * AA;
+    VDEF[5] = VDEF[5] + (X13 * V13[0] - X24 * V24[0]) * DETA1;
+    VDEF[6] = VDEF[6] + (Y13 * V13[1] - Y24 * V24[1]) * DETA1;
+    VDEF[7] = VDEF[7] + (X13 * V13[1] - X24 * V24[1] + Y13 * V13[0] - Y24 * V24[0]) * DETA1;
+    if (st->offg < ZERO) {
+      for (int k = 0; k < 8; ++k) VDEF[k] = ZERO;
+      for (int k = 0; k < 6; ++k) DHG[k] = ZERO;
+    }
+  }
+  /* ---- CZSTRA3 (czstra3.F), ISTRAIN = 0 ---- */
+  const double EXX = VDEF[0] * dt1c, EYY = VDEF[1] * dt1c, EXY = VDEF[2] * dt1c;
+  const double EYZ = VDEF[4] * dt1c, EXZ = VDEF[3] * dt1c;
+  const double KXX = VDEF[5] * dt1c, KYY = VDEF[6] * dt1c, KXY = VDEF[7] * dt1c;
+  /* ---- element equivalent strain rate (czforc3.F:554-561) ---- */
+  const double dtinv = dt1 / dmax(dt1 * dt1, EM20);
+  double epsd_pg;
+  {
+    double eps_k2 = (KXX * KXX + KYY * KYY + KXX * KYY + FOURTH * (KXY * KXY)) * K.ONE_OVER_9 *
+                    (st->thk * st->thk);
+    double eps_m2 = K.FOUR_OVER_3 * (EXX * EXX + EYY * EYY + EXX * EYY + FOURTH * (EXY * EXY));
+    epsd_pg = sqrt(eps_k2 + eps_m2) * dtinv;
+    st->epsd = ONE * epsd_pg + (ONE - ONE) * st->epsd;
+  }
+  /* ---- CMAIN3 -> MULAWC, IGTYP = 1, law 2 with the QEPH factors (flag_zcfac,
+   * mulawc.F90:509-510,565-569,2787-2794) ---- */
+  double degmb = st->forc[0] * EXX + st->forc[1] * EYY + st->forc[2] * EXY +
+                 st->forc[3] * EYZ + st->forc[4] * EXZ;
+  double degfx = st->mom[0] * KXX + st->mom[1] * KYY + st->mom[2] * KXY;
+  const double vol0 = AREA * thk0;
+  double thkn = st->thk;
+  double FOR[5] = {ZERO, ZERO, ZERO, ZERO, ZERO}, MOM[3] = {ZERO, ZERO, ZERO};
+  double sigy = ZERO, zcfac1 = ZERO, zcfac2 = ONE;
+  const double off_old = off;
+  double epchk = ZERO;
+  double viscmx = ZERO;
+  const double asrate = (m->israte > 0) ? dmin(ONE, m->asrate_pm9 * dt1) : ONE;
+  const int npt = p->npt;
+  double z0[5], wf[5], wm[5];
+  oracle_shell_tables(npt, z0, wf, wm);
+  for (int it = 0; it < npt; ++it) {
+    double thklyl = wf[it] * thk0;
+    double wmc = wm[it];
+    double zt = z0[it] * thk0;
+    double deps[5];
+    deps[0] = EXX + zt * KXX;
+    deps[1] = EYY + zt * KYY;
+    deps[2] = EXY + zt * KXY;
+    deps[3] = EYZ;
+    deps[4] = EXZ;
+    double sign[5], etse = ONE;
+    sigeps02c(m, GS, dt1, asrate, epsd_pg, off_old, &off, &st->pla[it], &st->epsdp[it],
+              &epchk, &sigy, &thkn, thklyl, npt, deps, st->sig[it], sign, &etse);
+    viscmx = dmax(p->dm, viscmx);
+    for (int c = 0; c < 5; ++c) st->sig[it][c] = sign[c] * ONE;
+    for (int c = 0; c < 5; ++c) FOR[c] = FOR[c] + wf[it] * sign[c];
+    for (int c = 0; c < 3; ++c) MOM[c] = MOM[c] + wmc * sign[c];
+    zcfac1 = zcfac1 + etse / npt;
+    zcfac2 = dmin(etse, zcfac2);
+  }
+  st->thk = dmax(thkn, K.EM30);
+  {
+    double fact = K.ONEP414 * p->dm;
+    double visc = fact * SSP * sqrt(AREA) * dtinv * RHO;
+    FOR[0] = FOR[0] + visc * (EXX + HALF * EYY);
+    FOR[1] = FOR[1] + visc * (EYY + HALF * EXX);
+    FOR[2] = FOR[2] + visc * EXY * K.THIRD;
+  }
+  for (int c = 0; c < 5; ++c) FOR[c] = FOR[c] * off * ONE;
+  for (int c = 0; c < 3; ++c) MOM[c] = MOM[c] * off * ONE;
+  degmb = degmb + FOR[0] * EXX + FOR[1] * EYY + FOR[2] * EXY + FOR[3] * EYZ + FOR[4] * EXZ;
+  degfx = degfx + MOM[0] * KXX + MOM[1] * KYY + MOM[2] * KXY;
+  {
+    double vol2 = HALF * vol0;
+    st->eint[0] = st->eint[0] + degmb * vol2;
+    st->eint[1] = st->eint[1] + degfx * thk0 * vol2;
+  }
+  for (int c = 0; c < 5; ++c) st->forc[c] = FOR[c];
+  for (int c = 0; c < 3; ++c) st->mom[c] = MOM[c];
+  /* ---- CNDT3 (sh3n/coquedk/cndt3.F:84-300), NODADT = 0, IDTMINS = 0 ---- */
+  double ALDT = LL;
+  viscmx = dmax(viscmx, AMU);
+  viscmx = sqrt(ONE + viscmx * viscmx) - viscmx;
+  ALDT = ALDT * viscmx / sqrt(ONE);                 /* ALPE = 1 */
+  double F_OSET = ONE + K.ZEP1 * TWO * fabs(ZERO) / thk0;   /* ZOFFSET = 0 */
+  double F_DTE = ONE / sqrt(F_OSET);
+  double DT = p->dtfac * F_DTE * ALDT / SSP;
+  if (DT <= p->dtmin && off >= ONE && st->offg != TWO && st->offg >= ZERO) off = ZERO;
+  out->dt = DT;
+  out->alive = (st->offg > ZERO && off != ZERO);
+  double sti, stir;
+  {
+    double DIVM = dmax(ALDT * ALDT, EM20);
+    sti = HALF * F_OSET * VOL0 * A11 * off / DIVM;
+    stir = ZERO;
+  }
+  /* ---- CZFINTCE (czfintce.F) ---- */
+  double VF[3][4], VM[2][4];
+  {
+    const double THK = thk0, C1 = thk02, A_I = AREA_I;
+    const double *VS = st->forc, *MS = st->mom;
+    double X13S8 = X13 * MS[2], X24S8 = X24 * MS[2], Y13S8 = Y13 * MS[2], Y24S8 = Y24 * MS[2];
+    double S1 = (MY34 * MX23 - MY23 * MX34) * THK;
+    double S42S = S1 * VS[4], S52S = S1 * VS[3];
+    VF[0][0] = THK * (Y24 * VS[0] - X24 * VS[2]);
+    VF[1][0] = THK * (-X24 * VS[1] + Y24 * VS[2]);
+    VF[2][0] = THK * (-X24 * VS[3] + Y24 * VS[4]);
+    VM[0][0] = C1 * (X24 * MS[1] - Y24S8) - MY13 * VF[2][0];
+    VM[1][0] = C1 * (Y24 * MS[0] - X24S8) + MX13 * VF[2][0];
+    VM[0][2] = -S52S;
+    VM[1][2] = S42S;
+    VF[0][1] = THK * (-Y13 * VS[0] + X13 * VS[2]);
+    VF[1][1] = THK * (X13 * VS[1] - Y13 * VS[2]);
+    VF[2][1] = THK * (X13 * VS[3] - Y13 * VS[4]);
+    VM[0][1] = C1 * (-X13 * MS[1] + Y13S8) + MY13 * VF[2][1];
+    VM[1][1] = C1 * (-Y13 * MS[0] + X13S8) - MX13 * VF[2][1];
+    VM[0][3] = VM[0][2];
+    VM[1][3] = VM[1][2];
+    double C2 = C1 * Z1 * 4. * A_I;
+    VF[0][0] = VF[0][0] + C2 * (X13 * MS[0] + Y13S8);
+    VF[1][0] = VF[1][0] + C2 * (Y13 * MS[1] + X13S8);
+    VF[0][1] = VF[0][1] - C2 * (X24 * MS[0] + Y24S8);
+    VF[1][1] = VF[1][1] - C2 * (Y24 * MS[1] + X24S8);
+    VF[0][2] = VF[0][3] = VF[1][2] = VF[1][3] = VF[2][2] = VF[2][3] = ZERO;
+  }
+  /* ---- CZFINTN1 (czfintn.F:29-500), MTN = 2 ---- */
+  double TESY;
+  {
+    double *VG = st->hour, *VGX = st->hourx;
+#define VGL(k) ((k) <= 5 ? VG[(k) - 1] : VGX[(k) - 6])
+    const double THK = thk0, C1 = thk02, AA = AREA_I, FAC1 = p->cvis;
+    const double C7 = K.FOUR_OVER_3, COEFH = K.ZEP999, COEF = K.ZEP85, STIER = K.FIVEP333;
+    const double TOLH = K.EM18;
+    double FBEND_V = K.THREEP464;
+    const double COEF1 = (npt == 0) ? 16.0 : 25.0;
+    double FBEND;
+    if (npt == 1) { FBEND = ZERO; FBEND_V = ZERO; } else { FBEND = K.ONE_OVER_12; }
+    const double UNDOUZSR = sqrt(K.ONE_OVER_12);
+    double dhg[6];
+    for (int k = 0; k < 6; ++k) dhg[k] = DHG[k] * dt1;
+    double C3 = FOUR * AA;
+    double HXX = C3 * MY34, HYY = C3 * MX34, HXX_K = C3 * MY23, HYY_K = C3 * MX23;
+    double CXX = HXX * dhg[0], CYY = HYY * dhg[1], CXX_K = HXX_K * dhg[0], CYY_K = HYY_K * dhg[1];
+    double BXX = HXX * dhg[2], BYY = HYY * dhg[3], BXX_K = HXX_K * dhg[2], BYY_K = HYY_K * dhg[3];
+    double C1M = A11 * FAC1, C2M = A12 * FAC1;
+    double C6 = C1 * FBEND;
+    double SS1 = MY34 * VGL(1) + MY23 * VGL(7);
+    double SS2 = MX23 * VGL(8) + MX34 * VGL(2);
+    double SF1 = MY34 * VGL(3) + MY23 * VGL(9);
+    double SF2 = -MX23 * VGL(10) - MX34 * VGL(4);
+    double SC5 = MY34 * VGL(5) + MX34 * VGL(6);
+    double SC6 = MY23 * VGL(11) + MX23 * VGL(12);
+    double C5 = HALF * off * THK * C7;
+    double ESX = SS1 * dhg[0] + SS2 * dhg[1];
+    double ETMP1 = C5 * (ESX + FOURTH * (SC5 * dhg[4] + SC6 * dhg[5]));
+    double EMX = (SF1 * dhg[2] - SF2 * dhg[3]) * C6;
+    double ETMP2 = C5 * EMX;
+    double DG[13];
+    DG[1] = C1M * CXX - C2M * CYY;
+    DG[2] = C1M * CYY - C2M * CXX;
+    DG[3] = C1M * BXX - C2M * BYY;
+    DG[4] = C1M * BYY - C2M * BXX;
+    DG[7] = C1M * CXX_K - C2M * CYY_K;
+    DG[8] = C1M * CYY_K - C2M * CXX_K;
+    DG[9] = C1M * BXX_K - C2M * BYY_K;
+    DG[10] = C1M * BYY_K - C2M * BXX_K;
+    double C2 = FAC1 * G * SHF * K.ONE_OVER_64;
+    DG[5] = C2 * HXX * dhg[4];
+    DG[6] = C2 * HYY * dhg[4];
+    DG[11] = C2 * HXX_K * dhg[5];
+    DG[12] = C2 * HYY_K * dhg[5];
+    double vgl[13];
+    for (int k = 1; k <= 12; ++k) vgl[k] = VGL(k);
+    vgl[1] = vgl[1] + DG[1];
+    vgl[2] = vgl[2] + DG[2];
+    vgl[3] = vgl[3] + DG[3];
+    vgl[4] = vgl[4] + DG[4];
+    vgl[7] = vgl[7] + DG[7];
+    vgl[8] = vgl[8] + DG[8];
+    vgl[9] = vgl[9] + DG[9];
+    vgl[10] = vgl[10] + DG[10];
+    vgl[5] = vgl[5] + DG[5];
+    vgl[6] = vgl[6] + DG[6];
+    vgl[11] = vgl[11] + DG[11];
+    vgl[12] = vgl[12] + DG[12];
+    st->eint[0] = st->eint[0] + ETMP1;
+    st->eint[1] = st->eint[1] + ETMP2;
+    if (sigy < K.ZEP9EP30) {
+      double UFAC = fabs(dmin(zcfac1, zcfac2) - ONE);
+      double SIGY2 = sigy * sigy;
+      double SVM = ZERO, SXY0 = ZERO, MXY0;
+      const double *VS = st->forc, *MS = st->mom;
+      if (UFAC < TOLH) {
+        SXY0 = VS[0] * VS[0] + VS[1] * VS[1] - VS[0] * VS[1] + THREE * VS[2] * VS[2];
+        MXY0 = MS[0] * MS[0] + MS[1] * MS[1] - MS[0] * MS[1] + THREE * MS[2] * MS[2];
+        double CNN = COEF, CMM = COEF * THK * K.ONE_OVER_16;
+        double CNNX = CNN * vgl[1], CNNY = CNN * vgl[2], CNNX_K = CNN * vgl[7], CNNY_K = CNN * vgl[8];
+        double CMMX = CMM * vgl[3], CMMY = CMM * vgl[4], CMMX_K = CMM * vgl[9], CMMY_K = CMM * vgl[10];
+        SXY0 = SXY0 + CNNX * CNNX + CNNY * CNNY - CNNX * CNNY;
+        MXY0 = MXY0 + CMMX * CMMX + CMMY * CMMY - CMMX * CMMY;
+        SXY0 = SXY0 + CNNX_K * CNNX_K + CNNY_K * CNNY_K - CNNX_K * CNNY_K;
+        MXY0 = MXY0 + CMMX_K * CMMX_K + CMMY_K * CMMY_K - CMMX_K * CMMY_K;
+        SXY0 = SXY0 + fabs(CNNX * (TWO * CNNX_K - CNNY_K) + CNNY * (TWO * CNNY_K - CNNX_K));
+        MXY0 = MXY0 + fabs(CMMX * (TWO * CMMX_K - CMMY_K) + CMMY * (TWO * CMMY_K - CMMX_K));
+        SVM = SXY0 + COEF1 * MXY0;
+      }
+      if (UFAC >= TOLH || SVM > SIGY2) {
+        double EH1 = dmin(SXY0 / dmax(SIGY2, TOLH), ONE);
+        EH1 = dmax(COEFH * EH1, (ONE - zcfac1));
+        double EH2 = dmax(COEFH, (ONE - zcfac2));
+        if (ESX < ZERO) EH1 = ZERO;
+        if (EMX < ZERO) EH2 = ZERO;
+        vgl[1] = vgl[1] - EH1 * DG[1];
+        vgl[2] = vgl[2] - EH1 * DG[2];
+        vgl[7] = vgl[7] - EH1 * DG[7];
+        vgl[8] = vgl[8] - EH1 * DG[8];
+        vgl[3] = vgl[3] - EH2 * DG[3];
+        vgl[4] = vgl[4] - EH2 * DG[4];
+        vgl[9] = vgl[9] - EH2 * DG[9];
+        vgl[10] = vgl[10] - EH2 * DG[10];
+      }
+    }
+    double C8 = C7 * off;
+    SS1 = (MY34 * vgl[1] + MY23 * vgl[7]) * C8;
+    SS2 = (MX23 * vgl[8] + MX34 * vgl[2]) * C8;
+    SF1 = (MY34 * vgl[3] + MY23 * vgl[9]) * C8;
+    SF2 = -(MX23 * vgl[10] + MX34 * vgl[4]) * C8;
+    double HSURA = THK * AA;
+    C2 = C8 * THK;
+    SC5 = (MY34 * vgl[5] + MX34 * vgl[6]) * C2;
+    SC6 = (MY23 * vgl[11] + MX23 * vgl[12]) * C2;
+    double SS3 = SC5 + SC6;
+    double HVL = AMU * sqrt(RHO * AREA * FAC1) * off;
+    double SSV0 = MY23 * MY23, SSV1 = MY34 * MY34, SSV2 = MX23 * MX23, SSV3 = MX34 * MX34;
+    double HXX_V = STIER * (SSV1 + SSV0);
+    double HXY_V = -STIER * (MY34 * MX34 + MY23 * MX23);
+    double HYY_V = STIER * (SSV2 + SSV3);
+    C2 = HVL * GSR * SHFSR * UNDOUZSR;
+    double CXZ_V = (SSV1 + SSV3) * C2;
+    double CYZ_V = (SSV2 + SSV0) * C2;
+    double AUX = AA * HVL;
+    C1M = A11SR * AUX;
+    C2M = A12SR * AUX;
+    double CXX_V = C1M * HXX_V, CYY_V = C1M * HYY_V, CXY_V = C2M * HXY_V;
+    double SS1_V = CXX_V * DHG[0] + CXY_V * DHG[1];
+    double SS2_V = CYY_V * DHG[1] + CXY_V * DHG[0];
+    double SF1_V = (CXX_V * DHG[2] + CXY_V * DHG[3]) * FBEND_V;
+    double SF2_V = (-CYY_V * DHG[3] - CXY_V * DHG[2]) * FBEND_V;
+    double SC5_V = CXZ_V * DHG[4] * HSURA;
+    double SC6_V = CYZ_V * DHG[5] * HSURA;
+    double SS3_V = SC5_V + SC6_V;
+    SS1 = SS1 + SS1_V;
+    SS2 = SS2 + SS2_V;
+    SS3 = SS3 + SS3_V;
+    SC5 = SC5 + SC5_V;
+    SC6 = SC6 + SC6_V;
+    SF1 = SF1 + SF1_V;
+    SF2 = SF2 + SF2_V;
+    double Y13S = MY13 * SS3, X13S = MX13 * SS3;
+    double Y34S6 = MY34 * SC6, Y23S5 = MY23 * SC5, X23S5 = MX23 * SC5, X34S6 = MX34 * SC6;
+    C2 = FOURTH * THK;
+    double B13 = (MY13 * X24 - MX13 * Y24) * HSURA;
+    VF[0][0] = VF[0][0] + B13 * SS1;
+    VF[0][2] = C2 * SS1;
+    VF[1][0] = VF[1][0] + B13 * SS2;
+    VF[1][2] = C2 * SS2;
+    VF[2][2] = SS3;
+    double B24 = (MX13 * Y13 - MY13 * X13) * HSURA;
+    VF[0][1] = VF[0][1] + B24 * SS1;
+    VF[0][3] = -VF[0][2];
+    VF[1][1] = VF[1][1] + B24 * SS2;
+    VF[1][3] = -VF[1][2];
+    VF[2][3] = -VF[2][2];
+    C3 = C6 * B13;
+    double C4 = C6 * C2;
+    VM[0][0] = VM[0][0] + C3 * SF2 + Y23S5 + Y34S6;
+    VM[0][2] = VM[0][2] + C4 * SF2 - Y13S;
+    VM[1][0] = VM[1][0] + C3 * SF1 - X23S5 - X34S6;
+    VM[1][2] = VM[1][2] + C4 * SF1 + X13S;
+    C3 = C6 * B24;
+    VM[0][1] = VM[0][1] + C3 * SF2 + Y23S5 - Y34S6;
+    VM[0][3] = VM[0][3] - C4 * SF2 - Y13S;
+    VM[1][1] = VM[1][1] + C3 * SF1 - X23S5 + X34S6;
+    VM[1][3] = VM[1][3] - C4 * SF1 + X13S;
+    C2 = Z1 * HSURA;
+    VF[2][0] = VF[2][0] + C2 * (SS1 * Y24 - SS2 * X24);
+    VF[2][1] = VF[2][1] + C2 * (-SS1 * Y13 + SS2 * X13);
+    double ESY = ((SS1 - SS1_V) * dhg[0] + (SS2 - SS2_V) * dhg[1]) * THK +
+                 FOURTH * ((SC5 - SC5_V) * dhg[4] + (SC6 - SC6_V) * dhg[5]);
+    ETMP1 = HALF * ESY;
+    double EMY = (SF1 - SF1_V) * dhg[2] - (SF2 - SF2_V) * dhg[3];
+    ETMP2 = HALF * C6 * EMY * THK;
+    TESY = (SS1_V * dhg[0] + SS2_V * dhg[1]) * THK +
+           (SF1_V * dhg[2] - SF2_V * dhg[3]) * THK * C6 +
+           FOURTH * (SC5_V * dhg[4] + SC6_V * dhg[5]);
+    st->eint[0] = st->eint[0] + ETMP1;
+    st->eint[1] = st->eint[1] + ETMP2;
+    for (int k = 1; k <= 5; ++k) VG[k - 1] = vgl[k];
+    for (int k = 6; k <= 12; ++k) VGX[k - 6] = vgl[k];
+#undef VGL
+  }
+  /* ---- CZPROJ1 -> CZPROJN, IFINI = 0 (czproj.F:880-1470) ---- */
+  double F[3][4], M[3][4];
+  {
+    double FL[3][4], ML[2][4];
+    for (int d = 0; d < 3; ++d) {
+      FL[d][0] = VF[d][0] + VF[d][2];
+      FL[d][1] = VF[d][1] + VF[d][3];
+      FL[d][2] = -VF[d][0] + VF[d][2];
+      FL[d][3] = -VF[d][1] + VF[d][3];
+    }
+    for (int d = 0; d < 2; ++d) {
+      ML[d][0] = VM[d][0] + VM[d][2];
+      ML[d][1] = VM[d][1] + VM[d][3];
+      ML[d][2] = -VM[d][0] + VM[d][2];
+      ML[d][3] = -VM[d][1] + VM[d][3];
+    }
+    if (plat) {
+      for (int c = 0; c < 4; ++c)
+        for (int d = 0; d < 3; ++d) {
+          F[d][c] = e1[d] * FL[0][c] + e2[d] * FL[1][c] + e3[d] * FL[2][c];
+          M[d][c] = e1[d] * ML[0][c] + e2[d] * ML[1][c];
+        }
+    } else {
+      double AR[3], AD[4], DBAD[3], ALR[3], ALD[4], MM[3][4];
+      AR[0] = -Z1 * (FL[1][0] - FL[1][1] + FL[1][2] - FL[1][3]) + COREL[1][0] * FL[2][0] +
+              ML[0][0] + COREL[1][1] * FL[2][1] + ML[0][1] + COREL[1][2] * FL[2][2] + ML[0][2] +
+              COREL[1][3] * FL[2][3] + ML[0][3];
+      AR[1] = Z1 * (FL[0][0] - FL[0][1] + FL[0][2] - FL[0][3]) - COREL[0][0] * FL[2][0] +
+              ML[1][0] - COREL[0][1] * FL[2][1] + ML[1][1] - COREL[0][2] * FL[2][2] + ML[1][2] -
+              COREL[0][3] * FL[2][3] + ML[1][3];
+      AR[2] = -COREL[1][0] * FL[0][0] + COREL[0][0] * FL[1][0] - COREL[1][1] * FL[0][1] +
+              COREL[0][1] * FL[1][1] - COREL[1][2] * FL[0][2] + COREL[0][2] * FL[1][2] -
+              COREL[1][3] * FL[0][3] + COREL[0][3] * FL[1][3];
+      for (int c = 0; c < 4; ++c) AD[c] = VQN[0][c] * ML[0][c] + VQN[1][c] * ML[1][c];
+      for (int k = 0; k < 3; ++k)
+        DBAD[k] = DB[k][0] * AD[0] + DB[k][1] * AD[1] + DB[k][2] * AD[2] + DB[k][3] * AD[3];
+      ALR[0] = DI[0] * AR[0] + DI[3] * AR[1] + DI[4] * AR[2] - DBAD[0];
+      ALR[1] = DI[3] * AR[0] + DI[1] * AR[1] + DI[5] * AR[2] - DBAD[1];
+      ALR[2] = DI[4] * AR[0] + DI[5] * AR[1] + DI[2] * AR[2] - DBAD[2];
+      for (int c = 0; c < 4; ++c)
+        ALD[c] = AD[c] + VQN[0][c] * DBAD[0] + VQN[1][c] * DBAD[1] + VQN[2][c] * DBAD[2] -
+                 DB[0][c] * AR[0] - DB[1][c] * AR[1] - DB[2][c] * AR[2];
+      double C1 = Z1 * ALR[1];
+      FL[0][0] = FL[0][0] - C1 + COREL[1][0] * ALR[2];
+      FL[0][1] = FL[0][1] + C1 + COREL[1][1] * ALR[2];
+      FL[0][2] = FL[0][2] - C1 + COREL[1][2] * ALR[2];
+      FL[0][3] = FL[0][3] + C1 + COREL[1][3] * ALR[2];
+      C1 = Z1 * ALR[0];
+      FL[1][0] = FL[1][0] + C1 - COREL[0][0] * ALR[2];
+      FL[1][1] = FL[1][1] - C1 - COREL[0][1] * ALR[2];
+      FL[1][2] = FL[1][2] + C1 - COREL[0][2] * ALR[2];
+      FL[1][3] = FL[1][3] - C1 - COREL[0][3] * ALR[2];
+      for (int c = 0; c < 4; ++c) {
+        FL[2][c] = FL[2][c] - COREL[1][c] * ALR[0] + COREL[0][c] * ALR[1];
+        MM[0][c] = ML[0][c] - ALR[0] - VQN[0][c] * ALD[c];
+        MM[1][c] = ML[1][c] - ALR[1] - VQN[1][c] * ALD[c];
+        MM[2][c] = -ALR[2] - VQN[2][c] * ALD[c];
+      }
+      for (int c = 0; c < 4; ++c)
+        for (int d = 0; d < 3; ++d) {
+          F[d][c] = e1[d] * FL[0][c] + e2[d] * FL[1][c] + e3[d] * FL[2][c];
+          M[d][c] = e1[d] * MM[0][c] + e2[d] * MM[1][c] + e3[d] * MM[2][c];
+        }
+    }
+  }
+  /* ---- CUPDTN3P (cupdtn3.F:488-700) ---- */
+  if (off < ONE) st->offg = off;
+  if (st->offg < ZERO) {
+    for (int d = 0; d < 3; ++d)
+      for (int k = 0; k < 4; ++k) { F[d][k] = ZERO; M[d][k] = ZERO; }
+    sti = ZERO; stir = ZERO;
+  }
+  for (int d = 0; d < 3; ++d)
+    for (int k = 0; k < 4; ++k) { out->f[d][k] = -F[d][k]; out->m[d][k] = -M[d][k]; }
+  out->sti = sti;
+  out->stir = stir;
+  out->stic[0] = sti * FACN[0];
+  out->stic[1] = sti * FACN[1];
+  out->stic[2] = sti * FACN[0];
+  out->stic[3] = sti * FACN[1];
+  out->ehou = TESY;
 }
 
 /* Generalised FILLCNE + W_PON ordering for nc corners per element. */
@@ -716,13 +1456,19 @@ void oracle_shell_cycle(orad_oshell_t *m) {
     }
     int tri = (m->ixc[4 * e + 2] == m->ixc[4 * e + 3]);
     orad_shout_t o;
-    oracle_shell_element((const double(*)[4])xg, (const double(*)[4])vg, (const double(*)[4])vrg,
-                         m->thke[e], tri, &m->st[e], dt1, &m->mat, &m->prop, &o);
+    if (m->prop.ishell == 24)
+      oracle_qeph_element((const double(*)[4])xg, (const double(*)[4])vg,
+                          (const double(*)[4])vrg, m->thke[e], &m->st[e], dt1, &m->mat,
+                          &m->prop, &o);
+    else
+      oracle_shell_element((const double(*)[4])xg, (const double(*)[4])vg,
+                           (const double(*)[4])vrg, m->thke[e], tri, &m->st[e], dt1, &m->mat,
+                           &m->prop, &o);
     for (int k = 0; k < 4; ++k) {
       double *f = m->fsky + 8 * (size_t)m->iads[4 * e + k];
       f[0] = o.f[0][k]; f[1] = o.f[1][k]; f[2] = o.f[2][k];
       f[3] = o.m[0][k]; f[4] = o.m[1][k]; f[5] = o.m[2][k];
-      f[6] = o.sti; f[7] = o.stir;
+      f[6] = o.stic[k]; f[7] = o.stir;
     }
     if (o.alive && o.dt < dt2t) dt2t = o.dt;
     ehou[e] = o.ehou;

@@ -61,7 +61,9 @@ SH_OFFG = 17
 SH_SIG0 = 18          # sig[pt][c] at 18 + 5*pt + c
 SH_PLA0 = 43          # pla[pt]
 SH_EPSDP0 = 48        # epsdp[pt]
-SH_NST = 53
+SH_HOURX = slice(53, 60)   # QEPH: HOURG(6:12)
+SH_NST = 60
+SH_NST_BT = 53        # the Belytschko-Tsay state ends before HOURG(6:12)
 
 
 class ShMat(C.Structure):
@@ -73,13 +75,14 @@ class ShMat(C.Structure):
 
 class ShProp(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("hm", "hf", "hr", "dm", "fsh", "dtfac", "dtmin")] + \
-               [(n, C.c_int) for n in ("ish", "npt")]
+               [(n, C.c_int) for n in ("ish", "npt", "ishell", "pad")] + \
+               [(n, C.c_double) for n in ("dn", "cvis")]
 
 
 class ShOut(C.Structure):
     _fields_ = [("f", C.c_double * 12), ("m", C.c_double * 12), ("sti", C.c_double),
                 ("stir", C.c_double), ("dt", C.c_double), ("ehou", C.c_double),
-                ("alive", C.c_int), ("pad", C.c_int)]
+                ("alive", C.c_int), ("pad", C.c_int), ("stic", C.c_double * 4)]
 
 
 class OShell(C.Structure):
@@ -120,6 +123,9 @@ def lib():
                                               C.c_int, C.c_void_p, C.c_double,
                                               C.POINTER(ShMat), C.POINTER(ShProp),
                                               C.POINTER(ShOut)]
+        _lib.oracle_qeph_element.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                             C.c_void_p, C.c_double, C.POINTER(ShMat),
+                                             C.POINTER(ShProp), C.POINTER(ShOut)]
         _lib.oracle_shell_tables.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.oracle_shell_constants.argtypes = [C.c_void_p]
         _lib.oracle_build_pon_nc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p,
@@ -144,6 +150,8 @@ def ref():
         _ref.ref_constants.argtypes = [C.c_void_p]
         _ref.ref_shell_group.argtypes = [C.c_int] + [C.c_void_p] * 15 + \
             [C.c_double] + [C.c_void_p] * 6
+        _ref.ref_qeph_group.argtypes = [C.c_int] + [C.c_void_p] * 14 + \
+            [C.c_double] + [C.c_void_p] * 7
         _ref.ref_shell_constants.argtypes = [C.c_void_p]
         _ref.ref_shell_tables.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     return _ref
@@ -296,7 +304,8 @@ def shmat(p):
 
 def shprop(p):
     return ShProp(*[p[k] for k in ("hm", "hf", "hr", "dm", "fsh", "dtfac", "dtmin")],
-                  p["ish"], p["npt"])
+                  p["ish"], p["npt"], p.get("ishell", 1), 0, p.get("dn", 0.015),
+                  p.get("cvis", 1.0))
 
 
 def shell_state(ne, thk, offg=1.0):
@@ -316,6 +325,19 @@ def shell_element(xg, vg, vrg, thke, state, dt1, mat, prop, tri=False):
                                C.byref(out))
     return dict(f=np.array(out.f).reshape(3, 4), m=np.array(out.m).reshape(3, 4),
                 sti=out.sti, stir=out.stir, dt=out.dt, ehou=out.ehou, alive=out.alive)
+
+
+def qeph_element(xg, vg, vrg, thke, state, dt1, mat, prop):
+    """One QEPH quadrilateral (Ishell=24) through the oracle chain; state (60,) in
+    place.  stic: FSKY(7) of each corner."""
+    xg, vg, vrg = (np.ascontiguousarray(a, dtype=np.float64) for a in (xg, vg, vrg))
+    assert xg.shape == (3, 4) and state.shape == (SH_NST,) and state.flags.c_contiguous
+    out = ShOut()
+    lib().oracle_qeph_element(_ptr(xg), _ptr(vg), _ptr(vrg), float(thke), _ptr(state), dt1,
+                              C.byref(shmat(mat)), C.byref(shprop(prop)), C.byref(out))
+    return dict(f=np.array(out.f).reshape(3, 4), m=np.array(out.m).reshape(3, 4),
+                sti=out.sti, stir=out.stir, stic=np.array(out.stic), dt=out.dt,
+                ehou=out.ehou, alive=out.alive)
 
 
 def shell_tables(npt):
@@ -405,6 +427,58 @@ def ref_shell_group(xg, vg, vrg, thke, state, dt1, mat, prop, tri=None):
         state[:, SH_PLA0 + p_] = pla[:, p_]
         state[:, SH_EPSDP0 + p_] = epsdp[:, p_]
     return dict(fsky=fsky.reshape(nel, 4, 8), dt=dte, ehou=ehou)
+
+
+def ref_qeph_group(xg, vg, vrg, thke, state, dt1, mat, prop):
+    """Reference QEPH routines on a group: xg, vg, vrg (nel,3,4); state (nel,60)
+    updated in place.  Returns fsky (nel,4,8), per-element dt, ehou and DT2T."""
+    nel = xg.shape[0]
+    assert nel <= 128 and state.shape == (nel, SH_NST)
+    npt = prop["npt"]
+    xg, vg, vrg = (np.ascontiguousarray(a, dtype=np.float64) for a in (xg, vg, vrg))
+    thke = np.ascontiguousarray(thke, dtype=np.float64)
+    forc = np.asfortranarray(state[:, SH_FOR])
+    mom = np.asfortranarray(state[:, SH_MOM])
+    thk = np.ascontiguousarray(state[:, SH_THK])
+    eint = np.asfortranarray(state[:, SH_EINT])
+    epsd = np.ascontiguousarray(state[:, SH_EPSD])
+    hourg = np.asfortranarray(np.concatenate([state[:, SH_HOUR], state[:, SH_HOURX]], axis=1))
+    offg = np.ascontiguousarray(state[:, SH_OFFG])
+    sig = np.zeros((nel, 5, npt), order="F")
+    pla = np.zeros((nel, npt), order="F")
+    epsdp = np.zeros((nel, npt), order="F")
+    for p_ in range(npt):
+        sig[:, :, p_] = state[:, SH_SIG0 + 5 * p_: SH_SIG0 + 5 * p_ + 5]
+        pla[:, p_] = state[:, SH_PLA0 + p_]
+        epsdp[:, p_] = state[:, SH_EPSDP0 + p_]
+    mpar = np.array([mat[k] for k in ("rho0", "young", "nu", "shear", "a11", "a12", "ssp",
+                                      "ca", "cb", "cn", "epmx", "ymax", "cc", "epdr", "mexp",
+                                      "asrate_pm9")], np.float64)
+    ipar = np.array([mat["icc"], mat["vp"], mat["israte"], mat["ipla"], npt, prop["ish"]],
+                    np.int32)
+    ppar = np.array([prop[k] for k in ("hm", "hf", "hr", "dm", "fsh", "dtfac", "dtmin",
+                                       "dn", "cvis")], np.float64)
+    fsky = np.zeros((4 * nel, 8))
+    dte = np.zeros(nel)
+    ehou = np.zeros(nel)
+    dt2t = np.zeros(1)
+    ref().ref_qeph_group(nel, _ptr(xg), _ptr(vg), _ptr(vrg), _ptr(thke), _ptr(forc),
+                         _ptr(mom), _ptr(thk), _ptr(eint), _ptr(epsd), _ptr(hourg), _ptr(offg),
+                         _ptr(sig), _ptr(pla), _ptr(epsdp), dt1, _ptr(mpar), _ptr(ipar),
+                         _ptr(ppar), _ptr(fsky), _ptr(dte), _ptr(ehou), _ptr(dt2t))
+    state[:, SH_FOR] = forc
+    state[:, SH_MOM] = mom
+    state[:, SH_THK] = thk
+    state[:, SH_EINT] = eint
+    state[:, SH_EPSD] = epsd
+    state[:, SH_HOUR] = hourg[:, 0:5]
+    state[:, SH_HOURX] = hourg[:, 5:12]
+    state[:, SH_OFFG] = offg
+    for p_ in range(npt):
+        state[:, SH_SIG0 + 5 * p_: SH_SIG0 + 5 * p_ + 5] = sig[:, :, p_]
+        state[:, SH_PLA0 + p_] = pla[:, p_]
+        state[:, SH_EPSDP0 + p_] = epsdp[:, p_]
+    return dict(fsky=fsky.reshape(nel, 4, 8), dt=dte, ehou=ehou, dt2t=float(dt2t[0]))
 
 
 class ShellModel:

@@ -32,6 +32,8 @@ def load_cases():
         c["mat"].update({k: int(v) for k, v in zip(IKEYS, c["imat"].tolist())})
         c["prop"] = dict(zip(PKEYS, c["prop"].tolist()))
         c["prop"].update({k: int(v) for k, v in zip(PIKEYS, c["iprop"].tolist())})
+        for k in ("state", "state_out"):          # BT states end before HOURG(6:12)
+            c[k] = np.pad(c[k], ((0, 0), (0, po.SH_NST - c[k].shape[1])))
         out[n] = c
     return out
 
