@@ -161,9 +161,19 @@ int orad_set_shell_law2(orad_ctx *ctx, const double *p, const int *ip);
  * (hm_read_prop01.F:155-244).  Ishell=1, Ismstr=2, Ithick=0 are implied. */
 int orad_set_shell_prop(orad_ctx *ctx, const double *g, int ish, int npt);
 
+/* Shell formulation, /PROP/SHELL Ishell (IPARG(23), hm_read_prop01.F:180-229):
+ * 1 = Belytschko-Tsay (CFORC3, the default), 24 = QEPH (CZFORC3,
+ * shell/coquez/czforc3.F:436-808) with numerical damping dn = Dn (GEO(13); 0
+ * means the default 0.015) and CVIS = 1.  Must precede orad_set_shells: QEPH
+ * elements keep 60 state values (HOURG(6:12) after the 53 of BT) and one STI per
+ * diagonal (FSKY(7) = STI*FACN, CUPDTN3P cupdtn3.F:488-700).  Batched models and
+ * the adjoint take Ishell=1 only. */
+int orad_set_shell_ishell(orad_ctx *ctx, int ishell, double dn);
+
 /* Drop-in for one shell group, CFORC3 semantics (shell/coque/cforc3.F:398-746,
- * called from forintc.F:420): elements NFT+1..NFT+NEL; writes their FSKY slots
- * (CUPDT3P), updates element state, lowers *dt2t to the group minimum (CDT3). */
+ * called from forintc.F:420), or CZFORC3 for QEPH: elements NFT+1..NFT+NEL;
+ * writes their FSKY slots (CUPDT3P / CUPDTN3P), updates element state, lowers
+ * *dt2t to the group minimum (CDT3 / CNDT3). */
 int orad_cforc3(orad_ctx *ctx, int nft, int nel, double dt1, double *dt2t);
 
 /* ASSPAR4 with rotations: A(3,NUMNOD), AR(3,NUMNOD), STIFN, STIFR. */
@@ -174,7 +184,8 @@ int orad_get_nodes_rot(orad_ctx *ctx, double *vr, double *ar);
 
 /* Shell element state as rows of 53 FP64 values per element:
  * FOR(5) MOM(3) THK EINT(2) EPSD HOUR(5) OFF, then SIG(5) for points 1..5,
- * PLA(1..5), point EPSD(1..5) (GBUF/LBUF of elbufdef_mod.F90). */
+ * PLA(1..5), point EPSD(1..5) (GBUF/LBUF of elbufdef_mod.F90); QEPH rows have
+ * 60 values, HOURG(6:12) appended (HOURG(1:5) in the HOUR slots). */
 int orad_get_shell_state(orad_ctx *ctx, double *rows);
 int orad_set_shell_state(orad_ctx *ctx, const double *rows);
 

@@ -24,6 +24,8 @@ SOURCES = [
     ("orad_brick_mixed.hip", ["-ffp-contract=fast"]),
     ("orad_shell_fp64.hip", ["-ffp-contract=off"]),
     ("orad_shell_mixed.hip", ["-ffp-contract=fast"]),
+    ("orad_qeph_fp64.hip", ["-ffp-contract=off"]),
+    ("orad_qeph_mixed.hip", ["-ffp-contract=fast"]),
     ("orad_nodal.hip", ["-ffp-contract=off"]),
     ("orad_adjoint.hip", ["-ffp-contract=off"]),
     ("orad_api.hip", []),

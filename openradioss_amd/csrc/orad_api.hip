@@ -114,6 +114,13 @@ static ShellArgs<T> shell_args(orad_ctx *c) {
 static hipError_t launch_elements(orad_ctx *c, int e0, int e1, const double *dt1p,
                                   unsigned long long *dt2t, double *ehou_part) {
   if (c->family == FAM_SHELL) {
+    if (c->qeph()) {
+      if (c->precision == ORAD_MIXED)
+        return launch_qeph_mixed(shell_args<float>(c), e0, e1, dt1p, dt2t, ehou_part,
+                                 c->xcd_remap, c->stream);
+      return launch_qeph_fp64(shell_args<double>(c), e0, e1, dt1p, dt2t, ehou_part, c->xcd_remap,
+                              c->stream);
+    }
     if (c->precision == ORAD_MIXED)
       return launch_shell_mixed(shell_args<float>(c), e0, e1, dt1p, dt2t, ehou_part, c->xcd_remap,
                                 c->stream);
@@ -716,6 +723,23 @@ static ShConst shell_constants() {
   k.three_over_4 = 3.0 / 4.0;
   k.third = 1.0 / 3.0;
   k.three_half = 3.0 / 2.0;
+  // QEPH chain (constant_mod.F: EM08 .. ZEP9EP30 from their definitions)
+  k.em08 = 1.0 / ep[8];
+  k.em10 = 1.0 / ep[10];
+  k.em18 = 1.0 / ep[18];
+  k.one_over_12 = 1.0 / 12.0;
+  k.one_over_16 = 1.0 / 16.0;
+  k.one_over_64 = 1.0 / 64.0;
+  {
+    const double em03 = 1.0 / ep[3];
+    const double zep05 = 5.0 / ep[2], zep15 = zep1 + zep05;
+    const double zep04 = 4.0 / ep[2], zep06 = zep1 - zep04;
+    k.zep85 = 1.0 - zep15;
+    k.zep999 = 1.0 - em03;
+    k.fivep333 = 16.0 / 3.0;
+    k.threep464 = 3.0 + zep4 + zep06 + fourem3;
+    k.zep9ep30 = 9.0 * (ep20 * ep[9]);
+  }
   return k;
 }
 
@@ -763,7 +787,7 @@ static int upload_codes(orad_ctx *c) {
 static int init_shell_state(orad_ctx *c) {
   if (!c->have_shells || c->state_init) return ORAD_OK;
   const size_t ne = (size_t)c->numelc, ts = c->tsize();
-  std::vector<char> all(SC_NCOL * ne * ts, 0);
+  std::vector<char> all((size_t)c->shncol() * ne * ts, 0);
   std::vector<double> thk(ne);
   HIPCHK(c, hipMemcpy(thk.data(), c->thke, ne * 8, hipMemcpyDeviceToHost));
   for (size_t e = 0; e < ne; ++e) {
@@ -873,7 +897,8 @@ int orad_set_shells(orad_ctx *c, int numelc, const int *ixc, const double *thke)
     for (int k = 0; k < 4; ++k) {
       conn_cm[k * ne + e] = conn_em[4 * e + k];
       iads_cm[k * ne + e] = c->h_iads[4 * e + k];
-      se[c->h_iads[4 * e + k]] = (int32_t)e;
+      // STI entry of the slot: the element's, or for QEPH its diagonal's
+      se[c->h_iads[4 * e + k]] = (int32_t)(c->ishell == 24 ? e + (k & 1) * ne : e);
     }
   const size_t ts = c->tsize();
   HIPCHK(c, dalloc(&c->conn, 4 * ne * 4));
@@ -884,15 +909,18 @@ int orad_set_shells(orad_ctx *c, int numelc, const int *ixc, const double *thke)
   HIPCHK(c, hipMemcpy(c->iads, iads_cm.data(), 4 * ne * 4, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->adsky, c->h_adsky.data(), (nn + 1) * 4, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->slot_elem, se.data(), 4 * ne * 4, hipMemcpyHostToDevice));
-  HIPCHK(c, dalloc(&c->state, SC_NCOL * ne * ts));
-  HIPCHK(c, hipMemset(c->state, 0, SC_NCOL * ne * ts));
+  const size_t ncol = (size_t)c->shncol(), nsti = (size_t)c->nsti();
+  HIPCHK(c, dalloc(&c->state, ncol * ne * ts));
+  HIPCHK(c, hipMemset(c->state, 0, ncol * ne * ts));
   HIPCHK(c, dalloc(&c->fsky, 4 * ne * 6 * ts));
   HIPCHK(c, hipMemset(c->fsky, 0, 4 * ne * 6 * ts));
-  HIPCHK(c, dalloc(&c->sti, ne * ts));
-  HIPCHK(c, hipMemset(c->sti, 0, ne * ts));
+  HIPCHK(c, dalloc(&c->sti, nsti * ne * ts));
+  HIPCHK(c, hipMemset(c->sti, 0, nsti * ne * ts));
   HIPCHK(c, dalloc(&c->thke, ne * 8));
   HIPCHK(c, hipMemcpy(c->thke, thke, ne * 8, hipMemcpyHostToDevice));
-  const int bb = c->precision == ORAD_MIXED ? shell_block_mixed() : shell_block_fp64();
+  const int bb = c->ishell == 24
+                     ? (c->precision == ORAD_MIXED ? qeph_block_mixed() : qeph_block_fp64())
+                     : (c->precision == ORAD_MIXED ? shell_block_mixed() : shell_block_fp64());
   c->n_ehou_part = (int)((ne + bb - 1) / bb);
   HIPCHK(c, dalloc(&c->ehou_part, (size_t)c->n_ehou_part * sizeof(double)));
   HIPCHK(c, hipMemset(c->ehou_part, 0, (size_t)c->n_ehou_part * sizeof(double)));
@@ -955,8 +983,28 @@ int orad_set_shell_prop(orad_ctx *c, const double *g, int ish, int npt) {
   if (!(p.hm > 0.0 || p.hf > 0.0) || !(p.dtfac > 0.0) || p.dm < 0.0 || !(p.fsh > 0.0))
     return fail(c, ORAD_EINVAL, "set_shell_prop: hm/hf, DTFAC and the shear factor must be positive");
   shell_tables(npt, p);
+  p.ishell = c->ishell;
+  p.dn = c->qeph_dn;
+  p.cvis = 1.0;
   c->shprop = p;
   c->have_shprop = true;
+  drop_graphs(c);
+  return ORAD_OK;
+}
+
+int orad_set_shell_ishell(orad_ctx *c, int ishell, double dn) {
+  if (!c) return ORAD_EINVAL;
+  if (c->have_shells)
+    return fail(c, ORAD_ESTATE, "set_shell_ishell: choose the formulation before orad_set_shells");
+  if (ishell != 1 && ishell != 24)
+    return fail(c, ORAD_EINVAL, "set_shell_ishell: Ishell must be 1 (BT) or 24 (QEPH)");
+  if (!(dn >= 0.0)) return fail(c, ORAD_EINVAL, "set_shell_ishell: Dn must be >= 0");
+  // Dn = 0 on input means 0.015 (ZEP01 + FIVEEM3, hm_read_prop01.F:199, cncoef3.F:259-264)
+  c->ishell = ishell;
+  c->qeph_dn = (dn == 0.0) ? 1.0 / 100.0 + 5.0 / 1000.0 : dn;
+  c->shprop.ishell = ishell;
+  c->shprop.dn = c->qeph_dn;
+  c->shprop.cvis = 1.0;
   drop_graphs(c);
   return ORAD_OK;
 }
@@ -1030,12 +1078,13 @@ int orad_get_shell_state(orad_ctx *c, double *rows) {
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const size_t ne = (size_t)c->numelc, ts = c->tsize();
-  std::vector<char> h(SC_NCOL * ne * ts);
+  const int nc = c->shncol();
+  std::vector<char> h((size_t)nc * ne * ts);
   HIPCHK(c, hipMemcpy(h.data(), c->state, h.size(), hipMemcpyDeviceToHost));
   for (size_t e = 0; e < ne; ++e)
-    for (int k = 0; k < SC_NCOL; ++k)
-      rows[e * SC_NCOL + k] = c->precision == ORAD_MIXED ? (double)((float *)h.data())[k * ne + e]
-                                                         : ((double *)h.data())[k * ne + e];
+    for (int k = 0; k < nc; ++k)
+      rows[e * nc + k] = c->precision == ORAD_MIXED ? (double)((float *)h.data())[k * ne + e]
+                                                    : ((double *)h.data())[k * ne + e];
   return ORAD_OK;
 }
 
@@ -1045,11 +1094,12 @@ int orad_set_shell_state(orad_ctx *c, const double *rows) {
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const size_t ne = (size_t)c->numelc, ts = c->tsize();
-  std::vector<char> h(SC_NCOL * ne * ts);
+  const int nc = c->shncol();
+  std::vector<char> h((size_t)nc * ne * ts);
   for (size_t e = 0; e < ne; ++e)
-    for (int k = 0; k < SC_NCOL; ++k) {
-      if (c->precision == ORAD_MIXED) ((float *)h.data())[k * ne + e] = (float)rows[e * SC_NCOL + k];
-      else ((double *)h.data())[k * ne + e] = rows[e * SC_NCOL + k];
+    for (int k = 0; k < nc; ++k) {
+      if (c->precision == ORAD_MIXED) ((float *)h.data())[k * ne + e] = (float)rows[e * nc + k];
+      else ((double *)h.data())[k * ne + e] = rows[e * nc + k];
     }
   HIPCHK(c, hipMemcpy(c->state, h.data(), h.size(), hipMemcpyHostToDevice));
   c->state_init = true;
@@ -1084,6 +1134,8 @@ int orad_set_shell_models(orad_ctx *c, int nmodel, const int *node_off, const in
   if (c->family != FAM_SHELL || !c->have_shmat)
     return fail(c, ORAD_ESTATE, "set_shell_models: set the shells and a law-2 material first");
   if (c->batched) return fail(c, ORAD_ESTATE, "set_shell_models: models already set");
+  if (c->qeph())
+    return fail(c, ORAD_ESTATE, "set_shell_models: Belytschko-Tsay shells only (Ishell=1)");
   if (nmodel < 1 || !node_off || !elem_off || (p == nullptr) != (ip == nullptr))
     return fail(c, ORAD_EINVAL, "set_shell_models: bad arguments");
   if (node_off[0] != 0 || elem_off[0] != 0 || node_off[nmodel] != c->numnod ||
@@ -1217,10 +1269,11 @@ struct Snapshot {
   hipError_t take(orad_ctx *c) {
     host_ncycle = c->host_ncycle;
     const size_t nn = (size_t)c->numnod, ne = (size_t)nelem(c);
-    const size_t ncol = c->family == FAM_SHELL ? SC_NCOL : NSTATE;
+    const size_t ncol = c->family == FAM_SHELL ? (size_t)c->shncol() : NSTATE;
+    const size_t nsti = c->family == FAM_SHELL ? (size_t)c->nsti() : 1;
     std::vector<std::pair<void *, size_t>> live = {
         {c->X4, nn * 32}, {c->V4, nn * 32}, {c->D, nn * 24}, {c->state, ncol * ne * c->tsize()},
-        {c->sti, ne * c->tsize()}, {c->ctl, c->ctl_bytes()}};
+        {c->sti, nsti * ne * c->tsize()}, {c->ctl, c->ctl_bytes()}};
     if (c->family == FAM_SHELL) live.push_back({c->VR4, nn * 32});
     for (auto &b : live) {
       void *p = nullptr;
@@ -1351,6 +1404,8 @@ int orad_adjoint_forward(orad_ctx *c, int ncycle, int every) {
   if (!c) return ORAD_EINVAL;
   if (c->family != FAM_SHELL || c->batched)
     return fail(c, ORAD_ESTATE, "adjoint_forward: one shell model only");
+  if (c->qeph())
+    return fail(c, ORAD_ESTATE, "adjoint_forward: Belytschko-Tsay shells only (Ishell=1)");
   int r = ready(c);
   if (r) return r;
   if (ncycle < 1 || every < 1) return fail(c, ORAD_EINVAL, "adjoint_forward: bad cycle counts");
@@ -1527,23 +1582,24 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
 static int get_fsky_shell(orad_ctx *c, double *fsky) {
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  const size_t ne = (size_t)c->numelc, ns = 4 * ne;
-  std::vector<double> f(6 * ns), st(ne);
+  const size_t ne = (size_t)c->numelc, ns = 4 * ne, nst = (size_t)c->nsti() * ne;
+  std::vector<double> f(6 * ns), st(nst);
   if (c->precision == ORAD_MIXED) {
-    std::vector<float> h(6 * ns), hs(ne);
+    std::vector<float> h(6 * ns), hs(nst);
     HIPCHK(c, hipMemcpy(h.data(), c->fsky, h.size() * 4, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(hs.data(), c->sti, ne * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(hs.data(), c->sti, nst * 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < h.size(); ++i) f[i] = h[i];
-    for (size_t i = 0; i < ne; ++i) st[i] = hs[i];
+    for (size_t i = 0; i < nst; ++i) st[i] = hs[i];
   } else {
     HIPCHK(c, hipMemcpy(f.data(), c->fsky, 6 * ns * 8, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(st.data(), c->sti, ne * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(st.data(), c->sti, nst * 8, hipMemcpyDeviceToHost));
   }
+  const bool qeph = c->qeph();
   for (size_t e = 0; e < ne; ++e)
     for (int k = 0; k < 4; ++k) {
       const size_t sl = (size_t)c->h_iads[4 * e + k];
       for (int j = 0; j < 6; ++j) fsky[8 * sl + j] = f[6 * sl + j];
-      fsky[8 * sl + 6] = st[e];
+      fsky[8 * sl + 6] = qeph ? st[e + (k & 1) * ne] : st[e];
       fsky[8 * sl + 7] = 0.0;
     }
   return ORAD_OK;

@@ -76,6 +76,10 @@ struct ShProp {
   double hm, hf, hr, dm, fsh, dtfac, dtmin;
   int ish, npt;
   double z0[5], wf[5], wm[5];
+  // formulation: 1 Belytschko-Tsay (CFORC3), 24 QEPH (CZFORC3) with its numerical
+  // damping Dn (GEO(13)) and hourglass stiffness factor CVIS (GEO(17), always 1)
+  int ishell;
+  double dn, cvis;
 };
 
 // constant_mod values used on the shell path, derived on the host from their
@@ -83,10 +87,15 @@ struct ShProp {
 struct ShConst {
   double zep81, zep072169, onep414, four_over_3, one_over_9, four_over_5, em30, sr2d2;
   double em15, em20, em07, three_over_4, third, three_half;
+  // QEPH chain
+  double em08, em10, em18, one_over_12, one_over_16, one_over_64, zep85, zep999, fivep333;
+  double threep464, zep9ep30;
 };
 
 // One shell PARITH/ON slot: corner force and moment (FSKY(1:6,·)).  STI
-// (FSKY(7,·)) is kept once per element; STIR (FSKY(8,·)) is zero on this path.
+// (FSKY(7,·)) is kept once per element (Belytschko-Tsay) or once per diagonal
+// (QEPH: sti[e] for corners 1,3, sti[ne+e] for corners 2,4); STIR (FSKY(8,·)) is
+// zero on this path.
 template <typename T> struct Slot6 { T fx, fy, fz, mx, my, mz; };
 
 // Shell element state columns (structure of arrays, column c at st[c*ne + e]).
@@ -95,7 +104,9 @@ enum {
   SC_SIG = 18,     // + 5*pt + component
   SC_PLA = 43,     // + pt
   SC_EPSDP = 48,   // + pt
-  SC_NCOL = 53
+  SC_NCOL = 53,
+  // QEPH (Ishell=24) keeps HOURG(6:12) after the Belytschko-Tsay columns
+  SC_HOURX = 53, SC_NCOL_QEPH = 60
 };
 
 template <typename T>
@@ -107,8 +118,8 @@ struct ShellArgs {
   const double4 *V4;        // (vx, vy, vz, codes)
   const double4 *VR4;       // (vrx, vry, vrz, IN)
   const double *thke;       // element thickness THKE (FP64)
-  T *st;                    // [SC_NCOL][ne]
-  T *sti;                   // STI per element (FSKY(7,·))
+  T *st;                    // [SC_NCOL][ne], [SC_NCOL_QEPH][ne] for QEPH
+  T *sti;                   // STI per element (FSKY(7,·)), per diagonal for QEPH
   Slot6<T> *fsky;
   CycleCtl *ctl;
   ShMat mat;

@@ -33,6 +33,12 @@ hipError_t launch_shell_batch_fp64(const ShellArgs<double> &, const BatchArgs &,
                                    int, int, hipStream_t);
 hipError_t launch_shell_batch_mixed(const ShellArgs<float> &, const BatchArgs &, int, int, int,
                                     int, int, hipStream_t);
+hipError_t launch_qeph_fp64(const ShellArgs<double> &, int, int, const double *,
+                            unsigned long long *, double *, int, hipStream_t);
+hipError_t launch_qeph_mixed(const ShellArgs<float> &, int, int, const double *,
+                             unsigned long long *, double *, int, hipStream_t);
+int qeph_block_fp64();
+int qeph_block_mixed();
 int shell_batch_occupancy_fp64(int);
 int shell_batch_occupancy_mixed(int);
 template <typename S> struct AdjArgs;
@@ -73,6 +79,8 @@ struct orad_ctx {
   ShMat shmat{};
   ShProp shprop{};
   ShConst shk{};
+  int ishell = 1;           // shell formulation (orad_set_shell_ishell): 1 BT, 24 QEPH
+  double qeph_dn = 0.015;   // QEPH numerical damping Dn
   double4 *VR4 = nullptr;
   double *AR = nullptr, *STIFR = nullptr, *thke = nullptr;
   // imposed velocities (FIXVEL) and rotational codes, folded into V4.w on upload
@@ -87,8 +95,8 @@ struct orad_ctx {
   void *state = nullptr;  // SoA element state of the storage type (NSTATE columns)
   double *vol0 = nullptr; // initial volumes, FP64
   void *fsky = nullptr;
-  void *sti = nullptr;              // per-element STI/4
-  int32_t *slot_elem = nullptr;     // element owning each slot
+  void *sti = nullptr;              // per-element STI (QEPH: per diagonal, 2*numelc)
+  int32_t *slot_elem = nullptr;     // STI entry of each slot (its element; QEPH: + diagonal)
   double *ehou_part = nullptr;
   int n_ehou_part = 0;
   CycleCtl *ctl = nullptr;
@@ -123,6 +131,9 @@ struct orad_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
   size_t ctl_bytes() const { return (size_t)nmodel * sizeof(CycleCtl); }
+  bool qeph() const { return family == FAM_SHELL && ishell == 24; }
+  int shncol() const { return ishell == 24 ? SC_NCOL_QEPH : SC_NCOL; }   // shell state columns
+  int nsti() const { return ishell == 24 ? 2 : 1; }                      // STI entries per shell
   size_t tsize() const { return precision == ORAD_MIXED ? sizeof(float) : sizeof(double); }
 };
 

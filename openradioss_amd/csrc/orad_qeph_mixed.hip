@@ -1,0 +1,37 @@
+// orad_qeph_mixed.hip -- QEPH shell element kernel, mixed form: FP32 state, law,
+// hourglass and force stages; nodes stay FP64.  The frame, warped-element
+// projection and velocity stage runs on coordinates and translational velocities
+// taken relative to corner 1 in FP64, in the precision ORAD_QEPH_KIN_FP64 selects.
+#include "orad_qeph_kernel.hpp"
+
+#ifndef ORAD_QEPH_KIN_FP64
+#define ORAD_QEPH_KIN_FP64 1
+#endif
+
+#ifndef ORAD_QEPH_BLOCK
+#define ORAD_QEPH_BLOCK 256
+#endif
+
+namespace orad {
+
+#if ORAD_QEPH_KIN_FP64
+typedef double QephKinT;
+#else
+typedef float QephKinT;
+#endif
+
+int qeph_block_mixed() { return ORAD_QEPH_BLOCK; }
+
+hipError_t launch_qeph_mixed(const ShellArgs<float> &a, int e_begin, int e_end, const double *dt1p,
+                             unsigned long long *dt2t, double *ehou_part, int xcd_remap,
+                             hipStream_t s) {
+  const int n = e_end - e_begin;
+  if (n <= 0) return hipSuccess;
+  const int nb = (n + ORAD_QEPH_BLOCK - 1) / ORAD_QEPH_BLOCK;
+  hipLaunchKernelGGL((qeph_forces_kernel<float, QephKinT, true, ORAD_QEPH_BLOCK>), dim3(nb),
+                     dim3(ORAD_QEPH_BLOCK), 0, s, a, e_begin, e_end, dt1p, dt2t, ehou_part,
+                     xcd_remap);
+  return hipGetLastError();
+}
+
+}  // namespace orad

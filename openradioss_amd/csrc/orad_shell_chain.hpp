@@ -42,13 +42,17 @@ template <typename T> __device__ __forceinline__ T tabs(T a) { return a < T(0) ?
 
 // SIGEPS02C + M2CPLR for one through-thickness point (FISOKIN = 0, IFORM = 0,
 // TSTAR = 0).  off: element OFF, lowered by the ductile-rupture test; thkn: the
-// element thickness updated point by point.
+// element thickness updated point by point; etse: ET/E of the point (1 when
+// elastic, m2cplr.F:99,227,282,414); sigy: SIGY, to which the point adds its
+// yield stress / NPT (sigeps02c.F:187).  The QEPH chain reads etse and sigy; in
+// the Belytschko-Tsay chain they are unused and compile away.
 template <typename T>
 __device__ __forceinline__ void shell_law2_point(const ShMat &m, const T jc[3], const ShConst &k,
                                                  T gs, T dt1,
                                                  T asrate, T epsd_pg, T off_old, T &off, T &pla,
                                                  T &epsdp, T &epchk, T &thkn, T thklyl,
-                                                 const T deps[5], T s[5]) {
+                                                 const T deps[5], T s[5], T &etse, T &sigy,
+                                                 T fnpt) {
   const T ZERO = T(0), ONE = T(1), TWO = T(2), THREE = T(3), HALF = T(0.5);
   const T young = T(m.young), g = T(m.shear), nu = T(m.nu);
   const T a11 = M<T>::div(young, ONE - nu * nu);
@@ -76,6 +80,7 @@ __device__ __forceinline__ void shell_law2_point(const ShMat &m, const T jc[3], 
   }
   // ---- M2CPLR ----
   T ezz = ZERO, h = ZERO, ca = ca0, cb = cb0, ymax = ymax0, dpla = ZERO;
+  etse = ONE;
   s[0] = s[0] + a11 * deps[0] + a12 * deps[1];
   s[1] = s[1] + a12 * deps[0] + a11 * deps[1];
   s[2] = s[2] + g * deps[2];
@@ -109,6 +114,9 @@ __device__ __forceinline__ void shell_law2_point(const ShMat &m, const T jc[3], 
       ezz = M<T>::div(dpla * s1, yld);
       pla = pla + dpla;
       epchk = tmax(pla, epchk);
+      if (yld >= ymax) h = ZERO;
+      else h = cn * cb * M<T>::powe(pla + SMALL, cn - ONE);
+      etse = M<T>::div(h, h + young);
     }
   } else if (m.ipla == 1) {
     const T s1 = s[0] + s[1], s2 = s[0] - s[1], s3 = s[2];
@@ -120,6 +128,7 @@ __device__ __forceinline__ void shell_law2_point(const ShMat &m, const T jc[3], 
       if (yld >= ymax) h = ZERO;
       else h = cn * cb * M<T>::powe(pla + SMALL, cn - ONE);
       T dpla_j = M<T>::div(svm - yld, THREE * g + h);
+      etse = M<T>::div(h, h + young);
       const T anu1 = A * nu1, bnu2 = THREE * B * nu2, h2 = TWO * h;
       T dpla_i = ZERO, dr = ZERO, p = ZERO, q = ZERO;
       for (int it = 0; it < 3; ++it) {
@@ -154,6 +163,7 @@ __device__ __forceinline__ void shell_law2_point(const ShMat &m, const T jc[3], 
     if (svm2 > yld2 && off == ONE) {
       if (yld >= ymax) h = ZERO;
       else h = cn * cb * M<T>::powe(pla + SMALL, cn - ONE);
+      etse = M<T>::div(h, h + young);
       const T aa = M<T>::div(svm2 - yld2, T(5) * svm2 + THREE * (-s[0] * s[1] + s[2] * s[2]));
       const T s1 = (ONE - TWO * aa) * s[0] + aa * s[1];
       const T s2 = aa * s[0] + (ONE - TWO * aa) * s[1];
@@ -173,6 +183,7 @@ __device__ __forceinline__ void shell_law2_point(const ShMat &m, const T jc[3], 
     const T epsdot_p = M<T>::div(dpla, tmax(EM20, dt1));
     epsdp = asrate * epsdot_p + (ONE - asrate) * epsdp;
   }
+  sigy = sigy + M<T>::div(yld, fnpt);
   if (off == off_old && off > ZERO) {
     if (off == ONE && epchk >= T(m.epmx)) off = T(k.four_over_5);
     else if (off < ONE) off = off * T(k.four_over_5);
@@ -419,8 +430,9 @@ __device__ __forceinline__ void shell_chain(const TG xg[3][4], const TG vg[3][4]
 #pragma unroll
     for (int c = 0; c < 5; ++c) s[c] = io.get(SC_SIG + 5 * it + c);
     T pla = io.get(SC_PLA + it), epsdp = io.get(SC_EPSDP + it);
+    T etse_unused, sigy_unused = ZERO;
     shell_law2_point<T>(m, jc, k, GS, dt1, asrate, epsd_pg, off_old, off, pla, epsdp, epchk, thkn,
-                        thklyl, deps, s);
+                        thklyl, deps, s, etse_unused, sigy_unused, ONE);
     viscmx = tmax(T(p.dm), viscmx);
 #pragma unroll
     for (int c = 0; c < 5; ++c) io.put(SC_SIG + 5 * it + c, s[c] * ONE);

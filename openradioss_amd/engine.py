@@ -23,13 +23,14 @@ EXPORTS = (
     "orad_sforc3", "orad_asspar4", "orad_run_cycles", "orad_profile_cycles", "orad_get_nodes",
     "orad_get_brick_state", "orad_set_brick_state", "orad_get_fsky", "orad_energies",
     "orad_set_fixvel", "orad_set_nodes_rot", "orad_set_shells", "orad_set_shell_law2",
-    "orad_set_shell_prop", "orad_cforc3", "orad_asspar4_rot", "orad_get_nodes_rot",
+    "orad_set_shell_prop", "orad_set_shell_ishell", "orad_cforc3", "orad_asspar4_rot", "orad_get_nodes_rot",
     "orad_get_shell_state", "orad_set_shell_state", "orad_set_shell_models",
     "orad_get_model_time", "orad_model_energies", "orad_set_launch", "orad_get_launch",
     "orad_autotune", "orad_adjoint_forward", "orad_adjoint_reverse",
 )
 NIXC = 7
-SH_NCOL = 53
+SH_NCOL = 53          # Belytschko-Tsay state values per shell
+SH_NCOL_QEPH = 60     # QEPH: HOURG(6:12) appended
 
 
 class TimeState(C.Structure):
@@ -91,6 +92,7 @@ def load_library(path=LIB_PATH):
         "orad_set_shells": ([P, I, P, P], I),
         "orad_set_shell_law2": ([P, P, P], I),
         "orad_set_shell_prop": ([P, P, I, I], I),
+        "orad_set_shell_ishell": ([P, I, D], I),
         "orad_cforc3": ([P, I, I, D, C.POINTER(D)], I),
         "orad_asspar4_rot": ([P, P, P, P, P], I),
         "orad_get_nodes_rot": ([P, P, P], I),
@@ -302,8 +304,9 @@ SHMAT_KEYS = ("rho0", "young", "nu", "shear", "a11", "a12", "ssp", "ca", "cb", "
 
 
 class ShellSolver:
-    """One Belytschko-Tsay shell model (law 2) resident on one GPU: the CFORC3
-    path with 6-DOF nodes."""
+    """One shell model (law 2) resident on one GPU with 6-DOF nodes: the CFORC3
+    path (Belytschko-Tsay, prop ishell = 1) or the CZFORC3 path (QEPH, ishell = 24,
+    numerical damping prop dn)."""
 
     def __init__(self, deck, precision=FP64, device=0):
         self.lib = load_library()
@@ -333,6 +336,9 @@ class ShellSolver:
         g = np.array([prop[k] for k in ("hm", "hf", "hr", "dm", "fsh", "dtfac", "dtmin")],
                      np.float64)
         self._chk(self.lib.orad_set_shell_prop(h, _p(g), int(prop["ish"]), int(prop["npt"])))
+        self.ishell = int(prop.get("ishell", 1))
+        self._chk(self.lib.orad_set_shell_ishell(h, self.ishell, float(prop.get("dn", 0.0))))
+        self.ncol = SH_NCOL_QEPH if self.ishell == 24 else SH_NCOL
         ixc = ixc_from_deck(deck)
         thk = _f64(deck["thk"], (self.numelc,))
         self._chk(self.lib.orad_set_shells(h, self.numelc, _p(ixc), _p(thk)))
@@ -423,12 +429,12 @@ class ShellSolver:
         return dict(x=x, v=v, d=d, a=a, stifn=s, vr=vr, ar=ar)
 
     def state(self):
-        rows = np.zeros((self.numelc, SH_NCOL))
+        rows = np.zeros((self.numelc, self.ncol))
         self._chk(self.lib.orad_get_shell_state(self.h, _p(rows)))
         return rows
 
     def set_state(self, rows):
-        rows = _f64(rows, (self.numelc, SH_NCOL))
+        rows = _f64(rows, (self.numelc, self.ncol))
         self._chk(self.lib.orad_set_shell_state(self.h, _p(rows)))
 
     def fsky(self):

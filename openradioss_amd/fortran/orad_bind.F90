@@ -136,6 +136,12 @@ module orad_bind
       real(c_double) :: g(*)
       integer(c_int), value :: ish, npt
     end function
+    integer(c_int) function orad_set_shell_ishell(ctx, ishell, dn) bind(c, name='orad_set_shell_ishell')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ishell
+      real(c_double), value :: dn
+    end function
     integer(c_int) function orad_cforc3(ctx, nft, nel, dt1, dt2t) bind(c, name='orad_cforc3')
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
@@ -153,15 +159,17 @@ module orad_bind
       type(c_ptr), value :: ctx
       real(c_double) :: vr(3,*), ar(3,*)
     end function
+    ! rows: 53 values per shell, 60 with QEPH (orad_set_shell_ishell)
     integer(c_int) function orad_get_shell_state(ctx, rows) bind(c, name='orad_get_shell_state')
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
-      real(c_double) :: rows(53,*)
+      real(c_double) :: rows(*)
     end function
+    ! rows: 53 values per shell, 60 with QEPH (orad_set_shell_ishell)
     integer(c_int) function orad_set_shell_state(ctx, rows) bind(c, name='orad_set_shell_state')
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
-      real(c_double) :: rows(53,*)
+      real(c_double) :: rows(*)
     end function
     ! p and ip: c_loc of the per-model law-2 records, or c_null_ptr
     integer(c_int) function orad_set_shell_models(ctx, nmodel, node_off, elem_off, p, ip) &
