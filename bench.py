@@ -44,14 +44,16 @@ def node_bytes(ne, nn, tsize):
     return 8 * ne * 3 * tsize + nn * (8 + 32 + 32 + 24 + 32 + 32 + 24)
 
 
-def shell_elem_bytes(ne, nn, tsize):
+def shell_elem_bytes(ne, nn, tsize, qeph=False):
     """Algorithmic bytes of one shell element-kernel launch (5 points): connectivity
     and slot addresses (4+4 int32), THKE (FP64), the 53 state columns read
     (element: FOR(5) MOM(3) THK EINT(2) EPSD HOUR(5) OFFG; points: SIG(5) PLA EPSD)
     and all but OFFG written, STI, 4 slots of 6 values, and the X4/V4/VR4 node
-    records once per node."""
-    reads = 16 + 16 + 8 + 53 * tsize
-    writes = 52 * tsize + tsize + 4 * 6 * tsize
+    records once per node.  QEPH adds HOURG(6:12) (60 columns) and writes one STI
+    per diagonal."""
+    ncol, nsti = (60, 2) if qeph else (53, 1)
+    reads = 16 + 16 + 8 + ncol * tsize
+    writes = (ncol - 1) * tsize + nsti * tsize + 4 * 6 * tsize
     return ne * (reads + writes) + nn * 96
 
 
@@ -71,13 +73,17 @@ def _shell_forces(s, dt1):
     return a, ar
 
 
-def shell_plate_leg(n, steps, err_cycles):
-    """The 512 x 512 BT plate (Ishell=1, 5 points, law 2, clamped edges, central
-    32 x 32 patch at 15 m/s +-10%): element-cycles/s of both forms and the mixed
-    form's error against the FP64 form after `steps` and after `err_cycles` cycles."""
+def shell_plate_leg(n, steps, err_cycles, ishell=1):
+    """The 512 x 512 plate (5 points, law 2, clamped edges, central 32 x 32 patch at
+    15 m/s +-10%) with Belytschko-Tsay (ishell 1) or QEPH (ishell 24) shells:
+    element-cycles/s of both forms and the mixed form's error against the FP64 form
+    after `steps` and after `err_cycles` cycles."""
     deck = dk.shell_plate(n=n)
+    qeph = ishell == 24
+    deck["prop"] = dict(deck["prop"], ishell=ishell)
     ne, nn = deck["ixc"].shape[0], deck["x"].shape[0]
-    res = {"workload": f"BT shell plate {n}x{n} ({ne} shells, {nn} nodes), 5 points, law 2",
+    kind = "QEPH (Ishell=24)" if qeph else "BT"
+    res = {"workload": f"{kind} shell plate {n}x{n} ({ne} shells, {nn} nodes), 5 points, law 2",
            "elements": ne, "nodes": nn, "cycles_timed": steps - 2}
     snaps = {}
     for prec, name in ((en.MIXED, "mixed"), (en.FP64, "fp64")):
@@ -99,13 +105,13 @@ def shell_plate_leg(n, steps, err_cycles):
         snaps[name] = snap
         s.close()
         ts = 4 if prec == en.MIXED else 8
-        eb = shell_elem_bytes(ne, nn, ts)
+        eb = shell_elem_bytes(ne, nn, ts, qeph)
         res[name] = dict(value=ne * (steps - 2) / dt, ms_per_step=1e3 * dt / (steps - 2),
                          elem_kernel_ms=el, node_kernel_ms=nd,
                          elem_kernel_gbs=eb / (el * 1e-3) / 1e9,
                          elem_algorithmic_bytes=eb,
                          node_kernel_gbs=shell_node_bytes(ne, nn, ts) / (nd * 1e-3) / 1e9)
-        print(f"shell plate {name}: {res[name]['value']:.4e} el-cyc/s, elem {el:.3f} ms, "
+        print(f"{kind} plate {name}: {res[name]['value']:.4e} el-cyc/s, elem {el:.3f} ms, "
               f"node {nd:.3f} ms", file=sys.stderr, flush=True)
     l2 = lambda x, y: float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
     res["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
@@ -239,6 +245,9 @@ def main():
     ap.add_argument("--shell-n", type=int, default=512)
     ap.add_argument("--shell-steps", type=int, default=1000)
     ap.add_argument("--shell-err-cycles", type=int, default=5000)
+    ap.add_argument("--no-qeph", action="store_true", help="skip the 512^2 QEPH plate leg")
+    ap.add_argument("--qeph-steps", type=int, default=1000)
+    ap.add_argument("--qeph-err-cycles", type=int, default=1000)
     ap.add_argument("--no-tubes", action="store_true", help="skip the 256-tube batch leg")
     ap.add_argument("--tubes", type=int, default=256)
     ap.add_argument("--tube-steps", type=int, default=3000)
@@ -279,6 +288,9 @@ def main():
     shell = None
     if not args.no_shell:
         shell = shell_plate_leg(args.shell_n, args.shell_steps, args.shell_err_cycles)
+    qeph = None
+    if not args.no_qeph:
+        qeph = shell_plate_leg(args.shell_n, args.qeph_steps, args.qeph_err_cycles, ishell=24)
     tubes = None
     if not args.no_tubes:
         tubes = tube_batch_leg(args.tubes, args.tube_steps, args.tube_solo_cycles,
@@ -318,6 +330,7 @@ def main():
         "fp64": fp64,
         "mixed_over_fp64": (value / fp64["value"]) if fp64 else None,
         "shell_plate": shell,
+        "qeph_plate": qeph,
         "tube_batch": tubes,
         "cpu_baseline": cpu,
     }

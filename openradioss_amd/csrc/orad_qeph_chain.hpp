@@ -34,15 +34,16 @@ struct QephOut {
   int alive;       // the candidate enters DT2T
 };
 
-// CZCORC1 + CZCORP5 results that the rest of the chain reads.
+// CZCORC1 + CZCORP5 + CZDEF results that the rest of the chain reads.  CZDEF runs
+// in this stage because its transverse-shear and hourglass rates difference the
+// out-of-plane velocities against the rotations, which nearly cancel in bending.
 template <typename TG>
 struct QephKin {
   TG e1[3], e2[3], e3[3];
   TG AREA, AREA_I, X13, X24, Y13, Y24, MX13, MX23, MX34, MY13, MY23, MY34, Z1;
   TG COREL[2][4];
   TG LL, FACN[2];
-  TG V13[3], V24[3], VHI[3];   // scaled as CZCORC1 returns them
-  TG RL[2][4];                 // RLXYZ after the projection
+  TG VDEF[8], DHG[6];          // CZDEF: generalised strain rates and hourglass rates
   TG VQN[3][4], DI[6];
   bool plat;
   bool degenerate;             // CLSKEW3 found a zero-area frame and set OFFG = 0
@@ -164,13 +165,13 @@ __device__ __forceinline__ void qeph_kinematics(const TG xg[3][4], const TG vg[3
     S1 = tmax(S1, TG(k.em10));
     o.LL = M<TG>::div(AREA, S1);
   }
-  TG(&RL)[2][4] = o.RL;
+  TG RL[2][4];                 // RLXYZ, projected below for warped elements
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     RL[0][c] = E1X * vrg[0][c] + E1Y * vrg[1][c] + E1Z * vrg[2][c];
     RL[1][c] = E2X * vrg[0][c] + E2Y * vrg[1][c] + E2Z * vrg[2][c];
   }
-  TG *V13 = o.V13, *V24 = o.V24, *VHI = o.VHI;
+  TG V13[3], V24[3], VHI[3];
   {
     TG G13[3], G24[3], GHI[3];
 #pragma unroll
@@ -348,6 +349,55 @@ __device__ __forceinline__ void qeph_kinematics(const TG xg[3][4], const TG vg[3
     V24[q] = V24[q] * AREA_I;
     VHI[q] = VHI[q] * FOURTH;
   }
+  // ---- CZDEF ----
+  {
+    TG *VDEF = o.VDEF, *DHG = o.DHG;
+    const TG MX23 = o.MX23, MX34 = o.MX34, MY23 = o.MY23, MY34 = o.MY34;
+    const TG AA = AREA_I;
+    TG R13[2], R24[2], RSOM[2], RHI[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      R13[q] = (RL[q][0] - RL[q][2]) * AA;
+      R24[q] = (RL[q][1] - RL[q][3]) * AA;
+      RSOM[q] = (RL[q][3] + RL[q][2] + RL[q][0] + RL[q][1]) * AA;
+      RHI[q] = (RL[q][0] - RL[q][1] + RL[q][2] - RL[q][3]) * FOURTH;
+    }
+    VDEF[0] = Y24 * V13[0] - Y13 * V24[0];
+    VDEF[1] = -X24 * V13[1] + X13 * V24[1];
+    const TG BXV2 = Y24 * V13[1] - Y13 * V24[1];
+    const TG BYV1 = -X24 * V13[0] + X13 * V24[0];
+    VDEF[2] = BXV2 + BYV1;
+    VDEF[5] = Y24 * R13[1] - Y13 * R24[1];
+    VDEF[6] = X24 * R13[0] - X13 * R24[0];
+    const TG BXR1 = Y13 * R24[0] - Y24 * R13[0];
+    const TG BYR2 = -X24 * R13[1] + X13 * R24[1];
+    VDEF[7] = BXR1 + BYR2;
+    const TG BCXY = AREA * FOURTH;
+    const TG BCX = V13[2] - MY13 * R13[0] + MX13 * R13[1];
+    const TG BCY = V24[2] + MY13 * R24[0] - MX13 * R24[1];
+    VDEF[3] = Y24 * BCX - Y13 * BCY + BCXY * RSOM[1];
+    VDEF[4] = X13 * BCY - X24 * BCX - BCXY * RSOM[0];
+    DHG[0] = VHI[0] - MX13 * VDEF[0] - MY13 * BYV1;
+    DHG[1] = VHI[1] - MX13 * BXV2 - MY13 * VDEF[1];
+    DHG[2] = RHI[1] - MX13 * VDEF[5] - MY13 * BYR2;
+    DHG[3] = -RHI[0] - MX13 * BXR1 - MY13 * VDEF[6];
+    DHG[4] = (VHI[2] * FOUR - (MY13 * RSOM[0] - MY23 * (R13[0] + R24[0]) + MX23 * (R13[1] + R24[1]) -
+                               MX13 * RSOM[1]) * AREA) * FOUR;
+    DHG[5] = (VHI[2] * FOUR - (MY13 * RSOM[0] - MY34 * (R13[0] - R24[0]) + MX34 * (R13[1] - R24[1]) -
+                               MX13 * RSOM[1]) * AREA) * FOUR;
+    DHG[0] = DHG[0] + (Y24 * V13[2] - Y13 * V24[2]) * Z1;
+    DHG[1] = DHG[1] + (-X24 * V13[2] + X13 * V24[2]) * Z1;
+    const TG DETA1 = Z1 * FOUR * AA;
+    VDEF[5] = VDEF[5] + (X13 * V13[0] - X24 * V24[0]) * DETA1;
+    VDEF[6] = VDEF[6] + (Y13 * V13[1] - Y24 * V24[1]) * DETA1;
+    VDEF[7] = VDEF[7] + (X13 * V13[1] - X24 * V24[1] + Y13 * V13[0] - Y24 * V24[0]) * DETA1;
+    if (offg < ZERO) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) VDEF[q] = ZERO;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) DHG[q] = ZERO;
+    }
+  }
 }
 
 // IO: get(col) / put(col, v) on this element's state columns (SC_NCOL_QEPH of them).
@@ -373,11 +423,6 @@ __device__ __forceinline__ void qeph_chain(const TG xg[3][4], const TG vg[3][4],
   const T X13 = T(g.X13), X24 = T(g.X24), Y13 = T(g.Y13), Y24 = T(g.Y24);
   const T MX13 = T(g.MX13), MX23 = T(g.MX23), MX34 = T(g.MX34);
   const T MY13 = T(g.MY13), MY23 = T(g.MY23), MY34 = T(g.MY34), Z1 = T(g.Z1);
-  T V13[3], V24[3], VHI[3], RL[2][4];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) { V13[q] = T(g.V13[q]); V24[q] = T(g.V24[q]); VHI[q] = T(g.VHI[q]); }
-#pragma unroll
-  for (int c = 0; c < 4; ++c) { RL[0][c] = T(g.RL[0][c]); RL[1][c] = T(g.RL[1][c]); }
   // ---- CNCOEF3B, IGTYP = 1, law 2, ITHK = 0 (moduli from PM, resol_init.F:689-696) ----
   const T thk02 = thk0 * thk0, VOL0 = thk0 * AREA;
   const T RHO = T(m.rho0), G = T(m.shear);
@@ -387,55 +432,13 @@ __device__ __forceinline__ void qeph_chain(const TG xg[3][4], const TG vg[3][4],
   const T SHF = T(p.fsh), SHFSR = M<T>::sqrt_(T(p.fsh));
   const T GS = G * SHF;
   const T AMU = T(p.dn);
-  // ---- CZDEF ----
+  // ---- CZDEF (in the kinematics stage) ----
   T VDEF[8], DHG[6];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) VDEF[q] = T(g.VDEF[q]);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) DHG[q] = T(g.DHG[q]);
   T off = tmin(ONE, tabs(offg));
-  {
-    const T AA = AREA_I;
-    T R13[2], R24[2], RSOM[2], RHI[2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      R13[q] = (RL[q][0] - RL[q][2]) * AA;
-      R24[q] = (RL[q][1] - RL[q][3]) * AA;
-      RSOM[q] = (RL[q][3] + RL[q][2] + RL[q][0] + RL[q][1]) * AA;
-      RHI[q] = (RL[q][0] - RL[q][1] + RL[q][2] - RL[q][3]) * FOURTH;
-    }
-    VDEF[0] = Y24 * V13[0] - Y13 * V24[0];
-    VDEF[1] = -X24 * V13[1] + X13 * V24[1];
-    const T BXV2 = Y24 * V13[1] - Y13 * V24[1];
-    const T BYV1 = -X24 * V13[0] + X13 * V24[0];
-    VDEF[2] = BXV2 + BYV1;
-    VDEF[5] = Y24 * R13[1] - Y13 * R24[1];
-    VDEF[6] = X24 * R13[0] - X13 * R24[0];
-    const T BXR1 = Y13 * R24[0] - Y24 * R13[0];
-    const T BYR2 = -X24 * R13[1] + X13 * R24[1];
-    VDEF[7] = BXR1 + BYR2;
-    const T BCXY = AREA * FOURTH;
-    const T BCX = V13[2] - MY13 * R13[0] + MX13 * R13[1];
-    const T BCY = V24[2] + MY13 * R24[0] - MX13 * R24[1];
-    VDEF[3] = Y24 * BCX - Y13 * BCY + BCXY * RSOM[1];
-    VDEF[4] = X13 * BCY - X24 * BCX - BCXY * RSOM[0];
-    DHG[0] = VHI[0] - MX13 * VDEF[0] - MY13 * BYV1;
-    DHG[1] = VHI[1] - MX13 * BXV2 - MY13 * VDEF[1];
-    DHG[2] = RHI[1] - MX13 * VDEF[5] - MY13 * BYR2;
-    DHG[3] = -RHI[0] - MX13 * BXR1 - MY13 * VDEF[6];
-    DHG[4] = (VHI[2] * FOUR - (MY13 * RSOM[0] - MY23 * (R13[0] + R24[0]) + MX23 * (R13[1] + R24[1]) -
-                               MX13 * RSOM[1]) * AREA) * FOUR;
-    DHG[5] = (VHI[2] * FOUR - (MY13 * RSOM[0] - MY34 * (R13[0] - R24[0]) + MX34 * (R13[1] - R24[1]) -
-                               MX13 * RSOM[1]) * AREA) * FOUR;
-    DHG[0] = DHG[0] + (Y24 * V13[2] - Y13 * V24[2]) * Z1;
-    DHG[1] = DHG[1] + (-X24 * V13[2] + X13 * V24[2]) * Z1;
-    const T DETA1 = Z1 * FOUR * AA;
-    VDEF[5] = VDEF[5] + (X13 * V13[0] - X24 * V24[0]) * DETA1;
-    VDEF[6] = VDEF[6] + (Y13 * V13[1] - Y24 * V24[1]) * DETA1;
-    VDEF[7] = VDEF[7] + (X13 * V13[1] - X24 * V24[1] + Y13 * V13[0] - Y24 * V24[0]) * DETA1;
-    if (offg < ZERO) {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) VDEF[q] = ZERO;
-#pragma unroll
-      for (int q = 0; q < 6; ++q) DHG[q] = ZERO;
-    }
-  }
   // ---- CZSTRA3 ----
   const T EXX = VDEF[0] * dt1, EYY = VDEF[1] * dt1, EXY = VDEF[2] * dt1;
   const T EYZ = VDEF[4] * dt1, EXZ = VDEF[3] * dt1;
