@@ -39,10 +39,12 @@ struct BrickIO {
   __device__ __forceinline__ void put(int f, T s) const { stnt(&field_ptr(a, f)[e], s); }
   __device__ __forceinline__ double vol0() const { return ldnt(&a.vol0[e]); }
   __device__ __forceinline__ void set_sti(T s) const { stnt(&a.sti[e], s); }
-  __device__ __forceinline__ void slot(int k, T fx, T fy, T fz) const {
+  // slot store at a slot address loaded beforehand (iads(k)), so the address load's
+  // latency overlaps the corner gathers instead of following the chain
+  __device__ __forceinline__ void slot_at(int sl, T fx, T fy, T fz) const {
     Slot3<T> f;
     f.x = fx; f.y = fy; f.z = fz;
-    a.fsky[iads(k)] = f;
+    a.fsky[sl] = f;
   }
 
  private:

@@ -60,9 +60,11 @@ void brick_forces_kernel(
   double my_dt = 1.0e300, my_eh = 0.0;
   if (e < e_end) {
     const BrickIO<T> io(a, e);
-    int nd[8];
+    int nd[8], sl[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) nd[k] = io.conn(k);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sl[k] = io.iads(k);
     T xd[8], yd[8], zd[8], vx[8], vy[8], vz[8];
     double voldp = 0.0;
     if (REL) {
@@ -103,7 +105,7 @@ void brick_forces_kernel(
     BrickOut<T> o;
     brick_chain<T, T *>(xd, yd, zd, vx, vy, vz, sig, eint, rho, vol0, voldp, pla, qvis, offg, epsd,
                         dt1, a.mat, a.prop, o,
-                        [&](int k, T fx, T fy, T fz) { io.slot(k, fx, fy, fz); });
+                        [&](int k, T fx, T fy, T fz) { io.slot_at(sl[k], fx, fy, fz); });
 #pragma unroll
     for (int c = 0; c < 6; ++c) io.set_sig(c, sig[c]);
     io.put(F_EINT, eint); io.put(F_RHO, rho); io.put(F_PLA, pla);

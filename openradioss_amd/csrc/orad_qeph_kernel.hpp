@@ -27,22 +27,23 @@ __global__ __launch_bounds__(BLOCK) void qeph_forces_kernel(ShellArgs<T> a, int 
   double my_dt = 1.0e300, my_eh = 0.0;
   if (e < e_end) {
     const int ne = a.numelc;
-    int nd[4];
+    int nd[4], sl[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) nd[c] = ldnt(&a.conn[(size_t)c * ne + e]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sl[c] = ldnt(&a.iadc[(size_t)c * ne + e]);
     TG xg[3][4], vg[3][4], vrg[3][4];
     shell_gather<T, TG, REL>(a, nd, xg, vg, vrg);
     const ShellIO<T> io{a.st, e, ne};
     const T thk0 = T(ldnt(&a.thke[e]));
     QephOut<T> o;
     Slot6<T> *fsky = a.fsky;
-    const int32_t *iadc = a.iadc;
     const T jc[3] = {T(a.mat.ca), T(a.mat.cb), T(a.mat.cn)};
     qeph_chain<T, TG>(xg, vg, vrg, thk0, io, dt1, a.mat, jc, a.prop, a.k, o,
                       [&](int c, T fx, T fy, T fz, T mx, T my, T mz) {
                         Slot6<T> f;
                         f.fx = fx; f.fy = fy; f.fz = fz; f.mx = mx; f.my = my; f.mz = mz;
-                        fsky[ldnt(&iadc[(size_t)c * ne + e])] = f;
+                        fsky[sl[c]] = f;
                       });
     stnt(&a.sti[e], o.sti13);
     stnt(&a.sti[(size_t)ne + e], o.sti24);

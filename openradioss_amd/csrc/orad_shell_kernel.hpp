@@ -71,22 +71,25 @@ __global__ __launch_bounds__(BLOCK) void shell_forces_kernel(ShellArgs<T> a, int
   double my_dt = 1.0e300, my_eh = 0.0;
   if (e < e_end) {
     const int ne = a.numelc;
-    int nd[4];
+    // slot addresses loaded with the connectivity, so their latency overlaps the
+    // corner gathers instead of following the chain
+    int nd[4], sl[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) nd[c] = ldnt(&a.conn[(size_t)c * ne + e]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sl[c] = ldnt(&a.iadc[(size_t)c * ne + e]);
     TG xg[3][4], vg[3][4], vrg[3][4];
     shell_gather<T, TG, REL>(a, nd, xg, vg, vrg);
     const ShellIO<T> io{a.st, e, ne};
     const T thk0 = T(ldnt(&a.thke[e]));
     ShellOut<T> o;
     Slot6<T> *fsky = a.fsky;
-    const int32_t *iadc = a.iadc;
     const T jc[3] = {T(a.mat.ca), T(a.mat.cb), T(a.mat.cn)};
     shell_chain<T, TG>(xg, vg, vrg, thk0, nd[2] == nd[3], io, dt1, a.mat, jc, a.prop, a.k, o,
                    [&](int c, T fx, T fy, T fz, T mx, T my, T mz) {
                      Slot6<T> f;
                      f.fx = fx; f.fy = fy; f.fz = fz; f.mx = mx; f.my = my; f.mz = mz;
-                     fsky[ldnt(&iadc[(size_t)c * ne + e])] = f;
+                     fsky[sl[c]] = f;
                    });
     stnt(&a.sti[e], o.sti);
     if (o.alive) my_dt = double(o.dt);
@@ -157,22 +160,23 @@ __global__ __launch_bounds__(BLOCK) void shell_batch_kernel(ShellArgs<T> a, Batc
         const int e = c0 + sub * BLOCK + (int)threadIdx.x;
         double my_eh = 0.0;
         if (e < td.z) {
-          int nd[4];
+          int nd[4], sl[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) nd[c] = ldnt(&a.conn[(size_t)c * ne + e]);
+#pragma unroll
+          for (int c = 0; c < 4; ++c) sl[c] = ldnt(&a.iadc[(size_t)c * ne + e]);
           TG xg[3][4], vg[3][4], vrg[3][4];
           shell_gather<T, TG, REL>(a, nd, xg, vg, vrg);
           const ShellIO<T> io{a.st, e, ne};
           const T thk0 = T(ldnt(&a.thke[e]));
           ShellOut<T> o;
           Slot6<T> *fsky = a.fsky;
-          const int32_t *iadc = a.iadc;
-          const T jc[3] = {T(mat.ca), T(mat.cb), T(mat.cn)};
+                const T jc[3] = {T(mat.ca), T(mat.cb), T(mat.cn)};
           shell_chain<T, TG>(xg, vg, vrg, thk0, nd[2] == nd[3], io, dt1, mat, jc, a.prop,
                              a.k, o, [&](int c, T fx, T fy, T fz, T mx, T my, T mz) {
                                Slot6<T> f;
                                f.fx = fx; f.fy = fy; f.fz = fz; f.mx = mx; f.my = my; f.mz = mz;
-                               fsky[ldnt(&iadc[(size_t)c * ne + e])] = f;
+                               fsky[sl[c]] = f;
                              });
           stnt(&a.sti[e], o.sti);
           if (o.alive) my_dt = (double(o.dt) < my_dt) ? double(o.dt) : my_dt;
