@@ -166,8 +166,8 @@ int orad_set_shell_prop(orad_ctx *ctx, const double *g, int ish, int npt);
  * shell/coquez/czforc3.F:436-808) with numerical damping dn = Dn (GEO(13); 0
  * means the default 0.015) and CVIS = 1.  Must precede orad_set_shells: QEPH
  * elements keep 60 state values (HOURG(6:12) after the 53 of BT) and one STI per
- * diagonal (FSKY(7) = STI*FACN, CUPDTN3P cupdtn3.F:488-700).  Batched models and
- * the adjoint take Ishell=1 only. */
+ * diagonal (FSKY(7) = STI*FACN, CUPDTN3P cupdtn3.F:488-700).  Batched models take
+ * either formulation; the adjoint takes Ishell=1 only. */
 int orad_set_shell_ishell(orad_ctx *ctx, int ishell, double dn);
 
 /* Drop-in for one shell group, CFORC3 semantics (shell/coque/cforc3.F:398-746,

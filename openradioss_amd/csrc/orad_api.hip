@@ -141,6 +141,13 @@ static int nelem(const orad_ctx *c) { return c->family == FAM_SHELL ? c->numelc 
 static hipError_t launch_cycle_elements(orad_ctx *c, int p) {
   if (c->batched) {
     const int grid = c->persist ? std::min(c->batch_grid, c->ntiles) : c->ntiles;
+    if (c->qeph()) {
+      if (c->precision == ORAD_MIXED)
+        return launch_qeph_batch_mixed(shell_args<float>(c), batch_args(c), p, c->persist, grid,
+                                       c->batch_lds, c->xcd_remap, c->stream);
+      return launch_qeph_batch_fp64(shell_args<double>(c), batch_args(c), p, c->persist, grid,
+                                    c->batch_lds, c->xcd_remap, c->stream);
+    }
     if (c->precision == ORAD_MIXED)
       return launch_shell_batch_mixed(shell_args<float>(c), batch_args(c), p, c->persist, grid,
                                       c->batch_lds, c->xcd_remap, c->stream);
@@ -1111,6 +1118,9 @@ int orad_set_shell_state(orad_ctx *c, const double *rows) {
 // ======================================================= batched models ===
 
 static int batch_occupancy(orad_ctx *c) {
+  if (c->qeph())
+    return c->precision == ORAD_MIXED ? qeph_batch_occupancy_mixed(c->batch_lds)
+                                      : qeph_batch_occupancy_fp64(c->batch_lds);
   return c->precision == ORAD_MIXED ? shell_batch_occupancy_mixed(c->batch_lds)
                                     : shell_batch_occupancy_fp64(c->batch_lds);
 }
@@ -1134,8 +1144,6 @@ int orad_set_shell_models(orad_ctx *c, int nmodel, const int *node_off, const in
   if (c->family != FAM_SHELL || !c->have_shmat)
     return fail(c, ORAD_ESTATE, "set_shell_models: set the shells and a law-2 material first");
   if (c->batched) return fail(c, ORAD_ESTATE, "set_shell_models: models already set");
-  if (c->qeph())
-    return fail(c, ORAD_ESTATE, "set_shell_models: Belytschko-Tsay shells only (Ishell=1)");
   if (nmodel < 1 || !node_off || !elem_off || (p == nullptr) != (ip == nullptr))
     return fail(c, ORAD_EINVAL, "set_shell_models: bad arguments");
   if (node_off[0] != 0 || elem_off[0] != 0 || node_off[nmodel] != c->numnod ||

@@ -39,6 +39,12 @@ hipError_t launch_qeph_mixed(const ShellArgs<float> &, int, int, const double *,
                              unsigned long long *, double *, int, hipStream_t);
 int qeph_block_fp64();
 int qeph_block_mixed();
+hipError_t launch_qeph_batch_fp64(const ShellArgs<double> &, const BatchArgs &, int, int, int,
+                                  int, int, hipStream_t);
+hipError_t launch_qeph_batch_mixed(const ShellArgs<float> &, const BatchArgs &, int, int, int,
+                                   int, int, hipStream_t);
+int qeph_batch_occupancy_fp64(int);
+int qeph_batch_occupancy_mixed(int);
 int shell_batch_occupancy_fp64(int);
 int shell_batch_occupancy_mixed(int);
 template <typename S> struct AdjArgs;

@@ -2,6 +2,7 @@
 // hourglass and force stages; nodes stay FP64.  The frame, warped-element
 // projection and velocity stage runs on coordinates and translational velocities
 // taken relative to corner 1 in FP64, in the precision ORAD_QEPH_KIN_FP64 selects.
+#include "orad_batch_kernel.hpp"
 #include "orad_qeph_kernel.hpp"
 
 #ifndef ORAD_QEPH_KIN_FP64
@@ -32,6 +33,32 @@ hipError_t launch_qeph_mixed(const ShellArgs<float> &a, int e_begin, int e_end, 
                      dim3(ORAD_QEPH_BLOCK), 0, s, a, e_begin, e_end, dt1p, dt2t, ehou_part,
                      xcd_remap);
   return hipGetLastError();
+}
+
+// Batched QEPH models: one workgroup per tile, or (persist) a resident grid taking
+// tiles from the counter; dyn_lds reserves LDS per workgroup to cap residency per CU.
+hipError_t launch_qeph_batch_mixed(const ShellArgs<float> &a, const BatchArgs &b, int parity,
+                                   int persist, int grid, int dyn_lds, int xcd_remap,
+                                   hipStream_t s) {
+  if (b.ntiles <= 0) return hipSuccess;
+  const size_t lds = dyn_lds > 0 ? (size_t)dyn_lds : 0;
+  if (persist)
+    hipLaunchKernelGGL((shell_batch_kernel<float, QephKinT, true, ORAD_QEPH_BLOCK, true, true>),
+                       dim3(grid), dim3(ORAD_QEPH_BLOCK), lds, s, a, b, parity, xcd_remap);
+  else
+    hipLaunchKernelGGL((shell_batch_kernel<float, QephKinT, true, ORAD_QEPH_BLOCK, false, true>),
+                       dim3(b.ntiles), dim3(ORAD_QEPH_BLOCK), lds, s, a, b, parity, xcd_remap);
+  return hipGetLastError();
+}
+
+// Resident workgroups per CU of the persistent batched QEPH kernel at dyn_lds bytes.
+int qeph_batch_occupancy_mixed(int dyn_lds) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &nb, shell_batch_kernel<float, QephKinT, true, ORAD_QEPH_BLOCK, true, true>,
+          ORAD_QEPH_BLOCK, dyn_lds > 0 ? (size_t)dyn_lds : 0) != hipSuccess)
+    return 0;
+  return nb;
 }
 
 }  // namespace orad

@@ -1,6 +1,6 @@
 // orad_shell_fp64.hip -- shell element kernel, FP64 form. Built with -ffp-contract=off so its arithmetic follows the
 // reference's double-precision build operation by operation.
-#include "orad_shell_kernel.hpp"
+#include "orad_batch_kernel.hpp"
 
 #ifndef ORAD_SHELL_BLOCK
 #define ORAD_SHELL_BLOCK 64
@@ -30,10 +30,10 @@ hipError_t launch_shell_batch_fp64(const ShellArgs<double> &a, const BatchArgs &
   if (b.ntiles <= 0) return hipSuccess;
   const size_t lds = dyn_lds > 0 ? (size_t)dyn_lds : 0;
   if (persist)
-    hipLaunchKernelGGL((shell_batch_kernel<double, double, false, ORAD_SHELL_BLOCK, true>), dim3(grid),
+    hipLaunchKernelGGL((shell_batch_kernel<double, double, false, ORAD_SHELL_BLOCK, true, false>), dim3(grid),
                        dim3(ORAD_SHELL_BLOCK), lds, s, a, b, parity, xcd_remap);
   else
-    hipLaunchKernelGGL((shell_batch_kernel<double, double, false, ORAD_SHELL_BLOCK, false>),
+    hipLaunchKernelGGL((shell_batch_kernel<double, double, false, ORAD_SHELL_BLOCK, false, false>),
                        dim3(b.ntiles), dim3(ORAD_SHELL_BLOCK), lds, s, a, b, parity, xcd_remap);
   return hipGetLastError();
 }
@@ -42,7 +42,7 @@ hipError_t launch_shell_batch_fp64(const ShellArgs<double> &a, const BatchArgs &
 int shell_batch_occupancy_fp64(int dyn_lds) {
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &nb, shell_batch_kernel<double, double, false, ORAD_SHELL_BLOCK, true>, ORAD_SHELL_BLOCK,
+          &nb, shell_batch_kernel<double, double, false, ORAD_SHELL_BLOCK, true, false>, ORAD_SHELL_BLOCK,
           dyn_lds > 0 ? (size_t)dyn_lds : 0) != hipSuccess)
     return 0;
   return nb;

@@ -2,7 +2,7 @@
 // nodes stay FP64.  The frame and kinematics stage (local frame, shape derivatives,
 // strain and curvature rates) runs on coordinates and translational velocities
 // taken relative to corner 1 in FP64, in the precision ORAD_SHELL_KIN_FP64 selects.
-#include "orad_shell_kernel.hpp"
+#include "orad_batch_kernel.hpp"
 
 #ifndef ORAD_SHELL_KIN_FP64
 #define ORAD_SHELL_KIN_FP64 1
@@ -42,10 +42,10 @@ hipError_t launch_shell_batch_mixed(const ShellArgs<float> &a, const BatchArgs &
   if (b.ntiles <= 0) return hipSuccess;
   const size_t lds = dyn_lds > 0 ? (size_t)dyn_lds : 0;
   if (persist)
-    hipLaunchKernelGGL((shell_batch_kernel<float, ShellKinT, true, ORAD_SHELL_BLOCK, true>), dim3(grid),
+    hipLaunchKernelGGL((shell_batch_kernel<float, ShellKinT, true, ORAD_SHELL_BLOCK, true, false>), dim3(grid),
                        dim3(ORAD_SHELL_BLOCK), lds, s, a, b, parity, xcd_remap);
   else
-    hipLaunchKernelGGL((shell_batch_kernel<float, ShellKinT, true, ORAD_SHELL_BLOCK, false>),
+    hipLaunchKernelGGL((shell_batch_kernel<float, ShellKinT, true, ORAD_SHELL_BLOCK, false, false>),
                        dim3(b.ntiles), dim3(ORAD_SHELL_BLOCK), lds, s, a, b, parity, xcd_remap);
   return hipGetLastError();
 }
@@ -54,7 +54,7 @@ hipError_t launch_shell_batch_mixed(const ShellArgs<float> &a, const BatchArgs &
 int shell_batch_occupancy_mixed(int dyn_lds) {
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &nb, shell_batch_kernel<float, ShellKinT, true, ORAD_SHELL_BLOCK, true>, ORAD_SHELL_BLOCK,
+          &nb, shell_batch_kernel<float, ShellKinT, true, ORAD_SHELL_BLOCK, true, false>, ORAD_SHELL_BLOCK,
           dyn_lds > 0 ? (size_t)dyn_lds : 0) != hipSuccess)
     return 0;
   return nb;

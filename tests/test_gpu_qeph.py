@@ -196,12 +196,93 @@ def test_qeph_run_to_run_bitwise():
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
 
 
-def test_qeph_rejects_batched_models_and_adjoint():
+def test_qeph_rejects_adjoint():
     d = qeph_plate(8, 2, 1.0e4)
     s = en.ShellSolver(d, en.FP64)
-    nn, ne = d["x"].shape[0], d["ixc"].shape[0]
-    with pytest.raises(en.OradError):
-        s.set_models([0, nn], [0, ne])
     with pytest.raises(en.OradError):
         s.adjoint_forward(10, 5)
     s.close()
+
+
+NODE_KEYS = ("x", "v", "vr", "ms", "in", "icodt", "icodr", "fv_dir", "fv_val")
+
+
+def qeph_tubes(n, seed):
+    d = dk.shell_tubes(ntube=n, side=(8.0, 16.0), length=(12.0, 30.0), h=(2.0, 4.0),
+                       thk=(0.8, 2.5), ca=(200.0, 400.0), seed=seed)
+    d["prop"] = dict(d["prop"], ishell=24)
+    return d
+
+
+def model_deck(d, m):
+    n0, n1 = d["node_off"][m], d["node_off"][m + 1]
+    e0, e1 = d["elem_off"][m], d["elem_off"][m + 1]
+    out = {k: d[k][n0:n1] for k in NODE_KEYS}
+    out["ixc"] = d["ixc"][e0:e1] - n0
+    out["uid"] = d["uid"][e0:e1]
+    out["thk"] = d["thk"][e0:e1]
+    out["mat"] = d["mats"][m]
+    out["prop"] = d["prop"]
+    return out
+
+
+@pytest.mark.parametrize("prec", [en.FP64, en.MIXED])
+def test_qeph_batch_equals_models_run_alone(prec):
+    # batched QEPH tubes (own material and time step each) against each tube in its
+    # own context: FP64 bitwise on nodes, state (60 values) and time; the mixed form
+    # (a different kernel, compiled with contraction) within 1e-6
+    d = qeph_tubes(4, 9)
+    s = en.ShellSolver(d, prec)
+    s.run_cycles(200)
+    nd, st = s.nodes(), s.state()
+    times = [s.model_time(m) for m in range(s.nmodel)]
+    s.close()
+    assert st.shape[1] == po.SH_NST
+    for m in range(4):
+        n0, n1 = d["node_off"][m], d["node_off"][m + 1]
+        e0, e1 = d["elem_off"][m], d["elem_off"][m + 1]
+        one = en.ShellSolver(model_deck(d, m), prec)
+        one.run_cycles(200)
+        n1d, s1, t1 = one.nodes(), one.state(), one.time()
+        one.close()
+        if prec == en.FP64:
+            for k in ("x", "v", "vr", "d"):
+                assert np.array_equal(nd[k][n0:n1], n1d[k]), (m, k)
+            assert np.array_equal(st[e0:e1], s1), m
+            assert (times[m].tt, times[m].dt2, times[m].ncycle) == (t1.tt, t1.dt2, t1.ncycle), m
+        else:
+            for k in ("x", "v", "vr", "d"):
+                assert rel(nd[k][n0:n1], n1d[k]) < 1e-6, (m, k)
+            assert times[m].ncycle == t1.ncycle and times[m].tt == pytest.approx(t1.tt, rel=1e-6)
+    assert len({t.dt2 for t in times}) == len(times)
+
+
+def test_qeph_batch_model_vs_oracle_and_launches():
+    d = qeph_tubes(3, 13)
+    m = 2
+    md = model_deck(d, m)
+    o = po.ShellModel(md, nthreads=4)
+    o.cycle(150)
+    ref = None
+    for launch in (None, dict(persistent=1, lds_kb=0, xcd_remap=0),
+                   dict(persistent=0, lds_kb=40, xcd_remap=1)):
+        s = en.ShellSolver(d, en.FP64)
+        if launch:
+            s.set_launch(**launch)
+        s.run_cycles(150)
+        got = (s.nodes(), s.state(), s.model_time(m))
+        s.close()
+        if ref is None:
+            ref = got
+        else:
+            for k in ("x", "v", "vr", "d"):
+                assert np.array_equal(got[0][k], ref[0][k]), (launch, k)
+            assert np.array_equal(got[1], ref[1]), launch
+    n0, n1 = d["node_off"][m], d["node_off"][m + 1]
+    e0, e1 = d["elem_off"][m], d["elem_off"][m + 1]
+    nd, st, t = ref
+    assert t.tt == pytest.approx(o.tt, rel=1e-12)
+    for k, r in (("x", o.x), ("v", o.v), ("vr", o.vr), ("d", o.d)):
+        assert rel(nd[k][n0:n1], r) < 1e-8, k
+    for cols in (slice(po.SH_SIG0, po.SH_SIG0 + 25), po.SH_HOUR, po.SH_HOURX):
+        assert rel(st[e0:e1, cols], o.st[:, cols]) < 1e-8
