@@ -216,6 +216,46 @@ def copy_calibration():
     return 0
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=20):
+    """Achievable HBM bandwidth on this box: a plain 1 GiB device-to-device copy
+    (the runtime's copy kernel; read + write counted), timed with HIP events over
+    `reps` copies after a warm-up, through the HIP runtime liborad.so uses."""
+    import ctypes as C
+    en.load_library()
+    hip = C.CDLL("libamdhip64.so", mode=C.RTLD_GLOBAL)
+    P = C.c_void_p
+    hip.hipMalloc.argtypes = [C.POINTER(P), C.c_size_t]
+    hip.hipFree.argtypes = [P]
+    hip.hipMemcpy.argtypes = [P, P, C.c_size_t, C.c_int]
+    hip.hipEventCreate.argtypes = [C.POINTER(P)]
+    hip.hipEventRecord.argtypes = [P, P]
+    hip.hipEventSynchronize.argtypes = [P]
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), P, P]
+    hip.hipEventDestroy.argtypes = [P]
+    D2D = 3                                      # hipMemcpyDeviceToDevice
+    x, y, e0, e1 = P(), P(), P(), P()
+    if hip.hipMalloc(C.byref(x), nbytes) or hip.hipMalloc(C.byref(y), nbytes):
+        raise RuntimeError("copy_bandwidth: hipMalloc failed")
+    try:
+        for _ in range(3):
+            hip.hipMemcpy(y, x, nbytes, D2D)
+        hip.hipEventCreate(C.byref(e0))
+        hip.hipEventCreate(C.byref(e1))
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpy(y, x, nbytes, D2D)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(x)
+        hip.hipFree(y)
+    return 2.0 * nbytes / (ms.value / reps * 1e-3) / 1e9
+
+
 def measured_traffic(prefix):
     """HBM bytes per launch of the kernel whose name starts with `prefix`, from the
     committed counter summary (profiles/counters_200cube.json), or None."""
@@ -241,6 +281,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-copy", action="store_true",
+                    help="skip the device-copy bandwidth measurement")
     ap.add_argument("--no-shell", action="store_true", help="skip the 512^2 shell plate leg")
     ap.add_argument("--shell-n", type=int, default=512)
     ap.add_argument("--shell-steps", type=int, default=1000)
@@ -295,6 +337,10 @@ def main():
     if not args.no_tubes:
         tubes = tube_batch_leg(args.tubes, args.tube_steps, args.tube_solo_cycles,
                                autotune=not args.no_autotune)
+    copy_gbs = None
+    if not args.no_copy:
+        copy_gbs = copy_bandwidth()
+        print(f"device copy: {copy_gbs:.0f} GB/s (read + write)", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(deck, args.cpu_cycles)
@@ -323,7 +369,9 @@ def main():
                      "frac": ach / HBM_PEAK_GBS,
                      "traffic": (measured_traffic("brick_forces_kernel<float")
                                  if n == 200 else None),
-                     "algorithmic_bytes_per_launch": eb, "mean_launch_ms": el32},
+                     "algorithmic_bytes_per_launch": eb, "mean_launch_ms": el32,
+                     "measured_copy_gbs": copy_gbs,
+                     "frac_of_measured_copy": (ach / copy_gbs) if copy_gbs else None},
         "cycle_hbm_frac": cycle_gbs / HBM_PEAK_GBS,
         "node_kernel": {"mean_launch_ms": nd32,
                         "achieved_gbs": nb / (nd32 * 1e-3) / 1e9, "algorithmic_bytes": nb},
