@@ -52,18 +52,20 @@ def build(verbose=False):
 
 FORTRAN_DIR = os.path.join(HERE, "fortran")
 DRIVER = os.path.join(FORTRAN_DIR, "resol_brick")
+DRIVER_SHELL = os.path.join(FORTRAN_DIR, "resol_shell")
 
 
 def build_fortran(verbose=False):
-    """ISO_C_BINDING module + RESOL-style driver, linked against liborad.so."""
+    """ISO_C_BINDING module + RESOL-style drivers (bricks, shells), linked against liborad.so."""
     fc = os.environ.get("FC", "/opt/rocm/bin/amdflang")
-    cmd = [fc, "-O2", "-ffp-contract=off", "-J", BUILD,
-           os.path.join(FORTRAN_DIR, "orad_bind.F90"), os.path.join(FORTRAN_DIR, "resol_brick.F90"),
-           "-L", HERE, "-lorad", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER + ".tmp"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(DRIVER + ".tmp", DRIVER)
+    for drv in (DRIVER, DRIVER_SHELL):
+        cmd = [fc, "-O2", "-ffp-contract=off", "-J", BUILD,
+               os.path.join(FORTRAN_DIR, "orad_bind.F90"), drv + ".F90",
+               "-L", HERE, "-lorad", "-Wl,-rpath,$ORIGIN/..", "-o", drv + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(drv + ".tmp", drv)
     return DRIVER
 
 

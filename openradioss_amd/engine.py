@@ -464,6 +464,34 @@ def write_deck(deck, path):
         np.array([prop[k] for k in ("qa", "qb", "hvis", "dtfac")], np.float64).tofile(f)
 
 
+def write_shell_deck(deck, path):
+    """Binary shell deck for the Fortran driver (openradioss_amd/fortran/resol_shell)."""
+    nn, ne = deck["x"].shape[0], deck["ixc"].shape[0]
+    mat, prop = deck["mat"], deck["prop"]
+    with open(path, "wb") as f:
+        np.array([DECK_MAGIC, 2, nn, ne], np.int32).tofile(f)
+        for k in ("x", "v"):
+            np.ascontiguousarray(deck[k], np.float64).tofile(f)
+        np.ascontiguousarray(deck.get("vr", np.zeros((nn, 3))), np.float64).tofile(f)
+        np.ascontiguousarray(deck["ms"], np.float64).tofile(f)
+        np.ascontiguousarray(deck["in"], np.float64).tofile(f)
+        np.ascontiguousarray(deck["icodt"], np.int32).tofile(f)
+        np.ascontiguousarray(deck["icodr"], np.int32).tofile(f)
+        ixc_from_deck(deck).tofile(f)
+        np.ascontiguousarray(deck["thk"], np.float64).tofile(f)
+        np.array([mat[k] for k in SHMAT_KEYS], np.float64).tofile(f)
+        np.array([mat["icc"], mat["vp"], mat["israte"], mat["ipla"]], np.int32).tofile(f)
+        np.array([prop[k] for k in ("hm", "hf", "hr", "dm", "fsh", "dtfac", "dtmin")],
+                 np.float64).tofile(f)
+        np.array([prop["ish"], prop["npt"], prop.get("ishell", 1), 0], np.int32).tofile(f)
+        np.array([prop.get("dn", 0.0)], np.float64).tofile(f)
+
+
+def read_shell_driver_output(path, numnod):
+    a = np.fromfile(path, np.float64).reshape(4, numnod, 3)
+    return a[0], a[1], a[2], a[3]
+
+
 def read_driver_output(path, numnod):
     a = np.fromfile(path, np.float64)
     x, v, d = a[:3 * numnod], a[3 * numnod:6 * numnod], a[6 * numnod:9 * numnod]

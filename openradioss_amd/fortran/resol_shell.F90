@@ -1,0 +1,125 @@
+!  resol_shell.F90 -- a minimal RESOL-style explicit shell driver over the C ABI.
+!
+!  usage: resol_shell <deck.bin> <ncycle> <group|device> <out.bin> [mixed]
+!
+!  group : the reference's call surface.  Each cycle follows RESOL
+!          (engine/source/engine/resol.F:2586-9595): DT1=DT2, DT2=EP06; the
+!          FORINTC group loop calls CFORC3's (or, for Ishell=24, CZFORC3's)
+!          drop-in orad_cforc3 per NVSIZ=128 group; ASSPAR4's drop-in
+!          orad_asspar4_rot returns A, AR, STIFN, STIFR; then DT2 selection,
+!          ACCELE (translations and rotations, accele.F:64-109), BCS10 (ICODT,
+!          ICODR), VELOCITY and DEPLA run here on the host in FP64, and X, V, VR
+!          go back to the device.
+!  device: the whole cycle on the GPU, orad_run_cycles.
+!
+!  deck.bin (written by openradioss_amd.engine.write_shell_deck): int32 magic,
+!  version 2, NUMNOD, NUMELC; X(3,NUMNOD), V(3,NUMNOD), VR(3,NUMNOD), MS(NUMNOD),
+!  IN(NUMNOD) real(8); ICODT(NUMNOD), ICODR(NUMNOD), IXC(7,NUMELC) int32;
+!  THKE(NUMELC), law-2 parameters(16) real(8); ICC, VP, ISRATE, IPLA int32;
+!  hm hf hr dm fsh DTFAC1(3) DTMIN1(3) real(8); ISH, NPT, ISHELL, pad int32; Dn real(8).
+program resol_shell
+  use iso_c_binding
+  use orad_bind
+  implicit none
+  integer, parameter :: NVSIZ = 128
+  character(len=512) :: deckf, outf, arg
+  integer(c_int) :: hdr(4), numnod, numelc, ncycle, rc, prec, ips(4), iprop(4)
+  integer :: ncyc, nft, nel, n, u
+  real(c_double), allocatable :: x(:,:), v(:,:), vr(:,:), d(:,:), a(:,:), ar(:,:)
+  real(c_double), allocatable :: ms(:), in(:), stifn(:), stifr(:), thke(:)
+  integer(c_int), allocatable :: icodt(:), icodr(:), ixc(:,:)
+  real(c_double) :: ps(16), g7(7), dn, tt, dt1, dt2, dt12, dt2old, dtmx, dt2t, rtmp, vdt
+  type(c_ptr) :: ctx
+  type(orad_time_t) :: t
+  logical :: group_mode
+
+  call get_command_argument(1, deckf)
+  call get_command_argument(2, arg); read(arg, *) ncycle
+  call get_command_argument(3, arg); group_mode = (trim(arg) == 'group')
+  call get_command_argument(4, outf)
+  prec = ORAD_FP64
+  if (command_argument_count() >= 5) then
+    call get_command_argument(5, arg)
+    if (trim(arg) == 'mixed') prec = ORAD_MIXED
+  end if
+
+  open(newunit=u, file=trim(deckf), access='stream', form='unformatted', status='old')
+  read(u) hdr
+  if (hdr(1) /= int(z'4F524144', c_int) .or. hdr(2) /= 2) stop 'bad shell deck file'
+  numnod = hdr(3); numelc = hdr(4)
+  allocate(x(3,numnod), v(3,numnod), vr(3,numnod), d(3,numnod), a(3,numnod), ar(3,numnod))
+  allocate(ms(numnod), in(numnod), stifn(numnod), stifr(numnod), thke(numelc))
+  allocate(icodt(numnod), icodr(numnod), ixc(7,numelc))
+  read(u) x, v, vr, ms, in, icodt, icodr, ixc, thke, ps, ips, g7, iprop, dn
+  close(u)
+
+  call orad_check(ctx, orad_create(ctx, 0_c_int, prec), 'create')
+  call orad_check(ctx, orad_set_nodes(ctx, numnod, x, v, ms, icodt), 'set_nodes')
+  call orad_check(ctx, orad_set_nodes_rot(ctx, vr, in, icodr), 'set_nodes_rot')
+  call orad_check(ctx, orad_set_shell_law2(ctx, ps, ips), 'set_shell_law2')
+  call orad_check(ctx, orad_set_shell_prop(ctx, g7, iprop(1), iprop(2)), 'set_shell_prop')
+  call orad_check(ctx, orad_set_shell_ishell(ctx, iprop(3), dn), 'set_shell_ishell')
+  call orad_check(ctx, orad_set_shells(ctx, numelc, ixc, thke), 'set_shells')
+  d = 0.0d0
+
+  if (.not. group_mode) then
+    call orad_check(ctx, orad_run_cycles(ctx, ncycle, t), 'run_cycles')
+    call orad_check(ctx, orad_get_nodes(ctx, x, v, d, a, stifn), 'get_nodes')
+    call orad_check(ctx, orad_get_nodes_rot(ctx, vr, ar), 'get_nodes_rot')
+  else
+    ! starter values (contrl.F:1108-1113) and engine DTMX default
+    tt = 0.0d0; dt1 = 0.0d0; dt2 = 0.0d0; dt2old = 1.0d6/1.1d0; dtmx = 1.0d20
+    do ncyc = 1, ncycle
+      dt1 = dt2
+      dt2 = 1.0d6
+      dt2t = 1.0d30
+      do nft = 0, numelc - 1, NVSIZ
+        nel = min(NVSIZ, numelc - nft)
+        call orad_check(ctx, orad_cforc3(ctx, nft, nel, dt1, dt2t), 'cforc3')
+      end do
+      call orad_check(ctx, orad_asspar4_rot(ctx, a, ar, stifn, stifr), 'asspar4_rot')
+      if (dt2t < dt2) dt2 = dt2t
+      dt2 = min(dt2, 1.1*dt2old, dtmx)
+      dt2old = dt2
+      dt12 = 0.5d0*(dt1 + dt2)
+      do n = 1, numnod
+        if (ms(n) > 0.0d0) then                       ! ACCELE, translations
+          rtmp = 1.0d0/ms(n)
+          a(1,n) = a(1,n)*rtmp; a(2,n) = a(2,n)*rtmp; a(3,n) = a(3,n)*rtmp
+        else
+          a(:,n) = 0.0d0
+        end if
+        if (in(n) > 0.0d0) then                       ! ACCELE, rotations
+          rtmp = 1.0d0/in(n)
+          ar(1,n) = ar(1,n)*rtmp; ar(2,n) = ar(2,n)*rtmp; ar(3,n) = ar(3,n)*rtmp
+        else
+          ar(:,n) = 0.0d0
+        end if
+        if (iand(icodt(n), 4) /= 0) then; v(1,n) = 0.0d0; a(1,n) = 0.0d0; end if   ! BCS10
+        if (iand(icodt(n), 2) /= 0) then; v(2,n) = 0.0d0; a(2,n) = 0.0d0; end if
+        if (iand(icodt(n), 1) /= 0) then; v(3,n) = 0.0d0; a(3,n) = 0.0d0; end if
+        if (iand(icodr(n), 4) /= 0) then; vr(1,n) = 0.0d0; ar(1,n) = 0.0d0; end if
+        if (iand(icodr(n), 2) /= 0) then; vr(2,n) = 0.0d0; ar(2,n) = 0.0d0; end if
+        if (iand(icodr(n), 1) /= 0) then; vr(3,n) = 0.0d0; ar(3,n) = 0.0d0; end if
+        v(1,n) = v(1,n) + dt12*a(1,n)                ! VELOCITY
+        v(2,n) = v(2,n) + dt12*a(2,n)
+        v(3,n) = v(3,n) + dt12*a(3,n)
+        vr(1,n) = vr(1,n) + dt12*ar(1,n)
+        vr(2,n) = vr(2,n) + dt12*ar(2,n)
+        vr(3,n) = vr(3,n) + dt12*ar(3,n)
+        vdt = dt2*v(1,n); d(1,n) = d(1,n) + vdt; x(1,n) = x(1,n) + vdt     ! DEPLA
+        vdt = dt2*v(2,n); d(2,n) = d(2,n) + vdt; x(2,n) = x(2,n) + vdt
+        vdt = dt2*v(3,n); d(3,n) = d(3,n) + vdt; x(3,n) = x(3,n) + vdt
+      end do
+      tt = tt + dt2
+      call orad_check(ctx, orad_set_nodes(ctx, numnod, x, v, ms, icodt), 'set_nodes')
+      call orad_check(ctx, orad_set_nodes_rot(ctx, vr, in, icodr), 'set_nodes_rot')
+    end do
+    write(*,'(a,i0,a,es22.15,a,es22.15)') 'cycles ', ncycle, ' tt ', tt, ' dt2 ', dt2
+  end if
+
+  open(newunit=u, file=trim(outf), access='stream', form='unformatted', status='replace')
+  write(u) x, v, d, vr
+  close(u)
+  rc = orad_destroy(ctx)
+end program resol_shell
