@@ -166,8 +166,8 @@ int orad_set_shell_prop(orad_ctx *ctx, const double *g, int ish, int npt);
  * shell/coquez/czforc3.F:436-808) with numerical damping dn = Dn (GEO(13); 0
  * means the default 0.015) and CVIS = 1.  Must precede orad_set_shells: QEPH
  * elements keep 60 state values (HOURG(6:12) after the 53 of BT) and one STI per
- * diagonal (FSKY(7) = STI*FACN, CUPDTN3P cupdtn3.F:488-700).  Batched models take
- * either formulation; the adjoint takes Ishell=1 only. */
+ * diagonal (FSKY(7) = STI*FACN, CUPDTN3P cupdtn3.F:488-700).  Batched models and
+ * the adjoint take either formulation. */
 int orad_set_shell_ishell(orad_ctx *ctx, int ishell, double dn);
 
 /* Drop-in for one shell group, CFORC3 semantics (shell/coque/cforc3.F:398-746,
@@ -218,8 +218,9 @@ int orad_autotune(orad_ctx *ctx, int cycles, orad_launch_t *best, double *best_m
  * m2cplr.F:448-469).  orad_adjoint_forward runs ncycle cycles from the current
  * state as orad_run_cycles does, keeping a checkpoint every `every` cycles.
  * orad_adjoint_reverse takes the adjoints of the final state -- lam_x, lam_v,
- * lam_vr, lam_d as X(3,NUMNOD) arrays, lam_state as NUMELC rows of 33
- * (SIG(5) of points 1..5, PLA(1..5), HOUR(1:3)); null = zero -- and, recomputing
+ * lam_vr, lam_d as X(3,NUMNOD) arrays, lam_state as NUMELC rows of 33 for
+ * Ishell=1 (SIG(5) of points 1..5, PLA(1..5), HOUR(1:3)) or 42 for QEPH (SIG, PLA,
+ * HOURG(1:12)); null = zero -- and, recomputing
  * each segment from its checkpoint, returns the gradients with respect to the
  * law-2 A, B, n (g_jc[3]), the element thicknesses THKE (g_thk[NUMELC]), the
  * nodal masses and rotational inertias (g_ms, g_in [NUMNOD]), and the adjoints of

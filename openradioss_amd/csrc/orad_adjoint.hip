@@ -12,15 +12,17 @@
 //   adj_node_kernel    adjoints of v_{k+1}, x, d -> adjoint of S and R per node,
 //                      of v_k through the update, and the MS / IN sensitivities;
 //   adj_elem_kernel    vector-Jacobian product of the element chain: the CFORC3
-//                      chain (orad_shell_chain.hpp) instantiated on Dual<K> seeds K
-//                      of the element's 73 inputs per thread (4 corners x (x, v, vr),
-//                      25 stresses, 5 plastic strains, 3 incremental hourglass
-//                      forces, thickness, A, B, n) and contracts the tangents of
-//                      its outputs (slots, new state) with their adjoints;
+//                      chain (orad_shell_chain.hpp), or for QEPH the CZFORC3 chain
+//                      (orad_qeph_chain.hpp), instantiated on Dual<K> seeds K of the
+//                      element's inputs per thread (4 corners x (x, v, vr), 25
+//                      stresses, 5 plastic strains, the incremental hourglass forces
+//                      -- 3 for BT, 12 for QEPH --, thickness, A, B, n: 73 or 82) and
+//                      contracts the tangents of its outputs (slots, new state) with
+//                      their adjoints;
 //   adj_gather_kernel  per node, the corner contributions in slot order (the
 //                      PARITH/ON CSR), so the sums are deterministic.
 #include "orad_dual.hpp"
-#include "orad_shell_chain.hpp"
+#include "orad_qeph_chain.hpp"
 #include "orad_brick_io.hpp"
 #include "orad_adjoint_args.hpp"
 
@@ -29,7 +31,7 @@ namespace orad {
 constexpr int ADJ_BLOCK = 64;
 constexpr int ADJ_K = 4;         // tangents per thread
 
-template <typename S, int K>
+template <typename S, int K, bool QEPH>
 struct AdjIO {
   const S *st;
   int e, ne, i0;
@@ -37,7 +39,7 @@ struct AdjIO {
   double *acc;
   __device__ __forceinline__ Dual<K> get(int c) const {
     Dual<K> r{double(st[(size_t)c * ne + e])};
-    const int j = adj_state_index(c);
+    const int j = adj_state_index<QEPH>(c);
     if (j >= 0) {
       const int q = 36 + j - i0;
 #pragma unroll
@@ -46,7 +48,7 @@ struct AdjIO {
     return r;
   }
   __device__ __forceinline__ void put(int c, const Dual<K> &v) const {
-    const int j = adj_state_index(c);
+    const int j = adj_state_index<QEPH>(c);
     if (j < 0) return;
     const double l = lst_in[(size_t)j * ne + e];
 #pragma unroll
@@ -54,9 +56,10 @@ struct AdjIO {
   }
 };
 
-template <typename S, int K>
+template <typename S, int K, bool QEPH>
 __global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_kernel(AdjArgs<S> a, int nchunk) {
   typedef Dual<K> D;
+  typedef AdjShape<QEPH> SH;
   const int ne = a.numelc;
   const long idx = (long)blockIdx.x * ADJ_BLOCK + threadIdx.x;
   if (idx >= (long)ne * nchunk) return;
@@ -87,34 +90,40 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_kernel(AdjArgs<S> a, int n
   D jc[3] = {D(a.mat.ca), D(a.mat.cb), D(a.mat.cn)};
 #pragma unroll
   for (int t = 0; t < K; ++t) {
-    thk0.d[t] = (69 - i0 == t) ? 1.0 : 0.0;
+    thk0.d[t] = (SH::I_THK - i0 == t) ? 1.0 : 0.0;
 #pragma unroll
-    for (int j = 0; j < 3; ++j) jc[j].d[t] = (70 + j - i0 == t) ? 1.0 : 0.0;
+    for (int j = 0; j < 3; ++j) jc[j].d[t] = (SH::I_JC + j - i0 == t) ? 1.0 : 0.0;
   }
   double acc[K];
 #pragma unroll
   for (int t = 0; t < K; ++t) acc[t] = 0.0;
-  const AdjIO<S, K> io{a.st, e, ne, i0, a.lst_in, acc};
-  ShellOut<D> o;
+  const AdjIO<S, K, QEPH> io{a.st, e, ne, i0, a.lst_in, acc};
   const double *lamS = a.lamS, *lamR = a.lamR;
-  shell_chain<D, D>(xg, vg, vrg, thk0, nd[2] == nd[3], io, a.dt1, a.mat, jc, a.prop, a.k, o,
-                    [&](int c, const D &fx, const D &fy, const D &fz, const D &mx, const D &my,
-                        const D &mz) {
-                      const size_t n3 = 3 * (size_t)nd[c];
-                      const double l0 = lamS[n3], l1 = lamS[n3 + 1], l2 = lamS[n3 + 2];
-                      const double r0 = lamR[n3], r1 = lamR[n3 + 1], r2 = lamR[n3 + 2];
+  auto emit = [&](int c, const D &fx, const D &fy, const D &fz, const D &mx, const D &my,
+                  const D &mz) {
+    const size_t n3 = 3 * (size_t)nd[c];
+    const double l0 = lamS[n3], l1 = lamS[n3 + 1], l2 = lamS[n3 + 2];
+    const double r0 = lamR[n3], r1 = lamR[n3 + 1], r2 = lamR[n3 + 2];
 #pragma unroll
-                      for (int t = 0; t < K; ++t)
-                        acc[t] += l0 * fx.d[t] + l1 * fy.d[t] + l2 * fz.d[t] + r0 * mx.d[t] +
-                                  r1 * my.d[t] + r2 * mz.d[t];
-                    });
+    for (int t = 0; t < K; ++t)
+      acc[t] += l0 * fx.d[t] + l1 * fy.d[t] + l2 * fz.d[t] + r0 * mx.d[t] + r1 * my.d[t] +
+                r2 * mz.d[t];
+  };
+  if constexpr (QEPH) {
+    QephOut<D> o;
+    qeph_chain<D, D>(xg, vg, vrg, thk0, io, a.dt1, a.mat, jc, a.prop, a.k, o, emit);
+  } else {
+    ShellOut<D> o;
+    shell_chain<D, D>(xg, vg, vrg, thk0, nd[2] == nd[3], io, a.dt1, a.mat, jc, a.prop, a.k, o,
+                      emit);
+  }
 #pragma unroll
   for (int t = 0; t < K; ++t) {
     const int i = i0 + t;
     if (i < 36) a.lcorner[(size_t)i * ne + e] = acc[t];
-    else if (i < 36 + ADJ_NST) a.lst_out[(size_t)(i - 36) * ne + e] = acc[t];
-    else if (i == 69) a.g_thk[e] += acc[t];
-    else if (i < ADJ_NIN) a.g_jc[(size_t)(i - 70) * ne + e] += acc[t];
+    else if (i < SH::I_THK) a.lst_out[(size_t)(i - 36) * ne + e] = acc[t];
+    else if (i == SH::I_THK) a.g_thk[e] += acc[t];
+    else if (i < SH::NIN) a.g_jc[(size_t)(i - SH::I_JC) * ne + e] += acc[t];
   }
 }
 
@@ -195,10 +204,21 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(int numnod, int numelc,
   }
 }
 
-int adj_nchunk() { return (ADJ_NIN + ADJ_K - 1) / ADJ_K; }
+template <typename S>
+static void launch_adj_elem_form(bool qeph, const AdjArgs<S> &a, int nb, int nchunk,
+                                 hipStream_t s) {
+  if (qeph)
+    hipLaunchKernelGGL((adj_elem_kernel<S, ADJ_K, true>), dim3(nb), dim3(ADJ_BLOCK), 0, s, a,
+                       nchunk);
+  else
+    hipLaunchKernelGGL((adj_elem_kernel<S, ADJ_K, false>), dim3(nb), dim3(ADJ_BLOCK), 0, s, a,
+                       nchunk);
+}
 
-hipError_t launch_adj_elem(bool fp32_state, const AdjArgs<double> &a64, hipStream_t s) {
-  const int nchunk = adj_nchunk();
+hipError_t launch_adj_elem(bool fp32_state, bool qeph, const AdjArgs<double> &a64,
+                           hipStream_t s) {
+  const int nin = qeph ? AdjShape<true>::NIN : AdjShape<false>::NIN;
+  const int nchunk = (nin + ADJ_K - 1) / ADJ_K;
   const long nt = (long)a64.numelc * nchunk;
   const int nb = (int)((nt + ADJ_BLOCK - 1) / ADJ_BLOCK);
   if (nb <= 0) return hipSuccess;
@@ -209,11 +229,9 @@ hipError_t launch_adj_elem(bool fp32_state, const AdjArgs<double> &a64, hipStrea
     a.thke = a64.thke; a.lamS = a64.lamS; a.lamR = a64.lamR; a.lst_in = a64.lst_in;
     a.lst_out = a64.lst_out; a.lcorner = a64.lcorner; a.g_thk = a64.g_thk; a.g_jc = a64.g_jc;
     a.mat = a64.mat; a.prop = a64.prop; a.k = a64.k; a.dt1 = a64.dt1;
-    hipLaunchKernelGGL((adj_elem_kernel<float, ADJ_K>), dim3(nb), dim3(ADJ_BLOCK), 0, s, a,
-                       nchunk);
+    launch_adj_elem_form(qeph, a, nb, nchunk, s);
   } else {
-    hipLaunchKernelGGL((adj_elem_kernel<double, ADJ_K>), dim3(nb), dim3(ADJ_BLOCK), 0, s, a64,
-                       nchunk);
+    launch_adj_elem_form(qeph, a64, nb, nchunk, s);
   }
   return hipGetLastError();
 }

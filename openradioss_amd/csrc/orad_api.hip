@@ -1412,8 +1412,6 @@ int orad_adjoint_forward(orad_ctx *c, int ncycle, int every) {
   if (!c) return ORAD_EINVAL;
   if (c->family != FAM_SHELL || c->batched)
     return fail(c, ORAD_ESTATE, "adjoint_forward: one shell model only");
-  if (c->qeph())
-    return fail(c, ORAD_ESTATE, "adjoint_forward: Belytschko-Tsay shells only (Ishell=1)");
   int r = ready(c);
   if (r) return r;
   if (ncycle < 1 || every < 1) return fail(c, ORAD_EINVAL, "adjoint_forward: bad cycle counts");
@@ -1452,11 +1450,12 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
   HIPCHK(c, hipStreamSynchronize(c->stream));
   AdjStore *A = c->adj;
   const size_t nn = (size_t)c->numnod, ne = (size_t)c->numelc, ts = c->tsize();
-  const size_t stb = SC_NCOL * ne * ts;
+  const size_t stb = (size_t)c->shncol() * ne * ts;
+  const int nst = c->qeph() ? AdjShape<true>::NST : AdjShape<false>::NST;
   const int L = A->every;
   DevBufs B;
   double *lx, *lv, *ld, *lvr, *lamS, *lamR, *gms, *gin, *lst[2], *lcorner, *gthk, *gjc;
-  int32_t *scorner;
+  int32_t *scorner, *selem;
   double4 *sx, *sv, *svr;
   double *sa, *sar;
   char *sst;
@@ -1469,12 +1468,13 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
   HIPCHK(c, B.get(&lamR, 3 * nn * 8));
   HIPCHK(c, B.get(&gms, nn * 8));
   HIPCHK(c, B.get(&gin, nn * 8));
-  HIPCHK(c, B.get(&lst[0], 33 * ne * 8));
-  HIPCHK(c, B.get(&lst[1], 33 * ne * 8));
+  HIPCHK(c, B.get(&lst[0], (size_t)nst * ne * 8));
+  HIPCHK(c, B.get(&lst[1], (size_t)nst * ne * 8));
   HIPCHK(c, B.get(&lcorner, 36 * ne * 8));
   HIPCHK(c, B.get(&gthk, ne * 8));
   HIPCHK(c, B.get(&gjc, 3 * ne * 8));
   HIPCHK(c, B.get(&scorner, 4 * ne * 4, false));
+  HIPCHK(c, B.get(&selem, 4 * ne * 4, false));
   HIPCHK(c, B.get(&sx, (size_t)L * nn * 32, false));
   HIPCHK(c, B.get(&sv, (size_t)L * nn * 32, false));
   HIPCHK(c, B.get(&svr, (size_t)L * nn * 32, false));
@@ -1483,21 +1483,26 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
   HIPCHK(c, B.get(&sst, (size_t)L * stb, false));
   HIPCHK(c, B.get(&sscal, (size_t)L * sizeof(Scal), false));
   {
-    std::vector<int32_t> sc(4 * ne);
+    // element and corner of every slot, for the ordered gather of corner adjoints
+    std::vector<int32_t> sc(4 * ne), se(4 * ne);
     for (size_t e = 0; e < ne; ++e)
-      for (int k = 0; k < 4; ++k) sc[c->h_iads[4 * e + k]] = k;
+      for (int k = 0; k < 4; ++k) {
+        sc[c->h_iads[4 * e + k]] = k;
+        se[c->h_iads[4 * e + k]] = (int32_t)e;
+      }
     HIPCHK(c, hipMemcpy(scorner, sc.data(), 4 * ne * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(selem, se.data(), 4 * ne * 4, hipMemcpyHostToDevice));
   }
   if (lam_x) HIPCHK(c, hipMemcpy(lx, lam_x, 3 * nn * 8, hipMemcpyHostToDevice));
   if (lam_v) HIPCHK(c, hipMemcpy(lv, lam_v, 3 * nn * 8, hipMemcpyHostToDevice));
   if (lam_vr) HIPCHK(c, hipMemcpy(lvr, lam_vr, 3 * nn * 8, hipMemcpyHostToDevice));
   if (lam_d) HIPCHK(c, hipMemcpy(ld, lam_d, 3 * nn * 8, hipMemcpyHostToDevice));
   if (lam_state) {
-    // rows of 33 (SIG, PLA, HOUR(1:3)) -> columns
-    std::vector<double> col(33 * ne);
+    // rows of nst (SIG, PLA, then HOUR(1:3) or HOURG(1:12)) -> columns
+    std::vector<double> col((size_t)nst * ne);
     for (size_t e = 0; e < ne; ++e)
-      for (int j = 0; j < 33; ++j) col[j * ne + e] = lam_state[33 * e + j];
-    HIPCHK(c, hipMemcpy(lst[0], col.data(), 33 * ne * 8, hipMemcpyHostToDevice));
+      for (int j = 0; j < nst; ++j) col[j * ne + e] = lam_state[(size_t)nst * e + j];
+    HIPCHK(c, hipMemcpy(lst[0], col.data(), (size_t)nst * ne * 8, hipMemcpyHostToDevice));
   }
   int cur = 0;
   AdjArgs<double> ag{};
@@ -1557,8 +1562,8 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
       ag.lst_in = lst[cur];
       ag.lst_out = lst[cur ^ 1];
       ag.dt1 = hs[j].dt1;
-      HIPCHK(c, launch_adj_elem(c->precision == ORAD_MIXED, ag, c->stream));
-      HIPCHK(c, launch_adj_gather(c->numnod, c->numelc, c->adsky, c->slot_elem, scorner, lcorner,
+      HIPCHK(c, launch_adj_elem(c->precision == ORAD_MIXED, c->qeph(), ag, c->stream));
+      HIPCHK(c, launch_adj_gather(c->numnod, c->numelc, c->adsky, selem, scorner, lcorner,
                                   lx, lv, lvr, c->stream));
       cur ^= 1;
     }

@@ -48,7 +48,7 @@ int qeph_batch_occupancy_mixed(int);
 int shell_batch_occupancy_fp64(int);
 int shell_batch_occupancy_mixed(int);
 template <typename S> struct AdjArgs;
-hipError_t launch_adj_elem(bool, const AdjArgs<double> &, hipStream_t);
+hipError_t launch_adj_elem(bool, bool, const AdjArgs<double> &, hipStream_t);
 hipError_t launch_adj_node(int, const double4 *, const double4 *, const double4 *,
                            const double *, const double *, double, double, const double *,
                            double *, const double *, double *, double *, double *, double *,

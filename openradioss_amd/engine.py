@@ -366,13 +366,19 @@ class ShellSolver:
         """Run ncycle cycles keeping a checkpoint every `every` cycles."""
         self._chk(self.lib.orad_adjoint_forward(self.h, ncycle, every))
 
+    def adjoint_nstate(self):
+        """State values with an adjoint per element: SIG(5) of points 1..5, PLA(1..5),
+        then HOUR(1:3) for Belytschko-Tsay (33) or HOURG(1:12) for QEPH (42)."""
+        return 42 if self.ishell == 24 else 33
+
     def adjoint_reverse(self, lam_x=None, lam_v=None, lam_vr=None, lam_d=None, lam_state=None):
         """Gradients from the adjoints of the final state: dict with g_jc (dA, dB, dn),
-        g_thk, g_ms, g_in and the adjoints of the initial x, v, vr."""
+        g_thk, g_ms, g_in and the adjoints of the initial x, v, vr.  lam_state has
+        adjoint_nstate() values per element."""
         nn, ne = self.numnod, self.numelc
         f = lambda a, shape: None if a is None else _f64(a, shape)
         ins = [f(lam_x, (nn, 3)), f(lam_v, (nn, 3)), f(lam_vr, (nn, 3)), f(lam_d, (nn, 3)),
-               f(lam_state, (ne, 33))]
+               f(lam_state, (ne, self.adjoint_nstate()))]
         out = dict(g_jc=np.zeros(3), g_thk=np.zeros(ne), g_ms=np.zeros(nn), g_in=np.zeros(nn),
                    lam_x0=np.zeros((nn, 3)), lam_v0=np.zeros((nn, 3)), lam_vr0=np.zeros((nn, 3)))
         args = [None if a is None else _p(a) for a in ins] + [_p(out[k]) for k in

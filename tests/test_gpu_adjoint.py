@@ -23,6 +23,8 @@ Bars:
     with h = 1e-4 are printed beside them: over 2000 cycles so many points start to
     yield inside a 1e-4 step that they are not derivative estimates (measured: 1%
     to 5x away from both the adjoint and the h = 1e-6 quotients);
+  * QEPH (Ishell=24): the same short runs at the same bar; over 2000 cycles a
+    twisted elastic plate (test_adjoint_qeph_plate_128_vs_fd says why);
   * FP32-element forward with the FP64 adjoint vs the all-FP64 gradient: printed.
 """
 import numpy as np
@@ -73,7 +75,7 @@ class ShellRun(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gd, gpla):
         s = ctx.solver
-        lam_state = np.zeros((s.numelc, 33))
+        lam_state = np.zeros((s.numelc, s.adjoint_nstate()))
         lam_state[:, 25:30] = gpla.numpy()
         g = s.adjoint_reverse(lam_d=gd.numpy(), lam_state=lam_state)
         s.close()
@@ -123,8 +125,16 @@ def pinned_dt(deck):
     return 0.8 * dt
 
 
-def setup_plate(n, patch, thk_noise=0.1, seed=20261203):
+def setup_plate(n, patch, thk_noise=0.1, seed=20261203, ishell=1, twist=0.0, ca_scale=1.0):
+    """The clamped plate; `twist` c lifts it to the hyperbolic paraboloid
+    z = c (x - n/2)(y - n/2), which gives every element the same warp c/4."""
     deck = dk.shell_plate(n=n, patch=patch, thk_noise=thk_noise, seed=seed)
+    deck["prop"] = dict(deck["prop"], ishell=ishell)
+    deck["mat"] = dict(deck["mat"], ca=deck["mat"]["ca"] * ca_scale)
+    if twist:
+        x = deck["x"].copy()
+        x[:, 2] = twist * (x[:, 0] - 0.5 * n) * (x[:, 1] - 0.5 * n)
+        deck["x"] = x
     ni = n + 1
     centre = (n // 2) + ni * (n // 2)
     _, wf, _ = po.shell_tables(deck["prop"]["npt"])
@@ -139,7 +149,7 @@ def gradients(deck, prec, ncyc, every, dtmx, centre, wf):
     thk = torch.tensor(deck["thk"], dtype=torch.float64, requires_grad=True)
     J = objective(deck, (A, B, n, thk), prec, ncyc, every, dtmx, centre, wf)
     J.backward()
-    return float(J), np.array([A.grad.item(), B.grad.item(), n.grad.item()]), thk.grad.numpy()
+    return float(J.detach()), np.array([A.grad.item(), B.grad.item(), n.grad.item()]), thk.grad.numpy()
 
 
 def fd(deck, which, idx, h, ncyc, dtmx, centre, wf):
@@ -159,8 +169,9 @@ def fd(deck, which, idx, h, ncyc, dtmx, centre, wf):
     return (vals[0] - vals[1]) / (2.0 * step), tt
 
 
-def test_adjoint_short_runs_vs_fd():
-    deck, centre, wf, dtmx = setup_plate(n=16, patch=6)
+@pytest.mark.parametrize("ishell", [1, 24])
+def test_adjoint_short_runs_vs_fd(ishell):
+    deck, centre, wf, dtmx = setup_plate(n=16, patch=6, ishell=ishell)
     ncyc, every = 60, 7                         # segment boundaries inside the run
     J, gjc, gthk = gradients(deck, en.FP64, ncyc, every, dtmx, centre, wf)
     J0, tt = forward_only(deck, [deck["mat"]["ca"], deck["mat"]["cb"], deck["mat"]["cn"],
@@ -175,8 +186,9 @@ def test_adjoint_short_runs_vs_fd():
         assert abs(g - gf) <= 1e-5 * max(abs(gf), 1e-3 * np.abs(gthk).max()), (which, idx, g, gf)
 
 
-def test_adjoint_initial_velocity():
-    deck, centre, wf, dtmx = setup_plate(n=12, patch=4)
+@pytest.mark.parametrize("ishell", [1, 24])
+def test_adjoint_initial_velocity(ishell):
+    deck, centre, wf, dtmx = setup_plate(n=12, patch=4, ishell=ishell)
     s = en.ShellSolver(deck, en.FP64)
     s.set_time(dtmx=dtmx)
     s.adjoint_forward(40, 9)
@@ -223,11 +235,56 @@ def test_adjoint_plate_128_vs_fd():
     assert max(errs[1e-6]) < 1e-3
 
 
-def test_adjoint_mixed_forward_close_to_fp64():
-    deck, centre, wf, dtmx = setup_plate(n=32, patch=8)
+@pytest.mark.parametrize("ishell", [1, 24])
+def test_adjoint_mixed_forward_close_to_fp64(ishell):
+    deck, centre, wf, dtmx = setup_plate(n=32, patch=8, ishell=ishell)
     _, g64, t64 = gradients(deck, en.FP64, 300, 50, dtmx, centre, wf)
     _, g32, t32 = gradients(deck, en.MIXED, 300, 50, dtmx, centre, wf)
     ej = np.abs(g32 - g64) / np.abs(g64)
     et = np.linalg.norm(t32 - t64) / np.linalg.norm(t64)
     print(f"FP32-forward gradient vs FP64: A,B,n {ej}, thickness L2 {et:.2e}")
     assert ej.max() < 1e-2 and et < 1e-2
+
+
+def test_adjoint_qeph_plate_128_vs_fd():
+    """QEPH, 128 x 128, 2000 cycles, checkpoint every 100, on a plate twisted to
+    z = 0.005 (x - 64)(y - 64) with a yield stress 1000x the plate's, so no point
+    yields: relative error < 1e-3 against FD with h = 1e-5 and 1e-6 on 17
+    thicknesses (measured on MI355X: 5.8e-6 and 8.4e-5 at most).
+
+    A flat or a yielding QEPH plate has no such check over 2000 cycles, because
+    the CZFORC3 cycle is discontinuous in its inputs there, and FD quotients of
+    it are not derivative estimates (measured: off by 1 to 1e4x at h = 1e-5 ..
+    1e-7 on both):
+      * CZCORP5 switches an element between the flat and the warped projection
+        when Z1^2 crosses LM * 1e-8 (czcorp5.F:84-88), and the two differ by a
+        finite force at the switch; a flat plate starts at Z1 = 0 and its bending
+        moves thousands of elements across it; the twist keeps every element at
+        |Z1| >= 7.6e-4, 5x above the threshold, for the whole run;
+      * CZFINTN1 cuts the hourglass increment by EH = max(.., 1 - ZCFAC) once
+        a point has yielded or the hourglass stress passes the yield surface
+        (czfintn.F:303-345), ZCFAC being the mean and the minimum of the points'
+        ET/E (1 elastic, H/(H+E) plastic, m2cplr.F:99,227); it jumps whenever a
+        point starts or stops yielding.
+    The 60-cycle runs above (plastic, flat) stay clear of both and pass at 1e-5."""
+    deck, centre, wf, dtmx = setup_plate(n=128, patch=32, ishell=24, twist=0.005,
+                                         ca_scale=1.0e3)
+    ncyc, every = 2000, 100
+    J, gjc, gthk = gradients(deck, en.FP64, ncyc, every, dtmx, centre, wf)
+    assert np.all(gjc == 0.0)                   # nothing yields
+    rng = np.random.default_rng(20261203)
+    ne = deck["ixc"].shape[0]
+    patch_elems = np.flatnonzero(np.abs(deck["x"][deck["ixc"][:, 0], 0] - 64.0) < 18.0)
+    elems = list(rng.choice(patch_elems, 9, replace=False)) + list(rng.choice(ne, 8, replace=False))
+    errs = {1e-5: [], 1e-6: []}
+    for e in elems:
+        line = f"thickness {int(e)} adjoint {gthk[int(e)]:.6e}"
+        for h in errs:
+            gf, tt = fd(deck, 3, int(e), h, ncyc, dtmx, centre, wf)
+            errs[h].append(abs(gthk[int(e)] - gf) / max(abs(gf), 1e-30))
+            line += f" | fd(h={h:g}) {gf:.6e} rel {errs[h][-1]:.2e}"
+            assert tt == pytest.approx(ncyc * dtmx, rel=1e-12)
+        print(line, flush=True)
+    for h, e in errs.items():
+        print(f"h={h:g}: max relative error over {len(e)} thicknesses {max(e):.2e}")
+        assert max(e) < 1e-3

@@ -196,14 +196,6 @@ def test_qeph_run_to_run_bitwise():
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
 
 
-def test_qeph_rejects_adjoint():
-    d = qeph_plate(8, 2, 1.0e4)
-    s = en.ShellSolver(d, en.FP64)
-    with pytest.raises(en.OradError):
-        s.adjoint_forward(10, 5)
-    s.close()
-
-
 NODE_KEYS = ("x", "v", "vr", "ms", "in", "icodt", "icodr", "fv_dir", "fv_val")
 
 
