@@ -608,3 +608,12 @@ void oracle_energies(const orad_omodel_t *m, double *eint, double *ekin) {
   }
   *eint = ei; *ekin = ek;
 }
+
+void oracle_brick_group(int n, const double *xg, const double *vg, orad_bstate_t *st,
+                        double dt1, const orad_law2_t *mat, const orad_sprop_t *prop,
+                        orad_bout_t *out) {
+  for (int e = 0; e < n; ++e)
+    oracle_brick_element((const double(*)[8])(xg + 24 * (size_t)e),
+                         (const double(*)[8])(vg + 24 * (size_t)e), st + e, dt1, mat, prop,
+                         out + e);
+}

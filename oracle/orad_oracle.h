@@ -67,6 +67,13 @@ void oracle_brick_element(const double xg[3][8], const double vg[3][8],
                           const orad_law2_t *mat, const orad_sprop_t *prop,
                           orad_bout_t *out);
 
+/* oracle_brick_element over n elements in turn, one thread (for timing the restatement
+ * against the reference's own routines on the same group): xg, vg [n][3][8],
+ * st [n] updated in place, out [n]. */
+void oracle_brick_group(int n, const double *xg, const double *vg, orad_bstate_t *st,
+                        double dt1, const orad_law2_t *mat, const orad_sprop_t *prop,
+                        orad_bout_t *out);
+
 /* Initial volume as the starter computes it (same determinant as SDERI3, global frame). */
 double oracle_brick_volume(const double xg[3][8]);
 

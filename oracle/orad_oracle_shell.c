@@ -970,7 +970,7 @@ void oracle_qeph_element(const double xg[3][4], const double vg[3][4], const dou
   }
   /* ---- CNCOEF3B (cncoef3.F:29-293), IGTYP = 1, law 2, ITHK = 0 ---- */
   const double thk0 = thke, thk02 = thk0 * thk0, VOL0 = thk0 * AREA, dt1c = dt1;
-  const double RHO = m->rho0, YM = m->young, NU = m->nu, G = m->shear;
+  const double RHO = m->rho0, NU = m->nu, G = m->shear;
   const double A11 = m->a11, A12 = m->a12, SSP = m->ssp;
   const double GSR = sqrt(fabs(G)), A11SR = sqrt(fabs(A11)), A12SR = sqrt(fabs(A12));
   (void)NU;                                         /* resol_init.F:689-696 */
