@@ -63,7 +63,9 @@ int orad_set_nodes(orad_ctx *ctx, int numnod, const double *x, const double *v,
  * vol0 = initial volumes GBUF%VOL.  Builds the PARITH/ON CSR on the host with
  * FILLCNE/W_PON ordering (starter/source/spmd/domdec2.F:2100-2150,
  * starter/source/restart/ddsplit/w_pon.F:550-573) and initialises element state
- * (SIG=0, EINT=0, RHO=rho0, PLA=0, OFF=1) once a material is set. */
+ * (SIG=0, EINT=0, RHO=rho0, PLA=0, OFF=1) once a material is set.  Slots are
+ * addressed by 32-bit indices like the reference's IADS, so NUMELS <= 268435455
+ * (ORAD_EINVAL beyond). */
 int orad_set_bricks(orad_ctx *ctx, int numels, const int *ixs, const double *vol0);
 
 /* Law 2 (Johnson-Cook) parameters, layout of mat_param for M2LAW
@@ -145,7 +147,8 @@ int orad_set_nodes_rot(orad_ctx *ctx, const double *vr, const double *in, const 
 /* Shells: IXC(NIXC=7,NUMELC) as the engine stores it (ccoor3.F:69-78):
  * IXC(1)=material, IXC(2..5)=nodes (1-based, IXC(5)=IXC(4) for a triangle),
  * IXC(6)=property, IXC(7)=user id; thke = element thickness THKE.  Builds the
- * PARITH/ON CSR like orad_set_bricks and initialises the state (THK=THKE, OFF=1). */
+ * PARITH/ON CSR like orad_set_bricks and initialises the state (THK=THKE, OFF=1).
+ * NUMELC <= 536870911 (32-bit slot indices, as IADC). */
 int orad_set_shells(orad_ctx *ctx, int numelc, const int *ixc, const double *thke);
 
 /* Law 2 for shells: the starter's derived values (hm_read_mat02_jc.F90:234-339)

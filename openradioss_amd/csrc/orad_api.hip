@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -290,6 +291,9 @@ int orad_set_bricks(orad_ctx *c, int numels, const int *ixs, const double *vol0)
   if (!c->have_nodes) return fail(c, ORAD_ESTATE, "set_bricks: call orad_set_nodes first");
   if (c->family != FAM_NONE) return fail(c, ORAD_ESTATE, "set_bricks: mesh already set");
   if (numels <= 0 || !ixs || !vol0) return fail(c, ORAD_EINVAL, "set_bricks: bad arguments");
+  // FSKY slots are addressed by 32-bit indices, as the reference's IADS (INTEGER)
+  if (numels > INT_MAX / 8)
+    return fail(c, ORAD_EINVAL, "set_bricks: more than 268435455 bricks (8 slots each, 32-bit IADS)");
   HIPCHK(c, hipSetDevice(c->device));
   const size_t ne = (size_t)numels;
   const int nn = c->numnod;
@@ -871,6 +875,8 @@ int orad_set_shells(orad_ctx *c, int numelc, const int *ixc, const double *thke)
   if (!c->have_nodes) return fail(c, ORAD_ESTATE, "set_shells: call orad_set_nodes first");
   if (c->family != FAM_NONE) return fail(c, ORAD_ESTATE, "set_shells: mesh already set");
   if (numelc <= 0 || !ixc || !thke) return fail(c, ORAD_EINVAL, "set_shells: bad arguments");
+  if (numelc > INT_MAX / 4)
+    return fail(c, ORAD_EINVAL, "set_shells: more than 536870911 shells (4 slots each, 32-bit IADC)");
   HIPCHK(c, hipSetDevice(c->device));
   const size_t ne = (size_t)numelc;
   const int nn = c->numnod;

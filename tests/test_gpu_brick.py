@@ -9,6 +9,8 @@ Tolerances:
     within 1e-4 relative (L2 over all nodes) after 1000 cycles;
   * run-to-run: bitwise identical (PARITH/ON assembly, fixed-order reductions).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -229,6 +231,28 @@ def test_error_paths_return_codes():
     with pytest.raises(en.OradError) as ei:
         en.BrickSolver(bad, en.FP64)
     assert ei.value.code == en.ORAD_EINVAL
+
+
+def test_slot_index_range_is_checked():
+    """8 * NUMELS slots must fit the 32-bit IADS indices; one brick more is refused
+    before any array is read (the tiny arrays passed are never touched)."""
+    lib = en.load_library()
+    h = C.c_void_p()
+    assert lib.orad_create(C.byref(h), 0, en.FP64) == en.ORAD_OK
+    try:
+        x = np.zeros((8, 3))
+        x[:, 0] = [0, 1, 1, 0, 0, 1, 1, 0]
+        x[:, 1] = [0, 0, 1, 1, 0, 0, 1, 1]
+        x[:, 2] = [0, 0, 0, 0, 1, 1, 1, 1]
+        v, ms, icodt = np.zeros((8, 3)), np.ones(8), np.zeros(8, np.int32)
+        P = lambda a: a.ctypes.data_as(C.c_void_p)
+        assert lib.orad_set_nodes(h, 8, P(x), P(v), P(ms), P(icodt)) == en.ORAD_OK
+        ixs = np.zeros(11, np.int32)
+        vol0 = np.ones(1)
+        assert lib.orad_set_bricks(h, 268435456, P(ixs), P(vol0)) == en.ORAD_EINVAL
+        assert b"32-bit" in lib.orad_last_error(h)
+    finally:
+        lib.orad_destroy(h)
 
 
 def test_negative_volume_sets_mstop():
