@@ -124,6 +124,48 @@ def shell_plate_leg(n, steps, err_cycles, ishell=1):
     return res
 
 
+def adjoint_leg(n=128, ncyc=2000, every=100, ishell=1):
+    """Adjoint of an n x n plate run (thickness x (1 + U(-0.1, 0.1)) per element,
+    central patch impact, ncyc cycles, checkpoint every `every`), FP64: times a plain
+    forward run, the checkpointed forward run and the reverse sweep (the gradient of
+    the centre node's final z displacement w.r.t. every thickness, A, B, n, masses,
+    inertias and initial X, V, VR).  The time step is pinned by DTMX at 0.8 x the
+    initial element time step, as the adjoint assumes a fixed time-step sequence."""
+    deck = dk.shell_plate(n=n, patch=n // 4, thk_noise=0.1, seed=20261203)
+    deck["prop"] = dict(deck["prop"], ishell=ishell)
+    s = en.ShellSolver(deck, en.FP64)
+    dtmx = 0.8 * s.cforc3(0, s.numelc, 0.0)
+    s.close()
+    times = {}
+    s = en.ShellSolver(deck, en.FP64)
+    s.set_time(dtmx=dtmx)
+    t0 = time.perf_counter()
+    s.run_cycles(ncyc)
+    times["forward_ms"] = 1e3 * (time.perf_counter() - t0)
+    s.close()
+    s = en.ShellSolver(deck, en.FP64)
+    s.set_time(dtmx=dtmx)
+    t0 = time.perf_counter()
+    s.adjoint_forward(ncyc, every)
+    times["checkpointed_forward_ms"] = 1e3 * (time.perf_counter() - t0)
+    lam_d = np.zeros((s.numnod, 3))
+    lam_d[(n // 2) * (n + 2), 2] = 1.0                 # centre node, z
+    t0 = time.perf_counter()
+    g = s.adjoint_reverse(lam_d=lam_d)
+    times["reverse_ms"] = 1e3 * (time.perf_counter() - t0)
+    s.close()
+    res = dict(workload=f"{'QEPH' if ishell == 24 else 'BT'} shell plate {n}x{n} "
+                        f"({n * n} shells), thickness +-10%, {ncyc} cycles, checkpoint every "
+                        f"{every}, gradient of the centre node's final z displacement",
+               shells=n * n, cycles=ncyc, every=every, **times,
+               reverse_over_forward=times["reverse_ms"] / times["forward_ms"],
+               g_jc=[float(v) for v in g["g_jc"]])
+    print(f"adjoint: forward {times['forward_ms']:.0f} ms, checkpointed forward "
+          f"{times['checkpointed_forward_ms']:.0f} ms, reverse {times['reverse_ms']:.0f} ms",
+          file=sys.stderr, flush=True)
+    return res
+
+
 def tube_batch_leg(ntube, steps, solo_cycles, autotune=True):
     """256 independent square tubes (one model each, per-tube A, thickness, size and
     time step), crushed axially at 10 m/s by FIXVEL: all tubes in one batched cycle
@@ -296,6 +338,7 @@ def main():
     ap.add_argument("--tube-solo-cycles", type=int, default=200)
     ap.add_argument("--no-autotune", action="store_true",
                     help="tube leg: keep the default launch configuration")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 128^2 adjoint leg")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -337,6 +380,9 @@ def main():
     if not args.no_tubes:
         tubes = tube_batch_leg(args.tubes, args.tube_steps, args.tube_solo_cycles,
                                autotune=not args.no_autotune)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg()
     copy_gbs = None
     if not args.no_copy:
         copy_gbs = copy_bandwidth()
@@ -380,6 +426,7 @@ def main():
         "shell_plate": shell,
         "qeph_plate": qeph,
         "tube_batch": tubes,
+        "adjoint_plate": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -23,6 +23,8 @@
 //                      PARITH/ON CSR), so the sums are deterministic.
 #include "orad_dual.hpp"
 #include "orad_qeph_chain.hpp"
+
+#include <type_traits>
 #include "orad_brick_io.hpp"
 #include "orad_adjoint_args.hpp"
 
@@ -56,33 +58,40 @@ struct AdjIO {
   }
 };
 
-template <typename S, int K, bool QEPH>
-__global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_kernel(AdjArgs<S> a, int nchunk) {
+// KIN: the chunks seed corner inputs (i < 36), so the kinematics carry tangents; the
+// chunks of state, thickness and material inputs run the kinematics in plain FP64
+// (TG = double), whose values are the same and whose tangents would all be zero.
+template <typename S, int K, bool QEPH, bool KIN>
+__global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_kernel(AdjArgs<S> a, int chunk0,
+                                                             int nchunk) {
   typedef Dual<K> D;
+  typedef typename std::conditional<KIN, D, double>::type G;
   typedef AdjShape<QEPH> SH;
   const int ne = a.numelc;
   const long idx = (long)blockIdx.x * ADJ_BLOCK + threadIdx.x;
   if (idx >= (long)ne * nchunk) return;
-  const int chunk = (int)(idx / ne), e = (int)(idx - (long)chunk * ne);
-  const int i0 = chunk * K;
+  const int cl = (int)(idx / ne), e = (int)(idx - (long)cl * ne);
+  const int i0 = (chunk0 + cl) * K;
   int nd[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) nd[c] = a.conn[(size_t)c * ne + e];
-  D xg[3][4], vg[3][4], vrg[3][4];
+  G xg[3][4], vg[3][4], vrg[3][4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const double4 x = a.X4[nd[c]], v = a.V4[nd[c]], w = a.VR4[nd[c]];
     const double xv[3] = {x.x, x.y, x.z}, vv[3] = {v.x, v.y, v.z}, wv[3] = {w.x, w.y, w.z};
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      xg[i][c] = D(xv[i]);
-      vg[i][c] = D(vv[i]);
-      vrg[i][c] = D(wv[i]);
+      xg[i][c] = G(xv[i]);
+      vg[i][c] = G(vv[i]);
+      vrg[i][c] = G(wv[i]);
+      if constexpr (KIN) {
 #pragma unroll
-      for (int t = 0; t < K; ++t) {
-        xg[i][c].d[t] = (9 * c + i - i0 == t) ? 1.0 : 0.0;
-        vg[i][c].d[t] = (9 * c + 3 + i - i0 == t) ? 1.0 : 0.0;
-        vrg[i][c].d[t] = (9 * c + 6 + i - i0 == t) ? 1.0 : 0.0;
+        for (int t = 0; t < K; ++t) {
+          xg[i][c].d[t] = (9 * c + i - i0 == t) ? 1.0 : 0.0;
+          vg[i][c].d[t] = (9 * c + 3 + i - i0 == t) ? 1.0 : 0.0;
+          vrg[i][c].d[t] = (9 * c + 6 + i - i0 == t) ? 1.0 : 0.0;
+        }
       }
     }
   }
@@ -111,10 +120,10 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_kernel(AdjArgs<S> a, int n
   };
   if constexpr (QEPH) {
     QephOut<D> o;
-    qeph_chain<D, D>(xg, vg, vrg, thk0, io, a.dt1, a.mat, jc, a.prop, a.k, o, emit);
+    qeph_chain<D, G>(xg, vg, vrg, thk0, io, a.dt1, a.mat, jc, a.prop, a.k, o, emit);
   } else {
     ShellOut<D> o;
-    shell_chain<D, D>(xg, vg, vrg, thk0, nd[2] == nd[3], io, a.dt1, a.mat, jc, a.prop, a.k, o,
+    shell_chain<D, G>(xg, vg, vrg, thk0, nd[2] == nd[3], io, a.dt1, a.mat, jc, a.prop, a.k, o,
                       emit);
   }
 #pragma unroll
@@ -204,24 +213,31 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(int numnod, int numelc,
   }
 }
 
+static_assert(36 % ADJ_K == 0, "corner inputs must fill whole chunks");
+
+// Chunks 0 .. 36/K - 1 seed the corner inputs, the rest the state, thickness and
+// material inputs: one launch each.
+template <typename S, bool QEPH>
+static void launch_adj_elem_chunks(const AdjArgs<S> &a, hipStream_t s) {
+  const int nchunk = (AdjShape<QEPH>::NIN + ADJ_K - 1) / ADJ_K, nkin = 36 / ADJ_K;
+  const long ne = a.numelc;
+  const int nb0 = (int)((ne * nkin + ADJ_BLOCK - 1) / ADJ_BLOCK);
+  const int nb1 = (int)((ne * (nchunk - nkin) + ADJ_BLOCK - 1) / ADJ_BLOCK);
+  hipLaunchKernelGGL((adj_elem_kernel<S, ADJ_K, QEPH, true>), dim3(nb0), dim3(ADJ_BLOCK), 0, s,
+                     a, 0, nkin);
+  hipLaunchKernelGGL((adj_elem_kernel<S, ADJ_K, QEPH, false>), dim3(nb1), dim3(ADJ_BLOCK), 0, s,
+                     a, nkin, nchunk - nkin);
+}
+
 template <typename S>
-static void launch_adj_elem_form(bool qeph, const AdjArgs<S> &a, int nb, int nchunk,
-                                 hipStream_t s) {
-  if (qeph)
-    hipLaunchKernelGGL((adj_elem_kernel<S, ADJ_K, true>), dim3(nb), dim3(ADJ_BLOCK), 0, s, a,
-                       nchunk);
-  else
-    hipLaunchKernelGGL((adj_elem_kernel<S, ADJ_K, false>), dim3(nb), dim3(ADJ_BLOCK), 0, s, a,
-                       nchunk);
+static void launch_adj_elem_form(bool qeph, const AdjArgs<S> &a, hipStream_t s) {
+  if (qeph) launch_adj_elem_chunks<S, true>(a, s);
+  else launch_adj_elem_chunks<S, false>(a, s);
 }
 
 hipError_t launch_adj_elem(bool fp32_state, bool qeph, const AdjArgs<double> &a64,
                            hipStream_t s) {
-  const int nin = qeph ? AdjShape<true>::NIN : AdjShape<false>::NIN;
-  const int nchunk = (nin + ADJ_K - 1) / ADJ_K;
-  const long nt = (long)a64.numelc * nchunk;
-  const int nb = (int)((nt + ADJ_BLOCK - 1) / ADJ_BLOCK);
-  if (nb <= 0) return hipSuccess;
+  if (a64.numelc <= 0) return hipSuccess;
   if (fp32_state) {
     AdjArgs<float> a;
     a.numelc = a64.numelc; a.conn = a64.conn; a.X4 = a64.X4; a.V4 = a64.V4; a.VR4 = a64.VR4;
@@ -229,9 +245,9 @@ hipError_t launch_adj_elem(bool fp32_state, bool qeph, const AdjArgs<double> &a6
     a.thke = a64.thke; a.lamS = a64.lamS; a.lamR = a64.lamR; a.lst_in = a64.lst_in;
     a.lst_out = a64.lst_out; a.lcorner = a64.lcorner; a.g_thk = a64.g_thk; a.g_jc = a64.g_jc;
     a.mat = a64.mat; a.prop = a64.prop; a.k = a64.k; a.dt1 = a64.dt1;
-    launch_adj_elem_form(qeph, a, nb, nchunk, s);
+    launch_adj_elem_form(qeph, a, s);
   } else {
-    launch_adj_elem_form(qeph, a64, nb, nchunk, s);
+    launch_adj_elem_form(qeph, a64, s);
   }
   return hipGetLastError();
 }

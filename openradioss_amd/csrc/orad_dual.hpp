@@ -9,6 +9,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "orad_math.hpp"
 
 namespace orad {
@@ -26,6 +28,13 @@ struct Dual {
     for (int i = 0; i < K; ++i) d[i] = 0.0;
   }
 };
+
+// A dual number as the kinematics type: itself, or its value when the kinematics run
+// in plain FP64 (used only for inputs without a seeded tangent).
+template <typename TG, int K> __device__ __forceinline__ TG as_kin(const Dual<K> &x) {
+  if constexpr (std::is_same<TG, Dual<K>>::value) return x;
+  else return TG(x.v);
+}
 
 template <int K>
 __device__ __forceinline__ Dual<K> operator+(const Dual<K> &a, const Dual<K> &b) {

@@ -51,6 +51,12 @@ template <> struct M<float> {
   }
 };
 
+// x as the kinematics type TG (a dual number's overload, orad_dual.hpp, gives its value
+// when TG is a plain type).
+template <typename TG, typename T> __device__ __forceinline__ TG as_kin(const T &x) {
+  return TG(x);
+}
+
 template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return (a < b) ? a : b; }
 template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return (a > b) ? a : b; }
 

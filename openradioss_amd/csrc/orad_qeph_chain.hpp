@@ -414,7 +414,7 @@ __device__ __forceinline__ void qeph_chain(const TG xg[3][4], const TG vg[3][4],
   const T dt1 = T(dt1d);
   T offg = io.get(SC_OFFG);
   QephKin<TG> g;
-  qeph_kinematics<TG>(xg, vg, vrg, TG(dt1d), TG(offg), p.npt, k, g);
+  qeph_kinematics<TG>(xg, vg, vrg, TG(dt1d), as_kin<TG>(offg), p.npt, k, g);
   if (g.degenerate) {
     offg = ZERO;
     io.put(SC_OFFG, offg);
