@@ -13,25 +13,37 @@
 //                      of v_k through the update, and the MS / IN sensitivities;
 //   adj_elem_kernel    vector-Jacobian product of the element chain: the CFORC3
 //                      chain (orad_shell_chain.hpp), or for QEPH the CZFORC3 chain
-//                      (orad_qeph_chain.hpp), instantiated on Dual<K> seeds K of the
+//                      (orad_qeph_chain.hpp), instantiated on Dual<K>, seeds K of the
 //                      element's inputs per thread (4 corners x (x, v, vr), 25
 //                      stresses, 5 plastic strains, the incremental hourglass forces
-//                      -- 3 for BT, 12 for QEPH --, thickness, A, B, n: 73 or 82) and
+//                      -- 3 for BT, 12 for QEPH --, thickness, A, B, n: 73 or 82;
+//                      K = 1 for the corner inputs, 2 for the others) and
 //                      contracts the tangents of its outputs (slots, new state) with
 //                      their adjoints;
 //   adj_gather_kernel  per node, the corner contributions in slot order (the
 //                      PARITH/ON CSR), so the sums are deterministic.
 #include "orad_dual.hpp"
 #include "orad_qeph_chain.hpp"
-
-#include <type_traits>
 #include "orad_brick_io.hpp"
 #include "orad_adjoint_args.hpp"
+
+#include <type_traits>
 
 namespace orad {
 
 constexpr int ADJ_BLOCK = 64;
-constexpr int ADJ_K = 4;         // tangents per thread
+// Tangents per thread, for the chunks of corner inputs and of the other inputs.
+// Measured on MI355X (128^2 plate, 2000 cycles, reverse sweep BT / QEPH): 1 and 2
+// give 632 / 1164 ms, 1 and 3 663 / 1177, 1 and 4 712 / 1325, 2 and 2 701 / 1350,
+// 4 and 4 1071 / 2372 ms -- wider duals spill (up to 3 KB per lane at 4), which costs
+// more than the extra primal evaluations of narrow ones.
+#ifndef ORAD_ADJ_K_KIN
+#define ORAD_ADJ_K_KIN 1
+#endif
+#ifndef ORAD_ADJ_K_ST
+#define ORAD_ADJ_K_ST 2
+#endif
+constexpr int ADJ_K_KIN = ORAD_ADJ_K_KIN, ADJ_K_ST = ORAD_ADJ_K_ST;
 
 template <typename S, int K, bool QEPH>
 struct AdjIO {
@@ -62,7 +74,7 @@ struct AdjIO {
 // chunks of state, thickness and material inputs run the kinematics in plain FP64
 // (TG = double), whose values are the same and whose tangents would all be zero.
 template <typename S, int K, bool QEPH, bool KIN>
-__global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_kernel(AdjArgs<S> a, int chunk0,
+__global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_kernel(AdjArgs<S> a, int base,
                                                              int nchunk) {
   typedef Dual<K> D;
   typedef typename std::conditional<KIN, D, double>::type G;
@@ -71,7 +83,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_kernel(AdjArgs<S> a, int c
   const long idx = (long)blockIdx.x * ADJ_BLOCK + threadIdx.x;
   if (idx >= (long)ne * nchunk) return;
   const int cl = (int)(idx / ne), e = (int)(idx - (long)cl * ne);
-  const int i0 = (chunk0 + cl) * K;
+  const int i0 = base + cl * K;     // first input of this thread's chunk
   int nd[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) nd[c] = a.conn[(size_t)c * ne + e];
@@ -213,20 +225,21 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(int numnod, int numelc,
   }
 }
 
-static_assert(36 % ADJ_K == 0, "corner inputs must fill whole chunks");
+static_assert(36 % ADJ_K_KIN == 0, "corner inputs must fill whole chunks");
 
-// Chunks 0 .. 36/K - 1 seed the corner inputs, the rest the state, thickness and
-// material inputs: one launch each.
+// One launch for the chunks of corner inputs 0..35, one for the inputs from 36 on
+// (state, thickness, material).
 template <typename S, bool QEPH>
 static void launch_adj_elem_chunks(const AdjArgs<S> &a, hipStream_t s) {
-  const int nchunk = (AdjShape<QEPH>::NIN + ADJ_K - 1) / ADJ_K, nkin = 36 / ADJ_K;
+  const int nkin = 36 / ADJ_K_KIN;
+  const int nst = (AdjShape<QEPH>::NIN - 36 + ADJ_K_ST - 1) / ADJ_K_ST;
   const long ne = a.numelc;
   const int nb0 = (int)((ne * nkin + ADJ_BLOCK - 1) / ADJ_BLOCK);
-  const int nb1 = (int)((ne * (nchunk - nkin) + ADJ_BLOCK - 1) / ADJ_BLOCK);
-  hipLaunchKernelGGL((adj_elem_kernel<S, ADJ_K, QEPH, true>), dim3(nb0), dim3(ADJ_BLOCK), 0, s,
-                     a, 0, nkin);
-  hipLaunchKernelGGL((adj_elem_kernel<S, ADJ_K, QEPH, false>), dim3(nb1), dim3(ADJ_BLOCK), 0, s,
-                     a, nkin, nchunk - nkin);
+  const int nb1 = (int)((ne * nst + ADJ_BLOCK - 1) / ADJ_BLOCK);
+  hipLaunchKernelGGL((adj_elem_kernel<S, ADJ_K_KIN, QEPH, true>), dim3(nb0), dim3(ADJ_BLOCK), 0,
+                     s, a, 0, nkin);
+  hipLaunchKernelGGL((adj_elem_kernel<S, ADJ_K_ST, QEPH, false>), dim3(nb1), dim3(ADJ_BLOCK), 0,
+                     s, a, 36, nst);
 }
 
 template <typename S>
