@@ -31,7 +31,12 @@
 
 namespace orad {
 
-constexpr int ADJ_BLOCK = 64;
+// Workgroup size of the adjoint kernels; 64, 128 and 256 measured within 2% of each
+// other on the 128^2 plate's reverse sweep (633 / 635 / 646 ms).
+#ifndef ORAD_ADJ_BLOCK
+#define ORAD_ADJ_BLOCK 64
+#endif
+constexpr int ADJ_BLOCK = ORAD_ADJ_BLOCK;
 // Tangents per thread, for the chunks of corner inputs and of the other inputs.
 // Measured on MI355X (128^2 plate, 2000 cycles, reverse sweep BT / QEPH): 1 and 2
 // give 632 / 1164 ms, 1 and 3 663 / 1177, 1 and 4 712 / 1325, 2 and 2 701 / 1350,
