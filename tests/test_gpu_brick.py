@@ -235,7 +235,8 @@ def test_error_paths_return_codes():
 
 def test_slot_index_range_is_checked():
     """8 * NUMELS slots must fit the 32-bit IADS indices; one brick more is refused
-    before any array is read (the tiny arrays passed are never touched)."""
+    before any array is read (the tiny arrays passed are never touched), as is an
+    empty mesh."""
     lib = en.load_library()
     h = C.c_void_p()
     assert lib.orad_create(C.byref(h), 0, en.FP64) == en.ORAD_OK
@@ -249,10 +250,33 @@ def test_slot_index_range_is_checked():
         assert lib.orad_set_nodes(h, 8, P(x), P(v), P(ms), P(icodt)) == en.ORAD_OK
         ixs = np.zeros(11, np.int32)
         vol0 = np.ones(1)
+        assert lib.orad_set_bricks(h, 0, P(ixs), P(vol0)) == en.ORAD_EINVAL   # empty mesh
         assert lib.orad_set_bricks(h, 268435456, P(ixs), P(vol0)) == en.ORAD_EINVAL
         assert b"32-bit" in lib.orad_last_error(h)
     finally:
         lib.orad_destroy(h)
+
+
+def test_empty_group_is_a_no_op():
+    """NEL = 0 (an empty group) touches nothing: DT2T, FSKY and the state stay as
+    they were, for bricks and both shell formulations."""
+    d = dk.brick_block(4, 3, 2, v_top=-5.0e4)
+    s = en.BrickSolver(d, en.FP64)
+    s.sforc3(0, s.numels, 1.0e-7)
+    f0, st0 = s.fsky(), s.state()
+    assert s.sforc3(7, 0, 1.0e-7, dt2t=0.5) == 0.5
+    assert np.array_equal(s.fsky(), f0)
+    assert all(np.array_equal(s.state()[k], st0[k]) for k in st0)
+    s.close()
+    for ishell in (1, 24):
+        ds = dk.shell_plate(n=6, patch=2)
+        ds["prop"] = dict(ds["prop"], ishell=ishell)
+        sh = en.ShellSolver(ds, en.FP64)
+        sh.cforc3(0, sh.numelc, 1.0e-7)
+        g0, r0 = sh.fsky(), sh.state()
+        assert sh.cforc3(5, 0, 1.0e-7, dt2t=0.25) == 0.25
+        assert np.array_equal(sh.fsky(), g0) and np.array_equal(sh.state(), r0)
+        sh.close()
 
 
 def test_negative_volume_sets_mstop():
