@@ -1,0 +1,80 @@
+"""torch.autograd binding of the shell adjoint (orad_adjoint_forward / orad_adjoint_reverse).
+
+ShellRun maps the law-2 parameters A, B, n, the element thicknesses THKE and the nodal
+masses and rotational inertias of a shell deck to the final displacements D(NUMNOD, 3)
+and plastic strains PLA(NUMELC, 5) after `ncycle` cycles; its backward pass runs the
+library's adjoint sweep (checkpoint every `every` cycles, segment recomputation).  The
+time-step sequence is taken as fixed, so pin it with DTMX (`dtmx`) when the gradient
+must be that of the run.  The forward run is the library's own (FP64 or mixed); the
+adjoint is always the FP64 chain's derivative on the forward states.
+
+cinmas gives the nodal masses and inertias as torch functions of the thicknesses, as
+the starter computes them for isotropic shells (cinmas.F:1294-1296,1322), so that mass
+sensitivities reach the thickness gradient.
+"""
+import numpy as np
+import torch
+
+from . import engine as en
+
+
+def cinmas(x, ixc, thk, rho0):
+    """Nodal masses MS, inertias IN and element areas of a shell mesh: x (NUMNOD, 3)
+    and thk (NUMELC,) tensors, ixc (NUMELC, 4) 0-based node ids (int64 tensor)."""
+    p = x[ixc]
+    area = 0.5 * torch.linalg.norm(torch.cross(p[:, 2] - p[:, 0], p[:, 3] - p[:, 1], dim=1),
+                                   dim=1)
+    ems = rho0 * thk * area * 0.25
+    xi = ems * (area / 9.0 + thk * thk / 12.0)
+    nn = x.shape[0]
+    ms = torch.zeros(nn, dtype=x.dtype)
+    inr = torch.zeros(nn, dtype=x.dtype)
+    for k in range(4):
+        ms = ms.index_add(0, ixc[:, k], ems)
+        inr = inr.index_add(0, ixc[:, k], xi)
+    return ms, inr, area
+
+
+class ShellRun(torch.autograd.Function):
+    """(A, B, n, thk, ms, in) -> (D, PLA) after `ncycle` cycles of a shell deck (a dict
+    as openradioss_amd.deck builds it); every tensor input is float64 on the CPU."""
+
+    @staticmethod
+    def forward(ctx, A, B, n, thk, ms, inr, deck, precision, ncycle, every, dtmx):
+        d = dict(deck)
+        d["mat"] = dict(deck["mat"], ca=float(A), cb=float(B), cn=float(n))
+        d["thk"] = thk.detach().numpy()
+        d["ms"] = ms.detach().numpy()
+        d["in"] = inr.detach().numpy()
+        s = en.ShellSolver(d, precision)
+        try:
+            if dtmx is not None:
+                s.set_time(dtmx=dtmx)
+            s.adjoint_forward(ncycle, every)
+            nodes, st = s.nodes(), s.state()
+        except Exception:
+            s.close()
+            raise
+        ctx.solver = s
+        return (torch.from_numpy(nodes["d"]),
+                torch.from_numpy(st[:, en.SH_PLA0:en.SH_PLA0 + 5].copy()))
+
+    @staticmethod
+    def backward(ctx, gd, gpla):
+        s = ctx.solver
+        try:
+            lam_state = np.zeros((s.numelc, s.adjoint_nstate()))
+            lam_state[:, 25:30] = gpla.numpy()          # PLA(1..5) in the adjoint state
+            g = s.adjoint_reverse(lam_d=gd.numpy(), lam_state=lam_state)
+        finally:
+            s.close()
+        t = torch.tensor
+        return (t(g["g_jc"][0]), t(g["g_jc"][1]), t(g["g_jc"][2]), torch.from_numpy(g["g_thk"]),
+                torch.from_numpy(g["g_ms"]), torch.from_numpy(g["g_in"]), None, None, None, None,
+                None)
+
+
+def shell_run(A, B, n, thk, ms, inr, deck, precision=en.FP64, ncycle=1000, every=100,
+              dtmx=None):
+    """Functional form of ShellRun.apply."""
+    return ShellRun.apply(A, B, n, thk, ms, inr, deck, precision, ncycle, every, dtmx)

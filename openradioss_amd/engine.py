@@ -31,6 +31,7 @@ EXPORTS = (
 NIXC = 7
 SH_NCOL = 53          # Belytschko-Tsay state values per shell
 SH_NCOL_QEPH = 60     # QEPH: HOURG(6:12) appended
+SH_PLA0 = 43          # state column of PLA of point 1 (PLA(1..5) at 43..47)
 
 
 class TimeState(C.Structure):

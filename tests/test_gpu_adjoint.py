@@ -34,55 +34,12 @@ import torch
 import pyoracle as po
 from openradioss_amd import deck as dk
 from openradioss_amd import engine as en
+from openradioss_amd.autograd import ShellRun, cinmas
 
 pytestmark = pytest.mark.gpu
 
 
-def masses_torch(x, ixc, thk, rho0):
-    """CINMAS for isotropic shells (cinmas.F:1294-1296,1322) in torch."""
-    p = x[ixc]
-    area = 0.5 * torch.linalg.norm(torch.cross(p[:, 2] - p[:, 0], p[:, 3] - p[:, 1], dim=1), dim=1)
-    ems = rho0 * thk * area * 0.25
-    xi = ems * (area / 9.0 + thk * thk / 12.0)
-    nn = x.shape[0]
-    ms = torch.zeros(nn, dtype=torch.float64)
-    inr = torch.zeros(nn, dtype=torch.float64)
-    for k in range(4):
-        ms = ms.index_add(0, ixc[:, k], ems)
-        inr = inr.index_add(0, ixc[:, k], xi)
-    return ms, inr, area
-
-
-class ShellRun(torch.autograd.Function):
-    """(A, B, n, thk, ms, in) -> (final displacement d, final plastic strains pla)
-    of `ncyc` cycles of a shell deck; backward runs the library's adjoint."""
-
-    @staticmethod
-    def forward(ctx, A, B, n, thk, ms, inr, deck, prec, ncyc, every, dtmx):
-        d = dict(deck)
-        d["mat"] = dict(deck["mat"], ca=float(A), cb=float(B), cn=float(n))
-        d["thk"] = thk.detach().numpy()
-        d["ms"] = ms.detach().numpy()
-        d["in"] = inr.detach().numpy()
-        s = en.ShellSolver(d, prec)
-        s.set_time(dtmx=dtmx)
-        s.adjoint_forward(ncyc, every)
-        nodes, st = s.nodes(), s.state()
-        ctx.solver = s
-        ctx.tt = s.time().tt
-        return torch.from_numpy(nodes["d"]), torch.from_numpy(st[:, po.SH_PLA0:po.SH_PLA0 + 5].copy())
-
-    @staticmethod
-    def backward(ctx, gd, gpla):
-        s = ctx.solver
-        lam_state = np.zeros((s.numelc, s.adjoint_nstate()))
-        lam_state[:, 25:30] = gpla.numpy()
-        g = s.adjoint_reverse(lam_d=gd.numpy(), lam_state=lam_state)
-        s.close()
-        t = torch.tensor
-        return (t(g["g_jc"][0]), t(g["g_jc"][1]), t(g["g_jc"][2]), torch.from_numpy(g["g_thk"]),
-                torch.from_numpy(g["g_ms"]), torch.from_numpy(g["g_in"]), None, None, None, None,
-                None)
+masses_torch = cinmas
 
 
 def objective(deck, theta, prec, ncyc, every, dtmx, centre, wf):
