@@ -380,9 +380,10 @@ def main():
     if not args.no_tubes:
         tubes = tube_batch_leg(args.tubes, args.tube_steps, args.tube_solo_cycles,
                                autotune=not args.no_autotune)
-    adjoint = None
+    adjoint = adjoint_qeph = None
     if not args.no_adjoint:
         adjoint = adjoint_leg()
+        adjoint_qeph = adjoint_leg(ishell=24)
     copy_gbs = None
     if not args.no_copy:
         copy_gbs = copy_bandwidth()
@@ -427,6 +428,7 @@ def main():
         "qeph_plate": qeph,
         "tube_batch": tubes,
         "adjoint_plate": adjoint,
+        "adjoint_plate_qeph": adjoint_qeph,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
