@@ -230,7 +230,9 @@ int orad_autotune(orad_ctx *ctx, int cycles, orad_launch_t *best, double *best_m
  * the initial X, V, VR (any output may be null).  The time-step sequence is taken
  * as fixed (exact when DTMX pins it).  The element derivatives are those of the
  * FP64 chain evaluated on the forward states, in FP64 for either form; the context
- * is left at the end of the forward run. */
+ * is left at the end of the forward run.  Law 2 without the strain-rate term only
+ * (C = 0; ORAD_EINVAL otherwise): the rate filter EPSDP and the thickness THK that
+ * the rate factor reads are not part of the adjoint state. */
 int orad_adjoint_forward(orad_ctx *ctx, int ncycle, int every);
 int orad_adjoint_reverse(orad_ctx *ctx, const double *lam_x, const double *lam_v,
                          const double *lam_vr, const double *lam_d, const double *lam_state,

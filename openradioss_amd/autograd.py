@@ -14,6 +14,7 @@ sensitivities reach the thickness gradient.
 """
 import numpy as np
 import torch
+from torch.autograd.function import once_differentiable
 
 from . import engine as en
 
@@ -55,13 +56,24 @@ class ShellRun(torch.autograd.Function):
         except Exception:
             s.close()
             raise
-        ctx.solver = s
+        if any(ctx.needs_input_grad[:6]):
+            ctx.solver = s
+        else:
+            # no input needs a gradient: release the device checkpoints at once
+            s.close()
+            ctx.solver = None
         return (torch.from_numpy(nodes["d"]),
                 torch.from_numpy(st[:, en.SH_PLA0:en.SH_PLA0 + 5].copy()))
 
     @staticmethod
+    @once_differentiable
     def backward(ctx, gd, gpla):
         s = ctx.solver
+        if s is None:
+            raise RuntimeError("ShellRun.backward: the adjoint sweep of this run has already "
+                               "been taken (its checkpoints are released after one backward "
+                               "pass), or no input required a gradient")
+        ctx.solver = None
         try:
             lam_state = np.zeros((s.numelc, s.adjoint_nstate()))
             lam_state[:, 25:30] = gpla.numpy()          # PLA(1..5) in the adjoint state

@@ -257,17 +257,8 @@ int orad_set_nodes(orad_ctx *c, int numnod, const double *x, const double *v, co
   if (numnod <= 0 || !x || !v || !ms) return fail(c, ORAD_EINVAL, "set_nodes: bad arguments");
   if (c->have_nodes && numnod != c->numnod)
     return fail(c, ORAD_EINVAL, "set_nodes: node count differs from the mesh");
-  HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  drop_graphs(c);
   const size_t n = (size_t)numnod;
-  if (!c->have_nodes) {
-    HIPCHK(c, dalloc(&c->X4, n * sizeof(double4)));
-    HIPCHK(c, dalloc(&c->V4, n * sizeof(double4)));
-    HIPCHK(c, dalloc(&c->D, 3 * n * sizeof(double)));
-    HIPCHK(c, dalloc(&c->A, 3 * n * sizeof(double)));
-    HIPCHK(c, dalloc(&c->STIFN, n * sizeof(double)));
-  }
+  // validate and pack on the host before any device allocation
   std::vector<double4> hx(n), hv(n);
   for (size_t i = 0; i < n; ++i) {
     hx[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], ms[i]);
@@ -276,6 +267,14 @@ int orad_set_nodes(orad_ctx *c, int numnod, const double *x, const double *v, co
     const int hi = c->h_code_hi.empty() ? 0 : c->h_code_hi[i];
     hv[i] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], (double)(code | hi));
   }
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  if (!c->X4) HIPCHK(c, dalloc(&c->X4, n * sizeof(double4)));
+  if (!c->V4) HIPCHK(c, dalloc(&c->V4, n * sizeof(double4)));
+  if (!c->D) HIPCHK(c, dalloc(&c->D, 3 * n * sizeof(double)));
+  if (!c->A) HIPCHK(c, dalloc(&c->A, 3 * n * sizeof(double)));
+  if (!c->STIFN) HIPCHK(c, dalloc(&c->STIFN, n * sizeof(double)));
   HIPCHK(c, hipMemcpy(c->X4, hx.data(), n * sizeof(double4), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->V4, hv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemset(c->D, 0, 3 * n * sizeof(double)));
@@ -452,7 +451,7 @@ int orad_sforc3(orad_ctx *c, int nft, int nel, double dt1, double *dt2t) {
   if (c->family != FAM_BRICK) return fail(c, ORAD_ESTATE, "sforc3: the mesh is not of bricks");
   int r = ready(c);
   if (r) return r;
-  if (nft < 0 || nel < 0 || nft + nel > c->numels)
+  if (nft < 0 || nel < 0 || nel > c->numels - nft)
     return fail(c, ORAD_EINVAL, "sforc3: group outside the element range");
   HIPCHK(c, hipSetDevice(c->device));
   unsigned long long bits = 0x7FF0000000000000ull;
@@ -1027,7 +1026,7 @@ int orad_cforc3(orad_ctx *c, int nft, int nel, double dt1, double *dt2t) {
   if (c->family != FAM_SHELL) return fail(c, ORAD_ESTATE, "cforc3: the mesh is not of shells");
   int r = ready(c);
   if (r) return r;
-  if (nft < 0 || nel < 0 || nft + nel > c->numelc)
+  if (nft < 0 || nel < 0 || nel > c->numelc - nft)
     return fail(c, ORAD_EINVAL, "cforc3: group outside the element range");
   HIPCHK(c, hipSetDevice(c->device));
   unsigned long long bits = 0x7FF0000000000000ull;
@@ -1038,7 +1037,7 @@ int orad_cforc3(orad_ctx *c, int nft, int nel, double dt1, double *dt2t) {
     // a group of batched models lies within one model and takes that model's material
     const int m = (int)(std::upper_bound(c->h_elem_off.begin(), c->h_elem_off.end(), nft) -
                         c->h_elem_off.begin()) - 1;
-    if (nft + nel > c->h_elem_off[m + 1])
+    if (nel > c->h_elem_off[m + 1] - nft)
       return fail(c, ORAD_EINVAL, "cforc3: a group of batched models must lie in one model");
     c->shmat = c->h_mats[m];
   }
@@ -1421,6 +1420,11 @@ int orad_adjoint_forward(orad_ctx *c, int ncycle, int every) {
   int r = ready(c);
   if (r) return r;
   if (ncycle < 1 || every < 1) return fail(c, ORAD_EINVAL, "adjoint_forward: bad cycle counts");
+  // The adjoint state carries SIG, PLA and the hourglass forces.  The strain-rate
+  // filter EPSDP and the thickness THK are also carried from cycle to cycle, but
+  // they reach the forces only through the Johnson-Cook rate factor, i.e. when C != 0.
+  if (c->shmat.cc != 0.0)
+    return fail(c, ORAD_EINVAL, "adjoint_forward: rate-dependent law 2 (C != 0) not differentiated");
   HIPCHK(c, hipSetDevice(c->device));
   free_adjoint(c);
   AdjStore *a = new (std::nothrow) AdjStore();

@@ -245,3 +245,32 @@ def test_adjoint_qeph_plate_128_vs_fd():
     for h, e in errs.items():
         print(f"h={h:g}: max relative error over {len(e)} thicknesses {max(e):.2e}")
         assert max(e) < 1e-3
+
+
+def test_adjoint_refuses_rate_dependent_law_and_second_backward():
+    """C != 0 makes the rate filter EPSDP and the thickness THK part of the forces,
+    which the adjoint state does not carry: the forward run is refused.  A second
+    backward through the same run raises a clear error (the checkpoints are released
+    after the first)."""
+    deck, centre, wf, dtmx = setup_plate(8, 2)
+    rate = dict(deck)
+    rate["mat"] = dict(deck["mat"], cc=0.05, epdr=1.0)
+    s = en.ShellSolver(rate, en.FP64)
+    with pytest.raises(en.OradError) as ei:
+        s.adjoint_forward(4, 2)
+    assert ei.value.code == en.ORAD_EINVAL and "C != 0" in str(ei.value)
+    s.close()
+    m = deck["mat"]
+    A = torch.tensor(m["ca"], dtype=torch.float64, requires_grad=True)
+    B = torch.tensor(m["cb"], dtype=torch.float64)
+    n = torch.tensor(m["cn"], dtype=torch.float64)
+    thk = torch.tensor(deck["thk"], dtype=torch.float64)
+    x = torch.from_numpy(deck["x"])
+    ixc = torch.from_numpy(deck["ixc"].astype(np.int64))
+    ms, inr, _ = masses_torch(x, ixc, thk, m["rho0"])
+    d, pla = ShellRun.apply(A, B, n, thk, ms, inr, deck, en.FP64, 6, 3, dtmx)
+    J = d[centre, 2] + pla.sum()
+    J.backward(retain_graph=True)
+    assert np.isfinite(A.grad.item())
+    with pytest.raises(RuntimeError, match="already"):
+        J.backward()

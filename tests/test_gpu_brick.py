@@ -225,6 +225,13 @@ def test_error_paths_return_codes():
     with pytest.raises(en.OradError) as ei:
         s.sforc3(20, 100, 1e-7)
     assert ei.value.code == en.ORAD_EINVAL
+    # NFT + NEL would overflow a 32-bit int: refused, not wrapped into range
+    with pytest.raises(en.OradError) as ei:
+        s.sforc3(2**31 - 10, 100, 1e-7)
+    assert ei.value.code == en.ORAD_EINVAL
+    with pytest.raises(en.OradError) as ei:
+        s.sforc3(5, 2**31 - 1, 1e-7)
+    assert ei.value.code == en.ORAD_EINVAL
     bad = dict(d)
     bad["ixs"] = d["ixs"].copy()
     bad["ixs"][0, 0] = d["x"].shape[0] + 5
@@ -253,6 +260,29 @@ def test_slot_index_range_is_checked():
         assert lib.orad_set_bricks(h, 0, P(ixs), P(vol0)) == en.ORAD_EINVAL   # empty mesh
         assert lib.orad_set_bricks(h, 268435456, P(ixs), P(vol0)) == en.ORAD_EINVAL
         assert b"32-bit" in lib.orad_last_error(h)
+    finally:
+        lib.orad_destroy(h)
+
+
+def test_set_nodes_validates_before_allocating():
+    """A first orad_set_nodes with a bad ICODT fails before any device array exists;
+    the retry with good codes then sets the nodes (and nothing is allocated twice)."""
+    lib = en.load_library()
+    h = C.c_void_p()
+    assert lib.orad_create(C.byref(h), 0, en.FP64) == en.ORAD_OK
+    try:
+        P = lambda a: a.ctypes.data_as(C.c_void_p)
+        x, v, ms = np.zeros((4, 3)), np.zeros((4, 3)), np.ones(4)
+        bad = np.array([0, 9, 0, 0], np.int32)
+        assert lib.orad_set_nodes(h, 4, P(x), P(v), P(ms), P(bad)) == en.ORAD_EINVAL
+        assert b"ICODT" in lib.orad_last_error(h)
+        good = np.array([0, 7, 1, 0], np.int32)
+        assert lib.orad_set_nodes(h, 4, P(x), P(v), P(ms), P(good)) == en.ORAD_OK
+        x[:, 0] = 1.0
+        assert lib.orad_set_nodes(h, 4, P(x), P(v), P(ms), P(good)) == en.ORAD_OK
+        got = np.zeros((4, 3))
+        assert lib.orad_get_nodes(h, P(got), None, None, None, None) == en.ORAD_OK
+        assert np.array_equal(got, x)
     finally:
         lib.orad_destroy(h)
 

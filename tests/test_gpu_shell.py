@@ -239,6 +239,12 @@ def test_shell_error_paths():
     assert s.lib.orad_cforc3(s.h, 0, 1, 1e-7, ctypes.byref(dt2t)) == en.ORAD_ESTATE
     s.close()
     dp = dk.shell_plate(n=4, patch=2)
+    s = en.ShellSolver(dp, en.FP64)
+    # NFT + NEL beyond INT_MAX is refused rather than wrapped
+    assert s.lib.orad_cforc3(s.h, 2**31 - 8, 16, 1e-7, ctypes.byref(dt2t)) == en.ORAD_EINVAL
+    assert s.lib.orad_cforc3(s.h, 3, 2**31 - 1, 1e-7, ctypes.byref(dt2t)) == en.ORAD_EINVAL
+    s.close()
+    dp = dk.shell_plate(n=4, patch=2)
     dp["prop"] = dict(dp["prop"], npt=6)
     with pytest.raises(en.OradError) as ei:
         en.ShellSolver(dp, en.FP64)
