@@ -130,6 +130,17 @@ int orad_get_fsky(orad_ctx *ctx, double *fsky);
  * order, so they are run-to-run reproducible. */
 int orad_energies(orad_ctx *ctx, double *eint, double *ekin, double *ehour);
 
+/* orad_energies without any device allocation: the same kernels and results
+ * (bitwise), with their partial sums in a fixed 1 MiB scratch array that the
+ * library keeps on each GPU for the life of the process.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight monitoring loops of a serving process, harnesses that
+ * forbid hipMalloc).  Limit: ceil(elements/256) + ceil(nodes/256) + 2 <= 131072
+ * (about 33 million elements plus nodes), ORAD_EINVAL beyond.  Every context on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (from several host
+ * threads) race; calls made one after another do not. */
+int orad_energies_noalloc(orad_ctx *ctx, double *eint, double *ekin, double *ehour);
+
 /* Imposed velocities, FIXVEL semantics for a constant velocity curve
  * (constraints/general/impvel/fixvel.F:362-375): after BCS10, the imposed
  * component's acceleration becomes (VEL - V)/DT12.  nodes 1-based, dir 1..3

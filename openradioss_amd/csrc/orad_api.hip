@@ -23,6 +23,35 @@
 #include "orad_ctx.hpp"
 #include "orad_adjoint_args.hpp"
 
+#include <mutex>
+
+// Scratch of orad_energies_noalloc: the per-workgroup partial sums and the two
+// results of the energy reduction, in a file-scope device array instead of a
+// per-call allocation.  Each GPU the library runs on holds its own copy until the
+// process ends; host code reaches it only through the per-device address taken
+// with hipGetSymbolAddress at context creation (the array's name is not a device
+// address on the host).
+static constexpr size_t NOALLOC_SCRATCH_DOUBLES = 131072;   // 1 MiB
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+static constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Address of this device's scratch array; called with the device current.
+static hipError_t scratch_address(int device, double **out) {
+  *out = nullptr;
+  if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = (double *)p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
 template <typename T>
 static BrickArgs<T> brick_args(orad_ctx *c) {
   BrickArgs<T> a{};
@@ -201,6 +230,7 @@ int orad_create(orad_ctx **out, int device, int precision) {
   if (const char *xr = std::getenv("ORAD_XCD_REMAP")) c->xcd_remap = std::atoi(xr) != 0;
   if (const char *el = std::getenv("ORAD_ELEM_LDS_KB")) c->elem_lds = 1024 * std::atoi(el);
   hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = scratch_address(device, &c->noalloc_scratch);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = dalloc(&c->ctl, sizeof(CycleCtl));
   if (e == hipSuccess) e = dalloc(&c->scratch_dt1, sizeof(double));
@@ -660,14 +690,21 @@ int orad_get_fsky(orad_ctx *c, double *fsky) {
 
 // Internal energy of elements [e0, e1), kinetic energy of nodes [n0, n1) and the
 // hourglass energy totals of models [m0, m1).
+// `scr` (nullable): a buffer of at least energy_scratch_doubles() doubles to use
+// instead of a per-call allocation.
+static size_t energy_scratch_doubles(size_t me, size_t mn) {
+  return (me + 255) / 256 + (mn + 255) / 256 + 2;
+}
+
 static int energies_range(orad_ctx *c, size_t e0, size_t e1, size_t n0, size_t n1, int m0,
-                          int m1, double *eint, double *ekin, double *ehour) {
+                          int m1, double *eint, double *ekin, double *ehour,
+                          double *scr = nullptr) {
   HIPCHK(c, hipSetDevice(c->device));
   const bool shell = c->family == FAM_SHELL;
   const size_t ne = (size_t)nelem(c), ts = c->tsize(), me = e1 - e0, mn = n1 - n0;
   const size_t nbe = (me + 255) / 256, nbk = (mn + 255) / 256;
-  double *part = nullptr;
-  HIPCHK(c, dalloc(&part, (nbe + nbk + 2) * sizeof(double)));
+  double *part = scr;
+  if (!part) HIPCHK(c, dalloc(&part, energy_scratch_doubles(me, mn) * sizeof(double)));
   const char *st = (const char *)c->state;
   hipError_t e = launch_energy(c->precision == ORAD_MIXED, (int)me,
                                shell ? st + (SC_EINT * ne + e0) * ts : st + (C_EINT * ne + e0) * ts,
@@ -682,7 +719,7 @@ static int energies_range(orad_ctx *c, size_t e0, size_t e1, size_t n0, size_t n
     e = hipMemcpy(&h, &c->ctl[m].ehour, sizeof(double), hipMemcpyDeviceToHost);
     eh += h;
   }
-  dfree(part);
+  if (!scr) dfree(part);
   if (e != hipSuccess) return fail(c, ORAD_EHIP, std::string("energies: ") + hipGetErrorString(e));
   if (eint) *eint = res[0];
   if (ekin) *ekin = res[1];
@@ -696,6 +733,21 @@ int orad_energies(orad_ctx *c, double *eint, double *ekin, double *ehour) {
   if (r) return r;
   return energies_range(c, 0, (size_t)nelem(c), 0, (size_t)c->numnod, 0, c->nmodel, eint, ekin,
                         ehour);
+}
+
+int orad_energies_noalloc(orad_ctx *c, double *eint, double *ekin, double *ehour) {
+  if (!c) return ORAD_EINVAL;
+  int r = ready(c);
+  if (r) return r;
+  if (!c->noalloc_scratch) return fail(c, ORAD_ESTATE, "energies_noalloc: no device scratch");
+  const size_t need = energy_scratch_doubles((size_t)nelem(c), (size_t)c->numnod);
+  if (need > NOALLOC_SCRATCH_DOUBLES)
+    return fail(c, ORAD_EINVAL,
+                "energies_noalloc: mesh too large for the fixed 1 MiB device scratch "
+                "(ceil(elements/256) + ceil(nodes/256) + 2 must not exceed 131072); "
+                "use orad_energies");
+  return energies_range(c, 0, (size_t)nelem(c), 0, (size_t)c->numnod, 0, c->nmodel, eint, ekin,
+                        ehour, c->noalloc_scratch);
 }
 
 }  // extern "C"

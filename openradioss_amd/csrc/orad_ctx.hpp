@@ -108,6 +108,7 @@ struct orad_ctx {
   CycleCtl *ctl = nullptr;
   double *scratch_dt1 = nullptr;
   unsigned long long *scratch_dt2t = nullptr;
+  double *noalloc_scratch = nullptr;   // this device's file-scope scratch array (not owned)
 
   // batched independent shell models (orad_set_shell_models): per-model material,
   // time scalars and time step; element tiles of 256 elements within one model

@@ -22,6 +22,7 @@ EXPORTS = (
     "orad_set_law2", "orad_set_solid_prop", "orad_get_pon", "orad_set_time", "orad_get_time",
     "orad_sforc3", "orad_asspar4", "orad_run_cycles", "orad_profile_cycles", "orad_get_nodes",
     "orad_get_brick_state", "orad_set_brick_state", "orad_get_fsky", "orad_energies",
+    "orad_energies_noalloc",
     "orad_set_fixvel", "orad_set_nodes_rot", "orad_set_shells", "orad_set_shell_law2",
     "orad_set_shell_prop", "orad_set_shell_ishell", "orad_cforc3", "orad_asspar4_rot", "orad_get_nodes_rot",
     "orad_get_shell_state", "orad_set_shell_state", "orad_set_shell_models",
@@ -88,6 +89,7 @@ def load_library(path=LIB_PATH):
         "orad_set_brick_state": ([P, P, P, P, P, P, P], I),
         "orad_get_fsky": ([P, P], I),
         "orad_energies": ([P, C.POINTER(D), C.POINTER(D), C.POINTER(D)], I),
+        "orad_energies_noalloc": ([P, C.POINTER(D), C.POINTER(D), C.POINTER(D)], I),
         "orad_set_fixvel": ([P, I, P, P, P], I),
         "orad_set_nodes_rot": ([P, P, P, P], I),
         "orad_set_shells": ([P, I, P, P], I),

@@ -103,6 +103,12 @@ module orad_bind
       type(c_ptr), value :: ctx
       real(c_double) :: eint, ekin, ehour
     end function
+    integer(c_int) function orad_energies_noalloc(ctx, eint, ekin, ehour) &
+        bind(c, name='orad_energies_noalloc')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: eint, ekin, ehour
+    end function
     integer(c_int) function orad_set_fixvel(ctx, n, nodes, dir, vel) bind(c, name='orad_set_fixvel')
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
