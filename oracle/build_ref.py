@@ -1,7 +1,7 @@
 """Recipe for oracle/_ref: the reference's own brick and shell leaf kernels,
 compiled from their sources where they lie under the reference tree, plus the
-test harnesses oracle/ref/harness_brick.F, harness_shell.F and harness_qeph.F,
-linked into
+test harnesses oracle/ref/harness_brick.F, harness_shell.F, harness_qeph.F and
+harness_nodal.F, linked into
 oracle/_ref/libref.so.
 
 Test infrastructure only.  Runs in the build container (amdflang from ROCm);
@@ -69,6 +69,22 @@ LEAF = [
     "engine/source/elements/shell/coquez/czfintn.F",
     "engine/source/elements/shell/coquez/czproj.F",
     "engine/source/elements/shell/coque/cupdtn3.F",
+    "engine/source/assembly/accele.F",
+    "engine/source/constraints/general/bcs/bcs10.F",
+    "engine/source/constraints/general/bcs/bcs1.F",
+    "engine/source/constraints/general/impvel/fixvel.F",
+    "engine/source/tools/curve/vinterdp.F",
+    "engine/source/assembly/velocity.F",
+    "engine/source/assembly/displacement.F",
+]
+
+
+# C++ sources of the reference that the Fortran routines above call (FIXVEL's
+# function-evaluation path reaches the hash map and the Python-function modules);
+# compiled with g++ from where they lie, only standard headers needed.
+CXX = [
+    "common_source/tools/container/umap.cpp",
+    "common_source/modules/cpp_python_funct.cpp",
 ]
 
 
@@ -133,6 +149,12 @@ def main():
     comp(os.path.join(HERE, "ref", "harness_brick.F"))
     comp(os.path.join(HERE, "ref", "harness_shell.F"))
     comp(os.path.join(HERE, "ref", "harness_qeph.F"))
+    comp(os.path.join(HERE, "ref", "harness_nodal.F"))
+    for rel in CXX:
+        obj = os.path.join(OUT, re.sub(r"[^A-Za-z0-9_]", "_", rel) + ".o")
+        subprocess.run(["g++", "-c", "-fPIC", "-O2", "-std=c++17", os.path.join(REF, rel),
+                        "-o", obj], check=True)
+        objs.append(obj)
     lib = os.path.join(OUT, "libref.so")
     # Routines referenced on branches the path does not take (error messages,
     # SPH, orthotropic frames) are compiled from their own reference sources too,
@@ -144,7 +166,7 @@ def main():
     # than run a substitute.
     subs = subroutine_index()
     for _ in range(32):
-        subprocess.run(["amdflang", "-shared", "-o", lib] + objs, check=True)
+        subprocess.run(["amdflang", "-shared", "-o", lib] + objs + LIBS, check=True)
         und = subprocess.run(["nm", "-u", lib], capture_output=True, text=True).stdout.split()
         need = sorted({s[:-1] for s in und if re.fullmatch(r"[a-z][a-z0-9_]*_", s)} - {""})
         need = [s for s in need if s in subs and s not in ERROR_PATH]
@@ -157,7 +179,7 @@ def main():
         weak += ["--weaken-symbol", s + "_"]
     for o in objs:
         subprocess.run(["objcopy"] + weak + [o], check=True)
-    subprocess.run(["amdflang", "-shared", "-o", lib] + objs, check=True)
+    subprocess.run(["amdflang", "-shared", "-o", lib] + objs + LIBS, check=True)
     und = subprocess.run(["nm", "-u", lib], capture_output=True, text=True).stdout.splitlines()
     left = [ln.split()[-1] for ln in und
             if ln.split()[0] == "U" and re.fullmatch(r"[a-z][a-z0-9_]*_", ln.split()[-1])]
@@ -166,6 +188,8 @@ def main():
     print(f"built {lib} from {len(objs)} objects")
     return 0
 
+
+LIBS = ["-lstdc++", "-ldl"]
 
 ERROR_PATH = ("ancmsg", "arret",
               # Called only by CNCOEF3 (sh3n/coquedk/cncoef3.F:610-939), the

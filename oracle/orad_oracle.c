@@ -556,20 +556,42 @@ void oracle_cycle(orad_omodel_t *m) {
     }
     m->a[3*n] = ax; m->a[3*n + 1] = ay; m->a[3*n + 2] = az; m->stifn[n] = sf;
   }
-  /* dt (resol.F:2640, 5857-5861, 6080, 6224-6226) */
+  orad_onodes_t nd = {0};
+  nd.numnod = nn; nd.rot = 0;
+  nd.x = m->x; nd.v = m->v; nd.d = m->d; nd.a = m->a; nd.ms = m->ms; nd.icodt = m->icodt;
+  nd.fv_dir = m->fv_dir; nd.fv_val = m->fv_val; nd.fv_mem = m->fv_mem;
+  nd.tt = m->tt; nd.dt1 = m->dt1; nd.dt2 = m->dt2; nd.dt12 = m->dt12; nd.dt2old = m->dt2old;
+  nd.dtmx = m->dtmx; nd.wfext = m->wfext; nd.ncycle = m->ncycle; nd.nthreads = m->nthreads;
+  oracle_nodal_update(&nd, dt2t);
+  m->tt = nd.tt; m->dt1 = nd.dt1; m->dt2 = nd.dt2; m->dt12 = nd.dt12; m->dt2old = nd.dt2old;
+  m->wfext = nd.wfext; m->ncycle = nd.ncycle;
+  m->mstop |= mstop;
+}
+
+/* Nodal part of the cycle (declared in orad_oracle.h).  The FIXVEL loop runs over the
+ * imposed nodes in increasing node order, the order of the reference's entries. */
+void oracle_nodal_update(orad_onodes_t *m, double dt2t) {
+  const int nn = m->numnod;
+#ifdef _OPENMP
+  int nth = m->nthreads > 0 ? m->nthreads : omp_get_max_threads();
+#endif
+  /* resol.F:2639-2640 */
+  m->dt1 = m->dt2;
+  const double dt1 = m->dt1;
+  /* resol.F:5857-5861 */
   double dt2 = 1.0e6;                         /* DT2 = EP06 */
   if (dt2t < dt2) dt2 = dt2t;
   /* resol.F:6080 writes 1.1*DT2OLD with a default-REAL literal */
   dt2 = dmin(dmin(dt2, (double)1.1f*m->dt2old), m->dtmx);
+  /* resol.F:6224-6226 */
   m->dt2old = dt2;
   m->dt2 = dt2;
   m->dt12 = HALF*(dt1 + dt2);
   const double dt12 = m->dt12;
-  /* ACCELE -> BCS10 -> FIXVEL -> VELOCITY -> DEPLA (accele.F:64-82, bcs1.F, fixvel.F:362-375,
-   * displacement.F:88-101) */
+  /* ACCELE (accele.F:64-110) and BCS10 -> BCS1V (bcs1.F:46-76), node by node */
 #pragma omp parallel for num_threads(nth) schedule(static)
   for (int n = 0; n < nn; ++n) {
-    double a[3] = {m->a[3*n], m->a[3*n + 1], m->a[3*n + 2]};
+    double *a = m->a + 3*n, *v = m->v + 3*n;
     if (m->ms[n] > ZERO) {
       double RTMP = ONE/m->ms[n];
       a[0] = a[0]*RTMP; a[1] = a[1]*RTMP; a[2] = a[2]*RTMP;
@@ -577,26 +599,60 @@ void oracle_cycle(orad_omodel_t *m) {
       a[0] = ZERO; a[1] = ZERO; a[2] = ZERO;
     }
     int c = m->icodt[n];
-    double *v = m->v + 3*n;
     if (c & 4) { v[0] = ZERO; a[0] = ZERO; }
     if (c & 2) { v[1] = ZERO; a[1] = ZERO; }
     if (c & 1) { v[2] = ZERO; a[2] = ZERO; }
-    if (m->fv_dir && m->fv_dir[n] > 0) {
-      /* FIXVEL (constraints/general/impvel/fixvel.F:362-375), constant curve */
-      int d = m->fv_dir[n] - 1;
-      a[d] = (m->fv_val[n] - v[d]) / dt12;
+    if (m->rot) {
+      double *ar = m->ar + 3*n, *vr = m->vr + 3*n;
+      if (m->inr[n] > ZERO) {
+        double RTMP = ONE/m->inr[n];
+        ar[0] = ar[0]*RTMP; ar[1] = ar[1]*RTMP; ar[2] = ar[2]*RTMP;
+      } else {
+        ar[0] = ZERO; ar[1] = ZERO; ar[2] = ZERO;
+      }
+      int r = m->icodr[n];
+      if (r & 4) { vr[0] = ZERO; ar[0] = ZERO; }
+      if (r & 2) { vr[1] = ZERO; ar[1] = ZERO; }
+      if (r & 1) { vr[2] = ZERO; ar[2] = ZERO; }
+    }
+  }
+  /* FIXVEL, constant curve, global direction (fixvel.F:333-374): the imposed component's
+   * acceleration becomes (VEL - V)/DT12; its work DW = MS/4 (A DT12 + 2V)(A - AOLD) enters
+   * WFEXT weighted DT1 now and DT2 at the next cycle (fixvel.F:333-335, 834-836). */
+  if (m->fv_dir) {
+    double wf = m->wfext;
+    for (int n = 0; n < nn; ++n) {
+      if (m->fv_dir[n] <= 0) continue;
+      const int j = m->fv_dir[n] - 1;
+      double *a = m->a + 3*n, *v = m->v + 3*n;
+      double dwm = m->fv_mem ? m->fv_mem[n] : ZERO;
+      wf = wf + ONE*dwm;
+      const double yc = (m->fv_val[n] - v[j]) / dt12;
+      const double aold = a[j];
+      a[j] = yc;
+      const double dw = FOURTH*m->ms[n]*ONE*ONE*(a[j]*dt12 + TWO*v[j])*(a[j] - aold);
+      wf = wf + dt1*dw;
+      if (m->fv_mem) m->fv_mem[n] = ZERO + m->dt2*dw;
+    }
+    m->wfext = wf;
+  }
+  /* TT (resol.F:8256-8266), VELOCITY (velocity.F:56-83), DEPLA (displacement.F:88-101) */
+  m->tt = m->tt + dt2;
+#pragma omp parallel for num_threads(nth) schedule(static)
+  for (int n = 0; n < nn; ++n) {
+    double *a = m->a + 3*n, *v = m->v + 3*n;
+    for (int d = 0; d < 3; ++d) v[d] = v[d] + dt12*a[d];
+    if (m->rot) {
+      double *ar = m->ar + 3*n, *vr = m->vr + 3*n;
+      for (int d = 0; d < 3; ++d) vr[d] = vr[d] + dt12*ar[d];
     }
     for (int d = 0; d < 3; ++d) {
-      m->a[3*n + d] = a[d];
-      v[d] = v[d] + dt12*a[d];
       double VDT = dt2*v[d];
       m->d[3*n + d] = m->d[3*n + d] + VDT;
       m->x[3*n + d] = m->x[3*n + d] + VDT;
     }
   }
-  m->tt = m->tt + dt2;
   m->ncycle += 1;
-  m->mstop |= mstop;
 }
 
 void oracle_energies(const orad_omodel_t *m, double *eint, double *ekin) {

@@ -84,6 +84,30 @@ double oracle_brick_volume(const double xg[3][8]);
 int oracle_build_pon(int numnod, int numels, const int32_t *ixs,
                      const int32_t *uid, int32_t *adsky, int32_t *iads);
 
+/* ---- Nodal part of one explicit cycle, in RESOL order ----
+ * time step (resol.F:2639-2640, 5857-5861, 6080, 6224-6226), ACCELE (accele.F:64-110),
+ * BCS10/BCS1V (bcs10.F, bcs1.F), FIXVEL for a constant curve with its external work
+ * (fixvel.F:333-374, 834-836), TT (resol.F:8256-8266), VELOCITY (velocity.F:56-83),
+ * DEPLA (displacement.F:88-101).  a (ar) holds the assembled forces (moments) on
+ * entry and the accelerations VELOCITY consumed on return. */
+typedef struct {
+  int numnod, rot;                /* rot != 0: rotational DOF (IRODDL = 1)          */
+  double *x, *v, *d, *a;          /* 3*numnod each                                  */
+  const double *ms;
+  const int32_t *icodt;
+  double *vr, *ar;                /* rot only                                       */
+  const double *inr;
+  const int32_t *icodr;
+  const int32_t *fv_dir;          /* FIXVEL direction per node (1..3), 0 = none, or null */
+  const double *fv_val;           /* imposed velocity per node                      */
+  double *fv_mem;                 /* per node: FIXVEL's VEL(4,N) energy memory, or null */
+  double tt, dt1, dt2, dt12, dt2old, dtmx;
+  double wfext;                   /* external work of the imposed velocities (WFEXT) */
+  int    ncycle, nthreads;
+} orad_onodes_t;
+
+void oracle_nodal_update(orad_onodes_t *m, double dt2t);
+
 /* ---- Whole-model explicit cycle (RESOL order, resol.F:2586-9595) ---- */
 typedef struct {
   int numnod, numels;
@@ -104,6 +128,8 @@ typedef struct {
   int    pad;
   const int32_t *fv_dir;          /* FIXVEL direction per node (1..3), 0 = none, or null */
   const double *fv_val;           /* FIXVEL imposed velocity per node              */
+  double *fv_mem;                 /* FIXVEL energy memory per node, or null        */
+  double wfext;                   /* external work of the imposed velocities       */
 } orad_omodel_t;
 
 /* One full cycle: element forces -> assembly -> dt -> accel -> BC -> vel -> disp. */
@@ -207,6 +233,8 @@ typedef struct {
   double tt, dt1, dt2, dt12, dt2old, dtmx;
   double ehour;
   int    ncycle, nthreads;
+  double *fv_mem;                 /* FIXVEL energy memory per node, or null      */
+  double wfext;                   /* external work of the imposed velocities     */
 } orad_oshell_t;
 
 void oracle_shell_cycle(orad_oshell_t *m);

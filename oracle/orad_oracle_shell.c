@@ -1487,57 +1487,18 @@ void oracle_shell_cycle(orad_oshell_t *m) {
     m->stifn[n] = s[6];
     m->stifr[n] = s[7];
   }
-  double dt2 = 1.0e6;
-  if (dt2t < dt2) dt2 = dt2t;
-  /* resol.F:6080 writes 1.1*DT2OLD with a default-REAL literal */
-  dt2 = dmin(dmin(dt2, (double)1.1f * m->dt2old), m->dtmx);
-  m->dt2old = dt2;
-  m->dt2 = dt2;
-  m->dt12 = HALF * (dt1 + dt2);
-  const double dt12 = m->dt12;
-  /* ACCELE (accele.F:64-110) -> BCS10 -> FIXVEL -> VELOCITY -> DEPLA */
-#pragma omp parallel for num_threads(nth) schedule(static)
-  for (int n = 0; n < nn; ++n) {
-    double a[3] = {m->a[3 * n], m->a[3 * n + 1], m->a[3 * n + 2]};
-    double ar[3] = {m->ar[3 * n], m->ar[3 * n + 1], m->ar[3 * n + 2]};
-    if (m->ms[n] > ZERO) {
-      double RTMP = ONE / m->ms[n];
-      a[0] = a[0] * RTMP; a[1] = a[1] * RTMP; a[2] = a[2] * RTMP;
-    } else {
-      a[0] = a[1] = a[2] = ZERO;
-    }
-    if (m->inr[n] > ZERO) {
-      double RTMP = ONE / m->inr[n];
-      ar[0] = ar[0] * RTMP; ar[1] = ar[1] * RTMP; ar[2] = ar[2] * RTMP;
-    } else {
-      ar[0] = ar[1] = ar[2] = ZERO;
-    }
-    double *v = m->v + 3 * n, *vr = m->vr + 3 * n;
-    int c = m->icodt[n], r = m->icodr[n];
-    if (c & 4) { v[0] = ZERO; a[0] = ZERO; }
-    if (c & 2) { v[1] = ZERO; a[1] = ZERO; }
-    if (c & 1) { v[2] = ZERO; a[2] = ZERO; }
-    if (r & 4) { vr[0] = ZERO; ar[0] = ZERO; }
-    if (r & 2) { vr[1] = ZERO; ar[1] = ZERO; }
-    if (r & 1) { vr[2] = ZERO; ar[2] = ZERO; }
-    if (m->fv_dir && m->fv_dir[n] > 0) {
-      /* FIXVEL (constraints/general/impvel/fixvel.F:36): the imposed component's
-       * acceleration is set so that V(t+dt12) equals the imposed value. */
-      int d = m->fv_dir[n] - 1;
-      a[d] = (m->fv_val[n] - v[d]) / dt12;
-    }
-    for (int d = 0; d < 3; ++d) {
-      m->a[3 * n + d] = a[d];
-      m->ar[3 * n + d] = ar[d];
-      v[d] = v[d] + dt12 * a[d];
-      vr[d] = vr[d] + dt12 * ar[d];
-      double VDT = dt2 * v[d];
-      m->d[3 * n + d] = m->d[3 * n + d] + VDT;
-      m->x[3 * n + d] = m->x[3 * n + d] + VDT;
-    }
-  }
-  m->tt = m->tt + dt2;
-  m->ncycle += 1;
+  /* time step, ACCELE, BCS10, FIXVEL, VELOCITY, DEPLA with rotations (RESOL order) */
+  orad_onodes_t nd = {0};
+  nd.numnod = nn; nd.rot = 1;
+  nd.x = m->x; nd.v = m->v; nd.d = m->d; nd.a = m->a; nd.ms = m->ms; nd.icodt = m->icodt;
+  nd.vr = m->vr; nd.ar = m->ar; nd.inr = m->inr; nd.icodr = m->icodr;
+  nd.fv_dir = m->fv_dir; nd.fv_val = m->fv_val; nd.fv_mem = m->fv_mem;
+  nd.tt = m->tt; nd.dt1 = m->dt1; nd.dt2 = m->dt2; nd.dt12 = m->dt12; nd.dt2old = m->dt2old;
+  nd.dtmx = m->dtmx; nd.wfext = m->wfext; nd.ncycle = m->ncycle; nd.nthreads = m->nthreads;
+  oracle_nodal_update(&nd, dt2t);
+  m->tt = nd.tt; m->dt1 = nd.dt1; m->dt2 = nd.dt2; m->dt12 = nd.dt12; m->dt2old = nd.dt2old;
+  m->wfext = nd.wfext; m->ncycle = nd.ncycle;
+  (void)dt1;
 }
 
 void oracle_shell_energies(const orad_oshell_t *m, double *eint, double *ekin) {

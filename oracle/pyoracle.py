@@ -47,7 +47,18 @@ class OModel(C.Structure):
                 ("dt12", C.c_double), ("dt2old", C.c_double), ("dtmx", C.c_double),
                 ("ehour", C.c_double), ("ncycle", C.c_int), ("mstop", C.c_int),
                 ("nthreads", C.c_int), ("pad", C.c_int),
-                ("fv_dir", C.c_void_p), ("fv_val", C.c_void_p)]
+                ("fv_dir", C.c_void_p), ("fv_val", C.c_void_p),
+                ("fv_mem", C.c_void_p), ("wfext", C.c_double)]
+
+
+class ONodes(C.Structure):
+    """orad_onodes_t: the nodal part of one cycle (oracle_nodal_update)."""
+    _fields_ = [("numnod", C.c_int), ("rot", C.c_int)] + \
+               [(n, C.c_void_p) for n in ("x", "v", "d", "a", "ms", "icodt", "vr", "ar",
+                                          "inr", "icodr", "fv_dir", "fv_val", "fv_mem")] + \
+               [(n, C.c_double) for n in ("tt", "dt1", "dt2", "dt12", "dt2old", "dtmx",
+                                          "wfext")] + \
+               [("ncycle", C.c_int), ("nthreads", C.c_int)]
 
 
 # ---- shells: one element's state row (orad_shstate_t), npt <= 5 ----
@@ -93,7 +104,8 @@ class OShell(C.Structure):
                [("mat", ShMat), ("prop", ShProp)] + \
                [(n, C.c_double) for n in ("tt", "dt1", "dt2", "dt12", "dt2old", "dtmx",
                                           "ehour")] + \
-               [("ncycle", C.c_int), ("nthreads", C.c_int)]
+               [("ncycle", C.c_int), ("nthreads", C.c_int),
+                ("fv_mem", C.c_void_p), ("wfext", C.c_double)]
 
 
 def _ptr(a):
@@ -133,6 +145,7 @@ def lib():
         _lib.oracle_shell_cycle.argtypes = [C.POINTER(OShell)]
         _lib.oracle_shell_energies.argtypes = [C.POINTER(OShell), C.POINTER(C.c_double),
                                                C.POINTER(C.c_double)]
+        _lib.oracle_nodal_update.argtypes = [C.POINTER(ONodes), C.c_double]
     return _lib
 
 
@@ -154,6 +167,8 @@ def ref():
             [C.c_double] + [C.c_void_p] * 7
         _ref.ref_shell_constants.argtypes = [C.c_void_p]
         _ref.ref_shell_tables.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        _ref.ref_nodal_cycle.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 10 + \
+            [C.c_int] + [C.c_void_p] * 4 + [C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
     return _ref
 
 
@@ -267,6 +282,8 @@ class Model:
             self.fv_dir = np.ascontiguousarray(deck["fv_dir"], dtype=np.int32)
             self.fv_val = np.ascontiguousarray(deck["fv_val"], dtype=np.float64)
             m.fv_dir, m.fv_val = _ptr(self.fv_dir), _ptr(self.fv_val)
+        self.fv_mem = np.zeros(self.x.shape[0])
+        m.fv_mem = _ptr(self.fv_mem)
         m.tt, m.dt1, m.dt2, m.dt12 = 0.0, 0.0, 0.0, 0.0
         m.dt2old = 1.0e6 / 1.1          # starter contrl.F:1108-1110: DTINI/ONEP1
         m.dtmx = 1.0e20
@@ -515,6 +532,8 @@ class ShellModel:
             setattr(m, name, _ptr(getattr(self, name)))
         m.mat = shmat(deck["mat"])
         m.prop = shprop(deck["prop"])
+        self.fv_mem = np.zeros(nn)
+        m.fv_mem = _ptr(self.fv_mem)
         m.dt2old = 1.0e6 / 1.1
         m.dtmx = 1.0e20
         m.nthreads = nthreads
@@ -536,3 +555,74 @@ class ShellModel:
     @property
     def tt(self):
         return self.m.tt
+
+
+# ============================ nodal update ============================
+
+TIME_KEYS = ("tt", "dt1", "dt2", "dt12", "dt2old", "dtmx")
+
+
+def _nodal_arrays(nodes):
+    """Contiguous copies of a node set dict: x, v, d, a (forces), ms, icodt and, with
+    rotations, vr, ar (moments), in, icodr; fv_dir, fv_val, fv_mem (per node)."""
+    nn = nodes["x"].shape[0]
+    f = lambda k, dt=np.float64, shape=None: np.ascontiguousarray(
+        nodes[k] if k in nodes else np.zeros(shape if shape else (nn, 3), dt), dt).copy()
+    out = dict(x=f("x"), v=f("v"), d=f("d"), a=f("a"), ms=f("ms", shape=(nn,)),
+               icodt=f("icodt", np.int32, (nn,)), vr=f("vr"), ar=f("ar"),
+               inr=f("in", shape=(nn,)), icodr=f("icodr", np.int32, (nn,)),
+               fv_dir=f("fv_dir", np.int32, (nn,)), fv_val=f("fv_val", shape=(nn,)),
+               fv_mem=f("fv_mem", shape=(nn,)))
+    return out
+
+
+def nodal_update(nodes, time, dt2t, rot=False, nthreads=1):
+    """Oracle nodal update (oracle_nodal_update) of one cycle.  nodes: dict of arrays
+    (see _nodal_arrays; a/ar = assembled forces/moments); time: dict of TIME_KEYS plus
+    wfext and ncycle.  Returns (new node arrays, new time dict); a/ar then hold the
+    accelerations VELOCITY consumed."""
+    o = _nodal_arrays(nodes)
+    m = ONodes()
+    m.numnod, m.rot = o["x"].shape[0], int(rot)
+    for k in ("x", "v", "d", "a", "ms", "icodt", "vr", "ar", "inr", "icodr", "fv_dir",
+              "fv_val", "fv_mem"):
+        setattr(m, k, _ptr(o[k]))
+    for k in TIME_KEYS:
+        setattr(m, k, float(time[k]))
+    m.wfext = float(time.get("wfext", 0.0))
+    m.ncycle = int(time.get("ncycle", 0))
+    m.nthreads = nthreads
+    lib().oracle_nodal_update(C.byref(m), float(dt2t))
+    t = {k: getattr(m, k) for k in TIME_KEYS}
+    t["wfext"], t["ncycle"] = m.wfext, m.ncycle
+    return o, t
+
+
+def ref_nodal_cycle(nodes, time, dt2t, rot=False):
+    """The reference's own ACCELE/BCS10/FIXVEL/VELOCITY/DEPLA (oracle/_ref, harness_nodal.F)
+    on the same inputs as nodal_update.  FIXVEL entries are the imposed nodes in increasing
+    node order; their energy memory travels in fv_mem."""
+    o = _nodal_arrays(nodes)
+    nn = o["x"].shape[0]
+    fxn = np.nonzero(o["fv_dir"] > 0)[0]
+    fxnode = np.ascontiguousarray(fxn + 1, np.int32)
+    fxdir = np.ascontiguousarray(o["fv_dir"][fxn], np.int32)
+    fxval = np.ascontiguousarray(o["fv_val"][fxn], np.float64)
+    fxmem = np.zeros((len(fxn), 2))
+    fxmem[:, 0] = o["fv_mem"][fxn]
+    times = np.array([float(time[k]) for k in TIME_KEYS])
+    ncyc = C.c_int(int(time.get("ncycle", 0)))
+    wf = C.c_double(float(time.get("wfext", 0.0)))
+    if len(fxn) == 0:
+        fxnode, fxdir, fxval, fxmem = (np.zeros(1, np.int32), np.zeros(1, np.int32),
+                                       np.zeros(1), np.zeros((1, 2)))
+    ref().ref_nodal_cycle(nn, int(rot), _ptr(o["x"]), _ptr(o["v"]), _ptr(o["vr"]),
+                          _ptr(o["d"]), _ptr(o["a"]), _ptr(o["ar"]), _ptr(o["ms"]),
+                          _ptr(o["inr"]), _ptr(o["icodt"]), _ptr(o["icodr"]), len(fxn),
+                          _ptr(fxnode), _ptr(fxdir), _ptr(fxval), _ptr(fxmem), float(dt2t),
+                          _ptr(times), C.byref(ncyc), C.byref(wf))
+    if len(fxn):
+        o["fv_mem"][fxn] = fxmem[:, 0]
+    t = dict(zip(TIME_KEYS, times.tolist()))
+    t["wfext"], t["ncycle"] = wf.value, ncyc.value
+    return o, t

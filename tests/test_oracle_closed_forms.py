@@ -238,3 +238,30 @@ def test_dt_growth_limit_uses_the_real_literal():
     m.cycle(1)
     assert m.dt2 == float(np.float32(1.1)) * 1.0e-9
     assert m.dt2 != 1.1 * 1.0e-9
+
+
+@pytest.mark.parametrize("s", [0.01, 0.03])
+def test_law2_prologue_pure_volumetric_step(s):
+    """MMAIN's law-2 prologue (mmain.F90:678-737) with M2LAW's pressure and energy
+    (m2law.F:448-470) on a uniformly compressed cube, v = -r x, from the initial state
+    (SIG = 0, EINT = 0, RHO = rho0, V0 = 1) without bulk viscosity (qa = qb = 0):
+      RHO = rho0 V0 / V,  AMU = RHO/rho0 - 1,  SIG_ii = -K AMU,  deviator 0,
+      EINT V0 = VOL_AVG * sum_i D_i SIG_i * DT1/2,  VOL_AVG = V - DVOL/2 = (V + V0)/2,
+    with D_ii = -r up to the co-rotational mid-step correction O(r DT1).  An inverted
+    AMU (V/V0 - 1), VOL_AVG = V or a missing 1/2 fail it."""
+    r, dt = 10.0, 1.0e-7
+    x = (1.0 - s) * CUBE
+    v = -r * x
+    prop = dk.solid_params(qa=0.0, qb=0.0)
+    st = fresh_state(1.0)
+    o = po.brick_element(x, v, st, dt, MAT, prop)
+    V = (1.0 - s) ** 3
+    assert o["voln"] == pytest.approx(V, rel=1e-14)
+    assert st[po.ST_RHO] == pytest.approx(MAT["rho0"] / V, rel=1e-14)
+    amu = 1.0 / V - 1.0
+    K = MAT["bulk"]
+    np.testing.assert_allclose(st[0:3], -K * amu, rtol=1e-12)
+    assert np.abs(st[3:6]).max() < 1e-9 * K * amu
+    einc = 0.5 * (V + 1.0) * 3.0 * (-r) * (-K * amu) * 0.5 * dt
+    assert st[po.ST_EINT] == pytest.approx(einc, rel=1e-5)
+    assert st[po.ST_QVIS] == 0.0

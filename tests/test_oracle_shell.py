@@ -184,3 +184,22 @@ def test_plate_impact_is_bounded_and_plastic():
     ei, ek, eh = m.energies()
     assert 0.85 * ek0 < ei + ek + eh < 1.15 * ek0
     assert m.st[:, po.SH_PLA0:po.SH_PLA0 + 5].max() > 1e-3
+
+
+@pytest.mark.parametrize("case", ["stretch", "biaxial", "bending"])
+def test_element_strain_rate_is_the_shell_energy_equivalent(case):
+    """CFORC3's element strain rate (cforc3.F:530-546), the rate of the energy-equivalent
+    strain e_eq = sqrt(4/3 (exx^2 + eyy^2 + exx eyy + exy^2/4) + t^2/9 (kxx^2 + kyy^2 +
+    kxx kyy + kxy^2/4)): a uniaxial membrane rate r gives 2 r / sqrt(3), an equal biaxial
+    one 2 r, a pure bending rate k of a shell of thickness t gives k t / 3.  The stored
+    GBUF%EPSD is that rate (ASRATE = 1)."""
+    r, k, t = 40.0, 0.05, 1.3
+    zero = 0 * SQ[0]
+    if case == "stretch":
+        v, vr, expect = np.stack([r * SQ[0], zero, zero]), np.zeros((3, 4)), 2 * r / np.sqrt(3)
+    elif case == "biaxial":
+        v, vr, expect = np.stack([r * SQ[0], r * SQ[1], zero]), np.zeros((3, 4)), 2 * r
+    else:
+        v, vr, expect = np.zeros((3, 4)), np.stack([zero, k * SQ[0], zero]), k * t / 3
+    o, st = _elem(v, vr, dt1=1e-7, thk=t)
+    assert st[po.SH_EPSD] == pytest.approx(expect, rel=1e-5)
