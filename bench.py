@@ -229,6 +229,25 @@ def run(prec, deck, steps, warmup):
     return s, (t1 - t0), t, el_ms, nd_ms
 
 
+def mixed_vs_fp64_error(s32, s64, cycles):
+    """Both forms continued to the same cycle count (`cycles` in all), then the relative
+    L2 distance of their assembled nodal forces (ASSPAR4 of the last cycle's slots),
+    displacements and stresses."""
+    for s in (s32, s64):
+        left = cycles - s.time().ncycle
+        if left > 0:
+            s.run_cycles(left)
+    if s32.time().ncycle != s64.time().ncycle:
+        return None
+    a32, _ = s32.asspar4()
+    a64, _ = s64.asspar4()
+    d32, d64 = s32.nodes()["d"], s64.nodes()["d"]
+    g32, g64 = s32.state()["sig"], s64.state()["sig"]
+    rl2 = lambda x, y: float(np.linalg.norm(x - y) / np.linalg.norm(y))
+    return dict(cycles=int(s64.time().ncycle), nodal_force_rel_l2=rl2(a32, a64),
+                displacement_rel_l2=rl2(d32, d64), stress_rel_l2=rl2(g32, g64))
+
+
 def cpu_baseline(deck, cycles):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
@@ -320,6 +339,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=200, help="bricks per edge of the block")
     ap.add_argument("--cpu-cycles", type=int, default=6)
+    ap.add_argument("--err-cycles", type=int, default=1000,
+                    help="cycles after which the mixed and FP64 forms are compared")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -352,18 +373,22 @@ def main():
     print(f"deck {n}^3: {ne} bricks, {nn} nodes", file=sys.stderr, flush=True)
 
     s32, dt32, t32, el32, nd32 = run(en.MIXED, deck, args.steps, args.warmup)
-    s32.close()
     print(f"mixed: {ne * args.steps / dt32:.4e} el-cyc/s, elem {el32:.3f} ms, node {nd32:.3f} ms",
           file=sys.stderr, flush=True)
     fp64 = None
+    err = None
     if not args.no_fp64:
         s64, dt64, t64, el64, nd64 = run(en.FP64, deck, args.steps, args.warmup)
-        s64.close()
         fp64 = dict(value=ne * args.steps / dt64, ms_per_step=1e3 * dt64 / args.steps,
                     elem_kernel_ms=el64, node_kernel_ms=nd64,
                     elem_kernel_gbs=elem_bytes(ne, nn, 8) / (el64 * 1e-3) / 1e9)
         print(f"fp64: {fp64['value']:.4e} el-cyc/s, elem {el64:.3f} ms, node {nd64:.3f} ms",
               file=sys.stderr, flush=True)
+        err = mixed_vs_fp64_error(s32, s64, args.err_cycles)
+        print(f"mixed vs fp64 after {args.err_cycles} cycles: {err}", file=sys.stderr,
+              flush=True)
+        s64.close()
+    s32.close()
 
     eb = elem_bytes(ne, nn, 4)
     nb = node_bytes(ne, nn, 4)
@@ -424,6 +449,7 @@ def main():
                         "achieved_gbs": nb / (nd32 * 1e-3) / 1e9, "algorithmic_bytes": nb},
         "fp64": fp64,
         "mixed_over_fp64": (value / fp64["value"]) if fp64 else None,
+        "mixed_vs_fp64_error": err,
         "shell_plate": shell,
         "qeph_plate": qeph,
         "tube_batch": tubes,
