@@ -33,9 +33,10 @@ HBM_PEAK_GBS = 8000.0
 def elem_bytes(ne, nn, tsize):
     """Algorithmic HBM bytes of one element-kernel launch: every array element it
     must touch, once, at its stored width; nodes once per node (ideal reuse)."""
-    reads = 8 * 4 + 8 * 4 + 12 * tsize + 8   # conn, IADS, sig(6) eint rho pla qvis offg epsd, vol0
+    reads = 8 * 4 + 8 + 12 * tsize + 8       # conn, 8 corner ranks, sig(6) eint rho pla qvis
+    #                                          offg epsd, vol0
     writes = 12 * tsize + 8 * 3 * tsize      # sig(6) eint rho pla qvis epsd sti, 8 force slots
-    return ne * (reads + writes) + nn * (32 + 32)   # X4, V4 records
+    return ne * (reads + writes) + nn * (32 + 32 + 4)   # X4, V4 records, ADSKY
 
 
 def node_bytes(ne, nn, tsize):

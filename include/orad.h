@@ -36,14 +36,25 @@ typedef struct {
   int ncycle, mstop;
 } orad_time_t;
 
-/* Launch configuration of the element kernels (orad_set_launch, orad_autotune).
+/* Launch configuration of the kernels (orad_set_launch, orad_autotune).
  * persistent: batched shells only -- a resident grid takes 256-element tiles from
  * a counter instead of one workgroup per tile; lds_kb: dynamic LDS reserved per
- * element workgroup, which caps resident workgroups per CU (-1 = the form's
+ * element workgroup, which caps resident workgroups per CU (-1 = the instance's
  * default; bricks and batched shells); xcd_remap: give each of the 8 XCDs a
- * contiguous run of workgroups; grid: the persistent grid size (output). */
+ * contiguous run of workgroups; grid: the persistent grid size (output).
+ * Bricks also choose: block and waves -- the element-kernel instance (threads per
+ * workgroup, minimum waves per SIMD its registers are allocated for; mixed form
+ * 256/2 (default), 256/3, 128/2, 64/2; FP64 form 64/1 (default), 128/1, 256/1,
+ * 64/2; 0 = default); order_rows -- the library's element order, columns of that
+ * many element rows walked layer by layer (1..128; 0 = the engine's order, -1 =
+ * keep the current one).  Every family: node_tile -- PARITH/ON slots the nodal
+ * kernel stages per LDS tile (1024, 2048, 4096 within 48 KB; 0 = default).
+ * orad_get_launch reports the values in use.  Nodes, slots and element state do
+ * not depend on any of these; the element order changes only the rounding of the
+ * cumulative hourglass energy (its per-workgroup partial sums). */
 typedef struct {
   int persistent, lds_kb, xcd_remap, grid;
+  int block, waves, order_rows, node_tile;
 } orad_launch_t;
 
 /* Create/destroy a context on one HIP device.  No reference counterpart: the
@@ -222,9 +233,13 @@ int orad_model_energies(orad_ctx *ctx, int model, double *eint, double *ekin, do
  * size NVSIZ per architecture, archloops.inc:32-64).  Results do not depend on it. */
 int orad_set_launch(orad_ctx *ctx, const orad_launch_t *l);
 int orad_get_launch(orad_ctx *ctx, orad_launch_t *l);
-/* Times `cycles` whole cycles under each candidate configuration (LDS residency
- * cap, XCD mapping, persistent tiles) from the current state, restores the state
- * after every trial, and keeps the fastest; *best_ms = its time per cycle. */
+/* Times `cycles` whole cycles under each candidate configuration from the current
+ * state, restores the state after every trial, and keeps the fastest; *best_ms = its
+ * time per cycle.  Bricks: element order (engine, 8, 16, 32 rows), then element
+ * instance (block, waves) with its LDS cap, XCD mapping and the nodal slot tile,
+ * searched one axis at a time from the best so far; batched shells: LDS cap x XCD
+ * mapping x persistent tiles, then the slot tile; one shell model: XCD mapping and
+ * slot tile. */
 int orad_autotune(orad_ctx *ctx, int cycles, orad_launch_t *best, double *best_ms);
 
 /* Adjoint of a run of one shell model (no reference counterpart: the reference's

@@ -28,6 +28,7 @@ SOURCES = [
     ("orad_qeph_mixed.hip", ["-ffp-contract=fast"]),
     ("orad_nodal.hip", ["-ffp-contract=off"]),
     ("orad_adjoint.hip", ["-ffp-contract=off"]),
+    ("orad_brick_order.hip", []),
     ("orad_api.hip", []),
 ]
 

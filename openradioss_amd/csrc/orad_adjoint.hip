@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void adj_node_kernel(int numnod, const double4
                                                        double *g_in) {
   const int n = blockIdx.x * 256 + threadIdx.x;
   if (n >= numnod) return;
-  const int code = (int)V4[n].w;
+  const int code = word_code(V4[n].w);
   const int ct = code_icodt(code), cr = code_icodr(code), fd = code_fvdir(code);
   const double ms = X4[n].w, in = VR4[n].w;
   double gm = 0.0, gi = 0.0;

@@ -59,7 +59,9 @@ static BrickArgs<T> brick_args(orad_ctx *c) {
   T *base = (T *)c->state;
   a.numels = c->numels;
   a.conn = c->conn;
-  a.iads = c->iads;
+  a.rank = c->rank;
+  a.emap = nullptr;
+  a.adsky = c->adsky;
   a.X4 = c->X4;
   a.V4 = c->V4;
   a.sig = base;
@@ -141,8 +143,11 @@ static ShellArgs<T> shell_args(orad_ctx *c) {
   return a;
 }
 
+// [e0, e1): the library's element order, or with `group` one engine-numbered group
+// (bricks map it through emap; shells keep the engine's order).
 static hipError_t launch_elements(orad_ctx *c, int e0, int e1, const double *dt1p,
-                                  unsigned long long *dt2t, double *ehou_part) {
+                                  unsigned long long *dt2t, double *ehou_part,
+                                  bool group = false) {
   if (c->family == FAM_SHELL) {
     if (c->qeph()) {
       if (c->precision == ORAD_MIXED)
@@ -157,11 +162,16 @@ static hipError_t launch_elements(orad_ctx *c, int e0, int e1, const double *dt1
     return launch_shell_fp64(shell_args<double>(c), e0, e1, dt1p, dt2t, ehou_part, c->xcd_remap,
                              c->stream);
   }
-  if (c->precision == ORAD_MIXED)
-    return launch_brick_mixed(brick_args<float>(c), e0, e1, dt1p, dt2t, ehou_part, c->xcd_remap,
-                              c->elem_lds, c->stream);
-  return launch_brick_fp64(brick_args<double>(c), e0, e1, dt1p, dt2t, ehou_part, c->xcd_remap,
-                           c->elem_lds, c->stream);
+  if (c->precision == ORAD_MIXED) {
+    BrickArgs<float> a = brick_args<float>(c);
+    if (group) a.emap = c->emap;
+    return launch_brick_mixed(a, e0, e1, dt1p, dt2t, ehou_part, c->xcd_remap, c->elem_lds,
+                              c->elem_block, c->elem_waves, c->stream);
+  }
+  BrickArgs<double> a = brick_args<double>(c);
+  if (group) a.emap = c->emap;
+  return launch_brick_fp64(a, e0, e1, dt1p, dt2t, ehou_part, c->xcd_remap, c->elem_lds,
+                           c->elem_block, c->elem_waves, c->stream);
 }
 
 // One cycle's launches at parity p (no host synchronisation, graph-capturable).
@@ -191,7 +201,7 @@ static hipError_t enqueue_cycle(orad_ctx *c, int p) {
   hipError_t e = launch_cycle_elements(c, p);
   if (e != hipSuccess) return e;
   return launch_node(node_args(c), c->precision == ORAD_MIXED, p, 1, c->cycle_write_a,
-                     c->xcd_remap, c->stream);
+                     c->xcd_remap, c->stream, c->node_tile);
 }
 
 static int init_state(orad_ctx *c) {
@@ -216,6 +226,24 @@ static int init_state(orad_ctx *c) {
 
 static int get_fsky_shell(orad_ctx *c, double *fsky);
 static void free_adjoint(orad_ctx *c);
+
+// Re-packs every node word (V4.w): ICODT kept, ICODR and the FIXVEL direction from
+// h_code_hi.
+static int pack_node_words(orad_ctx *c) {
+  if (!c->have_nodes) return ORAD_OK;
+  const size_t n = (size_t)c->numnod;
+  std::vector<double4> hv(n);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(hv.data(), c->V4, n * sizeof(double4), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) {
+    const int code = word_code(hv[i].w);
+    const int hi = c->h_code_hi.empty() ? 0 : c->h_code_hi[i];
+    hv[i].w = node_word(code_icodt(code) | hi);
+  }
+  HIPCHK(c, hipMemcpy(c->V4, hv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+  return ORAD_OK;
+}
 
 extern "C" {
 
@@ -270,7 +298,8 @@ int orad_destroy(orad_ctx *c) {
                   (void *)c->scratch_dt2t, (void *)c->VR4, (void *)c->AR, (void *)c->STIFR,
                   (void *)c->thke, (void *)c->fv_val, (void *)c->node_model,
                   (void *)c->model_node0, (void *)c->model_tiles, (void *)c->tiles,
-                  (void *)c->mats, (void *)c->queue, (void *)c->chunk_ehou})
+                  (void *)c->mats, (void *)c->queue, (void *)c->chunk_ehou, (void *)c->rank,
+                  (void *)c->emap})
     dfree(p);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -295,7 +324,7 @@ int orad_set_nodes(orad_ctx *c, int numnod, const double *x, const double *v, co
     const int code = icodt ? icodt[i] : 0;
     if (code < 0 || code > 7) return fail(c, ORAD_EINVAL, "set_nodes: ICODT outside 0..7");
     const int hi = c->h_code_hi.empty() ? 0 : c->h_code_hi[i];
-    hv[i] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], (double)(code | hi));
+    hv[i] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], node_word(code | hi));
   }
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -353,18 +382,28 @@ int orad_set_bricks(orad_ctx *c, int numels, const int *ixs, const double *vol0)
     const int32_t e = order[j];
     for (int k = 0; k < 8; ++k) c->h_iads[8 * (size_t)e + k] = cnt[conn_em[8 * (size_t)e + k]]++;
   }
-  // Device copies, corner-major for coalesced loads.
-  std::vector<int32_t> conn_cm(8 * ne), iads_cm(8 * ne);
+  // Corner ranks: slot of (e,k) = ADSKY(node) + rank, rank < 256 (8 bits per corner).
+  std::vector<uint64_t> rank(ne, 0);
   for (size_t e = 0; e < ne; ++e)
     for (int k = 0; k < 8; ++k) {
-      conn_cm[k * ne + e] = conn_em[8 * e + k];
-      iads_cm[k * ne + e] = c->h_iads[8 * e + k];
+      const int32_t r = c->h_iads[8 * e + k] - c->h_adsky[conn_em[8 * e + k]];
+      if (r < 0 || r > 255) {
+        c->h_adsky.clear();
+        c->h_iads.clear();
+        return fail(c, ORAD_EINVAL, "set_bricks: a node is shared by more than 256 brick corners");
+      }
+      rank[e] |= (uint64_t)r << (8 * k);
     }
+  // Device copies in the engine's order, corner-major for coalesced loads; the
+  // library's column order is applied below.
+  std::vector<int32_t> conn_cm(8 * ne);
+  for (size_t e = 0; e < ne; ++e)
+    for (int k = 0; k < 8; ++k) conn_cm[k * ne + e] = conn_em[8 * e + k];
   HIPCHK(c, dalloc(&c->conn, 8 * ne * sizeof(int32_t)));
-  HIPCHK(c, dalloc(&c->iads, 8 * ne * sizeof(int32_t)));
+  HIPCHK(c, dalloc(&c->rank, ne * sizeof(uint64_t)));
   HIPCHK(c, dalloc(&c->adsky, (nn + 1) * sizeof(int32_t)));
   HIPCHK(c, hipMemcpy(c->conn, conn_cm.data(), 8 * ne * 4, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(c->iads, iads_cm.data(), 8 * ne * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->rank, rank.data(), ne * 8, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->adsky, c->h_adsky.data(), (nn + 1) * 4, hipMemcpyHostToDevice));
   const size_t ts = c->tsize();
   HIPCHK(c, dalloc(&c->state, NSTATE * ne * ts));
@@ -380,10 +419,11 @@ int orad_set_bricks(orad_ctx *c, int numels, const int *ixs, const double *vol0)
     HIPCHK(c, dalloc(&c->slot_elem, 8 * ne * sizeof(int32_t)));
     HIPCHK(c, hipMemcpy(c->slot_elem, se.data(), 8 * ne * 4, hipMemcpyHostToDevice));
   }
+  // hourglass-energy partials: one per element workgroup, room for the smallest (64)
   const int bb = c->precision == ORAD_MIXED ? brick_block_mixed() : brick_block_fp64();
   c->n_ehou_part = (int)((ne + bb - 1) / bb);
-  HIPCHK(c, dalloc(&c->ehou_part, (size_t)c->n_ehou_part * sizeof(double)));
-  HIPCHK(c, hipMemset(c->ehou_part, 0, (size_t)c->n_ehou_part * sizeof(double)));
+  HIPCHK(c, dalloc(&c->ehou_part, ((ne + 63) / 64) * sizeof(double)));
+  HIPCHK(c, hipMemset(c->ehou_part, 0, ((ne + 63) / 64) * sizeof(double)));
   HIPCHK(c, dalloc(&c->vol0, ne * sizeof(double)));
   HIPCHK(c, hipMemcpy(c->vol0, vol0, ne * sizeof(double), hipMemcpyHostToDevice));
   c->numels = numels;
@@ -391,7 +431,10 @@ int orad_set_bricks(orad_ctx *c, int numels, const int *ixs, const double *vol0)
   c->family = FAM_BRICK;
   c->nc = 8;
   drop_graphs(c);
-  return init_state(c);
+  int r = brick_bins(c, ne, conn_em, vol0);
+  if (!r) r = brick_reorder(c, c->order_rows, {});
+  if (!r) r = init_state(c);
+  return r;
 }
 
 int orad_set_law2(orad_ctx *c, const double *p, const int *ip) {
@@ -487,7 +530,7 @@ int orad_sforc3(orad_ctx *c, int nft, int nel, double dt1, double *dt2t) {
   unsigned long long bits = 0x7FF0000000000000ull;
   HIPCHK(c, hipMemcpyAsync(c->scratch_dt1, &dt1, sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->scratch_dt2t, &bits, sizeof(bits), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, launch_elements(c, nft, nft + nel, c->scratch_dt1, c->scratch_dt2t, nullptr));
+  HIPCHK(c, launch_elements(c, nft, nft + nel, c->scratch_dt1, c->scratch_dt2t, nullptr, true));
   HIPCHK(c, hipMemcpyAsync(&bits, c->scratch_dt2t, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double g;
@@ -504,7 +547,7 @@ int orad_asspar4(orad_ctx *c, double *a, double *stifn) {
   if (r) return r;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, launch_node(node_args(c), c->precision == ORAD_MIXED, c->host_ncycle & 1, 0, 1,
-                        c->xcd_remap, c->stream));
+                        c->xcd_remap, c->stream, c->node_tile));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (a) HIPCHK(c, hipMemcpy(a, c->A, 3 * (size_t)c->numnod * 8, hipMemcpyDeviceToHost));
   if (stifn) HIPCHK(c, hipMemcpy(stifn, c->STIFN, (size_t)c->numnod * 8, hipMemcpyDeviceToHost));
@@ -568,7 +611,8 @@ int orad_profile_cycles(orad_ctx *c, int ncycle, double *elem_ms, double *node_m
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     HIPCHK(c, launch_cycle_elements(c, p));
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(c, launch_node(node_args(c), c->precision == ORAD_MIXED, p, 1, 0, c->xcd_remap, c->stream));
+    HIPCHK(c, launch_node(node_args(c), c->precision == ORAD_MIXED, p, 1, 0, c->xcd_remap, c->stream,
+                          c->node_tile));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev[2]));
     float a = 0.f, b = 0.f;
@@ -615,9 +659,11 @@ int orad_get_brick_state(orad_ctx *c, double *sig, double *eint, double *rho, do
   const size_t ne = (size_t)c->numels, ts = c->tsize();
   std::vector<char> h(NSTATE * ne * ts);
   HIPCHK(c, hipMemcpy(h.data(), c->state, h.size(), hipMemcpyDeviceToHost));
+  // e: engine element; its column entry is at its library position
   auto col = [&](int k, size_t e) -> double {
-    return c->precision == ORAD_MIXED ? (double)((float *)h.data())[k * ne + e]
-                                      : ((double *)h.data())[k * ne + e];
+    const size_t i = c->h_iperm.empty() ? e : (size_t)c->h_iperm[e];
+    return c->precision == ORAD_MIXED ? (double)((float *)h.data())[k * ne + i]
+                                      : ((double *)h.data())[k * ne + i];
   };
   for (size_t e = 0; e < ne; ++e) {
     if (sig)
@@ -642,8 +688,9 @@ int orad_set_brick_state(orad_ctx *c, const double *sig, const double *eint, con
   std::vector<char> h(NSTATE * ne * ts);
   HIPCHK(c, hipMemcpy(h.data(), c->state, h.size(), hipMemcpyDeviceToHost));
   auto put = [&](int k, size_t e, double v) {
-    if (c->precision == ORAD_MIXED) ((float *)h.data())[k * ne + e] = (float)v;
-    else ((double *)h.data())[k * ne + e] = v;
+    const size_t i = c->h_iperm.empty() ? e : (size_t)c->h_iperm[e];
+    if (c->precision == ORAD_MIXED) ((float *)h.data())[k * ne + i] = (float)v;
+    else ((double *)h.data())[k * ne + i] = v;
   };
   for (size_t e = 0; e < ne; ++e) {
     if (sig)
@@ -677,14 +724,16 @@ int orad_get_fsky(orad_ctx *c, double *fsky) {
     HIPCHK(c, hipMemcpy(f.data(), c->fsky, 3 * ns * 8, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(st.data(), c->sti, ne * 8, hipMemcpyDeviceToHost));
   }
-  for (size_t e = 0; e < ne; ++e)
+  for (size_t e = 0; e < ne; ++e) {
+    const double sti = st[c->h_iperm.empty() ? e : (size_t)c->h_iperm[e]];
     for (int k = 0; k < 8; ++k) {
       const size_t sl = (size_t)c->h_iads[8 * e + k];
       fsky[4 * sl] = f[3 * sl];
       fsky[4 * sl + 1] = f[3 * sl + 1];
       fsky[4 * sl + 2] = f[3 * sl + 2];
-      fsky[4 * sl + 3] = st[e];
+      fsky[4 * sl + 3] = sti;
     }
+  }
   return ORAD_OK;
 }
 
@@ -709,7 +758,8 @@ static int energies_range(orad_ctx *c, size_t e0, size_t e1, size_t n0, size_t n
   hipError_t e = launch_energy(c->precision == ORAD_MIXED, (int)me,
                                shell ? st + (SC_EINT * ne + e0) * ts : st + (C_EINT * ne + e0) * ts,
                                shell ? st + ((SC_EINT + 1) * ne + e0) * ts : nullptr,
-                               shell ? nullptr : c->vol0 + e0, (int)mn, c->X4 + n0, c->V4 + n0,
+                               shell ? nullptr : c->vol0 + e0, shell ? nullptr : c->emap,
+                               (int)mn, c->X4 + n0, c->V4 + n0,
                                shell ? c->VR4 + n0 : nullptr, part, part + nbe + nbk, c->stream);
   double res[2] = {0, 0}, eh = 0;
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -831,19 +881,7 @@ static void shell_tables(int npt, ShProp &p) {
 }
 
 static int upload_codes(orad_ctx *c) {
-  // Re-pack V4.w after a change of ICODR or FIXVEL: keep ICODT, replace the high bits.
-  if (!c->have_nodes) return ORAD_OK;
-  const size_t n = (size_t)c->numnod;
-  std::vector<double4> hv(n);
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, hipMemcpy(hv.data(), c->V4, n * sizeof(double4), hipMemcpyDeviceToHost));
-  for (size_t i = 0; i < n; ++i) {
-    const int code = (int)hv[i].w;
-    const int hi = c->h_code_hi.empty() ? 0 : c->h_code_hi[i];
-    hv[i].w = (double)(code_icodt(code) | hi);
-  }
-  HIPCHK(c, hipMemcpy(c->V4, hv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
-  return ORAD_OK;
+  return pack_node_words(c);
 }
 
 static int init_shell_state(orad_ctx *c) {
@@ -1111,7 +1149,7 @@ int orad_asspar4_rot(orad_ctx *c, double *a, double *ar, double *stifn, double *
   if (r) return r;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, launch_node(node_args(c), c->precision == ORAD_MIXED, c->host_ncycle & 1, 0, 1,
-                        c->xcd_remap, c->stream));
+                        c->xcd_remap, c->stream, c->node_tile));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const size_t n = (size_t)c->numnod;
   if (a) HIPCHK(c, hipMemcpy(a, c->A, 3 * n * 8, hipMemcpyDeviceToHost));
@@ -1291,10 +1329,19 @@ int orad_model_energies(orad_ctx *c, int model, double *eint, double *ekin, doub
 
 int orad_get_launch(orad_ctx *c, orad_launch_t *l) {
   if (!c || !l) return ORAD_EINVAL;
+  const bool mixed = c->precision == ORAD_MIXED;
   l->persistent = c->persist;
   l->lds_kb = c->batched ? c->batch_lds / 1024 : (c->elem_lds < 0 ? -1 : c->elem_lds / 1024);
   l->xcd_remap = c->xcd_remap;
   l->grid = c->batched && c->persist ? std::min(c->batch_grid, c->ntiles) : 0;
+  l->block = l->waves = 0;
+  l->order_rows = 0;
+  if (c->family == FAM_BRICK || c->family == FAM_NONE) {
+    l->block = c->elem_block > 0 ? c->elem_block : (mixed ? brick_block_mixed() : brick_block_fp64());
+    l->waves = c->elem_waves > 0 ? c->elem_waves : (mixed ? brick_waves_mixed() : brick_waves_fp64());
+    l->order_rows = c->order_rows;
+  }
+  l->node_tile = c->node_tile > 0 ? c->node_tile : node_tile_default(mixed);
   return ORAD_OK;
 }
 
@@ -1302,10 +1349,40 @@ int orad_set_launch(orad_ctx *c, const orad_launch_t *l) {
   if (!c || !l) return ORAD_EINVAL;
   if (l->lds_kb > 160 || l->lds_kb < -1 || (l->persistent && !c->batched))
     return fail(c, ORAD_EINVAL, "set_launch: LDS in -1..160 KB; persistent needs batched models");
+  const bool mixed = c->precision == ORAD_MIXED;
+  const bool brick = c->family == FAM_BRICK || c->family == FAM_NONE;
+  if (brick && (l->block || l->waves)) {
+    const int b = l->block ? l->block : (mixed ? brick_block_mixed() : brick_block_fp64());
+    const int w = l->waves ? l->waves : (mixed ? brick_waves_mixed() : brick_waves_fp64());
+    if (!(mixed ? brick_shape_ok_mixed(b, w) : brick_shape_ok_fp64(b, w)))
+      return fail(c, ORAD_EINVAL, "set_launch: no element-kernel instance of this block and waves");
+  }
+  if (!brick && (l->block || l->waves))
+    return fail(c, ORAD_EINVAL, "set_launch: block and waves are chosen for bricks only");
+  if (l->order_rows < -1 || l->order_rows > 128 || (!brick && l->order_rows > 0))
+    return fail(c, ORAD_EINVAL, "set_launch: order_rows in -1..128, bricks only");
+  if (l->node_tile && !node_tile_ok(mixed, c->family == FAM_SHELL, l->node_tile))
+    return fail(c, ORAD_EINVAL, "set_launch: node_tile 1024, 2048 or 4096 slots within 48 KB");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const int keep_p = c->persist, keep_b = c->batch_lds, keep_e = c->elem_lds;
   c->xcd_remap = l->xcd_remap != 0;
+  c->node_tile = l->node_tile;
+  if (brick) {
+    c->elem_block = l->block;
+    c->elem_waves = l->waves;
+    if (c->have_bricks) {
+      const int b = c->elem_block ? c->elem_block : (mixed ? brick_block_mixed() : brick_block_fp64());
+      c->n_ehou_part = (c->numels + b - 1) / b;
+    }
+    if (l->order_rows >= 0) {
+      if (c->have_bricks && l->order_rows != c->order_rows) {
+        const int r = brick_reorder(c, l->order_rows, {});
+        if (r) return r;
+      }
+      c->order_rows = l->order_rows;
+    }
+  }
   if (c->batched) {
     c->persist = l->persistent != 0;
     c->batch_lds = l->lds_kb > 0 ? 1024 * l->lds_kb : 0;
@@ -1326,21 +1403,26 @@ int orad_set_launch(orad_ctx *c, const orad_launch_t *l) {
 
 }  // extern "C"
 
-// Device state a cycle changes, saved and restored around autotuner trials.
+// Device state a cycle changes, saved and restored around autotuner trials.  The
+// live buffers are looked up again at every restore: the brick order swaps them.
+static std::vector<std::pair<void *, size_t>> cycle_buffers(orad_ctx *c) {
+  const size_t nn = (size_t)c->numnod, ne = (size_t)nelem(c);
+  const size_t ncol = c->family == FAM_SHELL ? (size_t)c->shncol() : NSTATE;
+  const size_t nsti = c->family == FAM_SHELL ? (size_t)c->nsti() : 1;
+  std::vector<std::pair<void *, size_t>> live = {
+      {c->X4, nn * 32}, {c->V4, nn * 32}, {c->D, nn * 24}, {c->state, ncol * ne * c->tsize()},
+      {c->sti, nsti * ne * c->tsize()}, {c->ctl, c->ctl_bytes()}};
+  if (c->family == FAM_SHELL) live.push_back({c->VR4, nn * 32});
+  return live;
+}
+
 struct Snapshot {
-  std::vector<std::pair<void *, void *>> bufs;  // (live, copy)
+  std::vector<void *> copies;   // in cycle_buffers order
   std::vector<size_t> sizes;
   int host_ncycle = 0;
   hipError_t take(orad_ctx *c) {
     host_ncycle = c->host_ncycle;
-    const size_t nn = (size_t)c->numnod, ne = (size_t)nelem(c);
-    const size_t ncol = c->family == FAM_SHELL ? (size_t)c->shncol() : NSTATE;
-    const size_t nsti = c->family == FAM_SHELL ? (size_t)c->nsti() : 1;
-    std::vector<std::pair<void *, size_t>> live = {
-        {c->X4, nn * 32}, {c->V4, nn * 32}, {c->D, nn * 24}, {c->state, ncol * ne * c->tsize()},
-        {c->sti, nsti * ne * c->tsize()}, {c->ctl, c->ctl_bytes()}};
-    if (c->family == FAM_SHELL) live.push_back({c->VR4, nn * 32});
-    for (auto &b : live) {
+    for (auto &b : cycle_buffers(c)) {
       void *p = nullptr;
       hipError_t e = dalloc(&p, b.second);
       if (e == hipSuccess)
@@ -1349,25 +1431,48 @@ struct Snapshot {
         dfree(p);
         return e;
       }
-      bufs.push_back({b.first, p});
+      copies.push_back(p);
       sizes.push_back(b.second);
     }
     return hipStreamSynchronize(c->stream);
   }
   // copies on the context's stream, ordered with its kernels
   hipError_t restore(orad_ctx *c) {
-    for (size_t i = 0; i < bufs.size(); ++i) {
-      hipError_t e = hipMemcpyAsync(bufs[i].first, bufs[i].second, sizes[i],
-                                    hipMemcpyDeviceToDevice, c->stream);
+    const auto live = cycle_buffers(c);
+    for (size_t i = 0; i < copies.size(); ++i) {
+      hipError_t e = hipMemcpyAsync(live[i].first, copies[i], sizes[i], hipMemcpyDeviceToDevice,
+                                    c->stream);
       if (e != hipSuccess) return e;
     }
     c->host_ncycle = host_ncycle;
     return hipStreamSynchronize(c->stream);
   }
+  void *state_copy() const { return copies.size() > 3 ? copies[3] : nullptr; }
+  void *sti_copy() const { return copies.size() > 4 ? copies[4] : nullptr; }
   ~Snapshot() {
-    for (auto &b : bufs) dfree(b.second);
+    for (void *p : copies) dfree(p);
   }
 };
+
+// Time per cycle of `cycles` graph-replayed cycles from the snapshot under the current
+// configuration; +inf when a cycle reports a non-positive volume.
+static int time_trial(orad_ctx *c, Snapshot &snap, int cycles, double *per) {
+  *per = 1e300;
+  HIPCHK(c, snap.restore(c));
+  int r = orad_run_cycles(c, 2, nullptr);                     // captures the graphs
+  if (r == ORAD_OK) {
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    r = orad_run_cycles(c, cycles, nullptr);
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[1]));
+  }
+  if (r == ORAD_ENEGVOL) return ORAD_OK;
+  if (r != ORAD_OK) return r;
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  *per = ms / cycles;
+  return ORAD_OK;
+}
 
 extern "C" {
 
@@ -1379,43 +1484,88 @@ int orad_autotune(orad_ctx *c, int cycles, orad_launch_t *best, double *best_ms)
   cycles += cycles & 1;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  std::vector<orad_launch_t> cand;
-  auto add = [&](int pers, int lds, int xcd) { cand.push_back(orad_launch_t{pers, lds, xcd, 0}); };
-  if (c->family == FAM_BRICK) {
-    for (int lds : {-1, 0, 40, 54, 80})
-      for (int xcd : {0, 1}) add(0, lds, xcd);
-  } else if (c->batched) {
-    for (int lds : {0, 40, 80}) {
-      for (int xcd : {0, 1}) add(0, lds, xcd);
-      add(1, lds, 0);
-    }
-  } else {
-    for (int xcd : {0, 1}) add(0, -1, xcd);
-  }
   orad_launch_t keep{};
   orad_get_launch(c, &keep);
   Snapshot snap;
   HIPCHK(c, snap.take(c));
   double tbest = 1e300;
   orad_launch_t lbest = keep;
-  for (const orad_launch_t &l : cand) {
-    if (orad_set_launch(c, &l) != ORAD_OK) continue;      // does not fit: skip
-    HIPCHK(c, snap.restore(c));
-    r = orad_run_cycles(c, 2, nullptr);                     // captures the graphs
-    if (r == ORAD_OK) {
-      HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-      r = orad_run_cycles(c, cycles, nullptr);
-      HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-      HIPCHK(c, hipEventSynchronize(c->ev[1]));
-    }
-    if (r == ORAD_ENEGVOL) continue;
-    if (r != ORAD_OK) return r;
-    float ms = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    const double per = ms / cycles;
+  lbest.order_rows = -1;
+  // one candidate: `l` applied, timed, kept when faster
+  auto trial = [&](orad_launch_t l) -> int {
+    if (orad_set_launch(c, &l) != ORAD_OK) return ORAD_OK;        // does not fit: skip
+    double per = 1e300;
+    const int rr = time_trial(c, snap, cycles, &per);
+    if (rr) return rr;
     if (per < tbest) {
       tbest = per;
       lbest = l;
+    }
+    return ORAD_OK;
+  };
+  const bool mixed = c->precision == ORAD_MIXED;
+  if (c->family == FAM_BRICK) {
+    // 1. element order: the state in the snapshot is permuted along with the live one
+    int best_rows = c->order_rows;
+    for (int rows : {0, 8, 16, 32}) {
+      r = brick_reorder(c, rows, {{snap.state_copy(), snap.sti_copy()}});
+      if (r) return r;
+      orad_launch_t l = lbest;
+      l.order_rows = -1;
+      const double before = tbest;
+      r = trial(l);
+      if (r) return r;
+      if (tbest < before) best_rows = rows;
+    }
+    r = brick_reorder(c, best_rows, {{snap.state_copy(), snap.sti_copy()}});
+    if (r) return r;
+    // 2. element-kernel instance, 3. LDS cap, 4. XCD mapping, 5. nodal slot tile
+    const int shapes[][2] = {{256, 2}, {256, 3}, {128, 2}, {64, 2}, {64, 1}, {128, 1}, {256, 1}};
+    orad_launch_t base = lbest;
+    for (const auto &sh : shapes) {
+      if (!(mixed ? brick_shape_ok_mixed(sh[0], sh[1]) : brick_shape_ok_fp64(sh[0], sh[1])))
+        continue;
+      orad_launch_t l = base;
+      l.block = sh[0]; l.waves = sh[1]; l.lds_kb = -1;
+      if ((r = trial(l))) return r;
+    }
+    base = lbest;
+    for (int lds : {0, 40, 54, 80}) {
+      orad_launch_t l = base;
+      l.lds_kb = lds;
+      if ((r = trial(l))) return r;
+    }
+    base = lbest;
+    {
+      orad_launch_t l = base;
+      l.xcd_remap = !base.xcd_remap;
+      if ((r = trial(l))) return r;
+    }
+  } else if (c->batched) {
+    for (int lds : {0, 40, 80}) {
+      for (int xcd : {0, 1}) {
+        orad_launch_t l = keep;
+        l.persistent = 0; l.lds_kb = lds; l.xcd_remap = xcd;
+        if ((r = trial(l))) return r;
+      }
+      orad_launch_t l = keep;
+      l.persistent = 1; l.lds_kb = lds; l.xcd_remap = 0;
+      if ((r = trial(l))) return r;
+    }
+  } else {
+    for (int xcd : {0, 1}) {
+      orad_launch_t l = keep;
+      l.xcd_remap = xcd;
+      if ((r = trial(l))) return r;
+    }
+  }
+  {
+    const orad_launch_t base = lbest;
+    for (int tile : {1024, 2048, 4096}) {
+      if (!node_tile_ok(mixed, c->family == FAM_SHELL, tile) || tile == base.node_tile) continue;
+      orad_launch_t l = base;
+      l.node_tile = tile;
+      if ((r = trial(l))) return r;
     }
   }
   HIPCHK(c, snap.restore(c));

@@ -2,34 +2,35 @@
 // arithmetic follows the reference's double-precision build operation by operation.
 #include "orad_brick_kernel.hpp"
 
-// Launch shape (measured on the 200^3 deck): MINW = minimum waves per SIMD asked of
-// the register allocator, BLOCK = threads per workgroup, LDS_KB = dynamic LDS
-// reserved per workgroup, which caps the groups resident on a CU (160 KB of LDS):
-// fewer groups in flight keep more of the shared corner-node records in cache.
-#ifndef ORAD_MINW
-#define ORAD_MINW 1
-#endif
-#ifndef ORAD_BLOCK
-#define ORAD_BLOCK 64
-#endif
-#ifndef ORAD_LDS_KB
-#define ORAD_LDS_KB 0
-#endif
-
+// Instances (threads per workgroup, minimum waves per SIMD) the launch configuration
+// and the autotuner choose from.  The default, 64 threads at one wave per SIMD and no
+// LDS reservation, was the fastest measured on the 200^3 deck.
 namespace orad {
 
-int brick_block_fp64() { return ORAD_BLOCK; }
+int brick_block_fp64() { return 64; }
+int brick_waves_fp64() { return 1; }
+
+bool brick_shape_ok_fp64(int block, int waves) {
+  return (waves == 1 && (block == 64 || block == 128 || block == 256)) ||
+         (block == 64 && waves == 2);
+}
 
 hipError_t launch_brick_fp64(const BrickArgs<double> &a, int e_begin, int e_end,
                              const double *dt1p, unsigned long long *dt2t, double *ehou_part,
-                             int xcd_remap, int dyn_lds, hipStream_t s) {
-  const int n = e_end - e_begin;
-  if (n <= 0) return hipSuccess;
-  const int nb = (n + ORAD_BLOCK - 1) / ORAD_BLOCK;
-  const unsigned lds = dyn_lds < 0 ? 1024u * ORAD_LDS_KB : (unsigned)dyn_lds;
-  hipLaunchKernelGGL((brick_forces_kernel<double, false, ORAD_MINW, ORAD_BLOCK>), dim3(nb),
-                     dim3(ORAD_BLOCK), lds, s, a, e_begin, e_end, dt1p, dt2t, ehou_part, xcd_remap);
-  return hipGetLastError();
+                             int xcd_remap, int dyn_lds, int block, int waves, hipStream_t s) {
+  if (block <= 0) block = brick_block_fp64();
+  if (waves <= 0) waves = brick_waves_fp64();
+  const unsigned lds = dyn_lds < 0 ? 0u : (unsigned)dyn_lds;
+#define ORAD_CASE(B, W)                                                                       \
+  if (block == B && waves == W)                                                               \
+    return launch_brick_instance<double, false, W, B>(a, e_begin, e_end, dt1p, dt2t, ehou_part, \
+                                                      xcd_remap, lds, s);
+  ORAD_CASE(64, 1)
+  ORAD_CASE(128, 1)
+  ORAD_CASE(256, 1)
+  ORAD_CASE(64, 2)
+#undef ORAD_CASE
+  return hipErrorInvalidConfiguration;
 }
 
 }  // namespace orad

@@ -30,17 +30,20 @@ struct BrickIO {
 
   __device__ __forceinline__ BrickIO(const BrickArgs<T> &args, int el) : a(args), e(el), ne(args.numels) {}
   __device__ __forceinline__ int conn(int k) const { return ldnt(&a.conn[(size_t)k * ne + e]); }
-  __device__ __forceinline__ int iads(int k) const { return ldnt(&a.iads[(size_t)k * ne + e]); }
+  // the eight corner ranks, byte k for corner k
+  __device__ __forceinline__ uint64_t ranks() const { return ldnt(&a.rank[e]); }
   __device__ __forceinline__ D3 x(int n) const { return rec(&a.X4[n]); }
   __device__ __forceinline__ D3 v(int n) const { return rec(&a.V4[n]); }
+  // first PARITH/ON slot of node n (ADSKY; a small array that stays in cache)
+  __device__ __forceinline__ int slot0(int n) const { return a.adsky[n]; }
   __device__ __forceinline__ T sig(int c) const { return ldnt(&a.sig[(size_t)c * ne + e]); }
   __device__ __forceinline__ void set_sig(int c, T s) const { stnt(&a.sig[(size_t)c * ne + e], s); }
   __device__ __forceinline__ T get(int f) const { return ldnt(&field_ptr(a, f)[e]); }
   __device__ __forceinline__ void put(int f, T s) const { stnt(&field_ptr(a, f)[e], s); }
   __device__ __forceinline__ double vol0() const { return ldnt(&a.vol0[e]); }
   __device__ __forceinline__ void set_sti(T s) const { stnt(&a.sti[e], s); }
-  // slot store at a slot address loaded beforehand (iads(k)), so the address load's
-  // latency overlaps the corner gathers instead of following the chain
+  // slot store at a slot address formed beforehand (the node's first slot plus the
+  // corner's rank), so no address load follows the chain
   __device__ __forceinline__ void slot_at(int sl, T fx, T fy, T fz) const {
     Slot3<T> f;
     f.x = fx; f.y = fy; f.z = fz;

@@ -3,7 +3,8 @@
 // Per element it gathers the eight corners' X and V (FP64, 32-byte records),
 // runs the SFORC3 chain (orad_brick_chain.hpp), writes the element state back
 // (structure of arrays, coalesced, non-temporal), STI once per element, and the
-// eight PARITH/ON corner force slots at IADS (SCUMU3P).  The group's stable time step is a wavefront shuffle min,
+// eight PARITH/ON corner force slots (SCUMU3P): corner k of the element writes slot
+// ADSKY(node) + rank_k (an 8-bit rank per corner in place of a 32-bit IADS).  The group's stable time step is a wavefront shuffle min,
 // then one 64-bit atomicMin per workgroup on the bits of a positive double
 // (order independent, so deterministic).  Hourglass energy is summed per
 // workgroup in a fixed tree order into ehou_part[block].
@@ -55,16 +56,22 @@ void brick_forces_kernel(
   __shared__ double s_eh[BLOCK / 64];
   const int nb = gridDim.x;
   const int b = xcd_remap ? xcd_block(blockIdx.x, nb) : (int)blockIdx.x;
-  const int e = e_begin + b * BLOCK + (int)threadIdx.x;
+  // t runs over [e_begin, e_end) in the library's element order, or in the engine's
+  // when emap is set (one engine group, SFORC3's NFT+1..NFT+NEL)
+  const int t = e_begin + b * BLOCK + (int)threadIdx.x;
   const T dt1 = T(*dt1p);
   double my_dt = 1.0e300, my_eh = 0.0;
-  if (e < e_end) {
+  if (t < e_end) {
+    const int e = a.emap ? a.emap[t] : t;
     const BrickIO<T> io(a, e);
     int nd[8], sl[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) nd[k] = io.conn(k);
+    // slot of corner k: the node's first slot plus the corner's rank, formed with
+    // the corner gathers so no address load follows the chain
+    const uint64_t rk = io.ranks();
 #pragma unroll
-    for (int k = 0; k < 8; ++k) sl[k] = io.iads(k);
+    for (int k = 0; k < 8; ++k) sl[k] = io.slot0(nd[k]) + (int)((rk >> (8 * k)) & 0xFF);
     T xd[8], yd[8], zd[8], vx[8], vy[8], vz[8];
     double voldp = 0.0;
     if (REL) {
@@ -127,6 +134,33 @@ void brick_forces_kernel(
     if (d < 1.0e300) atomicMin(dt2t_out, (unsigned long long)__double_as_longlong(d));
     if (ehou_part) ehou_part[b] = h;
   }
+}
+
+}  // namespace orad
+
+namespace orad {
+
+// One instance of the element kernel: W = minimum waves per SIMD asked of the
+// register allocator, B = threads per workgroup; `lds` = dynamic LDS per workgroup,
+// which caps the workgroups resident on a CU (160 KB of LDS per CU).
+template <typename T, bool REL, int W, int B>
+static hipError_t launch_brick_instance(const BrickArgs<T> &a, int e_begin, int e_end,
+                                        const double *dt1p, unsigned long long *dt2t,
+                                        double *ehou_part, int xcd_remap, unsigned lds,
+                                        hipStream_t s) {
+  const int n = e_end - e_begin;
+  if (n <= 0) return hipSuccess;
+  const int nb = (n + B - 1) / B;
+  hipLaunchKernelGGL((brick_forces_kernel<T, REL, W, B>), dim3(nb), dim3(B), lds, s, a,
+                     e_begin, e_end, dt1p, dt2t, ehou_part, xcd_remap);
+  return hipGetLastError();
+}
+
+// Dynamic LDS that holds a CU to `waves` resident waves per SIMD for workgroups of
+// `block` threads: (waves * 256 / block) workgroups fit 160 KB, one more does not.
+inline unsigned lds_for_waves(int block, int waves) {
+  const int wgs = waves * 256 / block;
+  return 1024u * (unsigned)(160 / (wgs + 1) + 1);
 }
 
 }  // namespace orad

@@ -1,12 +1,15 @@
 // orad_common.hpp -- device data layout shared by the kernels and the C ABI.
 //
 // HBM layout (one context, one GPU):
-//   nodes (FP64 always):  X4[n] = (x, y, z, MS), V4[n] = (vx, vy, vz, ICODT as double),
+//   nodes (FP64 always):  X4[n] = (x, y, z, MS), V4[n] = (vx, vy, vz, node word),
 //                         D[3n] displacement, A[3n] / STIFN[n] assembled outputs.
+//                         The node word holds the node's boundary codes (node_word).
 //   bricks (T = double for the FP64 form, float for the mixed form), structure of
-//   arrays indexed by element:  conn[k*ne+e], iads[k*ne+e] (int32, k = corner),
-//   sig[c*ne+e] (c = 0..5), eint, rho, pla, qvis, offg, epsd; vol0[e] is FP64
-//   in both forms (the reference's single-precision build keeps VOL0DP too).
+//   arrays indexed by the library's element order (a permutation of the engine's,
+//   see orad_brick_order.hip):  conn[k*ne+e] (int32, k = corner), rank[e] (eight
+//   8-bit corner ranks: corner k's slot is adsky[node] + byte k), sig[c*ne+e]
+//   (c = 0..5), eint, rho, pla, qvis, offg, epsd; vol0[e] is FP64 in both forms
+//   (the reference's single-precision build keeps VOL0DP too).
 //   PARITH/ON slots: fsky[s] = (fx, fy, fz) in precision T, slots in node-major
 //   CSR order (adsky[n]..adsky[n+1]-1); sti[e] = STI/4 once per element.
 #pragma once
@@ -43,11 +46,17 @@ struct CycleCtl {
 // for the eight corners of a brick and is kept once per element.
 template <typename T> struct Slot3 { T x, y, z; };
 
+// Node word (V4.w): the node's codes as a small whole number held in a double.
+__host__ __device__ __forceinline__ double node_word(int code) { return (double)code; }
+__host__ __device__ __forceinline__ int word_code(double w) { return (int)w; }
+
 template <typename T>
 struct BrickArgs {
   int numels;
-  const int32_t *conn;      // [8][ne]
-  const int32_t *iads;      // [8][ne]
+  const int32_t *conn;      // [8][ne], library element order
+  const uint64_t *rank;     // [ne]: byte k = slot of corner k within its node's slots
+  const int32_t *emap;      // engine element -> library element (group calls), or null
+  const int32_t *adsky;     // [numnod+1] first slot of each node (ADSKY, 0-based)
   const double4 *X4;
   const double4 *V4;
   T *sig;                   // [6][ne]
@@ -152,6 +161,12 @@ struct NodeArgs {
   double *AR;               // angular accelerations (assembled moments in ASSPAR4)
   double *STIFR;
   const double *fv_val;     // FIXVEL value per node, null when none is imposed
+  // energy balance (one model per context; null otherwise): per-workgroup kinetic
+  // energies of the cycle at parity p, bal_part[p][2][nblocks] (translation, rotation),
+  // and per imposed node the FIXVEL energy memory DT2*DW and the running external work
+  double *bal_part;
+  double *fv_mem;
+  double *fv_work;
   // batched independent models: ctl[model], node -> model, first node of each model
   const int32_t *node_model;   // null: one model (ctl[0])
   const int32_t *model_node0;

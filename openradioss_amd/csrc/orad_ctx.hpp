@@ -14,15 +14,21 @@
 
 namespace orad {
 hipError_t launch_brick_fp64(const BrickArgs<double> &, int, int, const double *,
-                             unsigned long long *, double *, int, int, hipStream_t);
+                             unsigned long long *, double *, int, int, int, int, hipStream_t);
 hipError_t launch_brick_mixed(const BrickArgs<float> &, int, int, const double *,
-                              unsigned long long *, double *, int, int, hipStream_t);
+                              unsigned long long *, double *, int, int, int, int, hipStream_t);
 int brick_block_fp64();
 int brick_block_mixed();
-hipError_t launch_node(const NodeArgs &, bool, int, int, int, int, hipStream_t);
-hipError_t launch_energy(bool, int, const void *, const void *, const double *, int,
-                         const double4 *, const double4 *, const double4 *, double *, double *,
-                         hipStream_t);
+int brick_waves_fp64();
+int brick_waves_mixed();
+bool brick_shape_ok_fp64(int block, int waves);
+bool brick_shape_ok_mixed(int block, int waves);
+hipError_t launch_node(const NodeArgs &, bool, int, int, int, int, hipStream_t, int tile);
+int node_tile_default(bool slot_fp32);
+bool node_tile_ok(bool slot_fp32, bool rot, int tile);
+hipError_t launch_energy(bool, int, const void *, const void *, const double *,
+                         const int32_t *, int, const double4 *, const double4 *, const double4 *,
+                         double *, double *, hipStream_t);
 hipError_t launch_shell_fp64(const ShellArgs<double> &, int, int, const double *,
                              unsigned long long *, double *, int, hipStream_t);
 hipError_t launch_shell_mixed(const ShellArgs<float> &, int, int, const double *,
@@ -58,6 +64,10 @@ hipError_t launch_adj_gather(int, int, const int32_t *, const int32_t *, const i
 }  // namespace orad
 
 struct AdjStore;   // checkpoints of an adjoint forward run (orad_api.hip)
+struct orad_ctx;
+// orad_brick_order.hip
+int brick_bins(orad_ctx *c, size_t ne, const std::vector<int32_t> &conn_em, const double *vol0);
+int brick_reorder(orad_ctx *c, int rows, const std::vector<std::pair<void *, void *>> &extra);
 
 using namespace orad;
 
@@ -94,10 +104,18 @@ struct orad_ctx {
   std::vector<double> h_fv_val;
   double *fv_val = nullptr;
 
+  // bricks: the library's element order (orad_brick_order.hip)
+  int order_rows = 16;                // column height in element rows; 0 = engine order
+  std::vector<int32_t> h_perm;        // library element -> engine element
+  std::vector<int32_t> h_iperm;       // engine element -> library element
+  std::vector<unsigned long long> h_bin;  // centroid bins, engine order
+  uint64_t *rank = nullptr;           // eight 8-bit corner ranks per brick, library order
+  int32_t *emap = nullptr;            // engine element -> library element (device)
+
   // device buffers
   double4 *X4 = nullptr, *V4 = nullptr;
   double *D = nullptr, *A = nullptr, *STIFN = nullptr;
-  int32_t *conn = nullptr, *iads = nullptr, *adsky = nullptr;
+  int32_t *conn = nullptr, *iads = nullptr, *adsky = nullptr;   // iads: shells' IADC
   void *state = nullptr;  // SoA element state of the storage type (NSTATE columns)
   double *vol0 = nullptr; // initial volumes, FP64
   void *fsky = nullptr;
@@ -135,6 +153,9 @@ struct orad_ctx {
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // two cycles starting at parity 0 / 1
   int xcd_remap = 1;
   int elem_lds = -1;        // dynamic LDS bytes per element workgroup; < 0: the form's default
+  int elem_block = 0;       // bricks: element workgroup size (0: the form's default)
+  int elem_waves = 0;       // bricks: waves per SIMD of the element-kernel instance (0: default)
+  int node_tile = 0;        // nodal kernel: slots per LDS tile (0: default)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
   size_t ctl_bytes() const { return (size_t)nmodel * sizeof(CycleCtl); }

@@ -11,6 +11,13 @@
 //            on the imposed component;
 //   VELOCITY (assembly/velocity.F:56-64,77-83) v = v + DT12 a (and vr = vr + DT12 ar);
 //   DEPLA    (assembly/displacement.F:88-101) d = d + DT2 v, x = x + DT2 v.
+// Energy balance (one model per context): between FIXVEL and VELOCITY, where RESOL's
+// SORTIE_MAIN calls ECRIT, each workgroup sums the kinetic energies at the listing's
+// velocity V + DT1/2 A (output/ecrit.F:213-226, 303-325) in a fixed order into
+// bal_part[parity]; each imposed node adds its FIXVEL external work -- the energy
+// memory DT2*DW of the last cycle plus DT1*DW of this one, DW = MS/4 (A DT12 + 2V)
+// (A - AOLD) (constraints/general/impvel/fixvel.F:333-374, 834-836) -- to its own
+// running total fv_work[n].
 // ROT = true adds the rotational DOF of shells: the slots carry moments, ACCELE
 // divides by the nodal inertia IN (accele.F:95-109) and BCS10 applies ICODR.
 // Workgroup 0 also publishes the next cycle's time scalars (double-buffered by
@@ -25,9 +32,12 @@ namespace orad {
 
 constexpr int NODE_BLOCK = 256;
 
+// Slots staged per LDS tile: a launch-time choice among instances whose tile holds at
+// most 48 KB (default 2048 FP32 or 1024 FP64 slots).
 template <typename R> struct SlotTile;
 template <> struct SlotTile<float> { static constexpr int TILE = 2048; };
 template <> struct SlotTile<double> { static constexpr int TILE = 1024; };
+constexpr int NODE_TILE_BYTES = 49152;
 
 // LDS index swizzle: permutes each aligned run of 32 words, so the per-node
 // strided reads below (about 8 words apart) spread over the banks.
@@ -36,12 +46,11 @@ __device__ __forceinline__ int lds_sw(int i) { return i ^ ((i >> 5) & 31); }
 template <typename R, bool ROT> struct SlotOf { typedef Slot3<R> type; };
 template <typename R> struct SlotOf<R, true> { typedef Slot6<R> type; };
 
-template <typename R, bool ROT>
+template <typename R, bool ROT, int TILE>
 __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity, int mode,
                                                           int write_a, int xcd_remap) {
   typedef typename SlotOf<R, ROT>::type S;
   constexpr int NC = ROT ? 6 : 3;
-  constexpr int TILE = SlotTile<R>::TILE;
   __shared__ R t[NC][TILE];
   const int nb = gridDim.x;
   const int b = xcd_remap ? xcd_block(blockIdx.x, nb) : (int)blockIdx.x;
@@ -69,9 +78,9 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
   CycleCtl *ctl = a.ctl + m;
   const Scal sc = ctl->s[parity];
 
-  double dt2 = 0.0, dt12 = 0.0;
+  double dt1 = 0.0, dt2 = 0.0, dt12 = 0.0;
   if (mode == 1) {
-    const double dt1 = sc.dt2;
+    dt1 = sc.dt2;
     const double dt2t = __longlong_as_double((long long)ctl->dt2t[parity]);
     dt2 = 1.0e6;                                  // DT2 = EP06
     if (dt2t < dt2) dt2 = dt2t;
@@ -134,9 +143,9 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
       for (int c = 0; c < NC; ++c) acc[c] = acc[c] + double(t[c][j]);
     }
   }
-  if (!live) return;
   double ax = acc[0], ay = acc[1], az = acc[2];
   if (mode == 0) {
+    if (!live) return;
     // STIFN (FSKY(7,·) summed in slot order); not consumed by the explicit cycle.
     double sf = 0.0;
     const R *sti = reinterpret_cast<const R *>(a.sti);
@@ -149,6 +158,8 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
     }
     return;
   }
+  double ek = 0.0, er = 0.0;
+  if (live) {
   double4 X = a.X4[n];
   double4 V = a.V4[n];
   const double ms = X.w;
@@ -158,7 +169,7 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
   } else {
     ax = 0.0; ay = 0.0; az = 0.0;
   }
-  const int code = (int)V.w;
+  const int code = word_code(V.w);
   double rx = 0.0, ry = 0.0, rz = 0.0;
   double4 W = make_double4(0.0, 0.0, 0.0, 0.0);
   if (ROT) {
@@ -183,9 +194,26 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
     if (cr & 1) { W.z = 0.0; rz = 0.0; }
   }
   const int fd = code_fvdir(code);
-  if (fd == 1) ax = (a.fv_val[n] - V.x) / dt12;
-  if (fd == 2) ay = (a.fv_val[n] - V.y) / dt12;
-  if (fd == 3) az = (a.fv_val[n] - V.z) / dt12;
+  if (fd) {
+    double &aj = fd == 1 ? ax : (fd == 2 ? ay : az);
+    const double vj = fd == 1 ? V.x : (fd == 2 ? V.y : V.z);
+    const double aold = aj;
+    aj = (a.fv_val[n] - vj) / dt12;
+    if (a.fv_mem) {
+      const double dw = 0.25 * ms * (aj * dt12 + 2.0 * vj) * (aj - aold);
+      a.fv_work[n] = a.fv_work[n] + (a.fv_mem[n] + dt1 * dw);
+      a.fv_mem[n] = dt2 * dw;
+    }
+  }
+  if (a.bal_part) {
+    const double dt05 = 0.5 * dt1;
+    const double ux = V.x + dt05 * ax, uy = V.y + dt05 * ay, uz = V.z + dt05 * az;
+    ek = (ux * ux + uy * uy + uz * uz) * 0.5 * ms;
+    if (ROT) {
+      const double wx = W.x + dt05 * rx, wy = W.y + dt05 * ry, wz = W.z + dt05 * rz;
+      er = (wx * wx + wy * wy + wz * wz) * 0.5 * W.w;
+    }
+  }
   if (write_a) {
     a.A[3 * n] = ax; a.A[3 * n + 1] = ay; a.A[3 * n + 2] = az;
     if (ROT) { a.AR[3 * n] = rx; a.AR[3 * n + 1] = ry; a.AR[3 * n + 2] = rz; }
@@ -205,6 +233,29 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
   X.x = X.x + vdx; X.y = X.y + vdy; X.z = X.z + vdz;
   a.X4[n] = X;
   a.V4[n] = V;
+  }  // live
+  if (a.bal_part) {
+    // workgroup sums in a fixed order: 64-lane butterflies, then the four waves
+    __shared__ double s_bal[2][NODE_BLOCK / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      ek += __shfl_xor(ek, o, 64);
+      er += __shfl_xor(er, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      s_bal[0][threadIdx.x >> 6] = ek;
+      s_bal[1][threadIdx.x >> 6] = er;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double se = s_bal[0][0], sr = s_bal[1][0];
+#pragma unroll
+      for (int w = 1; w < NODE_BLOCK / 64; ++w) { se += s_bal[0][w]; sr += s_bal[1][w]; }
+      double *bp = a.bal_part + (size_t)parity * 2 * nbn;
+      bp[b] = se;
+      bp[nbn + b] = sr;
+    }
+  }
 }
 
 // Energy partial sums: block b sums its range in a fixed tree order.
@@ -212,16 +263,20 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
 // plus bending energy eint[e] + eint2[e]; kinetic energy adds 0.5*IN*|vr|^2.
 template <typename T>
 __global__ __launch_bounds__(NODE_BLOCK) void energy_partial_kernel(
-    int numels, const T *eint, const T *eint2, const double *vol0, int numnod, const double4 *X4,
-    const double4 *V4, const double4 *VR4, double *part_e, double *part_k, int nbe) {
+    int numels, const T *eint, const T *eint2, const double *vol0, const int32_t *emap,
+    int numnod, const double4 *X4, const double4 *V4, const double4 *VR4, double *part_e,
+    double *part_k, int nbe) {
   __shared__ double s[NODE_BLOCK];
   const int b = blockIdx.x;
   double v = 0.0;
   double *out;
   if (b < nbe) {
     const int e = b * NODE_BLOCK + threadIdx.x;
-    if (e < numels)
-      v = vol0 ? double(eint[e]) * double(vol0[e]) : double(eint[e]) + double(eint2[e]);
+    if (e < numels) {
+      // engine order whatever the storage order (emap: engine -> library element)
+      const int s = emap ? emap[e] : e;
+      v = vol0 ? double(eint[s]) * double(vol0[s]) : double(eint[s]) + double(eint2[s]);
+    }
     out = part_e + b;
   } else {
     const int n = (b - nbe) * NODE_BLOCK + threadIdx.x;
@@ -264,48 +319,71 @@ __global__ __launch_bounds__(NODE_BLOCK) void energy_final_kernel(const double *
   if (threadIdx.x == 0) { res[0] = s[0][0]; res[1] = s[1][0]; }
 }
 
+template <typename R, bool ROT, int TILE>
+static hipError_t node_instance(const NodeArgs &a, int nb, int parity, int mode, int write_a,
+                                int xcd_remap, hipStream_t s) {
+  static_assert((ROT ? 6 : 3) * TILE * sizeof(R) <= NODE_TILE_BYTES, "tile over 48 KB");
+  hipLaunchKernelGGL((node_kernel<R, ROT, TILE>), dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity,
+                     mode, write_a, xcd_remap);
+  return hipGetLastError();
+}
+
+int node_tile_default(bool slot_fp32) {
+  return slot_fp32 ? SlotTile<float>::TILE : SlotTile<double>::TILE;
+}
+
+// Tiles built: 1024, 2048, 4096 slots where the tile fits 48 KB of LDS.
+bool node_tile_ok(bool slot_fp32, bool rot, int tile) {
+  if (tile != 1024 && tile != 2048 && tile != 4096) return false;
+  return (rot ? 6 : 3) * tile * (slot_fp32 ? 4 : 8) <= NODE_TILE_BYTES;
+}
+
 hipError_t launch_node(const NodeArgs &a, bool slot_fp32, int parity, int mode, int write_a,
-                       int xcd_remap, hipStream_t s) {
+                       int xcd_remap, hipStream_t s, int tile) {
   const int nb = (a.numnod + NODE_BLOCK - 1) / NODE_BLOCK + (mode == 1 ? a.nmodel : 0);
   if (nb <= 0) return hipSuccess;
   const bool rot = a.VR4 != nullptr;
-  if (slot_fp32 && rot)
-    hipLaunchKernelGGL((node_kernel<float, true>), dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity,
-                       mode, write_a, xcd_remap);
-  else if (slot_fp32)
-    hipLaunchKernelGGL((node_kernel<float, false>), dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity,
-                       mode, write_a, xcd_remap);
-  else if (rot)
-    hipLaunchKernelGGL((node_kernel<double, true>), dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity,
-                       mode, write_a, xcd_remap);
-  else
-    hipLaunchKernelGGL((node_kernel<double, false>), dim3(nb), dim3(NODE_BLOCK), 0, s, a, parity,
-                       mode, write_a, xcd_remap);
-  return hipGetLastError();
+  if (tile <= 0) tile = node_tile_default(slot_fp32);
+  if (!node_tile_ok(slot_fp32, rot, tile)) return hipErrorInvalidConfiguration;
+#define ORAD_NODE(R, ROT, T)                                                              \
+  if ((sizeof(R) == 4) == slot_fp32 && rot == ROT && tile == T)                           \
+    return node_instance<R, ROT, T>(a, nb, parity, mode, write_a, xcd_remap, s);
+  ORAD_NODE(float, false, 1024)
+  ORAD_NODE(float, false, 2048)
+  ORAD_NODE(float, false, 4096)
+  ORAD_NODE(float, true, 1024)
+  ORAD_NODE(float, true, 2048)
+  ORAD_NODE(double, false, 1024)
+  ORAD_NODE(double, false, 2048)
+  ORAD_NODE(double, true, 1024)
+#undef ORAD_NODE
+  return hipErrorInvalidConfiguration;
 }
 
 template <typename T>
 static hipError_t launch_energy_t(int numels, const T *eint, const T *eint2, const double *vol0,
-                                  int numnod, const double4 *X4, const double4 *V4,
-                                  const double4 *VR4, double *part, double *res, hipStream_t s) {
+                                  const int32_t *emap, int numnod, const double4 *X4,
+                                  const double4 *V4, const double4 *VR4, double *part,
+                                  double *res, hipStream_t s) {
   const int nbe = (numels + NODE_BLOCK - 1) / NODE_BLOCK;
   const int nbk = (numnod + NODE_BLOCK - 1) / NODE_BLOCK;
   if (nbe + nbk > 0)
     hipLaunchKernelGGL(energy_partial_kernel<T>, dim3(nbe + nbk), dim3(NODE_BLOCK), 0, s, numels,
-                       eint, eint2, vol0, numnod, X4, V4, VR4, part, part + nbe, nbe);
+                       eint, eint2, vol0, emap, numnod, X4, V4, VR4, part, part + nbe, nbe);
   hipLaunchKernelGGL(energy_final_kernel, dim3(1), dim3(NODE_BLOCK), 0, s, part, nbe, part + nbe,
                      nbk, res);
   return hipGetLastError();
 }
 
 hipError_t launch_energy(bool fp32, int numels, const void *eint, const void *eint2,
-                         const double *vol0, int numnod, const double4 *X4, const double4 *V4,
-                         const double4 *VR4, double *part, double *res, hipStream_t s) {
+                         const double *vol0, const int32_t *emap, int numnod, const double4 *X4,
+                         const double4 *V4, const double4 *VR4, double *part, double *res,
+                         hipStream_t s) {
   if (fp32)
-    return launch_energy_t<float>(numels, (const float *)eint, (const float *)eint2, vol0, numnod,
-                                  X4, V4, VR4, part, res, s);
+    return launch_energy_t<float>(numels, (const float *)eint, (const float *)eint2, vol0, emap,
+                                  numnod, X4, V4, VR4, part, res, s);
   return launch_energy_t<double>(numels, (const double *)eint, (const double *)eint2, vol0,
-                                 numnod, X4, V4, VR4, part, res, s);
+                                 emap, numnod, X4, V4, VR4, part, res, s);
 }
 
 }  // namespace orad

@@ -43,9 +43,10 @@ class TimeState(C.Structure):
 
 
 class Launch(C.Structure):
-    """orad_launch_t: element-kernel launch configuration."""
-    _fields_ = [("persistent", C.c_int), ("lds_kb", C.c_int), ("xcd_remap", C.c_int),
-                ("grid", C.c_int)]
+    """orad_launch_t: launch configuration (element-kernel instance, residency cap, XCD
+    mapping, brick element order, nodal slot tile)."""
+    _fields_ = [(k, C.c_int) for k in ("persistent", "lds_kb", "xcd_remap", "grid", "block",
+                                       "waves", "order_rows", "node_tile")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -200,8 +201,10 @@ class BrickSolver:
             pass
 
     # ---- reference-shaped operations ----
-    def set_launch(self, persistent=0, lds_kb=-1, xcd_remap=1):
-        l = Launch(persistent, lds_kb, xcd_remap, 0)
+    def set_launch(self, persistent=0, lds_kb=-1, xcd_remap=1, block=0, waves=0, order_rows=-1,
+                   node_tile=0, grid=0):
+        """orad_set_launch; order_rows=-1 keeps the current brick element order."""
+        l = Launch(persistent, lds_kb, xcd_remap, 0, block, waves, order_rows, node_tile)
         self._chk(self.lib.orad_set_launch(self.h, C.byref(l)))
 
     def launch(self):

@@ -565,6 +565,7 @@ void oracle_cycle(orad_omodel_t *m) {
   oracle_nodal_update(&nd, dt2t);
   m->tt = nd.tt; m->dt1 = nd.dt1; m->dt2 = nd.dt2; m->dt12 = nd.dt12; m->dt2old = nd.dt2old;
   m->wfext = nd.wfext; m->ncycle = nd.ncycle;
+  m->encin = nd.encin; m->enrot = nd.enrot;
   m->mstop |= mstop;
 }
 
@@ -635,6 +636,24 @@ void oracle_nodal_update(orad_onodes_t *m, double dt2t) {
       if (m->fv_mem) m->fv_mem[n] = ZERO + m->dt2*dw;
     }
     m->wfext = wf;
+  }
+  /* ECRIT (output/ecrit.F:192-226, 303-325), called from SORTIE_MAIN between FIXVEL and
+   * VELOCITY: kinetic energies at the listing's velocity V + DT1/2 A, in node order */
+  {
+    const double DT05 = HALF*dt1;
+    double ek = ZERO, er = ZERO;
+    for (int n = 0; n < nn; ++n) {
+      const double *a = m->a + 3*n, *v = m->v + 3*n;
+      const double VX = v[0] + DT05*a[0], VY = v[1] + DT05*a[1], VZ = v[2] + DT05*a[2];
+      ek = ek + (VX*VX + VY*VY + VZ*VZ)*HALF*m->ms[n];
+      if (m->rot) {
+        const double *ar = m->ar + 3*n, *vr = m->vr + 3*n;
+        const double WX = vr[0] + DT05*ar[0], WY = vr[1] + DT05*ar[1], WZ = vr[2] + DT05*ar[2];
+        er = er + (WX*WX + WY*WY + WZ*WZ)*HALF*m->inr[n];
+      }
+    }
+    m->encin = ek;
+    m->enrot = er;
   }
   /* TT (resol.F:8256-8266), VELOCITY (velocity.F:56-83), DEPLA (displacement.F:88-101) */
   m->tt = m->tt + dt2;

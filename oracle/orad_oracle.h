@@ -87,9 +87,10 @@ int oracle_build_pon(int numnod, int numels, const int32_t *ixs,
 /* ---- Nodal part of one explicit cycle, in RESOL order ----
  * time step (resol.F:2639-2640, 5857-5861, 6080, 6224-6226), ACCELE (accele.F:64-110),
  * BCS10/BCS1V (bcs10.F, bcs1.F), FIXVEL for a constant curve with its external work
- * (fixvel.F:333-374, 834-836), TT (resol.F:8256-8266), VELOCITY (velocity.F:56-83),
- * DEPLA (displacement.F:88-101).  a (ar) holds the assembled forces (moments) on
- * entry and the accelerations VELOCITY consumed on return. */
+ * (fixvel.F:333-374, 834-836), ECRIT's kinetic energies at V + DT1/2 A (ecrit.F:213-226,
+ * 303-325), TT (resol.F:8256-8266), VELOCITY (velocity.F:56-83), DEPLA
+ * (displacement.F:88-101).  a (ar) holds the assembled forces (moments) on entry and the
+ * accelerations VELOCITY consumed on return. */
 typedef struct {
   int numnod, rot;                /* rot != 0: rotational DOF (IRODDL = 1)          */
   double *x, *v, *d, *a;          /* 3*numnod each                                  */
@@ -104,6 +105,7 @@ typedef struct {
   double tt, dt1, dt2, dt12, dt2old, dtmx;
   double wfext;                   /* external work of the imposed velocities (WFEXT) */
   int    ncycle, nthreads;
+  double encin, enrot;            /* out: ECRIT's kinetic energies of this cycle     */
 } orad_onodes_t;
 
 void oracle_nodal_update(orad_onodes_t *m, double dt2t);
@@ -130,6 +132,7 @@ typedef struct {
   const double *fv_val;           /* FIXVEL imposed velocity per node              */
   double *fv_mem;                 /* FIXVEL energy memory per node, or null        */
   double wfext;                   /* external work of the imposed velocities       */
+  double encin, enrot;            /* ECRIT's kinetic energies of the last cycle    */
 } orad_omodel_t;
 
 /* One full cycle: element forces -> assembly -> dt -> accel -> BC -> vel -> disp. */
@@ -235,6 +238,7 @@ typedef struct {
   int    ncycle, nthreads;
   double *fv_mem;                 /* FIXVEL energy memory per node, or null      */
   double wfext;                   /* external work of the imposed velocities     */
+  double encin, enrot;            /* ECRIT's kinetic energies of the last cycle  */
 } orad_oshell_t;
 
 void oracle_shell_cycle(orad_oshell_t *m);

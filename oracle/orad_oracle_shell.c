@@ -1498,6 +1498,7 @@ void oracle_shell_cycle(orad_oshell_t *m) {
   oracle_nodal_update(&nd, dt2t);
   m->tt = nd.tt; m->dt1 = nd.dt1; m->dt2 = nd.dt2; m->dt12 = nd.dt12; m->dt2old = nd.dt2old;
   m->wfext = nd.wfext; m->ncycle = nd.ncycle;
+  m->encin = nd.encin; m->enrot = nd.enrot;
   (void)dt1;
 }
 

@@ -48,7 +48,8 @@ class OModel(C.Structure):
                 ("ehour", C.c_double), ("ncycle", C.c_int), ("mstop", C.c_int),
                 ("nthreads", C.c_int), ("pad", C.c_int),
                 ("fv_dir", C.c_void_p), ("fv_val", C.c_void_p),
-                ("fv_mem", C.c_void_p), ("wfext", C.c_double)]
+                ("fv_mem", C.c_void_p), ("wfext", C.c_double),
+                ("encin", C.c_double), ("enrot", C.c_double)]
 
 
 class ONodes(C.Structure):
@@ -58,7 +59,8 @@ class ONodes(C.Structure):
                                           "inr", "icodr", "fv_dir", "fv_val", "fv_mem")] + \
                [(n, C.c_double) for n in ("tt", "dt1", "dt2", "dt12", "dt2old", "dtmx",
                                           "wfext")] + \
-               [("ncycle", C.c_int), ("nthreads", C.c_int)]
+               [("ncycle", C.c_int), ("nthreads", C.c_int),
+                ("encin", C.c_double), ("enrot", C.c_double)]
 
 
 # ---- shells: one element's state row (orad_shstate_t), npt <= 5 ----
@@ -105,7 +107,8 @@ class OShell(C.Structure):
                [(n, C.c_double) for n in ("tt", "dt1", "dt2", "dt12", "dt2old", "dtmx",
                                           "ehour")] + \
                [("ncycle", C.c_int), ("nthreads", C.c_int),
-                ("fv_mem", C.c_void_p), ("wfext", C.c_double)]
+                ("fv_mem", C.c_void_p), ("wfext", C.c_double),
+                ("encin", C.c_double), ("enrot", C.c_double)]
 
 
 def _ptr(a):
@@ -292,7 +295,16 @@ class Model:
 
     def cycle(self, n=1):
         for _ in range(n):
+            first = self.m.ncycle == 0
             lib().oracle_cycle(C.byref(self.m))
+            if first:
+                self.entot0 = self.balance(first=True)
+
+    def balance(self, first=False):
+        """ECRIT's energy balance after the last cycle (output/ecrit.F:500-508):
+        ENTOT = ENCIN + ENINT + ENROT, ERR = ENTOT / (ENTOT0 + WFEXT) - 1 with ENTOT0 =
+        ENTOT - WFEXT at the first cycle."""
+        return _balance(self, lib().oracle_energies, first)
 
     def energies(self):
         """(internal, kinetic, cumulative hourglass) energy."""
@@ -307,6 +319,19 @@ class Model:
     @property
     def tt(self):
         return self.m.tt
+
+
+def _balance(model, energies, first):
+    ei, ek = C.c_double(), C.c_double()
+    energies(C.byref(model.m), C.byref(ei), C.byref(ek))
+    m = model.m
+    entot = m.encin + ei.value + m.enrot
+    if first:
+        return entot - m.wfext
+    den = model.entot0 + m.wfext
+    err = entot / den - 1.0 if abs(den) > 1e-20 else 0.0
+    return dict(encin=m.encin, enrot=m.enrot, enint=ei.value, ehour=m.ehour, wfext=m.wfext,
+                entot=entot, entot0=model.entot0, err=err, tt=m.tt, ncycle=m.ncycle)
 
 
 # ============================== shells ==============================
@@ -541,7 +566,14 @@ class ShellModel:
 
     def cycle(self, n=1):
         for _ in range(n):
+            first = self.m.ncycle == 0
             lib().oracle_shell_cycle(C.byref(self.m))
+            if first:
+                self.entot0 = self.balance(first=True)
+
+    def balance(self, first=False):
+        """ECRIT's energy balance after the last cycle (see Model.balance)."""
+        return _balance(self, lib().oracle_shell_energies, first)
 
     def energies(self):
         ei, ek = C.c_double(), C.c_double()
@@ -595,6 +627,7 @@ def nodal_update(nodes, time, dt2t, rot=False, nthreads=1):
     lib().oracle_nodal_update(C.byref(m), float(dt2t))
     t = {k: getattr(m, k) for k in TIME_KEYS}
     t["wfext"], t["ncycle"] = m.wfext, m.ncycle
+    t["encin"], t["enrot"] = m.encin, m.enrot
     return o, t
 
 
