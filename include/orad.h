@@ -141,6 +141,27 @@ int orad_get_fsky(orad_ctx *ctx, double *fsky);
  * order, so they are run-to-run reproducible. */
 int orad_energies(orad_ctx *ctx, double *eint, double *ekin, double *ehour);
 
+/* Energy balance as ECRIT lists it (output/ecrit.F:192-226, 303-331, 480-508), for the
+ * last cycle run (one model per context):
+ *   encin, enrot  kinetic energies at the listing's velocity V + DT1/2 A (the nodal
+ *                 kernel sums them per workgroup between FIXVEL and VELOCITY, where
+ *                 SORTIE_MAIN calls ECRIT);
+ *   enint         internal energy (as orad_energies); ehour the cumulative hourglass
+ *                 energy, listed apart and not part of ENTOT, as in ECRIT;
+ *   wfext         work of the imposed velocities since the context was created: per
+ *                 imposed node DT1*DW of the cycle plus the DT2*DW kept from the one
+ *                 before, DW = MS/4 (A DT12 + 2V)(A - AOLD)
+ *                 (constraints/general/impvel/fixvel.F:333-374, 834-836);
+ *   entot = encin + enint + enrot; entot0 = ENTOT - WFEXT of the first cycle run by
+ *   orad_run_cycles; err = entot / (entot0 + wfext) - 1 (ERR, x100 in the listing).
+ * Sums are in a fixed order (run-to-run reproducible); WFEXT is summed per node, then
+ * over nodes, where the reference sums its entries each cycle. */
+typedef struct {
+  double encin, enrot, enint, ehour, wfext, entot, entot0, err, tt;
+  int ncycle, pad;
+} orad_balance_t;
+int orad_energy_balance(orad_ctx *ctx, orad_balance_t *b);
+
 /* orad_energies without any device allocation: the same kernels and results
  * (bitwise), with their partial sums in a fixed 1 MiB scratch array that the
  * library keeps on each GPU for the life of the process.  No counterpart in the

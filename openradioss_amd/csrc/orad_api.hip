@@ -99,7 +99,14 @@ static NodeArgs node_args(orad_ctx *c) {
   a.AR = c->AR;
   a.STIFR = c->STIFR;
   a.fv_val = c->fv_val;
+  a.bal_part = c->bal_part;
+  a.fv_mem = c->fv_mem;
+  a.fv_work = c->fv_work;
   if (c->batched) {
+    // the energy balance is kept for one model per context
+    a.bal_part = nullptr;
+    a.fv_mem = nullptr;
+    a.fv_work = nullptr;
     a.node_model = c->node_model;
     a.model_node0 = c->model_node0;
     a.tile_queue = c->queue;
@@ -299,7 +306,7 @@ int orad_destroy(orad_ctx *c) {
                   (void *)c->thke, (void *)c->fv_val, (void *)c->node_model,
                   (void *)c->model_node0, (void *)c->model_tiles, (void *)c->tiles,
                   (void *)c->mats, (void *)c->queue, (void *)c->chunk_ehou, (void *)c->rank,
-                  (void *)c->emap})
+                  (void *)c->emap, (void *)c->bal_part, (void *)c->fv_mem, (void *)c->fv_work})
     dfree(p);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -334,6 +341,15 @@ int orad_set_nodes(orad_ctx *c, int numnod, const double *x, const double *v, co
   if (!c->D) HIPCHK(c, dalloc(&c->D, 3 * n * sizeof(double)));
   if (!c->A) HIPCHK(c, dalloc(&c->A, 3 * n * sizeof(double)));
   if (!c->STIFN) HIPCHK(c, dalloc(&c->STIFN, n * sizeof(double)));
+  if (!c->bal_part) {
+    const size_t nbn = (n + 255) / 256;
+    HIPCHK(c, dalloc(&c->bal_part, 4 * nbn * sizeof(double)));
+    HIPCHK(c, hipMemset(c->bal_part, 0, 4 * nbn * sizeof(double)));
+    HIPCHK(c, dalloc(&c->fv_mem, n * sizeof(double)));
+    HIPCHK(c, hipMemset(c->fv_mem, 0, n * sizeof(double)));
+    HIPCHK(c, dalloc(&c->fv_work, n * sizeof(double)));
+    HIPCHK(c, hipMemset(c->fv_work, 0, n * sizeof(double)));
+  }
   HIPCHK(c, hipMemcpy(c->X4, hx.data(), n * sizeof(double4), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->V4, hv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemset(c->D, 0, 3 * n * sizeof(double)));
@@ -554,6 +570,51 @@ int orad_asspar4(orad_ctx *c, double *a, double *stifn) {
   return ORAD_OK;
 }
 
+static int energies_range(orad_ctx *c, size_t e0, size_t e1, size_t n0, size_t n1, int m0,
+                          int m1, double *eint, double *ekin, double *ehour,
+                          double *scr = nullptr);
+
+// ECRIT's energy terms after the last cycle run (single-model contexts): the kinetic
+// energies the nodal kernel summed per workgroup at the listing's velocity, the
+// internal and hourglass energies, and the imposed-velocity work summed over nodes.
+static int balance_now(orad_ctx *c, orad_balance_t *b) {
+  const size_t nn = (size_t)c->numnod, nbn = (nn + 255) / 256;
+  const int p = (c->host_ncycle - 1) & 1;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<double> part(2 * nbn);
+  HIPCHK(c, hipMemcpy(part.data(), c->bal_part + (size_t)p * 2 * nbn, 2 * nbn * sizeof(double),
+                      hipMemcpyDeviceToHost));
+  double ek = 0.0, er = 0.0;
+  for (size_t i = 0; i < nbn; ++i) ek += part[i];
+  for (size_t i = 0; i < nbn; ++i) er += part[nbn + i];
+  double *scr = nullptr;
+  HIPCHK(c, dalloc(&scr, (nbn + 2) * sizeof(double)));
+  hipError_t e = launch_sum(c->fv_work, (int)nn, scr, scr + nbn, c->stream);
+  double wf[2] = {0.0, 0.0};
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(wf, scr + nbn, 2 * sizeof(double), hipMemcpyDeviceToHost);
+  dfree(scr);
+  if (e != hipSuccess) return fail(c, ORAD_EHIP, std::string("energy balance: ") + hipGetErrorString(e));
+  double ei = 0.0, ekin_state = 0.0, eh = 0.0;
+  int r = energies_range(c, 0, (size_t)nelem(c), 0, nn, 0, c->nmodel, &ei, &ekin_state, &eh);
+  if (r) return r;
+  orad_time_t t{};
+  r = model_time(c, 0, &t);
+  if (r) return r;
+  b->encin = ek;
+  b->enrot = er;
+  b->enint = ei;
+  b->ehour = eh;
+  b->wfext = wf[0];
+  b->entot = ek + ei + er;
+  b->entot0 = c->entot0;
+  const double den = c->entot0 + b->wfext;
+  b->err = (c->have_entot0 && std::fabs(den) > 1.0e-20) ? b->entot / den - 1.0 : 0.0;
+  b->tt = t.tt;
+  b->ncycle = t.ncycle;
+  return ORAD_OK;
+}
+
 static int build_graph(orad_ctx *c, int p) {
   hipGraph_t g = nullptr;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -578,6 +639,17 @@ int orad_run_cycles(orad_ctx *c, int ncycle, orad_time_t *t) {
   if (ncycle < 0) return fail(c, ORAD_EINVAL, "run_cycles: negative cycle count");
   HIPCHK(c, hipSetDevice(c->device));
   int left = ncycle;
+  if (c->host_ncycle == 0 && left > 0 && !c->batched) {
+    // ECRIT's reference total at the first cycle: ENTOT0 = ENTOT - WFEXT (ecrit.F:500-503)
+    HIPCHK(c, enqueue_cycle(c, 0));
+    c->host_ncycle = 1;
+    left -= 1;
+    orad_balance_t b{};
+    r = balance_now(c, &b);
+    if (r) return r;
+    c->entot0 = b.entot - b.wfext;
+    c->have_entot0 = true;
+  }
   while (left >= 2) {
     const int p = c->host_ncycle & 1;
     if (!c->graph[p]) {
@@ -746,8 +818,7 @@ static size_t energy_scratch_doubles(size_t me, size_t mn) {
 }
 
 static int energies_range(orad_ctx *c, size_t e0, size_t e1, size_t n0, size_t n1, int m0,
-                          int m1, double *eint, double *ekin, double *ehour,
-                          double *scr = nullptr) {
+                          int m1, double *eint, double *ekin, double *ehour, double *scr) {
   HIPCHK(c, hipSetDevice(c->device));
   const bool shell = c->family == FAM_SHELL;
   const size_t ne = (size_t)nelem(c), ts = c->tsize(), me = e1 - e0, mn = n1 - n0;
@@ -783,6 +854,15 @@ int orad_energies(orad_ctx *c, double *eint, double *ekin, double *ehour) {
   if (r) return r;
   return energies_range(c, 0, (size_t)nelem(c), 0, (size_t)c->numnod, 0, c->nmodel, eint, ekin,
                         ehour);
+}
+
+int orad_energy_balance(orad_ctx *c, orad_balance_t *b) {
+  if (!c || !b) return ORAD_EINVAL;
+  int r = ready(c);
+  if (r) return r;
+  if (c->batched) return fail(c, ORAD_ESTATE, "energy_balance: one model per context");
+  if (c->host_ncycle < 1) return fail(c, ORAD_ESTATE, "energy_balance: run a cycle first");
+  return balance_now(c, b);
 }
 
 int orad_energies_noalloc(orad_ctx *c, double *eint, double *ekin, double *ehour) {
@@ -1413,6 +1493,9 @@ static std::vector<std::pair<void *, size_t>> cycle_buffers(orad_ctx *c) {
       {c->X4, nn * 32}, {c->V4, nn * 32}, {c->D, nn * 24}, {c->state, ncol * ne * c->tsize()},
       {c->sti, nsti * ne * c->tsize()}, {c->ctl, c->ctl_bytes()}};
   if (c->family == FAM_SHELL) live.push_back({c->VR4, nn * 32});
+  live.push_back({c->bal_part, 4 * ((nn + 255) / 256) * sizeof(double)});
+  live.push_back({c->fv_mem, nn * sizeof(double)});
+  live.push_back({c->fv_work, nn * sizeof(double)});
   return live;
 }
 

@@ -29,6 +29,7 @@ bool node_tile_ok(bool slot_fp32, bool rot, int tile);
 hipError_t launch_energy(bool, int, const void *, const void *, const double *,
                          const int32_t *, int, const double4 *, const double4 *, const double4 *,
                          double *, double *, hipStream_t);
+hipError_t launch_sum(const double *x, int n, double *part, double *res, hipStream_t s);
 hipError_t launch_shell_fp64(const ShellArgs<double> &, int, int, const double *,
                              unsigned long long *, double *, int, hipStream_t);
 hipError_t launch_shell_mixed(const ShellArgs<float> &, int, int, const double *,
@@ -103,6 +104,11 @@ struct orad_ctx {
   std::vector<int> h_code_hi;   // per node: ICODR << 3 | FIXVEL direction << 6
   std::vector<double> h_fv_val;
   double *fv_val = nullptr;
+  // energy balance (ECRIT): per-workgroup kinetic energies of the last two cycles, FIXVEL
+  // energy memory and running external work per node, ENTOT0 of the first cycle
+  double *bal_part = nullptr, *fv_mem = nullptr, *fv_work = nullptr;
+  double entot0 = 0.0;
+  bool have_entot0 = false;
 
   // bricks: the library's element order (orad_brick_order.hip)
   int order_rows = 16;                // column height in element rows; 0 = engine order

@@ -375,6 +375,30 @@ static hipError_t launch_energy_t(int numels, const T *eint, const T *eint2, con
   return hipGetLastError();
 }
 
+// Fixed-order sum of x[0..n): one partial per 256 values, then the partials.
+__global__ __launch_bounds__(NODE_BLOCK) void sum_partial_kernel(const double *x, int n,
+                                                                 double *part) {
+  __shared__ double s[NODE_BLOCK];
+  const int i = blockIdx.x * NODE_BLOCK + threadIdx.x;
+  s[threadIdx.x] = i < n ? x[i] : 0.0;
+  __syncthreads();
+  for (int w = NODE_BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// part: (n + 255) / 256 doubles; res: two doubles (the sum, then zero)
+hipError_t launch_sum(const double *x, int n, double *part, double *res, hipStream_t s) {
+  const int nb = (n + NODE_BLOCK - 1) / NODE_BLOCK;
+  if (nb > 0)
+    hipLaunchKernelGGL(sum_partial_kernel, dim3(nb), dim3(NODE_BLOCK), 0, s, x, n, part);
+  hipLaunchKernelGGL(energy_final_kernel, dim3(1), dim3(NODE_BLOCK), 0, s, part, nb, part, 0,
+                     res);
+  return hipGetLastError();
+}
+
 hipError_t launch_energy(bool fp32, int numels, const void *eint, const void *eint2,
                          const double *vol0, const int32_t *emap, int numnod, const double4 *X4,
                          const double4 *V4, const double4 *VR4, double *part, double *res,

@@ -22,7 +22,7 @@ EXPORTS = (
     "orad_set_law2", "orad_set_solid_prop", "orad_get_pon", "orad_set_time", "orad_get_time",
     "orad_sforc3", "orad_asspar4", "orad_run_cycles", "orad_profile_cycles", "orad_get_nodes",
     "orad_get_brick_state", "orad_set_brick_state", "orad_get_fsky", "orad_energies",
-    "orad_energies_noalloc",
+    "orad_energies_noalloc", "orad_energy_balance",
     "orad_set_fixvel", "orad_set_nodes_rot", "orad_set_shells", "orad_set_shell_law2",
     "orad_set_shell_prop", "orad_set_shell_ishell", "orad_cforc3", "orad_asspar4_rot", "orad_get_nodes_rot",
     "orad_get_shell_state", "orad_set_shell_state", "orad_set_shell_models",
@@ -40,6 +40,16 @@ class TimeState(C.Structure):
     _fields_ = [("tt", C.c_double), ("dt1", C.c_double), ("dt2", C.c_double),
                 ("dt12", C.c_double), ("dt2old", C.c_double), ("dtmx", C.c_double),
                 ("ncycle", C.c_int), ("mstop", C.c_int)]
+
+
+class Balance(C.Structure):
+    """orad_balance_t: ECRIT's energy balance of the last cycle."""
+    _fields_ = [(k, C.c_double) for k in ("encin", "enrot", "enint", "ehour", "wfext", "entot",
+                                          "entot0", "err", "tt")] + \
+               [("ncycle", C.c_int), ("pad", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
 class Launch(C.Structure):
@@ -91,6 +101,7 @@ def load_library(path=LIB_PATH):
         "orad_get_fsky": ([P, P], I),
         "orad_energies": ([P, C.POINTER(D), C.POINTER(D), C.POINTER(D)], I),
         "orad_energies_noalloc": ([P, C.POINTER(D), C.POINTER(D), C.POINTER(D)], I),
+        "orad_energy_balance": ([P, C.POINTER(Balance)], I),
         "orad_set_fixvel": ([P, I, P, P, P], I),
         "orad_set_nodes_rot": ([P, P, P, P], I),
         "orad_set_shells": ([P, I, P, P], I),
@@ -292,6 +303,12 @@ class BrickSolver:
         self._chk(self.lib.orad_energies(self.h, C.byref(ei), C.byref(ek), C.byref(eh)))
         return ei.value, ek.value, eh.value
 
+    def balance(self):
+        """ECRIT's energy balance of the last cycle (orad_energy_balance) as a dict."""
+        b = Balance()
+        self._chk(self.lib.orad_energy_balance(self.h, C.byref(b)))
+        return b.as_dict()
+
 
 def ixc_from_deck(deck):
     """IXC(NIXC=7, NUMELC) in the engine's layout (ccoor3.F:69-78): material,
@@ -413,6 +430,7 @@ class ShellSolver:
     time = BrickSolver.time
     set_time = BrickSolver.set_time
     energies = BrickSolver.energies
+    balance = BrickSolver.balance
 
     def pon(self):
         adsky = np.zeros(self.numnod + 1, np.int32)

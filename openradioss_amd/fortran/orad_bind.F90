@@ -17,6 +17,11 @@ module orad_bind
     integer(c_int) :: ncycle, mstop
   end type orad_time_t
 
+  type, bind(c) :: orad_balance_t
+    real(c_double) :: encin, enrot, enint, ehour, wfext, entot, entot0, err, tt
+    integer(c_int) :: ncycle, pad
+  end type orad_balance_t
+
   type, bind(c) :: orad_launch_t
     integer(c_int) :: persistent, lds_kb, xcd_remap, grid
     integer(c_int) :: block, waves, order_rows, node_tile
@@ -103,6 +108,11 @@ module orad_bind
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
       real(c_double) :: eint, ekin, ehour
+    end function
+    integer(c_int) function orad_energy_balance(ctx, b) bind(c, name='orad_energy_balance')
+      import :: c_ptr, c_int, orad_balance_t
+      type(c_ptr), value :: ctx
+      type(orad_balance_t) :: b
     end function
     integer(c_int) function orad_energies_noalloc(ctx, eint, ekin, ehour) &
         bind(c, name='orad_energies_noalloc')
