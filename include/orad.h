@@ -49,12 +49,17 @@ typedef struct {
  * many element rows walked layer by layer (1..128; 0 = the engine's order, -1 =
  * keep the current one).  Every family: node_tile -- PARITH/ON slots the nodal
  * kernel stages per LDS tile (1024, 2048, 4096 within 48 KB; 0 = default).
+ * Shells: adj_vjp -- the element vector-Jacobian product of orad_adjoint_reverse:
+ * 0 = the hand-written reverse pass where the formulation has one (Belytschko-Tsay
+ * with law 2 radial return, IPLA = 0), forward-mode dual numbers otherwise; 1 =
+ * forward-mode dual numbers always (both give the same derivative to rounding).
  * orad_get_launch reports the values in use.  Nodes, slots and element state do
  * not depend on any of these; the element order changes only the rounding of the
  * cumulative hourglass energy (its per-workgroup partial sums). */
 typedef struct {
   int persistent, lds_kb, xcd_remap, grid;
   int block, waves, order_rows, node_tile;
+  int adj_vjp;
 } orad_launch_t;
 
 /* Create/destroy a context on one HIP device.  No reference counterpart: the

@@ -11,7 +11,10 @@
 // Reverse of cycle k, three launches:
 //   adj_node_kernel    adjoints of v_{k+1}, x, d -> adjoint of S and R per node,
 //                      of v_k through the update, and the MS / IN sensitivities;
-//   adj_elem_kernel    vector-Jacobian product of the element chain: the CFORC3
+//   adj_elem_rev_kernel  Belytschko-Tsay with radial return (IPLA = 0): the
+//                      hand-written reverse pass of the CFORC3 chain
+//                      (orad_shell_adjoint.hpp), one thread per element;
+//   adj_elem_kernel    otherwise, vector-Jacobian product of the element chain: the CFORC3
 //                      chain (orad_shell_chain.hpp), or for QEPH the CZFORC3 chain
 //                      (orad_qeph_chain.hpp), instantiated on Dual<K>, seeds K of the
 //                      element's inputs per thread (4 corners x (x, v, vr), 25
@@ -21,12 +24,18 @@
 //                      contracts the tangents of its outputs (slots, new state) with
 //                      their adjoints;
 //   adj_gather_kernel  per node, the corner contributions in slot order (the
-//                      PARITH/ON CSR), so the sums are deterministic.
+//                      PARITH/ON CSR), so the sums are deterministic; within a
+//                      checkpoint segment fused with the next (earlier) cycle's
+//                      adj_node_kernel (adj_gather_node_kernel).
+// The recompute of a segment stashes each cycle's inputs and the previous cycle's
+// accelerations and scalars with one copy launch per cycle (adj_stash_kernel).
 #include "orad_dual.hpp"
 #include "orad_qeph_chain.hpp"
+#include "orad_shell_adjoint.hpp"
 #include "orad_brick_io.hpp"
 #include "orad_adjoint_args.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace orad {
@@ -153,23 +162,77 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_kernel(AdjArgs<S> a, int b
   }
 }
 
-// Reverse of the nodal update of one cycle.  In: lx, lv, ld, lvr = adjoints of
-// x, v, d, vr after the cycle; A, AR = the cycle's accelerations (after BCS10).
+// One Belytschko-Tsay element per thread through the hand-written reverse pass: the
+// same outputs as the dual chunks (lcorner and lst_out set, g_thk and g_jc added to).
+template <typename S>
+__global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_rev_kernel(AdjArgs<S> a) {
+  const int ne = a.numelc;
+  const int e = blockIdx.x * ADJ_BLOCK + threadIdx.x;
+  if (e >= ne) return;
+  int nd[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) nd[c] = a.conn[(size_t)c * ne + e];
+  double xg[3][4], vg[3][4], vrg[3][4];
+  double lx[3][4], lv[3][4], lvr[3][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double4 x = a.X4[nd[c]], v = a.V4[nd[c]], w = a.VR4[nd[c]];
+    xg[0][c] = x.x; xg[1][c] = x.y; xg[2][c] = x.z;
+    vg[0][c] = v.x; vg[1][c] = v.y; vg[2][c] = v.z;
+    vrg[0][c] = w.x; vrg[1][c] = w.y; vrg[2][c] = w.z;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) lx[i][c] = lv[i][c] = lvr[i][c] = 0.0;
+  }
+  const double jc[3] = {a.mat.ca, a.mat.cb, a.mat.cn};
+  double gthk = 0.0, gjc[3] = {0.0, 0.0, 0.0};
+  const S *st = a.st;
+  const double *lin = a.lst_in, *lamS = a.lamS, *lamR = a.lamR;
+  double *lout = a.lst_out;
+  auto stv = [&](int col) { return double(st[(size_t)col * ne + e]); };
+  auto lpost = [&](int j) { return lin[(size_t)j * ne + e]; };
+  auto lslot = [&](int c, int i) {
+    return i < 3 ? lamS[3 * (size_t)nd[c] + i] : lamR[3 * (size_t)nd[c] + i - 3];
+  };
+  auto lpre = [&](int j, double v) { lout[(size_t)j * ne + e] = v; };
+  shell_vjp_bt(xg, vg, vrg, a.thke[e], nd[2] == nd[3], stv, a.dt1, a.mat, jc, a.prop, a.k, lpost,
+               lslot, lpre, lx, lv, lvr, gthk, gjc);
+  for (int it = a.prop.npt; it < 5; ++it) {     // points not in use carry no adjoint
+#pragma unroll
+    for (int c = 0; c < 5; ++c) lout[(size_t)(5 * it + c) * ne + e] = 0.0;
+    lout[(size_t)(25 + it) * ne + e] = 0.0;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      a.lcorner[(size_t)(9 * c + i) * ne + e] = lx[i][c];
+      a.lcorner[(size_t)(9 * c + 3 + i) * ne + e] = lv[i][c];
+      a.lcorner[(size_t)(9 * c + 6 + i) * ne + e] = lvr[i][c];
+    }
+  a.g_thk[e] += gthk;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) a.g_jc[(size_t)j * ne + e] += gjc[j];
+}
+
+// Reverse of the nodal update of one cycle for node n.  In: lx, lv, ld, lvr = adjoints
+// of x, v, d, vr after the cycle; A, AR = the cycle's accelerations (after BCS10).
 // Out: lv, lvr = adjoints of v_k, vr_k through the update (element terms are added
 // by the gather), lamS / lamR = adjoints of the slot sums, MS / IN sensitivities
 // accumulated into g_ms / g_in.
-__global__ __launch_bounds__(256) void adj_node_kernel(int numnod, const double4 *X4,
-                                                       const double4 *V4, const double4 *VR4,
-                                                       const double *A, const double *AR,
-                                                       double dt2, double dt12, const double *lx,
-                                                       double *lv, const double *ld, double *lvr,
-                                                       double *lamS, double *lamR, double *g_ms,
-                                                       double *g_in) {
-  const int n = blockIdx.x * 256 + threadIdx.x;
-  if (n >= numnod) return;
-  const int code = word_code(V4[n].w);
+struct AdjNodeArgs {
+  const double4 *X4, *V4, *VR4;
+  const double *A, *AR;
+  double dt2, dt12;
+};
+
+__device__ __forceinline__ void adj_node_one(int n, const AdjNodeArgs &q, const double *lx,
+                                             double *lv, const double *ld, double *lvr,
+                                             double *lamS, double *lamR, double *g_ms,
+                                             double *g_in) {
+  const int code = word_code(q.V4[n].w);
   const int ct = code_icodt(code), cr = code_icodr(code), fd = code_fvdir(code);
-  const double ms = X4[n].w, in = VR4[n].w;
+  const double ms = q.X4[n].w, in = q.VR4[n].w;
+  const double dt2 = q.dt2, dt12 = q.dt12;
   double gm = 0.0, gi = 0.0;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -181,7 +244,7 @@ __global__ __launch_bounds__(256) void adj_node_kernel(int numnod, const double4
       lvn = u;
       if (ms > 0.0) {
         ls = dt12 * u / ms;
-        gm = gm - dt12 * u * A[j] / ms;
+        gm = gm - dt12 * u * q.A[j] / ms;
       }
     }
     lv[j] = lvn;
@@ -193,7 +256,7 @@ __global__ __launch_bounds__(256) void adj_node_kernel(int numnod, const double4
       lvrn = ur;
       if (in > 0.0) {
         lr = dt12 * ur / in;
-        gi = gi - dt12 * ur * AR[j] / in;
+        gi = gi - dt12 * ur * q.AR[j] / in;
       }
     }
     lvr[j] = lvrn;
@@ -203,15 +266,22 @@ __global__ __launch_bounds__(256) void adj_node_kernel(int numnod, const double4
   g_in[n] += gi;
 }
 
-// Element corner contributions to the node adjoints, per node in slot order.
-__global__ __launch_bounds__(256) void adj_gather_kernel(int numnod, int numelc,
-                                                         const int32_t *adsky,
-                                                         const int32_t *slot_elem,
-                                                         const int32_t *slot_corner,
-                                                         const double *lcorner, double *lx,
-                                                         double *lv, double *lvr) {
+__global__ __launch_bounds__(256) void adj_node_kernel(int numnod, AdjNodeArgs q,
+                                                       const double *lx, double *lv,
+                                                       const double *ld, double *lvr,
+                                                       double *lamS, double *lamR, double *g_ms,
+                                                       double *g_in) {
   const int n = blockIdx.x * 256 + threadIdx.x;
   if (n >= numnod) return;
+  adj_node_one(n, q, lx, lv, ld, lvr, lamS, lamR, g_ms, g_in);
+}
+
+// Element corner contributions to the adjoints of node n, in slot order.
+__device__ __forceinline__ void adj_gather_one(int n, int numelc, const int32_t *adsky,
+                                               const int32_t *slot_elem,
+                                               const int32_t *slot_corner,
+                                               const double *lcorner, double *lx, double *lv,
+                                               double *lvr) {
   double ax[3] = {0, 0, 0}, av[3] = {0, 0, 0}, ar[3] = {0, 0, 0};
   for (int s = adsky[n]; s < adsky[n + 1]; ++s) {
     const int e = slot_elem[s], c = slot_corner[s];
@@ -227,6 +297,56 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(int numnod, int numelc,
     lx[3 * (size_t)n + i] += ax[i];
     lv[3 * (size_t)n + i] += av[i];
     lvr[3 * (size_t)n + i] += ar[i];
+  }
+}
+
+__global__ __launch_bounds__(256) void adj_gather_kernel(int numnod, int numelc,
+                                                         const int32_t *adsky,
+                                                         const int32_t *slot_elem,
+                                                         const int32_t *slot_corner,
+                                                         const double *lcorner, double *lx,
+                                                         double *lv, double *lvr) {
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= numnod) return;
+  adj_gather_one(n, numelc, adsky, slot_elem, slot_corner, lcorner, lx, lv, lvr);
+}
+
+// The gather of cycle k followed, node by node, by the nodal reverse of cycle k-1
+// (both touch only node n's entries, so one thread does both in order).
+__global__ __launch_bounds__(256) void adj_gather_node_kernel(
+    int numnod, int numelc, const int32_t *adsky, const int32_t *slot_elem,
+    const int32_t *slot_corner, const double *lcorner, AdjNodeArgs q, double *lx, double *lv,
+    const double *ld, double *lvr, double *lamS, double *lamR, double *g_ms, double *g_in) {
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= numnod) return;
+  adj_gather_one(n, numelc, adsky, slot_elem, slot_corner, lcorner, lx, lv, lvr);
+  adj_node_one(n, q, lx, lv, ld, lvr, lamS, lamR, g_ms, g_in);
+}
+
+// Copies of up to 7 device ranges in one launch (blockIdx.y = range): the inputs of
+// a recomputed cycle and the accelerations and scalars of the one before it.  A range
+// whose ends are 16-byte aligned moves in 16-byte words, others in 4-byte words.
+struct StashJob {
+  const void *src[7];
+  void *dst[7];
+  unsigned long long bytes[7];
+};
+
+__global__ __launch_bounds__(256) void adj_stash_kernel(StashJob j) {
+  const int r = blockIdx.y;
+  const unsigned long long nb = j.bytes[r];
+  const char *src = (const char *)j.src[r];
+  char *dst = (char *)j.dst[r];
+  const size_t stride = (size_t)gridDim.x * 256;
+  const bool wide = (((uintptr_t)src | (uintptr_t)dst | nb) & 15) == 0;
+  if (wide) {
+    const uint4 *s4 = (const uint4 *)src;
+    uint4 *d4 = (uint4 *)dst;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < nb / 16; i += stride) d4[i] = s4[i];
+  } else {
+    const uint32_t *s1 = (const uint32_t *)src;
+    uint32_t *d1 = (uint32_t *)dst;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < nb / 4; i += stride) d1[i] = s1[i];
   }
 }
 
@@ -248,12 +368,18 @@ static void launch_adj_elem_chunks(const AdjArgs<S> &a, hipStream_t s) {
 }
 
 template <typename S>
-static void launch_adj_elem_form(bool qeph, const AdjArgs<S> &a, hipStream_t s) {
-  if (qeph) launch_adj_elem_chunks<S, true>(a, s);
-  else launch_adj_elem_chunks<S, false>(a, s);
+static void launch_adj_elem_form(bool qeph, bool rev, const AdjArgs<S> &a, hipStream_t s) {
+  if (rev && !qeph) {
+    const int nb = (a.numelc + ADJ_BLOCK - 1) / ADJ_BLOCK;
+    hipLaunchKernelGGL(adj_elem_rev_kernel<S>, dim3(nb), dim3(ADJ_BLOCK), 0, s, a);
+  } else if (qeph) {
+    launch_adj_elem_chunks<S, true>(a, s);
+  } else {
+    launch_adj_elem_chunks<S, false>(a, s);
+  }
 }
 
-hipError_t launch_adj_elem(bool fp32_state, bool qeph, const AdjArgs<double> &a64,
+hipError_t launch_adj_elem(bool fp32_state, bool qeph, bool rev, const AdjArgs<double> &a64,
                            hipStream_t s) {
   if (a64.numelc <= 0) return hipSuccess;
   if (fp32_state) {
@@ -263,9 +389,9 @@ hipError_t launch_adj_elem(bool fp32_state, bool qeph, const AdjArgs<double> &a6
     a.thke = a64.thke; a.lamS = a64.lamS; a.lamR = a64.lamR; a.lst_in = a64.lst_in;
     a.lst_out = a64.lst_out; a.lcorner = a64.lcorner; a.g_thk = a64.g_thk; a.g_jc = a64.g_jc;
     a.mat = a64.mat; a.prop = a64.prop; a.k = a64.k; a.dt1 = a64.dt1;
-    launch_adj_elem_form(qeph, a, s);
+    launch_adj_elem_form(qeph, rev, a, s);
   } else {
-    launch_adj_elem_form(qeph, a64, s);
+    launch_adj_elem_form(qeph, rev, a64, s);
   }
   return hipGetLastError();
 }
@@ -277,8 +403,9 @@ hipError_t launch_adj_node(int numnod, const double4 *X4, const double4 *V4, con
                            hipStream_t s) {
   const int nb = (numnod + 255) / 256;
   if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL(adj_node_kernel, dim3(nb), dim3(256), 0, s, numnod, X4, V4, VR4, A, AR,
-                     dt2, dt12, lx, lv, ld, lvr, lamS, lamR, g_ms, g_in);
+  const AdjNodeArgs q{X4, V4, VR4, A, AR, dt2, dt12};
+  hipLaunchKernelGGL(adj_node_kernel, dim3(nb), dim3(256), 0, s, numnod, q, lx, lv, ld, lvr,
+                     lamS, lamR, g_ms, g_in);
   return hipGetLastError();
 }
 
@@ -290,6 +417,39 @@ hipError_t launch_adj_gather(int numnod, int numelc, const int32_t *adsky,
   if (nb <= 0) return hipSuccess;
   hipLaunchKernelGGL(adj_gather_kernel, dim3(nb), dim3(256), 0, s, numnod, numelc, adsky,
                      slot_elem, slot_corner, lcorner, lx, lv, lvr);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_gather_node(int numnod, int numelc, const int32_t *adsky,
+                                  const int32_t *slot_elem, const int32_t *slot_corner,
+                                  const double *lcorner, const double4 *X4, const double4 *V4,
+                                  const double4 *VR4, const double *A, const double *AR,
+                                  double dt2, double dt12, double *lx, double *lv,
+                                  const double *ld, double *lvr, double *lamS, double *lamR,
+                                  double *g_ms, double *g_in, hipStream_t s) {
+  const int nb = (numnod + 255) / 256;
+  if (nb <= 0) return hipSuccess;
+  const AdjNodeArgs q{X4, V4, VR4, A, AR, dt2, dt12};
+  hipLaunchKernelGGL(adj_gather_node_kernel, dim3(nb), dim3(256), 0, s, numnod, numelc, adsky,
+                     slot_elem, slot_corner, lcorner, q, lx, lv, ld, lvr, lamS, lamR, g_ms, g_in);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_stash(int n, const void *const *src, void *const *dst,
+                            const size_t *bytes, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (n > 7) return hipErrorInvalidValue;
+  StashJob j{};
+  size_t most = 0;
+  for (int r = 0; r < n; ++r) {
+    if (bytes[r] % 4) return hipErrorInvalidValue;
+    j.src[r] = src[r];
+    j.dst[r] = dst[r];
+    j.bytes[r] = bytes[r];
+    most = std::max(most, bytes[r]);
+  }
+  const int gx = (int)std::min<size_t>(1024, std::max<size_t>(1, (most / 16 + 255) / 256));
+  hipLaunchKernelGGL(adj_stash_kernel, dim3(gx, n), dim3(256), 0, s, j);
   return hipGetLastError();
 }
 

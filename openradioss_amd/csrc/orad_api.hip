@@ -1422,6 +1422,7 @@ int orad_get_launch(orad_ctx *c, orad_launch_t *l) {
     l->order_rows = c->order_rows;
   }
   l->node_tile = c->node_tile > 0 ? c->node_tile : node_tile_default(mixed);
+  l->adj_vjp = c->adj_vjp;
   return ORAD_OK;
 }
 
@@ -1443,11 +1444,14 @@ int orad_set_launch(orad_ctx *c, const orad_launch_t *l) {
     return fail(c, ORAD_EINVAL, "set_launch: order_rows in -1..128, bricks only");
   if (l->node_tile && !node_tile_ok(mixed, c->family == FAM_SHELL, l->node_tile))
     return fail(c, ORAD_EINVAL, "set_launch: node_tile 1024, 2048 or 4096 slots within 48 KB");
+  if (l->adj_vjp != 0 && l->adj_vjp != 1)
+    return fail(c, ORAD_EINVAL, "set_launch: adj_vjp is 0 (reverse where available) or 1 (duals)");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const int keep_p = c->persist, keep_b = c->batch_lds, keep_e = c->elem_lds;
   c->xcd_remap = l->xcd_remap != 0;
   c->node_tile = l->node_tile;
+  c->adj_vjp = l->adj_vjp;
   if (brick) {
     c->elem_block = l->block;
     c->elem_waves = l->waves;
@@ -1770,12 +1774,15 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
   HIPCHK(c, B.get(&gjc, 3 * ne * 8));
   HIPCHK(c, B.get(&scorner, 4 * ne * 4, false));
   HIPCHK(c, B.get(&selem, 4 * ne * 4, false));
+  // per-cycle stash slots, strides rounded to 256 bytes so every slot is 16-byte aligned
+  const size_t sa_stride = (3 * nn + 31) / 32 * 32;        // doubles
+  const size_t st_stride = (stb + 255) / 256 * 256;        // bytes
   HIPCHK(c, B.get(&sx, (size_t)L * nn * 32, false));
   HIPCHK(c, B.get(&sv, (size_t)L * nn * 32, false));
   HIPCHK(c, B.get(&svr, (size_t)L * nn * 32, false));
-  HIPCHK(c, B.get(&sa, (size_t)L * 3 * nn * 8, false));
-  HIPCHK(c, B.get(&sar, (size_t)L * 3 * nn * 8, false));
-  HIPCHK(c, B.get(&sst, (size_t)L * stb, false));
+  HIPCHK(c, B.get(&sa, (size_t)L * sa_stride * 8, false));
+  HIPCHK(c, B.get(&sar, (size_t)L * sa_stride * 8, false));
+  HIPCHK(c, B.get(&sst, (size_t)L * st_stride, false));
   HIPCHK(c, B.get(&sscal, (size_t)L * sizeof(Scal), false));
   {
     // element and corner of every slot, for the ordered gather of corner adjoints
@@ -1812,54 +1819,81 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
   ag.mat = c->shmat;
   ag.prop = c->shprop;
   ag.k = c->shk;
+  // the hand-written reverse pass covers Belytschko-Tsay with radial return
+  const bool rev = !c->qeph() && c->shmat.ipla == 0 && c->adj_vjp == 0;
   const int nseg = (int)A->ck.size();
   std::vector<Scal> hs(L);
   for (int sgi = nseg - 1; sgi >= 0; --sgi) {
     const int k0 = sgi * A->every, len = std::min(A->every, A->ncycle - k0);
-    // recompute the segment, storing each cycle's inputs and accelerations
+    // recompute the segment: before each cycle one launch stashes its inputs (X, V,
+    // VR, element state) and the previous cycle's accelerations and scalars
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, A->ck[sgi]->restore(c));
     drop_graphs(c);
     c->cycle_write_a = 1;
     hipError_t e = hipSuccess;
+    auto stash = [&](int j, bool inputs) {
+      const void *src[7];
+      void *dst[7];
+      size_t bytes[7];
+      int n = 0;
+      auto add = [&](const void *a, void *b, size_t nb) {
+        src[n] = a; dst[n] = b; bytes[n] = nb; ++n;
+      };
+      if (inputs) {
+        add(c->X4, sx + (size_t)j * nn, nn * 32);
+        add(c->V4, sv + (size_t)j * nn, nn * 32);
+        add(c->VR4, svr + (size_t)j * nn, nn * 32);
+        add(c->state, sst + (size_t)j * st_stride, stb);
+      }
+      if (j > 0 || !inputs) {
+        const int jp = inputs ? j - 1 : j;
+        add(c->A, sa + (size_t)jp * sa_stride, 3 * nn * 8);
+        add(c->AR, sar + (size_t)jp * sa_stride, 3 * nn * 8);
+        // the scalars the previous cycle wrote sit in the current parity's slot
+        add(&c->ctl->s[c->host_ncycle & 1], sscal + jp, sizeof(Scal));
+      }
+      return launch_adj_stash(n, src, dst, bytes, c->stream);
+    };
     for (int j = 0; j < len && e == hipSuccess; ++j) {
-      const int p = c->host_ncycle & 1;
-      e = hipMemcpyAsync(sx + (size_t)j * nn, c->X4, nn * 32, hipMemcpyDeviceToDevice, c->stream);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(sv + (size_t)j * nn, c->V4, nn * 32, hipMemcpyDeviceToDevice, c->stream);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(svr + (size_t)j * nn, c->VR4, nn * 32, hipMemcpyDeviceToDevice, c->stream);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(sst + (size_t)j * stb, c->state, stb, hipMemcpyDeviceToDevice, c->stream);
-      if (e == hipSuccess) e = enqueue_cycle(c, p);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(sa + (size_t)j * 3 * nn, c->A, 3 * nn * 8, hipMemcpyDeviceToDevice, c->stream);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(sar + (size_t)j * 3 * nn, c->AR, 3 * nn * 8, hipMemcpyDeviceToDevice, c->stream);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(sscal + j, &c->ctl->s[p ^ 1], sizeof(Scal), hipMemcpyDeviceToDevice, c->stream);
+      e = stash(j, true);
+      if (e == hipSuccess) e = enqueue_cycle(c, c->host_ncycle & 1);
       c->host_ncycle += 1;
     }
+    if (e == hipSuccess) e = stash(len - 1, false);
     c->cycle_write_a = 0;
     HIPCHK(c, e);
     HIPCHK(c, hipMemcpyAsync(hs.data(), sscal, len * sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    // reverse through the segment
+    // reverse through the segment: the nodal reverse of the segment's last cycle, then
+    // per cycle the element VJP and the gather, fused with the nodal reverse of the
+    // cycle before it inside the segment
+    auto slotX = [&](int j) { return sx + (size_t)j * nn; };
+    auto slotV = [&](int j) { return sv + (size_t)j * nn; };
+    auto slotW = [&](int j) { return svr + (size_t)j * nn; };
+    auto slotA = [&](int j) { return sa + (size_t)j * sa_stride; };
+    auto slotAR = [&](int j) { return sar + (size_t)j * sa_stride; };
+    HIPCHK(c, launch_adj_node(c->numnod, slotX(len - 1), slotV(len - 1), slotW(len - 1),
+                              slotA(len - 1), slotAR(len - 1), hs[len - 1].dt2, hs[len - 1].dt12,
+                              lx, lv, ld, lvr, lamS, lamR, gms, gin, c->stream));
     for (int j = len - 1; j >= 0; --j) {
-      const double4 *X = sx + (size_t)j * nn, *V = sv + (size_t)j * nn, *W = svr + (size_t)j * nn;
-      HIPCHK(c, launch_adj_node(c->numnod, X, V, W, sa + (size_t)j * 3 * nn,
-                                sar + (size_t)j * 3 * nn, hs[j].dt2, hs[j].dt12, lx, lv, ld, lvr,
-                                lamS, lamR, gms, gin, c->stream));
-      ag.X4 = X;
-      ag.V4 = V;
-      ag.VR4 = W;
-      ag.st = reinterpret_cast<const double *>(sst + (size_t)j * stb);
+      ag.X4 = slotX(j);
+      ag.V4 = slotV(j);
+      ag.VR4 = slotW(j);
+      ag.st = reinterpret_cast<const double *>(sst + (size_t)j * st_stride);
       ag.lst_in = lst[cur];
       ag.lst_out = lst[cur ^ 1];
       ag.dt1 = hs[j].dt1;
-      HIPCHK(c, launch_adj_elem(c->precision == ORAD_MIXED, c->qeph(), ag, c->stream));
-      HIPCHK(c, launch_adj_gather(c->numnod, c->numelc, c->adsky, selem, scorner, lcorner,
-                                  lx, lv, lvr, c->stream));
+      HIPCHK(c, launch_adj_elem(c->precision == ORAD_MIXED, c->qeph(), rev, ag, c->stream));
+      if (j > 0) {
+        HIPCHK(c, launch_adj_gather_node(c->numnod, c->numelc, c->adsky, selem, scorner, lcorner,
+                                         slotX(j - 1), slotV(j - 1), slotW(j - 1), slotA(j - 1),
+                                         slotAR(j - 1), hs[j - 1].dt2, hs[j - 1].dt12, lx, lv, ld,
+                                         lvr, lamS, lamR, gms, gin, c->stream));
+      } else {
+        HIPCHK(c, launch_adj_gather(c->numnod, c->numelc, c->adsky, selem, scorner, lcorner,
+                                    lx, lv, lvr, c->stream));
+      }
       cur ^= 1;
     }
   }

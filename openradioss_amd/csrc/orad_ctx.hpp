@@ -55,13 +55,24 @@ int qeph_batch_occupancy_mixed(int);
 int shell_batch_occupancy_fp64(int);
 int shell_batch_occupancy_mixed(int);
 template <typename S> struct AdjArgs;
-hipError_t launch_adj_elem(bool, bool, const AdjArgs<double> &, hipStream_t);
+// fp32 state, QEPH, hand-written reverse (Belytschko-Tsay, IPLA = 0) instead of duals
+hipError_t launch_adj_elem(bool, bool, bool, const AdjArgs<double> &, hipStream_t);
 hipError_t launch_adj_node(int, const double4 *, const double4 *, const double4 *,
                            const double *, const double *, double, double, const double *,
                            double *, const double *, double *, double *, double *, double *,
                            double *, hipStream_t);
 hipError_t launch_adj_gather(int, int, const int32_t *, const int32_t *, const int32_t *,
                              const double *, double *, double *, double *, hipStream_t);
+hipError_t launch_adj_gather_node(int numnod, int numelc, const int32_t *adsky,
+                                  const int32_t *slot_elem, const int32_t *slot_corner,
+                                  const double *lcorner, const double4 *X4, const double4 *V4,
+                                  const double4 *VR4, const double *A, const double *AR,
+                                  double dt2, double dt12, double *lx, double *lv,
+                                  const double *ld, double *lvr, double *lamS, double *lamR,
+                                  double *g_ms, double *g_in, hipStream_t s);
+// up to 7 device-to-device copies (sizes multiples of 4 bytes) in one launch
+hipError_t launch_adj_stash(int n, const void *const *src, void *const *dst,
+                            const size_t *bytes, hipStream_t s);
 }  // namespace orad
 
 struct AdjStore;   // checkpoints of an adjoint forward run (orad_api.hip)
@@ -162,6 +173,7 @@ struct orad_ctx {
   int elem_block = 0;       // bricks: element workgroup size (0: the form's default)
   int elem_waves = 0;       // bricks: waves per SIMD of the element-kernel instance (0: default)
   int node_tile = 0;        // nodal kernel: slots per LDS tile (0: default)
+  int adj_vjp = 0;          // shell adjoint: 0 reverse pass where available, 1 duals
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
   size_t ctl_bytes() const { return (size_t)nmodel * sizeof(CycleCtl); }

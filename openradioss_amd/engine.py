@@ -54,9 +54,9 @@ class Balance(C.Structure):
 
 class Launch(C.Structure):
     """orad_launch_t: launch configuration (element-kernel instance, residency cap, XCD
-    mapping, brick element order, nodal slot tile)."""
+    mapping, brick element order, nodal slot tile, element VJP of the shell adjoint)."""
     _fields_ = [(k, C.c_int) for k in ("persistent", "lds_kb", "xcd_remap", "grid", "block",
-                                       "waves", "order_rows", "node_tile")]
+                                       "waves", "order_rows", "node_tile", "adj_vjp")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -213,9 +213,10 @@ class BrickSolver:
 
     # ---- reference-shaped operations ----
     def set_launch(self, persistent=0, lds_kb=-1, xcd_remap=1, block=0, waves=0, order_rows=-1,
-                   node_tile=0, grid=0):
-        """orad_set_launch; order_rows=-1 keeps the current brick element order."""
-        l = Launch(persistent, lds_kb, xcd_remap, 0, block, waves, order_rows, node_tile)
+                   node_tile=0, grid=0, adj_vjp=0):
+        """orad_set_launch; order_rows=-1 keeps the current brick element order; adj_vjp=1
+        makes the shell adjoint use forward-mode dual numbers for the element VJP."""
+        l = Launch(persistent, lds_kb, xcd_remap, 0, block, waves, order_rows, node_tile, adj_vjp)
         self._chk(self.lib.orad_set_launch(self.h, C.byref(l)))
 
     def launch(self):

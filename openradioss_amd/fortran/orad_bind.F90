@@ -25,6 +25,7 @@ module orad_bind
   type, bind(c) :: orad_launch_t
     integer(c_int) :: persistent, lds_kb, xcd_remap, grid
     integer(c_int) :: block, waves, order_rows, node_tile
+    integer(c_int) :: adj_vjp
   end type orad_launch_t
 
   interface

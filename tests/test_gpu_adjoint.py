@@ -25,7 +25,12 @@ Bars:
     to 5x away from both the adjoint and the h = 1e-6 quotients);
   * QEPH (Ishell=24): the same short runs at the same bar; over 2000 cycles a
     twisted elastic plate (test_adjoint_qeph_plate_128_vs_fd says why);
-  * FP32-element forward with the FP64 adjoint vs the all-FP64 gradient: printed.
+  * FP32-element forward with the FP64 adjoint vs the all-FP64 gradient: printed;
+  * the hand-written reverse pass of the Belytschko-Tsay element (the default) vs
+    the forward-mode dual-number VJP (adj_vjp=1) on the same forward run: every
+    output within 1e-10 of the largest of its kind (16 x 16 with plasticity,
+    triangles and random adjoints of the whole final state, both forms; the
+    128 x 128 adjoint study).
 """
 import numpy as np
 import pytest
@@ -245,6 +250,54 @@ def test_adjoint_qeph_plate_128_vs_fd():
     for h, e in errs.items():
         print(f"h={h:g}: max relative error over {len(e)} thicknesses {max(e):.2e}")
         assert max(e) < 1e-3
+
+
+def reverse_vs_duals(deck, prec, ncyc, every, dtmx, seeds):
+    """adjoint_reverse outputs under the reverse pass (adj_vjp=0) and the duals (1)."""
+    outs = []
+    for vjp in (0, 1):
+        s = en.ShellSolver(deck, prec)
+        s.set_time(dtmx=dtmx)
+        s.set_launch(adj_vjp=vjp)
+        assert s.launch()["adj_vjp"] == vjp
+        s.adjoint_forward(ncyc, every)
+        outs.append(s.adjoint_reverse(**seeds))
+        s.close()
+    return outs
+
+
+def assert_same_gradients(a, b, bar, what):
+    for k in a:
+        scale = np.abs(b[k]).max()
+        err = np.abs(a[k] - b[k]).max() / max(scale, 1e-300)
+        print(f"{what} {k}: max |reverse - duals| / max|duals| = {err:.2e} (scale {scale:.3e})",
+              flush=True)
+        assert err < bar, (what, k, err)
+
+
+@pytest.mark.parametrize("prec", [en.FP64, en.MIXED])
+def test_reverse_pass_matches_dual_vjp(prec):
+    deck, centre, wf, dtmx = setup_plate(n=16, patch=6, ca_scale=0.5)
+    ixc = deck["ixc"].copy()
+    for e in (3, 100, 201):
+        ixc[e, 3] = ixc[e, 2]                  # three triangles (corner 4 = corner 3)
+    deck["ixc"] = ixc
+    nn, ne = deck["x"].shape[0], ixc.shape[0]
+    rng = np.random.default_rng(11)
+    seeds = dict(lam_x=rng.standard_normal((nn, 3)), lam_v=1e-4 * rng.standard_normal((nn, 3)),
+                 lam_vr=1e-4 * rng.standard_normal((nn, 3)), lam_d=rng.standard_normal((nn, 3)),
+                 lam_state=1e-3 * rng.standard_normal((ne, 33)))
+    rev, dual = reverse_vs_duals(deck, prec, 60, 7, dtmx, seeds)
+    assert np.all(rev["g_jc"] != 0.0)          # points yielded
+    assert_same_gradients(rev, dual, 1e-10, f"16x16 prec {prec}")
+
+
+def test_reverse_pass_matches_dual_vjp_plate_128():
+    deck, centre, wf, dtmx = setup_plate(n=128, patch=32)
+    lam_d = np.zeros((deck["x"].shape[0], 3))
+    lam_d[centre, 2] = 1.0
+    rev, dual = reverse_vs_duals(deck, en.FP64, 2000, 100, dtmx, dict(lam_d=lam_d))
+    assert_same_gradients(rev, dual, 1e-10, "128x128 2000 cycles")
 
 
 def test_adjoint_refuses_rate_dependent_law_and_second_backward():
