@@ -24,6 +24,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+import torch  # noqa: E402
+
+from openradioss_amd import autograd as ag  # noqa: E402
 from openradioss_amd import deck as dk  # noqa: E402
 from openradioss_amd import engine as en  # noqa: E402
 
@@ -125,44 +128,118 @@ def shell_plate_leg(n, steps, err_cycles, ishell=1):
     return res
 
 
-def adjoint_leg(n=128, ncyc=2000, every=100, ishell=1):
-    """Adjoint of an n x n plate run (thickness x (1 + U(-0.1, 0.1)) per element,
-    central patch impact, ncyc cycles, checkpoint every `every`), FP64: times a plain
-    forward run, the checkpointed forward run and the reverse sweep (the gradient of
-    the centre node's final z displacement w.r.t. every thickness, A, B, n, masses,
-    inertias and initial X, V, VR).  The time step is pinned by DTMX at 0.8 x the
-    initial element time step, as the adjoint assumes a fixed time-step sequence."""
+def _plate_objective(deck, A, B, cn, thk, prec, ncyc, dtmx, centre):
+    """J = centre z displacement + plastic work of one forward run (no adjoint)."""
+    x = torch.from_numpy(deck["x"])
+    ixc = torch.from_numpy(deck["ixc"].astype(np.int64))
+    t = torch.from_numpy(thk)
+    ms, inr, area0 = ag.cinmas(x, ixc, t, deck["mat"]["rho0"])
+    d = dict(deck, thk=thk, ms=ms.numpy(), **{"in": inr.numpy()})
+    d["mat"] = dict(deck["mat"], ca=A, cb=B, cn=cn)
+    s = en.ShellSolver(d, prec)
+    s.set_time(dtmx=dtmx)
+    s.run_cycles(ncyc)
+    nd, st = s.nodes(), s.state()
+    s.close()
+    pla = torch.from_numpy(st[:, en.SH_PLA0:en.SH_PLA0 + 5].copy())
+    f = lambda v: torch.tensor(v, dtype=torch.float64)
+    W = ag.plastic_work(f(A), f(B), f(cn), t, area0, pla, deck["prop"]["npt"])
+    return float(nd["d"][centre, 2]) + float(W)
+
+
+def adjoint_leg(n=128, ncyc=2000, every=100, ishell=1, fd_params=20):
+    """The adjoint study on an n x n plate (thickness x (1 + U(-0.1, 0.1)) per element,
+    central patch impact, ncyc cycles, checkpoint every `every`): J = the centre node's
+    final z displacement + total plastic work, FP32-element forward with the FP64
+    adjoint, gradient w.r.t. A, B, n and every thickness (masses and inertias follow the
+    thicknesses).  Times a plain FP32 forward run, the checkpointed forward run and the
+    reverse sweep, and checks fd_params sampled parameters (A, B, n and thicknesses)
+    against central differences of FP64 forward runs at h = 1e-4 and 1e-6 relative.
+    The time step is pinned by DTMX at 0.8 x the initial element time step, as the
+    adjoint assumes a fixed time-step sequence."""
     deck = dk.shell_plate(n=n, patch=n // 4, thk_noise=0.1, seed=20261203)
     deck["prop"] = dict(deck["prop"], ishell=ishell)
     s = en.ShellSolver(deck, en.FP64)
     dtmx = 0.8 * s.cforc3(0, s.numelc, 0.0)
     s.close()
+    centre = (n // 2) * (n + 2)
     times = {}
-    s = en.ShellSolver(deck, en.FP64)
-    s.set_time(dtmx=dtmx)
+    for key, prec in (("forward_ms", en.MIXED), ("forward_fp64_ms", en.FP64)):
+        s = en.ShellSolver(deck, prec)
+        s.set_time(dtmx=dtmx)
+        s.run_cycles(10)                       # returns after the cycles have run
+        t0 = time.perf_counter()
+        s.run_cycles(ncyc)
+        times[key] = 1e3 * (time.perf_counter() - t0)
+        s.close()
+    m = deck["mat"]
+    A = torch.tensor(m["ca"], dtype=torch.float64, requires_grad=True)
+    B = torch.tensor(m["cb"], dtype=torch.float64, requires_grad=True)
+    cn = torch.tensor(m["cn"], dtype=torch.float64, requires_grad=True)
+    thk = torch.tensor(deck["thk"], dtype=torch.float64, requires_grad=True)
     t0 = time.perf_counter()
-    s.run_cycles(ncyc)
-    times["forward_ms"] = 1e3 * (time.perf_counter() - t0)
-    s.close()
-    s = en.ShellSolver(deck, en.FP64)
-    s.set_time(dtmx=dtmx)
-    t0 = time.perf_counter()
-    s.adjoint_forward(ncyc, every)
-    times["checkpointed_forward_ms"] = 1e3 * (time.perf_counter() - t0)
-    lam_d = np.zeros((s.numnod, 3))
-    lam_d[(n // 2) * (n + 2), 2] = 1.0                 # centre node, z
-    t0 = time.perf_counter()
-    g = s.adjoint_reverse(lam_d=lam_d)
-    times["reverse_ms"] = 1e3 * (time.perf_counter() - t0)
-    s.close()
+    J = ag.displacement_plus_plastic_work(deck, A, B, cn, thk, en.MIXED, ncyc, every, dtmx, centre)
+    t1 = time.perf_counter()
+    J.backward()
+    t2 = time.perf_counter()
+    times["checkpointed_forward_ms"] = 1e3 * (t1 - t0)
+    times["reverse_ms"] = 1e3 * (t2 - t1)
+    g = [float(A.grad), float(B.grad), float(cn.grad)]
+    gthk = thk.grad.numpy()
     res = dict(workload=f"{'QEPH' if ishell == 24 else 'BT'} shell plate {n}x{n} "
                         f"({n * n} shells), thickness +-10%, {ncyc} cycles, checkpoint every "
-                        f"{every}, gradient of the centre node's final z displacement",
-               shells=n * n, cycles=ncyc, every=every, **times,
+                        f"{every}, J = centre node final z displacement + plastic work, "
+                        f"FP32-element forward, FP64 adjoint",
+               shells=n * n, cycles=ncyc, every=every, J=float(J.detach()), **times,
                reverse_over_forward=times["reverse_ms"] / times["forward_ms"],
-               g_jc=[float(v) for v in g["g_jc"]])
-    print(f"adjoint: forward {times['forward_ms']:.0f} ms, checkpointed forward "
-          f"{times['checkpointed_forward_ms']:.0f} ms, reverse {times['reverse_ms']:.0f} ms",
+               reverse_over_fp64_forward=times["reverse_ms"] / times["forward_fp64_ms"],
+               g_jc=g)
+    if fd_params:
+        # the same gradient from an FP64 forward run, for the FD comparison
+        th = [torch.tensor(v, dtype=torch.float64, requires_grad=True)
+              for v in (m["ca"], m["cb"], m["cn"], deck["thk"])]
+        J64 = ag.displacement_plus_plastic_work(deck, *th, en.FP64, ncyc, every, dtmx, centre)
+        J64.backward()
+        grads = {"fp32_forward": (g, gthk),
+                 "fp64_forward": ([float(t.grad) for t in th[:3]], th[3].grad.numpy())}
+        rng = np.random.default_rng(20261203)
+        ne = n * n
+        patch = np.flatnonzero(np.abs(deck["x"][deck["ixc"][:, 0], 0] - 0.5 * n) < 18.0)
+        k = fd_params - 3
+        elems = list(rng.choice(patch, (k + 1) // 2, replace=False)) + \
+            list(rng.choice(ne, k // 2, replace=False))
+        base = [m["ca"], m["cb"], m["cn"], deck["thk"].copy()]
+        hs = (1e-4, 1e-6)
+        errs = {(key, h): [] for key in grads for h in hs}
+        t0 = time.perf_counter()
+        for which, idx in [(0, 0), (1, 0), (2, 0)] + [(3, int(e)) for e in elems]:
+            for h in hs:
+                vals = []
+                for sgn in (1.0, -1.0):
+                    th = [base[0], base[1], base[2], base[3].copy()]
+                    if which < 3:
+                        step = h * abs(base[which])
+                        th[which] = base[which] + sgn * step
+                    else:
+                        step = h * abs(base[3][idx])
+                        th[3][idx] = base[3][idx] + sgn * step
+                    vals.append(_plate_objective(deck, th[0], th[1], th[2], th[3], en.FP64, ncyc,
+                                                 dtmx, centre))
+                gf = (vals[0] - vals[1]) / (2.0 * step)
+                for key, (gj, gt) in grads.items():
+                    gadj = gj[which] if which < 3 else float(gt[idx])
+                    errs[(key, h)].append(abs(gadj - gf) / max(abs(gf), 1e-30))
+        res["fd_check"] = dict(
+            parameters=fd_params, fd_forward="FP64", seconds=time.perf_counter() - t0,
+            **{f"{key}_h={h:g}": dict(max_rel_err=max(e), median_rel_err=float(np.median(e)),
+                                      max_rel_err_A_B_n=max(e[:3]))
+               for (key, h), e in errs.items()})
+    print(f"adjoint: forward (FP32) {times['forward_ms']:.0f} ms, forward (FP64) "
+          f"{times['forward_fp64_ms']:.0f} ms, checkpointed forward "
+          f"{times['checkpointed_forward_ms']:.0f} ms, reverse {times['reverse_ms']:.0f} ms"
+          + (f"; FD h=1e-6 max rel err FP32-fwd "
+             f"{res['fd_check']['fp32_forward_h=1e-06']['max_rel_err']:.2e} FP64-fwd "
+             f"{res['fd_check']['fp64_forward_h=1e-06']['max_rel_err']:.2e}" if fd_params else ""),
           file=sys.stderr, flush=True)
     return res
 
@@ -409,7 +486,7 @@ def main():
     adjoint = adjoint_qeph = None
     if not args.no_adjoint:
         adjoint = adjoint_leg()
-        adjoint_qeph = adjoint_leg(ishell=24)
+        adjoint_qeph = adjoint_leg(ishell=24, fd_params=0)
     copy_gbs = None
     if not args.no_copy:
         copy_gbs = copy_bandwidth()

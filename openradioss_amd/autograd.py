@@ -90,3 +90,39 @@ def shell_run(A, B, n, thk, ms, inr, deck, precision=en.FP64, ncycle=1000, every
               dtmx=None):
     """Functional form of ShellRun.apply."""
     return ShellRun.apply(A, B, n, thk, ms, inr, deck, precision, ncycle, every, dtmx)
+
+
+# COQINI force weights through the thickness (coqini.F:45-121): single-precision DATA
+# literals, widened as the library holds them.
+_WF = {1: (1.0,), 2: (0.5, 0.5), 3: (0.25, 0.5, 0.25),
+       4: (0.1666667, 0.3333333, 0.3333333, 0.1666667), 5: (0.125, 0.25, 0.25, 0.25, 0.125)}
+
+
+def force_weights(npt):
+    """The npt points' force weights WF as float64 (widened from float32)."""
+    return np.array(_WF[npt], dtype=np.float32).astype(np.float64)
+
+
+def plastic_work(A, B, n, thk, area0, pla, npt):
+    """Total plastic work of law 2 with a hardening curve A + B pla^n: per point
+    V_pt (A pla + B pla^(n+1) / (n+1)), V_pt = WF_pt * thickness * initial area.
+    pla (NUMELC, 5) tensor; the points past npt hold zero."""
+    pos = pla > 0
+    safe = torch.where(pos, pla, torch.ones_like(pla))
+    w = torch.where(pos, A * pla + B * safe ** (n + 1.0) / (n + 1.0), torch.zeros_like(pla))
+    wf = np.zeros(5)
+    wf[:npt] = force_weights(npt)
+    vpt = torch.from_numpy(wf)[None, :] * (thk * area0)[:, None]
+    return (vpt * w).sum()
+
+
+def displacement_plus_plastic_work(deck, A, B, n, thk, precision, ncycle, every, dtmx, node,
+                                   axis=2):
+    """J = final displacement of `node` along `axis` + total plastic work after `ncycle`
+    cycles of a shell deck, as a torch function of A, B, n and the thicknesses (masses
+    and inertias follow the thicknesses through cinmas); J.backward() runs the adjoint."""
+    x = torch.from_numpy(deck["x"])
+    ixc = torch.from_numpy(deck["ixc"].astype(np.int64))
+    ms, inr, area0 = cinmas(x, ixc, thk, deck["mat"]["rho0"])
+    d, pla = ShellRun.apply(A, B, n, thk, ms, inr, deck, precision, ncycle, every, dtmx)
+    return d[node, axis] + plastic_work(A, B, n, thk, area0, pla, deck["prop"]["npt"])

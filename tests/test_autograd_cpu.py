@@ -2,6 +2,7 @@
 builder's NumPy CINMAS (cinmas.F:1294-1296,1322), and its thickness derivative against
 the closed form d(MS)/d(t) = sum over the node's shells of rho*A/4."""
 import numpy as np
+import pytest
 import torch
 
 from openradioss_amd import deck as dk
@@ -34,3 +35,28 @@ def test_cinmas_thickness_derivative_closed_form():
         if node in d["ixc"][e]:
             expect[e] = rho * area[e].item() / 4.0
     np.testing.assert_allclose(thk.grad.numpy(), expect, rtol=1e-14)
+
+
+def test_force_weights_match_the_oracle_tables():
+    import pyoracle as po
+    from openradioss_amd.autograd import force_weights
+    for npt in range(1, 6):
+        assert np.array_equal(force_weights(npt), po.shell_tables(npt)[1]), npt
+
+
+def test_plastic_work_closed_form_and_gradient():
+    """One element, two points: W = sum V_pt (A p + B p^(n+1)/(n+1)), dW/dp = V_pt (A + B p^n)."""
+    from openradioss_amd.autograd import plastic_work
+    A = torch.tensor(200.0, dtype=torch.float64, requires_grad=True)
+    B = torch.tensor(450.0, dtype=torch.float64, requires_grad=True)
+    n = torch.tensor(0.5, dtype=torch.float64, requires_grad=True)
+    thk = torch.tensor([1.2], dtype=torch.float64)
+    area = torch.tensor([4.0], dtype=torch.float64)
+    pla = torch.tensor([[0.01, 0.0, 0.0, 0.0, 0.0]], dtype=torch.float64, requires_grad=True)
+    W = plastic_work(A, B, n, thk, area, pla, 2)
+    V = 0.5 * 1.2 * 4.0
+    assert float(W) == pytest.approx(V * (200 * 0.01 + 450 * 0.01 ** 1.5 / 1.5), rel=1e-14)
+    W.backward()
+    assert float(pla.grad[0, 0]) == pytest.approx(V * (200 + 450 * 0.01 ** 0.5), rel=1e-14)
+    assert float(pla.grad[0, 1]) == 0.0
+    assert float(A.grad) == pytest.approx(V * 0.01, rel=1e-14)
