@@ -46,8 +46,8 @@ typedef struct {
  * workgroup, minimum waves per SIMD its registers are allocated for; mixed form
  * 256/2 (default), 256/3, 128/2, 64/2; FP64 form 64/1 (default), 128/1, 256/1,
  * 64/2; 0 = default); order_rows -- the library's element order, columns of that
- * many element rows walked layer by layer (1..128; 0 = the engine's order, -1 =
- * keep the current one).  Every family: node_tile -- PARITH/ON slots the nodal
+ * many element rows walked layer by layer (1..128, default 10; 0 = the engine's order,
+ * -1 = keep the current one).  Every family: node_tile -- PARITH/ON slots the nodal
  * kernel stages per LDS tile (1024, 2048, 4096 within 48 KB; 0 = default).
  * Shells: adj_vjp -- the element vector-Jacobian product of orad_adjoint_reverse:
  * 0 = the hand-written reverse pass where the formulation has one (Belytschko-Tsay
@@ -261,7 +261,7 @@ int orad_set_launch(orad_ctx *ctx, const orad_launch_t *l);
 int orad_get_launch(orad_ctx *ctx, orad_launch_t *l);
 /* Times `cycles` whole cycles under each candidate configuration from the current
  * state, restores the state after every trial, and keeps the fastest; *best_ms = its
- * time per cycle.  Bricks: element order (engine, 8, 16, 32 rows), then element
+ * time per cycle.  Bricks: element order (engine, 8, 10, 16, 32 rows), then element
  * instance (block, waves) with its LDS cap, XCD mapping and the nodal slot tile,
  * searched one axis at a time from the best so far; batched shells: LDS cap x XCD
  * mapping x persistent tiles, then the slot tile; one shell model: XCD mapping and

@@ -1594,7 +1594,7 @@ int orad_autotune(orad_ctx *c, int cycles, orad_launch_t *best, double *best_ms)
   if (c->family == FAM_BRICK) {
     // 1. element order: the state in the snapshot is permuted along with the live one
     int best_rows = c->order_rows;
-    for (int rows : {0, 8, 16, 32}) {
+    for (int rows : {0, 8, 10, 16, 32}) {
       r = brick_reorder(c, rows, {{snap.state_copy(), snap.sti_copy()}});
       if (r) return r;
       orad_launch_t l = lbest;

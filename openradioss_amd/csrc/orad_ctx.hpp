@@ -122,7 +122,7 @@ struct orad_ctx {
   bool have_entot0 = false;
 
   // bricks: the library's element order (orad_brick_order.hip)
-  int order_rows = 16;                // column height in element rows; 0 = engine order
+  int order_rows = 10;                // column height in element rows; 0 = engine order
   std::vector<int32_t> h_perm;        // library element -> engine element
   std::vector<int32_t> h_iperm;       // engine element -> library element
   std::vector<unsigned long long> h_bin;  // centroid bins, engine order

@@ -71,8 +71,8 @@ def test_default_order_is_columns_and_groups_follow_the_engine_numbering():
     numbered in the engine's order and must give the slots of the whole launch."""
     d = dk.brick_block(19, 14, 9, v_top=-5.0e4)
     a = en.BrickSolver(d, en.FP64)
-    assert a.launch()["order_rows"] == 16
-    a.set_launch(order_rows=3)            # 14 rows: the default would be one column
+    assert a.launch()["order_rows"] == 10
+    a.set_launch(order_rows=3)            # 14 rows in columns of 3
     t = a.run_cycles(30)
     b = en.BrickSolver(d, en.FP64)
     b.set_launch(order_rows=0)
