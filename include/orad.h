@@ -49,9 +49,9 @@ typedef struct {
  * many element rows walked layer by layer (1..128, default 10; 0 = the engine's order,
  * -1 = keep the current one).  Every family: node_tile -- PARITH/ON slots the nodal
  * kernel stages per LDS tile (1024, 2048, 4096 within 48 KB; 0 = default).
- * Shells: adj_vjp -- the element vector-Jacobian product of orad_adjoint_reverse:
- * 0 = the hand-written reverse pass where the formulation has one (Belytschko-Tsay
- * with law 2 radial return, IPLA = 0), forward-mode dual numbers otherwise; 1 =
+ * adj_vjp -- the element vector-Jacobian product of orad_adjoint_reverse: 0 = the
+ * hand-written reverse pass where the element has one (bricks; Belytschko-Tsay with
+ * the law-2 radial return, IPLA = 0), forward-mode dual numbers otherwise; 1 =
  * forward-mode dual numbers always (both give the same derivative to rounding).
  * orad_get_launch reports the values in use.  Nodes, slots and element state do
  * not depend on any of these; the element order changes only the rounding of the
@@ -268,23 +268,26 @@ int orad_get_launch(orad_ctx *ctx, orad_launch_t *l);
  * slot tile. */
 int orad_autotune(orad_ctx *ctx, int cycles, orad_launch_t *best, double *best_ms);
 
-/* Adjoint of a run of one shell model (no reference counterpart: the reference's
- * only derivative is the implicit consistent tangent of law 2, m2law.F:487-531,
- * m2cplr.F:448-469).  orad_adjoint_forward runs ncycle cycles from the current
- * state as orad_run_cycles does, keeping a checkpoint every `every` cycles.
+/* Adjoint of a run of one shell or brick model (no reference counterpart: the
+ * reference's only derivative is the implicit consistent tangent of law 2,
+ * m2law.F:487-531, m2cplr.F:448-469).  orad_adjoint_forward runs ncycle cycles from
+ * the current state as orad_run_cycles does, keeping a checkpoint every `every` cycles.
  * orad_adjoint_reverse takes the adjoints of the final state -- lam_x, lam_v,
- * lam_vr, lam_d as X(3,NUMNOD) arrays, lam_state as NUMELC rows of 33 for
- * Ishell=1 (SIG(5) of points 1..5, PLA(1..5), HOUR(1:3)) or 42 for QEPH (SIG, PLA,
- * HOURG(1:12)); null = zero -- and, recomputing
- * each segment from its checkpoint, returns the gradients with respect to the
- * law-2 A, B, n (g_jc[3]), the element thicknesses THKE (g_thk[NUMELC]), the
- * nodal masses and rotational inertias (g_ms, g_in [NUMNOD]), and the adjoints of
- * the initial X, V, VR (any output may be null).  The time-step sequence is taken
- * as fixed (exact when DTMX pins it).  The element derivatives are those of the
- * FP64 chain evaluated on the forward states, in FP64 for either form; the context
- * is left at the end of the forward run.  Law 2 without the strain-rate term only
- * (C = 0; ORAD_EINVAL otherwise): the rate filter EPSDP and the thickness THK that
- * the rate factor reads are not part of the adjoint state. */
+ * lam_vr, lam_d as X(3,NUMNOD) arrays, lam_state as one row per element in the
+ * engine's numbering: shells 33 for Ishell=1 (SIG(5) of points 1..5, PLA(1..5),
+ * HOUR(1:3)) or 42 for QEPH (SIG, PLA, HOURG(1:12)), bricks 7 (SIG(6), PLA); null =
+ * zero -- and, recomputing each segment from its checkpoint, returns the gradients
+ * with respect to the law-2 A, B, n (g_jc[3]), the shell thicknesses THKE
+ * (g_thk[NUMELC]; not touched for bricks), the nodal masses and rotational inertias
+ * (g_ms, g_in [NUMNOD]; g_in zero for bricks), and the adjoints of the initial X, V,
+ * VR (lam_vr0 zero for bricks); any output may be null.  The time-step sequence is
+ * taken as fixed (exact when DTMX pins it).  The element derivatives are those of the
+ * FP64 chain evaluated on the forward states, in FP64 for either form: bricks and
+ * Belytschko-Tsay shells with IPLA = 0 by a hand-written reverse pass, QEPH and IPLA
+ * 1/2 by dual numbers (orad_launch_t.adj_vjp); the context is left at the end of the
+ * forward run.  Law 2 without the strain-rate term only (C = 0; ORAD_EINVAL
+ * otherwise): the rate filters (EPSD, EPSDP) and the shell thickness THK that the
+ * rate factor reads are not part of the adjoint state. */
 int orad_adjoint_forward(orad_ctx *ctx, int ncycle, int every);
 int orad_adjoint_reverse(orad_ctx *ctx, const double *lam_x, const double *lam_v,
                          const double *lam_vr, const double *lam_d, const double *lam_state,

@@ -32,6 +32,7 @@
 #include "orad_dual.hpp"
 #include "orad_qeph_chain.hpp"
 #include "orad_shell_adjoint.hpp"
+#include "orad_brick_adjoint.hpp"
 #include "orad_brick_io.hpp"
 #include "orad_adjoint_args.hpp"
 
@@ -214,11 +215,135 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_rev_kernel(AdjArgs<S> a) {
   for (int j = 0; j < 3; ++j) a.g_jc[(size_t)j * ne + e] += gjc[j];
 }
 
+// Bricks, state columns: SIG(6), then EINT, RHO, PLA, QVIS, OFF, EPSD.
+constexpr int BC_EINT = 6, BC_RHO = 7, BC_PLA = 8, BC_QVIS = 9, BC_OFFG = 10, BC_EPSD = 11;
+
+// One brick per thread through the hand-written reverse pass (orad_brick_adjoint.hpp).
+template <typename S>
+__global__ __launch_bounds__(ADJ_BLOCK) void adj_brick_rev_kernel(AdjBrickArgs<S> a) {
+  const int ne = a.numels;
+  const int e = blockIdx.x * ADJ_BLOCK + threadIdx.x;
+  if (e >= ne) return;
+  int nd[8];
+  double x[3][8], v[3][8], lx[3][8], lv[3][8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    nd[k] = a.conn[(size_t)k * ne + e];
+    const double4 p = a.X4[nd[k]], q = a.V4[nd[k]];
+    x[0][k] = p.x; x[1][k] = p.y; x[2][k] = p.z;
+    v[0][k] = q.x; v[1][k] = q.y; v[2][k] = q.z;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) lx[d][k] = lv[d][k] = 0.0;
+  }
+  const S *st = a.st;
+  auto col = [&](int c) { return double(st[(size_t)c * ne + e]); };
+  double sig[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) sig[i] = col(i);
+  const double jc[3] = {a.mat.ca, a.mat.cb, a.mat.cn};
+  double gjc[3] = {0.0, 0.0, 0.0};
+  const double *lin = a.lst_in, *lamS = a.lamS;
+  double *lout = a.lst_out;
+  brick_vjp(x, v, sig, col(BC_PLA), col(BC_RHO), col(BC_OFFG), a.vol0[e], a.dt1, a.mat, jc,
+            a.prop, [&](int j) { return lin[(size_t)j * ne + e]; },
+            [&](int k, int i) { return lamS[3 * (size_t)nd[k] + i]; },
+            [&](int j, double val) { lout[(size_t)j * ne + e] = val; }, lx, lv, gjc);
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      a.lcorner[(size_t)(6 * k + d) * ne + e] = lx[d][k];
+      a.lcorner[(size_t)(6 * k + 3 + d) * ne + e] = lv[d][k];
+    }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) a.g_jc[(size_t)j * ne + e] += gjc[j];
+}
+
+// The same VJP by forward-mode dual numbers, K inputs per thread: the brick chain
+// (FP64 form) instantiated on Dual<K>.
+template <typename S, int K>
+__global__ __launch_bounds__(ADJ_BLOCK) void adj_brick_dual_kernel(AdjBrickArgs<S> a,
+                                                                   int nchunk) {
+  typedef Dual<K> D;
+  const int ne = a.numels;
+  const long idx = (long)blockIdx.x * ADJ_BLOCK + threadIdx.x;
+  if (idx >= (long)ne * nchunk) return;
+  const int cl = (int)(idx / ne), e = (int)(idx - (long)cl * ne);
+  const int i0 = cl * K;
+  auto seed = [&](D &u, int i) {
+#pragma unroll
+    for (int t = 0; t < K; ++t) u.d[t] = (i - i0 == t) ? 1.0 : 0.0;
+  };
+  int nd[8];
+  D xd[8], yd[8], zd[8], vx[8], vy[8], vz[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    nd[k] = a.conn[(size_t)k * ne + e];
+    const double4 p = a.X4[nd[k]], q = a.V4[nd[k]];
+    xd[k] = D(p.x); yd[k] = D(p.y); zd[k] = D(p.z);
+    vx[k] = D(q.x); vy[k] = D(q.y); vz[k] = D(q.z);
+    seed(xd[k], 6 * k); seed(yd[k], 6 * k + 1); seed(zd[k], 6 * k + 2);
+    seed(vx[k], 6 * k + 3); seed(vy[k], 6 * k + 4); seed(vz[k], 6 * k + 5);
+  }
+  const S *st = a.st;
+  auto col = [&](int c) { return D(double(st[(size_t)c * ne + e])); };
+  D sig[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) { sig[i] = col(i); seed(sig[i], 48 + i); }
+  D pla = col(BC_PLA);
+  seed(pla, 54);
+  D jc[3] = {D(a.mat.ca), D(a.mat.cb), D(a.mat.cn)};
+#pragma unroll
+  for (int j = 0; j < 3; ++j) seed(jc[j], ADJB_I_JC + j);
+  D eint = col(BC_EINT), rho = col(BC_RHO), qvis = col(BC_QVIS), offg = col(BC_OFFG);
+  D epsd = col(BC_EPSD);
+  double acc[K];
+#pragma unroll
+  for (int t = 0; t < K; ++t) acc[t] = 0.0;
+  const double *lamS = a.lamS;
+  BrickOut<D> o;
+  brick_chain<D, D *>(xd, yd, zd, vx, vy, vz, sig, eint, rho, a.vol0[e], 0.0, pla, qvis, offg,
+                      epsd, D(a.dt1), a.mat, jc, a.prop, o,
+                      [&](int k, const D &fx, const D &fy, const D &fz) {
+                        const size_t n3 = 3 * (size_t)nd[k];
+#pragma unroll
+                        for (int t = 0; t < K; ++t)
+                          acc[t] += lamS[n3] * fx.d[t] + lamS[n3 + 1] * fy.d[t] +
+                                    lamS[n3 + 2] * fz.d[t];
+                      });
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    double s2 = a.lst_in[(size_t)6 * ne + e] * pla.d[t];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) s2 += a.lst_in[(size_t)i * ne + e] * sig[i].d[t];
+    acc[t] += s2;
+    const int i = i0 + t;
+    if (i < 48) a.lcorner[(size_t)i * ne + e] = acc[t];
+    else if (i < ADJB_I_JC) a.lst_out[(size_t)(i - 48) * ne + e] = acc[t];
+    else if (i < ADJB_NIN) a.g_jc[(size_t)(i - ADJB_I_JC) * ne + e] += acc[t];
+  }
+}
+
+template <typename S>
+static void launch_adj_brick_form(bool rev, const AdjBrickArgs<S> &a, hipStream_t s) {
+  if (rev) {
+    const int nb = (a.numels + ADJ_BLOCK - 1) / ADJ_BLOCK;
+    hipLaunchKernelGGL(adj_brick_rev_kernel<S>, dim3(nb), dim3(ADJ_BLOCK), 0, s, a);
+  } else {
+    const int nchunk = (ADJB_NIN + 1) / 2;
+    const long n = (long)a.numels * nchunk;
+    const int nb = (int)((n + ADJ_BLOCK - 1) / ADJ_BLOCK);
+    hipLaunchKernelGGL((adj_brick_dual_kernel<S, 2>), dim3(nb), dim3(ADJ_BLOCK), 0, s, a,
+                       nchunk);
+  }
+}
+
 // Reverse of the nodal update of one cycle for node n.  In: lx, lv, ld, lvr = adjoints
 // of x, v, d, vr after the cycle; A, AR = the cycle's accelerations (after BCS10).
 // Out: lv, lvr = adjoints of v_k, vr_k through the update (element terms are added
 // by the gather), lamS / lamR = adjoints of the slot sums, MS / IN sensitivities
-// accumulated into g_ms / g_in.
+// accumulated into g_ms / g_in.  Without rotational DOF (bricks) VR4 is null and
+// lvr, lamR, g_in are not touched.
 struct AdjNodeArgs {
   const double4 *X4, *V4, *VR4;
   const double *A, *AR;
@@ -231,7 +356,8 @@ __device__ __forceinline__ void adj_node_one(int n, const AdjNodeArgs &q, const 
                                              double *g_in) {
   const int code = word_code(q.V4[n].w);
   const int ct = code_icodt(code), cr = code_icodr(code), fd = code_fvdir(code);
-  const double ms = q.X4[n].w, in = q.VR4[n].w;
+  const bool rot = q.VR4 != nullptr;
+  const double ms = q.X4[n].w, in = rot ? q.VR4[n].w : 0.0;
   const double dt2 = q.dt2, dt12 = q.dt12;
   double gm = 0.0, gi = 0.0;
 #pragma unroll
@@ -249,6 +375,7 @@ __device__ __forceinline__ void adj_node_one(int n, const AdjNodeArgs &q, const 
     }
     lv[j] = lvn;
     lamS[j] = ls;
+    if (!rot) continue;
     const double ur = lvr[j];
     const bool rfixed = (cr & (4 >> i)) != 0;
     double lr = 0.0, lvrn = 0.0;
@@ -263,7 +390,7 @@ __device__ __forceinline__ void adj_node_one(int n, const AdjNodeArgs &q, const 
     lamR[j] = lr;
   }
   g_ms[n] += gm;
-  g_in[n] += gi;
+  if (rot) g_in[n] += gi;
 }
 
 __global__ __launch_bounds__(256) void adj_node_kernel(int numnod, AdjNodeArgs q,
@@ -276,31 +403,33 @@ __global__ __launch_bounds__(256) void adj_node_kernel(int numnod, AdjNodeArgs q
   adj_node_one(n, q, lx, lv, ld, lvr, lamS, lamR, g_ms, g_in);
 }
 
-// Element corner contributions to the adjoints of node n, in slot order.
-__device__ __forceinline__ void adj_gather_one(int n, int numelc, const int32_t *adsky,
+// Element corner contributions to the adjoints of node n, in slot order.  `per`
+// inputs per corner: 9 for shells (x, v, vr), 6 for bricks (x, v; lvr untouched).
+__device__ __forceinline__ void adj_gather_one(int n, int numelc, int per, const int32_t *adsky,
                                                const int32_t *slot_elem,
                                                const int32_t *slot_corner,
                                                const double *lcorner, double *lx, double *lv,
                                                double *lvr) {
   double ax[3] = {0, 0, 0}, av[3] = {0, 0, 0}, ar[3] = {0, 0, 0};
+  const bool rot = per == 9;
   for (int s = adsky[n]; s < adsky[n + 1]; ++s) {
     const int e = slot_elem[s], c = slot_corner[s];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      ax[i] += lcorner[(size_t)(9 * c + i) * numelc + e];
-      av[i] += lcorner[(size_t)(9 * c + 3 + i) * numelc + e];
-      ar[i] += lcorner[(size_t)(9 * c + 6 + i) * numelc + e];
+      ax[i] += lcorner[(size_t)(per * c + i) * numelc + e];
+      av[i] += lcorner[(size_t)(per * c + 3 + i) * numelc + e];
+      if (rot) ar[i] += lcorner[(size_t)(per * c + 6 + i) * numelc + e];
     }
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     lx[3 * (size_t)n + i] += ax[i];
     lv[3 * (size_t)n + i] += av[i];
-    lvr[3 * (size_t)n + i] += ar[i];
+    if (rot) lvr[3 * (size_t)n + i] += ar[i];
   }
 }
 
-__global__ __launch_bounds__(256) void adj_gather_kernel(int numnod, int numelc,
+__global__ __launch_bounds__(256) void adj_gather_kernel(int numnod, int numelc, int per,
                                                          const int32_t *adsky,
                                                          const int32_t *slot_elem,
                                                          const int32_t *slot_corner,
@@ -308,18 +437,18 @@ __global__ __launch_bounds__(256) void adj_gather_kernel(int numnod, int numelc,
                                                          double *lv, double *lvr) {
   const int n = blockIdx.x * 256 + threadIdx.x;
   if (n >= numnod) return;
-  adj_gather_one(n, numelc, adsky, slot_elem, slot_corner, lcorner, lx, lv, lvr);
+  adj_gather_one(n, numelc, per, adsky, slot_elem, slot_corner, lcorner, lx, lv, lvr);
 }
 
 // The gather of cycle k followed, node by node, by the nodal reverse of cycle k-1
 // (both touch only node n's entries, so one thread does both in order).
 __global__ __launch_bounds__(256) void adj_gather_node_kernel(
-    int numnod, int numelc, const int32_t *adsky, const int32_t *slot_elem,
+    int numnod, int numelc, int per, const int32_t *adsky, const int32_t *slot_elem,
     const int32_t *slot_corner, const double *lcorner, AdjNodeArgs q, double *lx, double *lv,
     const double *ld, double *lvr, double *lamS, double *lamR, double *g_ms, double *g_in) {
   const int n = blockIdx.x * 256 + threadIdx.x;
   if (n >= numnod) return;
-  adj_gather_one(n, numelc, adsky, slot_elem, slot_corner, lcorner, lx, lv, lvr);
+  adj_gather_one(n, numelc, per, adsky, slot_elem, slot_corner, lcorner, lx, lv, lvr);
   adj_node_one(n, q, lx, lv, ld, lvr, lamS, lamR, g_ms, g_in);
 }
 
@@ -396,6 +525,23 @@ hipError_t launch_adj_elem(bool fp32_state, bool qeph, bool rev, const AdjArgs<d
   return hipGetLastError();
 }
 
+hipError_t launch_adj_brick_elem(bool fp32_state, bool rev, const AdjBrickArgs<double> &a64,
+                                 hipStream_t s) {
+  if (a64.numels <= 0) return hipSuccess;
+  if (fp32_state) {
+    AdjBrickArgs<float> a;
+    a.numels = a64.numels; a.conn = a64.conn; a.X4 = a64.X4; a.V4 = a64.V4;
+    a.st = reinterpret_cast<const float *>(a64.st);
+    a.vol0 = a64.vol0; a.lamS = a64.lamS; a.lst_in = a64.lst_in; a.lst_out = a64.lst_out;
+    a.lcorner = a64.lcorner; a.g_jc = a64.g_jc; a.mat = a64.mat; a.prop = a64.prop;
+    a.dt1 = a64.dt1;
+    launch_adj_brick_form(rev, a, s);
+  } else {
+    launch_adj_brick_form(rev, a64, s);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_adj_node(int numnod, const double4 *X4, const double4 *V4, const double4 *VR4,
                            const double *A, const double *AR, double dt2, double dt12,
                            const double *lx, double *lv, const double *ld, double *lvr,
@@ -409,18 +555,18 @@ hipError_t launch_adj_node(int numnod, const double4 *X4, const double4 *V4, con
   return hipGetLastError();
 }
 
-hipError_t launch_adj_gather(int numnod, int numelc, const int32_t *adsky,
+hipError_t launch_adj_gather(int numnod, int numelc, int per, const int32_t *adsky,
                              const int32_t *slot_elem, const int32_t *slot_corner,
                              const double *lcorner, double *lx, double *lv, double *lvr,
                              hipStream_t s) {
   const int nb = (numnod + 255) / 256;
   if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL(adj_gather_kernel, dim3(nb), dim3(256), 0, s, numnod, numelc, adsky,
+  hipLaunchKernelGGL(adj_gather_kernel, dim3(nb), dim3(256), 0, s, numnod, numelc, per, adsky,
                      slot_elem, slot_corner, lcorner, lx, lv, lvr);
   return hipGetLastError();
 }
 
-hipError_t launch_adj_gather_node(int numnod, int numelc, const int32_t *adsky,
+hipError_t launch_adj_gather_node(int numnod, int numelc, int per, const int32_t *adsky,
                                   const int32_t *slot_elem, const int32_t *slot_corner,
                                   const double *lcorner, const double4 *X4, const double4 *V4,
                                   const double4 *VR4, const double *A, const double *AR,
@@ -430,7 +576,7 @@ hipError_t launch_adj_gather_node(int numnod, int numelc, const int32_t *adsky,
   const int nb = (numnod + 255) / 256;
   if (nb <= 0) return hipSuccess;
   const AdjNodeArgs q{X4, V4, VR4, A, AR, dt2, dt12};
-  hipLaunchKernelGGL(adj_gather_node_kernel, dim3(nb), dim3(256), 0, s, numnod, numelc, adsky,
+  hipLaunchKernelGGL(adj_gather_node_kernel, dim3(nb), dim3(256), 0, s, numnod, numelc, per, adsky,
                      slot_elem, slot_corner, lcorner, q, lx, lv, ld, lvr, lamS, lamR, g_ms, g_in);
   return hipGetLastError();
 }

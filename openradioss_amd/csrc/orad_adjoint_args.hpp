@@ -50,4 +50,25 @@ struct AdjArgs {
   double dt1;
 };
 
+// Brick element inputs with an adjoint, in this order: 8 corners x (x, y, z, vx, vy,
+// vz) = 48, the carried state that reaches the forces (SIG(6), PLA) = 7, then A, B, n.
+constexpr int ADJB_NST = 7, ADJB_I_JC = 55, ADJB_NIN = 58;
+
+template <typename S>
+struct AdjBrickArgs {
+  int numels;
+  const int32_t *conn;     // [8][ne], library element order
+  const double4 *X4, *V4;
+  const S *st;             // [12][ne] pre-cycle element state: SIG(6) EINT RHO PLA QVIS OFF EPSD
+  const double *vol0;
+  const double *lamS;      // [3 * numnod] adjoints of the slot sums
+  const double *lst_in;    // [7][ne] adjoint of the post-cycle SIG, PLA
+  double *lst_out;         // [7][ne] adjoint of the pre-cycle SIG, PLA
+  double *lcorner;         // [48][ne] corner k, 6k + (x, y, z, vx, vy, vz)
+  double *g_jc;            // [3][ne]
+  Law2 mat;
+  SProp prop;
+  double dt1;
+};
+
 }  // namespace orad

@@ -1704,15 +1704,16 @@ extern "C" {
 
 int orad_adjoint_forward(orad_ctx *c, int ncycle, int every) {
   if (!c) return ORAD_EINVAL;
-  if (c->family != FAM_SHELL || c->batched)
-    return fail(c, ORAD_ESTATE, "adjoint_forward: one shell model only");
+  if ((c->family != FAM_SHELL && c->family != FAM_BRICK) || c->batched)
+    return fail(c, ORAD_ESTATE, "adjoint_forward: one shell or brick model only");
   int r = ready(c);
   if (r) return r;
   if (ncycle < 1 || every < 1) return fail(c, ORAD_EINVAL, "adjoint_forward: bad cycle counts");
-  // The adjoint state carries SIG, PLA and the hourglass forces.  The strain-rate
-  // filter EPSDP and the thickness THK are also carried from cycle to cycle, but
-  // they reach the forces only through the Johnson-Cook rate factor, i.e. when C != 0.
-  if (c->shmat.cc != 0.0)
+  // The adjoint state carries SIG, PLA and (shells) the hourglass forces.  The
+  // strain-rate filters (EPSD, EPSDP) and the shell thickness THK are also carried from
+  // cycle to cycle, but they reach the forces only through the Johnson-Cook rate
+  // factor, i.e. when C != 0.
+  if ((c->family == FAM_SHELL ? c->shmat.cc : c->mat.cc) != 0.0)
     return fail(c, ORAD_EINVAL, "adjoint_forward: rate-dependent law 2 (C != 0) not differentiated");
   HIPCHK(c, hipSetDevice(c->device));
   free_adjoint(c);
@@ -1748,9 +1749,12 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   AdjStore *A = c->adj;
-  const size_t nn = (size_t)c->numnod, ne = (size_t)c->numelc, ts = c->tsize();
-  const size_t stb = (size_t)c->shncol() * ne * ts;
-  const int nst = c->qeph() ? AdjShape<true>::NST : AdjShape<false>::NST;
+  const bool brick = c->family == FAM_BRICK;
+  const size_t nn = (size_t)c->numnod, ts = c->tsize();
+  const size_t ne = brick ? (size_t)c->numels : (size_t)c->numelc;
+  const size_t stb = (size_t)(brick ? NSTATE : c->shncol()) * ne * ts;
+  const int nst = brick ? ADJB_NST : c->qeph() ? AdjShape<true>::NST : AdjShape<false>::NST;
+  const int nco = brick ? 8 : 4, per = brick ? 6 : 9;   // corners, inputs per corner
   const int L = A->every;
   DevBufs B;
   double *lx, *lv, *ld, *lvr, *lamS, *lamR, *gms, *gin, *lst[2], *lcorner, *gthk, *gjc;
@@ -1769,41 +1773,43 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
   HIPCHK(c, B.get(&gin, nn * 8));
   HIPCHK(c, B.get(&lst[0], (size_t)nst * ne * 8));
   HIPCHK(c, B.get(&lst[1], (size_t)nst * ne * 8));
-  HIPCHK(c, B.get(&lcorner, 36 * ne * 8));
+  HIPCHK(c, B.get(&lcorner, (size_t)nco * per * ne * 8));
   HIPCHK(c, B.get(&gthk, ne * 8));
   HIPCHK(c, B.get(&gjc, 3 * ne * 8));
-  HIPCHK(c, B.get(&scorner, 4 * ne * 4, false));
-  HIPCHK(c, B.get(&selem, 4 * ne * 4, false));
+  HIPCHK(c, B.get(&scorner, (size_t)nco * ne * 4, false));
+  HIPCHK(c, B.get(&selem, (size_t)nco * ne * 4, false));
   // per-cycle stash slots, strides rounded to 256 bytes so every slot is 16-byte aligned
   const size_t sa_stride = (3 * nn + 31) / 32 * 32;        // doubles
   const size_t st_stride = (stb + 255) / 256 * 256;        // bytes
   HIPCHK(c, B.get(&sx, (size_t)L * nn * 32, false));
   HIPCHK(c, B.get(&sv, (size_t)L * nn * 32, false));
-  HIPCHK(c, B.get(&svr, (size_t)L * nn * 32, false));
+  HIPCHK(c, B.get(&svr, (size_t)L * nn * 32 * (brick ? 0 : 1) + 256, false));
   HIPCHK(c, B.get(&sa, (size_t)L * sa_stride * 8, false));
-  HIPCHK(c, B.get(&sar, (size_t)L * sa_stride * 8, false));
+  HIPCHK(c, B.get(&sar, (size_t)L * sa_stride * 8 * (brick ? 0 : 1) + 256, false));
   HIPCHK(c, B.get(&sst, (size_t)L * st_stride, false));
   HIPCHK(c, B.get(&sscal, (size_t)L * sizeof(Scal), false));
+  // device element index of an engine element (bricks are stored in the library's order)
+  auto dev_elem = [&](size_t e) { return brick ? (size_t)c->h_iperm[e] : e; };
   {
     // element and corner of every slot, for the ordered gather of corner adjoints
-    std::vector<int32_t> sc(4 * ne), se(4 * ne);
+    std::vector<int32_t> sc((size_t)nco * ne), se((size_t)nco * ne);
     for (size_t e = 0; e < ne; ++e)
-      for (int k = 0; k < 4; ++k) {
-        sc[c->h_iads[4 * e + k]] = k;
-        se[c->h_iads[4 * e + k]] = (int32_t)e;
+      for (int k = 0; k < nco; ++k) {
+        sc[c->h_iads[(size_t)nco * e + k]] = k;
+        se[c->h_iads[(size_t)nco * e + k]] = (int32_t)dev_elem(e);
       }
-    HIPCHK(c, hipMemcpy(scorner, sc.data(), 4 * ne * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(selem, se.data(), 4 * ne * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(scorner, sc.data(), (size_t)nco * ne * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(selem, se.data(), (size_t)nco * ne * 4, hipMemcpyHostToDevice));
   }
   if (lam_x) HIPCHK(c, hipMemcpy(lx, lam_x, 3 * nn * 8, hipMemcpyHostToDevice));
   if (lam_v) HIPCHK(c, hipMemcpy(lv, lam_v, 3 * nn * 8, hipMemcpyHostToDevice));
-  if (lam_vr) HIPCHK(c, hipMemcpy(lvr, lam_vr, 3 * nn * 8, hipMemcpyHostToDevice));
+  if (lam_vr && !brick) HIPCHK(c, hipMemcpy(lvr, lam_vr, 3 * nn * 8, hipMemcpyHostToDevice));
   if (lam_d) HIPCHK(c, hipMemcpy(ld, lam_d, 3 * nn * 8, hipMemcpyHostToDevice));
   if (lam_state) {
     // rows of nst (SIG, PLA, then HOUR(1:3) or HOURG(1:12)) -> columns
     std::vector<double> col((size_t)nst * ne);
     for (size_t e = 0; e < ne; ++e)
-      for (int j = 0; j < nst; ++j) col[j * ne + e] = lam_state[(size_t)nst * e + j];
+      for (int j = 0; j < nst; ++j) col[j * ne + dev_elem(e)] = lam_state[(size_t)nst * e + j];
     HIPCHK(c, hipMemcpy(lst[0], col.data(), (size_t)nst * ne * 8, hipMemcpyHostToDevice));
   }
   int cur = 0;
@@ -1819,8 +1825,17 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
   ag.mat = c->shmat;
   ag.prop = c->shprop;
   ag.k = c->shk;
-  // the hand-written reverse pass covers Belytschko-Tsay with radial return
-  const bool rev = !c->qeph() && c->shmat.ipla == 0 && c->adj_vjp == 0;
+  AdjBrickArgs<double> ab{};
+  ab.numels = c->numels;
+  ab.conn = c->conn;
+  ab.vol0 = c->vol0;
+  ab.lamS = lamS;
+  ab.lcorner = lcorner;
+  ab.g_jc = gjc;
+  ab.mat = c->mat;
+  ab.prop = c->prop;
+  // the hand-written reverse pass covers bricks and Belytschko-Tsay with radial return
+  const bool rev = c->adj_vjp == 0 && (brick || (!c->qeph() && c->shmat.ipla == 0));
   const int nseg = (int)A->ck.size();
   std::vector<Scal> hs(L);
   for (int sgi = nseg - 1; sgi >= 0; --sgi) {
@@ -1843,13 +1858,13 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
       if (inputs) {
         add(c->X4, sx + (size_t)j * nn, nn * 32);
         add(c->V4, sv + (size_t)j * nn, nn * 32);
-        add(c->VR4, svr + (size_t)j * nn, nn * 32);
+        if (!brick) add(c->VR4, svr + (size_t)j * nn, nn * 32);
         add(c->state, sst + (size_t)j * st_stride, stb);
       }
       if (j > 0 || !inputs) {
         const int jp = inputs ? j - 1 : j;
         add(c->A, sa + (size_t)jp * sa_stride, 3 * nn * 8);
-        add(c->AR, sar + (size_t)jp * sa_stride, 3 * nn * 8);
+        if (!brick) add(c->AR, sar + (size_t)jp * sa_stride, 3 * nn * 8);
         // the scalars the previous cycle wrote sit in the current parity's slot
         add(&c->ctl->s[c->host_ncycle & 1], sscal + jp, sizeof(Scal));
       }
@@ -1870,28 +1885,40 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
     // cycle before it inside the segment
     auto slotX = [&](int j) { return sx + (size_t)j * nn; };
     auto slotV = [&](int j) { return sv + (size_t)j * nn; };
-    auto slotW = [&](int j) { return svr + (size_t)j * nn; };
+    // no rotational DOF for bricks: the nodal reverse skips them on null VR4 / AR
+    auto slotW = [&](int j) { return brick ? (const double4 *)nullptr : svr + (size_t)j * nn; };
     auto slotA = [&](int j) { return sa + (size_t)j * sa_stride; };
-    auto slotAR = [&](int j) { return sar + (size_t)j * sa_stride; };
+    auto slotAR = [&](int j) { return brick ? (const double *)nullptr : sar + (size_t)j * sa_stride; };
     HIPCHK(c, launch_adj_node(c->numnod, slotX(len - 1), slotV(len - 1), slotW(len - 1),
                               slotA(len - 1), slotAR(len - 1), hs[len - 1].dt2, hs[len - 1].dt12,
                               lx, lv, ld, lvr, lamS, lamR, gms, gin, c->stream));
     for (int j = len - 1; j >= 0; --j) {
-      ag.X4 = slotX(j);
-      ag.V4 = slotV(j);
-      ag.VR4 = slotW(j);
-      ag.st = reinterpret_cast<const double *>(sst + (size_t)j * st_stride);
-      ag.lst_in = lst[cur];
-      ag.lst_out = lst[cur ^ 1];
-      ag.dt1 = hs[j].dt1;
-      HIPCHK(c, launch_adj_elem(c->precision == ORAD_MIXED, c->qeph(), rev, ag, c->stream));
+      const double *stj = reinterpret_cast<const double *>(sst + (size_t)j * st_stride);
+      if (brick) {
+        ab.X4 = slotX(j);
+        ab.V4 = slotV(j);
+        ab.st = stj;
+        ab.lst_in = lst[cur];
+        ab.lst_out = lst[cur ^ 1];
+        ab.dt1 = hs[j].dt1;
+        HIPCHK(c, launch_adj_brick_elem(c->precision == ORAD_MIXED, rev, ab, c->stream));
+      } else {
+        ag.X4 = slotX(j);
+        ag.V4 = slotV(j);
+        ag.VR4 = slotW(j);
+        ag.st = stj;
+        ag.lst_in = lst[cur];
+        ag.lst_out = lst[cur ^ 1];
+        ag.dt1 = hs[j].dt1;
+        HIPCHK(c, launch_adj_elem(c->precision == ORAD_MIXED, c->qeph(), rev, ag, c->stream));
+      }
       if (j > 0) {
-        HIPCHK(c, launch_adj_gather_node(c->numnod, c->numelc, c->adsky, selem, scorner, lcorner,
+        HIPCHK(c, launch_adj_gather_node(c->numnod, (int)ne, per, c->adsky, selem, scorner, lcorner,
                                          slotX(j - 1), slotV(j - 1), slotW(j - 1), slotA(j - 1),
                                          slotAR(j - 1), hs[j - 1].dt2, hs[j - 1].dt12, lx, lv, ld,
                                          lvr, lamS, lamR, gms, gin, c->stream));
       } else {
-        HIPCHK(c, launch_adj_gather(c->numnod, c->numelc, c->adsky, selem, scorner, lcorner,
+        HIPCHK(c, launch_adj_gather(c->numnod, (int)ne, per, c->adsky, selem, scorner, lcorner,
                                     lx, lv, lvr, c->stream));
       }
       cur ^= 1;
@@ -1901,18 +1928,25 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
   if (g_jc) {
     std::vector<double> h(3 * ne);
     HIPCHK(c, hipMemcpy(h.data(), gjc, 3 * ne * 8, hipMemcpyDeviceToHost));
+    // summed in the engine's element order, whatever the device order
     for (int j = 0; j < 3; ++j) {
       double acc = 0.0;
-      for (size_t e2 = 0; e2 < ne; ++e2) acc += h[j * ne + e2];
+      for (size_t e2 = 0; e2 < ne; ++e2) acc += h[j * ne + dev_elem(e2)];
       g_jc[j] = acc;
     }
   }
-  if (g_thk) HIPCHK(c, hipMemcpy(g_thk, gthk, ne * 8, hipMemcpyDeviceToHost));
+  if (g_thk && !brick) HIPCHK(c, hipMemcpy(g_thk, gthk, ne * 8, hipMemcpyDeviceToHost));
   if (g_ms) HIPCHK(c, hipMemcpy(g_ms, gms, nn * 8, hipMemcpyDeviceToHost));
-  if (g_in) HIPCHK(c, hipMemcpy(g_in, gin, nn * 8, hipMemcpyDeviceToHost));
+  if (g_in) {
+    if (brick) std::fill(g_in, g_in + nn, 0.0);
+    else HIPCHK(c, hipMemcpy(g_in, gin, nn * 8, hipMemcpyDeviceToHost));
+  }
   if (lam_x0) HIPCHK(c, hipMemcpy(lam_x0, lx, 3 * nn * 8, hipMemcpyDeviceToHost));
   if (lam_v0) HIPCHK(c, hipMemcpy(lam_v0, lv, 3 * nn * 8, hipMemcpyDeviceToHost));
-  if (lam_vr0) HIPCHK(c, hipMemcpy(lam_vr0, lvr, 3 * nn * 8, hipMemcpyDeviceToHost));
+  if (lam_vr0) {
+    if (brick) std::fill(lam_vr0, lam_vr0 + 3 * nn, 0.0);
+    else HIPCHK(c, hipMemcpy(lam_vr0, lvr, 3 * nn * 8, hipMemcpyDeviceToHost));
+  }
   // leave the context at the end of the forward run
   HIPCHK(c, A->fin->restore(c));
   drop_graphs(c);

@@ -8,7 +8,10 @@
 // file instantiating it is compiled without FMA contraction).  T = float is the
 // mixed form: the caller passes coordinates and velocities relative to corner 1,
 // formed in FP64, so every later difference is exact to FP32 rounding of O(h).
+// Any other T (a dual number, orad_dual.hpp) takes the FP64 form's steps.
 #pragma once
+#include <type_traits>
+
 #include "orad_common.hpp"
 #include "orad_math.hpp"
 
@@ -67,14 +70,20 @@ struct Col {
   __device__ __forceinline__ T &operator[](int k) const { return p[k * STRIDE]; }
 };
 
+// jc = (A, B, n) of the hardening curve (mat.ca, mat.cb, mat.cn; separate so that an
+// adjoint can seed them).
 template <typename T, typename VA, typename Emit>
 __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA vy, VA vz,
                                             T sig[6], T &st_eint, T &st_rho, double vol0d,
                                             double voldp, T &st_pla,
                                             T &st_qvis, T &st_offg, T &st_epsd, T dt1,
-                                            const Law2 &mat, const SProp &prop,
+                                            const Law2 &mat, const T jc[3], const SProp &prop,
                                             BrickOut<T> &out, Emit emit) {
+  constexpr bool FP32 = std::is_same<T, float>::value;   // the mixed form's factored steps
   const T ZERO = T(0), ONE = T(1), TWO = T(2), THREE = T(3), FOUR = T(4), HALF = T(0.5);
+  const T THIRD = T(K<double>::THIRD), ONEP333 = T(K<double>::ONEP333);
+  const T EM15 = T(K<double>::EM15), EM20 = T(K<double>::EM20);
+  const T ONE_OVER_64 = T(K<double>::ONE_OVER_64);
   T offg = st_offg;
   int mstop = 0;
 
@@ -166,9 +175,9 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
     T JAC_59_68 = JAC5 * JAC9 - JAC6 * JAC8;
     T JAC_67_49 = JAC6 * JAC7 - JAC4 * JAC9;
     T JAC_48_57 = JAC4 * JAC8 - JAC5 * JAC7;
-    T det = (sizeof(T) == sizeof(double))
-                ? K<T>::ONE_OVER_64 * (JAC1 * JAC_59_68 + JAC2 * JAC_67_49 + JAC3 * JAC_48_57)
-                : T(voldp);
+    T det;
+    if constexpr (!FP32) det = ONE_OVER_64 * (JAC1 * JAC_59_68 + JAC2 * JAC_67_49 + JAC3 * JAC_48_57);
+    else det = T(voldp);
     if (off == ZERO) {
       det = ONE;
     } else if (offg > ONE) {
@@ -177,7 +186,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
       det = ONE; off = ZERO; offg = ZERO; mstop = 1;
     }
     voln = det;
-    T DETT = M<T>::div(K<T>::ONE_OVER_64, det);
+    T DETT = M<T>::div(ONE_OVER_64, det);
     T JACI1 = DETT * JAC_59_68, JACI4 = DETT * JAC_67_49, JACI7 = DETT * JAC_48_57;
     T JACI2 = DETT * (JAC3 * JAC8 - JAC2 * JAC9);
     T JACI5 = DETT * (JAC1 * JAC9 - JAC3 * JAC7);
@@ -213,7 +222,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
   // ---- SDLEN3 / SLEN (sdlen3.F:109-167, slen.F:67-87) ----
   T deltax;
   {
-    T aream = K<T>::EM20;
+    T aream = EM20;
     const int fc[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4},
                           {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
 #pragma unroll
@@ -265,7 +274,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
   const T rho0 = T(mat.rho0);
   const T vol0 = T(vol0d);
   T dvol, rho, amu;
-  if (sizeof(T) == sizeof(double)) {
+  if constexpr (!FP32) {
     if (offg == ZERO && voln == ONE) voln = vol0;
     dvol = voln - (rho0 / st_rho) * vol0;
     rho = rho0 * (vol0 / voln);
@@ -287,10 +296,10 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
   const T G = T(mat.shear) * off;
   const T bulk = T(mat.bulk);
   T SIGMX = T(mat.sigmx);
-  T P = -K<T>::THIRD * (sig[0] + sig[1] + sig[2]);
-  T DAV = -K<T>::THIRD * (DXX + DYY + DZZ);
+  T P = -THIRD * (sig[0] + sig[1] + sig[2]);
+  T DAV = -THIRD * (DXX + DYY + DZZ);
   T G1 = dt1 * G, G2 = TWO * G1;
-  T SSP = M<T>::sqrt_(M<T>::div(K<T>::ONEP333 * G + bulk, rho0));
+  T SSP = M<T>::sqrt_(M<T>::div(ONEP333 * G + bulk, rho0));
   sig[0] = sig[0] + P + G2 * (DXX + DAV);
   sig[1] = sig[1] + P + G2 * (DYY + DAV);
   sig[2] = sig[2] + P + G2 * (DZZ + DAV);
@@ -306,7 +315,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
     T E4 = HALF * D4, E5 = HALF * D5, E6 = HALF * D6;
     EPSDOT = M<T>::sqrt_(DXX * DXX + DYY * DYY + DZZ * DZZ + TWO * (E4 * E4 + E5 * E5 + E6 * E6));
   } else {
-    T dav = (DXX + DYY + DZZ) * K<T>::THIRD;
+    T dav = (DXX + DYY + DZZ) * THIRD;
     T E1 = DXX - dav, E2 = DYY - dav, E3 = DZZ - dav;
     T E4 = HALF * D4, E5 = HALF * D5, E6 = HALF * D6;
     T EPSP = HALF * (E1 * E1 + E2 * E2 + E3 * E3) + E4 * E4 + E5 * E5 + E6 * E6;
@@ -320,7 +329,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
   // Temperature stays at TREF on this path (TSTAR = 0), so 1 - TSTAR**m = 1.
   T EPD = ONE;
   if (mat.cc != 0.0) {
-    EPD = (mat.vp == 1) ? tmax(epsd, T(mat.epdr)) : tmax(epsd, K<T>::EM15);
+    EPD = (mat.vp == 1) ? tmax(epsd, T(mat.epdr)) : tmax(epsd, EM15);
     EPD = M<T>::log_(M<T>::div(EPD, T(mat.epdr)));
     EPD = tmax(ZERO, EPD);
     EPD = (ONE + T(mat.cc) * EPD) * ONE;
@@ -329,7 +338,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
     if (mat.icc == 1) SIGMX = T(mat.sigmx) * EPD;
   }
   T EPXE = st_pla, AK, QH;
-  const T CA = T(mat.ca), CB = T(mat.cb), CN = T(mat.cn);
+  const T CA = jc[0], CB = jc[1], CN = jc[2];
   if (CN == ONE) {
     AK = CA + CB * EPXE;
     QH = CB * EPD;
@@ -343,10 +352,10 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
   AK = AK * EPD;
   if (SIGMX < AK) { AK = SIGMX; QH = ZERO; }
   if (EPXE > T(mat.epmx)) { AK = ZERO; QH = ZERO; }
-  T SCALE = tmin(ONE, M<T>::div(AK, tmax(AJ2, K<T>::EM15)));
-  T DPLA = M<T>::div((ONE - SCALE) * AJ2, tmax(THREE * G + QH, K<T>::EM15));
+  T SCALE = tmin(ONE, M<T>::div(AK, tmax(AJ2, EM15)));
+  T DPLA = M<T>::div((ONE - SCALE) * AJ2, tmax(THREE * G + QH, EM15));
   AK = AK + ONE * DPLA * QH;
-  SCALE = tmin(ONE, M<T>::div(AK, tmax(AJ2, K<T>::EM15)));
+  SCALE = tmin(ONE, M<T>::div(AK, tmax(AJ2, EM15)));
 #pragma unroll
   for (int i = 0; i < 6; ++i) sig[i] = SCALE * sig[i];
   EPXE = EPXE + DPLA;
@@ -363,7 +372,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
   T QAA = QA * QA * AD;
   T QX = QB * SSP + AL * QAA;
   T qnew = rho * AD * AL * (QAA * AL + QB * SSP);
-  T SSP_EQ = tmax(K<T>::EM20, QX + M<T>::sqrt_(QX * QX + CX * CX));
+  T SSP_EQ = tmax(EM20, QX + M<T>::sqrt_(QX * QX + CX * CX));
   T DTX = M<T>::div(deltax, SSP_EQ);
   T STI = ZERO;
   if (!(off == ZERO || offg < ZERO)) {
@@ -390,7 +399,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
     T E5 = D5 * (sold[4] + sig[4]);
     T E6 = D6 * (sold[5] + sig[5]);
     T EINC = vol_avg * (E1 + E2 + E3 + E4 + E5 + E6) * DTA - HALF * dvol * (st_qvis + qnew);
-    eint = M<T>::div(eint + EINC * off, tmax(K<T>::EM15, vol0));
+    eint = M<T>::div(eint + EINC * off, tmax(EM15, vol0));
   }
 
   // ---- SHVIS3 (shvis3.F:163-238,317-428) ----
@@ -404,7 +413,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const VA w = (d == 0) ? vx : (d == 1) ? vy : vz;
-    if (sizeof(T) == sizeof(double)) {
+    if constexpr (!FP32) {
       // FP64 form: the reference's G = Gamma -/+ PxHy vectors and operation order.
 #pragma unroll
       for (int m = 0; m < 3; ++m)
@@ -425,7 +434,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
     HG[d][3] = w[0] - w[1] + w[2] - w[3] - w[4] + w[5] - w[6] + w[7];
 #pragma unroll
     for (int m = 0; m < 4; ++m) HH[d][m] = HG[d][m] * (FCL + ((HG[d][m] < ZERO) ? -HG[d][m] : HG[d][m]) * FCQ);
-    if (sizeof(T) != sizeof(double)) {
+    if constexpr (FP32) {
 #pragma unroll
       for (int a = 0; a < 4; ++a) Q[d][a] = PH[0][a] * HH[d][0] + PH[1][a] * HH[d][1] + PH[2][a] * HH[d][2];
     }
@@ -460,7 +469,7 @@ __device__ __forceinline__ void brick_chain(T xd[8], T yd[8], T zd[8], VA vx, VA
       const T h0 = HH[d][0], h1 = HH[d][1], h2 = HH[d][2];
       const T h4 = (s4[k] > 0) ? HH[d][3] : -HH[d][3];
       T hg;
-      if (sizeof(T) == sizeof(double)) {
+      if constexpr (!FP32) {
         hg = -G_of<T>(0, k, PH) * h0 - G_of<T>(1, k, PH) * h1 - G_of<T>(2, k, PH) * h2 + h4;
       } else {
         const T gh = gamma_of(0, k) * h0 + gamma_of(1, k) * h1 + gamma_of(2, k) * h2;

@@ -110,8 +110,9 @@ void brick_forces_kernel(
     const double vol0 = io.vol0();
     const T offg_in = offg;
     BrickOut<T> o;
+    const T jc[3] = {T(a.mat.ca), T(a.mat.cb), T(a.mat.cn)};
     brick_chain<T, T *>(xd, yd, zd, vx, vy, vz, sig, eint, rho, vol0, voldp, pla, qvis, offg, epsd,
-                        dt1, a.mat, a.prop, o,
+                        dt1, a.mat, jc, a.prop, o,
                         [&](int k, T fx, T fy, T fz) { io.slot_at(sl[k], fx, fy, fz); });
 #pragma unroll
     for (int c = 0; c < 6; ++c) io.set_sig(c, sig[c]);

@@ -55,15 +55,18 @@ int qeph_batch_occupancy_mixed(int);
 int shell_batch_occupancy_fp64(int);
 int shell_batch_occupancy_mixed(int);
 template <typename S> struct AdjArgs;
+template <typename S> struct AdjBrickArgs;
 // fp32 state, QEPH, hand-written reverse (Belytschko-Tsay, IPLA = 0) instead of duals
 hipError_t launch_adj_elem(bool, bool, bool, const AdjArgs<double> &, hipStream_t);
+// bricks: fp32 state, hand-written reverse (else dual numbers)
+hipError_t launch_adj_brick_elem(bool, bool, const AdjBrickArgs<double> &, hipStream_t);
 hipError_t launch_adj_node(int, const double4 *, const double4 *, const double4 *,
                            const double *, const double *, double, double, const double *,
                            double *, const double *, double *, double *, double *, double *,
                            double *, hipStream_t);
-hipError_t launch_adj_gather(int, int, const int32_t *, const int32_t *, const int32_t *,
+hipError_t launch_adj_gather(int, int, int, const int32_t *, const int32_t *, const int32_t *,
                              const double *, double *, double *, double *, hipStream_t);
-hipError_t launch_adj_gather_node(int numnod, int numelc, const int32_t *adsky,
+hipError_t launch_adj_gather_node(int numnod, int numelc, int per, const int32_t *adsky,
                                   const int32_t *slot_elem, const int32_t *slot_corner,
                                   const double *lcorner, const double4 *X4, const double4 *V4,
                                   const double4 *VR4, const double *A, const double *AR,

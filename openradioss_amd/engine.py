@@ -238,6 +238,30 @@ class BrickSolver:
         self._chk(self.lib.orad_get_pon(self.h, _p(adsky), _p(iads)))
         return adsky, iads
 
+    def adjoint_forward(self, ncycle, every):
+        """Run ncycle cycles keeping a checkpoint every `every` cycles."""
+        self._chk(self.lib.orad_adjoint_forward(self.h, ncycle, every))
+
+    def adjoint_nstate(self):
+        """State values with an adjoint per brick: SIG(6), PLA."""
+        return 7
+
+    def adjoint_reverse(self, lam_x=None, lam_v=None, lam_d=None, lam_state=None):
+        """Gradients from the adjoints of the final state: dict with g_jc (dA, dB, dn),
+        g_ms and the adjoints of the initial x, v.  lam_state has 7 values per brick
+        (SIG(6), PLA), in the engine's element order."""
+        nn, ne = self.numnod, self.numels
+        f = lambda a, shape: None if a is None else _f64(a, shape)
+        ins = [f(lam_x, (nn, 3)), f(lam_v, (nn, 3)), None, f(lam_d, (nn, 3)),
+               f(lam_state, (ne, 7))]
+        out = dict(g_jc=np.zeros(3), g_ms=np.zeros(nn), lam_x0=np.zeros((nn, 3)),
+                   lam_v0=np.zeros((nn, 3)))
+        args = [None if a is None else _p(a) for a in ins] + [
+            _p(out["g_jc"]), None, _p(out["g_ms"]), None, _p(out["lam_x0"]), _p(out["lam_v0"]),
+            None]
+        self._chk(self.lib.orad_adjoint_reverse(self.h, *args))
+        return out
+
     def sforc3(self, nft, nel, dt1, dt2t=1.0e30):
         """Element forces of elements nft+1..nft+nel (SFORC3); returns the group DT2T."""
         d = C.c_double(dt2t)
