@@ -244,6 +244,55 @@ def adjoint_leg(n=128, ncyc=2000, every=100, ishell=1, fd_params=20):
     return res
 
 
+def brick_adjoint_leg(n=60, ncyc=1000, every=100):
+    """Adjoint of an n^3 brick block run whose top nodes start downward at 50 m/s:
+    J = final z displacement of a top node + sum of V0 * 1e3 * PLA, gradient w.r.t.
+    A, B, n, the nodal masses and the initial X, V (FP64 adjoint of an FP32-element
+    forward run).  Times a plain forward run of each form, the checkpointed forward run
+    and the reverse sweep; the time step is pinned by DTMX at 0.8 x the initial one."""
+    deck = dk.brick_block(n, n, n, v_top=-5.0e4)
+    s = en.BrickSolver(deck, en.FP64)
+    dtmx = 0.8 * s.sforc3(0, s.numels, 0.0)
+    s.close()
+    times = {}
+    for key, prec in (("forward_ms", en.MIXED), ("forward_fp64_ms", en.FP64)):
+        s = en.BrickSolver(deck, prec)
+        s.set_time(dtmx=dtmx)
+        s.run_cycles(10)
+        t0 = time.perf_counter()
+        s.run_cycles(ncyc)
+        times[key] = 1e3 * (time.perf_counter() - t0)
+        s.close()
+    top = np.flatnonzero(deck["x"][:, 2] > deck["x"][:, 2].max() - 1e-6)
+    node = int(top[len(top) // 2])
+    s = en.BrickSolver(deck, en.MIXED)
+    s.set_time(dtmx=dtmx)
+    t0 = time.perf_counter()
+    s.adjoint_forward(ncyc, every)
+    times["checkpointed_forward_ms"] = 1e3 * (time.perf_counter() - t0)
+    lam_d = np.zeros((s.numnod, 3))
+    lam_d[node, 2] = 1.0
+    lam_state = np.zeros((s.numels, 7))
+    lam_state[:, 6] = deck["vol0"] * 1.0e3
+    t0 = time.perf_counter()
+    g = s.adjoint_reverse(lam_d=lam_d, lam_state=lam_state)
+    times["reverse_ms"] = 1e3 * (time.perf_counter() - t0)
+    s.close()
+    ne = deck["ixs"].shape[0]
+    res = dict(workload=f"brick block {n}^3 ({ne} bricks), top at 50 m/s, {ncyc} cycles, "
+                        f"checkpoint every {every}, J = top node z displacement + "
+                        f"V0-weighted plastic strain, FP32-element forward, FP64 adjoint",
+               bricks=ne, cycles=ncyc, every=every, **times,
+               reverse_over_forward=times["reverse_ms"] / times["forward_ms"],
+               reverse_over_fp64_forward=times["reverse_ms"] / times["forward_fp64_ms"],
+               g_jc=[float(v) for v in g["g_jc"]])
+    print(f"brick adjoint: forward (FP32) {times['forward_ms']:.0f} ms, forward (FP64) "
+          f"{times['forward_fp64_ms']:.0f} ms, checkpointed forward "
+          f"{times['checkpointed_forward_ms']:.0f} ms, reverse {times['reverse_ms']:.0f} ms",
+          file=sys.stderr, flush=True)
+    return res
+
+
 def tube_batch_leg(ntube, steps, solo_cycles, autotune=True):
     """256 independent square tubes (one model each, per-tube A, thickness, size and
     time step), crushed axially at 10 m/s by FIXVEL: all tubes in one batched cycle
@@ -483,10 +532,11 @@ def main():
     if not args.no_tubes:
         tubes = tube_batch_leg(args.tubes, args.tube_steps, args.tube_solo_cycles,
                                autotune=not args.no_autotune)
-    adjoint = adjoint_qeph = None
+    adjoint = adjoint_qeph = adjoint_brick = None
     if not args.no_adjoint:
         adjoint = adjoint_leg()
         adjoint_qeph = adjoint_leg(ishell=24, fd_params=0)
+        adjoint_brick = brick_adjoint_leg()
     copy_gbs = None
     if not args.no_copy:
         copy_gbs = copy_bandwidth()
@@ -533,6 +583,7 @@ def main():
         "tube_batch": tubes,
         "adjoint_plate": adjoint,
         "adjoint_plate_qeph": adjoint_qeph,
+        "adjoint_brick": adjoint_brick,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

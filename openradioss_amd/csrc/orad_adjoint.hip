@@ -225,16 +225,9 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_brick_rev_kernel(AdjBrickArgs<S
   const int e = blockIdx.x * ADJ_BLOCK + threadIdx.x;
   if (e >= ne) return;
   int nd[8];
-  double x[3][8], v[3][8], lx[3][8], lv[3][8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    nd[k] = a.conn[(size_t)k * ne + e];
-    const double4 p = a.X4[nd[k]], q = a.V4[nd[k]];
-    x[0][k] = p.x; x[1][k] = p.y; x[2][k] = p.z;
-    v[0][k] = q.x; v[1][k] = q.y; v[2][k] = q.z;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) lx[d][k] = lv[d][k] = 0.0;
-  }
+  for (int k = 0; k < 8; ++k) nd[k] = a.conn[(size_t)k * ne + e];
+  const double4 *X4 = a.X4, *V4 = a.V4;
   const S *st = a.st;
   auto col = [&](int c) { return double(st[(size_t)c * ne + e]); };
   double sig[6];
@@ -244,17 +237,23 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_brick_rev_kernel(AdjBrickArgs<S
   double gjc[3] = {0.0, 0.0, 0.0};
   const double *lin = a.lst_in, *lamS = a.lamS;
   double *lout = a.lst_out;
-  brick_vjp(x, v, sig, col(BC_PLA), col(BC_RHO), col(BC_OFFG), a.vol0[e], a.dt1, a.mat, jc,
+  brick_vjp([&](int k, double *xk, double *vk) {
+              const double4 p = X4[nd[k]], q = V4[nd[k]];
+              xk[0] = p.x; xk[1] = p.y; xk[2] = p.z;
+              vk[0] = q.x; vk[1] = q.y; vk[2] = q.z;
+            },
+            sig, col(BC_PLA), col(BC_RHO), col(BC_OFFG), a.vol0[e], a.dt1, a.mat, jc,
             a.prop, [&](int j) { return lin[(size_t)j * ne + e]; },
             [&](int k, int i) { return lamS[3 * (size_t)nd[k] + i]; },
-            [&](int j, double val) { lout[(size_t)j * ne + e] = val; }, lx, lv, gjc);
+            [&](int j, double val) { lout[(size_t)j * ne + e] = val; },
+            [&](int k, const double *lxk, const double *lvk) {
 #pragma unroll
-  for (int k = 0; k < 8; ++k)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      a.lcorner[(size_t)(6 * k + d) * ne + e] = lx[d][k];
-      a.lcorner[(size_t)(6 * k + 3 + d) * ne + e] = lv[d][k];
-    }
+              for (int d = 0; d < 3; ++d) {
+                a.lcorner[(size_t)(6 * k + d) * ne + e] = lxk[d];
+                a.lcorner[(size_t)(6 * k + 3 + d) * ne + e] = lvk[d];
+              }
+            },
+            gjc);
 #pragma unroll
   for (int j = 0; j < 3; ++j) a.g_jc[(size_t)j * ne + e] += gjc[j];
 }

@@ -49,19 +49,55 @@ __device__ __forceinline__ void unit_rev(const double w[3], const double u[3], c
   for (int i = 0; i < 3; ++i) lw[i] += (lu[i] - p * u[i]) / n;
 }
 
+// SORTHO3's vectors after n iterations (n = 0: the normalised r, s, t), and with
+// want_e the unnormalised vectors of iteration n (n < 3) that produce stage n + 1.
+__device__ inline void sortho_stage(const double r0[3], const double s0[3], const double t0[3],
+                                    int n, double U[3], double V[3], double W[3], bool want_e,
+                                    double E1[3], double E2[3], double E3[3]) {
+  double aa = M<double>::rnorm(dot3(r0, r0));
+  for (int i = 0; i < 3; ++i) U[i] = r0[i] * aa;
+  aa = M<double>::rnorm(dot3(s0, s0));
+  for (int i = 0; i < 3; ++i) V[i] = s0[i] * aa;
+  aa = M<double>::rnorm(dot3(t0, t0));
+  for (int i = 0; i < 3; ++i) W[i] = t0[i] * aa;
+  const int last = want_e ? n + 1 : n;
+  for (int it = 0; it < last; ++it) {
+    double c[3], a1[3], a2[3], a3[3];
+    cross3(V, W, c);
+    for (int i = 0; i < 3; ++i) a1[i] = c[i] + U[i];
+    cross3(W, U, c);
+    for (int i = 0; i < 3; ++i) a2[i] = c[i] + V[i];
+    cross3(U, V, c);
+    for (int i = 0; i < 3; ++i) a3[i] = c[i] + W[i];
+    if (want_e && it == n) {
+      for (int i = 0; i < 3; ++i) { E1[i] = a1[i]; E2[i] = a2[i]; E3[i] = a3[i]; }
+      return;
+    }
+    double bb = M<double>::rnorm(dot3(a1, a1));
+    for (int i = 0; i < 3; ++i) U[i] = a1[i] * bb;
+    bb = M<double>::rnorm(dot3(a2, a2));
+    for (int i = 0; i < 3; ++i) V[i] = a2[i] * bb;
+    bb = M<double>::rnorm(dot3(a3, a3));
+    for (int i = 0; i < 3; ++i) W[i] = a3[i] * bb;
+  }
+}
+
 }  // namespace badj
 
-// Corner data as x[d][k] (d = direction, k = corner 0..7), global frame, FP64.
+// corner(k, xk, vk) loads corner k's coordinates and velocity (global frame, FP64);
+// it is called in the forward pass and again in the reverse, so that they need not be
+// held in registers in between.
 // st_eint, st_rho, st_qvis, st_offg, st_epsd: the incoming values of the state
 // without an adjoint.  lpost(j): adjoint of the outgoing SIG(j) (j < 6) and PLA (j = 6);
 // lslot(k, i): adjoint of corner k's slot component i; lpre(j, v) is called once for
-// j = 0..6 with the adjoint of the incoming SIG(j) / PLA; lx, lv and g_jc are added to.
-template <typename LPOST, typename LSLOT, typename LPRE>
-__device__ inline void brick_vjp(const double x[3][8], const double v[3][8], const double sig_in[6],
+// j = 0..6 with the adjoint of the incoming SIG(j) / PLA; lcorner(k, lx, lv) once per
+// corner with the adjoints of its coordinates and velocity; g_jc is added to.
+template <typename CORNER, typename LPOST, typename LSLOT, typename LPRE, typename LCORNER>
+__device__ inline void brick_vjp(CORNER corner, const double sig_in[6],
                                  double st_pla, double st_rho, double st_offg, double vol0d,
                                  double dt1, const Law2 &mat, const double jc[3],
                                  const SProp &prop, LPOST lpost, LSLOT lslot, LPRE lpre,
-                                 double lx[3][8], double lv[3][8], double g_jc[3]) {
+                                 LCORNER lcorner, double g_jc[3]) {
   using namespace badj;
   const double ZERO = 0.0, ONE = 1.0, TWO = 2.0, THREE = 3.0, FOUR = 4.0, HALF = 0.5;
   const double THIRD = K<double>::THIRD, ONEP333 = K<double>::ONEP333;
@@ -69,8 +105,19 @@ __device__ inline void brick_vjp(const double x[3][8], const double v[3][8], con
   (void)st_rho;
   double offg = st_offg;
   // ================= forward =================
-  // ---- SREPISO3 ----
   double r0[3], s0[3], t0[3];
+  double R[3][3], e1[3], e3u[3], e3[3], e2[3], U3[3], V3v[3];
+  double xl[3][8], vl[3][8];
+  double off = (offg < ZERO) ? -offg : offg;
+  const bool vdead = offg < ZERO;
+  {
+  double x[3][8], v[3][8];
+  for (int k = 0; k < 8; ++k) {
+    double xk[3], vk[3];
+    corner(k, xk, vk);
+    for (int d = 0; d < 3; ++d) { x[d][k] = xk[d]; v[d][k] = vk[d]; }
+  }
+  // ---- SREPISO3 ----
   for (int d = 0; d < 3; ++d) {
     const double *q = x[d];
     const double A17 = (q[6] - q[0]) + (q[5] - q[3]), A28 = (q[7] - q[1]) + (q[4] - q[2]);
@@ -78,51 +125,26 @@ __device__ inline void brick_vjp(const double x[3][8], const double v[3][8], con
     s0[d] = A17 + A28;
     t0[d] = A17 - A28;
   }
-  // ---- SORTHO3: three iterations, kept for the reverse ----
-  double U[4][3], V[4][3], W[4][3];   // normalised vectors before iteration n (n = 0..3)
-  double E1[3][3], E2[3][3], E3[3][3];  // unnormalised vectors of iteration n
+  // ---- SORTHO3 (its iterations are recomputed in the reverse, not kept) ----
   {
-    double aa = M<double>::rnorm(dot3(r0, r0));
-    for (int i = 0; i < 3; ++i) U[0][i] = r0[i] * aa;
-    aa = M<double>::rnorm(dot3(s0, s0));
-    for (int i = 0; i < 3; ++i) V[0][i] = s0[i] * aa;
-    aa = M<double>::rnorm(dot3(t0, t0));
-    for (int i = 0; i < 3; ++i) W[0][i] = t0[i] * aa;
-    for (int n = 0; n < 3; ++n) {
-      double c[3];
-      cross3(V[n], W[n], c);
-      for (int i = 0; i < 3; ++i) E1[n][i] = c[i] + U[n][i];
-      cross3(W[n], U[n], c);
-      for (int i = 0; i < 3; ++i) E2[n][i] = c[i] + V[n][i];
-      cross3(U[n], V[n], c);
-      for (int i = 0; i < 3; ++i) E3[n][i] = c[i] + W[n][i];
-      double bb = M<double>::rnorm(dot3(E1[n], E1[n]));
-      for (int i = 0; i < 3; ++i) U[n + 1][i] = E1[n][i] * bb;
-      bb = M<double>::rnorm(dot3(E2[n], E2[n]));
-      for (int i = 0; i < 3; ++i) V[n + 1][i] = E2[n][i] * bb;
-      bb = M<double>::rnorm(dot3(E3[n], E3[n]));
-      for (int i = 0; i < 3; ++i) W[n + 1][i] = E3[n][i] * bb;
-    }
+    double W3[3], d0[3], d1[3], d2[3];
+    sortho_stage(r0, s0, t0, 3, U3, V3v, W3, false, d0, d1, d2);
   }
-  double e1[3], e3u[3], e3[3], e2[3];
-  for (int i = 0; i < 3; ++i) e1[i] = U[3][i];
-  cross3(e1, V[3], e3u);
+  for (int i = 0; i < 3; ++i) e1[i] = U3[i];
+  cross3(e1, V3v, e3u);
   {
     const double aa = M<double>::rnorm(dot3(e3u, e3u));
     for (int i = 0; i < 3; ++i) e3[i] = e3u[i] * aa;
   }
   cross3(e3, e1, e2);
-  double R[3][3];
   for (int i = 0; i < 3; ++i) { R[i][0] = e1[i]; R[i][1] = e2[i]; R[i][2] = e3[i]; }
   // ---- local coordinates and velocities ----
-  double xl[3][8], vl[3][8];
-  double off = (offg < ZERO) ? -offg : offg;
-  const bool vdead = offg < ZERO;
   for (int k = 0; k < 8; ++k)
     for (int j = 0; j < 3; ++j) {
       xl[j][k] = R[0][j] * x[0][k] + R[1][j] * x[1][k] + R[2][j] * x[2][k];
       vl[j][k] = vdead ? ZERO : R[0][j] * v[0][k] + R[1][j] * v[1][k] + R[2][j] * v[2][k];
     }
+  }
   // ---- SDERI3 ----
   double J[9];
   {
@@ -295,21 +317,20 @@ __device__ inline void brick_vjp(const double x[3][8], const double v[3][8], con
   const double Sd[6] = {(sgo[0] - qnew) * voln, (sgo[1] - qnew) * voln, (sgo[2] - qnew) * voln,
                         sgo[3] * voln, sgo[4] * voln, sgo[5] * voln};
   // FI[0][a] = S1 PX + S4 PY + S6 PZ; FI[1][a] = S2 PY + S4 PX + S5 PZ; FI[2][a] = S3 PZ + S6 PX + S5 PY
-  double Fl[3][8];
   const bool zero_f = offg < ZERO;
   const int s4[8] = {-1, 1, -1, 1, 1, -1, 1, -1};
   const int ak[8] = {0, 1, 2, 3, 2, 3, 0, 1};
-  for (int k = 0; k < 8; ++k)
-    for (int d = 0; d < 3; ++d) {
-      const int a = ak[k];
-      double FI;
-      if (d == 0) FI = Sd[0] * PX[a] + Sd[3] * PY[a] + Sd[5] * PZ[a];
-      else if (d == 1) FI = Sd[1] * PY[a] + Sd[3] * PX[a] + Sd[4] * PZ[a];
-      else FI = Sd[2] * PZ[a] + Sd[5] * PX[a] + Sd[4] * PY[a];
-      const double hg = -G_of<double>(0, k, PH) * HH[d][0] - G_of<double>(1, k, PH) * HH[d][1] -
-                        G_of<double>(2, k, PH) * HH[d][2] + s4[k] * HH[d][3];
-      Fl[d][k] = (k < 4) ? hg - FI : hg + FI;
-    }
+  // local corner force Fl (SHVIS3 hourglass force -/+ SFINT3 pair term)
+  auto fl_of = [&](int d, int k) {
+    const int a = ak[k];
+    double FI;
+    if (d == 0) FI = Sd[0] * PX[a] + Sd[3] * PY[a] + Sd[5] * PZ[a];
+    else if (d == 1) FI = Sd[1] * PY[a] + Sd[3] * PX[a] + Sd[4] * PZ[a];
+    else FI = Sd[2] * PZ[a] + Sd[5] * PX[a] + Sd[4] * PY[a];
+    const double hg = -G_of<double>(0, k, PH) * HH[d][0] - G_of<double>(1, k, PH) * HH[d][1] -
+                      G_of<double>(2, k, PH) * HH[d][2] + s4[k] * HH[d][3];
+    return (k < 4) ? hg - FI : hg + FI;
+  };
 
   // ================= reverse =================
   double lR[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -323,7 +344,8 @@ __device__ inline void brick_vjp(const double x[3][8], const double v[3][8], con
       double lFl[3];
       for (int j = 0; j < 3; ++j) {
         lFl[j] = R[0][j] * lF[0] + R[1][j] * lF[1] + R[2][j] * lF[2];
-        for (int i = 0; i < 3; ++i) lR[i][j] += lF[i] * Fl[j][k];
+        const double flj = fl_of(j, k);
+        for (int i = 0; i < 3; ++i) lR[i][j] += lF[i] * flj;
       }
       const int a = ak[k];
       const double sg = (k < 4) ? -1.0 : 1.0;   // Fl = hg -/+ FI
@@ -358,9 +380,12 @@ __device__ inline void brick_vjp(const double x[3][8], const double v[3][8], con
     if (i < 3) lqnew -= lS[i] * voln;
   }
   // ---- SHVIS3 ----
-  double lvl[3][8];
+  // The local velocities' adjoint is kept compact until the corners are emitted:
+  // lhg[d][m] = adjoint of HG[d][m] (HG = sum_k G(m,k) vl[d][k], G(3,k) = g4[k]) and
+  // lVd[d][a] = adjoint of Vd[d][a] = vl[d][pa] - vl[d][pb] (SDEFO3, below).
+  double lhg[3][4], lVd[3][4];
   for (int d = 0; d < 3; ++d)
-    for (int k = 0; k < 8; ++k) lvl[d][k] = 0.0;
+    for (int a = 0; a < 4; ++a) lVd[d][a] = 0.0;
   double lFCL = 0.0, lFCQ = 0.0;
   for (int d = 0; d < 3; ++d) {
     for (int m = 0; m < 4; ++m) {
@@ -369,17 +394,19 @@ __device__ inline void brick_vjp(const double x[3][8], const double v[3][8], con
       const double lh = l * (FCL + 2.0 * ah * FCQ);
       lFCL += l * h;
       lFCQ += l * h * ah;
-      if (m < 3) {
-        for (int k = 0; k < 8; ++k) {
-          lvl[d][k] += G_of<double>(m, k, PH) * lh;
-          lPH[m][ak[k]] += ((k < 4) ? -1.0 : 1.0) * vl[d][k] * lh;
-        }
-      } else {
-        const int g4[8] = {1, -1, 1, -1, -1, 1, -1, 1};
-        for (int k = 0; k < 8; ++k) lvl[d][k] += g4[k] * lh;
-      }
+      lhg[d][m] = lh;
+      // G(m,k) = gamma -/+ PH[m][a(k)]: corners a and pb[a] share PH[m][a] with signs
+      // -, +, so HG picks up -PH[m][a] Vd[d][a]
+      if (m < 3)
+        for (int a = 0; a < 4; ++a) lPH[m][a] -= Vd[d][a] * lh;
     }
   }
+  auto lvl_of = [&](int d, int k) {
+    const int g4[8] = {1, -1, 1, -1, -1, 1, -1, 1};
+    double l = G_of<double>(0, k, PH) * lhg[d][0] + G_of<double>(1, k, PH) * lhg[d][1] +
+               G_of<double>(2, k, PH) * lhg[d][2] + g4[k] * lhg[d][3];
+    return (k < 4) ? l + lVd[d][k] : l - lVd[d][ak[k]];
+  };
   // FCL = CAQ rho V23 SSP, FCQ = CAQ rho V23 CAQ 100
   const double lFCL0 = lFCL * SSP + lFCQ * CAQ * 100.0;
   double lrho = lFCL0 * CAQ * V23;
@@ -500,8 +527,7 @@ __device__ inline void brick_vjp(const double x[3][8], const double v[3][8], con
       for (int a = 0; a < 4; ++a) {
         lPd[j][a] += lDg[i][j] * Vd[i][a];
         const double lvd = lDg[i][j] * Pd[j][a];
-        lvl[i][pa[a]] += lvd;
-        lvl[i][pb[a]] -= lvd;
+        lVd[i][a] += lvd;
       }
   // ---- SDERI3 ----
   double lxl[3][8];
@@ -562,17 +588,16 @@ __device__ inline void brick_vjp(const double x[3][8], const double v[3][8], con
     lxl[d][4] += l35; lxl[d][2] -= l35;
     lxl[d][5] += l46; lxl[d][3] -= l46;
   }
-  // ---- local frame: xl[j][k] = sum_i R[i][j] x[i][k], vl likewise ----
-  for (int k = 0; k < 8; ++k)
+  // ---- local frame: xl[j][k] = sum_i R[i][j] x[i][k], vl likewise: the frame's adjoint ----
+  for (int k = 0; k < 8; ++k) {
+    double xk[3], vk[3];
+    corner(k, xk, vk);
     for (int j = 0; j < 3; ++j)
       for (int i = 0; i < 3; ++i) {
-        lR[i][j] += lxl[j][k] * x[i][k];
-        lx[i][k] += R[i][j] * lxl[j][k];
-        if (!vdead) {
-          lR[i][j] += lvl[j][k] * v[i][k];
-          lv[i][k] += R[i][j] * lvl[j][k];
-        }
+        lR[i][j] += lxl[j][k] * xk[i];
+        if (!vdead) lR[i][j] += lvl_of(j, k) * vk[i];
       }
+  }
   // ---- SORTHO3 reverse ----
   double le1[3], le2[3], le3[3];
   for (int i = 0; i < 3; ++i) { le1[i] = lR[i][0]; le2[i] = lR[i][1]; le3[i] = lR[i][2]; }
@@ -582,34 +607,68 @@ __device__ inline void brick_vjp(const double x[3][8], const double v[3][8], con
   double le3u[3] = {0, 0, 0};
   unit_rev(e3u, e3, le3, le3u);
   double lU[3] = {le1[0], le1[1], le1[2]}, lV[3] = {0, 0, 0}, lW[3] = {0, 0, 0};
-  cross3_rev(e1, V[3], le3u, lU, lV);
+  cross3_rev(e1, V3v, le3u, lU, lV);
   for (int n = 2; n >= 0; --n) {
-    // U[n+1] = unit(E1[n]) etc.
+    // stage n vectors and the unnormalised E of iteration n, recomputed
+    double Un[3], Vn[3], Wn[3], E1n[3], E2n[3], E3n[3];
+    sortho_stage(r0, s0, t0, n, Un, Vn, Wn, true, E1n, E2n, E3n);
+    double U1[3], V1[3], W1[3];
+    {
+      double bb = M<double>::rnorm(dot3(E1n, E1n));
+      for (int i = 0; i < 3; ++i) U1[i] = E1n[i] * bb;
+      bb = M<double>::rnorm(dot3(E2n, E2n));
+      for (int i = 0; i < 3; ++i) V1[i] = E2n[i] * bb;
+      bb = M<double>::rnorm(dot3(E3n, E3n));
+      for (int i = 0; i < 3; ++i) W1[i] = E3n[i] * bb;
+    }
     double lE1[3] = {0, 0, 0}, lE2[3] = {0, 0, 0}, lE3[3] = {0, 0, 0};
-    unit_rev(E1[n], U[n + 1], lU, lE1);
-    unit_rev(E2[n], V[n + 1], lV, lE2);
-    unit_rev(E3[n], W[n + 1], lW, lE3);
+    unit_rev(E1n, U1, lU, lE1);
+    unit_rev(E2n, V1, lV, lE2);
+    unit_rev(E3n, W1, lW, lE3);
     double nU[3] = {lE1[0], lE1[1], lE1[2]}, nV[3] = {lE2[0], lE2[1], lE2[2]};
     double nW[3] = {lE3[0], lE3[1], lE3[2]};
-    cross3_rev(V[n], W[n], lE1, nV, nW);   // E1 = V x W + U
-    cross3_rev(W[n], U[n], lE2, nW, nU);   // E2 = W x U + V
-    cross3_rev(U[n], V[n], lE3, nU, nV);   // E3 = U x V + W
+    cross3_rev(Vn, Wn, lE1, nV, nW);   // E1 = V x W + U
+    cross3_rev(Wn, Un, lE2, nW, nU);   // E2 = W x U + V
+    cross3_rev(Un, Vn, lE3, nU, nV);   // E3 = U x V + W
     for (int i = 0; i < 3; ++i) { lU[i] = nU[i]; lV[i] = nV[i]; lW[i] = nW[i]; }
   }
   double lr[3] = {0, 0, 0}, ls[3] = {0, 0, 0}, lt[3] = {0, 0, 0};
-  unit_rev(r0, U[0], lU, lr);
-  unit_rev(s0, V[0], lV, ls);
-  unit_rev(t0, W[0], lW, lt);
-  // ---- SREPISO3 reverse: r = 17 + 28 - 35 - 46; s = 17 + 46 + 28 + 35; t = 17 + 46 - 28 - 35 ----
+  {
+    double U0[3], V0[3], W0[3], d0[3], d1[3], d2[3];
+    sortho_stage(r0, s0, t0, 0, U0, V0, W0, false, d0, d1, d2);
+    unit_rev(r0, U0, lU, lr);
+    unit_rev(s0, V0, lV, ls);
+    unit_rev(t0, W0, lW, lt);
+  }
+  // ---- corner adjoints: the local frame's x = R xl, v = R vl, plus SREPISO3's
+  // r = 17 + 28 - 35 - 46; s = 17 + 46 + 28 + 35; t = 17 + 46 - 28 - 35 (pairs 1-7,
+  // 2-8, 3-5, 4-6 with the higher corner positive) ----
+  double l17[3], l28[3], l35[3], l46[3];
   for (int d = 0; d < 3; ++d) {
-    const double l17 = lr[d] + ls[d] + lt[d];
-    const double l28 = lr[d] + ls[d] - lt[d];
-    const double l35 = -lr[d] + ls[d] - lt[d];
-    const double l46 = -lr[d] + ls[d] + lt[d];
-    lx[d][6] += l17; lx[d][0] -= l17;
-    lx[d][7] += l28; lx[d][1] -= l28;
-    lx[d][4] += l35; lx[d][2] -= l35;
-    lx[d][5] += l46; lx[d][3] -= l46;
+    l17[d] = lr[d] + ls[d] + lt[d];
+    l28[d] = lr[d] + ls[d] - lt[d];
+    l35[d] = -lr[d] + ls[d] - lt[d];
+    l46[d] = -lr[d] + ls[d] + lt[d];
+  }
+  for (int k = 0; k < 8; ++k) {
+    double lxk[3], lvk[3];
+    for (int i = 0; i < 3; ++i) {
+      lxk[i] = R[i][0] * lxl[0][k] + R[i][1] * lxl[1][k] + R[i][2] * lxl[2][k];
+      lvk[i] = vdead ? 0.0 : R[i][0] * lvl_of(0, k) + R[i][1] * lvl_of(1, k) + R[i][2] * lvl_of(2, k);
+      double sr = 0.0;
+      switch (k) {
+        case 0: sr = -l17[i]; break;
+        case 6: sr = l17[i]; break;
+        case 1: sr = -l28[i]; break;
+        case 7: sr = l28[i]; break;
+        case 2: sr = -l35[i]; break;
+        case 4: sr = l35[i]; break;
+        case 3: sr = -l46[i]; break;
+        default: sr = l46[i]; break;   // corner 5
+      }
+      lxk[i] += sr;
+    }
+    lcorner(k, lxk, lvk);
   }
 }
 

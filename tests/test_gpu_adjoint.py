@@ -42,7 +42,7 @@ import torch
 import pyoracle as po
 from openradioss_amd import deck as dk
 from openradioss_amd import engine as en
-from openradioss_amd.autograd import cinmas, displacement_plus_plastic_work, force_weights
+from openradioss_amd.autograd import ShellRun, cinmas, displacement_plus_plastic_work, force_weights
 
 pytestmark = pytest.mark.gpu
 
