@@ -244,6 +244,33 @@ def adjoint_leg(n=128, ncyc=2000, every=100, ishell=1, fd_params=20):
     return res
 
 
+def group_path_leg(deck, prec, sizes=(128, 8192), reps=3):
+    """The engine-side call surface on the same deck: one element pass as the FORINT
+    loop drives it, orad_sforc3_async per group of NVSIZ elements and one
+    orad_group_wait, timed on the host; and orad_update_xv of the whole X, V."""
+    s = en.BrickSolver(deck, prec)
+    s.run_cycles(5)
+    dt1 = s.time().dt2
+    res = {}
+    for nv in sizes:
+        best = 1e30
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            for nft in range(0, s.numels, nv):
+                s.sforc3_async(nft, min(nv, s.numels - nft), dt1)
+            s.group_wait()
+            best = min(best, time.perf_counter() - t0)
+        res[f"element_pass_ms_groups_of_{nv}"] = 1e3 * best
+    nd = s.nodes()
+    t0 = time.perf_counter()
+    s.update_xv(nd["x"], nd["v"])
+    res["update_xv_ms"] = 1e3 * (time.perf_counter() - t0)
+    s.close()
+    print(f"group path: " + ", ".join(f"{k} {v:.1f}" for k, v in res.items()), file=sys.stderr,
+          flush=True)
+    return res
+
+
 def brick_adjoint_leg(n=60, ncyc=1000, every=100):
     """Adjoint of an n^3 brick block run whose top nodes start downward at 50 m/s:
     J = final z displacement of a top node + sum of V0 * 1e3 * PLA, gradient w.r.t.
@@ -487,6 +514,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="tube leg: keep the default launch configuration")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 128^2 adjoint leg")
+    ap.add_argument("--no-group-path", action="store_true",
+                    help="skip timing the per-group (SFORC3 drop-in) element pass")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -517,6 +546,7 @@ def main():
         s64.close()
     s32.close()
 
+    group_path = None if args.no_group_path else group_path_leg(deck, en.MIXED)
     eb = elem_bytes(ne, nn, 4)
     nb = node_bytes(ne, nn, 4)
     ach = eb / (el32 * 1e-3) / 1e9
@@ -584,6 +614,7 @@ def main():
         "adjoint_plate": adjoint,
         "adjoint_plate_qeph": adjoint_qeph,
         "adjoint_brick": adjoint_brick,
+        "group_path": group_path,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -108,6 +108,22 @@ int orad_get_time(orad_ctx *ctx, orad_time_t *t);
  * element state; writes their FSKY slots (SCUMU3P, scumu3p.F:259-308), updates
  * element state, and returns the group's DT2T candidate (min with *dt2t on entry). */
 int orad_sforc3(orad_ctx *ctx, int nft, int nel, double dt1, double *dt2t);
+/* The same group call without waiting (no reference counterpart: SFORC3 returns when
+ * its group is done): enqueues the element pass of elements nft+1..nft+nel and returns;
+ * the groups' time-step minimum is kept on the device.  orad_group_wait returns once
+ * every queued group has run, lowers *dt2t to their minimum (the merge of the FORINT
+ * group loop, resol.F:4134-4140) and returns ORAD_ENEGVOL if a brick saw a
+ * non-positive volume.  orad_cforc3_async is the shell counterpart of orad_cforc3. */
+int orad_sforc3_async(orad_ctx *ctx, int nft, int nel, double dt1);
+int orad_cforc3_async(orad_ctx *ctx, int nft, int nel, double dt1);
+int orad_group_wait(orad_ctx *ctx, double *dt2t);
+/* The engine's X(3,NUMNOD), V(3,NUMNOD) and, for shells, VR(3,NUMNOD) (null for
+ * bricks) after its own ACCELE/BCS10/VELOCITY/DEPLA (accele.F, bcs10.F, velocity.F,
+ * displacement.F), handed to the device before the next group loop (no reference
+ * counterpart: the engine's arrays are the kernels' in the reference).  Unlike
+ * orad_set_nodes / orad_set_nodes_rot it keeps MS, IN, the boundary codes, D, A,
+ * STIFN and the captured cycle graphs. */
+int orad_update_xv(orad_ctx *ctx, const double *x, const double *v, const double *vr);
 
 /* ASSPAR4 (assembly/asspar4.F:157-175): ordered per-node sum of the FSKY slots
  * into A(3,NUMNOD) and STIFN(NUMNOD), returned to the host when pointers are

@@ -306,7 +306,8 @@ int orad_destroy(orad_ctx *c) {
                   (void *)c->thke, (void *)c->fv_val, (void *)c->node_model,
                   (void *)c->model_node0, (void *)c->model_tiles, (void *)c->tiles,
                   (void *)c->mats, (void *)c->queue, (void *)c->chunk_ehou, (void *)c->rank,
-                  (void *)c->emap, (void *)c->bal_part, (void *)c->fv_mem, (void *)c->fv_work})
+                  (void *)c->emap, (void *)c->bal_part, (void *)c->fv_mem, (void *)c->fv_work,
+                  (void *)c->xv_stage})
     dfree(p);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -535,26 +536,114 @@ static int ready(orad_ctx *c) {
   return ORAD_OK;
 }
 
-int orad_sforc3(orad_ctx *c, int nft, int nel, double dt1, double *dt2t) {
-  if (!c || !dt2t) return ORAD_EINVAL;
-  if (c->family != FAM_BRICK) return fail(c, ORAD_ESTATE, "sforc3: the mesh is not of bricks");
+}  // extern "C"
+
+// One element group enqueued on the context's stream (bricks: SFORC3, shells: CFORC3
+// or CZFORC3).  The group's time-step minimum goes into the device accumulator
+// scratch_dt2t, reset by the first group after a wait; dt1 is uploaded when it changes.
+static int group_enqueue(orad_ctx *c, int fam, int nft, int nel, double dt1) {
+  const char *who = fam == FAM_BRICK ? "sforc3" : "cforc3";
+  if (c->family != fam)
+    return fail(c, ORAD_ESTATE, std::string(who) + ": the mesh is not of " +
+                                    (fam == FAM_BRICK ? "bricks" : "shells"));
   int r = ready(c);
   if (r) return r;
-  if (nft < 0 || nel < 0 || nel > c->numels - nft)
-    return fail(c, ORAD_EINVAL, "sforc3: group outside the element range");
+  const int ne = fam == FAM_BRICK ? c->numels : c->numelc;
+  if (nft < 0 || nel < 0 || nel > ne - nft)
+    return fail(c, ORAD_EINVAL, std::string(who) + ": group outside the element range");
   HIPCHK(c, hipSetDevice(c->device));
-  unsigned long long bits = 0x7FF0000000000000ull;
-  HIPCHK(c, hipMemcpyAsync(c->scratch_dt1, &dt1, sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->scratch_dt2t, &bits, sizeof(bits), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, launch_elements(c, nft, nft + nel, c->scratch_dt1, c->scratch_dt2t, nullptr, true));
-  HIPCHK(c, hipMemcpyAsync(&bits, c->scratch_dt2t, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+  if (!c->group_open) {
+    c->group_bits = 0x7FF0000000000000ull;
+    HIPCHK(c, hipMemcpyAsync(c->scratch_dt2t, &c->group_bits, sizeof(c->group_bits),
+                             hipMemcpyHostToDevice, c->stream));
+  }
+  if (!c->group_open || dt1 != c->group_dt1[c->group_slot]) {
+    // a ring of host values, so that an earlier upload's source outlives its copy
+    c->group_slot = (c->group_slot + 1) % 8;
+    c->group_dt1[c->group_slot] = dt1;
+    HIPCHK(c, hipMemcpyAsync(c->scratch_dt1, &c->group_dt1[c->group_slot], sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+  }
+  c->group_open = true;
+  if (fam == FAM_BRICK) {
+    HIPCHK(c, launch_elements(c, nft, nft + nel, c->scratch_dt1, c->scratch_dt2t, nullptr, true));
+    return ORAD_OK;
+  }
+  const ShMat keep = c->shmat;
+  if (c->batched && nel > 0) {
+    // a group of batched models lies within one model and takes that model's material
+    const int m = (int)(std::upper_bound(c->h_elem_off.begin(), c->h_elem_off.end(), nft) -
+                        c->h_elem_off.begin()) - 1;
+    if (nel > c->h_elem_off[m + 1] - nft)
+      return fail(c, ORAD_EINVAL, "cforc3: a group of batched models must lie in one model");
+    c->shmat = c->h_mats[m];
+  }
+  hipError_t le = launch_elements(c, nft, nft + nel, c->scratch_dt1, c->scratch_dt2t, nullptr);
+  c->shmat = keep;
+  HIPCHK(c, le);
+  return ORAD_OK;
+}
+
+// Waits for the queued groups; lowers *dt2t to their minimum; MSTOP of bricks.
+static int group_wait(orad_ctx *c, double *dt2t) {
+  if (!c->group_open) return ORAD_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(&c->group_bits, c->scratch_dt2t, sizeof(c->group_bits),
+                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->group_open = false;
   double g;
-  std::memcpy(&g, &bits, sizeof(g));
-  if (g < *dt2t) *dt2t = g;
-  int ms = 0;
-  HIPCHK(c, hipMemcpy(&ms, &c->ctl->mstop, sizeof(int), hipMemcpyDeviceToHost));
-  return ms ? fail(c, ORAD_ENEGVOL, "non-positive element volume (MSTOP)") : ORAD_OK;
+  std::memcpy(&g, &c->group_bits, sizeof(g));
+  if (dt2t && g < *dt2t) *dt2t = g;
+  if (c->family == FAM_BRICK) {
+    int ms = 0;
+    HIPCHK(c, hipMemcpy(&ms, &c->ctl->mstop, sizeof(int), hipMemcpyDeviceToHost));
+    if (ms) return fail(c, ORAD_ENEGVOL, "non-positive element volume (MSTOP)");
+  }
+  return ORAD_OK;
+}
+
+extern "C" {
+
+int orad_sforc3(orad_ctx *c, int nft, int nel, double dt1, double *dt2t) {
+  if (!c || !dt2t) return ORAD_EINVAL;
+  int r = group_wait(c, nullptr);
+  if (!r) r = group_enqueue(c, FAM_BRICK, nft, nel, dt1);
+  if (!r) r = group_wait(c, dt2t);
+  return r;
+}
+
+int orad_sforc3_async(orad_ctx *c, int nft, int nel, double dt1) {
+  if (!c) return ORAD_EINVAL;
+  return group_enqueue(c, FAM_BRICK, nft, nel, dt1);
+}
+
+int orad_cforc3_async(orad_ctx *c, int nft, int nel, double dt1) {
+  if (!c) return ORAD_EINVAL;
+  return group_enqueue(c, FAM_SHELL, nft, nel, dt1);
+}
+
+int orad_group_wait(orad_ctx *c, double *dt2t) {
+  if (!c || !dt2t) return ORAD_EINVAL;
+  return group_wait(c, dt2t);
+}
+
+int orad_update_xv(orad_ctx *c, const double *x, const double *v, const double *vr) {
+  if (!c || !x || !v) return ORAD_EINVAL;
+  if (!c->have_nodes) return fail(c, ORAD_ESTATE, "update_xv: call orad_set_nodes first");
+  if (vr && !c->have_rot)
+    return fail(c, ORAD_ESTATE, "update_xv: rotational velocities need orad_set_nodes_rot");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t n = (size_t)c->numnod;
+  if (!c->xv_stage) HIPCHK(c, dalloc(&c->xv_stage, 9 * n * sizeof(double)));
+  HIPCHK(c, hipMemcpyAsync(c->xv_stage, x, 3 * n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->xv_stage + 3 * n, v, 3 * n * 8, hipMemcpyHostToDevice, c->stream));
+  if (vr)
+    HIPCHK(c, hipMemcpyAsync(c->xv_stage + 6 * n, vr, 3 * n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch_pack_xv(c->numnod, c->xv_stage, c->X4, c->V4, vr ? c->VR4 : nullptr,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ORAD_OK;
 }
 
 int orad_asspar4(orad_ctx *c, double *a, double *stifn) {
@@ -1193,33 +1282,10 @@ int orad_set_shell_ishell(orad_ctx *c, int ishell, double dn) {
 
 int orad_cforc3(orad_ctx *c, int nft, int nel, double dt1, double *dt2t) {
   if (!c || !dt2t) return ORAD_EINVAL;
-  if (c->family != FAM_SHELL) return fail(c, ORAD_ESTATE, "cforc3: the mesh is not of shells");
-  int r = ready(c);
-  if (r) return r;
-  if (nft < 0 || nel < 0 || nel > c->numelc - nft)
-    return fail(c, ORAD_EINVAL, "cforc3: group outside the element range");
-  HIPCHK(c, hipSetDevice(c->device));
-  unsigned long long bits = 0x7FF0000000000000ull;
-  HIPCHK(c, hipMemcpyAsync(c->scratch_dt1, &dt1, sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->scratch_dt2t, &bits, sizeof(bits), hipMemcpyHostToDevice, c->stream));
-  const ShMat keep = c->shmat;
-  if (c->batched && nel > 0) {
-    // a group of batched models lies within one model and takes that model's material
-    const int m = (int)(std::upper_bound(c->h_elem_off.begin(), c->h_elem_off.end(), nft) -
-                        c->h_elem_off.begin()) - 1;
-    if (nel > c->h_elem_off[m + 1] - nft)
-      return fail(c, ORAD_EINVAL, "cforc3: a group of batched models must lie in one model");
-    c->shmat = c->h_mats[m];
-  }
-  hipError_t le = launch_elements(c, nft, nft + nel, c->scratch_dt1, c->scratch_dt2t, nullptr);
-  c->shmat = keep;
-  HIPCHK(c, le);
-  HIPCHK(c, hipMemcpyAsync(&bits, c->scratch_dt2t, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  double g;
-  std::memcpy(&g, &bits, sizeof(g));
-  if (g < *dt2t) *dt2t = g;
-  return ORAD_OK;
+  int r = group_wait(c, nullptr);
+  if (!r) r = group_enqueue(c, FAM_SHELL, nft, nel, dt1);
+  if (!r) r = group_wait(c, dt2t);
+  return r;
 }
 
 int orad_asspar4_rot(orad_ctx *c, double *a, double *ar, double *stifn, double *stifr) {

@@ -74,6 +74,8 @@ hipError_t launch_adj_gather_node(int numnod, int numelc, int per, const int32_t
                                   const double *ld, double *lvr, double *lamS, double *lamR,
                                   double *g_ms, double *g_in, hipStream_t s);
 // up to 7 device-to-device copies (sizes multiples of 4 bytes) in one launch
+hipError_t launch_pack_xv(int numnod, const double *xv, double4 *X4, double4 *V4, double4 *VR4,
+                          hipStream_t s);
 hipError_t launch_adj_stash(int n, const void *const *src, void *const *dst,
                             const size_t *bytes, hipStream_t s);
 }  // namespace orad
@@ -146,6 +148,12 @@ struct orad_ctx {
   CycleCtl *ctl = nullptr;
   double *scratch_dt1 = nullptr;
   unsigned long long *scratch_dt2t = nullptr;
+  // asynchronous group calls (orad_sforc3_async / orad_cforc3_async, orad_group_wait)
+  bool group_open = false;
+  unsigned long long group_bits = 0;
+  double group_dt1[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int group_slot = 0;
+  double *xv_stage = nullptr;         // orad_update_xv staging: X, V, VR (3 * numnod each)
   double *noalloc_scratch = nullptr;   // this device's file-scope scratch array (not owned)
 
   // batched independent shell models (orad_set_shell_models): per-model material,

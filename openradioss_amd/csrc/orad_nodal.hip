@@ -410,4 +410,28 @@ hipError_t launch_energy(bool fp32, int numels, const void *eint, const void *ei
                                  emap, numnod, X4, V4, VR4, part, res, s);
 }
 
+// The engine's X(3,n), V(3,n) and, when VR4 is given, VR(3,n) (staged one after the
+// other in xv) into the node records, keeping MS, the node word and IN in the fourth
+// components.
+__global__ __launch_bounds__(256) void pack_xv_kernel(int numnod, const double *xv, double4 *X4,
+                                                      double4 *V4, double4 *VR4) {
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= numnod) return;
+  const double *x = xv + 3 * (size_t)n, *v = xv + 3 * ((size_t)numnod + n);
+  X4[n] = make_double4(x[0], x[1], x[2], X4[n].w);
+  V4[n] = make_double4(v[0], v[1], v[2], V4[n].w);
+  if (VR4) {
+    const double *w = xv + 3 * (2 * (size_t)numnod + n);
+    VR4[n] = make_double4(w[0], w[1], w[2], VR4[n].w);
+  }
+}
+
+hipError_t launch_pack_xv(int numnod, const double *xv, double4 *X4, double4 *V4, double4 *VR4,
+                          hipStream_t s) {
+  const int nb = (numnod + 255) / 256;
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pack_xv_kernel, dim3(nb), dim3(256), 0, s, numnod, xv, X4, V4, VR4);
+  return hipGetLastError();
+}
+
 }  // namespace orad

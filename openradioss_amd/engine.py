@@ -20,7 +20,8 @@ NIXS = 11
 EXPORTS = (
     "orad_create", "orad_destroy", "orad_last_error", "orad_set_nodes", "orad_set_bricks",
     "orad_set_law2", "orad_set_solid_prop", "orad_get_pon", "orad_set_time", "orad_get_time",
-    "orad_sforc3", "orad_asspar4", "orad_run_cycles", "orad_profile_cycles", "orad_get_nodes",
+    "orad_sforc3", "orad_sforc3_async", "orad_cforc3_async", "orad_group_wait",
+    "orad_update_xv", "orad_asspar4", "orad_run_cycles", "orad_profile_cycles", "orad_get_nodes",
     "orad_get_brick_state", "orad_set_brick_state", "orad_get_fsky", "orad_energies",
     "orad_energies_noalloc", "orad_energy_balance",
     "orad_set_fixvel", "orad_set_nodes_rot", "orad_set_shells", "orad_set_shell_law2",
@@ -92,6 +93,10 @@ def load_library(path=LIB_PATH):
         "orad_set_time": ([P, C.POINTER(TimeState)], I),
         "orad_get_time": ([P, C.POINTER(TimeState)], I),
         "orad_sforc3": ([P, I, I, D, C.POINTER(D)], I),
+        "orad_sforc3_async": ([P, I, I, D], I),
+        "orad_cforc3_async": ([P, I, I, D], I),
+        "orad_group_wait": ([P, C.POINTER(D)], I),
+        "orad_update_xv": ([P, P, P, P], I),
         "orad_asspar4": ([P, P, P], I),
         "orad_run_cycles": ([P, I, C.POINTER(TimeState)], I),
         "orad_profile_cycles": ([P, I, C.POINTER(D), C.POINTER(D)], I),
@@ -261,6 +266,22 @@ class BrickSolver:
             None]
         self._chk(self.lib.orad_adjoint_reverse(self.h, *args))
         return out
+
+    def sforc3_async(self, nft, nel, dt1):
+        """Enqueue the group nft+1..nft+nel (orad_sforc3_async); see group_wait."""
+        self._chk(self.lib.orad_sforc3_async(self.h, nft, nel, dt1))
+
+    def group_wait(self, dt2t=1.0e30):
+        """Wait for the queued groups; returns min(dt2t, their DT2T)."""
+        d = C.c_double(dt2t)
+        self._chk(self.lib.orad_group_wait(self.h, C.byref(d)))
+        return d.value
+
+    def update_xv(self, x, v, vr=None):
+        """orad_update_xv: the engine's new X, V (and shells' VR) into the device node records."""
+        nn = self.numnod
+        w = None if vr is None else _p(_f64(vr, (nn, 3)))
+        self._chk(self.lib.orad_update_xv(self.h, _p(_f64(x, (nn, 3))), _p(_f64(v, (nn, 3))), w))
 
     def sforc3(self, nft, nel, dt1, dt2t=1.0e30):
         """Element forces of elements nft+1..nft+nel (SFORC3); returns the group DT2T."""
@@ -446,6 +467,8 @@ class ShellSolver:
 
     _chk = BrickSolver._chk
     set_launch = BrickSolver.set_launch
+    group_wait = BrickSolver.group_wait
+    update_xv = BrickSolver.update_xv
     launch = BrickSolver.launch
     autotune = BrickSolver.autotune
     close = BrickSolver.close
@@ -462,6 +485,10 @@ class ShellSolver:
         iadc = np.zeros((self.numelc, 4), np.int32)
         self._chk(self.lib.orad_get_pon(self.h, _p(adsky), _p(iadc)))
         return adsky, iadc
+
+    def cforc3_async(self, nft, nel, dt1):
+        """Enqueue the shell group nft+1..nft+nel (orad_cforc3_async); see group_wait."""
+        self._chk(self.lib.orad_cforc3_async(self.h, nft, nel, dt1))
 
     def cforc3(self, nft, nel, dt1, dt2t=1.0e30):
         """Forces of shells nft+1..nft+nel (CFORC3); returns the group DT2T."""

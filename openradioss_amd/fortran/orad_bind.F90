@@ -89,6 +89,29 @@ module orad_bind
       real(c_double), value :: dt1
       real(c_double) :: dt2t
     end function
+    integer(c_int) function orad_sforc3_async(ctx, nft, nel, dt1) bind(c, name='orad_sforc3_async')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nft, nel
+      real(c_double), value :: dt1
+    end function
+    integer(c_int) function orad_cforc3_async(ctx, nft, nel, dt1) bind(c, name='orad_cforc3_async')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nft, nel
+      real(c_double), value :: dt1
+    end function
+    integer(c_int) function orad_group_wait(ctx, dt2t) bind(c, name='orad_group_wait')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: dt2t
+    end function
+    integer(c_int) function orad_update_xv(ctx, x, v, vr) bind(c, name='orad_update_xv')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: x(3,*), v(3,*)
+      real(c_double), optional :: vr(3,*)
+    end function
     integer(c_int) function orad_asspar4(ctx, a, stifn) bind(c, name='orad_asspar4')
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx

@@ -5,9 +5,10 @@
 !  group : the reference's call surface.  Each cycle follows RESOL
 !          (engine/source/engine/resol.F:2586-9595): DT1=DT2, DT2=EP06; the
 !          FORINT group loop (elements/forint.F:325-347) calls SFORC3's drop-in
-!          orad_sforc3 per NVSIZ=128 group; ASSPAR4's drop-in orad_asspar4
-!          returns A and STIFN; then DT2 selection, ACCELE, BCS10, VELOCITY and
-!          DEPLA run here on the host in FP64, and X, V go back to the device.
+!          orad_sforc3_async per NVSIZ=128 group and orad_group_wait after the
+!          loop (the DT2T merge); ASSPAR4's drop-in orad_asspar4 returns A and
+!          STIFN; then DT2 selection, ACCELE, BCS10, VELOCITY and DEPLA run here on
+!          the host in FP64, and orad_update_xv hands X, V back to the device.
 !  device: the whole cycle on the GPU, orad_run_cycles.
 !
 !  deck.bin (written by openradioss_amd.engine.write_deck): int32 magic, version,
@@ -65,10 +66,11 @@ program resol_brick
       dt1 = dt2
       dt2 = 1.0d6
       dt2t = 1.0d30
-      do nft = 0, numels - 1, NVSIZ
+      do nft = 0, numels - 1, NVSIZ                  ! FORINT group loop, SFORC3 per group
         nel = min(NVSIZ, numels - nft)
-        call orad_check(ctx, orad_sforc3(ctx, nft, nel, dt1, dt2t), 'sforc3')
+        call orad_check(ctx, orad_sforc3_async(ctx, nft, nel, dt1), 'sforc3')
       end do
+      call orad_check(ctx, orad_group_wait(ctx, dt2t), 'group_wait')
       call orad_check(ctx, orad_asspar4(ctx, a, stifn), 'asspar4')
       if (dt2t < dt2) dt2 = dt2t
       dt2 = min(dt2, 1.1*dt2old, dtmx)
@@ -92,7 +94,7 @@ program resol_brick
         vdt = dt2*v(3,n); d(3,n) = d(3,n) + vdt; x(3,n) = x(3,n) + vdt
       end do
       tt = tt + dt2
-      call orad_check(ctx, orad_set_nodes(ctx, numnod, x, v, ms, icodt), 'set_nodes')
+      call orad_check(ctx, orad_update_xv(ctx, x, v), 'update_xv')   ! no VR for bricks
     end do
     write(*,'(a,i0,a,es22.15,a,es22.15)') 'cycles ', ncycle, ' tt ', tt, ' dt2 ', dt2
   end if

@@ -5,7 +5,7 @@
 !  group : the reference's call surface.  Each cycle follows RESOL
 !          (engine/source/engine/resol.F:2586-9595): DT1=DT2, DT2=EP06; the
 !          FORINTC group loop calls CFORC3's (or, for Ishell=24, CZFORC3's)
-!          drop-in orad_cforc3 per NVSIZ=128 group; ASSPAR4's drop-in
+!          drop-in orad_cforc3_async per NVSIZ=128 group (orad_group_wait after the loop); ASSPAR4's drop-in
 !          orad_asspar4_rot returns A, AR, STIFN, STIFR; then DT2 selection,
 !          ACCELE (translations and rotations, accele.F:64-109), BCS10 (ICODT,
 !          ICODR), VELOCITY and DEPLA run here on the host in FP64, and X, V, VR
@@ -75,8 +75,9 @@ program resol_shell
       dt2t = 1.0d30
       do nft = 0, numelc - 1, NVSIZ
         nel = min(NVSIZ, numelc - nft)
-        call orad_check(ctx, orad_cforc3(ctx, nft, nel, dt1, dt2t), 'cforc3')
+        call orad_check(ctx, orad_cforc3_async(ctx, nft, nel, dt1), 'cforc3')
       end do
+      call orad_check(ctx, orad_group_wait(ctx, dt2t), 'group_wait')
       call orad_check(ctx, orad_asspar4_rot(ctx, a, ar, stifn, stifr), 'asspar4_rot')
       if (dt2t < dt2) dt2 = dt2t
       dt2 = min(dt2, 1.1*dt2old, dtmx)
@@ -112,8 +113,7 @@ program resol_shell
         vdt = dt2*v(3,n); d(3,n) = d(3,n) + vdt; x(3,n) = x(3,n) + vdt
       end do
       tt = tt + dt2
-      call orad_check(ctx, orad_set_nodes(ctx, numnod, x, v, ms, icodt), 'set_nodes')
-      call orad_check(ctx, orad_set_nodes_rot(ctx, vr, in, icodr), 'set_nodes_rot')
+      call orad_check(ctx, orad_update_xv(ctx, x, v, vr), 'update_xv')
     end do
     write(*,'(a,i0,a,es22.15,a,es22.15)') 'cycles ', ncycle, ' tt ', tt, ' dt2 ', dt2
   end if

@@ -1,0 +1,91 @@
+"""The engine-side group path: orad_sforc3_async / orad_cforc3_async per NVSIZ group with
+one orad_group_wait, and orad_update_xv handing the engine's X, V (VR) back.
+
+Bars (bitwise): the asynchronous group loop writes the same slots and element state and
+returns the same DT2T as the synchronous per-group calls; a run interrupted by
+orad_update_xv with its own X, V continues exactly as the uninterrupted run (D, A, the
+masses, codes and the captured graphs are kept), bricks and shells; orad_group_wait with
+nothing queued leaves DT2T alone.
+"""
+import numpy as np
+import pytest
+
+from openradioss_amd import deck as dk
+from openradioss_amd import engine as en
+
+pytestmark = pytest.mark.gpu
+
+NVSIZ = 128
+
+
+def test_brick_async_groups_match_synchronous_groups():
+    d = dk.brick_block(17, 13, 9, v_top=-5.0e4)
+    outs = []
+    for mode in ("sync", "async"):
+        s = en.BrickSolver(d, en.MIXED)
+        s.run_cycles(20)
+        dt1 = s.time().dt2
+        dt2t = 1.0e30
+        for nft in range(0, s.numels, NVSIZ):
+            nel = min(NVSIZ, s.numels - nft)
+            if mode == "sync":
+                dt2t = s.sforc3(nft, nel, dt1, dt2t)
+            else:
+                s.sforc3_async(nft, nel, dt1)
+        if mode == "async":
+            dt2t = s.group_wait(dt2t)
+        outs.append((dt2t, s.fsky(), s.state()))
+        s.close()
+    (t0, f0, s0), (t1, f1, s1) = outs
+    assert t0 == t1 and t0 < 1.0e30
+    assert np.array_equal(f0, f1)
+    for k in s0:
+        assert np.array_equal(s0[k], s1[k]), k
+
+
+def test_group_wait_without_groups_keeps_dt2t():
+    d = dk.brick_block(4, 4, 4)
+    s = en.BrickSolver(d, en.FP64)
+    assert s.group_wait(3.0e-7) == 3.0e-7
+    s.close()
+
+
+def test_brick_update_xv_keeps_the_run():
+    d = dk.brick_block(12, 10, 8, v_top=-5.0e4)
+    a = en.BrickSolver(d, en.FP64)
+    a.run_cycles(60)
+    b = en.BrickSolver(d, en.FP64)
+    b.run_cycles(30)
+    nd = b.nodes()
+    b.update_xv(nd["x"], nd["v"])
+    b.run_cycles(30)
+    na, nb = a.nodes(), b.nodes()
+    for k in ("x", "v", "d", "a"):
+        assert np.array_equal(na[k], nb[k]), k
+    assert a.time().tt == b.time().tt
+    a.close()
+    b.close()
+
+
+def test_shell_async_groups_and_update_xv():
+    d = dk.shell_plate(n=24, patch=8)
+    a = en.ShellSolver(d, en.MIXED)
+    a.run_cycles(40)
+    b = en.ShellSolver(d, en.MIXED)
+    b.run_cycles(20)
+    nd = b.nodes()
+    b.update_xv(nd["x"], nd["v"], nd["vr"])
+    b.run_cycles(20)
+    na, nb = a.nodes(), b.nodes()
+    for k in ("x", "v", "d", "vr"):
+        assert np.array_equal(na[k], nb[k]), k
+    # one more element pass: whole deck (sync) vs async groups
+    dt1 = a.time().dt2
+    ta = a.cforc3(0, a.numelc, dt1)
+    for nft in range(0, b.numelc, NVSIZ):
+        b.cforc3_async(nft, min(NVSIZ, b.numelc - nft), dt1)
+    tb = b.group_wait()
+    assert ta == tb
+    assert np.array_equal(a.fsky(), b.fsky())
+    a.close()
+    b.close()
