@@ -109,11 +109,13 @@ int orad_get_time(orad_ctx *ctx, orad_time_t *t);
  * element state, and returns the group's DT2T candidate (min with *dt2t on entry). */
 int orad_sforc3(orad_ctx *ctx, int nft, int nel, double dt1, double *dt2t);
 /* The same group call without waiting (no reference counterpart: SFORC3 returns when
- * its group is done): enqueues the element pass of elements nft+1..nft+nel and returns;
- * the groups' time-step minimum is kept on the device.  orad_group_wait returns once
- * every queued group has run, lowers *dt2t to their minimum (the merge of the FORINT
- * group loop, resol.F:4134-4140) and returns ORAD_ENEGVOL if a brick saw a
- * non-positive volume.  orad_cforc3_async is the shell counterpart of orad_cforc3. */
+ * its group is done): records the element pass of elements nft+1..nft+nel and returns;
+ * consecutive groups with the same DT1 run as one launch (each element is computed on
+ * its own, so the results are those of separate calls), launched at the next call that
+ * does not continue the run or at any other entry point.  orad_group_wait returns once
+ * every group has run, lowers *dt2t to their minimum (the merge of the FORINT group
+ * loop, resol.F:4134-4140) and returns ORAD_ENEGVOL if a brick saw a non-positive
+ * volume.  orad_cforc3_async is the shell counterpart of orad_cforc3. */
 int orad_sforc3_async(orad_ctx *ctx, int nft, int nel, double dt1);
 int orad_cforc3_async(orad_ctx *ctx, int nft, int nel, double dt1);
 int orad_group_wait(orad_ctx *ctx, double *dt2t);

@@ -252,6 +252,8 @@ static int pack_node_words(orad_ctx *c) {
   return ORAD_OK;
 }
 
+static int flush_groups(orad_ctx *c);
+
 extern "C" {
 
 int orad_create(orad_ctx **out, int device, int precision) {
@@ -294,6 +296,7 @@ int orad_create(orad_ctx **out, int device, int precision) {
 
 int orad_destroy(orad_ctx *c) {
   if (!c) return ORAD_OK;
+  c->pend = false;          // groups recorded and never waited for are dropped
   (void)hipSetDevice(c->device);
   drop_graphs(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -321,6 +324,7 @@ const char *orad_last_error(const orad_ctx *c) { return c ? c->err.c_str() : "nu
 int orad_set_nodes(orad_ctx *c, int numnod, const double *x, const double *v, const double *ms,
                    const int *icodt) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (numnod <= 0 || !x || !v || !ms) return fail(c, ORAD_EINVAL, "set_nodes: bad arguments");
   if (c->have_nodes && numnod != c->numnod)
     return fail(c, ORAD_EINVAL, "set_nodes: node count differs from the mesh");
@@ -363,6 +367,7 @@ int orad_set_nodes(orad_ctx *c, int numnod, const double *x, const double *v, co
 
 int orad_set_bricks(orad_ctx *c, int numels, const int *ixs, const double *vol0) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (!c->have_nodes) return fail(c, ORAD_ESTATE, "set_bricks: call orad_set_nodes first");
   if (c->family != FAM_NONE) return fail(c, ORAD_ESTATE, "set_bricks: mesh already set");
   if (numels <= 0 || !ixs || !vol0) return fail(c, ORAD_EINVAL, "set_bricks: bad arguments");
@@ -456,6 +461,7 @@ int orad_set_bricks(orad_ctx *c, int numels, const int *ixs, const double *vol0)
 
 int orad_set_law2(orad_ctx *c, const double *p, const int *ip) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (!p || !ip) return fail(c, ORAD_EINVAL, "set_law2: null parameters");
   if (!(p[0] > 0.0) || !(p[1] > 0.0) || !(p[2] > 0.0))
     return fail(c, ORAD_EINVAL, "set_law2: rho0, G and K must be positive");
@@ -473,6 +479,7 @@ int orad_set_law2(orad_ctx *c, const double *p, const int *ip) {
 
 int orad_set_solid_prop(orad_ctx *c, double qa, double qb, double hvis, double dtfac) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (qa < 0.0 || qb < 0.0 || hvis < 0.0 || !(dtfac > 0.0))
     return fail(c, ORAD_EINVAL, "set_solid_prop: negative coefficient or DTFAC <= 0");
   c->prop = SProp{qa, qb, hvis, dtfac};
@@ -483,6 +490,7 @@ int orad_set_solid_prop(orad_ctx *c, double qa, double qb, double hvis, double d
 
 int orad_get_pon(orad_ctx *c, int *adsky, int *iads) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (c->family == FAM_NONE) return fail(c, ORAD_ESTATE, "get_pon: no mesh");
   if (adsky)
     for (int n = 0; n <= c->numnod; ++n) adsky[n] = c->h_adsky[n] + 1;
@@ -493,6 +501,7 @@ int orad_get_pon(orad_ctx *c, int *adsky, int *iads) {
 
 int orad_set_time(orad_ctx *c, const orad_time_t *t) {
   if (!c || !t) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->host_ncycle = t->ncycle;
@@ -521,6 +530,7 @@ static int model_time(orad_ctx *c, int m, orad_time_t *t) {
 
 int orad_get_time(orad_ctx *c, orad_time_t *t) {
   if (!c || !t) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   return model_time(c, 0, t);
 }
 
@@ -538,19 +548,10 @@ static int ready(orad_ctx *c) {
 
 }  // extern "C"
 
-// One element group enqueued on the context's stream (bricks: SFORC3, shells: CFORC3
-// or CZFORC3).  The group's time-step minimum goes into the device accumulator
-// scratch_dt2t, reset by the first group after a wait; dt1 is uploaded when it changes.
-static int group_enqueue(orad_ctx *c, int fam, int nft, int nel, double dt1) {
-  const char *who = fam == FAM_BRICK ? "sforc3" : "cforc3";
-  if (c->family != fam)
-    return fail(c, ORAD_ESTATE, std::string(who) + ": the mesh is not of " +
-                                    (fam == FAM_BRICK ? "bricks" : "shells"));
-  int r = ready(c);
-  if (r) return r;
-  const int ne = fam == FAM_BRICK ? c->numels : c->numelc;
-  if (nft < 0 || nel < 0 || nel > ne - nft)
-    return fail(c, ORAD_EINVAL, std::string(who) + ": group outside the element range");
+// Launches the pending run of groups (bricks: SFORC3, shells: CFORC3 or CZFORC3) on the
+// context's stream.  Its time-step minimum goes into the device accumulator
+// scratch_dt2t, reset by the first launch after a wait; dt1 is uploaded when it changes.
+static int group_launch(orad_ctx *c, int fam, int b, int e, double dt1) {
   HIPCHK(c, hipSetDevice(c->device));
   if (!c->group_open) {
     c->group_bits = 0x7FF0000000000000ull;
@@ -566,26 +567,69 @@ static int group_enqueue(orad_ctx *c, int fam, int nft, int nel, double dt1) {
   }
   c->group_open = true;
   if (fam == FAM_BRICK) {
-    HIPCHK(c, launch_elements(c, nft, nft + nel, c->scratch_dt1, c->scratch_dt2t, nullptr, true));
+    HIPCHK(c, launch_elements(c, b, e, c->scratch_dt1, c->scratch_dt2t, nullptr, true));
     return ORAD_OK;
   }
   const ShMat keep = c->shmat;
-  if (c->batched && nel > 0) {
+  if (c->batched && e > b) {
     // a group of batched models lies within one model and takes that model's material
-    const int m = (int)(std::upper_bound(c->h_elem_off.begin(), c->h_elem_off.end(), nft) -
+    const int m = (int)(std::upper_bound(c->h_elem_off.begin(), c->h_elem_off.end(), b) -
                         c->h_elem_off.begin()) - 1;
-    if (nel > c->h_elem_off[m + 1] - nft)
+    if (e > c->h_elem_off[m + 1])
       return fail(c, ORAD_EINVAL, "cforc3: a group of batched models must lie in one model");
     c->shmat = c->h_mats[m];
   }
-  hipError_t le = launch_elements(c, nft, nft + nel, c->scratch_dt1, c->scratch_dt2t, nullptr);
+  hipError_t le = launch_elements(c, b, e, c->scratch_dt1, c->scratch_dt2t, nullptr);
   c->shmat = keep;
   HIPCHK(c, le);
   return ORAD_OK;
 }
 
+// Launches the groups recorded by the asynchronous group calls, if any.  Every entry
+// point but the asynchronous group calls does this first, so that what it reads or
+// enqueues follows the groups.
+static int flush_groups(orad_ctx *c) {
+  if (!c->pend) return ORAD_OK;
+  c->pend = false;
+  return group_launch(c, c->pend_fam, c->pend_b, c->pend_e, c->pend_dt1);
+}
+
+// One group call: validated, then recorded; consecutive groups of the engine's
+// numbering with the same DT1 are run by one launch (a group's elements are computed
+// independently, so the slots, state and time-step minimum are those of separate
+// launches).
+static int group_enqueue(orad_ctx *c, int fam, int nft, int nel, double dt1) {
+  const char *who = fam == FAM_BRICK ? "sforc3" : "cforc3";
+  if (c->family != fam)
+    return fail(c, ORAD_ESTATE, std::string(who) + ": the mesh is not of " +
+                                    (fam == FAM_BRICK ? "bricks" : "shells"));
+  int r = ready(c);
+  if (r) return r;
+  const int ne = fam == FAM_BRICK ? c->numels : c->numelc;
+  if (nft < 0 || nel < 0 || nel > ne - nft)
+    return fail(c, ORAD_EINVAL, std::string(who) + ": group outside the element range");
+  if (c->batched) {
+    // batched models: one model's material per launch, so no merging across groups
+    if ((r = flush_groups(c))) return r;
+    return group_launch(c, fam, nft, nft + nel, dt1);
+  }
+  if (c->pend && c->pend_fam == fam && c->pend_e == nft && c->pend_dt1 == dt1) {
+    c->pend_e = nft + nel;
+    return ORAD_OK;
+  }
+  if ((r = flush_groups(c))) return r;
+  c->pend = true;
+  c->pend_fam = fam;
+  c->pend_b = nft;
+  c->pend_e = nft + nel;
+  c->pend_dt1 = dt1;
+  return ORAD_OK;
+}
+
 // Waits for the queued groups; lowers *dt2t to their minimum; MSTOP of bricks.
 static int group_wait(orad_ctx *c, double *dt2t) {
+  int r = flush_groups(c);
+  if (r) return r;
   if (!c->group_open) return ORAD_OK;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpyAsync(&c->group_bits, c->scratch_dt2t, sizeof(c->group_bits),
@@ -630,6 +674,7 @@ int orad_group_wait(orad_ctx *c, double *dt2t) {
 
 int orad_update_xv(orad_ctx *c, const double *x, const double *v, const double *vr) {
   if (!c || !x || !v) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (!c->have_nodes) return fail(c, ORAD_ESTATE, "update_xv: call orad_set_nodes first");
   if (vr && !c->have_rot)
     return fail(c, ORAD_ESTATE, "update_xv: rotational velocities need orad_set_nodes_rot");
@@ -648,6 +693,7 @@ int orad_update_xv(orad_ctx *c, const double *x, const double *v, const double *
 
 int orad_asspar4(orad_ctx *c, double *a, double *stifn) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   int r = ready(c);
   if (r) return r;
   HIPCHK(c, hipSetDevice(c->device));
@@ -723,6 +769,7 @@ static int build_graph(orad_ctx *c, int p) {
 
 int orad_run_cycles(orad_ctx *c, int ncycle, orad_time_t *t) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   int r = ready(c);
   if (r) return r;
   if (ncycle < 0) return fail(c, ORAD_EINVAL, "run_cycles: negative cycle count");
@@ -763,6 +810,7 @@ int orad_run_cycles(orad_ctx *c, int ncycle, orad_time_t *t) {
 
 int orad_profile_cycles(orad_ctx *c, int ncycle, double *elem_ms, double *node_ms) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   int r = ready(c);
   if (r) return r;
   HIPCHK(c, hipSetDevice(c->device));
@@ -790,6 +838,7 @@ int orad_profile_cycles(orad_ctx *c, int ncycle, double *elem_ms, double *node_m
 
 int orad_get_nodes(orad_ctx *c, double *x, double *v, double *d, double *a, double *stifn) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (!c->have_nodes) return fail(c, ORAD_ESTATE, "get_nodes: no nodes");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -814,6 +863,7 @@ int orad_get_nodes(orad_ctx *c, double *x, double *v, double *d, double *a, doub
 int orad_get_brick_state(orad_ctx *c, double *sig, double *eint, double *rho, double *pla,
                          double *qvis, double *offg, double *epsd) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (!c->have_bricks) return fail(c, ORAD_ESTATE, "get_brick_state: no mesh");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -842,6 +892,7 @@ int orad_get_brick_state(orad_ctx *c, double *sig, double *eint, double *rho, do
 int orad_set_brick_state(orad_ctx *c, const double *sig, const double *eint, const double *rho,
                          const double *pla, const double *qvis, const double *offg) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (!c->have_bricks) return fail(c, ORAD_ESTATE, "set_brick_state: no mesh");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -869,6 +920,7 @@ int orad_set_brick_state(orad_ctx *c, const double *sig, const double *eint, con
 
 int orad_get_fsky(orad_ctx *c, double *fsky) {
   if (!c || !fsky) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (c->family == FAM_SHELL) return get_fsky_shell(c, fsky);
   if (!c->have_bricks) return fail(c, ORAD_ESTATE, "get_fsky: no mesh");
   HIPCHK(c, hipSetDevice(c->device));
@@ -939,6 +991,7 @@ static int energies_range(orad_ctx *c, size_t e0, size_t e1, size_t n0, size_t n
 
 int orad_energies(orad_ctx *c, double *eint, double *ekin, double *ehour) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   int r = ready(c);
   if (r) return r;
   return energies_range(c, 0, (size_t)nelem(c), 0, (size_t)c->numnod, 0, c->nmodel, eint, ekin,
@@ -947,6 +1000,7 @@ int orad_energies(orad_ctx *c, double *eint, double *ekin, double *ehour) {
 
 int orad_energy_balance(orad_ctx *c, orad_balance_t *b) {
   if (!c || !b) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   int r = ready(c);
   if (r) return r;
   if (c->batched) return fail(c, ORAD_ESTATE, "energy_balance: one model per context");
@@ -956,6 +1010,7 @@ int orad_energy_balance(orad_ctx *c, orad_balance_t *b) {
 
 int orad_energies_noalloc(orad_ctx *c, double *eint, double *ekin, double *ehour) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   int r = ready(c);
   if (r) return r;
   if (!c->noalloc_scratch) return fail(c, ORAD_ESTATE, "energies_noalloc: no device scratch");
@@ -1077,6 +1132,7 @@ extern "C" {
 
 int orad_set_nodes_rot(orad_ctx *c, const double *vr, const double *in, const int *icodr) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (!c->have_nodes) return fail(c, ORAD_ESTATE, "set_nodes_rot: call orad_set_nodes first");
   if (!in) return fail(c, ORAD_EINVAL, "set_nodes_rot: nodal inertia IN is required");
   HIPCHK(c, hipSetDevice(c->device));
@@ -1106,6 +1162,7 @@ int orad_set_nodes_rot(orad_ctx *c, const double *vr, const double *in, const in
 
 int orad_set_fixvel(orad_ctx *c, int n, const int *nodes, const int *dir, const double *vel) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (!c->have_nodes) return fail(c, ORAD_ESTATE, "set_fixvel: call orad_set_nodes first");
   if (n < 0 || (n > 0 && (!nodes || !dir || !vel)))
     return fail(c, ORAD_EINVAL, "set_fixvel: bad arguments");
@@ -1130,6 +1187,7 @@ int orad_set_fixvel(orad_ctx *c, int n, const int *nodes, const int *dir, const 
 
 int orad_set_shells(orad_ctx *c, int numelc, const int *ixc, const double *thke) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (!c->have_nodes) return fail(c, ORAD_ESTATE, "set_shells: call orad_set_nodes first");
   if (c->family != FAM_NONE) return fail(c, ORAD_ESTATE, "set_shells: mesh already set");
   if (numelc <= 0 || !ixc || !thke) return fail(c, ORAD_EINVAL, "set_shells: bad arguments");
@@ -1226,6 +1284,7 @@ extern "C" {
 
 int orad_set_shell_law2(orad_ctx *c, const double *p, const int *ip) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (!p || !ip) return fail(c, ORAD_EINVAL, "set_shell_law2: null parameters");
   ShMat m;
   const std::string why = parse_shell_law2(p, ip, m);
@@ -1244,6 +1303,7 @@ int orad_set_shell_law2(orad_ctx *c, const double *p, const int *ip) {
 
 int orad_set_shell_prop(orad_ctx *c, const double *g, int ish, int npt) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (!g) return fail(c, ORAD_EINVAL, "set_shell_prop: null parameters");
   if (npt < 2 || npt > 5) return fail(c, ORAD_EINVAL, "set_shell_prop: NPT must be 2..5");
   if (ish < 0 || ish > 1) return fail(c, ORAD_EINVAL, "set_shell_prop: ISHEAR must be 0 or 1");
@@ -1265,6 +1325,7 @@ int orad_set_shell_prop(orad_ctx *c, const double *g, int ish, int npt) {
 
 int orad_set_shell_ishell(orad_ctx *c, int ishell, double dn) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (c->have_shells)
     return fail(c, ORAD_ESTATE, "set_shell_ishell: choose the formulation before orad_set_shells");
   if (ishell != 1 && ishell != 24)
@@ -1290,6 +1351,7 @@ int orad_cforc3(orad_ctx *c, int nft, int nel, double dt1, double *dt2t) {
 
 int orad_asspar4_rot(orad_ctx *c, double *a, double *ar, double *stifn, double *stifr) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (c->family != FAM_SHELL) return fail(c, ORAD_ESTATE, "asspar4_rot: the mesh is not of shells");
   int r = ready(c);
   if (r) return r;
@@ -1307,6 +1369,7 @@ int orad_asspar4_rot(orad_ctx *c, double *a, double *ar, double *stifn, double *
 
 int orad_get_nodes_rot(orad_ctx *c, double *vr, double *ar) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (!c->have_rot) return fail(c, ORAD_ESTATE, "get_nodes_rot: no rotational DOF");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1322,6 +1385,7 @@ int orad_get_nodes_rot(orad_ctx *c, double *vr, double *ar) {
 
 int orad_get_shell_state(orad_ctx *c, double *rows) {
   if (!c || !rows) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (c->family != FAM_SHELL) return fail(c, ORAD_ESTATE, "get_shell_state: no shell mesh");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1338,6 +1402,7 @@ int orad_get_shell_state(orad_ctx *c, double *rows) {
 
 int orad_set_shell_state(orad_ctx *c, const double *rows) {
   if (!c || !rows) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (c->family != FAM_SHELL) return fail(c, ORAD_ESTATE, "set_shell_state: no shell mesh");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1382,6 +1447,7 @@ extern "C" {
 int orad_set_shell_models(orad_ctx *c, int nmodel, const int *node_off, const int *elem_off,
                           const double *p, const int *ip) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (c->family != FAM_SHELL || !c->have_shmat)
     return fail(c, ORAD_ESTATE, "set_shell_models: set the shells and a law-2 material first");
   if (c->batched) return fail(c, ORAD_ESTATE, "set_shell_models: models already set");
@@ -1459,12 +1525,14 @@ int orad_set_shell_models(orad_ctx *c, int nmodel, const int *node_off, const in
 
 int orad_get_model_time(orad_ctx *c, int model, orad_time_t *t) {
   if (!c || !t) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (model < 0 || model >= c->nmodel) return fail(c, ORAD_EINVAL, "get_model_time: no such model");
   return model_time(c, model, t);
 }
 
 int orad_model_energies(orad_ctx *c, int model, double *eint, double *ekin, double *ehour) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   int r = ready(c);
   if (r) return r;
   if (model < 0 || model >= c->nmodel) return fail(c, ORAD_EINVAL, "model_energies: no such model");
@@ -1475,6 +1543,7 @@ int orad_model_energies(orad_ctx *c, int model, double *eint, double *ekin, doub
 
 int orad_get_launch(orad_ctx *c, orad_launch_t *l) {
   if (!c || !l) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   const bool mixed = c->precision == ORAD_MIXED;
   l->persistent = c->persist;
   l->lds_kb = c->batched ? c->batch_lds / 1024 : (c->elem_lds < 0 ? -1 : c->elem_lds / 1024);
@@ -1494,6 +1563,7 @@ int orad_get_launch(orad_ctx *c, orad_launch_t *l) {
 
 int orad_set_launch(orad_ctx *c, const orad_launch_t *l) {
   if (!c || !l) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (l->lds_kb > 160 || l->lds_kb < -1 || (l->persistent && !c->batched))
     return fail(c, ORAD_EINVAL, "set_launch: LDS in -1..160 KB; persistent needs batched models");
   const bool mixed = c->precision == ORAD_MIXED;
@@ -1631,6 +1701,7 @@ extern "C" {
 
 int orad_autotune(orad_ctx *c, int cycles, orad_launch_t *best, double *best_ms) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   int r = ready(c);
   if (r) return r;
   if (cycles < 2) return fail(c, ORAD_EINVAL, "autotune: at least 2 timed cycles");
@@ -1770,6 +1841,7 @@ extern "C" {
 
 int orad_adjoint_forward(orad_ctx *c, int ncycle, int every) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if ((c->family != FAM_SHELL && c->family != FAM_BRICK) || c->batched)
     return fail(c, ORAD_ESTATE, "adjoint_forward: one shell or brick model only");
   int r = ready(c);
@@ -1811,6 +1883,7 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
                          double *g_jc, double *g_thk, double *g_ms, double *g_in,
                          double *lam_x0, double *lam_v0, double *lam_vr0) {
   if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
   if (!c->adj || !c->adj->fin) return fail(c, ORAD_ESTATE, "adjoint_reverse: no forward run");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));

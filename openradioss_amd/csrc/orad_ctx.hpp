@@ -153,6 +153,9 @@ struct orad_ctx {
   unsigned long long group_bits = 0;
   double group_dt1[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int group_slot = 0;
+  bool pend = false;                  // groups recorded, not yet launched
+  int pend_fam = 0, pend_b = 0, pend_e = 0;
+  double pend_dt1 = 0.0;
   double *xv_stage = nullptr;         // orad_update_xv staging: X, V, VR (3 * numnod each)
   double *noalloc_scratch = nullptr;   // this device's file-scope scratch array (not owned)
 

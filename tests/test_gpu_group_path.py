@@ -89,3 +89,31 @@ def test_shell_async_groups_and_update_xv():
     assert np.array_equal(a.fsky(), b.fsky())
     a.close()
     b.close()
+
+
+def test_groups_out_of_order_and_reads_before_the_wait():
+    """Groups given backwards (no two consecutive, so each is its own launch) and a
+    read of the slots before orad_group_wait (it launches the recorded run first) give
+    the slots and DT2T of the synchronous calls."""
+    d = dk.brick_block(15, 11, 7, v_top=-5.0e4)
+    ref = en.BrickSolver(d, en.MIXED)
+    ref.run_cycles(10)
+    dt1 = ref.time().dt2
+    tref = ref.sforc3(0, ref.numels, dt1)
+    fref = ref.fsky()
+    tref2 = ref.sforc3(0, ref.numels, dt1)        # a second pass on the updated state
+    fref2 = ref.fsky()
+    ref.close()
+    s = en.BrickSolver(d, en.MIXED)
+    s.run_cycles(10)
+    starts = list(range(0, s.numels, NVSIZ))
+    for nft in reversed(starts):
+        s.sforc3_async(nft, min(NVSIZ, s.numels - nft), dt1)
+    assert np.array_equal(s.fsky(), fref)
+    assert s.group_wait() == tref
+    # consecutive groups again (one launch), read before the wait
+    for nft in starts:
+        s.sforc3_async(nft, min(NVSIZ, s.numels - nft), dt1)
+    assert np.array_equal(s.fsky(), fref2)
+    assert s.group_wait() == tref2
+    s.close()
