@@ -50,8 +50,9 @@ typedef struct {
  * -1 = keep the current one).  Every family: node_tile -- PARITH/ON slots the nodal
  * kernel stages per LDS tile (1024, 2048, 4096 within 48 KB; 0 = default).
  * adj_vjp -- the element vector-Jacobian product of orad_adjoint_reverse: 0 = the
- * hand-written reverse pass where the element has one (bricks; Belytschko-Tsay with
- * the law-2 radial return, IPLA = 0), forward-mode dual numbers otherwise; 1 =
+ * hand-written reverse pass where the element has one (bricks; Belytschko-Tsay and
+ * QEPH shells with the law-2 radial return, IPLA = 0), forward-mode dual numbers
+ * otherwise; 1 =
  * forward-mode dual numbers always (both give the same derivative to rounding).
  * orad_get_launch reports the values in use.  Nodes, slots and element state do
  * not depend on any of these; the element order changes only the rounding of the
@@ -301,8 +302,8 @@ int orad_autotune(orad_ctx *ctx, int cycles, orad_launch_t *best, double *best_m
  * VR (lam_vr0 zero for bricks); any output may be null.  The time-step sequence is
  * taken as fixed (exact when DTMX pins it).  The element derivatives are those of the
  * FP64 chain evaluated on the forward states, in FP64 for either form: bricks and
- * Belytschko-Tsay shells with IPLA = 0 by a hand-written reverse pass, QEPH and IPLA
- * 1/2 by dual numbers (orad_launch_t.adj_vjp); the context is left at the end of the
+ * both shell formulations with IPLA = 0 by a hand-written reverse pass, IPLA 1/2 by
+ * dual numbers (orad_launch_t.adj_vjp); the context is left at the end of the
  * forward run.  Law 2 without the strain-rate term only (C = 0; ORAD_EINVAL
  * otherwise): the rate filters (EPSD, EPSDP) and the shell thickness THK that the
  * rate factor reads are not part of the adjoint state. */

@@ -11,9 +11,10 @@
 // Reverse of cycle k, three launches:
 //   adj_node_kernel    adjoints of v_{k+1}, x, d -> adjoint of S and R per node,
 //                      of v_k through the update, and the MS / IN sensitivities;
-//   adj_elem_rev_kernel  Belytschko-Tsay with radial return (IPLA = 0): the
+//   adj_elem_rev_kernel  Belytschko-Tsay or QEPH with radial return (IPLA = 0): the
 //                      hand-written reverse pass of the CFORC3 chain
-//                      (orad_shell_adjoint.hpp), one thread per element;
+//                      (orad_shell_adjoint.hpp) or of the CZFORC3 chain
+//                      (orad_qeph_adjoint.hpp), one thread per element;
 //   adj_elem_kernel    otherwise, vector-Jacobian product of the element chain: the CFORC3
 //                      chain (orad_shell_chain.hpp), or for QEPH the CZFORC3 chain
 //                      (orad_qeph_chain.hpp), instantiated on Dual<K>, seeds K of the
@@ -32,6 +33,7 @@
 #include "orad_dual.hpp"
 #include "orad_qeph_chain.hpp"
 #include "orad_shell_adjoint.hpp"
+#include "orad_qeph_adjoint.hpp"
 #include "orad_brick_adjoint.hpp"
 #include "orad_brick_io.hpp"
 #include "orad_adjoint_args.hpp"
@@ -163,9 +165,10 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_kernel(AdjArgs<S> a, int b
   }
 }
 
-// One Belytschko-Tsay element per thread through the hand-written reverse pass: the
-// same outputs as the dual chunks (lcorner and lst_out set, g_thk and g_jc added to).
-template <typename S>
+// One Belytschko-Tsay or QEPH element per thread through the hand-written reverse
+// pass: the same outputs as the dual chunks (lcorner and lst_out set, g_thk and g_jc
+// added to).
+template <typename S, bool QEPH>
 __global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_rev_kernel(AdjArgs<S> a) {
   const int ne = a.numelc;
   const int e = blockIdx.x * ADJ_BLOCK + threadIdx.x;
@@ -195,8 +198,12 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_elem_rev_kernel(AdjArgs<S> a) {
     return i < 3 ? lamS[3 * (size_t)nd[c] + i] : lamR[3 * (size_t)nd[c] + i - 3];
   };
   auto lpre = [&](int j, double v) { lout[(size_t)j * ne + e] = v; };
-  shell_vjp_bt(xg, vg, vrg, a.thke[e], nd[2] == nd[3], stv, a.dt1, a.mat, jc, a.prop, a.k, lpost,
-               lslot, lpre, lx, lv, lvr, gthk, gjc);
+  if constexpr (QEPH)
+    qeph_vjp(xg, vg, vrg, a.thke[e], stv, a.dt1, a.mat, jc, a.prop, a.k, lpost, lslot, lpre, lx, lv,
+             lvr, gthk, gjc);
+  else
+    shell_vjp_bt(xg, vg, vrg, a.thke[e], nd[2] == nd[3], stv, a.dt1, a.mat, jc, a.prop, a.k,
+                 lpost, lslot, lpre, lx, lv, lvr, gthk, gjc);
   for (int it = a.prop.npt; it < 5; ++it) {     // points not in use carry no adjoint
 #pragma unroll
     for (int c = 0; c < 5; ++c) lout[(size_t)(5 * it + c) * ne + e] = 0.0;
@@ -497,9 +504,11 @@ static void launch_adj_elem_chunks(const AdjArgs<S> &a, hipStream_t s) {
 
 template <typename S>
 static void launch_adj_elem_form(bool qeph, bool rev, const AdjArgs<S> &a, hipStream_t s) {
-  if (rev && !qeph) {
-    const int nb = (a.numelc + ADJ_BLOCK - 1) / ADJ_BLOCK;
-    hipLaunchKernelGGL(adj_elem_rev_kernel<S>, dim3(nb), dim3(ADJ_BLOCK), 0, s, a);
+  const int nb = (a.numelc + ADJ_BLOCK - 1) / ADJ_BLOCK;
+  if (rev && qeph) {
+    hipLaunchKernelGGL((adj_elem_rev_kernel<S, true>), dim3(nb), dim3(ADJ_BLOCK), 0, s, a);
+  } else if (rev) {
+    hipLaunchKernelGGL((adj_elem_rev_kernel<S, false>), dim3(nb), dim3(ADJ_BLOCK), 0, s, a);
   } else if (qeph) {
     launch_adj_elem_chunks<S, true>(a, s);
   } else {

@@ -1973,8 +1973,9 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
   ab.g_jc = gjc;
   ab.mat = c->mat;
   ab.prop = c->prop;
-  // the hand-written reverse pass covers bricks and Belytschko-Tsay with radial return
-  const bool rev = c->adj_vjp == 0 && (brick || (!c->qeph() && c->shmat.ipla == 0));
+  // the hand-written reverse pass covers bricks and both shell formulations with
+  // radial return
+  const bool rev = c->adj_vjp == 0 && (brick || c->shmat.ipla == 0);
   const int nseg = (int)A->ck.size();
   std::vector<Scal> hs(L);
   for (int sgi = nseg - 1; sgi >= 0; --sgi) {

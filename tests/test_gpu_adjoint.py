@@ -29,11 +29,11 @@ Bars:
   * QEPH (Ishell=24): the same short runs at the same bar; over 2000 cycles a
     twisted elastic plate (test_adjoint_qeph_plate_128_vs_fd says why);
   * FP32-element forward with the FP64 adjoint vs the all-FP64 gradient: printed;
-  * the hand-written reverse pass of the Belytschko-Tsay element (the default) vs
-    the forward-mode dual-number VJP (adj_vjp=1) on the same forward run: every
-    output within 1e-10 of the largest of its kind (16 x 16 with plasticity,
-    triangles and random adjoints of the whole final state, both forms; the
-    128 x 128 adjoint study).
+  * the hand-written reverse passes of the Belytschko-Tsay and QEPH elements (the
+    default) vs the forward-mode dual-number VJP (adj_vjp=1) on the same forward
+    run: every output within 1e-10 of the largest of its kind (16 x 16 with
+    plasticity and random adjoints of the whole final state, both forms; BT with
+    triangles, QEPH flat and twisted; the 128 x 128 adjoint study for both).
 """
 import numpy as np
 import pytest
@@ -331,12 +331,32 @@ def test_reverse_pass_matches_dual_vjp(prec):
     assert_same_gradients(rev, dual, 1e-10, f"16x16 prec {prec}")
 
 
-def test_reverse_pass_matches_dual_vjp_plate_128():
-    deck, centre, wf, dtmx = setup_plate(n=128, patch=32)
+@pytest.mark.parametrize("prec", [en.FP64, en.MIXED])
+@pytest.mark.parametrize("twist", [0.0, 0.01])
+def test_reverse_pass_matches_dual_vjp_qeph(prec, twist):
+    """QEPH: the hand-written reverse pass of CZFORC3 (the default) vs the dual-number
+    VJP on the same forward run, random adjoints of the whole final state (SIG, PLA,
+    HOURG(1:12)).  Flat (the flat projection) and twisted (CZCORP5's warped
+    projection, every element at the same warp); points yield, so the hourglass
+    limiter's ZCFAC and SIGY terms carry derivatives."""
+    deck, centre, wf, dtmx = setup_plate(n=16, patch=6, ishell=24, twist=twist, ca_scale=0.5)
+    nn, ne = deck["x"].shape[0], deck["ixc"].shape[0]
+    rng = np.random.default_rng(12)
+    seeds = dict(lam_x=rng.standard_normal((nn, 3)), lam_v=1e-4 * rng.standard_normal((nn, 3)),
+                 lam_vr=1e-4 * rng.standard_normal((nn, 3)), lam_d=rng.standard_normal((nn, 3)),
+                 lam_state=1e-3 * rng.standard_normal((ne, 42)))
+    rev, dual = reverse_vs_duals(deck, prec, 60, 7, dtmx, seeds)
+    assert np.all(rev["g_jc"] != 0.0)          # points yielded
+    assert_same_gradients(rev, dual, 1e-10, f"QEPH 16x16 twist {twist} prec {prec}")
+
+
+@pytest.mark.parametrize("ishell", [1, 24])
+def test_reverse_pass_matches_dual_vjp_plate_128(ishell):
+    deck, centre, wf, dtmx = setup_plate(n=128, patch=32, ishell=ishell)
     lam_d = np.zeros((deck["x"].shape[0], 3))
     lam_d[centre, 2] = 1.0
     rev, dual = reverse_vs_duals(deck, en.FP64, 2000, 100, dtmx, dict(lam_d=lam_d))
-    assert_same_gradients(rev, dual, 1e-10, "128x128 2000 cycles")
+    assert_same_gradients(rev, dual, 1e-10, f"128x128 2000 cycles ishell {ishell}")
 
 
 def test_adjoint_refuses_rate_dependent_law_and_second_backward():
