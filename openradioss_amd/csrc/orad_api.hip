@@ -1837,6 +1837,34 @@ struct DevBufs {
   }
 };
 
+// The reverse sweep's stream (default priority, like the context's) and the events
+// that order it against the recompute on the context's stream.  Both streams are
+// drained before the buffers of the sweep are released.
+struct AdjStreams {
+  hipStream_t ctx_stream, s = nullptr;
+  hipEvent_t stashed[2] = {nullptr, nullptr}, read_done[2] = {nullptr, nullptr};
+  explicit AdjStreams(hipStream_t cs) : ctx_stream(cs) {}
+  hipError_t init() {
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+      e = hipEventCreateWithFlags(&stashed[i], hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&read_done[i], hipEventDisableTiming);
+    }
+    return e;
+  }
+  ~AdjStreams() {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+    (void)hipStreamSynchronize(ctx_stream);
+    for (int i = 0; i < 2; ++i) {
+      if (stashed[i]) (void)hipEventDestroy(stashed[i]);
+      if (read_done[i]) (void)hipEventDestroy(read_done[i]);
+    }
+  }
+};
+
 extern "C" {
 
 int orad_adjoint_forward(orad_ctx *c, int ncycle, int every) {
@@ -1898,10 +1926,6 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
   DevBufs B;
   double *lx, *lv, *ld, *lvr, *lamS, *lamR, *gms, *gin, *lst[2], *lcorner, *gthk, *gjc;
   int32_t *scorner, *selem;
-  double4 *sx, *sv, *svr;
-  double *sa, *sar;
-  char *sst;
-  Scal *sscal;
   HIPCHK(c, B.get(&lx, 3 * nn * 8));
   HIPCHK(c, B.get(&lv, 3 * nn * 8));
   HIPCHK(c, B.get(&ld, 3 * nn * 8));
@@ -1920,13 +1944,40 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
   // per-cycle stash slots, strides rounded to 256 bytes so every slot is 16-byte aligned
   const size_t sa_stride = (3 * nn + 31) / 32 * 32;        // doubles
   const size_t st_stride = (stb + 255) / 256 * 256;        // bytes
-  HIPCHK(c, B.get(&sx, (size_t)L * nn * 32, false));
-  HIPCHK(c, B.get(&sv, (size_t)L * nn * 32, false));
-  HIPCHK(c, B.get(&svr, (size_t)L * nn * 32 * (brick ? 0 : 1) + 256, false));
-  HIPCHK(c, B.get(&sa, (size_t)L * sa_stride * 8, false));
-  HIPCHK(c, B.get(&sar, (size_t)L * sa_stride * 8 * (brick ? 0 : 1) + 256, false));
-  HIPCHK(c, B.get(&sst, (size_t)L * st_stride, false));
-  HIPCHK(c, B.get(&sscal, (size_t)L * sizeof(Scal), false));
+  struct Stash {
+    double4 *sx, *sv, *svr;
+    double *sa, *sar;
+    char *sst;
+    Scal *sscal;
+  };
+  auto get_stash = [&](Stash &t) {
+    hipError_t e = B.get(&t.sx, (size_t)L * nn * 32, false);
+    if (e == hipSuccess) e = B.get(&t.sv, (size_t)L * nn * 32, false);
+    if (e == hipSuccess) e = B.get(&t.svr, (size_t)L * nn * 32 * (brick ? 0 : 1) + 256, false);
+    if (e == hipSuccess) e = B.get(&t.sa, (size_t)L * sa_stride * 8, false);
+    if (e == hipSuccess) e = B.get(&t.sar, (size_t)L * sa_stride * 8 * (brick ? 0 : 1) + 256, false);
+    if (e == hipSuccess) e = B.get(&t.sst, (size_t)L * st_stride, false);
+    if (e == hipSuccess) e = B.get(&t.sscal, (size_t)L * sizeof(Scal), false);
+    return e;
+  };
+  // Two stash sets when memory allows: the recompute of segment s-1 (on the context's
+  // stream) then runs while the reverse sweep of segment s (on its own stream) reads
+  // the other set; with one set the two alternate.
+  Stash SS[2];
+  HIPCHK(c, get_stash(SS[0]));
+  int nbuf = 1;
+  {
+    const size_t mark = B.ptrs.size();
+    if (get_stash(SS[1]) == hipSuccess) {
+      nbuf = 2;
+    } else {
+      (void)hipGetLastError();   // the failed allocation is not an error of the run
+      while (B.ptrs.size() > mark) { dfree(B.ptrs.back()); B.ptrs.pop_back(); }
+    }
+  }
+  AdjStreams R(c->stream);
+  HIPCHK(c, R.init());
+  hipStream_t as = R.s;
   // device element index of an engine element (bricks are stored in the library's order)
   auto dev_elem = [&](size_t e) { return brick ? (size_t)c->h_iperm[e] : e; };
   {
@@ -1977,12 +2028,17 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
   // radial return
   const bool rev = c->adj_vjp == 0 && (brick || c->shmat.ipla == 0);
   const int nseg = (int)A->ck.size();
-  std::vector<Scal> hs(L);
-  for (int sgi = nseg - 1; sgi >= 0; --sgi) {
-    const int k0 = sgi * A->every, len = std::min(A->every, A->ncycle - k0);
-    // recompute the segment: before each cycle one launch stashes its inputs (X, V,
-    // VR, element state) and the previous cycle's accelerations and scalars
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<Scal> hs[2] = {std::vector<Scal>(L), std::vector<Scal>(L)};
+  bool read_pending[2] = {false, false};   // a reverse sweep enqueued on `as` reads the set
+  auto seg_len = [&](int sgi) { return std::min(A->every, A->ncycle - sgi * A->every); };
+  auto buf_of = [&](int sgi) { return nbuf == 2 ? (sgi & 1) : 0; };
+  // recompute segment sgi into its stash set, on the context's stream: before each
+  // cycle one launch stashes its inputs (X, V, VR, element state) and the previous
+  // cycle's accelerations and scalars; ends with the scalars copied to the host
+  auto recompute = [&](int sgi) -> int {
+    const int b = buf_of(sgi), len = seg_len(sgi);
+    const Stash &t = SS[b];
+    if (read_pending[b]) HIPCHK(c, hipStreamWaitEvent(c->stream, R.read_done[b], 0));
     HIPCHK(c, A->ck[sgi]->restore(c));
     drop_graphs(c);
     c->cycle_write_a = 1;
@@ -1992,21 +2048,21 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
       void *dst[7];
       size_t bytes[7];
       int n = 0;
-      auto add = [&](const void *a, void *b, size_t nb) {
-        src[n] = a; dst[n] = b; bytes[n] = nb; ++n;
+      auto add = [&](const void *a, void *b2, size_t nb) {
+        src[n] = a; dst[n] = b2; bytes[n] = nb; ++n;
       };
       if (inputs) {
-        add(c->X4, sx + (size_t)j * nn, nn * 32);
-        add(c->V4, sv + (size_t)j * nn, nn * 32);
-        if (!brick) add(c->VR4, svr + (size_t)j * nn, nn * 32);
-        add(c->state, sst + (size_t)j * st_stride, stb);
+        add(c->X4, t.sx + (size_t)j * nn, nn * 32);
+        add(c->V4, t.sv + (size_t)j * nn, nn * 32);
+        if (!brick) add(c->VR4, t.svr + (size_t)j * nn, nn * 32);
+        add(c->state, t.sst + (size_t)j * st_stride, stb);
       }
       if (j > 0 || !inputs) {
         const int jp = inputs ? j - 1 : j;
-        add(c->A, sa + (size_t)jp * sa_stride, 3 * nn * 8);
-        if (!brick) add(c->AR, sar + (size_t)jp * sa_stride, 3 * nn * 8);
+        add(c->A, t.sa + (size_t)jp * sa_stride, 3 * nn * 8);
+        if (!brick) add(c->AR, t.sar + (size_t)jp * sa_stride, 3 * nn * 8);
         // the scalars the previous cycle wrote sit in the current parity's slot
-        add(&c->ctl->s[c->host_ncycle & 1], sscal + jp, sizeof(Scal));
+        add(&c->ctl->s[c->host_ncycle & 1], t.sscal + jp, sizeof(Scal));
       }
       return launch_adj_stash(n, src, dst, bytes, c->stream);
     };
@@ -2018,30 +2074,42 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
     if (e == hipSuccess) e = stash(len - 1, false);
     c->cycle_write_a = 0;
     HIPCHK(c, e);
-    HIPCHK(c, hipMemcpyAsync(hs.data(), sscal, len * sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hs[b].data(), t.sscal, len * sizeof(Scal), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipEventRecord(R.stashed[b], c->stream));
+    return ORAD_OK;
+  };
+  if (nseg > 0) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    // reverse through the segment: the nodal reverse of the segment's last cycle, then
-    // per cycle the element VJP and the gather, fused with the nodal reverse of the
-    // cycle before it inside the segment
-    auto slotX = [&](int j) { return sx + (size_t)j * nn; };
-    auto slotV = [&](int j) { return sv + (size_t)j * nn; };
+    if (int r = recompute(nseg - 1)) return r;
+  }
+  for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+    const int b = buf_of(sgi), len = seg_len(sgi);
+    const Stash &t = SS[b];
+    HIPCHK(c, hipEventSynchronize(R.stashed[b]));   // the segment's scalars are on the host
+    const std::vector<Scal> &h = hs[b];
+    // reverse through the segment on `as`: the nodal reverse of the segment's last
+    // cycle, then per cycle the element VJP and the gather, fused with the nodal
+    // reverse of the cycle before it inside the segment
+    auto slotX = [&](int j) { return t.sx + (size_t)j * nn; };
+    auto slotV = [&](int j) { return t.sv + (size_t)j * nn; };
     // no rotational DOF for bricks: the nodal reverse skips them on null VR4 / AR
-    auto slotW = [&](int j) { return brick ? (const double4 *)nullptr : svr + (size_t)j * nn; };
-    auto slotA = [&](int j) { return sa + (size_t)j * sa_stride; };
-    auto slotAR = [&](int j) { return brick ? (const double *)nullptr : sar + (size_t)j * sa_stride; };
+    auto slotW = [&](int j) { return brick ? (const double4 *)nullptr : t.svr + (size_t)j * nn; };
+    auto slotA = [&](int j) { return t.sa + (size_t)j * sa_stride; };
+    auto slotAR = [&](int j) { return brick ? (const double *)nullptr : t.sar + (size_t)j * sa_stride; };
     HIPCHK(c, launch_adj_node(c->numnod, slotX(len - 1), slotV(len - 1), slotW(len - 1),
-                              slotA(len - 1), slotAR(len - 1), hs[len - 1].dt2, hs[len - 1].dt12,
-                              lx, lv, ld, lvr, lamS, lamR, gms, gin, c->stream));
+                              slotA(len - 1), slotAR(len - 1), h[len - 1].dt2, h[len - 1].dt12,
+                              lx, lv, ld, lvr, lamS, lamR, gms, gin, as));
     for (int j = len - 1; j >= 0; --j) {
-      const double *stj = reinterpret_cast<const double *>(sst + (size_t)j * st_stride);
+      const double *stj = reinterpret_cast<const double *>(t.sst + (size_t)j * st_stride);
       if (brick) {
         ab.X4 = slotX(j);
         ab.V4 = slotV(j);
         ab.st = stj;
         ab.lst_in = lst[cur];
         ab.lst_out = lst[cur ^ 1];
-        ab.dt1 = hs[j].dt1;
-        HIPCHK(c, launch_adj_brick_elem(c->precision == ORAD_MIXED, rev, ab, c->stream));
+        ab.dt1 = h[j].dt1;
+        HIPCHK(c, launch_adj_brick_elem(c->precision == ORAD_MIXED, rev, ab, as));
       } else {
         ag.X4 = slotX(j);
         ag.V4 = slotV(j);
@@ -2049,21 +2117,28 @@ int orad_adjoint_reverse(orad_ctx *c, const double *lam_x, const double *lam_v,
         ag.st = stj;
         ag.lst_in = lst[cur];
         ag.lst_out = lst[cur ^ 1];
-        ag.dt1 = hs[j].dt1;
-        HIPCHK(c, launch_adj_elem(c->precision == ORAD_MIXED, c->qeph(), rev, ag, c->stream));
+        ag.dt1 = h[j].dt1;
+        HIPCHK(c, launch_adj_elem(c->precision == ORAD_MIXED, c->qeph(), rev, ag, as));
       }
       if (j > 0) {
         HIPCHK(c, launch_adj_gather_node(c->numnod, (int)ne, per, c->adsky, selem, scorner, lcorner,
                                          slotX(j - 1), slotV(j - 1), slotW(j - 1), slotA(j - 1),
-                                         slotAR(j - 1), hs[j - 1].dt2, hs[j - 1].dt12, lx, lv, ld,
-                                         lvr, lamS, lamR, gms, gin, c->stream));
+                                         slotAR(j - 1), h[j - 1].dt2, h[j - 1].dt12, lx, lv, ld,
+                                         lvr, lamS, lamR, gms, gin, as));
       } else {
         HIPCHK(c, launch_adj_gather(c->numnod, (int)ne, per, c->adsky, selem, scorner, lcorner,
-                                    lx, lv, lvr, c->stream));
+                                    lx, lv, lvr, as));
       }
       cur ^= 1;
     }
+    HIPCHK(c, hipEventRecord(R.read_done[b], as));
+    read_pending[b] = true;
+    // the next segment's recompute overlaps this sweep (or, with one stash set, waits
+    // for it)
+    if (sgi > 0)
+      if (int r = recompute(sgi - 1)) return r;
   }
+  HIPCHK(c, hipStreamSynchronize(as));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (g_jc) {
     std::vector<double> h(3 * ne);
