@@ -1,7 +1,7 @@
 """Recipe for oracle/_ref: the reference's own brick and shell leaf kernels,
 compiled from their sources where they lie under the reference tree, plus the
-test harnesses oracle/ref/harness_brick.F, harness_shell.F, harness_qeph.F and
-harness_nodal.F, linked into
+test harnesses oracle/ref/harness_brick.F, harness_shell.F, harness_qeph.F,
+harness_qbat.F and harness_nodal.F, linked into
 oracle/_ref/libref.so.
 
 Test infrastructure only.  Runs in the build container (amdflang from ROCm);
@@ -69,6 +69,13 @@ LEAF = [
     "engine/source/elements/shell/coquez/czfintn.F",
     "engine/source/elements/shell/coquez/czproj.F",
     "engine/source/elements/shell/coque/cupdtn3.F",
+    "engine/source/elements/shell/coqueba/cbacoor.F",
+    "engine/source/elements/shell/coqueba/cbadef.F",
+    "engine/source/elements/shell/coqueba/cbastra3.F",
+    "engine/source/elements/shell/coqueba/cbaener.F",
+    "engine/source/elements/shell/coqueba/cbavisc.F",
+    "engine/source/elements/shell/coqueba/cbafori.F",
+    "engine/source/elements/shell/coqueba/cbaproj.F",
     "engine/source/assembly/accele.F",
     "engine/source/constraints/general/bcs/bcs10.F",
     "engine/source/constraints/general/bcs/bcs1.F",
@@ -149,6 +156,7 @@ def main():
     comp(os.path.join(HERE, "ref", "harness_brick.F"))
     comp(os.path.join(HERE, "ref", "harness_shell.F"))
     comp(os.path.join(HERE, "ref", "harness_qeph.F"))
+    comp(os.path.join(HERE, "ref", "harness_qbat.F"))
     comp(os.path.join(HERE, "ref", "harness_nodal.F"))
     for rel in CXX:
         obj = os.path.join(OUT, re.sub(r"[^A-Za-z0-9_]", "_", rel) + ".o")

@@ -77,6 +77,14 @@ SH_EPSDP0 = 48        # epsdp[pt]
 SH_HOURX = slice(53, 60)   # QEPH: HOURG(6:12)
 SH_NST = 60
 SH_NST_BT = 53        # the Belytschko-Tsay state ends before HOURG(6:12)
+# QBAT (Ishell=12): columns 0..17 as above (HOUR unused), then per Gauss point g
+# (0..3, the order of CBAFORC3's IS/IR loops) and through-thickness point p:
+QB_SIG0 = 18          # sig[g][p][c] at 18 + 5*(5*g + p) + c
+QB_PLA0 = 118         # pla[g][p] at 118 + 5*g + p
+QB_EPSDP0 = 138       # epsdp[g][p]
+QB_FORPG0 = 158       # forpg[g][c] at 158 + 5*g + c (FORPG)
+QB_MOMPG0 = 178       # mompg[g][c] at 178 + 3*g + c (MOMPG)
+QB_NST = 190
 
 
 class ShMat(C.Structure):
@@ -167,6 +175,8 @@ def ref():
         _ref.ref_shell_group.argtypes = [C.c_int] + [C.c_void_p] * 15 + \
             [C.c_double] + [C.c_void_p] * 6
         _ref.ref_qeph_group.argtypes = [C.c_int] + [C.c_void_p] * 14 + \
+            [C.c_double] + [C.c_void_p] * 7
+        _ref.ref_qbat_group.argtypes = [C.c_int] + [C.c_void_p] * 15 + \
             [C.c_double] + [C.c_void_p] * 7
         _ref.ref_shell_constants.argtypes = [C.c_void_p]
         _ref.ref_shell_tables.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -521,6 +531,74 @@ def ref_qeph_group(xg, vg, vrg, thke, state, dt1, mat, prop):
         state[:, SH_PLA0 + p_] = pla[:, p_]
         state[:, SH_EPSDP0 + p_] = epsdp[:, p_]
     return dict(fsky=fsky.reshape(nel, 4, 8), dt=dte, ehou=ehou, dt2t=float(dt2t[0]))
+
+
+def qbat_state(ne, thk, offg=1.0):
+    st = np.zeros((ne, QB_NST))
+    st[:, SH_THK] = thk
+    st[:, SH_OFFG] = offg
+    return st
+
+
+def ref_qbat_group(xg, vg, vrg, thke, state, dt1, mat, prop):
+    """Reference QBAT routines (CBAFORC3 order, oracle/ref/harness_qbat.F) on a group:
+    xg, vg, vrg (nel,3,4); state (nel,190) updated in place.  Returns fsky (nel,4,8),
+    per-element dt, the viscous-damping energy increment (PARTSAV(8)) and DT2T."""
+    nel = xg.shape[0]
+    assert nel <= 128 and state.shape == (nel, QB_NST)
+    npt = prop["npt"]
+    xg, vg, vrg = (np.ascontiguousarray(a, dtype=np.float64) for a in (xg, vg, vrg))
+    thke = np.ascontiguousarray(thke, dtype=np.float64)
+    forc = np.asfortranarray(state[:, SH_FOR])
+    mom = np.asfortranarray(state[:, SH_MOM])
+    forpg = np.zeros((nel, 5, 4), order="F")
+    mompg = np.zeros((nel, 3, 4), order="F")
+    thk = np.ascontiguousarray(state[:, SH_THK])
+    eint = np.asfortranarray(state[:, SH_EINT])
+    epsd = np.ascontiguousarray(state[:, SH_EPSD])
+    offg = np.ascontiguousarray(state[:, SH_OFFG])
+    sig = np.zeros((nel, 5, 5, 4), order="F")
+    pla = np.zeros((nel, 5, 4), order="F")
+    epsdp = np.zeros((nel, 5, 4), order="F")
+    for g in range(4):
+        forpg[:, :, g] = state[:, QB_FORPG0 + 5 * g: QB_FORPG0 + 5 * g + 5]
+        mompg[:, :, g] = state[:, QB_MOMPG0 + 3 * g: QB_MOMPG0 + 3 * g + 3]
+        for p_ in range(5):
+            c0 = QB_SIG0 + 5 * (5 * g + p_)
+            sig[:, :, p_, g] = state[:, c0:c0 + 5]
+            pla[:, p_, g] = state[:, QB_PLA0 + 5 * g + p_]
+            epsdp[:, p_, g] = state[:, QB_EPSDP0 + 5 * g + p_]
+    mpar = np.array([mat[k] for k in ("rho0", "young", "nu", "shear", "a11", "a12", "ssp",
+                                      "ca", "cb", "cn", "epmx", "ymax", "cc", "epdr", "mexp",
+                                      "asrate_pm9")], np.float64)
+    ipar = np.array([mat["icc"], mat["vp"], mat["israte"], mat["ipla"], npt, prop["ish"]],
+                    np.int32)
+    ppar = np.array([prop[k] for k in ("hm", "hf", "hr", "dm", "fsh", "dtfac", "dtmin",
+                                       "dn", "cvis")], np.float64)
+    fsky = np.zeros((4 * nel, 8))
+    dte = np.zeros(nel)
+    evis = np.zeros(nel)
+    dt2t = np.zeros(1)
+    ref().ref_qbat_group(nel, _ptr(xg), _ptr(vg), _ptr(vrg), _ptr(thke), _ptr(forc),
+                         _ptr(mom), _ptr(forpg), _ptr(mompg), _ptr(thk), _ptr(eint),
+                         _ptr(epsd), _ptr(offg), _ptr(sig), _ptr(pla), _ptr(epsdp), dt1,
+                         _ptr(mpar), _ptr(ipar), _ptr(ppar), _ptr(fsky), _ptr(dte),
+                         _ptr(evis), _ptr(dt2t))
+    state[:, SH_FOR] = forc
+    state[:, SH_MOM] = mom
+    state[:, SH_THK] = thk
+    state[:, SH_EINT] = eint
+    state[:, SH_EPSD] = epsd
+    state[:, SH_OFFG] = offg
+    for g in range(4):
+        state[:, QB_FORPG0 + 5 * g: QB_FORPG0 + 5 * g + 5] = forpg[:, :, g]
+        state[:, QB_MOMPG0 + 3 * g: QB_MOMPG0 + 3 * g + 3] = mompg[:, :, g]
+        for p_ in range(npt):
+            c0 = QB_SIG0 + 5 * (5 * g + p_)
+            state[:, c0:c0 + 5] = sig[:, :, p_, g]
+            state[:, QB_PLA0 + 5 * g + p_] = pla[:, p_, g]
+            state[:, QB_EPSDP0 + 5 * g + p_] = epsdp[:, p_, g]
+    return dict(fsky=fsky.reshape(nel, 4, 8), dt=dte, evis=evis, dt2t=float(dt2t[0]))
 
 
 class ShellModel:
