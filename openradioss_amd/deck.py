@@ -140,11 +140,16 @@ def shell_mat_params(young=2.1e5, nu=0.3, rho0=7.85e-9, ca=250.0, cb=600.0, cn=0
 
 
 def shell_prop_params(npt=5, hm=0.01, hf=0.01, hr=0.01, dm=0.0, dtfac=0.9, dtmin=0.0,
-                      ishell=1, dn=0.015, cvis=1.0):
+                      ishell=1, dn=None, cvis=1.0):
     """/PROP/SHELL TYPE1 defaults (hm_read_prop01.F:155-244): shear factor 5/6
     (ISHEAR=0), hourglass coefficients 0.01 for Ishell=1.  Ishell=24 (QEPH) uses
     instead Dn (GEO(17) on input, 0.015 by default, moved to GEO(13)) and the
-    hourglass stiffness factor CVIS = 1 (GEO(17)) (hm_read_prop01.F:196-229)."""
+    hourglass stiffness factor CVIS = 1 (GEO(17)) (hm_read_prop01.F:196-229).
+    Ishell=12 (QBAT, IHBE=11) keeps Hm in GEO(13) (0.01 by default, :155,209),
+    which CNCOEF3 reads as its membrane damping coefficient (cncoef3.F:421-431);
+    dn is that GEO(13) value for both."""
+    if dn is None:
+        dn = hm if ishell == 12 else 0.015
     return dict(hm=hm, hf=hf, hr=hr, dm=dm, fsh=5.0 / 6.0, dtfac=dtfac, dtmin=dtmin,
                 ish=0, npt=npt, ishell=ishell, dn=dn, cvis=cvis)
 
@@ -169,8 +174,8 @@ def shell_masses(x, ixc, thk, rho0):
 
 
 def shell_plate(n=512, h=1.0, thk=1.0, patch=32, v0=1.5e4, noise=0.1, thk_noise=0.0,
-                seed=20261203, npt=5, ipla=0):
-    """Square BT shell plate: n x n elements of size h in the XY plane, clamped edges
+                seed=20261203, npt=5, ipla=0, ishell=1):
+    """Square shell plate (Ishell 1, 12 or 24): n x n elements of size h in the XY plane, clamped edges
     (all six DOF), the nodes of the central patch x patch elements starting at
     Vz = v0 * (1 + U(-noise, noise)); element thickness thk * (1 + U(-thk_noise,
     thk_noise)).  Counter-based (Philox) streams keyed on the seed."""
@@ -201,7 +206,7 @@ def shell_plate(n=512, h=1.0, thk=1.0, patch=32, v0=1.5e4, noise=0.1, thk_noise=
     ms, inr, area = shell_masses(x, ixc, t, mat["rho0"])
     return dict(x=x, v=v, vr=np.zeros_like(x), ms=ms, **{"in": inr}, icodt=icodt,
                 icodr=icodr, ixc=ixc, uid=np.arange(1, ne + 1, dtype=np.int32), thk=t,
-                mat=mat, prop=shell_prop_params(npt=npt))
+                mat=mat, prop=shell_prop_params(npt=npt, ishell=ishell))
 
 
 def square_tube(side, length, h, thk, mat, v_imp=-1.0e4, uid0=1):

@@ -208,6 +208,32 @@ void oracle_qeph_element(const double xg[3][4], const double vg[3][4], const dou
                          const orad_shmat_t *mat, const orad_shprop_t *prop,
                          orad_shout_t *out);
 
+/* QBAT (Ishell=12) element state: GBUF values of the BT layout (HOURG unused), then
+ * per Gauss point g (the order of CBAFORC3's IS/IR loops) and layer p the LBUF
+ * stress, plastic strain and strain rate, and the Gauss points' resultants
+ * FORPG/MOMPG (elbufdef_mod.F90); 190 doubles. */
+typedef struct {
+  double forc[5], mom[3];         /* GBUF%FOR, GBUF%MOM: Gauss point averages   */
+  double thk;                     /* GBUF%THK                                   */
+  double eint[2];                 /* GBUF%EINT                                  */
+  double epsd;                    /* GBUF%EPSD                                  */
+  double unused[5];
+  double offg;                    /* GBUF%OFF                                   */
+  double sig[4][ORAD_SH_NPT_MAX][5];
+  double pla[4][ORAD_SH_NPT_MAX];
+  double epsdp[4][ORAD_SH_NPT_MAX];
+  double forpg[4][5];             /* GBUF%FORPG                                 */
+  double mompg[4][3];             /* GBUF%MOMPG                                 */
+} orad_qbstate_t;
+
+/* CBAFORC3 chain (QBAT, Ishell=12, 2 x 2 Gauss points) for one quadrilateral
+ * (engine/source/elements/shell/coqueba/cbaforc3.F:440-1140).  out->ehou holds
+ * CBAVISC's damping energy increment (PARTSAV(8)). */
+void oracle_qbat_element(const double xg[3][4], const double vg[3][4], const double vrg[3][4],
+                         double thke, orad_qbstate_t *st, double dt1,
+                         const orad_shmat_t *mat, const orad_shprop_t *prop,
+                         orad_shout_t *out);
+
 /* COQINI tables for npt points (single-precision DATA values widened). */
 void oracle_shell_tables(int npt, double *z0, double *wf, double *wm);
 /* Derived constants used on the shell path, in a fixed order (for pinning). */
@@ -228,7 +254,8 @@ typedef struct {
   const int32_t *ixc;             /* 4*numelc, 0-based                           */
   const double *thke;             /* element thickness                           */
   const int32_t *adsky, *iads;    /* PARITH/ON CSR                               */
-  orad_shstate_t *st;
+  orad_shstate_t *st;             /* Ishell 1 and 24                            */
+  orad_qbstate_t *qst;            /* Ishell 12 (QBAT)                           */
   double *fsky;                   /* 8 per slot                                  */
   double *a, *ar, *stifn, *stifr;
   orad_shmat_t mat;

@@ -24,6 +24,7 @@
  * the constants against the reference's constant_mod values.
  */
 #include "orad_oracle.h"
+#include "orad_oracle_int.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -40,19 +41,7 @@
 #define HUNDRED 100.0
 #define TEN 10.0
 
-/* common_source/modules/constant_mod.F, re-derived from their definitions
- * (each a double operation on exact or previously rounded values). */
-typedef struct {
-  double EM01, EM02, EM03, EM04, EM05, EM06, EM07, EM15, EM20, EM30, EP20;
-  double ZEP1, ZEP01, ZEP03, ZEP07, ZEP09, ZEP19, ZEP81, ZEP4;
-  double TWOEM3, FOUREM3, FOUREM5, SIXEM5, NINEEM6, ZEP072169;
-  double ONEP4, ONEP41, ONEP414;
-  double ONE_OVER_9, ONE_OVER_12, FOUR_OVER_3, FOUR_OVER_5, FIVE_OVER_6, THIRD;
-  double THREE_OVER_4, THREE_HALF, SR2D2;
-  /* QEPH (CZFORC3 chain) */
-  double EM08, EM10, EM18, FIVE_OVER_4, ONE_OVER_16, ONE_OVER_64, ZEP85, ZEP999, FIVEP333;
-  double THREEP464, ZEP9EP30;
-} shconst_t;
+/* constant_mod.F values: shconst_t in orad_oracle_int.h */
 
 static shconst_t K;
 static int k_ready = 0;
@@ -121,7 +110,14 @@ static void init_constants(void) {
     K.THREEP464 = THREE + K.ZEP4 + ZEP06 + K.FOUREM3;
     K.ZEP9EP30 = 9.0 * EP29;
   }
+  K.EM12 = ONE / EP12;
+  K.ZEP3 = THREE / ten;
   k_ready = 1;
+}
+
+const shconst_t *oracle_shk(void) {
+  init_constants();
+  return &K;
 }
 
 void oracle_shell_constants(double *out) {
@@ -322,6 +318,16 @@ static void sigeps02c(const orad_shmat_t *m, double gs, double dt1, double asrat
   ezz = ezz / (ONE - nu);
   *thkn = *thkn + ezz * thklyl * (*off);
   sign[0] = sxx; sign[1] = syy; sign[2] = sxy; sign[3] = syz; sign[4] = szx;
+}
+
+void oracle_sigeps02c_point(const orad_shmat_t *m, double gs, double dt1, double asrate,
+                            double epsd_pg, double off_old, double *off, double *pla,
+                            double *epsdp, double *epchk, double *sigy, double *thkn,
+                            double thklyl, int npt, const double deps[5], const double sigo[5],
+                            double sign[5], double *etse_out) {
+  init_constants();
+  sigeps02c(m, gs, dt1, asrate, epsd_pg, off_old, off, pla, epsdp, epchk, sigy, thkn, thklyl,
+            npt, deps, sigo, sign, etse_out);
 }
 
 void oracle_shell_element(const double xg[3][4], const double vg[3][4], const double vrg[3][4],
@@ -1456,7 +1462,11 @@ void oracle_shell_cycle(orad_oshell_t *m) {
     }
     int tri = (m->ixc[4 * e + 2] == m->ixc[4 * e + 3]);
     orad_shout_t o;
-    if (m->prop.ishell == 24)
+    if (m->prop.ishell == 12)
+      oracle_qbat_element((const double(*)[4])xg, (const double(*)[4])vg,
+                          (const double(*)[4])vrg, m->thke[e], &m->qst[e], dt1, &m->mat,
+                          &m->prop, &o);
+    else if (m->prop.ishell == 24)
       oracle_qeph_element((const double(*)[4])xg, (const double(*)[4])vg,
                           (const double(*)[4])vrg, m->thke[e], &m->st[e], dt1, &m->mat,
                           &m->prop, &o);
@@ -1504,7 +1514,9 @@ void oracle_shell_cycle(orad_oshell_t *m) {
 
 void oracle_shell_energies(const orad_oshell_t *m, double *eint, double *ekin) {
   double ei = 0.0, ek = 0.0;
-  for (int e = 0; e < m->numelc; ++e) ei += m->st[e].eint[0] + m->st[e].eint[1];
+  for (int e = 0; e < m->numelc; ++e)
+    ei += (m->prop.ishell == 12) ? m->qst[e].eint[0] + m->qst[e].eint[1]
+                                 : m->st[e].eint[0] + m->st[e].eint[1];
   for (int n = 0; n < m->numnod; ++n) {
     const double *v = m->v + 3 * n, *w = m->vr + 3 * n;
     ek += HALF * m->ms[n] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) +

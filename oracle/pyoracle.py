@@ -110,7 +110,7 @@ class OShell(C.Structure):
     _fields_ = [("numnod", C.c_int), ("numelc", C.c_int)] + \
                [(n, C.c_void_p) for n in ("x", "v", "vr", "d", "ms", "inr", "icodt", "icodr",
                                           "fv_dir", "fv_val", "ixc", "thke", "adsky", "iads",
-                                          "st", "fsky", "a", "ar", "stifn", "stifr")] + \
+                                          "st", "qst", "fsky", "a", "ar", "stifn", "stifr")] + \
                [("mat", ShMat), ("prop", ShProp)] + \
                [(n, C.c_double) for n in ("tt", "dt1", "dt2", "dt12", "dt2old", "dtmx",
                                           "ehour")] + \
@@ -146,6 +146,9 @@ def lib():
                                               C.c_int, C.c_void_p, C.c_double,
                                               C.POINTER(ShMat), C.POINTER(ShProp),
                                               C.POINTER(ShOut)]
+        _lib.oracle_qbat_element.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                             C.c_void_p, C.c_double, C.POINTER(ShMat),
+                                             C.POINTER(ShProp), C.POINTER(ShOut)]
         _lib.oracle_qeph_element.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                              C.c_void_p, C.c_double, C.POINTER(ShMat),
                                              C.POINTER(ShProp), C.POINTER(ShOut)]
@@ -392,6 +395,19 @@ def qeph_element(xg, vg, vrg, thke, state, dt1, mat, prop):
                 ehou=out.ehou, alive=out.alive)
 
 
+def qbat_element(xg, vg, vrg, thke, state, dt1, mat, prop):
+    """One QBAT quadrilateral (Ishell=12) through the oracle chain; state (190,) in
+    place.  stic: FSKY(7) of each corner; ehou: CBAVISC's damping energy."""
+    xg, vg, vrg = (np.ascontiguousarray(a, dtype=np.float64) for a in (xg, vg, vrg))
+    assert xg.shape == (3, 4) and state.shape == (QB_NST,) and state.flags.c_contiguous
+    out = ShOut()
+    lib().oracle_qbat_element(_ptr(xg), _ptr(vg), _ptr(vrg), float(thke), _ptr(state), dt1,
+                              C.byref(shmat(mat)), C.byref(shprop(prop)), C.byref(out))
+    return dict(f=np.array(out.f).reshape(3, 4), m=np.array(out.m).reshape(3, 4),
+                sti=out.sti, stir=out.stir, stic=np.array(out.stic), dt=out.dt,
+                ehou=out.ehou, alive=out.alive)
+
+
 def shell_tables(npt):
     z, w, m = np.zeros(npt), np.zeros(npt), np.zeros(npt)
     lib().oracle_shell_tables(npt, _ptr(z), _ptr(w), _ptr(m))
@@ -621,7 +637,9 @@ class ShellModel:
         self.thke = np.ascontiguousarray(deck["thk"], dtype=np.float64)
         self.adsky, self.iads = build_pon_nc(nn, self.ixc, deck["uid"])
         ne = self.ixc.shape[0]
-        self.st = shell_state(ne, self.thke)
+        qbat = int(deck["prop"].get("ishell", 1)) == 12
+        self.st = qbat_state(ne, self.thke) if qbat else shell_state(ne, self.thke)
+        self.qst = self.st
         self.fsky = np.zeros((4 * ne, 8))
         self.a = np.zeros_like(self.x)
         self.ar = np.zeros_like(self.x)
@@ -630,8 +648,8 @@ class ShellModel:
         m = OShell()
         m.numnod, m.numelc = nn, ne
         for name in ("x", "v", "vr", "d", "ms", "inr", "icodt", "icodr", "fv_dir", "fv_val",
-                     "ixc", "thke", "adsky", "iads", "st", "fsky", "a", "ar", "stifn",
-                     "stifr"):
+                     "ixc", "thke", "adsky", "iads", "st", "qst", "fsky", "a", "ar",
+                     "stifn", "stifr"):
             setattr(m, name, _ptr(getattr(self, name)))
         m.mat = shmat(deck["mat"])
         m.prop = shprop(deck["prop"])

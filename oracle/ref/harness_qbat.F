@@ -100,7 +100,10 @@ C     ---- locals
      .   SIGKSI(MVSIZ,5),VOLPT(MVSIZ),ETSE(MVSIZ),G_IMP(MVSIZ),
      .   DPLA(MVSIZ),TSTAR(MVSIZ),HARDM(MVSIZ),LOFF(MVSIZ),
      .   SIGBX(MVSIZ),SIGBY(MVSIZ),SIGBXY(MVSIZ),DPLANL(MVSIZ),
-     .   WML(11,11), FORL(MVSIZ,5), MOML(MVSIZ,3)
+     .   WML(11,11)
+C     the Gauss point's resultants, shaped (NEL,.) as CBAFORC3 passes
+C     GBUF%FORPG(PTF) and GBUF%MOMPG(PTM) (cbaforc3.F:574-575,868-881)
+      my_real FORL(NEL_C,5), MOML(NEL_C,3)
       my_real
      .   STI(MVSIZ),STIR(MVSIZ),PARTSAV(29,MVSIZ),DT2T,BID(1),
      .   MSC(MVSIZ),DMELC(MVSIZ),DTEL(MVSIZ),SSP_MAX(MVSIZ),
