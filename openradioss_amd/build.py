@@ -26,6 +26,8 @@ SOURCES = [
     ("orad_shell_mixed.hip", ["-ffp-contract=fast"]),
     ("orad_qeph_fp64.hip", ["-ffp-contract=off"]),
     ("orad_qeph_mixed.hip", ["-ffp-contract=fast"]),
+    ("orad_qbat_fp64.hip", ["-ffp-contract=off"]),
+    ("orad_qbat_mixed.hip", ["-ffp-contract=fast"]),
     ("orad_nodal.hip", ["-ffp-contract=off"]),
     ("orad_adjoint.hip", ["-ffp-contract=off"]),
     ("orad_brick_order.hip", []),

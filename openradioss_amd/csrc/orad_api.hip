@@ -156,6 +156,13 @@ static hipError_t launch_elements(orad_ctx *c, int e0, int e1, const double *dt1
                                   unsigned long long *dt2t, double *ehou_part,
                                   bool group = false) {
   if (c->family == FAM_SHELL) {
+    if (c->qbat()) {
+      if (c->precision == ORAD_MIXED)
+        return launch_qbat_mixed(shell_args<float>(c), e0, e1, dt1p, dt2t, ehou_part,
+                                 c->xcd_remap, c->stream);
+      return launch_qbat_fp64(shell_args<double>(c), e0, e1, dt1p, dt2t, ehou_part, c->xcd_remap,
+                              c->stream);
+    }
     if (c->qeph()) {
       if (c->precision == ORAD_MIXED)
         return launch_qeph_mixed(shell_args<float>(c), e0, e1, dt1p, dt2t, ehou_part,
@@ -1075,6 +1082,10 @@ static ShConst shell_constants() {
     k.fivep333 = 16.0 / 3.0;
     k.threep464 = 3.0 + zep4 + zep06 + fourem3;
     k.zep9ep30 = 9.0 * (ep20 * ep[9]);
+    // QBAT chain
+    k.em12 = 1.0 / ep[12];
+    k.em03 = em03;
+    k.zep3 = 3.0 / ten;
   }
   return k;
 }
@@ -1227,7 +1238,7 @@ int orad_set_shells(orad_ctx *c, int numelc, const int *ixc, const double *thke)
       conn_cm[k * ne + e] = conn_em[4 * e + k];
       iads_cm[k * ne + e] = c->h_iads[4 * e + k];
       // STI entry of the slot: the element's, or for QEPH its diagonal's
-      se[c->h_iads[4 * e + k]] = (int32_t)(c->ishell == 24 ? e + (k & 1) * ne : e);
+      se[c->h_iads[4 * e + k]] = (int32_t)(c->nsti() == 2 ? e + (k & 1) * ne : e);
     }
   const size_t ts = c->tsize();
   HIPCHK(c, dalloc(&c->conn, 4 * ne * 4));
@@ -1247,9 +1258,10 @@ int orad_set_shells(orad_ctx *c, int numelc, const int *ixc, const double *thke)
   HIPCHK(c, hipMemset(c->sti, 0, nsti * ne * ts));
   HIPCHK(c, dalloc(&c->thke, ne * 8));
   HIPCHK(c, hipMemcpy(c->thke, thke, ne * 8, hipMemcpyHostToDevice));
-  const int bb = c->ishell == 24
-                     ? (c->precision == ORAD_MIXED ? qeph_block_mixed() : qeph_block_fp64())
-                     : (c->precision == ORAD_MIXED ? shell_block_mixed() : shell_block_fp64());
+  const bool mixed = c->precision == ORAD_MIXED;
+  const int bb = c->ishell == 24   ? (mixed ? qeph_block_mixed() : qeph_block_fp64())
+                 : c->ishell == 12 ? (mixed ? qbat_block_mixed() : qbat_block_fp64())
+                                   : (mixed ? shell_block_mixed() : shell_block_fp64());
   c->n_ehou_part = (int)((ne + bb - 1) / bb);
   HIPCHK(c, dalloc(&c->ehou_part, (size_t)c->n_ehou_part * sizeof(double)));
   HIPCHK(c, hipMemset(c->ehou_part, 0, (size_t)c->n_ehou_part * sizeof(double)));
@@ -1328,12 +1340,14 @@ int orad_set_shell_ishell(orad_ctx *c, int ishell, double dn) {
   if (int fr = flush_groups(c)) return fr;
   if (c->have_shells)
     return fail(c, ORAD_ESTATE, "set_shell_ishell: choose the formulation before orad_set_shells");
-  if (ishell != 1 && ishell != 24)
-    return fail(c, ORAD_EINVAL, "set_shell_ishell: Ishell must be 1 (BT) or 24 (QEPH)");
+  if (ishell != 1 && ishell != 24 && ishell != 12)
+    return fail(c, ORAD_EINVAL, "set_shell_ishell: Ishell must be 1 (BT), 24 (QEPH) or 12 (QBAT)");
   if (!(dn >= 0.0)) return fail(c, ORAD_EINVAL, "set_shell_ishell: Dn must be >= 0");
-  // Dn = 0 on input means 0.015 (ZEP01 + FIVEEM3, hm_read_prop01.F:199, cncoef3.F:259-264)
   c->ishell = ishell;
-  c->qeph_dn = (dn == 0.0) ? 1.0 / 100.0 + 5.0 / 1000.0 : dn;
+  // QEPH: Dn = 0 on input means 0.015 (ZEP01 + FIVEEM3, hm_read_prop01.F:199,
+  // cncoef3.F:259-264).  QBAT: GEO(13) as given; CNCOEF3 reads 0 as EM3
+  // (cncoef3.F:421-431), which the element chain applies.
+  c->qeph_dn = (ishell == 24 && dn == 0.0) ? 1.0 / 100.0 + 5.0 / 1000.0 : dn;
   c->shprop.ishell = ishell;
   c->shprop.dn = c->qeph_dn;
   c->shprop.cvis = 1.0;
@@ -1451,6 +1465,8 @@ int orad_set_shell_models(orad_ctx *c, int nmodel, const int *node_off, const in
   if (c->family != FAM_SHELL || !c->have_shmat)
     return fail(c, ORAD_ESTATE, "set_shell_models: set the shells and a law-2 material first");
   if (c->batched) return fail(c, ORAD_ESTATE, "set_shell_models: models already set");
+  if (c->qbat())
+    return fail(c, ORAD_EINVAL, "set_shell_models: batched models take Ishell 1 or 24, not QBAT");
   if (nmodel < 1 || !node_off || !elem_off || (p == nullptr) != (ip == nullptr))
     return fail(c, ORAD_EINVAL, "set_shell_models: bad arguments");
   if (node_off[0] != 0 || elem_off[0] != 0 || node_off[nmodel] != c->numnod ||
@@ -1881,6 +1897,8 @@ int orad_adjoint_forward(orad_ctx *c, int ncycle, int every) {
   // factor, i.e. when C != 0.
   if ((c->family == FAM_SHELL ? c->shmat.cc : c->mat.cc) != 0.0)
     return fail(c, ORAD_EINVAL, "adjoint_forward: rate-dependent law 2 (C != 0) not differentiated");
+  if (c->qbat())
+    return fail(c, ORAD_EINVAL, "adjoint_forward: QBAT shells (Ishell 12) are not differentiated");
   HIPCHK(c, hipSetDevice(c->device));
   free_adjoint(c);
   AdjStore *a = new (std::nothrow) AdjStore();
@@ -2185,12 +2203,12 @@ static int get_fsky_shell(orad_ctx *c, double *fsky) {
     HIPCHK(c, hipMemcpy(f.data(), c->fsky, 6 * ns * 8, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(st.data(), c->sti, nst * 8, hipMemcpyDeviceToHost));
   }
-  const bool qeph = c->qeph();
+  const bool two = c->nsti() == 2;   // STI per diagonal (QEPH, QBAT)
   for (size_t e = 0; e < ne; ++e)
     for (int k = 0; k < 4; ++k) {
       const size_t sl = (size_t)c->h_iads[4 * e + k];
       for (int j = 0; j < 6; ++j) fsky[8 * sl + j] = f[6 * sl + j];
-      fsky[8 * sl + 6] = qeph ? st[e + (k & 1) * ne] : st[e];
+      fsky[8 * sl + 6] = two ? st[e + (k & 1) * ne] : st[e];
       fsky[8 * sl + 7] = 0.0;
     }
   return ORAD_OK;

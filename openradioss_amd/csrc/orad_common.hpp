@@ -86,7 +86,8 @@ struct ShProp {
   int ish, npt;
   double z0[5], wf[5], wm[5];
   // formulation: 1 Belytschko-Tsay (CFORC3), 24 QEPH (CZFORC3) with its numerical
-  // damping Dn (GEO(13)) and hourglass stiffness factor CVIS (GEO(17), always 1)
+  // damping Dn (GEO(13)) and hourglass stiffness factor CVIS (GEO(17), always 1),
+  // 12 QBAT (CBAFORC3) with GEO(13) = Hm as its membrane damping DN (EM3 when 0)
   int ishell;
   double dn, cvis;
 };
@@ -99,6 +100,8 @@ struct ShConst {
   // QEPH chain
   double em08, em10, em18, one_over_12, one_over_16, one_over_64, zep85, zep999, fivep333;
   double threep464, zep9ep30;
+  // QBAT chain
+  double em12, em03, zep3;
 };
 
 // One shell PARITH/ON slot: corner force and moment (FSKY(1:6,·)).  STI
@@ -115,7 +118,15 @@ enum {
   SC_EPSDP = 48,   // + pt
   SC_NCOL = 53,
   // QEPH (Ishell=24) keeps HOURG(6:12) after the Belytschko-Tsay columns
-  SC_HOURX = 53, SC_NCOL_QEPH = 60
+  SC_HOURX = 53, SC_NCOL_QEPH = 60,
+  // QBAT (Ishell=12): columns 0..17 as above (HOURG unused), then per Gauss point
+  // g = 0..3 (CBAFORC3's IS/IR order) and layer p = 0..4
+  QC_SIG = 18,     // + 5*(5*g + p) + component
+  QC_PLA = 118,    // + 5*g + p
+  QC_EPSDP = 138,  // + 5*g + p
+  QC_FORPG = 158,  // + 5*g + component (GBUF%FORPG)
+  QC_MOMPG = 178,  // + 3*g + component (GBUF%MOMPG)
+  SC_NCOL_QBAT = 190
 };
 
 template <typename T>
@@ -127,8 +138,8 @@ struct ShellArgs {
   const double4 *V4;        // (vx, vy, vz, codes)
   const double4 *VR4;       // (vrx, vry, vrz, IN)
   const double *thke;       // element thickness THKE (FP64)
-  T *st;                    // [SC_NCOL][ne], [SC_NCOL_QEPH][ne] for QEPH
-  T *sti;                   // STI per element (FSKY(7,·)), per diagonal for QEPH
+  T *st;                    // [SC_NCOL][ne]; [SC_NCOL_QEPH][ne] QEPH; [SC_NCOL_QBAT][ne] QBAT
+  T *sti;                   // STI per element (FSKY(7,·)), per diagonal for QEPH and QBAT
   Slot6<T> *fsky;
   CycleCtl *ctl;
   ShMat mat;

@@ -51,6 +51,12 @@ hipError_t launch_qeph_batch_fp64(const ShellArgs<double> &, const BatchArgs &, 
 hipError_t launch_qeph_batch_mixed(const ShellArgs<float> &, const BatchArgs &, int, int, int,
                                    int, int, hipStream_t);
 int qeph_batch_occupancy_fp64(int);
+hipError_t launch_qbat_fp64(const ShellArgs<double> &, int, int, const double *,
+                            unsigned long long *, double *, int, hipStream_t);
+hipError_t launch_qbat_mixed(const ShellArgs<float> &, int, int, const double *,
+                             unsigned long long *, double *, int, hipStream_t);
+int qbat_block_fp64();
+int qbat_block_mixed();
 int qeph_batch_occupancy_mixed(int);
 int shell_batch_occupancy_fp64(int);
 int shell_batch_occupancy_mixed(int);
@@ -112,8 +118,8 @@ struct orad_ctx {
   ShMat shmat{};
   ShProp shprop{};
   ShConst shk{};
-  int ishell = 1;           // shell formulation (orad_set_shell_ishell): 1 BT, 24 QEPH
-  double qeph_dn = 0.015;   // QEPH numerical damping Dn
+  int ishell = 1;           // shell formulation (orad_set_shell_ishell): 1 BT, 24 QEPH, 12 QBAT
+  double qeph_dn = 0.015;   // GEO(13): QEPH numerical damping Dn, QBAT membrane damping Hm
   double4 *VR4 = nullptr;
   double *AR = nullptr, *STIFR = nullptr, *thke = nullptr;
   // imposed velocities (FIXVEL) and rotational codes, folded into V4.w on upload
@@ -192,8 +198,11 @@ struct orad_ctx {
 
   size_t ctl_bytes() const { return (size_t)nmodel * sizeof(CycleCtl); }
   bool qeph() const { return family == FAM_SHELL && ishell == 24; }
-  int shncol() const { return ishell == 24 ? SC_NCOL_QEPH : SC_NCOL; }   // shell state columns
-  int nsti() const { return ishell == 24 ? 2 : 1; }                      // STI entries per shell
+  bool qbat() const { return family == FAM_SHELL && ishell == 12; }
+  int shncol() const {                                                  // shell state columns
+    return ishell == 24 ? SC_NCOL_QEPH : ishell == 12 ? SC_NCOL_QBAT : SC_NCOL;
+  }
+  int nsti() const { return (ishell == 24 || ishell == 12) ? 2 : 1; }    // STI entries per shell
   size_t tsize() const { return precision == ORAD_MIXED ? sizeof(float) : sizeof(double); }
 };
 

@@ -33,6 +33,7 @@ EXPORTS = (
 NIXC = 7
 SH_NCOL = 53          # Belytschko-Tsay state values per shell
 SH_NCOL_QEPH = 60     # QEPH: HOURG(6:12) appended
+SH_NCOL_QBAT = 190    # QBAT: per Gauss point layer stresses, strains and resultants
 SH_PLA0 = 43          # state column of PLA of point 1 (PLA(1..5) at 43..47)
 
 
@@ -374,8 +375,9 @@ SHMAT_KEYS = ("rho0", "young", "nu", "shear", "a11", "a12", "ssp", "ca", "cb", "
 
 class ShellSolver:
     """One shell model (law 2) resident on one GPU with 6-DOF nodes: the CFORC3
-    path (Belytschko-Tsay, prop ishell = 1) or the CZFORC3 path (QEPH, ishell = 24,
-    numerical damping prop dn)."""
+    path (Belytschko-Tsay, prop ishell = 1), the CZFORC3 path (QEPH, ishell = 24,
+    numerical damping prop dn) or the CBAFORC3 path (QBAT, ishell = 12, membrane
+    damping prop dn = GEO(13))."""
 
     def __init__(self, deck, precision=FP64, device=0):
         self.lib = load_library()
@@ -407,7 +409,7 @@ class ShellSolver:
         self._chk(self.lib.orad_set_shell_prop(h, _p(g), int(prop["ish"]), int(prop["npt"])))
         self.ishell = int(prop.get("ishell", 1))
         self._chk(self.lib.orad_set_shell_ishell(h, self.ishell, float(prop.get("dn", 0.0))))
-        self.ncol = SH_NCOL_QEPH if self.ishell == 24 else SH_NCOL
+        self.ncol = {24: SH_NCOL_QEPH, 12: SH_NCOL_QBAT}.get(self.ishell, SH_NCOL)
         ixc = ixc_from_deck(deck)
         thk = _f64(deck["thk"], (self.numelc,))
         self._chk(self.lib.orad_set_shells(h, self.numelc, _p(ixc), _p(thk)))
