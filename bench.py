@@ -48,16 +48,23 @@ def node_bytes(ne, nn, tsize):
     return 8 * ne * 3 * tsize + nn * (8 + 32 + 32 + 24 + 32 + 32 + 24)
 
 
-def shell_elem_bytes(ne, nn, tsize, qeph=False):
+def shell_elem_bytes(ne, nn, tsize, ishell=1):
     """Algorithmic bytes of one shell element-kernel launch (5 points): connectivity
     and slot addresses (4+4 int32), THKE (FP64), the 53 state columns read
     (element: FOR(5) MOM(3) THK EINT(2) EPSD HOUR(5) OFFG; points: SIG(5) PLA EPSD)
     and all but OFFG written, STI, 4 slots of 6 values, and the X4/V4/VR4 node
-    records once per node.  QEPH adds HOURG(6:12) (60 columns) and writes one STI
-    per diagonal."""
-    ncol, nsti = (60, 2) if qeph else (53, 1)
-    reads = 16 + 16 + 8 + ncol * tsize
-    writes = (ncol - 1) * tsize + nsti * tsize + 4 * 6 * tsize
+    records once per node.  QEPH (ishell 24) adds HOURG(6:12) (60 columns) and writes
+    one STI per diagonal.  QBAT (ishell 12, 2 x 2 Gauss points) reads FOR(3), THK,
+    EINT(2), EPSD, OFFG and per Gauss point the 5 layers' SIG(5), PLA, EPSD and the
+    resultants FORPG(5), MOMPG(3) (178 columns), writes them back with FOR(5) and
+    MOM(3) (184 columns) and one STI per diagonal."""
+    if ishell == 12:
+        rcol, wcol, nsti = 178, 184, 2
+    else:
+        ncol, nsti = (60, 2) if ishell == 24 else (53, 1)
+        rcol, wcol = ncol, ncol - 1
+    reads = 16 + 16 + 8 + rcol * tsize
+    writes = wcol * tsize + nsti * tsize + 4 * 6 * tsize
     return ne * (reads + writes) + nn * 96
 
 
@@ -79,14 +86,12 @@ def _shell_forces(s, dt1):
 
 def shell_plate_leg(n, steps, err_cycles, ishell=1):
     """The 512 x 512 plate (5 points, law 2, clamped edges, central 32 x 32 patch at
-    15 m/s +-10%) with Belytschko-Tsay (ishell 1) or QEPH (ishell 24) shells:
-    element-cycles/s of both forms and the mixed form's error against the FP64 form
-    after `steps` and after `err_cycles` cycles."""
-    deck = dk.shell_plate(n=n)
-    qeph = ishell == 24
-    deck["prop"] = dict(deck["prop"], ishell=ishell)
+    15 m/s +-10%) with Belytschko-Tsay (ishell 1), QEPH (ishell 24) or QBAT (ishell
+    12) shells: element-cycles/s of both forms and the mixed form's error against the
+    FP64 form after `steps` and after `err_cycles` cycles."""
+    deck = dk.shell_plate(n=n, ishell=ishell)
     ne, nn = deck["ixc"].shape[0], deck["x"].shape[0]
-    kind = "QEPH (Ishell=24)" if qeph else "BT"
+    kind = {24: "QEPH (Ishell=24)", 12: "QBAT (Ishell=12)"}.get(ishell, "BT")
     res = {"workload": f"{kind} shell plate {n}x{n} ({ne} shells, {nn} nodes), 5 points, law 2",
            "elements": ne, "nodes": nn, "cycles_timed": steps - 2}
     snaps = {}
@@ -109,7 +114,7 @@ def shell_plate_leg(n, steps, err_cycles, ishell=1):
         snaps[name] = snap
         s.close()
         ts = 4 if prec == en.MIXED else 8
-        eb = shell_elem_bytes(ne, nn, ts, qeph)
+        eb = shell_elem_bytes(ne, nn, ts, ishell)
         res[name] = dict(value=ne * (steps - 2) / dt, ms_per_step=1e3 * dt / (steps - 2),
                          elem_kernel_ms=el, node_kernel_ms=nd,
                          elem_kernel_gbs=eb / (el * 1e-3) / 1e9,
@@ -507,6 +512,9 @@ def main():
     ap.add_argument("--no-qeph", action="store_true", help="skip the 512^2 QEPH plate leg")
     ap.add_argument("--qeph-steps", type=int, default=1000)
     ap.add_argument("--qeph-err-cycles", type=int, default=1000)
+    ap.add_argument("--no-qbat", action="store_true", help="skip the 512^2 QBAT plate leg")
+    ap.add_argument("--qbat-steps", type=int, default=500)
+    ap.add_argument("--qbat-err-cycles", type=int, default=1000)
     ap.add_argument("--no-tubes", action="store_true", help="skip the 256-tube batch leg")
     ap.add_argument("--tubes", type=int, default=256)
     ap.add_argument("--tube-steps", type=int, default=3000)
@@ -558,6 +566,9 @@ def main():
     qeph = None
     if not args.no_qeph:
         qeph = shell_plate_leg(args.shell_n, args.qeph_steps, args.qeph_err_cycles, ishell=24)
+    qbat = None
+    if not args.no_qbat:
+        qbat = shell_plate_leg(args.shell_n, args.qbat_steps, args.qbat_err_cycles, ishell=12)
     tubes = None
     if not args.no_tubes:
         tubes = tube_batch_leg(args.tubes, args.tube_steps, args.tube_solo_cycles,
@@ -610,6 +621,7 @@ def main():
         "mixed_vs_fp64_error": err,
         "shell_plate": shell,
         "qeph_plate": qeph,
+        "qbat_plate": qbat,
         "tube_batch": tubes,
         "adjoint_plate": adjoint,
         "adjoint_plate_qeph": adjoint_qeph,

@@ -234,10 +234,14 @@ int orad_set_shell_prop(orad_ctx *ctx, const double *g, int ish, int npt);
 /* Shell formulation, /PROP/SHELL Ishell (IPARG(23), hm_read_prop01.F:180-229):
  * 1 = Belytschko-Tsay (CFORC3, the default), 24 = QEPH (CZFORC3,
  * shell/coquez/czforc3.F:436-808) with numerical damping dn = Dn (GEO(13); 0
- * means the default 0.015) and CVIS = 1.  Must precede orad_set_shells: QEPH
- * elements keep 60 state values (HOURG(6:12) after the 53 of BT) and one STI per
- * diagonal (FSKY(7) = STI*FACN, CUPDTN3P cupdtn3.F:488-700).  Batched models and
- * the adjoint take either formulation. */
+ * means the default 0.015) and CVIS = 1, 12 = QBAT (CBAFORC3,
+ * shell/coqueba/cbaforc3.F:440-1140, 2 x 2 Gauss points) with dn = GEO(13) = Hm,
+ * the membrane damping of CBAVISC (0 means EM3, cncoef3.F:421-431).  Must precede
+ * orad_set_shells: QEPH elements keep 60 state values (HOURG(6:12) after the 53
+ * of BT), QBAT elements 190 (columns 0..17 as BT, then per Gauss point g and layer
+ * p: SIG at 18+5(5g+p), PLA at 118+5g+p, EPSD at 138+5g+p, FORPG at 158+5g,
+ * MOMPG at 178+3g); both keep one STI per diagonal (FSKY(7) = STI*FACN, CUPDTN3P
+ * cupdtn3.F:488-700).  Batched models and the adjoint take Ishell 1 or 24. */
 int orad_set_shell_ishell(orad_ctx *ctx, int ishell, double dn);
 
 /* Drop-in for one shell group, CFORC3 semantics (shell/coque/cforc3.F:398-746,

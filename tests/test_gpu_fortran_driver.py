@@ -54,12 +54,11 @@ def test_fortran_driver_mixed_runs(tmp_path):
     assert np.linalg.norm(dd - m.d) / np.linalg.norm(m.d) < 1e-4
 
 
-@pytest.mark.parametrize("ishell", [1, 24])
+@pytest.mark.parametrize("ishell", [1, 24, 12])
 def test_fortran_shell_group_loop_equals_device_cycle(tmp_path, ishell):
-    # CFORC3 / CZFORC3 drop-in per 128-shell group, ASSPAR4 with rotations, the 6-DOF
+    # CFORC3 / CZFORC3 / CBAFORC3 drop-in per 128-shell group, ASSPAR4 with rotations, the 6-DOF
     # nodal update on the host: bit for bit the device-resident cycle, and the oracle
-    d = dk.shell_plate(n=14, patch=4, v0=1.5e4)
-    d["prop"] = dict(d["prop"], ishell=ishell)
+    d = dk.shell_plate(n=14, patch=4, v0=1.5e4, ishell=ishell)
     deckf = tmp_path / "shell.bin"
     en.write_shell_deck(d, deckf)
     nn = d["x"].shape[0]
