@@ -291,6 +291,22 @@ int orad_get_launch(orad_ctx *ctx, orad_launch_t *l);
  * slot tile. */
 int orad_autotune(orad_ctx *ctx, int cycles, orad_launch_t *best, double *best_ms);
 
+/* Restart files of the dynamic state (replace the element-loop part of the engine's
+ * restart, output/restart/wrrestp.F:1142-1150 for X, D, V, VR and :1273 for the
+ * element buffers, read back by output/restart/rdresb.F:1055-1059): orad_write_restart
+ * writes the nodal X, V, D (and VR for shells), every element state column and the
+ * per-element nodal stiffness, the time record of each model (TT, DT1, DT2, NCYCLE),
+ * the energy balance's running sums and ENTOT0, in the layout the library holds them
+ * in (one file, written to `path`.tmp and renamed).  orad_read_restart loads such a
+ * file into a context built from the same model (same family, precision, Ishell,
+ * sizes, models and PARITH/ON slot table: a fingerprint checks them) and the run
+ * continues bit for bit as if it had not stopped.  Mesh, masses, materials,
+ * properties, imposed velocities and launch settings are the model's and are not in
+ * the file.  ORAD_EINVAL names the mismatch, a truncated or corrupt file (checksum)
+ * or an I/O failure; nothing is changed on the device unless the whole file passed. */
+int orad_write_restart(orad_ctx *ctx, const char *path);
+int orad_read_restart(orad_ctx *ctx, const char *path);
+
 /* Adjoint of a run of one shell or brick model (no reference counterpart: the
  * reference's only derivative is the implicit consistent tangent of law 2,
  * m2law.F:487-531, m2cplr.F:448-469).  orad_adjoint_forward runs ncycle cycles from

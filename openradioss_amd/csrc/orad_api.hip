@@ -1655,6 +1655,159 @@ static std::vector<std::pair<void *, size_t>> cycle_buffers(orad_ctx *c) {
   return live;
 }
 
+// ------------------------------------------------------------ restart files --
+// The dynamic state of a context (the cycle buffers above, the cycle counter and the
+// energy balance's ENTOT0) in a file, so a run can stop and resume bit for bit in a
+// context built from the same model.  Layout (little-endian): 8-byte magic
+// "ORADRST1"; uint32 version; uint32 number of sections; a header of int64 values
+// (family, precision, Ishell, numnod, elements, models, state columns, value size,
+// host cycle count, ENTOT0 known) and ENTOT0 as a double; the model fingerprint
+// (uint64); then per section its byte count (uint64) and bytes, in cycle_buffers
+// order; last a 64-bit checksum of every section's bytes.
+
+static const char RESTART_MAGIC[8] = {'O', 'R', 'A', 'D', 'R', 'S', 'T', '1'};
+static constexpr uint32_t RESTART_VERSION = 1;
+
+// 64-bit FNV-1a over bytes.
+static uint64_t fnv1a(uint64_t h, const void *p, size_t n) {
+  const unsigned char *b = (const unsigned char *)p;
+  for (size_t i = 0; i < n; ++i) {
+    h ^= b[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+// Fingerprint of what the state's layout depends on: the sizes, the PARITH/ON slot
+// table (ADSKY, IADS/IADC) and, for bricks, the library's element order.
+static uint64_t model_fingerprint(const orad_ctx *c) {
+  uint64_t h = 1469598103934665603ull;
+  const int64_t v[8] = {c->family, c->precision, c->ishell, c->numnod, nelem(c), c->nmodel,
+                        c->family == FAM_SHELL ? c->shncol() : NSTATE, (int64_t)c->tsize()};
+  h = fnv1a(h, v, sizeof v);
+  h = fnv1a(h, c->h_adsky.data(), c->h_adsky.size() * sizeof(int32_t));
+  h = fnv1a(h, c->h_iads.data(), c->h_iads.size() * sizeof(int32_t));
+  h = fnv1a(h, c->h_perm.data(), c->h_perm.size() * sizeof(int32_t));
+  return h;
+}
+
+static void restart_header(const orad_ctx *c, int64_t hdr[10]) {
+  hdr[0] = c->family;
+  hdr[1] = c->precision;
+  hdr[2] = c->ishell;
+  hdr[3] = c->numnod;
+  hdr[4] = nelem(c);
+  hdr[5] = c->nmodel;
+  hdr[6] = c->family == FAM_SHELL ? c->shncol() : NSTATE;
+  hdr[7] = (int64_t)c->tsize();
+  hdr[8] = c->host_ncycle;
+  hdr[9] = c->have_entot0 ? 1 : 0;
+}
+
+extern "C" {
+
+int orad_write_restart(orad_ctx *c, const char *path) {
+  if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
+  if (!path) return fail(c, ORAD_EINVAL, "write_restart: null path");
+  int r = ready(c);
+  if (r) return r;
+  if (c->family == FAM_BRICK ? (r = init_state(c)) : (r = init_shell_state(c))) return r;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const auto live = cycle_buffers(c);
+  std::vector<std::vector<char>> host(live.size());
+  uint64_t sum = 1469598103934665603ull;
+  for (size_t i = 0; i < live.size(); ++i) {
+    host[i].resize(live[i].second);
+    HIPCHK(c, hipMemcpy(host[i].data(), live[i].first, live[i].second, hipMemcpyDeviceToHost));
+    sum = fnv1a(sum, host[i].data(), host[i].size());
+  }
+  int64_t hdr[10];
+  restart_header(c, hdr);
+  const double entot0 = c->entot0;
+  const uint64_t fp = model_fingerprint(c);
+  const uint32_t ver = RESTART_VERSION, nsec = (uint32_t)live.size();
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE *fpo = std::fopen(tmp.c_str(), "wb");
+  if (!fpo) return fail(c, ORAD_EINVAL, std::string("write_restart: cannot open ") + tmp);
+  bool ok = std::fwrite(RESTART_MAGIC, 1, 8, fpo) == 8 && std::fwrite(&ver, 4, 1, fpo) == 1 &&
+            std::fwrite(&nsec, 4, 1, fpo) == 1 && std::fwrite(hdr, 8, 10, fpo) == 10 &&
+            std::fwrite(&entot0, 8, 1, fpo) == 1 && std::fwrite(&fp, 8, 1, fpo) == 1;
+  for (size_t i = 0; ok && i < host.size(); ++i) {
+    const uint64_t n = host[i].size();
+    ok = std::fwrite(&n, 8, 1, fpo) == 1 && std::fwrite(host[i].data(), 1, n, fpo) == n;
+  }
+  ok = ok && std::fwrite(&sum, 8, 1, fpo) == 1;
+  ok = (std::fclose(fpo) == 0) && ok;
+  if (!ok || std::rename(tmp.c_str(), path) != 0) {
+    std::remove(tmp.c_str());
+    return fail(c, ORAD_EINVAL, std::string("write_restart: cannot write ") + path);
+  }
+  return ORAD_OK;
+}
+
+int orad_read_restart(orad_ctx *c, const char *path) {
+  if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
+  if (!path) return fail(c, ORAD_EINVAL, "read_restart: null path");
+  int r = ready(c);
+  if (r) return r;
+  if (c->family == FAM_BRICK ? (r = init_state(c)) : (r = init_shell_state(c))) return r;
+  FILE *fpi = std::fopen(path, "rb");
+  if (!fpi) return fail(c, ORAD_EINVAL, std::string("read_restart: cannot open ") + path);
+  auto bad = [&](const std::string &why) {
+    std::fclose(fpi);
+    return fail(c, ORAD_EINVAL, "read_restart: " + why);
+  };
+  char magic[8];
+  uint32_t ver = 0, nsec = 0;
+  int64_t hdr[10], want[10];
+  double entot0 = 0.0;
+  uint64_t fp = 0;
+  if (std::fread(magic, 1, 8, fpi) != 8 || std::memcmp(magic, RESTART_MAGIC, 8) != 0)
+    return bad("not a restart file of this library");
+  if (std::fread(&ver, 4, 1, fpi) != 1 || ver != RESTART_VERSION)
+    return bad("unsupported restart version");
+  if (std::fread(&nsec, 4, 1, fpi) != 1 || std::fread(hdr, 8, 10, fpi) != 10 ||
+      std::fread(&entot0, 8, 1, fpi) != 1 || std::fread(&fp, 8, 1, fpi) != 1)
+    return bad("truncated header");
+  restart_header(c, want);
+  for (int i = 0; i < 8; ++i)
+    if (hdr[i] != want[i])
+      return bad("the file was written by a different model (family, precision, Ishell, "
+                 "sizes or state layout)");
+  if (fp != model_fingerprint(c))
+    return bad("the file was written by a model with another mesh or element order");
+  const auto live = cycle_buffers(c);
+  if (nsec != live.size()) return bad("section count does not match this model");
+  std::vector<std::vector<char>> host(live.size());
+  uint64_t sum = 1469598103934665603ull;
+  for (size_t i = 0; i < live.size(); ++i) {
+    uint64_t n = 0;
+    if (std::fread(&n, 8, 1, fpi) != 1 || n != live[i].second)
+      return bad("section size does not match this model");
+    host[i].resize(n);
+    if (std::fread(host[i].data(), 1, n, fpi) != n) return bad("truncated section");
+    sum = fnv1a(sum, host[i].data(), n);
+  }
+  uint64_t want_sum = 0;
+  if (std::fread(&want_sum, 8, 1, fpi) != 1 || want_sum != sum)
+    return bad("checksum mismatch (corrupt or truncated file)");
+  std::fclose(fpi);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < live.size(); ++i)
+    HIPCHK(c, hipMemcpy(live[i].first, host[i].data(), host[i].size(), hipMemcpyHostToDevice));
+  c->host_ncycle = (int)hdr[8];
+  c->have_entot0 = hdr[9] != 0;
+  c->entot0 = entot0;
+  drop_graphs(c);
+  return ORAD_OK;
+}
+
+}  // extern "C"
+
 struct Snapshot {
   std::vector<void *> copies;   // in cycle_buffers order
   std::vector<size_t> sizes;

@@ -29,6 +29,7 @@ EXPORTS = (
     "orad_get_shell_state", "orad_set_shell_state", "orad_set_shell_models",
     "orad_get_model_time", "orad_model_energies", "orad_set_launch", "orad_get_launch",
     "orad_autotune", "orad_adjoint_forward", "orad_adjoint_reverse",
+    "orad_write_restart", "orad_read_restart",
 )
 NIXC = 7
 SH_NCOL = 53          # Belytschko-Tsay state values per shell
@@ -125,6 +126,8 @@ def load_library(path=LIB_PATH):
         "orad_set_launch": ([P, C.POINTER(Launch)], I),
         "orad_get_launch": ([P, C.POINTER(Launch)], I),
         "orad_autotune": ([P, I, C.POINTER(Launch), C.POINTER(D)], I),
+        "orad_write_restart": ([P, C.c_char_p], I),
+        "orad_read_restart": ([P, C.c_char_p], I),
         "orad_adjoint_forward": ([P, I, I], I),
         "orad_adjoint_reverse": ([P] + [P] * 12, I),
     }
@@ -236,6 +239,15 @@ class BrickSolver:
         l, ms = Launch(), C.c_double()
         self._chk(self.lib.orad_autotune(self.h, cycles, C.byref(l), C.byref(ms)))
         return l.as_dict(), ms.value
+
+    def write_restart(self, path):
+        """orad_write_restart: nodes, element state, per-element stiffness, time and
+        energy-balance sums of this context into the file `path`."""
+        self._chk(self.lib.orad_write_restart(self.h, os.fsencode(path)))
+
+    def read_restart(self, path):
+        """orad_read_restart: resume from a file written by a context of the same model."""
+        self._chk(self.lib.orad_read_restart(self.h, os.fsencode(path)))
 
     def pon(self):
         """(ADSKY(NUMNOD+1), IADS(8,NUMELS)) 1-based, as the starter writes them."""
@@ -473,6 +485,8 @@ class ShellSolver:
     update_xv = BrickSolver.update_xv
     launch = BrickSolver.launch
     autotune = BrickSolver.autotune
+    write_restart = BrickSolver.write_restart
+    read_restart = BrickSolver.read_restart
     close = BrickSolver.close
     __del__ = BrickSolver.__del__
     run_cycles = BrickSolver.run_cycles

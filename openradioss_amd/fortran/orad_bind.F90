@@ -251,6 +251,16 @@ module orad_bind
       type(orad_launch_t) :: best
       real(c_double) :: best_ms
     end function
+    integer(c_int) function orad_write_restart(ctx, path) bind(c, name='orad_write_restart')
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*) :: path
+    end function
+    integer(c_int) function orad_read_restart(ctx, path) bind(c, name='orad_read_restart')
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*) :: path
+    end function
     integer(c_int) function orad_adjoint_forward(ctx, ncycle, every) &
         bind(c, name='orad_adjoint_forward')
       import :: c_ptr, c_int
