@@ -1,7 +1,7 @@
 """Recipe for oracle/_ref: the reference's own brick and shell leaf kernels,
 compiled from their sources where they lie under the reference tree, plus the
 test harnesses oracle/ref/harness_brick.F, harness_shell.F, harness_qeph.F,
-harness_qbat.F and harness_nodal.F, linked into
+harness_qbat.F, harness_nodal.F and harness_int7.F, linked into
 oracle/_ref/libref.so.
 
 Test infrastructure only.  Runs in the build container (amdflang from ROCm);
@@ -83,6 +83,10 @@ LEAF = [
     "engine/source/tools/curve/vinterdp.F",
     "engine/source/assembly/velocity.F",
     "engine/source/assembly/displacement.F",
+    "engine/source/interfaces/int07/i7cor3.F",
+    "engine/source/interfaces/int07/i7dst3.F",
+    "engine/source/interfaces/int07/i7for3.F",
+    "engine/source/interfaces/int07/i7ass3.F",
 ]
 
 
@@ -158,6 +162,7 @@ def main():
     comp(os.path.join(HERE, "ref", "harness_qeph.F"))
     comp(os.path.join(HERE, "ref", "harness_qbat.F"))
     comp(os.path.join(HERE, "ref", "harness_nodal.F"))
+    comp(os.path.join(HERE, "ref", "harness_int7.F"))
     for rel in CXX:
         obj = os.path.join(OUT, re.sub(r"[^A-Za-z0-9_]", "_", rel) + ".o")
         subprocess.run(["g++", "-c", "-fPIC", "-O2", "-std=c++17", os.path.join(REF, rel),
@@ -185,6 +190,8 @@ def main():
     weak = []
     for s in ERROR_PATH:
         weak += ["--weaken-symbol", s + "_"]
+    for s in C_ERROR_PATH:
+        weak += ["--weaken-symbol", s]
     for o in objs:
         subprocess.run(["objcopy"] + weak + [o], check=True)
     subprocess.run(["amdflang", "-shared", "-o", lib] + objs + LIBS, check=True)
@@ -204,7 +211,18 @@ ERROR_PATH = ("ancmsg", "arret",
               # coefficient routine of the DKT/QBAT shells, which shares its source
               # file with CNCOEF3B; their own file (coqueba/cmatc3.F) pulls in the
               # implicit solver's initialisation and from there the whole engine.
-              "gepm_lc", "cctoglob")
+              "gepm_lc", "cctoglob",
+              # Restart-file readers and writers in C (common_source/tools/
+              # input_output/write_routines.c, which needs zlib) referenced by the
+              # contact modules' restart routines; no harness reads or writes a
+              # restart file.
+              "read_i_c", "write_i_c", "read_i_array_c", "write_i_array_c")
+
+
+# C functions bound by OUTPUT_MOD's checksum module (common_source/output/checksum/
+# checksum.cpp, which needs an md5 header absent from the image); only the output
+# layer calls them, never the contact routines on the harness's path.
+C_ERROR_PATH = ("new_file_checksum_list", "print_checksum_list")
 
 
 def subroutine_index():

@@ -546,7 +546,13 @@ void oracle_cycle(orad_omodel_t *m) {
   /* Hourglass energy (SHVIS3 -> PARTSAV(8,.)), summed in element order. */
   for (int e = 0; e < ne; ++e) m->ehour += ehou[e];
   free(ehou);
-  /* ASSPAR4 (asspar4.F:157-175) ordered per-node gather */
+  /* /INTER/TYPE7 (i7mainf.F), its skyline entries FSKYI sorted by node */
+  double *isk = NULL;
+  int32_t *iord = NULL, *ifirst = NULL;
+  if (m->contact)
+    oracle_int7_cycle(m->contact, nn, m->x, m->v, m->ms, dt1, &dt2t, &isk, &iord, &ifirst);
+  /* ASSPAR4 (asspar4.F:157-175) ordered per-node gather, then the node's interface
+   * entries in skyline order (asspar4.F:560-583) */
 #pragma omp parallel for num_threads(nth) schedule(static)
   for (int n = 0; n < nn; ++n) {
     double ax = 0.0, ay = 0.0, az = 0.0, sf = 0.0;
@@ -554,8 +560,14 @@ void oracle_cycle(orad_omodel_t *m) {
       const double *f = m->fsky + 4*(size_t)k;
       ax = ax + f[0]; ay = ay + f[1]; az = az + f[2]; sf = sf + f[3];
     }
+    if (iord)
+      for (int k = ifirst[n]; k < ifirst[n + 1]; ++k) {
+        const double *f = isk + 4*(size_t)iord[k];
+        ax = ax + f[0]; ay = ay + f[1]; az = az + f[2]; sf = sf + f[3];
+      }
     m->a[3*n] = ax; m->a[3*n + 1] = ay; m->a[3*n + 2] = az; m->stifn[n] = sf;
   }
+  free(isk); free(iord); free(ifirst);
   orad_onodes_t nd = {0};
   nd.numnod = nn; nd.rot = 0;
   nd.x = m->x; nd.v = m->v; nd.d = m->d; nd.a = m->a; nd.ms = m->ms; nd.icodt = m->icodt;

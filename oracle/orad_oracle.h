@@ -110,6 +110,33 @@ typedef struct {
 
 void oracle_nodal_update(orad_onodes_t *m, double dt2t);
 
+/* ---- /INTER/TYPE7 node-to-segment penalty contact (orad_oracle_int7.c) ---- */
+typedef struct {
+  int nsn, nrtm;
+  const int32_t *nsv;             /* secondary nodes, 0-based                       */
+  const int32_t *irect;           /* 4*nrtm segment nodes, 0-based; 3rd = 4th: triangle */
+  const double *stfm;             /* segment stiffness STF                          */
+  double *stfns;                  /* secondary node factor STFN (negated when a node
+                                     is deactivated at full penetration)            */
+  double gap, fric, visc, viscf;  /* GAP, FRIC, VISs (critical damping), VISf       */
+  double econt, econtv, econtd;   /* cumulative elastic, friction, damping energies */
+  int npen, nsky;                 /* last call: penetrating pairs, skyline entries  */
+} orad_int7_t;
+
+/* Candidates by I7TRI's box test (widened by tz), listed by (secondary, segment);
+ * returns the count (entries beyond cap are counted, not written). */
+int oracle_int7_candidates(const orad_int7_t *it, const double *x, double tz, int cap,
+                           int32_t *cand_n, int32_t *cand_e);
+/* I7COR3 + I7DST3 + I7FOR3 + I7ASS2 over a candidate list: skyline entries (fx, fy, fz,
+ * stif per entry, node isky) in FSKYI order; lowers *dt2t; returns the entry count. */
+int oracle_int7_forces(orad_int7_t *it, const double *x, const double *v, const double *ms,
+                       int ncand, const int32_t *cand_n, const int32_t *cand_e, double dt1,
+                       double *dt2t, double *fsky, int32_t *isky);
+/* One cycle's contact for the whole-model cycles: entries and their per-node order. */
+int oracle_int7_cycle(orad_int7_t *it, int numnod, const double *x, const double *v,
+                      const double *ms, double dt1, double *dt2t, double **fsky,
+                      int32_t **order, int32_t **first);
+
 /* ---- Whole-model explicit cycle (RESOL order, resol.F:2586-9595) ---- */
 typedef struct {
   int numnod, numels;
@@ -133,6 +160,7 @@ typedef struct {
   double *fv_mem;                 /* FIXVEL energy memory per node, or null        */
   double wfext;                   /* external work of the imposed velocities       */
   double encin, enrot;            /* ECRIT's kinetic energies of the last cycle    */
+  orad_int7_t *contact;           /* /INTER/TYPE7, or null                          */
 } orad_omodel_t;
 
 /* One full cycle: element forces -> assembly -> dt -> accel -> BC -> vel -> disp. */
@@ -266,6 +294,7 @@ typedef struct {
   double *fv_mem;                 /* FIXVEL energy memory per node, or null      */
   double wfext;                   /* external work of the imposed velocities     */
   double encin, enrot;            /* ECRIT's kinetic energies of the last cycle  */
+  orad_int7_t *contact;           /* /INTER/TYPE7, or null                        */
 } orad_oshell_t;
 
 void oracle_shell_cycle(orad_oshell_t *m);

@@ -1485,7 +1485,13 @@ void oracle_shell_cycle(orad_oshell_t *m) {
   }
   for (int e = 0; e < ne; ++e) m->ehour += ehou[e];
   free(ehou);
-  /* ASSPAR4 (engine/source/assembly/asspar4.F) ordered per-node gather, 6 DOF */
+  /* /INTER/TYPE7 (i7mainf.F), its skyline entries FSKYI sorted by node */
+  double *isk = NULL;
+  int32_t *iord = NULL, *ifirst = NULL;
+  if (m->contact)
+    oracle_int7_cycle(m->contact, nn, m->x, m->v, m->ms, dt1, &dt2t, &isk, &iord, &ifirst);
+  /* ASSPAR4 (engine/source/assembly/asspar4.F) ordered per-node gather, 6 DOF, then the
+   * node's interface entries in skyline order (asspar4.F:560-583) */
 #pragma omp parallel for num_threads(nth) schedule(static)
   for (int n = 0; n < nn; ++n) {
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1493,10 +1499,19 @@ void oracle_shell_cycle(orad_oshell_t *m) {
       const double *f = m->fsky + 8 * (size_t)k;
       for (int c = 0; c < 8; ++c) s[c] = s[c] + f[c];
     }
+    if (iord)
+      for (int k = ifirst[n]; k < ifirst[n + 1]; ++k) {
+        const double *f = isk + 4 * (size_t)iord[k];
+        for (int c = 0; c < 3; ++c) s[c] = s[c] + f[c];
+        s[6] = s[6] + f[3];
+      }
     for (int d = 0; d < 3; ++d) { m->a[3 * n + d] = s[d]; m->ar[3 * n + d] = s[3 + d]; }
     m->stifn[n] = s[6];
     m->stifr[n] = s[7];
   }
+  free(isk);
+  free(iord);
+  free(ifirst);
   /* time step, ACCELE, BCS10, FIXVEL, VELOCITY, DEPLA with rotations (RESOL order) */
   orad_onodes_t nd = {0};
   nd.numnod = nn; nd.rot = 1;

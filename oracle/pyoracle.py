@@ -36,6 +36,15 @@ class BOut(C.Structure):
                 ("pad", C.c_int)]
 
 
+class Int7(C.Structure):
+    """orad_int7_t: one /INTER/TYPE7 interface."""
+    _fields_ = [("nsn", C.c_int), ("nrtm", C.c_int), ("nsv", C.c_void_p),
+                ("irect", C.c_void_p), ("stfm", C.c_void_p), ("stfns", C.c_void_p)] + \
+               [(n, C.c_double) for n in ("gap", "fric", "visc", "viscf", "econt", "econtv",
+                                          "econtd")] + \
+               [("npen", C.c_int), ("nsky", C.c_int)]
+
+
 class OModel(C.Structure):
     _fields_ = [("numnod", C.c_int), ("numels", C.c_int),
                 ("x", C.c_void_p), ("v", C.c_void_p), ("d", C.c_void_p),
@@ -49,7 +58,7 @@ class OModel(C.Structure):
                 ("nthreads", C.c_int), ("pad", C.c_int),
                 ("fv_dir", C.c_void_p), ("fv_val", C.c_void_p),
                 ("fv_mem", C.c_void_p), ("wfext", C.c_double),
-                ("encin", C.c_double), ("enrot", C.c_double)]
+                ("encin", C.c_double), ("enrot", C.c_double), ("contact", C.c_void_p)]
 
 
 class ONodes(C.Structure):
@@ -116,7 +125,7 @@ class OShell(C.Structure):
                                           "ehour")] + \
                [("ncycle", C.c_int), ("nthreads", C.c_int),
                 ("fv_mem", C.c_void_p), ("wfext", C.c_double),
-                ("encin", C.c_double), ("enrot", C.c_double)]
+                ("encin", C.c_double), ("enrot", C.c_double), ("contact", C.c_void_p)]
 
 
 def _ptr(a):
@@ -160,6 +169,12 @@ def lib():
         _lib.oracle_shell_energies.argtypes = [C.POINTER(OShell), C.POINTER(C.c_double),
                                                C.POINTER(C.c_double)]
         _lib.oracle_nodal_update.argtypes = [C.POINTER(ONodes), C.c_double]
+        _lib.oracle_int7_candidates.argtypes = [C.POINTER(Int7), C.c_void_p, C.c_double,
+                                                C.c_int, C.c_void_p, C.c_void_p]
+        _lib.oracle_int7_forces.argtypes = [C.POINTER(Int7), C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                            C.c_double, C.POINTER(C.c_double), C.c_void_p,
+                                            C.c_void_p]
     return _lib
 
 
@@ -185,6 +200,9 @@ def ref():
         _ref.ref_shell_tables.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _ref.ref_nodal_cycle.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 10 + \
             [C.c_int] + [C.c_void_p] * 4 + [C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+        _ref.ref_int7_batch.argtypes = [C.c_int] + [C.c_void_p] * 3 + [C.c_int, C.c_void_p,
+                                                                     C.c_int] + \
+            [C.c_void_p] * 3 + [C.c_int] + [C.c_void_p] * 8
     return _ref
 
 
@@ -305,6 +323,10 @@ class Model:
         m.dtmx = 1.0e20
         m.nthreads = nthreads
         self.m = m
+        self.contact = None
+        if deck.get("contact") is not None:
+            self.contact = Contact7(deck["contact"])
+            m.contact = C.cast(C.pointer(self.contact.it), C.c_void_p)
 
     def cycle(self, n=1):
         for _ in range(n):
@@ -659,6 +681,10 @@ class ShellModel:
         m.dtmx = 1.0e20
         m.nthreads = nthreads
         self.m = m
+        self.contact = None
+        if deck.get("contact") is not None:
+            self.contact = Contact7(deck["contact"])
+            m.contact = C.cast(C.pointer(self.contact.it), C.c_void_p)
 
     def cycle(self, n=1):
         for _ in range(n):
@@ -755,3 +781,89 @@ def ref_nodal_cycle(nodes, time, dt2t, rot=False):
     t = dict(zip(TIME_KEYS, times.tolist()))
     t["wfext"], t["ncycle"] = wf.value, ncyc.value
     return o, t
+
+
+def ref_int7(x, v, ms, nsv, irect, stfm, stfns, cand_n, cand_e, gap, fric, visc, viscf, dt1,
+             dt12, dt2t=1.0e30):
+    """The reference's own /INTER/TYPE7 routines (I7COR3, I7DST3, I7FOR3, I7ASS2 in
+    I7MAINF order, oracle/_ref, harness_int7.F) over a candidate list.  nsv, irect,
+    cand_n and cand_e are 0-based here (secondary node numbers, segment node numbers,
+    index into nsv, index into irect).  Returns the skyline entries in FSKYI order
+    (fsky[k] = FX, FY, FZ, STIF of node isky[k], 0-based), the lowered DT2T, the
+    elastic, friction and damping contact energies of the call, the number of
+    penetrating pairs and, per candidate, whether I7DST3 kept it."""
+    x = np.ascontiguousarray(x, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    ms = np.ascontiguousarray(ms, np.float64)
+    nsv1 = np.ascontiguousarray(np.asarray(nsv) + 1, np.int32)
+    irect1 = np.ascontiguousarray(np.asarray(irect) + 1, np.int32)
+    stfm = np.ascontiguousarray(stfm, np.float64)
+    stfns = np.array(stfns, np.float64)          # a copy: deactivation negates entries
+    cn = np.ascontiguousarray(np.asarray(cand_n) + 1, np.int32)
+    ce = np.ascontiguousarray(np.asarray(cand_e) + 1, np.int32)
+    nc = len(cn)
+    params = np.array([gap, fric, visc, viscf], np.float64)
+    dts = np.array([dt1, dt12], np.float64)
+    fsky = np.zeros((max(5 * nc, 1), 4))
+    isky = np.zeros(max(5 * nc, 1), np.int32)
+    out = np.array([dt2t, 0.0, 0.0, 0.0, 0.0])
+    pen = np.zeros(max(nc, 1), np.int32)
+    nsky = C.c_int(0)
+    ref().ref_int7_batch(x.shape[0], _ptr(x), _ptr(v), _ptr(ms), len(nsv1), _ptr(nsv1),
+                         irect1.shape[0], _ptr(irect1), _ptr(stfm), _ptr(stfns), nc,
+                         _ptr(cn if nc else np.zeros(1, np.int32)),
+                         _ptr(ce if nc else np.zeros(1, np.int32)), _ptr(params), _ptr(dts),
+                         C.byref(nsky), _ptr(fsky), _ptr(isky), _ptr(out), _ptr(pen))
+    n = nsky.value
+    return dict(fsky=fsky[:n].copy(), isky=isky[:n] - 1, dt2t=out[0], econt=out[1],
+                econtv=out[2], econtd=out[3], npen=int(out[4]), pen=pen[:nc].astype(bool),
+                stfns=stfns)
+
+
+class Contact7:
+    """One /INTER/TYPE7 interface for the oracle (orad_int7_t over arrays it keeps
+    alive).  `c` is a dict: nsv (secondary nodes), irect (nrtm, 4) segment nodes,
+    0-based; stfm (per segment), stfns (per secondary node); gap, fric, visc, viscf."""
+
+    def __init__(self, c):
+        self.nsv = np.ascontiguousarray(c["nsv"], np.int32)
+        self.irect = np.ascontiguousarray(c["irect"], np.int32).reshape(-1, 4)
+        self.stfm = np.ascontiguousarray(c["stfm"], np.float64)
+        self.stfns = np.ascontiguousarray(c["stfns"], np.float64).copy()
+        it = Int7()
+        it.nsn, it.nrtm = len(self.nsv), self.irect.shape[0]
+        it.nsv, it.irect = _ptr(self.nsv), _ptr(self.irect)
+        it.stfm, it.stfns = _ptr(self.stfm), _ptr(self.stfns)
+        it.gap, it.fric = float(c["gap"]), float(c.get("fric", 0.0))
+        it.visc, it.viscf = float(c.get("visc", 0.05)), float(c.get("viscf", 0.0))
+        self.it = it
+
+    def candidates(self, x, tz=None):
+        x = np.ascontiguousarray(x, np.float64)
+        tz = self.it.gap * 1.000001 if tz is None else tz
+        n = lib().oracle_int7_candidates(C.byref(self.it), _ptr(x), tz, 0, None, None)
+        cn = np.zeros(max(n, 1), np.int32)
+        ce = np.zeros(max(n, 1), np.int32)
+        lib().oracle_int7_candidates(C.byref(self.it), _ptr(x), tz, n, _ptr(cn), _ptr(ce))
+        return cn[:n], ce[:n]
+
+    def forces(self, x, v, ms, cand_n, cand_e, dt1, dt2t=1.0e30):
+        """Skyline entries of one call (see ref_int7): fsky (n, 4), isky, dt2t."""
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        ms = np.ascontiguousarray(ms, np.float64)
+        cn = np.ascontiguousarray(cand_n, np.int32)
+        ce = np.ascontiguousarray(cand_e, np.int32)
+        nc = len(cn)
+        fsky = np.zeros((max(5 * nc, 1), 4))
+        isky = np.zeros(max(5 * nc, 1), np.int32)
+        d = C.c_double(dt2t)
+        e0 = (self.it.econt, self.it.econtv, self.it.econtd)
+        n = lib().oracle_int7_forces(C.byref(self.it), _ptr(x), _ptr(v), _ptr(ms), nc,
+                                     _ptr(cn if nc else np.zeros(1, np.int32)),
+                                     _ptr(ce if nc else np.zeros(1, np.int32)), dt1,
+                                     C.byref(d), _ptr(fsky), _ptr(isky))
+        return dict(fsky=fsky[:n].copy(), isky=isky[:n].copy(), dt2t=d.value,
+                    econt=self.it.econt - e0[0], econtv=self.it.econtv - e0[1],
+                    econtd=self.it.econtd - e0[2], npen=self.it.npen,
+                    stfns=self.stfns.copy())
