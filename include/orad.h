@@ -291,6 +291,42 @@ int orad_get_launch(orad_ctx *ctx, orad_launch_t *l);
  * slot tile. */
 int orad_autotune(orad_ctx *ctx, int cycles, orad_launch_t *best, double *best_ms);
 
+/* /INTER/TYPE7 node-to-segment penalty contact, one interface per context (in place of
+ * the interface part of I7MAINF, interfaces/int07/i7mainf.F:529-737, and ASSPAR4's
+ * interface skyline, assembly/asspar4.F:126-154, 560-583).  The path: constant gap
+ * (IGAP=0), IGSTI<=1 (STIF = STFM(segment) * |STFNS(node)|), INACTI=0, no curvature,
+ * no heat exchange, no tied contact, IVIS2=0 critical damping, Coulomb friction in the
+ * total viscous formulation without filtering (IFQ=0, MFROT=0, isotropic).
+ *
+ * orad_set_contact7: NSV(NSN) secondary nodes and IRECT(4,NRTM) main segments, 1-based
+ * (IRECT(3)=IRECT(4): a triangle), their stiffness STFM(NRTM) and factor STFNS(NSN)
+ * (as the starter writes INTBUF_TAB%STFM/STFNS), params = GAP, FRIC, VISs (critical
+ * damping, the reference default 0.05), VISf (friction damping); cand_cap bounds the
+ * candidate pairs of one pass (<= 0: 16 per secondary node).  Every cycle of
+ * orad_run_cycles then searches the candidates on the device (I7TRI's box test widened
+ * by GAP), computes I7COR3/I7DST3/I7FOR3/I7ASS2 for the penetrating pairs in batches
+ * of 128, lowers DT2T with the contact time step and adds each node's skyline entries
+ * after its element slots, in skyline order.  The entries are the reference routines'
+ * bit for bit on the same candidate batches; the batches are the penetrating pairs in
+ * (secondary node, segment) order, where the reference batches the candidates its
+ * bucket sort keeps.  A pass that outgrows cand_cap, or a secondary node with more
+ * than 64 candidate segments, fails the run with ORAD_EINVAL.  Not with batched models.
+ *
+ * orad_contact7: one pass at the current X, V with the given DT1 (the group path:
+ * call it where I7MAINF runs; the next orad_asspar4 adds the entries); lowers *dt2t.
+ * orad_get_contact7: cumulative energies (ECONT elastic, ECONTV friction, ECONTD
+ * damping = 0 on this path), counts[4] = candidates, penetrating pairs and skyline
+ * entries of the last pass and the overflow flags; stfns (NSN) with the nodes that
+ * reached full penetration negated, as I7FOR3 marks them (i7for3.F:945-961).
+ * orad_get_contact7_sky: the last pass's skyline entries FSKYI(1:4) and their nodes
+ * ISKY (1-based) in skyline order; *nsky their count (null outputs: count only). */
+int orad_set_contact7(orad_ctx *ctx, int nsn, const int *nsv, int nrtm, const int *irect,
+                      const double *stfm, const double *stfns, const double *params,
+                      int cand_cap);
+int orad_contact7(orad_ctx *ctx, double dt1, double *dt2t);
+int orad_get_contact7(orad_ctx *ctx, double *energies, int *counts, double *stfns);
+int orad_get_contact7_sky(orad_ctx *ctx, int cap, double *fsky, int *isky, int *nsky);
+
 /* Restart files of the dynamic state (replace the element-loop part of the engine's
  * restart, output/restart/wrrestp.F:1142-1150 for X, D, V, VR and :1273 for the
  * element buffers, read back by output/restart/rdresb.F:1055-1059): orad_write_restart

@@ -29,6 +29,7 @@ SOURCES = [
     ("orad_qbat_fp64.hip", ["-ffp-contract=off"]),
     ("orad_qbat_mixed.hip", ["-ffp-contract=fast"]),
     ("orad_nodal.hip", ["-ffp-contract=off"]),
+    ("orad_contact7.hip", ["-ffp-contract=off"]),
     ("orad_adjoint.hip", ["-ffp-contract=off"]),
     ("orad_brick_order.hip", []),
     ("orad_api.hip", []),

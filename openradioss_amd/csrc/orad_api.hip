@@ -116,6 +116,21 @@ static NodeArgs node_args(orad_ctx *c) {
     a.model_chunks = c->model_tiles;
     a.chunk_ehou = c->chunk_ehou;
   }
+  if (c->ci) {
+    a.ci_cnt = c->ci->node_cnt;
+    a.ci_start = c->ci->node_bstart;
+    a.ci_bucket = c->ci->node_bucket;
+    a.ci_sky = c->ci->sky;
+  }
+  return a;
+}
+
+// The assembly-only launch (orad_asspar4): the interface entries of the last
+// orad_contact7 call are added once, then dropped.
+static NodeArgs assembly_args(orad_ctx *c) {
+  NodeArgs a = node_args(c);
+  if (!c->ci_fresh) a.ci_cnt = nullptr;
+  c->ci_fresh = false;
   return a;
 }
 
@@ -211,8 +226,17 @@ static hipError_t launch_cycle_elements(orad_ctx *c, int p) {
   return launch_elements(c, 0, nelem(c), &c->ctl->s[p].dt2, &c->ctl->dt2t[p], c->ehou_part);
 }
 
+// The contact pass of a cycle (I7MAINF's place, before ASSPAR4): DT1 of the cycle is
+// the time record's DT2, the contact time step lowers the cycle's DT2T.
+static hipError_t launch_cycle_contact(orad_ctx *c, int p) {
+  if (!c->ci) return hipSuccess;
+  return contact7_run(*c->ci, c->X4, c->V4, &c->ctl->s[p].dt2, &c->ctl->dt2t[p], c->stream);
+}
+
 static hipError_t enqueue_cycle(orad_ctx *c, int p) {
   hipError_t e = launch_cycle_elements(c, p);
+  if (e != hipSuccess) return e;
+  e = launch_cycle_contact(c, p);
   if (e != hipSuccess) return e;
   return launch_node(node_args(c), c->precision == ORAD_MIXED, p, 1, c->cycle_write_a,
                      c->xcd_remap, c->stream, c->node_tile);
@@ -240,6 +264,24 @@ static int init_state(orad_ctx *c) {
 
 static int get_fsky_shell(orad_ctx *c, double *fsky);
 static void free_adjoint(orad_ctx *c);
+
+static void free_contact(orad_ctx *c) {
+  Contact7 *k = c->ci;
+  if (!k) return;
+  for (void *p : {(void *)k->nsv, (void *)k->irect, (void *)k->stfm, (void *)k->stfns,
+                  (void *)k->bounds, (void *)k->grid, (void *)k->gdims, (void *)k->seg_cell,
+                  (void *)k->cell_count, (void *)k->cell_fill, (void *)k->cell_start,
+                  (void *)k->cell_seg, (void *)k->node_ncand, (void *)k->node_start,
+                  (void *)k->cand_n, (void *)k->cand_e, (void *)k->keep, (void *)k->rank,
+                  (void *)k->act, (void *)k->counters, (void *)k->sky, (void *)k->sky_node,
+                  (void *)k->batch_cnt, (void *)k->batch_e, (void *)k->energy,
+                  (void *)k->node_cnt, (void *)k->node_fill, (void *)k->node_bstart,
+                  (void *)k->node_bucket, (void *)k->scan_tmp})
+    dfree(p);
+  delete k;
+  c->ci = nullptr;
+  c->ci_fresh = false;
+}
 
 // Re-packs every node word (V4.w): ICODT kept, ICODR and the FIXVEL direction from
 // h_code_hi.
@@ -319,6 +361,7 @@ int orad_destroy(orad_ctx *c) {
                   (void *)c->emap, (void *)c->bal_part, (void *)c->fv_mem, (void *)c->fv_work,
                   (void *)c->xv_stage})
     dfree(p);
+  free_contact(c);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -704,7 +747,7 @@ int orad_asspar4(orad_ctx *c, double *a, double *stifn) {
   int r = ready(c);
   if (r) return r;
   HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, launch_node(node_args(c), c->precision == ORAD_MIXED, c->host_ncycle & 1, 0, 1,
+  HIPCHK(c, launch_node(assembly_args(c), c->precision == ORAD_MIXED, c->host_ncycle & 1, 0, 1,
                         c->xcd_remap, c->stream, c->node_tile));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (a) HIPCHK(c, hipMemcpy(a, c->A, 3 * (size_t)c->numnod * 8, hipMemcpyDeviceToHost));
@@ -774,6 +817,19 @@ static int build_graph(orad_ctx *c, int p) {
   return ORAD_OK;
 }
 
+// The contact pass's capacities are fixed at orad_set_contact7; a pass that outgrew
+// them (sticky flag) has dropped candidates, so the run is reported as failed.
+static int contact_overflow(orad_ctx *c) {
+  if (!c->ci) return ORAD_OK;
+  int ovf = 0;
+  HIPCHK(c, hipMemcpy(&ovf, c->ci->counters + I7C_OVERFLOW, sizeof(int), hipMemcpyDeviceToHost));
+  if (ovf & I7_OVF_NODE)
+    return fail(c, ORAD_EINVAL, "contact: a secondary node has more than 64 candidate segments");
+  if (ovf & I7_OVF_CAND)
+    return fail(c, ORAD_EINVAL, "contact: candidate capacity exceeded (orad_set_contact7 cand_cap)");
+  return ORAD_OK;
+}
+
 int orad_run_cycles(orad_ctx *c, int ncycle, orad_time_t *t) {
   if (!c) return ORAD_EINVAL;
   if (int fr = flush_groups(c)) return fr;
@@ -808,6 +864,7 @@ int orad_run_cycles(orad_ctx *c, int ncycle, orad_time_t *t) {
     c->host_ncycle += 1;
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if ((r = contact_overflow(c))) return r;
   orad_time_t tt{};
   r = orad_get_time(c, &tt);
   if (r) return r;
@@ -826,6 +883,7 @@ int orad_profile_cycles(orad_ctx *c, int ncycle, double *elem_ms, double *node_m
     const int p = c->host_ncycle & 1;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     HIPCHK(c, launch_cycle_elements(c, p));
+    HIPCHK(c, launch_cycle_contact(c, p));
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHK(c, launch_node(node_args(c), c->precision == ORAD_MIXED, p, 1, 0, c->xcd_remap, c->stream,
                           c->node_tile));
@@ -1370,7 +1428,7 @@ int orad_asspar4_rot(orad_ctx *c, double *a, double *ar, double *stifn, double *
   int r = ready(c);
   if (r) return r;
   HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, launch_node(node_args(c), c->precision == ORAD_MIXED, c->host_ncycle & 1, 0, 1,
+  HIPCHK(c, launch_node(assembly_args(c), c->precision == ORAD_MIXED, c->host_ncycle & 1, 0, 1,
                         c->xcd_remap, c->stream, c->node_tile));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const size_t n = (size_t)c->numnod;
@@ -1460,6 +1518,8 @@ extern "C" {
 
 int orad_set_shell_models(orad_ctx *c, int nmodel, const int *node_off, const int *elem_off,
                           const double *p, const int *ip) {
+  if (c && c->ci)
+    return fail(c, ORAD_EINVAL, "set_shell_models: a context with a contact interface takes one model");
   if (!c) return ORAD_EINVAL;
   if (int fr = flush_groups(c)) return fr;
   if (c->family != FAM_SHELL || !c->have_shmat)
@@ -1652,8 +1712,219 @@ static std::vector<std::pair<void *, size_t>> cycle_buffers(orad_ctx *c) {
   live.push_back({c->bal_part, 4 * ((nn + 255) / 256) * sizeof(double)});
   live.push_back({c->fv_mem, nn * sizeof(double)});
   live.push_back({c->fv_work, nn * sizeof(double)});
+  if (c->ci) {
+    // contact: deactivated secondary nodes (negated STFN) and the cumulative energies
+    live.push_back({c->ci->stfns, (size_t)c->ci->nsn * sizeof(double)});
+    live.push_back({c->ci->energy, 3 * sizeof(double)});
+  }
   return live;
 }
+
+// ---------------------------------------------------------------- contact --
+// constant_mod.F's EM10, EM20, EM30 and ZEP4, each a double operation on exact or
+// previously rounded values, as the reference defines them.
+static void contact_constants(Contact7 &k) {
+  volatile double ten = 10.0;
+  double ep = 100.0;                      // EP02
+  for (int i = 3; i <= 10; ++i) ep = ep * ten;
+  const double ep10 = ep;
+  for (int i = 11; i <= 20; ++i) ep = ep * ten;
+  const double ep20 = ep, ep30 = ep20 * ep10;
+  k.em10 = 1.0 / ep10;
+  k.em20 = 1.0 / ep20;
+  k.em30 = 1.0 / ep30;
+  k.zep4 = 4.0 / ten;
+}
+
+extern "C" {
+
+int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int *irect,
+                      const double *stfm, const double *stfns, const double *params,
+                      int cand_cap) {
+  if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
+  if (!c->have_nodes) return fail(c, ORAD_ESTATE, "set_contact7: nodes must be set first");
+  if (c->batched)
+    return fail(c, ORAD_EINVAL, "set_contact7: batched models take no contact interface");
+  if (nsn < 0 || nrtm < 0 || (nsn > 0 && (!nsv || !stfns)) ||
+      (nrtm > 0 && (!irect || !stfm)) || !params)
+    return fail(c, ORAD_EINVAL, "set_contact7: bad sizes or null arrays");
+  const double gap = params[0], fric = params[1], visc = params[2], viscf = params[3];
+  if (!(gap > 0.0) || !(fric >= 0.0) || !(visc >= 0.0) || !(viscf >= 0.0))
+    return fail(c, ORAD_EINVAL, "set_contact7: GAP > 0, FRIC, VISs, VISf >= 0");
+  const int nn = c->numnod;
+  for (int i = 0; i < nsn; ++i)
+    if (nsv[i] < 1 || nsv[i] > nn) return fail(c, ORAD_EINVAL, "set_contact7: NSV out of range");
+  for (int i = 0; i < 4 * nrtm; ++i)
+    if (irect[i] < 1 || irect[i] > nn)
+      return fail(c, ORAD_EINVAL, "set_contact7: IRECT out of range");
+  const long long cap = cand_cap > 0 ? (long long)cand_cap
+                                     : std::max<long long>(16LL * nsn, 1024LL);
+  const long long cells = std::max<long long>(2LL * nrtm, 1024LL);
+  const long long lim = contact7_scan_max();
+  if (cap > lim || cells > lim || nsn > lim || nn > lim)
+    return fail(c, ORAD_EINVAL, "set_contact7: interface too large for one pass");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  free_contact(c);
+  drop_graphs(c);
+  Contact7 *k = new (std::nothrow) Contact7();
+  if (!k) return fail(c, ORAD_EHIP, "set_contact7: out of host memory");
+  c->ci = k;
+  k->nsn = nsn;
+  k->nrtm = nrtm;
+  k->numnod = nn;
+  k->gap = gap;
+  k->fric = fric;
+  k->visc = visc;
+  k->viscf = viscf;
+  k->tz = gap * 1.000001;
+  contact_constants(*k);
+  k->cell_cap = (int)cells;
+  k->cand_cap = (int)cap;
+  k->nbatch_cap = (int)((cap + I7_NVSIZ - 1) / I7_NVSIZ);
+  const size_t nslots = (size_t)k->nbatch_cap * I7_SLOTS;
+  const size_t ntmp = contact7_scan_tmp_ints((int)std::max<long long>(
+      std::max<long long>(cap, cells), std::max<long long>(nsn, nn)));
+  hipError_t e = hipSuccess;
+  auto al = [&](auto **p, size_t bytes) {
+    if (e == hipSuccess) e = dalloc(p, bytes);
+  };
+  al(&k->nsv, (size_t)nsn * 4);
+  al(&k->irect, (size_t)nrtm * sizeof(int4));
+  al(&k->stfm, (size_t)nrtm * 8);
+  al(&k->stfns, (size_t)nsn * 8);
+  al(&k->bounds, 8 * 8);
+  al(&k->grid, 4 * 8);
+  al(&k->gdims, 4 * 4);
+  al(&k->seg_cell, (size_t)nrtm * 4);
+  al(&k->cell_count, (size_t)cells * 4);
+  al(&k->cell_fill, (size_t)cells * 4);
+  al(&k->cell_start, (size_t)(cells + 1) * 4);
+  al(&k->cell_seg, (size_t)nrtm * 4);
+  al(&k->node_ncand, (size_t)nsn * 4);
+  al(&k->node_start, (size_t)(nsn + 1) * 4);
+  al(&k->cand_n, (size_t)cap * 4);
+  al(&k->cand_e, (size_t)cap * 4);
+  al(&k->keep, (size_t)cap * 4);
+  al(&k->rank, (size_t)(cap + 1) * 4);
+  al(&k->act, (size_t)cap * 4);
+  al(&k->counters, I7C_N * 4);
+  al(&k->sky, nslots * sizeof(double4));
+  al(&k->sky_node, nslots * 4);
+  al(&k->batch_cnt, (size_t)k->nbatch_cap * 4);
+  al(&k->batch_e, (size_t)k->nbatch_cap * 16);
+  al(&k->energy, 3 * 8);
+  al(&k->node_cnt, (size_t)nn * 4);
+  al(&k->node_fill, (size_t)nn * 4);
+  al(&k->node_bstart, (size_t)(nn + 1) * 4);
+  al(&k->node_bucket, nslots * 4);
+  al(&k->scan_tmp, ntmp * 4);
+  if (e != hipSuccess) {
+    free_contact(c);
+    return fail(c, ORAD_EHIP, std::string("set_contact7: ") + hipGetErrorString(e));
+  }
+  std::vector<int32_t> hn(nsn);
+  for (int i = 0; i < nsn; ++i) hn[i] = nsv[i] - 1;
+  std::vector<int4> hr(nrtm);
+  for (int l = 0; l < nrtm; ++l)
+    hr[l] = make_int4(irect[4 * l] - 1, irect[4 * l + 1] - 1, irect[4 * l + 2] - 1,
+                      irect[4 * l + 3] - 1);
+  if (nsn) {
+    HIPCHK(c, hipMemcpy(k->nsv, hn.data(), (size_t)nsn * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(k->stfns, stfns, (size_t)nsn * 8, hipMemcpyHostToDevice));
+  }
+  if (nrtm) {
+    HIPCHK(c, hipMemcpy(k->irect, hr.data(), (size_t)nrtm * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(k->stfm, stfm, (size_t)nrtm * 8, hipMemcpyHostToDevice));
+  }
+  HIPCHK(c, hipMemset(k->counters, 0, I7C_N * 4));
+  HIPCHK(c, hipMemset(k->energy, 0, 3 * 8));
+  HIPCHK(c, hipMemset(k->node_cnt, 0, (size_t)nn * 4));
+  HIPCHK(c, hipMemset(k->node_bstart, 0, (size_t)(nn + 1) * 4));
+  HIPCHK(c, hipDeviceSynchronize());
+  return ORAD_OK;
+}
+
+int orad_contact7(orad_ctx *c, double dt1, double *dt2t) {
+  if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
+  if (!c->ci) return fail(c, ORAD_ESTATE, "contact7: no contact interface (orad_set_contact7)");
+  int r = ready(c);
+  if (r) return r;
+  HIPCHK(c, hipSetDevice(c->device));
+  const unsigned long long inf = 0x7FF0000000000000ull;
+  HIPCHK(c, hipMemcpyAsync(c->scratch_dt1, &dt1, 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->scratch_dt2t, &inf, 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, contact7_run(*c->ci, c->X4, c->V4, c->scratch_dt1, c->scratch_dt2t, c->stream));
+  unsigned long long bits = inf;
+  HIPCHK(c, hipMemcpyAsync(&bits, c->scratch_dt2t, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if ((r = contact_overflow(c))) return r;
+  c->ci_fresh = true;
+  double d;
+  std::memcpy(&d, &bits, 8);
+  if (dt2t && d < *dt2t) *dt2t = d;
+  return ORAD_OK;
+}
+
+int orad_get_contact7(orad_ctx *c, double *energies, int *counts, double *stfns) {
+  if (!c) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
+  if (!c->ci) return fail(c, ORAD_ESTATE, "get_contact7: no contact interface");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (energies) HIPCHK(c, hipMemcpy(energies, c->ci->energy, 3 * 8, hipMemcpyDeviceToHost));
+  if (counts) {
+    int k[I7C_N];
+    HIPCHK(c, hipMemcpy(k, c->ci->counters, sizeof k, hipMemcpyDeviceToHost));
+    counts[0] = k[I7C_NCAND];
+    counts[1] = k[I7C_NPEN];
+    counts[2] = k[I7C_NSKY];
+    counts[3] = k[I7C_OVERFLOW];
+  }
+  if (stfns && c->ci->nsn)
+    HIPCHK(c, hipMemcpy(stfns, c->ci->stfns, (size_t)c->ci->nsn * 8, hipMemcpyDeviceToHost));
+  return ORAD_OK;
+}
+
+int orad_get_contact7_sky(orad_ctx *c, int cap, double *fsky, int *isky, int *nsky) {
+  if (!c || !nsky) return ORAD_EINVAL;
+  if (int fr = flush_groups(c)) return fr;
+  if (!c->ci) return fail(c, ORAD_ESTATE, "get_contact7_sky: no contact interface");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const Contact7 &k = *c->ci;
+  int cnt[I7C_N];
+  HIPCHK(c, hipMemcpy(cnt, k.counters, sizeof cnt, hipMemcpyDeviceToHost));
+  const int nb = (cnt[I7C_NPEN] + I7_NVSIZ - 1) / I7_NVSIZ;
+  std::vector<int32_t> bc(nb > 0 ? nb : 1);
+  if (nb) HIPCHK(c, hipMemcpy(bc.data(), k.batch_cnt, (size_t)nb * 4, hipMemcpyDeviceToHost));
+  int n = 0;
+  for (int b = 0; b < nb; ++b) n += bc[b];
+  *nsky = n;
+  if (!fsky && !isky) return ORAD_OK;
+  if (n > cap) return fail(c, ORAD_EINVAL, "get_contact7_sky: output capacity too small");
+  std::vector<double4> sk((size_t)nb * I7_SLOTS + 1);
+  std::vector<int32_t> nd((size_t)nb * I7_SLOTS + 1);
+  if (nb) {
+    HIPCHK(c, hipMemcpy(sk.data(), k.sky, (size_t)nb * I7_SLOTS * sizeof(double4),
+                        hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(nd.data(), k.sky_node, (size_t)nb * I7_SLOTS * 4, hipMemcpyDeviceToHost));
+  }
+  int o = 0;
+  for (int b = 0; b < nb; ++b)
+    for (int j = 0; j < bc[b]; ++j, ++o) {
+      const double4 &v = sk[(size_t)b * I7_SLOTS + j];
+      if (fsky) {
+        fsky[4 * o] = v.x; fsky[4 * o + 1] = v.y; fsky[4 * o + 2] = v.z; fsky[4 * o + 3] = v.w;
+      }
+      if (isky) isky[o] = nd[(size_t)b * I7_SLOTS + j] + 1;
+    }
+  return ORAD_OK;
+}
+
+}  // extern "C"
 
 // ------------------------------------------------------------ restart files --
 // The dynamic state of a context (the cycle buffers above, the cycle counter and the
@@ -1688,6 +1959,8 @@ static uint64_t model_fingerprint(const orad_ctx *c) {
   h = fnv1a(h, c->h_adsky.data(), c->h_adsky.size() * sizeof(int32_t));
   h = fnv1a(h, c->h_iads.data(), c->h_iads.size() * sizeof(int32_t));
   h = fnv1a(h, c->h_perm.data(), c->h_perm.size() * sizeof(int32_t));
+  const int64_t ci[2] = {c->ci ? c->ci->nsn : -1, c->ci ? c->ci->nrtm : -1};
+  h = fnv1a(h, ci, sizeof ci);
   return h;
 }
 

@@ -1,0 +1,647 @@
+// orad_contact7.hip -- /INTER/TYPE7 node-to-segment penalty contact (see
+// orad_contact7.hpp for the pass).  FP64 throughout, built with -ffp-contract=off: the
+// per-pair arithmetic keeps the reference routines' operation order (I7COR3
+// i7cor3.F:99-261, I7DST3 i7dst3.F:124-470, I7FOR3 i7for3.F:230-1121, 1460-1466,
+// 1866-1875, 2444-2480, 4108-4121, I7ASS2 i7ass3.F:1188-1285) so the skyline entries
+// equal the FP64 oracle's bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "orad_contact7.hpp"
+
+namespace orad {
+
+// ------------------------------------------------------------------ scans --
+// Exclusive prefix sums of int32 with the length read on the device (or given), so
+// the launches are fixed for graph capture: out[i] = sum in[0..i), out[n] = total.
+constexpr int SCAN_T = 1024, SCAN_E = 4, SCAN_B = SCAN_T * SCAN_E;
+constexpr int SCAN_TOP_E = 16;                 // block sums per thread of the top scan
+constexpr int SCAN_MAX = SCAN_B * SCAN_T * SCAN_TOP_E;
+
+size_t contact7_scan_tmp_ints(int n) { return (size_t)(n + SCAN_B - 1) / SCAN_B + 1; }
+int contact7_scan_max() { return SCAN_MAX; }
+
+// Exclusive scan of one value per thread over a 1024-thread workgroup.
+__device__ __forceinline__ int block_excl(int v, int *s_w, int *total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int i = 0; i < SCAN_T / 64; ++i) {
+      const int t = s_w[i];
+      s_w[i] = acc;
+      acc += t;
+    }
+    s_w[SCAN_T / 64] = acc;
+  }
+  __syncthreads();
+  *total = s_w[SCAN_T / 64];
+  return x - v + s_w[w];
+}
+
+__global__ __launch_bounds__(SCAN_T) void scan_blocks(const int32_t *in, int32_t *out,
+                                                      const int *n_dev, int n_static,
+                                                      int32_t *bsum) {
+  __shared__ int s_w[SCAN_T / 64 + 1];
+  const int n = n_dev ? *n_dev : n_static;
+  const int base = (int)blockIdx.x * SCAN_B;
+  if (base >= n) return;
+  const int i0 = base + (int)threadIdx.x * SCAN_E;
+  int v[SCAN_E], sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_E; ++k) {
+    v[k] = (i0 + k < n) ? in[i0 + k] : 0;
+    sum += v[k];
+  }
+  int total;
+  int pre = block_excl(sum, s_w, &total);
+#pragma unroll
+  for (int k = 0; k < SCAN_E; ++k) {
+    if (i0 + k < n) out[i0 + k] = pre;
+    pre += v[k];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_T) void scan_top(int32_t *bsum, const int *n_dev,
+                                                   int n_static, int32_t *out) {
+  __shared__ int s_w[SCAN_T / 64 + 1];
+  const int n = n_dev ? *n_dev : n_static;
+  const int nb = (n + SCAN_B - 1) / SCAN_B;
+  const int i0 = (int)threadIdx.x * SCAN_TOP_E;
+  int v[SCAN_TOP_E], sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_TOP_E; ++k) {
+    v[k] = (i0 + k < nb) ? bsum[i0 + k] : 0;
+    sum += v[k];
+  }
+  int total;
+  int pre = block_excl(sum, s_w, &total);
+#pragma unroll
+  for (int k = 0; k < SCAN_TOP_E; ++k) {
+    if (i0 + k < nb) bsum[i0 + k] = pre;
+    pre += v[k];
+  }
+  if (threadIdx.x == 0) out[n] = total;
+}
+
+__global__ __launch_bounds__(256) void scan_add(int32_t *out, const int *n_dev, int n_static,
+                                                const int32_t *bsum) {
+  const int n = n_dev ? *n_dev : n_static;
+  const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (i >= n) return;
+  out[i] += bsum[i / SCAN_B];
+}
+
+static hipError_t scan_excl(const int32_t *in, int32_t *out, const int *n_dev, int n_static,
+                            int max_n, int32_t *tmp, hipStream_t s) {
+  if (max_n <= 0) max_n = 1;
+  const int nb = (max_n + SCAN_B - 1) / SCAN_B;
+  hipLaunchKernelGGL(scan_blocks, dim3(nb), dim3(SCAN_T), 0, s, in, out, n_dev, n_static, tmp);
+  hipLaunchKernelGGL(scan_top, dim3(1), dim3(SCAN_T), 0, s, tmp, n_dev, n_static, out);
+  hipLaunchKernelGGL(scan_add, dim3((max_n + 255) / 256), dim3(256), 0, s, out, n_dev,
+                     n_static, tmp);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- helpers --
+// Order-preserving 64-bit keys of doubles (for atomicMin/atomicMax of a bound).
+__device__ __forceinline__ unsigned long long okey(double d) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(d);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double odec(unsigned long long k) {
+  const unsigned long long b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+
+__device__ __forceinline__ void seg_box(const Contact7 &c, const double4 *X4, int l,
+                                        double lo[3], double hi[3]) {
+  const int4 r = c.irect[l];
+  const double4 p0 = X4[r.x], p1 = X4[r.y], p2 = X4[r.z], p3 = X4[r.w];
+  lo[0] = dmin(dmin(dmin(p0.x, p1.x), p2.x), p3.x);
+  lo[1] = dmin(dmin(dmin(p0.y, p1.y), p2.y), p3.y);
+  lo[2] = dmin(dmin(dmin(p0.z, p1.z), p2.z), p3.z);
+  hi[0] = dmax(dmax(dmax(p0.x, p1.x), p2.x), p3.x);
+  hi[1] = dmax(dmax(dmax(p0.y, p1.y), p2.y), p3.y);
+  hi[2] = dmax(dmax(dmax(p0.z, p1.z), p2.z), p3.z);
+}
+
+__device__ __forceinline__ int cell_axis(double x, double o, double inv, int n) {
+  double f = floor((x - o) * inv);
+  if (!(f >= -1.0)) f = -1.0;        // below the grid, or not a number
+  if (f > (double)n) f = (double)n;
+  return (int)f;
+}
+
+// ------------------------------------------------------------ search grid --
+__global__ __launch_bounds__(256) void i7_reset(Contact7 c) {
+  const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (i < c.cell_cap) {
+    c.cell_count[i] = 0;
+    c.cell_fill[i] = 0;
+  }
+  if (i < c.numnod) {
+    c.node_cnt[i] = 0;
+    c.node_fill[i] = 0;
+  }
+  if (i < 3) {
+    c.bounds[i] = ~0ull;
+    c.bounds[3 + i] = 0ull;
+  }
+  if (i == 3) c.bounds[6] = 0ull;
+  if (i < I7C_N && i != I7C_OVERFLOW) c.counters[i] = 0;   // the overflow flag is sticky
+}
+
+__global__ __launch_bounds__(256) void i7_seg_bounds(Contact7 c, const double4 *X4) {
+  const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (l >= c.nrtm) return;
+  double lo[3], hi[3];
+  seg_box(c, X4, l, lo, hi);
+  double ext = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const double m = 0.5 * (lo[d] + hi[d]);
+    atomicMin(&c.bounds[d], okey(m));
+    atomicMax(&c.bounds[3 + d], okey(m));
+    ext = dmax(ext, hi[d] - lo[d]);
+  }
+  atomicMax(&c.bounds[6], okey(ext));
+}
+
+// Cell size: the largest segment box edge plus 2 TZ, so the centre of every segment
+// whose widened box holds a node lies in the node's cell or a neighbour; grown until
+// the grid fits cell_cap.
+__global__ void i7_grid(Contact7 c) {
+  double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
+  if (c.nrtm > 0)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = odec(c.bounds[d]);
+      hi[d] = odec(c.bounds[3 + d]);
+    }
+  double s = (c.nrtm > 0 ? odec(c.bounds[6]) : 0.0) + 2.0 * c.tz;
+  if (!(s > 0.0)) s = 1.0;
+  int n[3] = {1, 1, 1};
+  bool fits = false;
+  for (int it = 0; it < 200 && !fits; ++it) {
+    double tot = 1.0;
+    for (int d = 0; d < 3; ++d) {
+      const double k = floor((hi[d] - lo[d]) / s) + 1.0;
+      n[d] = (k >= 1.0 && k <= 1.0e9) ? (int)k : (k > 1.0e9 ? 1000000000 : 1);
+      tot *= (double)n[d];
+    }
+    fits = tot <= (double)c.cell_cap;
+    if (!fits) s *= 1.5;
+  }
+  if (!fits) {
+    // non-finite coordinates: one cell holding every segment
+    n[0] = n[1] = n[2] = 1;
+    s = 1.0e300;
+  }
+  for (int d = 0; d < 3; ++d) {
+    c.grid[d] = lo[d];
+    c.gdims[d] = n[d];
+  }
+  c.grid[3] = s;
+  c.gdims[3] = n[0] * n[1] * n[2];
+}
+
+__global__ __launch_bounds__(256) void i7_seg_count(Contact7 c, const double4 *X4) {
+  const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (l >= c.nrtm) return;
+  double lo[3], hi[3];
+  seg_box(c, X4, l, lo, hi);
+  const double inv = 1.0 / c.grid[3];
+  int k[3];
+  for (int d = 0; d < 3; ++d) {
+    int q = cell_axis(0.5 * (lo[d] + hi[d]), c.grid[d], inv, c.gdims[d]);
+    k[d] = q < 0 ? 0 : (q >= c.gdims[d] ? c.gdims[d] - 1 : q);
+  }
+  const int cell = k[0] + c.gdims[0] * (k[1] + c.gdims[1] * k[2]);
+  c.seg_cell[l] = cell;
+  atomicAdd(&c.cell_count[cell], 1);
+}
+
+__global__ __launch_bounds__(256) void i7_seg_place(Contact7 c) {
+  const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (l >= c.nrtm) return;
+  const int cell = c.seg_cell[l];
+  c.cell_seg[c.cell_start[cell] + atomicAdd(&c.cell_fill[cell], 1)] = l;
+}
+
+// I7TRI's test (i7tri.F:530-566) for node nn and segment l, box widened by TZ.
+__device__ __forceinline__ bool i7_box(const Contact7 &c, const double4 *X4, int nn,
+                                       const double4 &x, int l) {
+  if (c.stfm[l] == 0.0) return false;
+  const int4 r = c.irect[l];
+  if (nn == r.x || nn == r.y || nn == r.z || nn == r.w) return false;
+  double lo[3], hi[3];
+  seg_box(c, X4, l, lo, hi);
+  return x.x > lo[0] - c.tz && x.x < hi[0] + c.tz && x.y > lo[1] - c.tz &&
+         x.y < hi[1] + c.tz && x.z > lo[2] - c.tz && x.z < hi[2] + c.tz;
+}
+
+// Each secondary node's candidate segments, in increasing segment order.
+template <bool WRITE>
+__global__ __launch_bounds__(128) void i7_node_cand(Contact7 c, const double4 *X4) {
+  const int i = (int)blockIdx.x * 128 + (int)threadIdx.x;
+  if (i >= c.nsn) return;
+  int list[I7_NODE_CAP];
+  int cnt = 0;
+  if (c.stfns[i] != 0.0) {
+    const int nn = c.nsv[i];
+    const double4 x = X4[nn];
+    const double inv = 1.0 / c.grid[3];
+    const int kx = cell_axis(x.x, c.grid[0], inv, c.gdims[0]);
+    const int ky = cell_axis(x.y, c.grid[1], inv, c.gdims[1]);
+    const int kz = cell_axis(x.z, c.grid[2], inv, c.gdims[2]);
+    for (int z = kz - 1; z <= kz + 1; ++z) {
+      if (z < 0 || z >= c.gdims[2]) continue;
+      for (int y = ky - 1; y <= ky + 1; ++y) {
+        if (y < 0 || y >= c.gdims[1]) continue;
+        for (int xx = kx - 1; xx <= kx + 1; ++xx) {
+          if (xx < 0 || xx >= c.gdims[0]) continue;
+          const int cell = xx + c.gdims[0] * (y + c.gdims[1] * z);
+          for (int j = c.cell_start[cell]; j < c.cell_start[cell + 1]; ++j) {
+            const int l = c.cell_seg[j];
+            if (!i7_box(c, X4, nn, x, l)) continue;
+            if (WRITE) {
+              // insertion in increasing segment order
+              if (cnt < I7_NODE_CAP) {
+                int p = cnt;
+                while (p > 0 && list[p - 1] > l) {
+                  list[p] = list[p - 1];
+                  --p;
+                }
+                list[p] = l;
+              }
+            }
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+  if (cnt > I7_NODE_CAP) {
+    if (!WRITE) atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_NODE);
+    cnt = I7_NODE_CAP;
+  }
+  if (!WRITE) {
+    c.node_ncand[i] = cnt;
+    return;
+  }
+  const int s = c.node_start[i];
+  for (int k = 0; k < cnt; ++k) {
+    if (s + k >= c.cand_cap) break;
+    c.cand_n[s + k] = i;
+    c.cand_e[s + k] = list[k];
+  }
+}
+
+__global__ void i7_cand_total(Contact7 c) {
+  const int tot = c.node_start[c.nsn];
+  if (tot > c.cand_cap) atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_CAND);
+  c.counters[I7C_NCAND] = tot > c.cand_cap ? c.cand_cap : tot;
+}
+
+// ------------------------------------------------------------- per pair --
+struct I7Pair {
+  int ix[4], nsvg;
+  double xi[3], vi[3], msi;
+  double x[4][3];
+  double nx[4][3], lb[4], lc[4], p[4], stif, gapv;
+};
+
+// I7COR3 + I7DST3 for candidate (ni, l); true when the pair penetrates (KEEP).
+__device__ __noinline__ bool i7_dist(const Contact7 &c, const double4 *X4, const double4 *V4,
+                                     int ni, int l, I7Pair &q) {
+  q.gapv = c.gap;
+  const int ig = c.nsv[ni];
+  q.nsvg = ig;
+  const double4 xi = X4[ig], vi = V4[ig];
+  q.xi[0] = xi.x; q.xi[1] = xi.y; q.xi[2] = xi.z;
+  q.vi[0] = vi.x; q.vi[1] = vi.y; q.vi[2] = vi.z;
+  q.msi = xi.w;
+  const int4 r = c.irect[l];
+  q.ix[0] = r.x; q.ix[1] = r.y; q.ix[2] = r.z; q.ix[3] = r.w;
+  for (int k = 0; k < 4; ++k) {
+    const double4 p = X4[q.ix[k]];
+    q.x[k][0] = p.x; q.x[k][1] = p.y; q.x[k][2] = p.z;
+  }
+  q.stif = c.stfm[l] * fabs(c.stfns[ni]);
+  double x0[3];
+  if (q.ix[2] != q.ix[3]) {
+    for (int d = 0; d < 3; ++d) x0[d] = 0.25 * (q.x[0][d] + q.x[1][d] + q.x[2][d] + q.x[3][d]);
+  } else {
+    for (int d = 0; d < 3; ++d) x0[d] = q.x[2][d];
+  }
+  double x0k[4][3], xik[4][3], xi0v[3];
+  for (int k = 0; k < 4; ++k)
+    for (int d = 0; d < 3; ++d) x0k[k][d] = q.x[k][d] - x0[d];
+  for (int d = 0; d < 3; ++d) xi0v[d] = x0[d] - q.xi[d];
+  for (int k = 0; k < 4; ++k)
+    for (int d = 0; d < 3; ++d) xik[k][d] = q.x[k][d] - q.xi[d];
+  double s[4][3];
+  for (int k = 0; k < 4; ++k) {
+    s[k][0] = xi0v[1] * xik[k][2] - xi0v[2] * xik[k][1];
+    s[k][1] = xi0v[2] * xik[k][0] - xi0v[0] * xik[k][2];
+    s[k][2] = xi0v[0] * xik[k][1] - xi0v[1] * xik[k][0];
+  }
+  double lb[4], lc[4];
+  for (int k = 0; k < 4; ++k) {
+    const int k1 = (k + 1) & 3;
+    const double sx0 = x0k[k][1] * x0k[k1][2] - x0k[k][2] * x0k[k1][1];
+    const double sy0 = x0k[k][2] * x0k[k1][0] - x0k[k][0] * x0k[k1][2];
+    const double sz0 = x0k[k][0] * x0k[k1][1] - x0k[k][1] * x0k[k1][0];
+    const double s2 = 1.0 / dmax(c.em30, sx0 * sx0 + sy0 * sy0 + sz0 * sz0);
+    lb[k] = -(sx0 * s[k1][0] + sy0 * s[k1][1] + sz0 * s[k1][2]) * s2;
+    lc[k] = (sx0 * s[k][0] + sy0 * s[k][1] + sz0 * s[k][2]) * s2;
+  }
+  double hlb[4], hlc[4], al[4];
+  for (int k = 0; k < 4; ++k) {
+    const int km = (k + 3) & 3;
+    const double aaa = 1.0 / dmax(c.em30, x0k[k][0] * x0k[k][0] + x0k[k][1] * x0k[k][1] +
+                                              x0k[k][2] * x0k[k][2]);
+    hlc[k] = lc[k] * fabs(lc[k]) * aaa;
+    hlb[km] = lb[km] * fabs(lb[km]) * aaa;
+    double a = -(xi0v[0] * x0k[k][0] + xi0v[1] * x0k[k][1] + xi0v[2] * x0k[k][2]) * aaa;
+    al[k] = dmax(0.0, dmin(1.0, a));
+  }
+  for (int k = 0; k < 4; ++k) {
+    const int k1 = (k + 1) & 3;
+    const double ex = q.x[k1][0] - q.x[k][0];
+    const double ey = q.x[k1][1] - q.x[k][1];
+    const double ez = q.x[k1][2] - q.x[k][2];
+    const double la = 1.0 - lb[k] - lc[k];
+    const double aaa = 1.0 / dmax(c.em20, ex * ex + ey * ey + ez * ez);
+    const double hla = la * fabs(la) * aaa;
+    if (la < 0.0 && hla <= hlb[k] && hla <= hlc[k]) {
+      double t = (xik[k1][0] * ex + xik[k1][1] * ey + xik[k1][2] * ez) * aaa;
+      lb[k] = dmax(0.0, dmin(1.0, t));
+      lc[k] = 1.0 - lb[k];
+    } else if (lb[k] < 0.0 && hlb[k] <= hlc[k] && hlb[k] <= hla) {
+      lb[k] = 0.0;
+      lc[k] = al[k1];
+    } else if (lc[k] < 0.0 && hlc[k] <= hla && hlc[k] <= hlb[k]) {
+      lc[k] = 0.0;
+      lb[k] = al[k];
+    }
+  }
+  const double gap2 = q.gapv * q.gapv;
+  double pene2 = 0.0;
+  for (int k = 0; k < 4; ++k) {
+    const int k1 = (k + 1) & 3;
+    for (int d = 0; d < 3; ++d)
+      q.nx[k][d] = q.xi[d] - (x0[d] + lb[k] * x0k[k][d] + lc[k] * x0k[k1][d]);
+    q.p[k] = q.nx[k][0] * q.nx[k][0] + q.nx[k][1] * q.nx[k][1] + q.nx[k][2] * q.nx[k][2];
+    const double dk = dmax(0.0, gap2 - q.p[k]);
+    pene2 = k == 0 ? dk : dmax(pene2, dk);
+    q.lb[k] = lb[k];
+    q.lc[k] = lc[k];
+  }
+  return pene2 != 0.0 && q.stif != 0.0;
+}
+
+__global__ __launch_bounds__(256) void i7_keep(Contact7 c, const double4 *X4, const double4 *V4) {
+  const int k = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (k >= c.counters[I7C_NCAND]) return;
+  I7Pair q;
+  c.keep[k] = i7_dist(c, X4, V4, c.cand_n[k], c.cand_e[k], q) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void i7_compact(Contact7 c) {
+  const int n = c.counters[I7C_NCAND];
+  const int k = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (k == 0) c.counters[I7C_NPEN] = c.rank[n];
+  if (k >= n) return;
+  if (c.keep[k]) c.act[c.rank[k]] = k;
+}
+
+// I7FOR3 + I7ASS2 for one batch of NVSIZ penetrating pairs, one lane per pair.
+__global__ __launch_bounds__(I7_NVSIZ) void i7_batch(Contact7 c, const double4 *X4,
+                                                     const double4 *V4, const double *dt1p,
+                                                     unsigned long long *dt2t) {
+  __shared__ double s_rd[I7_NVSIZ], s_ec[I7_NVSIZ], s_ev[I7_NVSIZ];
+  __shared__ int s_cnt[5][2];
+  const int npen = c.counters[I7C_NPEN];
+  const int b = (int)blockIdx.x;
+  if (b * I7_NVSIZ >= npen) return;
+  const int t = (int)threadIdx.x;
+  const int r = b * I7_NVSIZ + t;
+  const bool live = r < npen;
+  const int nlive = npen - b * I7_NVSIZ < I7_NVSIZ ? npen - b * I7_NVSIZ : I7_NVSIZ;
+  const double dt1 = *dt1p;
+  const double dt1inv = dt1 > 0.0 ? 1.0 / dt1 : 0.0;
+  double h[4] = {0.0, 0.0, 0.0, 0.0}, fxi[3] = {0.0, 0.0, 0.0}, stif = 0.0;
+  double rdist = 1.0e300, econtt = 0.0, econvt = 0.0;
+  int ix[4] = {0, 0, 0, 0}, nsvg = 0;
+  if (live) {
+    const int k = c.act[r];
+    const int ni = c.cand_n[k];
+    I7Pair q;
+    i7_dist(c, X4, V4, ni, c.cand_e[k], q);
+    for (int j = 0; j < 4; ++j) ix[j] = q.ix[j];
+    nsvg = q.nsvg;
+    stif = q.stif;
+    const double g = q.gapv;
+    double pp[4], a[4], n[3], pene;
+    for (int j = 0; j < 4; ++j) {
+      const double dk = sqrt(q.p[j]);
+      pp[j] = dmax(0.0, g - dk);
+      a[j] = pp[j] / dmax(c.em20, dk);
+    }
+    for (int d = 0; d < 3; ++d)
+      n[d] = a[0] * q.nx[0][d] + a[1] * q.nx[1][d] + a[2] * q.nx[2][d] + a[3] * q.nx[3][d];
+    if (q.ix[2] != q.ix[3]) {
+      pene = dmax(dmax(dmax(pp[0], pp[1]), pp[2]), pp[3]);
+      double la[4];
+      for (int j = 0; j < 4; ++j) la[j] = 1.0 - q.lb[j] - q.lc[j];
+      const double h0 = 0.25 * (pp[0] * la[0] + pp[1] * la[1] + pp[2] * la[2] + pp[3] * la[3]);
+      h[0] = h0 + pp[0] * q.lb[0] + pp[3] * q.lc[3];
+      h[1] = h0 + pp[1] * q.lb[1] + pp[0] * q.lc[0];
+      h[2] = h0 + pp[2] * q.lb[2] + pp[1] * q.lc[1];
+      h[3] = h0 + pp[3] * q.lb[3] + pp[2] * q.lc[2];
+      const double h00 = 1.0 / dmax(c.em20, h[0] + h[1] + h[2] + h[3]);
+      for (int j = 0; j < 4; ++j) h[j] = h[j] * h00;
+    } else {
+      pene = pp[0];
+      for (int d = 0; d < 3; ++d) n[d] = q.nx[0][d];
+      h[0] = q.lb[0];
+      h[1] = q.lc[0];
+      h[2] = 1.0 - q.lb[0] - q.lc[0];
+      h[3] = 0.0;
+    }
+    const double s2 = 1.0 / dmax(c.em30, sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]));
+    for (int d = 0; d < 3; ++d) n[d] = n[d] * s2;
+    double vx[3];
+    for (int d = 0; d < 3; ++d) {
+      const double v0 = d == 0 ? V4[ix[0]].x : (d == 1 ? V4[ix[0]].y : V4[ix[0]].z);
+      const double v1 = d == 0 ? V4[ix[1]].x : (d == 1 ? V4[ix[1]].y : V4[ix[1]].z);
+      const double v2 = d == 0 ? V4[ix[2]].x : (d == 1 ? V4[ix[2]].y : V4[ix[2]].z);
+      const double v3 = d == 0 ? V4[ix[3]].x : (d == 1 ? V4[ix[3]].y : V4[ix[3]].z);
+      vx[d] = q.vi[d] - h[0] * v0 - h[1] * v1 - h[2] * v2 - h[3] * v3;
+    }
+    const double vn = n[0] * vx[0] + n[1] * vx[1] + n[2] * vx[2];
+    double hc = -0.25 * (h[0] - h[1] + h[2] - h[3]);
+    hc = dmin(dmin(hc, h[1]), h[3]);
+    hc = dmax(dmax(hc, -h[0]), -h[2]);
+    if (q.ix[2] == q.ix[3]) hc = 0.0;
+    h[0] = h[0] + hc;
+    h[1] = h[1] - hc;
+    h[2] = h[2] + hc;
+    h[3] = h[3] - hc;
+    // contact time step of the pair
+    rdist = 0.5 * (g - pene) / dmax(c.em30, -vn);
+    // nonlinear penalty
+    const double fac = g / dmax(c.em10, g - pene);
+    const double facm1 = 1.0 / fac;
+    if ((g - pene) / g < c.em10 && stif > 0.0) {
+      stif = 0.0;
+      c.stfns[ni] = -fabs(c.stfns[ni]);
+    }
+    econtt = 0.5 * stif * (g * g) * (facm1 - 1.0 - log(dmax(2.2250738585072014e-308, facm1)));
+    stif = 0.5 * stif * fac;
+    double fni = -stif * pene;
+    double vis2 = 0.0;
+    if (c.visc != 0.0 || c.viscf != 0.0) {
+      vis2 = 2.0 * stif * q.msi;
+      if (vn < 0.0) vis2 = vis2 / dmax(c.em10, (g - pene) / g);
+      const double fac2 = stif / dmax(c.em30, stif);
+      const double vis = sqrt(vis2);
+      double ff = fac2 * (c.visc * vis + c.zep4 * c.zep4 * 2.0 * q.msi * dmax(0.0, -vn) /
+                                              dmax(g - pene, c.em10));
+      stif = stif * g / dmax(g - pene, c.em10);
+      stif = stif + ff * dt1inv;
+      stif = dmax(stif, fac2 * sqrt(c.viscf) * vis * dt1inv);
+      ff = ff * vn;
+      fni = fni + ff;
+    } else {
+      stif = stif * g / dmax(g - pene, c.em10);
+    }
+    for (int d = 0; d < 3; ++d) fxi[d] = n[d] * fni;
+    // Coulomb friction, total viscous formulation
+    double tv[3];
+    for (int d = 0; d < 3; ++d) tv[d] = vx[d] - n[d] * vn;
+    const double v2 = tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2];
+    const double vis2f = c.viscf * vis2;
+    const double fm2 = (c.fric * fni) * (c.fric * fni);
+    const double f2 = vis2f * v2;
+    const double a2 = dmin(f2, fm2) / dmax(c.em30, f2);
+    const double aa = sqrt(a2 * vis2f);
+    for (int d = 0; d < 3; ++d) fxi[d] = fxi[d] + aa * tv[d];
+    econvt = aa * v2 * dt1;
+  }
+  s_rd[t] = rdist;
+  s_ec[t] = econtt;
+  s_ev[t] = econvt;
+  // role counts per wave (skyline layout of I7ASS2)
+  const int lane = t & 63, w = t >> 6;
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  unsigned long long m[4];
+  for (int j = 0; j < 4; ++j) {
+    m[j] = __ballot(live && h[j] != 0.0);
+    if (lane == 0) s_cnt[j][w] = __popcll(m[j]);
+  }
+  __syncthreads();
+  if (t == 0) {
+    // batch contact time step (i7for3.F:772-870) and energies, in pair order
+    double dti = (double)1.E20f, ec = 0.0, ev = 0.0;
+    for (int i = 0; i < nlive; ++i) dti = dmin(s_rd[i], dti);
+    if (dti <= 0.0) dti = (double)1.E20f;
+    atomicMin(dt2t, (unsigned long long)__double_as_longlong(dti));
+    for (int i = 0; i < nlive; ++i) ec = ec + s_ec[i];
+    for (int i = 0; i < nlive; ++i) ev = ev + s_ev[i];
+    c.batch_e[2 * b] = ec;
+    c.batch_e[2 * b + 1] = ev;
+    int tot = 0;
+    for (int j = 0; j < 4; ++j) tot += s_cnt[j][0] + s_cnt[j][1];
+    c.batch_cnt[b] = tot + nlive;
+  }
+  if (!live) return;
+  double4 *sky = c.sky + (size_t)b * I7_SLOTS;
+  int32_t *node = c.sky_node + (size_t)b * I7_SLOTS;
+  int base = 0;
+  for (int j = 0; j < 4; ++j) {
+    if (h[j] != 0.0) {
+      const int pos = base + (w ? s_cnt[j][0] : 0) + __popcll(m[j] & lt);
+      sky[pos] = make_double4(fxi[0] * h[j], fxi[1] * h[j], fxi[2] * h[j], stif * fabs(h[j]));
+      node[pos] = ix[j];
+    }
+    base += s_cnt[j][0] + s_cnt[j][1];
+  }
+  sky[base + t] = make_double4(-fxi[0], -fxi[1], -fxi[2], stif);
+  node[base + t] = nsvg;
+}
+
+// ------------------------------------------------------ per-node buckets --
+__global__ __launch_bounds__(256) void i7_bucket(Contact7 c, int place) {
+  const int e = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  const int npen = c.counters[I7C_NPEN];
+  const int b = e / I7_SLOTS;
+  if (b * I7_NVSIZ >= npen) return;
+  if (e - b * I7_SLOTS >= c.batch_cnt[b]) return;
+  const int nd = c.sky_node[e];
+  if (!place) {
+    atomicAdd(&c.node_cnt[nd], 1);
+    return;
+  }
+  c.node_bucket[c.node_bstart[nd] + atomicAdd(&c.node_fill[nd], 1)] = e;
+}
+
+// Cumulative energies, batch by batch (ECONT = ECONT + ECONTT per I7FOR3 call).
+__global__ void i7_energy(Contact7 c) {
+  const int npen = c.counters[I7C_NPEN];
+  const int nb = (npen + I7_NVSIZ - 1) / I7_NVSIZ;
+  double ec = c.energy[0], ev = c.energy[1];
+  int nsky = 0;
+  for (int b = 0; b < nb; ++b) {
+    ec = ec + c.batch_e[2 * b];
+    ev = ev + c.batch_e[2 * b + 1];
+    nsky += c.batch_cnt[b];
+  }
+  c.energy[0] = ec;
+  c.energy[1] = ev;
+  c.counters[I7C_NSKY] = nsky;
+}
+
+hipError_t contact7_run(const Contact7 &c, const double4 *X4, const double4 *V4,
+                        const double *dt1, unsigned long long *dt2t, hipStream_t s) {
+  auto g = [](long n, int blk) { return dim3((unsigned)((n + blk - 1) / blk > 0 ? (n + blk - 1) / blk : 1)); };
+  long nreset = c.cell_cap;
+  if (c.numnod > nreset) nreset = c.numnod;
+  if (nreset < 8) nreset = 8;
+  hipLaunchKernelGGL(i7_reset, g(nreset, 256), dim3(256), 0, s, c);
+  hipLaunchKernelGGL(i7_seg_bounds, g(c.nrtm, 256), dim3(256), 0, s, c, X4);
+  hipLaunchKernelGGL(i7_grid, dim3(1), dim3(1), 0, s, c);
+  hipLaunchKernelGGL(i7_seg_count, g(c.nrtm, 256), dim3(256), 0, s, c, X4);
+  hipError_t e = scan_excl(c.cell_count, c.cell_start, c.gdims + 3, 0, c.cell_cap, c.scan_tmp, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(i7_seg_place, g(c.nrtm, 256), dim3(256), 0, s, c);
+  hipLaunchKernelGGL(i7_node_cand<false>, g(c.nsn, 128), dim3(128), 0, s, c, X4);
+  e = scan_excl(c.node_ncand, c.node_start, nullptr, c.nsn, c.nsn, c.scan_tmp, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(i7_node_cand<true>, g(c.nsn, 128), dim3(128), 0, s, c, X4);
+  hipLaunchKernelGGL(i7_cand_total, dim3(1), dim3(1), 0, s, c);
+  hipLaunchKernelGGL(i7_keep, g(c.cand_cap, 256), dim3(256), 0, s, c, X4, V4);
+  e = scan_excl(c.keep, c.rank, c.counters + I7C_NCAND, 0, c.cand_cap, c.scan_tmp, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(i7_compact, g(c.cand_cap, 256), dim3(256), 0, s, c);
+  hipLaunchKernelGGL(i7_batch, dim3(c.nbatch_cap > 0 ? c.nbatch_cap : 1), dim3(I7_NVSIZ), 0, s,
+                     c, X4, V4, dt1, dt2t);
+  const long nslots = (long)c.nbatch_cap * I7_SLOTS;
+  hipLaunchKernelGGL(i7_bucket, g(nslots, 256), dim3(256), 0, s, c, 0);
+  e = scan_excl(c.node_cnt, c.node_bstart, nullptr, c.numnod, c.numnod, c.scan_tmp, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(i7_bucket, g(nslots, 256), dim3(256), 0, s, c, 1);
+  hipLaunchKernelGGL(i7_energy, dim3(1), dim3(1), 0, s, c);
+  return hipGetLastError();
+}
+
+}  // namespace orad

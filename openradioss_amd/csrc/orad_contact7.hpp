@@ -1,0 +1,86 @@
+// orad_contact7.hpp -- /INTER/TYPE7 node-to-segment penalty contact on the device
+// (internal to liborad).
+//
+// Path: constant gap, IGSTI<=1, INACTI=0, no curvature, no heat exchange, no tied
+// contact, IVIS2=0 damping, Coulomb friction in the total viscous formulation without
+// filtering (IFQ=0, MFROT=0, isotropic).  Every cycle, on the context's stream and in
+// its captured graph:
+//   search      segments binned on a uniform grid by their box centre; each secondary
+//               node lists the segments whose box, widened by TZ, holds it (I7TRI's
+//               test), in increasing segment order, so the candidate list is ordered
+//               by (secondary node, segment) without a global sort;
+//   filter      I7COR3 + I7DST3 per candidate; penetrating pairs compacted in order;
+//   forces      one 128-lane workgroup per batch of NVSIZ = 128 penetrating pairs:
+//               I7FOR3 and I7ASS2's skyline layout (main node 1 of every pair, then
+//               nodes 2, 3, 4 where H /= 0, then the secondary nodes) in the batch's
+//               640-entry region; the batch's contact time step lowers DT2T;
+//   assembly    each node's entries gathered into a bucket; the nodal kernel adds
+//               them after the node's element slots in increasing entry order, i.e.
+//               ASSPAR4's skyline order.
+// Every order is fixed (atomics only count and place; buckets are read sorted), so a
+// run is bitwise reproducible and equal to the FP64 oracle's.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace orad {
+
+constexpr int I7_NVSIZ = 128;             // pairs per batch (the reference's NVSIZ)
+constexpr int I7_SLOTS = 5 * I7_NVSIZ;    // skyline entries one batch can write
+constexpr int I7_NODE_CAP = 64;           // candidates of one secondary node
+
+// counters[] of a pass
+enum { I7C_NCAND = 0, I7C_NPEN = 1, I7C_OVERFLOW = 2, I7C_NSKY = 3, I7C_N = 4 };
+// overflow bits
+enum { I7_OVF_NODE = 1, I7_OVF_CAND = 2 };
+
+struct Contact7 {
+  int nsn = 0, nrtm = 0, numnod = 0;
+  int32_t *nsv = nullptr;       // [nsn] secondary nodes, 0-based
+  int4 *irect = nullptr;        // [nrtm] segment nodes, 0-based (3rd = 4th: triangle)
+  double *stfm = nullptr;       // [nrtm] segment stiffness STF
+  double *stfns = nullptr;      // [nsn] secondary factor STFN, negated on deactivation
+  double gap = 0.0, fric = 0.0, visc = 0.0, viscf = 0.0, tz = 0.0;
+  double em10 = 0.0, em20 = 0.0, em30 = 0.0, zep4 = 0.0;   // constant_mod.F values
+  // search grid over the segments' box centres
+  int cell_cap = 0;
+  unsigned long long *bounds = nullptr;  // [7] ordered bits: min xyz, max xyz, max extent
+  double *grid = nullptr;                // [4] origin xyz, cell size
+  int *gdims = nullptr;                  // [4] cells per axis, total
+  int32_t *seg_cell = nullptr;           // [nrtm]
+  int32_t *cell_count = nullptr, *cell_fill = nullptr;   // [cell_cap]
+  int32_t *cell_start = nullptr;         // [cell_cap + 1]
+  int32_t *cell_seg = nullptr;           // [nrtm]
+  // candidates (secondary index into nsv, segment), then the penetrating ones
+  int cand_cap = 0, nbatch_cap = 0;
+  int32_t *node_ncand = nullptr;         // [nsn]
+  int32_t *node_start = nullptr;         // [nsn + 1]
+  int32_t *cand_n = nullptr, *cand_e = nullptr;   // [cand_cap]
+  int32_t *keep = nullptr;               // [cand_cap]
+  int32_t *rank = nullptr;               // [cand_cap + 1]
+  int32_t *act = nullptr;                // [cand_cap]
+  int *counters = nullptr;               // [I7C_N]
+  // skyline: batch b owns entries [b * I7_SLOTS, b * I7_SLOTS + batch_cnt[b])
+  double4 *sky = nullptr;                // (fx, fy, fz, stif)
+  int32_t *sky_node = nullptr;
+  int32_t *batch_cnt = nullptr;          // [nbatch_cap]
+  double *batch_e = nullptr;             // [2 * nbatch_cap] elastic, friction energy
+  double *energy = nullptr;              // [3] cumulative ECONT, ECONTV, ECONTD
+  // per-node buckets of entry numbers (ASSPAR4's sort of FSKYI by node)
+  int32_t *node_cnt = nullptr, *node_fill = nullptr;     // [numnod]
+  int32_t *node_bstart = nullptr;        // [numnod + 1]
+  int32_t *node_bucket = nullptr;        // [nbatch_cap * I7_SLOTS]
+  int32_t *scan_tmp = nullptr;           // block sums of the scans
+};
+
+// One contact pass: search, filter, forces, buckets.  dt1 is read on the device
+// (DT1 of the cycle); the batches' contact time steps lower *dt2t (bits of a positive
+// double, atomicMin).
+hipError_t contact7_run(const Contact7 &ci, const double4 *X4, const double4 *V4,
+                        const double *dt1, unsigned long long *dt2t, hipStream_t s);
+// Scan workspace (ints) for n elements, and the longest scan one pass can do.
+size_t contact7_scan_tmp_ints(int n);
+int contact7_scan_max();
+
+}  // namespace orad

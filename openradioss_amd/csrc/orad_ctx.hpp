@@ -11,6 +11,7 @@
 
 #include "../../include/orad.h"
 #include "orad_common.hpp"
+#include "orad_contact7.hpp"
 
 namespace orad {
 hipError_t launch_brick_fp64(const BrickArgs<double> &, int, int, const double *,
@@ -182,6 +183,10 @@ struct orad_ctx {
   int batch_lds = 0;        // batched shells: dynamic LDS bytes per workgroup (residency cap)
   int batch_grid = 0;       // persistent grid size (resident capacity at batch_lds)
   int num_cu = 0;
+
+  // /INTER/TYPE7 contact (orad_set_contact7), or null
+  Contact7 *ci = nullptr;
+  bool ci_fresh = false;    // orad_contact7 ran since the last orad_asspar4
 
   AdjStore *adj = nullptr;
   int cycle_write_a = 0;    // cycles also store accelerations A / AR (adjoint recomputation)

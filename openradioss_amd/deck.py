@@ -209,6 +209,91 @@ def shell_plate(n=512, h=1.0, thk=1.0, patch=32, v0=1.5e4, noise=0.1, thk_noise=
                 mat=mat, prop=shell_prop_params(npt=npt, ishell=ishell))
 
 
+def shell_plates_contact(n=16, m=8, h=1.0, thk=0.5, v0=-2.0e4, gap=None, fric=0.0,
+                         visc=0.05, viscf=0.0, stfac=1.0, ishell=1, ipla=0, tilt=0.0,
+                         seed=20261205):
+    """A free m x m shell plate dropped on a clamped n x n plate, with a /INTER/TYPE7
+    interface: the upper plate's nodes are secondary, the lower plate's shells the main
+    segments.  The upper plate starts gap/4 outside the gap, centred, tilted by `tilt`
+    (radians about the y axis) and moving at Vz = v0 (1 + U(-0.05, 0.05)).  Segment
+    stiffness 0.5 Stfac E t, the documented TYPE7 stiffness of a shell segment; gap
+    defaults to the mean of the two plates' half thicknesses summed (= thk)."""
+    lower = shell_plate(n=n, h=h, thk=thk, patch=0, v0=0.0, npt=5, ipla=ipla, ishell=ishell,
+                        seed=seed)
+    lower["v"][:] = 0.0
+    gap = thk if gap is None else gap
+    mi = m + 1
+    ii, jj = np.meshgrid(np.arange(mi), np.arange(mi), indexing="ij")
+    xu = np.zeros((mi * mi, 3))
+    nid = (ii + mi * jj).ravel(order="F")
+    off = 0.5 * (n - m) * h + 0.37 * h      # not aligned with the lower mesh
+    xu[nid, 0] = (ii * h).ravel(order="F") + off
+    xu[nid, 1] = (jj * h).ravel(order="F") + off
+    cx = off + 0.5 * m * h
+    xu[:, 2] = 1.25 * gap + np.sin(tilt) * (xu[:, 0] - cx)
+    ei, ej = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    ei, ej = ei.ravel(order="F"), ej.ravel(order="F")
+    n1 = ei + mi * ej
+    ixu = np.stack([n1, n1 + 1, n1 + 1 + mi, n1 + mi], axis=1).astype(np.int32)
+    nl = lower["x"].shape[0]
+    x = np.vstack([lower["x"], xu])
+    ixc = np.vstack([lower["ixc"], ixu + nl]).astype(np.int32)
+    ne = ixc.shape[0]
+    t = np.full(ne, thk)
+    mat = lower["mat"]
+    ms, inr, _ = shell_masses(x, ixc, t, mat["rho0"])
+    v = np.zeros_like(x)
+    rv = np.random.Generator(np.random.Philox(key=seed + 7))
+    v[nl:, 2] = v0 * (1.0 + rv.uniform(-0.05, 0.05, xu.shape[0]))
+    icodt = np.concatenate([lower["icodt"], np.zeros(xu.shape[0], np.int32)])
+    icodr = np.concatenate([lower["icodr"], np.zeros(xu.shape[0], np.int32)])
+    nsv = np.arange(nl, nl + xu.shape[0], dtype=np.int32)
+    contact = dict(nsv=nsv, irect=lower["ixc"].copy(),
+                   stfm=np.full(lower["ixc"].shape[0], 0.5 * stfac * mat["young"] * thk),
+                   stfns=np.ones(len(nsv)), gap=gap, fric=fric, visc=visc, viscf=viscf)
+    return dict(x=x, v=v, vr=np.zeros_like(x), ms=ms, **{"in": inr}, icodt=icodt,
+                icodr=icodr, ixc=ixc, uid=np.arange(1, ne + 1, dtype=np.int32), thk=t,
+                mat=mat, prop=lower["prop"], contact=contact)
+
+
+def brick_blocks_contact(nx=8, ny=8, nz=3, mx=4, my=4, mz=2, h=1.0, v0=-2.0e4, gap=0.2,
+                         fric=0.0, visc=0.05, viscf=0.0, stfac=1.0, seed=SEED_BASE):
+    """A free mx x my x mz brick block dropped on an nx x ny x nz block whose bottom
+    layer is fixed, with a /INTER/TYPE7 interface: the upper block's bottom nodes are
+    secondary, the lower block's top faces the main segments (outward normal +z).  The
+    upper block starts gap/4 outside the gap, off the lower mesh's node lines, moving at
+    Vz = v0.  Segment stiffness Stfac B A^2 / V of the brick under the face, the
+    documented TYPE7 stiffness of a solid segment (B the bulk modulus)."""
+    lower = brick_block(nx, ny, nz, h=h, jitter=0.0, seed=seed, v_top=0.0)
+    upper = brick_block(mx, my, mz, h=h, jitter=0.0, seed=seed + 1, v_top=0.0)
+    xu = upper["x"].copy()
+    off = np.array([0.5 * (nx - mx) * h + 0.31 * h, 0.5 * (ny - my) * h + 0.29 * h,
+                    nz * h + 1.25 * gap])
+    xu += off
+    nl = lower["x"].shape[0]
+    x = np.vstack([lower["x"], xu])
+    ixs = np.vstack([lower["ixs"], upper["ixs"] + nl]).astype(np.int32)
+    mat = lower["mat"]
+    vol0 = brick_volumes(x, ixs)
+    ms = nodal_mass(x.shape[0], ixs, vol0, mat["rho0"])
+    v = np.zeros_like(x)
+    v[nl:, 2] = v0
+    icodt = np.concatenate([lower["icodt"], np.zeros(xu.shape[0], np.int32)])
+    # top faces of the lower block: nodes 5-8 of its top element layer, counterclockwise
+    # seen from +z
+    top = lower["ixs"][-nx * ny:]
+    irect = top[:, [4, 5, 6, 7]].astype(np.int32)
+    area = h * h
+    stfm = stfac * mat["bulk"] * area * area / vol0[nx * ny * (nz - 1):nx * ny * nz]
+    kk = np.arange(xu.shape[0]) // ((mx + 1) * (my + 1))
+    nsv = (nl + np.nonzero(kk == 0)[0]).astype(np.int32)
+    contact = dict(nsv=nsv, irect=irect, stfm=stfm, stfns=np.ones(len(nsv)), gap=gap,
+                   fric=fric, visc=visc, viscf=viscf)
+    return dict(x=x, v=v, ms=ms, icodt=icodt, ixs=ixs,
+                uid=np.arange(1, ixs.shape[0] + 1, dtype=np.int32), vol0=vol0, mat=mat,
+                prop=lower["prop"], contact=contact)
+
+
 def square_tube(side, length, h, thk, mat, v_imp=-1.0e4, uid0=1):
     """One square thin-walled tube: cross-section side x side centred on the z axis,
     z from 0 to length; ceil(side/h) shells per face and ceil(length/h) along the

@@ -30,6 +30,7 @@ EXPORTS = (
     "orad_get_model_time", "orad_model_energies", "orad_set_launch", "orad_get_launch",
     "orad_autotune", "orad_adjoint_forward", "orad_adjoint_reverse",
     "orad_write_restart", "orad_read_restart",
+    "orad_set_contact7", "orad_contact7", "orad_get_contact7", "orad_get_contact7_sky",
 )
 NIXC = 7
 SH_NCOL = 53          # Belytschko-Tsay state values per shell
@@ -127,6 +128,10 @@ def load_library(path=LIB_PATH):
         "orad_get_launch": ([P, C.POINTER(Launch)], I),
         "orad_autotune": ([P, I, C.POINTER(Launch), C.POINTER(D)], I),
         "orad_write_restart": ([P, C.c_char_p], I),
+        "orad_set_contact7": ([P, I, P, I, P, P, P, P, I], I),
+        "orad_contact7": ([P, D, C.POINTER(D)], I),
+        "orad_get_contact7": ([P, P, P, P], I),
+        "orad_get_contact7_sky": ([P, I, P, P, C.POINTER(I)], I),
         "orad_read_restart": ([P, C.c_char_p], I),
         "orad_adjoint_forward": ([P, I, I], I),
         "orad_adjoint_reverse": ([P] + [P] * 12, I),
@@ -204,6 +209,9 @@ class BrickSolver:
         vol0 = _f64(deck["vol0"], (self.numels,))
         self._chk(self.lib.orad_set_bricks(h, self.numels, _p(ixs), _p(vol0)))
         _set_fixvel(self, deck)
+        self.nsn = 0
+        if deck.get("contact") is not None:
+            self.set_contact7(deck["contact"], deck["contact"].get("cand_cap", 0))
 
     def _chk(self, rc):
         if rc != ORAD_OK:
@@ -248,6 +256,44 @@ class BrickSolver:
     def read_restart(self, path):
         """orad_read_restart: resume from a file written by a context of the same model."""
         self._chk(self.lib.orad_read_restart(self.h, os.fsencode(path)))
+
+    def set_contact7(self, c, cand_cap=0):
+        """orad_set_contact7 from a dict: nsv (secondary nodes), irect (nrtm, 4) 0-based;
+        stfm, stfns; gap, fric, visc (VISs, default 0.05), viscf (VISf)."""
+        nsv = np.ascontiguousarray(np.asarray(c["nsv"]) + 1, np.int32)
+        irect = np.ascontiguousarray(np.asarray(c["irect"]).reshape(-1, 4) + 1, np.int32)
+        stfm = np.ascontiguousarray(c["stfm"], np.float64)
+        stfns = np.ascontiguousarray(c["stfns"], np.float64)
+        par = np.array([c["gap"], c.get("fric", 0.0), c.get("visc", 0.05), c.get("viscf", 0.0)],
+                       np.float64)
+        self._chk(self.lib.orad_set_contact7(self.h, len(nsv), _p(nsv), irect.shape[0],
+                                             _p(irect), _p(stfm), _p(stfns), _p(par),
+                                             int(cand_cap)))
+        self.nsn = len(nsv)
+
+    def contact7(self, dt1, dt2t=1.0e30):
+        """orad_contact7: one contact pass with DT1; returns the lowered DT2T."""
+        d = C.c_double(dt2t)
+        self._chk(self.lib.orad_contact7(self.h, float(dt1), C.byref(d)))
+        return d.value
+
+    def contact7_state(self):
+        """(ECONT, ECONTV, ECONTD), dict of counts, STFNS of orad_get_contact7."""
+        e = np.zeros(3)
+        k = np.zeros(4, np.int32)
+        st = np.zeros(max(self.nsn, 1))
+        self._chk(self.lib.orad_get_contact7(self.h, _p(e), _p(k), _p(st)))
+        return (tuple(e.tolist()), dict(ncand=int(k[0]), npen=int(k[1]), nsky=int(k[2]),
+                                        overflow=int(k[3])), st[:self.nsn].copy())
+
+    def contact7_sky(self):
+        """The last pass's skyline entries (n, 4) and their nodes (0-based)."""
+        n = C.c_int(0)
+        self._chk(self.lib.orad_get_contact7_sky(self.h, 0, None, None, C.byref(n)))
+        f = np.zeros((max(n.value, 1), 4))
+        i = np.zeros(max(n.value, 1), np.int32)
+        self._chk(self.lib.orad_get_contact7_sky(self.h, n.value, _p(f), _p(i), C.byref(n)))
+        return f[:n.value].copy(), i[:n.value] - 1
 
     def pon(self):
         """(ADSKY(NUMNOD+1), IADS(8,NUMELS)) 1-based, as the starter writes them."""
@@ -428,6 +474,9 @@ class ShellSolver:
         self.nmodel = 1
         if "node_off" in deck:
             self.set_models(deck["node_off"], deck["elem_off"], deck.get("mats"))
+        self.nsn = 0
+        if deck.get("contact") is not None:
+            self.set_contact7(deck["contact"], deck["contact"].get("cand_cap", 0))
 
     def set_models(self, node_off, elem_off, mats=None):
         """Declare batched independent models (orad_set_shell_models); mats: one
@@ -487,6 +536,10 @@ class ShellSolver:
     autotune = BrickSolver.autotune
     write_restart = BrickSolver.write_restart
     read_restart = BrickSolver.read_restart
+    set_contact7 = BrickSolver.set_contact7
+    contact7 = BrickSolver.contact7
+    contact7_state = BrickSolver.contact7_state
+    contact7_sky = BrickSolver.contact7_sky
     close = BrickSolver.close
     __del__ = BrickSolver.__del__
     run_cycles = BrickSolver.run_cycles

@@ -251,6 +251,35 @@ module orad_bind
       type(orad_launch_t) :: best
       real(c_double) :: best_ms
     end function
+    integer(c_int) function orad_set_contact7(ctx, nsn, nsv, nrtm, irect, stfm, stfns, params, &
+        cand_cap) bind(c, name='orad_set_contact7')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsn, nrtm, cand_cap
+      integer(c_int) :: nsv(*), irect(4, *)
+      real(c_double) :: stfm(*), stfns(*), params(4)
+    end function
+    integer(c_int) function orad_contact7(ctx, dt1, dt2t) bind(c, name='orad_contact7')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt1
+      real(c_double) :: dt2t
+    end function
+    integer(c_int) function orad_get_contact7(ctx, energies, counts, stfns) &
+        bind(c, name='orad_get_contact7')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: energies(3), stfns(*)
+      integer(c_int) :: counts(4)
+    end function
+    integer(c_int) function orad_get_contact7_sky(ctx, cap, fsky, isky, nsky) &
+        bind(c, name='orad_get_contact7_sky')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: cap
+      real(c_double) :: fsky(4, *)
+      integer(c_int) :: isky(*), nsky
+    end function
     integer(c_int) function orad_write_restart(ctx, path) bind(c, name='orad_write_restart')
       import :: c_ptr, c_int, c_char
       type(c_ptr), value :: ctx
