@@ -133,6 +133,41 @@ def shell_plate_leg(n, steps, err_cycles, ishell=1):
     return res
 
 
+def contact_plate_leg(n, m, steps, warmup):
+    """An m x m BT shell plate dropped at 20 m/s on a clamped n x n plate with one
+    /INTER/TYPE7 interface (every upper-plate node secondary, every lower-plate shell a
+    segment, GAP = thickness, VISs = 0.05, Coulomb friction 0.2 with VISf 1): mixed-form
+    cycles timed from cycle `warmup` (the plates are in contact) with and without the
+    interface; the difference is the contact pass (search, filter, forces, skyline)."""
+    deck = dk.shell_plates_contact(n=n, m=m, fric=0.2, viscf=1.0)
+    ne, nn = deck["ixc"].shape[0], deck["x"].shape[0]
+    nsn, nrtm = len(deck["contact"]["nsv"]), deck["contact"]["irect"].shape[0]
+    res = {"workload": f"BT shell plate {m}x{m} dropped on a clamped {n}x{n} plate "
+                       f"({ne} shells, {nn} nodes), TYPE7 interface of {nsn} secondary "
+                       f"nodes and {nrtm} segments",
+           "elements": ne, "nodes": nn, "secondary_nodes": nsn, "segments": nrtm,
+           "cycles_timed": steps}
+    for key, d in (("with_contact", deck), ("without_contact", dict(deck, contact=None))):
+        s = en.ShellSolver(d, en.MIXED)
+        s.run_cycles(warmup)
+        t0 = time.perf_counter()
+        s.run_cycles(steps)
+        dt = time.perf_counter() - t0
+        r = dict(value=ne * steps / dt, ms_per_step=1e3 * dt / steps)
+        if d.get("contact") is not None:
+            (ec, ev, _), cnt, _ = s.contact7_state()
+            r.update(candidates_last_cycle=cnt["ncand"], penetrating_pairs_last_cycle=cnt["npen"],
+                     skyline_entries_last_cycle=cnt["nsky"], contact_energy=ec,
+                     friction_energy=ev)
+        res[key] = r
+        s.close()
+        print(f"contact plate {key}: {r['ms_per_step']:.4f} ms/cycle", file=sys.stderr,
+              flush=True)
+    res["contact_ms_per_cycle"] = (res["with_contact"]["ms_per_step"] -
+                                   res["without_contact"]["ms_per_step"])
+    return res
+
+
 def _plate_objective(deck, A, B, cn, thk, prec, ncyc, dtmx, centre):
     """J = centre z displacement + plastic work of one forward run (no adjoint)."""
     x = torch.from_numpy(deck["x"])
@@ -515,6 +550,11 @@ def main():
     ap.add_argument("--no-qbat", action="store_true", help="skip the 512^2 QBAT plate leg")
     ap.add_argument("--qbat-steps", type=int, default=500)
     ap.add_argument("--qbat-err-cycles", type=int, default=1000)
+    ap.add_argument("--no-contact", action="store_true", help="skip the contact plate leg")
+    ap.add_argument("--contact-n", type=int, default=512)
+    ap.add_argument("--contact-m", type=int, default=256)
+    ap.add_argument("--contact-steps", type=int, default=200)
+    ap.add_argument("--contact-warmup", type=int, default=40)
     ap.add_argument("--no-tubes", action="store_true", help="skip the 256-tube batch leg")
     ap.add_argument("--tubes", type=int, default=256)
     ap.add_argument("--tube-steps", type=int, default=3000)
@@ -569,6 +609,10 @@ def main():
     qbat = None
     if not args.no_qbat:
         qbat = shell_plate_leg(args.shell_n, args.qbat_steps, args.qbat_err_cycles, ishell=12)
+    contact = None
+    if not args.no_contact:
+        contact = contact_plate_leg(args.contact_n, args.contact_m, args.contact_steps,
+                                    args.contact_warmup)
     tubes = None
     if not args.no_tubes:
         tubes = tube_batch_leg(args.tubes, args.tube_steps, args.tube_solo_cycles,
@@ -622,6 +666,7 @@ def main():
         "shell_plate": shell,
         "qeph_plate": qeph,
         "qbat_plate": qbat,
+        "contact_plate": contact,
         "tube_batch": tubes,
         "adjoint_plate": adjoint,
         "adjoint_plate_qeph": adjoint_qeph,

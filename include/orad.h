@@ -302,7 +302,7 @@ int orad_autotune(orad_ctx *ctx, int cycles, orad_launch_t *best, double *best_m
  * (IRECT(3)=IRECT(4): a triangle), their stiffness STFM(NRTM) and factor STFNS(NSN)
  * (as the starter writes INTBUF_TAB%STFM/STFNS), params = GAP, FRIC, VISs (critical
  * damping, the reference default 0.05), VISf (friction damping); cand_cap bounds the
- * candidate pairs of one pass (<= 0: 16 per secondary node).  Every cycle of
+ * penetrating pairs of one pass (<= 0: 8 per secondary node).  Every cycle of
  * orad_run_cycles then searches the candidates on the device (I7TRI's box test widened
  * by GAP), computes I7COR3/I7DST3/I7FOR3/I7ASS2 for the penetrating pairs in batches
  * of 128, lowers DT2T with the contact time step and adds each node's skyline entries
@@ -310,7 +310,8 @@ int orad_autotune(orad_ctx *ctx, int cycles, orad_launch_t *best, double *best_m
  * bit for bit on the same candidate batches; the batches are the penetrating pairs in
  * (secondary node, segment) order, where the reference batches the candidates its
  * bucket sort keeps.  A pass that outgrows cand_cap, or a secondary node with more
- * than 64 candidate segments, fails the run with ORAD_EINVAL.  Not with batched models.
+ * than 64 candidate segments or 16 penetrated ones, fails the run with ORAD_EINVAL.
+ * Not with batched models.
  *
  * orad_contact7: one pass at the current X, V with the given DT1 (the group path:
  * call it where I7MAINF runs; the next orad_asspar4 adds the entries); lowers *dt2t.

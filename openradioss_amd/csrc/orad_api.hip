@@ -117,9 +117,8 @@ static NodeArgs node_args(orad_ctx *c) {
     a.chunk_ehou = c->chunk_ehou;
   }
   if (c->ci) {
-    a.ci_cnt = c->ci->node_cnt;
-    a.ci_start = c->ci->node_bstart;
-    a.ci_bucket = c->ci->node_bucket;
+    a.ci_head = c->ci->node_head;
+    a.ci_next = c->ci->sky_next;
     a.ci_sky = c->ci->sky;
   }
   return a;
@@ -129,7 +128,7 @@ static NodeArgs node_args(orad_ctx *c) {
 // orad_contact7 call are added once, then dropped.
 static NodeArgs assembly_args(orad_ctx *c) {
   NodeArgs a = node_args(c);
-  if (!c->ci_fresh) a.ci_cnt = nullptr;
+  if (!c->ci_fresh) a.ci_head = nullptr;
   c->ci_fresh = false;
   return a;
 }
@@ -269,14 +268,14 @@ static void free_contact(orad_ctx *c) {
   Contact7 *k = c->ci;
   if (!k) return;
   for (void *p : {(void *)k->nsv, (void *)k->irect, (void *)k->stfm, (void *)k->stfns,
-                  (void *)k->bounds, (void *)k->grid, (void *)k->gdims, (void *)k->seg_cell,
-                  (void *)k->cell_count, (void *)k->cell_fill, (void *)k->cell_start,
-                  (void *)k->cell_seg, (void *)k->node_ncand, (void *)k->node_start,
-                  (void *)k->cand_n, (void *)k->cand_e, (void *)k->keep, (void *)k->rank,
-                  (void *)k->act, (void *)k->counters, (void *)k->sky, (void *)k->sky_node,
-                  (void *)k->batch_cnt, (void *)k->batch_e, (void *)k->energy,
-                  (void *)k->node_cnt, (void *)k->node_fill, (void *)k->node_bstart,
-                  (void *)k->node_bucket, (void *)k->scan_tmp})
+                  (void *)k->bounds, (void *)k->bpart, (void *)k->grid, (void *)k->gdims, (void *)k->seg_lo,
+                  (void *)k->seg_hi, (void *)k->seg_cell, (void *)k->cell_count,
+                  (void *)k->cell_fill, (void *)k->cell_start, (void *)k->cell_seg,
+                  (void *)k->node_keep, (void *)k->node_nkeep, (void *)k->node_kstart,
+                  (void *)k->counters, (void *)k->sky,
+                  (void *)k->sky_node, (void *)k->sky_next, (void *)k->batch_cnt,
+                  (void *)k->batch_e, (void *)k->energy, (void *)k->node_head,
+                  (void *)k->scan_tmp})
     dfree(p);
   delete k;
   c->ci = nullptr;
@@ -825,8 +824,11 @@ static int contact_overflow(orad_ctx *c) {
   HIPCHK(c, hipMemcpy(&ovf, c->ci->counters + I7C_OVERFLOW, sizeof(int), hipMemcpyDeviceToHost));
   if (ovf & I7_OVF_NODE)
     return fail(c, ORAD_EINVAL, "contact: a secondary node has more than 64 candidate segments");
-  if (ovf & I7_OVF_CAND)
-    return fail(c, ORAD_EINVAL, "contact: candidate capacity exceeded (orad_set_contact7 cand_cap)");
+  if (ovf & I7_OVF_KEEP)
+    return fail(c, ORAD_EINVAL, "contact: a secondary node penetrates more than 16 segments");
+  if (ovf & I7_OVF_PAIRS)
+    return fail(c, ORAD_EINVAL,
+                "contact: penetrating-pair capacity exceeded (orad_set_contact7 cand_cap)");
   return ORAD_OK;
 }
 
@@ -1759,10 +1761,12 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
     if (irect[i] < 1 || irect[i] > nn)
       return fail(c, ORAD_EINVAL, "set_contact7: IRECT out of range");
   const long long cap = cand_cap > 0 ? (long long)cand_cap
-                                     : std::max<long long>(16LL * nsn, 1024LL);
+                                     : std::max<long long>(8LL * nsn, 1024LL);
   const long long cells = std::max<long long>(2LL * nrtm, 1024LL);
   const long long lim = contact7_scan_max();
-  if (cap > lim || cells > lim || nsn > lim || nn > lim)
+  if (cap > lim || cells > lim || nsn > lim || nn > lim ||
+      (long long)nsn * I7_KEEP_CAP > (1LL << 31) - 1 ||
+      (cap + I7_NVSIZ - 1) / I7_NVSIZ * I7_SLOTS > (1LL << 31) - 1)
     return fail(c, ORAD_EINVAL, "set_contact7: interface too large for one pass");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1781,11 +1785,10 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
   k->tz = gap * 1.000001;
   contact_constants(*k);
   k->cell_cap = (int)cells;
-  k->cand_cap = (int)cap;
+  k->pair_cap = (int)cap;
   k->nbatch_cap = (int)((cap + I7_NVSIZ - 1) / I7_NVSIZ);
   const size_t nslots = (size_t)k->nbatch_cap * I7_SLOTS;
-  const size_t ntmp = contact7_scan_tmp_ints((int)std::max<long long>(
-      std::max<long long>(cap, cells), std::max<long long>(nsn, nn)));
+  const size_t ntmp = contact7_scan_tmp_ints((int)std::max<long long>(cells, nsn));
   hipError_t e = hipSuccess;
   auto al = [&](auto **p, size_t bytes) {
     if (e == hipSuccess) e = dalloc(p, bytes);
@@ -1795,30 +1798,28 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
   al(&k->stfm, (size_t)nrtm * 8);
   al(&k->stfns, (size_t)nsn * 8);
   al(&k->bounds, 8 * 8);
+  k->nbound_blocks = std::max(1, (nrtm + 255) / 256);
+  al(&k->bpart, (size_t)7 * k->nbound_blocks * 8);
   al(&k->grid, 4 * 8);
   al(&k->gdims, 4 * 4);
+  al(&k->seg_lo, (size_t)nrtm * sizeof(float4));
+  al(&k->seg_hi, (size_t)nrtm * sizeof(float4));
   al(&k->seg_cell, (size_t)nrtm * 4);
   al(&k->cell_count, (size_t)cells * 4);
   al(&k->cell_fill, (size_t)cells * 4);
   al(&k->cell_start, (size_t)(cells + 1) * 4);
   al(&k->cell_seg, (size_t)nrtm * 4);
-  al(&k->node_ncand, (size_t)nsn * 4);
-  al(&k->node_start, (size_t)(nsn + 1) * 4);
-  al(&k->cand_n, (size_t)cap * 4);
-  al(&k->cand_e, (size_t)cap * 4);
-  al(&k->keep, (size_t)cap * 4);
-  al(&k->rank, (size_t)(cap + 1) * 4);
-  al(&k->act, (size_t)cap * 4);
+  al(&k->node_keep, (size_t)nsn * I7_KEEP_CAP * 4);
+  al(&k->node_nkeep, (size_t)nsn * 4);
+  al(&k->node_kstart, (size_t)(nsn + 1) * 4);
   al(&k->counters, I7C_N * 4);
   al(&k->sky, nslots * sizeof(double4));
   al(&k->sky_node, nslots * 4);
+  al(&k->sky_next, nslots * 4);
   al(&k->batch_cnt, (size_t)k->nbatch_cap * 4);
   al(&k->batch_e, (size_t)k->nbatch_cap * 16);
   al(&k->energy, 3 * 8);
-  al(&k->node_cnt, (size_t)nn * 4);
-  al(&k->node_fill, (size_t)nn * 4);
-  al(&k->node_bstart, (size_t)(nn + 1) * 4);
-  al(&k->node_bucket, nslots * 4);
+  al(&k->node_head, (size_t)nn * 4);
   al(&k->scan_tmp, ntmp * 4);
   if (e != hipSuccess) {
     free_contact(c);
@@ -1840,8 +1841,7 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
   }
   HIPCHK(c, hipMemset(k->counters, 0, I7C_N * 4));
   HIPCHK(c, hipMemset(k->energy, 0, 3 * 8));
-  HIPCHK(c, hipMemset(k->node_cnt, 0, (size_t)nn * 4));
-  HIPCHK(c, hipMemset(k->node_bstart, 0, (size_t)(nn + 1) * 4));
+  HIPCHK(c, hipMemset(k->node_head, 0xFF, (size_t)nn * 4));      // empty lists (-1)
   HIPCHK(c, hipDeviceSynchronize());
   return ORAD_OK;
 }

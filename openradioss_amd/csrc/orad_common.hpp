@@ -185,12 +185,11 @@ struct NodeArgs {
   int nmodel;                  // batched: one extra workgroup per model sums its chunk energies
   const int32_t *model_chunks; // [nmodel+1]
   const double *chunk_ehou;
-  // /INTER/TYPE7 skyline entries (null: none): node n's entries are the entry numbers
-  // ci_bucket[ci_start[n] .. + ci_cnt[n]), added after its element slots in increasing
-  // entry order (ASSPAR4's skyline order)
-  const int32_t *ci_cnt;
-  const int32_t *ci_start;
-  const int32_t *ci_bucket;
+  // /INTER/TYPE7 skyline entries (null: none): node n's entries form the list
+  // ci_head[n], ci_next[e], ... (-1 ends it), added after its element slots in
+  // increasing entry order (ASSPAR4's skyline order)
+  const int32_t *ci_head;
+  const int32_t *ci_next;
   const double4 *ci_sky;
 };
 

@@ -149,10 +149,7 @@ __global__ __launch_bounds__(256) void i7_reset(Contact7 c) {
     c.cell_count[i] = 0;
     c.cell_fill[i] = 0;
   }
-  if (i < c.numnod) {
-    c.node_cnt[i] = 0;
-    c.node_fill[i] = 0;
-  }
+  if (i < c.numnod) c.node_head[i] = -1;
   if (i < 3) {
     c.bounds[i] = ~0ull;
     c.bounds[3 + i] = 0ull;
@@ -161,25 +158,16 @@ __global__ __launch_bounds__(256) void i7_reset(Contact7 c) {
   if (i < I7C_N && i != I7C_OVERFLOW) c.counters[i] = 0;   // the overflow flag is sticky
 }
 
-__global__ __launch_bounds__(256) void i7_seg_bounds(Contact7 c, const double4 *X4) {
-  const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
-  if (l >= c.nrtm) return;
-  double lo[3], hi[3];
-  seg_box(c, X4, l, lo, hi);
-  double ext = 0.0;
-  for (int d = 0; d < 3; ++d) {
-    const double m = 0.5 * (lo[d] + hi[d]);
-    atomicMin(&c.bounds[d], okey(m));
-    atomicMax(&c.bounds[3 + d], okey(m));
-    ext = dmax(ext, hi[d] - lo[d]);
-  }
-  atomicMax(&c.bounds[6], okey(ext));
-}
-
 // Cell size: the largest segment box edge plus 2 TZ, so the centre of every segment
 // whose widened box holds a node lies in the node's cell or a neighbour; grown until
-// the grid fits cell_cap.
-__global__ void i7_grid(Contact7 c) {
+// the grid fits cell_cap.  Every thread derives the same grid from the bounds.
+struct I7Grid {
+  double o[3], s, inv;
+  int n[3];
+};
+
+__device__ __forceinline__ I7Grid i7_grid(const Contact7 &c) {
+  I7Grid g;
   double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
   if (c.nrtm > 0)
     for (int d = 0; d < 3; ++d) {
@@ -206,23 +194,112 @@ __global__ void i7_grid(Contact7 c) {
     s = 1.0e300;
   }
   for (int d = 0; d < 3; ++d) {
-    c.grid[d] = lo[d];
-    c.gdims[d] = n[d];
+    g.o[d] = lo[d];
+    g.n[d] = n[d];
   }
-  c.grid[3] = s;
-  c.gdims[3] = n[0] * n[1] * n[2];
+  g.s = s;
+  g.inv = 1.0 / s;
+  return g;
 }
 
-__global__ __launch_bounds__(256) void i7_seg_count(Contact7 c, const double4 *X4) {
+// Segment boxes of the pass, and the bounds of their centres and their largest edge:
+// reduced over each wave by shuffles, then over the workgroup, then one atomic per bound
+// and workgroup.
+__global__ __launch_bounds__(256) void i7_seg_bounds(Contact7 c, const double4 *X4) {
+  __shared__ unsigned long long s_b[7][4];
+  const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  unsigned long long k[7] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull, 0ull};
+  if (l < c.nrtm) {
+    double lo[3], hi[3];
+    seg_box(c, X4, l, lo, hi);
+    // rounded outward: the FP32 box holds the FP64 one, so its test keeps every pair
+    // the exact test keeps; centres and edges below are the FP32 box's
+    const float4 fl = make_float4(__double2float_rd(lo[0]), __double2float_rd(lo[1]),
+                                  __double2float_rd(lo[2]), 0.0f);
+    const float4 fh = make_float4(__double2float_ru(hi[0]), __double2float_ru(hi[1]),
+                                  __double2float_ru(hi[2]), 0.0f);
+    c.seg_lo[l] = fl;
+    c.seg_hi[l] = fh;
+    lo[0] = fl.x; lo[1] = fl.y; lo[2] = fl.z;
+    hi[0] = fh.x; hi[1] = fh.y; hi[2] = fh.z;
+    double ext = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      const double m = 0.5 * (lo[d] + hi[d]);
+      k[d] = okey(m);
+      k[3 + d] = k[d];
+      ext = dmax(ext, hi[d] - lo[d]);
+    }
+    k[6] = okey(ext);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      const unsigned long long b = __shfl_xor(k[j], o, 64);
+      k[j] = j < 3 ? (b < k[j] ? b : k[j]) : (b > k[j] ? b : k[j]);
+    }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int j = 0; j < 7; ++j) s_b[j][w] = k[j];
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    const int j = threadIdx.x;
+    unsigned long long v = s_b[j][0];
+    for (int q = 1; q < 4; ++q) v = j < 3 ? (s_b[j][q] < v ? s_b[j][q] : v)
+                                          : (s_b[j][q] > v ? s_b[j][q] : v);
+    c.bpart[(size_t)j * c.nbound_blocks + blockIdx.x] = v;
+  }
+}
+
+// The workgroups' bounds reduced by one workgroup, then the grid of the pass.
+__global__ __launch_bounds__(1024) void i7_bounds_grid(Contact7 c) {
+  __shared__ unsigned long long s_b[7][16];
+  unsigned long long k[7] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull, 0ull};
+  for (int b = threadIdx.x; b < c.nbound_blocks; b += 1024)
+    for (int j = 0; j < 7; ++j) {
+      const unsigned long long v = c.bpart[(size_t)j * c.nbound_blocks + b];
+      k[j] = j < 3 ? (v < k[j] ? v : k[j]) : (v > k[j] ? v : k[j]);
+    }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      const unsigned long long b = __shfl_xor(k[j], o, 64);
+      k[j] = j < 3 ? (b < k[j] ? b : k[j]) : (b > k[j] ? b : k[j]);
+    }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int j = 0; j < 7; ++j) s_b[j][w] = k[j];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < 7; ++j) {
+      unsigned long long v = s_b[j][0];
+      for (int q = 1; q < 16; ++q) v = j < 3 ? (s_b[j][q] < v ? s_b[j][q] : v)
+                                             : (s_b[j][q] > v ? s_b[j][q] : v);
+      c.bounds[j] = v;
+    }
+    const I7Grid g = i7_grid(c);
+    for (int d = 0; d < 3; ++d) {
+      c.grid[d] = g.o[d];
+      c.gdims[d] = g.n[d];
+    }
+    c.grid[3] = g.s;
+    c.gdims[3] = g.n[0] * g.n[1] * g.n[2];
+  }
+}
+
+__global__ __launch_bounds__(256) void i7_seg_count(Contact7 c) {
   const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
   if (l >= c.nrtm) return;
-  double lo[3], hi[3];
-  seg_box(c, X4, l, lo, hi);
   const double inv = 1.0 / c.grid[3];
+  const float4 lo = c.seg_lo[l], hi = c.seg_hi[l];
+  const double m[3] = {0.5 * ((double)lo.x + (double)hi.x), 0.5 * ((double)lo.y + (double)hi.y),
+                       0.5 * ((double)lo.z + (double)hi.z)};
   int k[3];
   for (int d = 0; d < 3; ++d) {
-    int q = cell_axis(0.5 * (lo[d] + hi[d]), c.grid[d], inv, c.gdims[d]);
-    k[d] = q < 0 ? 0 : (q >= c.gdims[d] ? c.gdims[d] - 1 : q);
+    const int n = c.gdims[d];
+    const int q = cell_axis(m[d], c.grid[d], inv, n);
+    k[d] = q < 0 ? 0 : (q >= n ? n - 1 : q);
   }
   const int cell = k[0] + c.gdims[0] * (k[1] + c.gdims[1] * k[2]);
   c.seg_cell[l] = cell;
@@ -236,81 +313,6 @@ __global__ __launch_bounds__(256) void i7_seg_place(Contact7 c) {
   c.cell_seg[c.cell_start[cell] + atomicAdd(&c.cell_fill[cell], 1)] = l;
 }
 
-// I7TRI's test (i7tri.F:530-566) for node nn and segment l, box widened by TZ.
-__device__ __forceinline__ bool i7_box(const Contact7 &c, const double4 *X4, int nn,
-                                       const double4 &x, int l) {
-  if (c.stfm[l] == 0.0) return false;
-  const int4 r = c.irect[l];
-  if (nn == r.x || nn == r.y || nn == r.z || nn == r.w) return false;
-  double lo[3], hi[3];
-  seg_box(c, X4, l, lo, hi);
-  return x.x > lo[0] - c.tz && x.x < hi[0] + c.tz && x.y > lo[1] - c.tz &&
-         x.y < hi[1] + c.tz && x.z > lo[2] - c.tz && x.z < hi[2] + c.tz;
-}
-
-// Each secondary node's candidate segments, in increasing segment order.
-template <bool WRITE>
-__global__ __launch_bounds__(128) void i7_node_cand(Contact7 c, const double4 *X4) {
-  const int i = (int)blockIdx.x * 128 + (int)threadIdx.x;
-  if (i >= c.nsn) return;
-  int list[I7_NODE_CAP];
-  int cnt = 0;
-  if (c.stfns[i] != 0.0) {
-    const int nn = c.nsv[i];
-    const double4 x = X4[nn];
-    const double inv = 1.0 / c.grid[3];
-    const int kx = cell_axis(x.x, c.grid[0], inv, c.gdims[0]);
-    const int ky = cell_axis(x.y, c.grid[1], inv, c.gdims[1]);
-    const int kz = cell_axis(x.z, c.grid[2], inv, c.gdims[2]);
-    for (int z = kz - 1; z <= kz + 1; ++z) {
-      if (z < 0 || z >= c.gdims[2]) continue;
-      for (int y = ky - 1; y <= ky + 1; ++y) {
-        if (y < 0 || y >= c.gdims[1]) continue;
-        for (int xx = kx - 1; xx <= kx + 1; ++xx) {
-          if (xx < 0 || xx >= c.gdims[0]) continue;
-          const int cell = xx + c.gdims[0] * (y + c.gdims[1] * z);
-          for (int j = c.cell_start[cell]; j < c.cell_start[cell + 1]; ++j) {
-            const int l = c.cell_seg[j];
-            if (!i7_box(c, X4, nn, x, l)) continue;
-            if (WRITE) {
-              // insertion in increasing segment order
-              if (cnt < I7_NODE_CAP) {
-                int p = cnt;
-                while (p > 0 && list[p - 1] > l) {
-                  list[p] = list[p - 1];
-                  --p;
-                }
-                list[p] = l;
-              }
-            }
-            ++cnt;
-          }
-        }
-      }
-    }
-  }
-  if (cnt > I7_NODE_CAP) {
-    if (!WRITE) atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_NODE);
-    cnt = I7_NODE_CAP;
-  }
-  if (!WRITE) {
-    c.node_ncand[i] = cnt;
-    return;
-  }
-  const int s = c.node_start[i];
-  for (int k = 0; k < cnt; ++k) {
-    if (s + k >= c.cand_cap) break;
-    c.cand_n[s + k] = i;
-    c.cand_e[s + k] = list[k];
-  }
-}
-
-__global__ void i7_cand_total(Contact7 c) {
-  const int tot = c.node_start[c.nsn];
-  if (tot > c.cand_cap) atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_CAND);
-  c.counters[I7C_NCAND] = tot > c.cand_cap ? c.cand_cap : tot;
-}
-
 // ------------------------------------------------------------- per pair --
 struct I7Pair {
   int ix[4], nsvg;
@@ -320,7 +322,7 @@ struct I7Pair {
 };
 
 // I7COR3 + I7DST3 for candidate (ni, l); true when the pair penetrates (KEEP).
-__device__ __noinline__ bool i7_dist(const Contact7 &c, const double4 *X4, const double4 *V4,
+__device__ __forceinline__ bool i7_dist(const Contact7 &c, const double4 *X4, const double4 *V4,
                                      int ni, int l, I7Pair &q) {
   q.gapv = c.gap;
   const int ig = c.nsv[ni];
@@ -409,19 +411,79 @@ __device__ __noinline__ bool i7_dist(const Contact7 &c, const double4 *X4, const
   return pene2 != 0.0 && q.stif != 0.0;
 }
 
-__global__ __launch_bounds__(256) void i7_keep(Contact7 c, const double4 *X4, const double4 *V4) {
-  const int k = (int)blockIdx.x * 256 + (int)threadIdx.x;
-  if (k >= c.counters[I7C_NCAND]) return;
-  I7Pair q;
-  c.keep[k] = i7_dist(c, X4, V4, c.cand_n[k], c.cand_e[k], q) ? 1 : 0;
+// Each secondary node: its candidate segments (I7TRI's box test, i7tri.F:530-566,
+// widened by TZ) in increasing segment order, and among them the ones it penetrates
+// (I7COR3 + I7DST3), kept in that order.
+__global__ __launch_bounds__(128) void i7_node_pass(Contact7 c, const double4 *X4,
+                                                   const double4 *V4) {
+  __shared__ int s_list[I7_NODE_CAP][128];   // [k][lane]: the lane's sorted candidates
+  const int i = (int)blockIdx.x * 128 + (int)threadIdx.x;
+  if (i >= c.nsn) return;
+  int *list = &s_list[0][threadIdx.x];
+  int cnt = 0, nkeep = 0;
+  if (c.stfns[i] != 0.0) {
+    const int nn = c.nsv[i];
+    const double4 x = X4[nn];
+    const double o[3] = {c.grid[0], c.grid[1], c.grid[2]};
+    const double inv = 1.0 / c.grid[3];
+    const int n0 = c.gdims[0], n1 = c.gdims[1], n2 = c.gdims[2];
+    const int kx = cell_axis(x.x, o[0], inv, n0);
+    const int ky = cell_axis(x.y, o[1], inv, n1);
+    const int kz = cell_axis(x.z, o[2], inv, n2);
+    for (int z = kz - 1; z <= kz + 1; ++z) {
+      if (z < 0 || z >= n2) continue;
+      for (int y = ky - 1; y <= ky + 1; ++y) {
+        if (y < 0 || y >= n1) continue;
+        for (int xx = kx - 1; xx <= kx + 1; ++xx) {
+          if (xx < 0 || xx >= n0) continue;
+          const int cell = xx + n0 * (y + n1 * z);
+          for (int j = c.cell_start[cell]; j < c.cell_start[cell + 1]; ++j) {
+            const int l = c.cell_seg[j];
+            if (c.stfm[l] == 0.0) continue;
+            const float4 lo = c.seg_lo[l], hi = c.seg_hi[l];
+            if (!(x.x > (double)lo.x - c.tz && x.x < (double)hi.x + c.tz &&
+                  x.y > (double)lo.y - c.tz && x.y < (double)hi.y + c.tz &&
+                  x.z > (double)lo.z - c.tz && x.z < (double)hi.z + c.tz))
+              continue;
+            const int4 r = c.irect[l];
+            if (nn == r.x || nn == r.y || nn == r.z || nn == r.w) continue;
+            if (cnt < I7_NODE_CAP) {
+              int p = cnt;
+              while (p > 0 && list[(p - 1) * 128] > l) {
+                list[p * 128] = list[(p - 1) * 128];
+                --p;
+              }
+              list[p * 128] = l;
+            }
+            ++cnt;
+          }
+        }
+      }
+    }
+    if (cnt > I7_NODE_CAP) {
+      atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_NODE);
+      cnt = I7_NODE_CAP;
+    }
+    for (int k = 0; k < cnt; ++k) {
+      I7Pair q;
+      const int l = list[k * 128];
+      if (!i7_dist(c, X4, V4, i, l, q)) continue;
+      if (nkeep < I7_KEEP_CAP) c.node_keep[(size_t)i * I7_KEEP_CAP + nkeep] = l;
+      ++nkeep;
+    }
+    if (nkeep > I7_KEEP_CAP) {
+      atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_KEEP);
+      nkeep = I7_KEEP_CAP;
+    }
+    if (cnt) atomicAdd(&c.counters[I7C_NCAND], cnt);
+  }
+  c.node_nkeep[i] = nkeep;
 }
 
-__global__ __launch_bounds__(256) void i7_compact(Contact7 c) {
-  const int n = c.counters[I7C_NCAND];
-  const int k = (int)blockIdx.x * 256 + (int)threadIdx.x;
-  if (k == 0) c.counters[I7C_NPEN] = c.rank[n];
-  if (k >= n) return;
-  if (c.keep[k]) c.act[c.rank[k]] = k;
+// The number of penetrating pairs of the pass (the scan's total, capped).
+__device__ __forceinline__ int i7_npen(const Contact7 &c) {
+  const int tot = c.node_kstart[c.nsn];
+  return tot > c.pair_cap ? c.pair_cap : tot;
 }
 
 // I7FOR3 + I7ASS2 for one batch of NVSIZ penetrating pairs, one lane per pair.
@@ -430,8 +492,12 @@ __global__ __launch_bounds__(I7_NVSIZ) void i7_batch(Contact7 c, const double4 *
                                                      unsigned long long *dt2t) {
   __shared__ double s_rd[I7_NVSIZ], s_ec[I7_NVSIZ], s_ev[I7_NVSIZ];
   __shared__ int s_cnt[5][2];
-  const int npen = c.counters[I7C_NPEN];
+  const int npen = i7_npen(c);
   const int b = (int)blockIdx.x;
+  if (b == 0 && threadIdx.x == 0) {
+    if (c.node_kstart[c.nsn] > c.pair_cap) atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_PAIRS);
+    c.counters[I7C_NPEN] = npen;
+  }
   if (b * I7_NVSIZ >= npen) return;
   const int t = (int)threadIdx.x;
   const int r = b * I7_NVSIZ + t;
@@ -443,10 +509,17 @@ __global__ __launch_bounds__(I7_NVSIZ) void i7_batch(Contact7 c, const double4 *
   double rdist = 1.0e300, econtt = 0.0, econvt = 0.0;
   int ix[4] = {0, 0, 0, 0}, nsvg = 0;
   if (live) {
-    const int k = c.act[r];
-    const int ni = c.cand_n[k];
+    // the secondary node of pair r: the last i with node_kstart[i] <= r
+    int lo = 0, hi = c.nsn - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (c.node_kstart[mid] <= r) lo = mid;
+      else hi = mid - 1;
+    }
+    const int ni = lo;
+    const int l = c.node_keep[(size_t)ni * I7_KEEP_CAP + (r - c.node_kstart[ni])];
     I7Pair q;
-    i7_dist(c, X4, V4, ni, c.cand_e[k], q);
+    i7_dist(c, X4, V4, ni, l, q);
     for (int j = 0; j < 4; ++j) ix[j] = q.ix[j];
     nsvg = q.nsvg;
     stif = q.stif;
@@ -563,84 +636,64 @@ __global__ __launch_bounds__(I7_NVSIZ) void i7_batch(Contact7 c, const double4 *
     int tot = 0;
     for (int j = 0; j < 4; ++j) tot += s_cnt[j][0] + s_cnt[j][1];
     c.batch_cnt[b] = tot + nlive;
+    // the last batch to finish adds the batches' energies in batch order
+    // (ECONT = ECONT + ECONTT per I7FOR3 call)
+    __threadfence();
+    const int nb = (npen + I7_NVSIZ - 1) / I7_NVSIZ;
+    if (atomicAdd(&c.counters[I7C_DONE], 1) == nb - 1) {
+      __threadfence();
+      double et = c.energy[0], vt = c.energy[1];
+      int nsky = 0;
+      for (int q = 0; q < nb; ++q) {
+        et = et + __hip_atomic_load(&c.batch_e[2 * q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        vt = vt + __hip_atomic_load(&c.batch_e[2 * q + 1], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+        nsky += __hip_atomic_load(&c.batch_cnt[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      c.energy[0] = et;
+      c.energy[1] = vt;
+      c.counters[I7C_NSKY] = nsky;
+    }
   }
   if (!live) return;
-  double4 *sky = c.sky + (size_t)b * I7_SLOTS;
-  int32_t *node = c.sky_node + (size_t)b * I7_SLOTS;
+  const int e0 = b * I7_SLOTS;
   int base = 0;
   for (int j = 0; j < 4; ++j) {
     if (h[j] != 0.0) {
-      const int pos = base + (w ? s_cnt[j][0] : 0) + __popcll(m[j] & lt);
-      sky[pos] = make_double4(fxi[0] * h[j], fxi[1] * h[j], fxi[2] * h[j], stif * fabs(h[j]));
-      node[pos] = ix[j];
+      const int e = e0 + base + (w ? s_cnt[j][0] : 0) + __popcll(m[j] & lt);
+      c.sky[e] = make_double4(fxi[0] * h[j], fxi[1] * h[j], fxi[2] * h[j], stif * fabs(h[j]));
+      c.sky_node[e] = ix[j];
+      c.sky_next[e] = atomicExch(&c.node_head[ix[j]], e);
     }
     base += s_cnt[j][0] + s_cnt[j][1];
   }
-  sky[base + t] = make_double4(-fxi[0], -fxi[1], -fxi[2], stif);
-  node[base + t] = nsvg;
-}
-
-// ------------------------------------------------------ per-node buckets --
-__global__ __launch_bounds__(256) void i7_bucket(Contact7 c, int place) {
-  const int e = (int)blockIdx.x * 256 + (int)threadIdx.x;
-  const int npen = c.counters[I7C_NPEN];
-  const int b = e / I7_SLOTS;
-  if (b * I7_NVSIZ >= npen) return;
-  if (e - b * I7_SLOTS >= c.batch_cnt[b]) return;
-  const int nd = c.sky_node[e];
-  if (!place) {
-    atomicAdd(&c.node_cnt[nd], 1);
-    return;
-  }
-  c.node_bucket[c.node_bstart[nd] + atomicAdd(&c.node_fill[nd], 1)] = e;
-}
-
-// Cumulative energies, batch by batch (ECONT = ECONT + ECONTT per I7FOR3 call).
-__global__ void i7_energy(Contact7 c) {
-  const int npen = c.counters[I7C_NPEN];
-  const int nb = (npen + I7_NVSIZ - 1) / I7_NVSIZ;
-  double ec = c.energy[0], ev = c.energy[1];
-  int nsky = 0;
-  for (int b = 0; b < nb; ++b) {
-    ec = ec + c.batch_e[2 * b];
-    ev = ev + c.batch_e[2 * b + 1];
-    nsky += c.batch_cnt[b];
-  }
-  c.energy[0] = ec;
-  c.energy[1] = ev;
-  c.counters[I7C_NSKY] = nsky;
+  const int e = e0 + base + t;
+  c.sky[e] = make_double4(-fxi[0], -fxi[1], -fxi[2], stif);
+  c.sky_node[e] = nsvg;
+  c.sky_next[e] = atomicExch(&c.node_head[nsvg], e);
 }
 
 hipError_t contact7_run(const Contact7 &c, const double4 *X4, const double4 *V4,
                         const double *dt1, unsigned long long *dt2t, hipStream_t s) {
-  auto g = [](long n, int blk) { return dim3((unsigned)((n + blk - 1) / blk > 0 ? (n + blk - 1) / blk : 1)); };
+  auto g = [](long n, int blk) {
+    const long b = (n + blk - 1) / blk;
+    return dim3((unsigned)(b > 0 ? b : 1));
+  };
   long nreset = c.cell_cap;
   if (c.numnod > nreset) nreset = c.numnod;
   if (nreset < 8) nreset = 8;
   hipLaunchKernelGGL(i7_reset, g(nreset, 256), dim3(256), 0, s, c);
-  hipLaunchKernelGGL(i7_seg_bounds, g(c.nrtm, 256), dim3(256), 0, s, c, X4);
-  hipLaunchKernelGGL(i7_grid, dim3(1), dim3(1), 0, s, c);
-  hipLaunchKernelGGL(i7_seg_count, g(c.nrtm, 256), dim3(256), 0, s, c, X4);
+  hipLaunchKernelGGL(i7_seg_bounds, dim3(c.nbound_blocks), dim3(256), 0, s, c, X4);
+  hipLaunchKernelGGL(i7_bounds_grid, dim3(1), dim3(1024), 0, s, c);
+  hipLaunchKernelGGL(i7_seg_count, g(c.nrtm, 256), dim3(256), 0, s, c);
   hipError_t e = scan_excl(c.cell_count, c.cell_start, c.gdims + 3, 0, c.cell_cap, c.scan_tmp, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(i7_seg_place, g(c.nrtm, 256), dim3(256), 0, s, c);
-  hipLaunchKernelGGL(i7_node_cand<false>, g(c.nsn, 128), dim3(128), 0, s, c, X4);
-  e = scan_excl(c.node_ncand, c.node_start, nullptr, c.nsn, c.nsn, c.scan_tmp, s);
+  hipLaunchKernelGGL(i7_node_pass, g(c.nsn, 128), dim3(128), 0, s, c, X4, V4);
+  e = scan_excl(c.node_nkeep, c.node_kstart, nullptr, c.nsn, c.nsn, c.scan_tmp, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(i7_node_cand<true>, g(c.nsn, 128), dim3(128), 0, s, c, X4);
-  hipLaunchKernelGGL(i7_cand_total, dim3(1), dim3(1), 0, s, c);
-  hipLaunchKernelGGL(i7_keep, g(c.cand_cap, 256), dim3(256), 0, s, c, X4, V4);
-  e = scan_excl(c.keep, c.rank, c.counters + I7C_NCAND, 0, c.cand_cap, c.scan_tmp, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(i7_compact, g(c.cand_cap, 256), dim3(256), 0, s, c);
   hipLaunchKernelGGL(i7_batch, dim3(c.nbatch_cap > 0 ? c.nbatch_cap : 1), dim3(I7_NVSIZ), 0, s,
                      c, X4, V4, dt1, dt2t);
-  const long nslots = (long)c.nbatch_cap * I7_SLOTS;
-  hipLaunchKernelGGL(i7_bucket, g(nslots, 256), dim3(256), 0, s, c, 0);
-  e = scan_excl(c.node_cnt, c.node_bstart, nullptr, c.numnod, c.numnod, c.scan_tmp, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(i7_bucket, g(nslots, 256), dim3(256), 0, s, c, 1);
-  hipLaunchKernelGGL(i7_energy, dim3(1), dim3(1), 0, s, c);
   return hipGetLastError();
 }
 

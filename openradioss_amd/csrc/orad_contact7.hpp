@@ -7,18 +7,18 @@
 // its captured graph:
 //   search      segments binned on a uniform grid by their box centre; each secondary
 //               node lists the segments whose box, widened by TZ, holds it (I7TRI's
-//               test), in increasing segment order, so the candidate list is ordered
-//               by (secondary node, segment) without a global sort;
-//   filter      I7COR3 + I7DST3 per candidate; penetrating pairs compacted in order;
+//               test) in increasing segment order and keeps those it penetrates
+//               (I7COR3 + I7DST3), so the penetrating pairs come out in (secondary
+//               node, segment) order without a global sort;
 //   forces      one 128-lane workgroup per batch of NVSIZ = 128 penetrating pairs:
 //               I7FOR3 and I7ASS2's skyline layout (main node 1 of every pair, then
 //               nodes 2, 3, 4 where H /= 0, then the secondary nodes) in the batch's
 //               640-entry region; the batch's contact time step lowers DT2T;
-//   assembly    each node's entries gathered into a bucket; the nodal kernel adds
-//               them after the node's element slots in increasing entry order, i.e.
+//   assembly    each entry is pushed on its node's list; the nodal kernel adds a node's
+//               entries after its element slots in increasing entry order, i.e.
 //               ASSPAR4's skyline order.
-// Every order is fixed (atomics only count and place; buckets are read sorted), so a
-// run is bitwise reproducible and equal to the FP64 oracle's.
+// Every order is fixed (atomics only count, place and link; lists are read sorted), so
+// a run is bitwise reproducible and equal to the FP64 oracle's.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -28,12 +28,13 @@ namespace orad {
 
 constexpr int I7_NVSIZ = 128;             // pairs per batch (the reference's NVSIZ)
 constexpr int I7_SLOTS = 5 * I7_NVSIZ;    // skyline entries one batch can write
-constexpr int I7_NODE_CAP = 64;           // candidates of one secondary node
+constexpr int I7_NODE_CAP = 64;           // candidate segments of one secondary node
+constexpr int I7_KEEP_CAP = 16;           // penetrated segments of one secondary node
 
 // counters[] of a pass
-enum { I7C_NCAND = 0, I7C_NPEN = 1, I7C_OVERFLOW = 2, I7C_NSKY = 3, I7C_N = 4 };
-// overflow bits
-enum { I7_OVF_NODE = 1, I7_OVF_CAND = 2 };
+enum { I7C_NCAND = 0, I7C_NPEN = 1, I7C_OVERFLOW = 2, I7C_NSKY = 3, I7C_DONE = 4, I7C_N = 5 };
+// overflow bits (sticky until the interface is set again)
+enum { I7_OVF_NODE = 1, I7_OVF_PAIRS = 2, I7_OVF_KEEP = 4 };
 
 struct Contact7 {
   int nsn = 0, nrtm = 0, numnod = 0;
@@ -46,35 +47,35 @@ struct Contact7 {
   // search grid over the segments' box centres
   int cell_cap = 0;
   unsigned long long *bounds = nullptr;  // [7] ordered bits: min xyz, max xyz, max extent
+  unsigned long long *bpart = nullptr;   // [7 * nbound_blocks] the workgroups' bounds
+  int nbound_blocks = 0;
   double *grid = nullptr;                // [4] origin xyz, cell size
   int *gdims = nullptr;                  // [4] cells per axis, total
+  float4 *seg_lo = nullptr, *seg_hi = nullptr;   // [nrtm] segment boxes of the pass,
+                                                  // rounded outward to FP32
   int32_t *seg_cell = nullptr;           // [nrtm]
   int32_t *cell_count = nullptr, *cell_fill = nullptr;   // [cell_cap]
   int32_t *cell_start = nullptr;         // [cell_cap + 1]
   int32_t *cell_seg = nullptr;           // [nrtm]
-  // candidates (secondary index into nsv, segment), then the penetrating ones
-  int cand_cap = 0, nbatch_cap = 0;
-  int32_t *node_ncand = nullptr;         // [nsn]
-  int32_t *node_start = nullptr;         // [nsn + 1]
-  int32_t *cand_n = nullptr, *cand_e = nullptr;   // [cand_cap]
-  int32_t *keep = nullptr;               // [cand_cap]
-  int32_t *rank = nullptr;               // [cand_cap + 1]
-  int32_t *act = nullptr;                // [cand_cap]
+  // penetrating pairs: per secondary node its penetrated segments in increasing order;
+  // pair r of the pass is node i's (r - node_kstart[i])-th, node_kstart[i] <= r
+  int pair_cap = 0, nbatch_cap = 0;
+  int32_t *node_keep = nullptr;          // [nsn * I7_KEEP_CAP]
+  int32_t *node_nkeep = nullptr;         // [nsn]
+  int32_t *node_kstart = nullptr;        // [nsn + 1]
   int *counters = nullptr;               // [I7C_N]
   // skyline: batch b owns entries [b * I7_SLOTS, b * I7_SLOTS + batch_cnt[b])
   double4 *sky = nullptr;                // (fx, fy, fz, stif)
   int32_t *sky_node = nullptr;
+  int32_t *sky_next = nullptr;           // per entry: the next entry of its node's list
   int32_t *batch_cnt = nullptr;          // [nbatch_cap]
   double *batch_e = nullptr;             // [2 * nbatch_cap] elastic, friction energy
   double *energy = nullptr;              // [3] cumulative ECONT, ECONTV, ECONTD
-  // per-node buckets of entry numbers (ASSPAR4's sort of FSKYI by node)
-  int32_t *node_cnt = nullptr, *node_fill = nullptr;     // [numnod]
-  int32_t *node_bstart = nullptr;        // [numnod + 1]
-  int32_t *node_bucket = nullptr;        // [nbatch_cap * I7_SLOTS]
+  int32_t *node_head = nullptr;          // [numnod] first entry of each node's list, -1
   int32_t *scan_tmp = nullptr;           // block sums of the scans
 };
 
-// One contact pass: search, filter, forces, buckets.  dt1 is read on the device
+// One contact pass: search, filter, forces, lists.  dt1 is read on the device
 // (DT1 of the cycle); the batches' contact time steps lower *dt2t (bits of a positive
 // double, atomicMin).
 hipError_t contact7_run(const Contact7 &ci, const double4 *X4, const double4 *V4,

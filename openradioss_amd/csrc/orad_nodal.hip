@@ -145,22 +145,16 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
   }
   double ax = acc[0], ay = acc[1], az = acc[2];
   // /INTER/TYPE7: the node's skyline entries in increasing entry order
-  // (asspar4.F:560-583), selected one by one from its bucket
-  int nci = 0;
-  if (a.ci_cnt && live) nci = a.ci_cnt[n];
-  if (nci > 0) {
-    const int32_t *bk = a.ci_bucket + a.ci_start[n];
-    int last = -1;
-    for (int j = 0; j < nci; ++j) {
-      int e = 0x7FFFFFFF;
-      for (int q = 0; q < nci; ++q) {
-        const int v = bk[q];
-        if (v > last && v < e) e = v;
-      }
-      last = e;
-      const double4 f = a.ci_sky[e];
-      ax = ax + f.x; ay = ay + f.y; az = az + f.z;
-    }
+  // (asspar4.F:560-583), selected one by one from its list
+  const int head = (a.ci_head && live) ? a.ci_head[n] : -1;
+  for (int last = -1;;) {
+    int e = 0x7FFFFFFF;
+    for (int q = head; q >= 0; q = a.ci_next[q])
+      if (q > last && q < e) e = q;
+    if (e == 0x7FFFFFFF) break;
+    last = e;
+    const double4 f = a.ci_sky[e];
+    ax = ax + f.x; ay = ay + f.y; az = az + f.z;
   }
   if (mode == 0) {
     if (!live) return;
@@ -169,18 +163,13 @@ __global__ __launch_bounds__(NODE_BLOCK) void node_kernel(NodeArgs a, int parity
     double sf = 0.0;
     const R *sti = reinterpret_cast<const R *>(a.sti);
     for (int k = k0; k < k1; ++k) sf = sf + double(sti[a.slot_elem[k]]);
-    if (nci > 0) {
-      const int32_t *bk = a.ci_bucket + a.ci_start[n];
-      int last = -1;
-      for (int j = 0; j < nci; ++j) {
-        int e = 0x7FFFFFFF;
-        for (int q = 0; q < nci; ++q) {
-          const int v = bk[q];
-          if (v > last && v < e) e = v;
-        }
-        last = e;
-        sf = sf + a.ci_sky[e].w;
-      }
+    for (int last = -1;;) {
+      int e = 0x7FFFFFFF;
+      for (int q = head; q >= 0; q = a.ci_next[q])
+        if (q > last && q < e) e = q;
+      if (e == 0x7FFFFFFF) break;
+      last = e;
+      sf = sf + a.ci_sky[e].w;
     }
     a.A[3 * n] = ax; a.A[3 * n + 1] = ay; a.A[3 * n + 2] = az;
     a.STIFN[n] = sf;
