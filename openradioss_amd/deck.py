@@ -336,6 +336,24 @@ def square_tube(side, length, h, thk, mat, v_imp=-1.0e4, uid0=1):
                 fv_dir=fv_dir, fv_val=fv_val)
 
 
+def tube_self_contact(side=20.0, length=40.0, h=2.0, thk=1.2, v_imp=-5.0e4, gap=None, fric=0.2,
+                      visc=0.05, viscf=1.0, stfac=1.0, npt=5, ipla=0):
+    """One square tube crushed axially (square_tube: clamped base ring, the top ring driven
+    at v_imp) with a /INTER/TYPE7 self-contact interface: every node secondary, every shell
+    a segment (a node is never a candidate of a segment it belongs to).  Segment stiffness
+    0.5 Stfac E t; gap 0.6 t unless given."""
+    mat = shell_mat_params(ipla=ipla)
+    d = square_tube(side, length, h, thk, mat, v_imp=v_imp)
+    d["mat"] = mat
+    d["prop"] = shell_prop_params(npt=npt)
+    nn, ne = d["x"].shape[0], d["ixc"].shape[0]
+    d["contact"] = dict(nsv=np.arange(nn, dtype=np.int32), irect=d["ixc"].copy(),
+                        stfm=np.full(ne, 0.5 * stfac * mat["young"] * thk),
+                        stfns=np.ones(nn), gap=0.6 * thk if gap is None else gap, fric=fric,
+                        visc=visc, viscf=viscf)
+    return d
+
+
 def shell_tubes(ntube=256, side=(30.0, 80.0), length=(100.0, 400.0), h=(2.0, 5.0),
                 thk=(0.8, 2.5), ca=(200.0, 400.0), v_imp=-1.0e4, seed=20261203, npt=5, ipla=0):
     """Batch of independent square tubes crushed axially (one model each): side,

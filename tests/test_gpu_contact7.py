@@ -10,6 +10,9 @@ Tolerances:
     clamped plate, a brick block dropped on a fixed block): 1e-8 relative on nodes and
     energies after the impact and rebound, time within 1e-12;
   * mixed form vs FP64 form on the shell drop: displacement relative L2 < 1e-4;
+  * a tube crushed with self-contact: elastic, bit for bit the FP64 oracle's run (the
+    element arithmetic is the same on both sides); plastic, within 1e-5 (the law's
+    exp/pow differ in the last bit and the contact damping switches on at a finite value);
   * run-to-run and restart: bitwise.
 """
 import os
@@ -184,3 +187,56 @@ def test_contact_capacity_overflow_is_reported():
             s.run_cycles(200)
     finally:
         s.close()
+
+
+def _tube_run(d, chunks, every):
+    m, s = po.ShellModel(d, nthreads=8), en.ShellSolver(d, en.FP64)
+    worst, peak = 0.0, 0
+    try:
+        for _ in range(chunks):
+            m.cycle(every)
+            s.run_cycles(every)
+            _, cnt, _ = s.contact7_state()
+            assert cnt["npen"] == m.contact.it.npen
+            peak = max(peak, m.contact.it.npen)
+            worst = max(worst, rel(s.nodes()["d"], m.d))
+        nd = s.nodes()
+        (ec, ev, _), _, _ = s.contact7_state()
+        t = s.time()
+    finally:
+        s.close()
+    return m, nd, ec, ev, t, worst, peak
+
+
+def test_tube_crush_self_contact_elastic_bitwise():
+    """A square tube crushed at 200 m/s with a self-contact interface (every node against
+    every shell of the tube), elastic so that the element arithmetic of the device and the
+    oracle is the same: the whole run, through up to 160 penetrating pairs, is bit for bit
+    the FP64 oracle's."""
+    d = dk.tube_self_contact(v_imp=-2.0e5)
+    d["mat"] = dict(d["mat"], ca=1.0e7)          # the yield stress is never reached
+    m, nd, ec, ev, t, worst, peak = _tube_run(d, 12, 50)
+    assert peak > 50
+    assert worst == 0.0
+    assert np.array_equal(nd["x"], m.x) and np.array_equal(nd["v"], m.v)
+    assert np.array_equal(nd["vr"], m.vr)
+    assert t.tt == m.m.tt and t.dt2 == m.m.dt2
+    # the energies' LOG is the device's
+    assert ec == pytest.approx(m.contact.it.econt, rel=1e-13)
+    assert ev == pytest.approx(m.contact.it.econtv, rel=1e-13, abs=1e-300)
+
+
+def test_tube_crush_self_contact_plastic_vs_oracle():
+    """The same crush at 50 m/s with law-2 plasticity: the folds meet from about cycle 500.
+    The device's exp/pow in the law differ from the host's in the last bit, and TYPE7's
+    damping switches on at a finite value when a node enters the gap, so a pair at the
+    threshold can start one cycle apart: the runs stay within 1e-5 of each other (the same
+    penetrating pairs every 100 cycles), where without contact they agree to 1e-13."""
+    d = dk.tube_self_contact()
+    m, nd, ec, ev, t, worst, peak = _tube_run(d, 15, 100)
+    print(f"tube self-contact, plastic: worst displacement rel err {worst:.2e}, peak pairs {peak}")
+    assert peak > 0 and m.contact.it.econt > 0.0
+    assert worst < 1e-5
+    assert rel(nd["v"], m.v) < 1e-4
+    assert t.tt == pytest.approx(m.m.tt, rel=1e-6)      # the contact time step differs too
+    assert ec == pytest.approx(m.contact.it.econt, rel=1e-4)
