@@ -391,6 +391,26 @@ def tube_batch_leg(ntube, steps, solo_cycles, autotune=True):
         s.close()
         print(f"tubes {name}: {res[name]['value']:.4e} el-cyc/s, elem {el:.3f} ms, node {nd:.3f} ms,"
               f" launch {best}", file=sys.stderr, flush=True)
+        if prec == en.MIXED:
+            best_mixed = best
+    # the same batch with a TYPE7 self-contact interface per tube (every node against every
+    # shell of its tube), mixed form, the mixed run's launch configuration
+    cdeck = dk.tubes_self_contact(dict(deck), gap=0.5)
+    s = en.ShellSolver(cdeck, en.MIXED)
+    s.set_launch(**best_mixed)
+    s.run_cycles(2)
+    t0 = time.perf_counter()
+    s.run_cycles(steps)
+    dt = time.perf_counter() - t0
+    (ec, ev, _), cnt, _ = s.contact7_state()
+    s.close()
+    res["mixed_self_contact"] = dict(
+        value=ne * steps / dt, ms_per_step=1e3 * dt / steps,
+        contact_ms_per_cycle=1e3 * dt / steps - res["mixed"]["ms_per_step"],
+        secondary_nodes=nn, segments=ne, gap=0.5, contact_energy=ec, friction_energy=ev,
+        penetrating_pairs_last_cycle=cnt["npen"], candidates_last_cycle=cnt["ncand"])
+    print(f"tubes mixed with self-contact: {1e3 * dt / steps:.4f} ms/cycle, "
+          f"{cnt['npen']} pairs at the end", file=sys.stderr, flush=True)
     # one model at a time, mixed form
     no, eo = deck["node_off"], deck["elem_off"]
     keys = ("x", "v", "vr", "ms", "in", "icodt", "icodr", "fv_dir", "fv_val")

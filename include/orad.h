@@ -311,7 +311,11 @@ int orad_autotune(orad_ctx *ctx, int cycles, orad_launch_t *best, double *best_m
  * (secondary node, segment) order, where the reference batches the candidates its
  * bucket sort keeps.  A pass that outgrows cand_cap, or a secondary node with more
  * than 64 candidate segments or 16 penetrated ones, fails the run with ORAD_EINVAL.
- * Not with batched models.
+ * With batched models (orad_set_shell_models first) the interface is one per model in
+ * effect: NSV must increase, a pair's node and segment must lie in one model, a batch
+ * never mixes models and each model's contact time step lowers its own DT2T, so every
+ * model runs as it would alone (orad_contact7 is then refused: the contact runs in
+ * orad_run_cycles).
  *
  * orad_contact7: one pass at the current X, V with the given DT1 (the group path:
  * call it where I7MAINF runs; the next orad_asspar4 adds the entries); lowers *dt2t.

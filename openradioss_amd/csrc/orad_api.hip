@@ -229,7 +229,8 @@ static hipError_t launch_cycle_elements(orad_ctx *c, int p) {
 // the time record's DT2, the contact time step lowers the cycle's DT2T.
 static hipError_t launch_cycle_contact(orad_ctx *c, int p) {
   if (!c->ci) return hipSuccess;
-  return contact7_run(*c->ci, c->X4, c->V4, &c->ctl->s[p].dt2, &c->ctl->dt2t[p], c->stream);
+  return contact7_run(*c->ci, c->X4, c->V4, &c->ctl[0].s[p].dt2, &c->ctl[0].dt2t[p],
+                      sizeof(CycleCtl), c->stream);
 }
 
 static hipError_t enqueue_cycle(orad_ctx *c, int p) {
@@ -268,14 +269,15 @@ static void free_contact(orad_ctx *c) {
   Contact7 *k = c->ci;
   if (!k) return;
   for (void *p : {(void *)k->nsv, (void *)k->irect, (void *)k->stfm, (void *)k->stfns,
-                  (void *)k->bounds, (void *)k->bpart, (void *)k->grid, (void *)k->gdims, (void *)k->seg_lo,
+                  (void *)k->bounds, (void *)k->bpart, (void *)k->grid, (void *)k->seg_lo,
                   (void *)k->seg_hi, (void *)k->seg_cell, (void *)k->cell_count,
                   (void *)k->cell_fill, (void *)k->cell_start, (void *)k->cell_seg,
                   (void *)k->node_keep, (void *)k->node_nkeep, (void *)k->node_kstart,
                   (void *)k->counters, (void *)k->sky,
                   (void *)k->sky_node, (void *)k->sky_next, (void *)k->batch_cnt,
                   (void *)k->batch_e, (void *)k->energy, (void *)k->node_head,
-                  (void *)k->scan_tmp})
+                  (void *)k->scan_tmp, (void *)k->seg_model, (void *)k->nsv_off,
+                  (void *)k->model_bstart, (void *)k->model_ext})
     dfree(p);
   delete k;
   c->ci = nullptr;
@@ -1521,7 +1523,7 @@ extern "C" {
 int orad_set_shell_models(orad_ctx *c, int nmodel, const int *node_off, const int *elem_off,
                           const double *p, const int *ip) {
   if (c && c->ci)
-    return fail(c, ORAD_EINVAL, "set_shell_models: a context with a contact interface takes one model");
+    return fail(c, ORAD_ESTATE, "set_shell_models: set the models before the contact interface");
   if (!c) return ORAD_EINVAL;
   if (int fr = flush_groups(c)) return fr;
   if (c->family != FAM_SHELL || !c->have_shmat)
@@ -1746,8 +1748,6 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
   if (!c) return ORAD_EINVAL;
   if (int fr = flush_groups(c)) return fr;
   if (!c->have_nodes) return fail(c, ORAD_ESTATE, "set_contact7: nodes must be set first");
-  if (c->batched)
-    return fail(c, ORAD_EINVAL, "set_contact7: batched models take no contact interface");
   if (nsn < 0 || nrtm < 0 || (nsn > 0 && (!nsv || !stfns)) ||
       (nrtm > 0 && (!irect || !stfm)) || !params)
     return fail(c, ORAD_EINVAL, "set_contact7: bad sizes or null arrays");
@@ -1760,13 +1760,39 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
   for (int i = 0; i < 4 * nrtm; ++i)
     if (irect[i] < 1 || irect[i] > nn)
       return fail(c, ORAD_EINVAL, "set_contact7: IRECT out of range");
+  // batched models: a segment's nodes in one model, the secondary nodes in increasing
+  // order so that each model's are contiguous
+  const int nmodel = c->batched ? c->nmodel : 1;
+  std::vector<int32_t> h_nsv_off(nmodel + 1, 0), h_seg_model;
+  h_nsv_off[nmodel] = nsn;
+  if (c->batched) {
+    if (nmodel > 16384) return fail(c, ORAD_EINVAL, "set_contact7: at most 16384 models");
+    auto model_of = [&](int node0) {
+      return (int)(std::upper_bound(c->h_node_off.begin(), c->h_node_off.end(), node0) -
+                   c->h_node_off.begin()) - 1;
+    };
+    for (int i = 1; i < nsn; ++i)
+      if (nsv[i] <= nsv[i - 1])
+        return fail(c, ORAD_EINVAL, "set_contact7: with batched models NSV must increase");
+    for (int m = 0; m <= nmodel; ++m)
+      h_nsv_off[m] = (int)(std::lower_bound(nsv, nsv + nsn, c->h_node_off[m] + 1) - nsv);
+    h_seg_model.resize(nrtm);
+    for (int l = 0; l < nrtm; ++l) {
+      const int m = model_of(irect[4 * l] - 1);
+      for (int k = 1; k < 4; ++k)
+        if (model_of(irect[4 * l + k] - 1) != m)
+          return fail(c, ORAD_EINVAL, "set_contact7: a segment's nodes lie in two models");
+      h_seg_model[l] = m;
+    }
+  }
   const long long cap = cand_cap > 0 ? (long long)cand_cap
                                      : std::max<long long>(8LL * nsn, 1024LL);
   const long long cells = std::max<long long>(2LL * nrtm, 1024LL);
   const long long lim = contact7_scan_max();
   if (cap > lim || cells > lim || nsn > lim || nn > lim ||
       (long long)nsn * I7_KEEP_CAP > (1LL << 31) - 1 ||
-      (cap + I7_NVSIZ - 1) / I7_NVSIZ * I7_SLOTS > (1LL << 31) - 1)
+      ((cap + I7_NVSIZ - 1) / I7_NVSIZ + (c->batched ? c->nmodel : 1)) * I7_SLOTS >
+          (1LL << 31) - 1)
     return fail(c, ORAD_EINVAL, "set_contact7: interface too large for one pass");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1786,7 +1812,9 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
   contact_constants(*k);
   k->cell_cap = (int)cells;
   k->pair_cap = (int)cap;
-  k->nbatch_cap = (int)((cap + I7_NVSIZ - 1) / I7_NVSIZ);
+  k->nmodel = nmodel;
+  k->node_model = c->batched ? c->node_model : nullptr;
+  k->nbatch_cap = (int)((cap + I7_NVSIZ - 1) / I7_NVSIZ) + nmodel;
   const size_t nslots = (size_t)k->nbatch_cap * I7_SLOTS;
   const size_t ntmp = contact7_scan_tmp_ints((int)std::max<long long>(cells, nsn));
   hipError_t e = hipSuccess;
@@ -1801,7 +1829,6 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
   k->nbound_blocks = std::max(1, (nrtm + 255) / 256);
   al(&k->bpart, (size_t)7 * k->nbound_blocks * 8);
   al(&k->grid, 4 * 8);
-  al(&k->gdims, 4 * 4);
   al(&k->seg_lo, (size_t)nrtm * sizeof(float4));
   al(&k->seg_hi, (size_t)nrtm * sizeof(float4));
   al(&k->seg_cell, (size_t)nrtm * 4);
@@ -1821,6 +1848,10 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
   al(&k->energy, 3 * 8);
   al(&k->node_head, (size_t)nn * 4);
   al(&k->scan_tmp, ntmp * 4);
+  al(&k->nsv_off, (size_t)(nmodel + 1) * 4);
+  al(&k->model_bstart, (size_t)(nmodel + 1) * 4);
+  al(&k->model_ext, (size_t)nmodel * 8);
+  if (c->batched) al(&k->seg_model, (size_t)nrtm * 4);
   if (e != hipSuccess) {
     free_contact(c);
     return fail(c, ORAD_EHIP, std::string("set_contact7: ") + hipGetErrorString(e));
@@ -1839,6 +1870,12 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
     HIPCHK(c, hipMemcpy(k->irect, hr.data(), (size_t)nrtm * sizeof(int4), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(k->stfm, stfm, (size_t)nrtm * 8, hipMemcpyHostToDevice));
   }
+  HIPCHK(c, hipMemcpy(k->nsv_off, h_nsv_off.data(), (size_t)(nmodel + 1) * 4,
+                      hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemset(k->model_bstart, 0, (size_t)(nmodel + 1) * 4));
+  if (c->batched && nrtm)
+    HIPCHK(c, hipMemcpy(k->seg_model, h_seg_model.data(), (size_t)nrtm * 4,
+                        hipMemcpyHostToDevice));
   HIPCHK(c, hipMemset(k->counters, 0, I7C_N * 4));
   HIPCHK(c, hipMemset(k->energy, 0, 3 * 8));
   HIPCHK(c, hipMemset(k->node_head, 0xFF, (size_t)nn * 4));      // empty lists (-1)
@@ -1850,13 +1887,15 @@ int orad_contact7(orad_ctx *c, double dt1, double *dt2t) {
   if (!c) return ORAD_EINVAL;
   if (int fr = flush_groups(c)) return fr;
   if (!c->ci) return fail(c, ORAD_ESTATE, "contact7: no contact interface (orad_set_contact7)");
+  if (c->batched)
+    return fail(c, ORAD_EINVAL, "contact7: batched models run their contact in orad_run_cycles");
   int r = ready(c);
   if (r) return r;
   HIPCHK(c, hipSetDevice(c->device));
   const unsigned long long inf = 0x7FF0000000000000ull;
   HIPCHK(c, hipMemcpyAsync(c->scratch_dt1, &dt1, 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->scratch_dt2t, &inf, 8, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, contact7_run(*c->ci, c->X4, c->V4, c->scratch_dt1, c->scratch_dt2t, c->stream));
+  HIPCHK(c, contact7_run(*c->ci, c->X4, c->V4, c->scratch_dt1, c->scratch_dt2t, 0, c->stream));
   unsigned long long bits = inf;
   HIPCHK(c, hipMemcpyAsync(&bits, c->scratch_dt2t, 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1897,7 +1936,7 @@ int orad_get_contact7_sky(orad_ctx *c, int cap, double *fsky, int *isky, int *ns
   const Contact7 &k = *c->ci;
   int cnt[I7C_N];
   HIPCHK(c, hipMemcpy(cnt, k.counters, sizeof cnt, hipMemcpyDeviceToHost));
-  const int nb = (cnt[I7C_NPEN] + I7_NVSIZ - 1) / I7_NVSIZ;
+  const int nb = cnt[I7C_NBATCH];
   std::vector<int32_t> bc(nb > 0 ? nb : 1);
   if (nb) HIPCHK(c, hipMemcpy(bc.data(), k.batch_cnt, (size_t)nb * 4, hipMemcpyDeviceToHost));
   int n = 0;

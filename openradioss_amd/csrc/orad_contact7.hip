@@ -135,12 +135,6 @@ __device__ __forceinline__ void seg_box(const Contact7 &c, const double4 *X4, in
   hi[2] = dmax(dmax(dmax(p0.z, p1.z), p2.z), p3.z);
 }
 
-__device__ __forceinline__ int cell_axis(double x, double o, double inv, int n) {
-  double f = floor((x - o) * inv);
-  if (!(f >= -1.0)) f = -1.0;        // below the grid, or not a number
-  if (f > (double)n) f = (double)n;
-  return (int)f;
-}
 
 // ------------------------------------------------------------ search grid --
 __global__ __launch_bounds__(256) void i7_reset(Contact7 c) {
@@ -150,6 +144,7 @@ __global__ __launch_bounds__(256) void i7_reset(Contact7 c) {
     c.cell_fill[i] = 0;
   }
   if (i < c.numnod) c.node_head[i] = -1;
+  if (i < c.nmodel) c.model_ext[i] = 0ull;
   if (i < 3) {
     c.bounds[i] = ~0ull;
     c.bounds[3 + i] = 0ull;
@@ -158,48 +153,58 @@ __global__ __launch_bounds__(256) void i7_reset(Contact7 c) {
   if (i < I7C_N && i != I7C_OVERFLOW) c.counters[i] = 0;   // the overflow flag is sticky
 }
 
-// Cell size: the largest segment box edge plus 2 TZ, so the centre of every segment
-// whose widened box holds a node lies in the node's cell or a neighbour; grown until
-// the grid fits cell_cap.  Every thread derives the same grid from the bounds.
+// Cell size (per model): the model's largest segment box edge plus 2 TZ, so the centre
+// of every segment whose widened box holds a node lies within half a cell of the node on
+// each axis: in the node's cell or the neighbour on the side of the node's half,
+// 2 x 2 x 2 cells.  Cells are
+// hashed with the model into cell_cap buckets: models that overlap in space (batched
+// independent models) share no bucket but by collision, and the grid needs no extent.
 struct I7Grid {
   double o[3], s, inv;
-  int n[3];
 };
 
 __device__ __forceinline__ I7Grid i7_grid(const Contact7 &c) {
   I7Grid g;
-  double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
-  if (c.nrtm > 0)
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = odec(c.bounds[d]);
-      hi[d] = odec(c.bounds[3 + d]);
-    }
-  double s = (c.nrtm > 0 ? odec(c.bounds[6]) : 0.0) + 2.0 * c.tz;
+  for (int d = 0; d < 3; ++d) g.o[d] = c.nrtm > 0 ? odec(c.bounds[d]) : 0.0;
+  // padded by 1e-6 so a candidate's centre lies strictly within half a cell of the
+  // node on every axis, with room for the rounding of the cell coordinates
+  double s = ((c.nrtm > 0 ? odec(c.bounds[6]) : 0.0) + 2.0 * c.tz) * (1.0 + 1.0e-6);
   if (!(s > 0.0)) s = 1.0;
-  int n[3] = {1, 1, 1};
-  bool fits = false;
-  for (int it = 0; it < 200 && !fits; ++it) {
-    double tot = 1.0;
-    for (int d = 0; d < 3; ++d) {
-      const double k = floor((hi[d] - lo[d]) / s) + 1.0;
-      n[d] = (k >= 1.0 && k <= 1.0e9) ? (int)k : (k > 1.0e9 ? 1000000000 : 1);
-      tot *= (double)n[d];
-    }
-    fits = tot <= (double)c.cell_cap;
-    if (!fits) s *= 1.5;
-  }
-  if (!fits) {
-    // non-finite coordinates: one cell holding every segment
-    n[0] = n[1] = n[2] = 1;
+  if (!(s < 1.0e300) || !(g.o[0] == g.o[0] && g.o[1] == g.o[1] && g.o[2] == g.o[2])) {
+    // non-finite coordinates: one cell per model
     s = 1.0e300;
-  }
-  for (int d = 0; d < 3; ++d) {
-    g.o[d] = lo[d];
-    g.n[d] = n[d];
+    g.o[0] = g.o[1] = g.o[2] = 0.0;
   }
   g.s = s;
   g.inv = 1.0 / s;
   return g;
+}
+
+// Model m's cell size: its largest segment edge plus 2 TZ, padded as above.
+__device__ __forceinline__ double model_cell(const Contact7 &c, int m) {
+  double s = (odec(c.model_ext[m]) + 2.0 * c.tz) * (1.0 + 1.0e-6);
+  if (!(s > 0.0)) s = 1.0;
+  if (!(s < 1.0e300) || !(c.grid[3] < 1.0e300)) s = 1.0e300;
+  return s;
+}
+
+__device__ __forceinline__ long long cell_coord(double x, double o, double inv) {
+  double f = floor((x - o) * inv);
+  if (!(f > -1.0e15)) f = -1.0e15;      // far below, or not a number
+  if (f > 1.0e15) f = 1.0e15;
+  return (long long)f;
+}
+
+__device__ __forceinline__ int cell_hash(long long kx, long long ky, long long kz, int model,
+                                         int cap) {
+  unsigned long long h = (unsigned long long)kx * 0x9E3779B97F4A7C15ull;
+  h ^= (unsigned long long)ky * 0xC2B2AE3D27D4EB4Full;
+  h ^= (unsigned long long)kz * 0x165667B19E3779F9ull;
+  h ^= (unsigned long long)(unsigned)model * 0x27D4EB2F165667C5ull;
+  h ^= h >> 29;
+  h *= 0xBF58476D1CE4E5B9ull;
+  h ^= h >> 32;
+  return (int)(h % (unsigned long long)cap);
 }
 
 // Segment boxes of the pass, and the bounds of their centres and their largest edge:
@@ -209,6 +214,7 @@ __global__ __launch_bounds__(256) void i7_seg_bounds(Contact7 c, const double4 *
   __shared__ unsigned long long s_b[7][4];
   const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
   unsigned long long k[7] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull, 0ull};
+  const int model = (l < c.nrtm && c.seg_model) ? c.seg_model[l] : 0;
   if (l < c.nrtm) {
     double lo[3], hi[3];
     seg_box(c, X4, l, lo, hi);
@@ -230,6 +236,22 @@ __global__ __launch_bounds__(256) void i7_seg_bounds(Contact7 c, const double4 *
       ext = dmax(ext, hi[d] - lo[d]);
     }
     k[6] = okey(ext);
+  }
+  // the model's largest edge: one atomic per wave when the wave holds one model
+  {
+    const int m0 = __shfl(model, 0, 64);
+    const bool one = __ballot(l < c.nrtm && model != m0) == 0ull;
+    if (one) {
+      unsigned long long e = k[6];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long b = __shfl_xor(e, o, 64);
+        e = b > e ? b : e;
+      }
+      if ((threadIdx.x & 63) == 0 && e) atomicMax(&c.model_ext[m0], e);
+    } else if (l < c.nrtm) {
+      atomicMax(&c.model_ext[model], k[6]);
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1)
@@ -279,29 +301,21 @@ __global__ __launch_bounds__(1024) void i7_bounds_grid(Contact7 c) {
       c.bounds[j] = v;
     }
     const I7Grid g = i7_grid(c);
-    for (int d = 0; d < 3; ++d) {
-      c.grid[d] = g.o[d];
-      c.gdims[d] = g.n[d];
-    }
+    for (int d = 0; d < 3; ++d) c.grid[d] = g.o[d];
     c.grid[3] = g.s;
-    c.gdims[3] = g.n[0] * g.n[1] * g.n[2];
   }
 }
 
 __global__ __launch_bounds__(256) void i7_seg_count(Contact7 c) {
   const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
   if (l >= c.nrtm) return;
-  const double inv = 1.0 / c.grid[3];
+  const int model = c.seg_model ? c.seg_model[l] : 0;
+  const double inv = 1.0 / model_cell(c, model);
   const float4 lo = c.seg_lo[l], hi = c.seg_hi[l];
-  const double m[3] = {0.5 * ((double)lo.x + (double)hi.x), 0.5 * ((double)lo.y + (double)hi.y),
-                       0.5 * ((double)lo.z + (double)hi.z)};
-  int k[3];
-  for (int d = 0; d < 3; ++d) {
-    const int n = c.gdims[d];
-    const int q = cell_axis(m[d], c.grid[d], inv, n);
-    k[d] = q < 0 ? 0 : (q >= n ? n - 1 : q);
-  }
-  const int cell = k[0] + c.gdims[0] * (k[1] + c.gdims[1] * k[2]);
+  const long long kx = cell_coord(0.5 * ((double)lo.x + (double)hi.x), c.grid[0], inv);
+  const long long ky = cell_coord(0.5 * ((double)lo.y + (double)hi.y), c.grid[1], inv);
+  const long long kz = cell_coord(0.5 * ((double)lo.z + (double)hi.z), c.grid[2], inv);
+  const int cell = cell_hash(kx, ky, kz, model, c.cell_cap);
   c.seg_cell[l] = cell;
   atomicAdd(&c.cell_count[cell], 1);
 }
@@ -424,40 +438,39 @@ __global__ __launch_bounds__(128) void i7_node_pass(Contact7 c, const double4 *X
   if (c.stfns[i] != 0.0) {
     const int nn = c.nsv[i];
     const double4 x = X4[nn];
-    const double o[3] = {c.grid[0], c.grid[1], c.grid[2]};
-    const double inv = 1.0 / c.grid[3];
-    const int n0 = c.gdims[0], n1 = c.gdims[1], n2 = c.gdims[2];
-    const int kx = cell_axis(x.x, o[0], inv, n0);
-    const int ky = cell_axis(x.y, o[1], inv, n1);
-    const int kz = cell_axis(x.z, o[2], inv, n2);
-    for (int z = kz - 1; z <= kz + 1; ++z) {
-      if (z < 0 || z >= n2) continue;
-      for (int y = ky - 1; y <= ky + 1; ++y) {
-        if (y < 0 || y >= n1) continue;
-        for (int xx = kx - 1; xx <= kx + 1; ++xx) {
-          if (xx < 0 || xx >= n0) continue;
-          const int cell = xx + n0 * (y + n1 * z);
-          for (int j = c.cell_start[cell]; j < c.cell_start[cell + 1]; ++j) {
-            const int l = c.cell_seg[j];
-            if (c.stfm[l] == 0.0) continue;
-            const float4 lo = c.seg_lo[l], hi = c.seg_hi[l];
-            if (!(x.x > (double)lo.x - c.tz && x.x < (double)hi.x + c.tz &&
-                  x.y > (double)lo.y - c.tz && x.y < (double)hi.y + c.tz &&
-                  x.z > (double)lo.z - c.tz && x.z < (double)hi.z + c.tz))
-              continue;
-            const int4 r = c.irect[l];
-            if (nn == r.x || nn == r.y || nn == r.z || nn == r.w) continue;
-            if (cnt < I7_NODE_CAP) {
-              int p = cnt;
-              while (p > 0 && list[(p - 1) * 128] > l) {
-                list[p * 128] = list[(p - 1) * 128];
-                --p;
-              }
-              list[p * 128] = l;
-            }
-            ++cnt;
-          }
+    const int model = c.node_model ? c.node_model[nn] : 0;
+    const double inv = 1.0 / model_cell(c, model);
+    const long long kx = cell_coord(x.x, c.grid[0], inv);
+    const long long ky = cell_coord(x.y, c.grid[1], inv);
+    const long long kz = cell_coord(x.z, c.grid[2], inv);
+    // the neighbour on the side of the node's half of its cell, per axis
+    const long long nx = ((x.x - c.grid[0]) * inv - (double)kx < 0.5) ? kx - 1 : kx + 1;
+    const long long ny = ((x.y - c.grid[1]) * inv - (double)ky < 0.5) ? ky - 1 : ky + 1;
+    const long long nz = ((x.z - c.grid[2]) * inv - (double)kz < 0.5) ? kz - 1 : kz + 1;
+    for (int q = 0; q < 8; ++q) {
+      const long long xx = (q & 1) ? nx : kx, y = (q & 2) ? ny : ky, z = (q & 4) ? nz : kz;
+      const int cell = cell_hash(xx, y, z, model, c.cell_cap);
+      for (int j = c.cell_start[cell]; j < c.cell_start[cell + 1]; ++j) {
+        const int l = c.cell_seg[j];
+        if (c.stfm[l] == 0.0) continue;
+        const float4 lo = c.seg_lo[l], hi = c.seg_hi[l];
+        if (!(x.x > (double)lo.x - c.tz && x.x < (double)hi.x + c.tz &&
+              x.y > (double)lo.y - c.tz && x.y < (double)hi.y + c.tz &&
+              x.z > (double)lo.z - c.tz && x.z < (double)hi.z + c.tz))
+          continue;
+        const int4 r = c.irect[l];
+        if (nn == r.x || nn == r.y || nn == r.z || nn == r.w) continue;
+        if (c.node_model && model != c.seg_model[l]) continue;
+        if (cnt < I7_NODE_CAP) {
+          // sorted insertion; a segment met again (two of the cells in one bucket) is
+          // kept once
+          int p = cnt;
+          while (p > 0 && list[(p - 1) * 128] > l) --p;
+          if (p > 0 && list[(p - 1) * 128] == l) continue;
+          for (int u = cnt; u > p; --u) list[u * 128] = list[(u - 1) * 128];
+          list[p * 128] = l;
         }
+        ++cnt;
       }
     }
     if (cnt > I7_NODE_CAP) {
@@ -480,30 +493,86 @@ __global__ __launch_bounds__(128) void i7_node_pass(Contact7 c, const double4 *X
   c.node_nkeep[i] = nkeep;
 }
 
-// The number of penetrating pairs of the pass (the scan's total, capped).
-__device__ __forceinline__ int i7_npen(const Contact7 &c) {
-  const int tot = c.node_kstart[c.nsn];
-  return tot > c.pair_cap ? c.pair_cap : tot;
+// The penetrating pairs before secondary node k, capped at the pair capacity.
+__device__ __forceinline__ int i7_pairs_before(const Contact7 &c, int k) {
+  const int v = c.node_kstart[k];
+  return v > c.pair_cap ? c.pair_cap : v;
+}
+
+// Batches per model (NVSIZ pairs each, a model's last one ragged) and their prefix: one
+// workgroup.
+__global__ __launch_bounds__(1024) void i7_model_batches(Contact7 c) {
+  __shared__ int s_w[17];
+  constexpr int PER = 16;
+  const int m0 = (int)threadIdx.x * PER;
+  int nb[PER], sum = 0;
+  for (int k = 0; k < PER; ++k) {
+    const int m = m0 + k;
+    nb[k] = 0;
+    if (m < c.nmodel) {
+      const int n = i7_pairs_before(c, c.nsv_off[m + 1]) - i7_pairs_before(c, c.nsv_off[m]);
+      nb[k] = (n + I7_NVSIZ - 1) / I7_NVSIZ;
+    }
+    sum += nb[k];
+  }
+  // exclusive scan of the threads' sums
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = sum;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int i = 0; i < 16; ++i) {
+      const int t = s_w[i];
+      s_w[i] = acc;
+      acc += t;
+    }
+    s_w[16] = acc;
+    const int tot = c.node_kstart[c.nsn];
+    if (tot > c.pair_cap) atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_PAIRS);
+    c.counters[I7C_NPEN] = tot > c.pair_cap ? c.pair_cap : tot;
+    c.counters[I7C_NBATCH] = acc;
+    c.model_bstart[c.nmodel] = acc;
+  }
+  __syncthreads();
+  int pre = x - sum + s_w[w];
+  for (int k = 0; k < PER; ++k) {
+    const int m = m0 + k;
+    if (m < c.nmodel) c.model_bstart[m] = pre;
+    pre += nb[k];
+  }
 }
 
 // I7FOR3 + I7ASS2 for one batch of NVSIZ penetrating pairs, one lane per pair.
 __global__ __launch_bounds__(I7_NVSIZ) void i7_batch(Contact7 c, const double4 *X4,
-                                                     const double4 *V4, const double *dt1p,
-                                                     unsigned long long *dt2t) {
+                                                     const double4 *V4, const double *dt1b,
+                                                     unsigned long long *dt2tb, size_t stride) {
   __shared__ double s_rd[I7_NVSIZ], s_ec[I7_NVSIZ], s_ev[I7_NVSIZ];
   __shared__ int s_cnt[5][2];
-  const int npen = i7_npen(c);
+  const int nbt = c.model_bstart[c.nmodel];
   const int b = (int)blockIdx.x;
-  if (b == 0 && threadIdx.x == 0) {
-    if (c.node_kstart[c.nsn] > c.pair_cap) atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_PAIRS);
-    c.counters[I7C_NPEN] = npen;
+  if (b >= nbt) return;
+  // the batch's model: the last m with model_bstart[m] <= b
+  int mlo = 0, mhi = c.nmodel - 1;
+  while (mlo < mhi) {
+    const int mid = (mlo + mhi + 1) >> 1;
+    if (c.model_bstart[mid] <= b) mlo = mid;
+    else mhi = mid - 1;
   }
-  if (b * I7_NVSIZ >= npen) return;
+  const int model = mlo;
+  const int p0 = i7_pairs_before(c, c.nsv_off[model]) + (b - c.model_bstart[model]) * I7_NVSIZ;
+  const int p1 = i7_pairs_before(c, c.nsv_off[model + 1]);
   const int t = (int)threadIdx.x;
-  const int r = b * I7_NVSIZ + t;
-  const bool live = r < npen;
-  const int nlive = npen - b * I7_NVSIZ < I7_NVSIZ ? npen - b * I7_NVSIZ : I7_NVSIZ;
-  const double dt1 = *dt1p;
+  const int r = p0 + t;
+  const bool live = r < p1;
+  const int nlive = p1 - p0 < I7_NVSIZ ? p1 - p0 : I7_NVSIZ;
+  const double dt1 = *(const double *)((const char *)dt1b + (size_t)model * stride);
+  unsigned long long *dt2t =
+      (unsigned long long *)((char *)dt2tb + (size_t)model * stride);
   const double dt1inv = dt1 > 0.0 ? 1.0 / dt1 : 0.0;
   double h[4] = {0.0, 0.0, 0.0, 0.0}, fxi[3] = {0.0, 0.0, 0.0}, stif = 0.0;
   double rdist = 1.0e300, econtt = 0.0, econvt = 0.0;
@@ -639,7 +708,7 @@ __global__ __launch_bounds__(I7_NVSIZ) void i7_batch(Contact7 c, const double4 *
     // the last batch to finish adds the batches' energies in batch order
     // (ECONT = ECONT + ECONTT per I7FOR3 call)
     __threadfence();
-    const int nb = (npen + I7_NVSIZ - 1) / I7_NVSIZ;
+    const int nb = nbt;
     if (atomicAdd(&c.counters[I7C_DONE], 1) == nb - 1) {
       __threadfence();
       double et = c.energy[0], vt = c.energy[1];
@@ -674,26 +743,30 @@ __global__ __launch_bounds__(I7_NVSIZ) void i7_batch(Contact7 c, const double4 *
 }
 
 hipError_t contact7_run(const Contact7 &c, const double4 *X4, const double4 *V4,
-                        const double *dt1, unsigned long long *dt2t, hipStream_t s) {
+                        const double *dt1, unsigned long long *dt2t, size_t stride,
+                        hipStream_t s) {
   auto g = [](long n, int blk) {
     const long b = (n + blk - 1) / blk;
     return dim3((unsigned)(b > 0 ? b : 1));
   };
   long nreset = c.cell_cap;
   if (c.numnod > nreset) nreset = c.numnod;
+  if (c.nmodel > nreset) nreset = c.nmodel;
   if (nreset < 8) nreset = 8;
   hipLaunchKernelGGL(i7_reset, g(nreset, 256), dim3(256), 0, s, c);
   hipLaunchKernelGGL(i7_seg_bounds, dim3(c.nbound_blocks), dim3(256), 0, s, c, X4);
   hipLaunchKernelGGL(i7_bounds_grid, dim3(1), dim3(1024), 0, s, c);
   hipLaunchKernelGGL(i7_seg_count, g(c.nrtm, 256), dim3(256), 0, s, c);
-  hipError_t e = scan_excl(c.cell_count, c.cell_start, c.gdims + 3, 0, c.cell_cap, c.scan_tmp, s);
+  hipError_t e = scan_excl(c.cell_count, c.cell_start, nullptr, c.cell_cap, c.cell_cap,
+                           c.scan_tmp, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(i7_seg_place, g(c.nrtm, 256), dim3(256), 0, s, c);
   hipLaunchKernelGGL(i7_node_pass, g(c.nsn, 128), dim3(128), 0, s, c, X4, V4);
   e = scan_excl(c.node_nkeep, c.node_kstart, nullptr, c.nsn, c.nsn, c.scan_tmp, s);
   if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(i7_model_batches, dim3(1), dim3(1024), 0, s, c);
   hipLaunchKernelGGL(i7_batch, dim3(c.nbatch_cap > 0 ? c.nbatch_cap : 1), dim3(I7_NVSIZ), 0, s,
-                     c, X4, V4, dt1, dt2t);
+                     c, X4, V4, dt1, dt2t, stride);
   return hipGetLastError();
 }
 

@@ -32,7 +32,10 @@ constexpr int I7_NODE_CAP = 64;           // candidate segments of one secondary
 constexpr int I7_KEEP_CAP = 16;           // penetrated segments of one secondary node
 
 // counters[] of a pass
-enum { I7C_NCAND = 0, I7C_NPEN = 1, I7C_OVERFLOW = 2, I7C_NSKY = 3, I7C_DONE = 4, I7C_N = 5 };
+enum {
+  I7C_NCAND = 0, I7C_NPEN = 1, I7C_OVERFLOW = 2, I7C_NSKY = 3, I7C_DONE = 4, I7C_NBATCH = 5,
+  I7C_N = 6
+};
 // overflow bits (sticky until the interface is set again)
 enum { I7_OVF_NODE = 1, I7_OVF_PAIRS = 2, I7_OVF_KEEP = 4 };
 
@@ -44,13 +47,13 @@ struct Contact7 {
   double *stfns = nullptr;      // [nsn] secondary factor STFN, negated on deactivation
   double gap = 0.0, fric = 0.0, visc = 0.0, viscf = 0.0, tz = 0.0;
   double em10 = 0.0, em20 = 0.0, em30 = 0.0, zep4 = 0.0;   // constant_mod.F values
-  // search grid over the segments' box centres
+  // search grid over the segments' box centres, cells hashed with the model into
+  // cell_cap buckets
   int cell_cap = 0;
   unsigned long long *bounds = nullptr;  // [7] ordered bits: min xyz, max xyz, max extent
   unsigned long long *bpart = nullptr;   // [7 * nbound_blocks] the workgroups' bounds
   int nbound_blocks = 0;
   double *grid = nullptr;                // [4] origin xyz, cell size
-  int *gdims = nullptr;                  // [4] cells per axis, total
   float4 *seg_lo = nullptr, *seg_hi = nullptr;   // [nrtm] segment boxes of the pass,
                                                   // rounded outward to FP32
   int32_t *seg_cell = nullptr;           // [nrtm]
@@ -60,6 +63,15 @@ struct Contact7 {
   // penetrating pairs: per secondary node its penetrated segments in increasing order;
   // pair r of the pass is node i's (r - node_kstart[i])-th, node_kstart[i] <= r
   int pair_cap = 0, nbatch_cap = 0;
+  // models (batched independent models; one otherwise): a pair's two ends belong to one
+  // model, the secondary nodes of model m are nsv[nsv_off[m] .. nsv_off[m+1]), and a
+  // batch never mixes models, so each model's pairs batch as they would alone
+  int nmodel = 1;
+  const int32_t *node_model = nullptr;   // the context's node -> model (null: one model)
+  int32_t *seg_model = nullptr;          // [nrtm] (null: one model)
+  int32_t *nsv_off = nullptr;            // [nmodel + 1]
+  int32_t *model_bstart = nullptr;       // [nmodel + 1] first batch of each model
+  unsigned long long *model_ext = nullptr;   // [nmodel] ordered bits: largest segment edge
   int32_t *node_keep = nullptr;          // [nsn * I7_KEEP_CAP]
   int32_t *node_nkeep = nullptr;         // [nsn]
   int32_t *node_kstart = nullptr;        // [nsn + 1]
@@ -75,11 +87,13 @@ struct Contact7 {
   int32_t *scan_tmp = nullptr;           // block sums of the scans
 };
 
-// One contact pass: search, filter, forces, lists.  dt1 is read on the device
-// (DT1 of the cycle); the batches' contact time steps lower *dt2t (bits of a positive
-// double, atomicMin).
+// One contact pass: search, filter, forces, lists.  DT1 is read on the device; the
+// batches' contact time steps lower DT2T (bits of a positive double, atomicMin).
+// Model m's DT1 is at (char *)dt1 + m * stride and its DT2T accumulator at
+// (char *)dt2t + m * stride (the models' time records; stride 0 with one model).
 hipError_t contact7_run(const Contact7 &ci, const double4 *X4, const double4 *V4,
-                        const double *dt1, unsigned long long *dt2t, hipStream_t s);
+                        const double *dt1, unsigned long long *dt2t, size_t stride,
+                        hipStream_t s);
 // Scan workspace (ints) for n elements, and the longest scan one pass can do.
 size_t contact7_scan_tmp_ints(int n);
 int contact7_scan_max();

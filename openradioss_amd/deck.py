@@ -387,3 +387,17 @@ def shell_tubes(ntube=256, side=(30.0, 80.0), length=(100.0, 400.0), h=(2.0, 5.0
     cat["prop"] = shell_prop_params(npt=npt)
     cat["tubes"] = par
     return cat
+
+
+def tubes_self_contact(d, gap, fric=0.2, visc=0.05, viscf=1.0, stfac=1.0):
+    """Adds a /INTER/TYPE7 self-contact interface to a shell_tubes batch: every node
+    secondary, every shell a segment of its own tube (pairs never cross tubes); segment
+    stiffness 0.5 Stfac E t of the shell's tube."""
+    nn, ne = d["x"].shape[0], d["ixc"].shape[0]
+    young = np.empty(ne)
+    for m, mat in enumerate(d["mats"]):
+        young[d["elem_off"][m]:d["elem_off"][m + 1]] = mat["young"]
+    d["contact"] = dict(nsv=np.arange(nn, dtype=np.int32), irect=d["ixc"].copy(),
+                        stfm=0.5 * stfac * young * d["thk"], stfns=np.ones(nn), gap=gap,
+                        fric=fric, visc=visc, viscf=viscf)
+    return d

@@ -240,3 +240,54 @@ def test_tube_crush_self_contact_plastic_vs_oracle():
     assert rel(nd["v"], m.v) < 1e-4
     assert t.tt == pytest.approx(m.m.tt, rel=1e-6)      # the contact time step differs too
     assert ec == pytest.approx(m.contact.it.econt, rel=1e-4)
+
+
+def _tube_alone(d, m):
+    """Tube m of a batch as its own deck, with its own self-contact interface."""
+    n0, n1 = d["node_off"][m], d["node_off"][m + 1]
+    e0, e1 = d["elem_off"][m], d["elem_off"][m + 1]
+    out = {k: d[k][n0:n1] for k in ("x", "v", "vr", "ms", "in", "icodt", "icodr", "fv_dir",
+                                    "fv_val")}
+    out["ixc"] = d["ixc"][e0:e1] - n0
+    out["uid"] = d["uid"][e0:e1]
+    out["thk"] = d["thk"][e0:e1]
+    out["mat"] = d["mats"][m]
+    out["prop"] = d["prop"]
+    c = d["contact"]
+    out["contact"] = dict(c, nsv=np.arange(n1 - n0, dtype=np.int32), irect=out["ixc"].copy(),
+                          stfm=c["stfm"][e0:e1], stfns=c["stfns"][n0:n1])
+    return out
+
+
+@pytest.mark.parametrize("prec", [en.FP64, en.MIXED])
+def test_batched_tubes_with_self_contact_equal_each_tube_alone(prec):
+    """Five tubes crushed at 50 m/s in one batched context, each with self-contact (the
+    tubes overlap in space; pairs never cross models): every tube's nodes, state and time
+    record are bit for bit those of the tube run alone with its own interface."""
+    d = dk.shell_tubes(ntube=5, side=(16.0, 24.0), length=(30.0, 50.0), h=(1.8, 2.2),
+                       thk=(1.2, 1.2), ca=(200.0, 400.0), v_imp=-5.0e4, seed=11)
+    d = dk.tubes_self_contact(d, gap=0.72)
+    ncyc = 900
+    s = en.ShellSolver(d, prec)
+    try:
+        s.run_cycles(ncyc)
+        nd, st = s.nodes(), s.state()
+        times = [s.model_time(m) for m in range(s.nmodel)]
+        (ec, _, _), cnt, _ = s.contact7_state()
+    finally:
+        s.close()
+    assert ec > 0.0
+    for m in range(5):
+        dm = _tube_alone(d, m)
+        r = en.ShellSolver(dm, prec)
+        try:
+            r.run_cycles(ncyc)
+            ndm, stm, tm = r.nodes(), r.state(), r.time()
+        finally:
+            r.close()
+        n0, n1 = d["node_off"][m], d["node_off"][m + 1]
+        e0, e1 = d["elem_off"][m], d["elem_off"][m + 1]
+        for k in ("x", "v", "d", "vr"):
+            assert np.array_equal(nd[k][n0:n1], ndm[k]), (m, k)
+        assert np.array_equal(st[e0:e1], stm), m
+        assert (times[m].tt, times[m].dt2, times[m].ncycle) == (tm.tt, tm.dt2, tm.ncycle), m
