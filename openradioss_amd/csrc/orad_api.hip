@@ -271,8 +271,10 @@ static void free_contact(orad_ctx *c) {
   for (void *p : {(void *)k->nsv, (void *)k->irect, (void *)k->stfm, (void *)k->stfns,
                   (void *)k->bounds, (void *)k->bpart, (void *)k->grid, (void *)k->seg_lo,
                   (void *)k->seg_hi, (void *)k->seg_cell, (void *)k->cell_count,
-                  (void *)k->cell_fill, (void *)k->cell_start, (void *)k->cell_seg,
-                  (void *)k->node_keep, (void *)k->node_nkeep, (void *)k->node_kstart,
+                  (void *)k->cell_fill, (void *)k->cell_start, (void *)k->cell_lo,
+                  (void *)k->cell_hi,
+                  (void *)k->node_keep, (void *)k->node_nkeep,
+                  (void *)k->node_cand, (void *)k->node_ncand, (void *)k->node_kstart,
                   (void *)k->counters, (void *)k->sky,
                   (void *)k->sky_node, (void *)k->sky_next, (void *)k->batch_cnt,
                   (void *)k->batch_e, (void *)k->energy, (void *)k->node_head,
@@ -1787,7 +1789,8 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
   }
   const long long cap = cand_cap > 0 ? (long long)cand_cap
                                      : std::max<long long>(8LL * nsn, 1024LL);
-  const long long cells = std::max<long long>(2LL * nrtm, 1024LL);
+  long long cells = 1024;   // a power of two, at least 2 per segment
+  while (cells < 2LL * nrtm) cells *= 2;
   const long long lim = contact7_scan_max();
   if (cap > lim || cells > lim || nsn > lim || nn > lim ||
       (long long)nsn * I7_KEEP_CAP > (1LL << 31) - 1 ||
@@ -1835,8 +1838,11 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
   al(&k->cell_count, (size_t)cells * 4);
   al(&k->cell_fill, (size_t)cells * 4);
   al(&k->cell_start, (size_t)(cells + 1) * 4);
-  al(&k->cell_seg, (size_t)nrtm * 4);
+  al(&k->cell_lo, (size_t)nrtm * sizeof(float4));
+  al(&k->cell_hi, (size_t)nrtm * sizeof(float4));
   al(&k->node_keep, (size_t)nsn * I7_KEEP_CAP * 4);
+  al(&k->node_cand, (size_t)nsn * I7_NODE_CAP * 4);
+  al(&k->node_ncand, (size_t)nsn * 4);
   al(&k->node_nkeep, (size_t)nsn * 4);
   al(&k->node_kstart, (size_t)(nsn + 1) * 4);
   al(&k->counters, I7C_N * 4);

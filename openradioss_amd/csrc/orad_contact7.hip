@@ -182,7 +182,8 @@ __device__ __forceinline__ I7Grid i7_grid(const Contact7 &c) {
 
 // Model m's cell size: its largest segment edge plus 2 TZ, padded as above.
 __device__ __forceinline__ double model_cell(const Contact7 &c, int m) {
-  double s = (odec(c.model_ext[m]) + 2.0 * c.tz) * (1.0 + 1.0e-6);
+  const unsigned long long e = c.seg_model ? c.model_ext[m] : c.bounds[6];
+  double s = (odec(e) + 2.0 * c.tz) * (1.0 + 1.0e-6);
   if (!(s > 0.0)) s = 1.0;
   if (!(s < 1.0e300) || !(c.grid[3] < 1.0e300)) s = 1.0e300;
   return s;
@@ -204,7 +205,7 @@ __device__ __forceinline__ int cell_hash(long long kx, long long ky, long long k
   h ^= h >> 29;
   h *= 0xBF58476D1CE4E5B9ull;
   h ^= h >> 32;
-  return (int)(h % (unsigned long long)cap);
+  return (int)(h & (unsigned long long)(cap - 1));   // cap: a power of two
 }
 
 // Segment boxes of the pass, and the bounds of their centres and their largest edge:
@@ -237,8 +238,9 @@ __global__ __launch_bounds__(256) void i7_seg_bounds(Contact7 c, const double4 *
     }
     k[6] = okey(ext);
   }
-  // the model's largest edge: one atomic per wave when the wave holds one model
-  {
+  // batched models: the model's largest edge, one atomic per wave when the wave holds
+  // one model (one model: the largest edge of all, reduced below)
+  if (c.seg_model) {
     const int m0 = __shfl(model, 0, 64);
     const bool one = __ballot(l < c.nrtm && model != m0) == 0ull;
     if (one) {
@@ -309,6 +311,10 @@ __global__ __launch_bounds__(1024) void i7_bounds_grid(Contact7 c) {
 __global__ __launch_bounds__(256) void i7_seg_count(Contact7 c) {
   const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
   if (l >= c.nrtm) return;
+  if (c.stfm[l] == 0.0) {   // never a candidate
+    c.seg_cell[l] = -1;
+    return;
+  }
   const int model = c.seg_model ? c.seg_model[l] : 0;
   const double inv = 1.0 / model_cell(c, model);
   const float4 lo = c.seg_lo[l], hi = c.seg_hi[l];
@@ -324,7 +330,12 @@ __global__ __launch_bounds__(256) void i7_seg_place(Contact7 c) {
   const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
   if (l >= c.nrtm) return;
   const int cell = c.seg_cell[l];
-  c.cell_seg[c.cell_start[cell] + atomicAdd(&c.cell_fill[cell], 1)] = l;
+  if (cell < 0) return;
+  const int j = c.cell_start[cell] + atomicAdd(&c.cell_fill[cell], 1);
+  float4 lo = c.seg_lo[l];
+  lo.w = __int_as_float(l);
+  c.cell_lo[j] = lo;
+  c.cell_hi[j] = c.seg_hi[l];
 }
 
 // ------------------------------------------------------------- per pair --
@@ -428,13 +439,12 @@ __device__ __forceinline__ bool i7_dist(const Contact7 &c, const double4 *X4, co
 // Each secondary node: its candidate segments (I7TRI's box test, i7tri.F:530-566,
 // widened by TZ) in increasing segment order, and among them the ones it penetrates
 // (I7COR3 + I7DST3), kept in that order.
-__global__ __launch_bounds__(128) void i7_node_pass(Contact7 c, const double4 *X4,
-                                                   const double4 *V4) {
-  __shared__ int s_list[I7_NODE_CAP][128];   // [k][lane]: the lane's sorted candidates
-  const int i = (int)blockIdx.x * 128 + (int)threadIdx.x;
+// Each secondary node's candidates: the segments of the 2 x 2 x 2 cells around it whose
+// box widened by TZ holds it, but its own segments and other models' (found order).
+__global__ __launch_bounds__(256) void i7_node_search(Contact7 c, const double4 *X4) {
+  const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
   if (i >= c.nsn) return;
-  int *list = &s_list[0][threadIdx.x];
-  int cnt = 0, nkeep = 0;
+  int ncand = 0;
   if (c.stfns[i] != 0.0) {
     const int nn = c.nsv[i];
     const double4 x = X4[nn];
@@ -447,48 +457,66 @@ __global__ __launch_bounds__(128) void i7_node_pass(Contact7 c, const double4 *X
     const long long nx = ((x.x - c.grid[0]) * inv - (double)kx < 0.5) ? kx - 1 : kx + 1;
     const long long ny = ((x.y - c.grid[1]) * inv - (double)ky < 0.5) ? ky - 1 : ky + 1;
     const long long nz = ((x.z - c.grid[2]) * inv - (double)kz < 0.5) ? kz - 1 : kz + 1;
+    int32_t *cand = c.node_cand + (size_t)i * I7_NODE_CAP;
+    int seen[8];
+#pragma unroll
     for (int q = 0; q < 8; ++q) {
       const long long xx = (q & 1) ? nx : kx, y = (q & 2) ? ny : ky, z = (q & 4) ? nz : kz;
       const int cell = cell_hash(xx, y, z, model, c.cell_cap);
+      // two of the cells in one bucket: the bucket is visited once, so every segment
+      // is met once
+      bool dup = false;
+#pragma unroll
+      for (int p = 0; p < q; ++p) dup |= seen[p] == cell;
+      seen[q] = cell;
+      if (dup) continue;
       for (int j = c.cell_start[cell]; j < c.cell_start[cell + 1]; ++j) {
-        const int l = c.cell_seg[j];
-        if (c.stfm[l] == 0.0) continue;
-        const float4 lo = c.seg_lo[l], hi = c.seg_hi[l];
+        const float4 lo = c.cell_lo[j], hi = c.cell_hi[j];
         if (!(x.x > (double)lo.x - c.tz && x.x < (double)hi.x + c.tz &&
               x.y > (double)lo.y - c.tz && x.y < (double)hi.y + c.tz &&
               x.z > (double)lo.z - c.tz && x.z < (double)hi.z + c.tz))
           continue;
+        const int l = __float_as_int(lo.w);
         const int4 r = c.irect[l];
         if (nn == r.x || nn == r.y || nn == r.z || nn == r.w) continue;
         if (c.node_model && model != c.seg_model[l]) continue;
-        if (cnt < I7_NODE_CAP) {
-          // sorted insertion; a segment met again (two of the cells in one bucket) is
-          // kept once
-          int p = cnt;
-          while (p > 0 && list[(p - 1) * 128] > l) --p;
-          if (p > 0 && list[(p - 1) * 128] == l) continue;
-          for (int u = cnt; u > p; --u) list[u * 128] = list[(u - 1) * 128];
-          list[p * 128] = l;
-        }
-        ++cnt;
+        if (ncand < I7_NODE_CAP) cand[ncand] = l;
+        ++ncand;
       }
     }
-    if (cnt > I7_NODE_CAP) {
+    if (ncand > I7_NODE_CAP) {
       atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_NODE);
-      cnt = I7_NODE_CAP;
+      ncand = I7_NODE_CAP;
     }
-    for (int k = 0; k < cnt; ++k) {
-      I7Pair q;
-      const int l = list[k * 128];
-      if (!i7_dist(c, X4, V4, i, l, q)) continue;
-      if (nkeep < I7_KEEP_CAP) c.node_keep[(size_t)i * I7_KEEP_CAP + nkeep] = l;
-      ++nkeep;
+    if (ncand) atomicAdd(&c.counters[I7C_NCAND], ncand);
+  }
+  c.node_ncand[i] = ncand;
+}
+
+// I7COR3 + I7DST3 over each node's candidates; the penetrated segments in increasing
+// order.
+__global__ __launch_bounds__(128) void i7_node_filter(Contact7 c, const double4 *X4,
+                                                     const double4 *V4) {
+  const int i = (int)blockIdx.x * 128 + (int)threadIdx.x;
+  if (i >= c.nsn) return;
+  const int ncand = c.node_ncand[i];
+  const int32_t *cand = c.node_cand + (size_t)i * I7_NODE_CAP;
+  int32_t *keep = c.node_keep + (size_t)i * I7_KEEP_CAP;
+  int nkeep = 0;
+  for (int k = 0; k < ncand; ++k) {
+    const int l = cand[k];
+    I7Pair q;
+    if (!i7_dist(c, X4, V4, i, l, q)) continue;
+    if (nkeep < I7_KEEP_CAP) {
+      int p = nkeep;
+      for (; p > 0 && keep[p - 1] > l; --p) keep[p] = keep[p - 1];
+      keep[p] = l;
     }
-    if (nkeep > I7_KEEP_CAP) {
-      atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_KEEP);
-      nkeep = I7_KEEP_CAP;
-    }
-    if (cnt) atomicAdd(&c.counters[I7C_NCAND], cnt);
+    ++nkeep;
+  }
+  if (nkeep > I7_KEEP_CAP) {
+    atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_KEEP);
+    nkeep = I7_KEEP_CAP;
   }
   c.node_nkeep[i] = nkeep;
 }
@@ -761,7 +789,8 @@ hipError_t contact7_run(const Contact7 &c, const double4 *X4, const double4 *V4,
                            c.scan_tmp, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(i7_seg_place, g(c.nrtm, 256), dim3(256), 0, s, c);
-  hipLaunchKernelGGL(i7_node_pass, g(c.nsn, 128), dim3(128), 0, s, c, X4, V4);
+  hipLaunchKernelGGL(i7_node_search, g(c.nsn, 256), dim3(256), 0, s, c, X4);
+  hipLaunchKernelGGL(i7_node_filter, g(c.nsn, 128), dim3(128), 0, s, c, X4, V4);
   e = scan_excl(c.node_nkeep, c.node_kstart, nullptr, c.nsn, c.nsn, c.scan_tmp, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(i7_model_batches, dim3(1), dim3(1024), 0, s, c);

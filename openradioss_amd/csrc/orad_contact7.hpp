@@ -5,11 +5,13 @@
 // contact, IVIS2=0 damping, Coulomb friction in the total viscous formulation without
 // filtering (IFQ=0, MFROT=0, isotropic).  Every cycle, on the context's stream and in
 // its captured graph:
-//   search      segments binned on a uniform grid by their box centre; each secondary
-//               node lists the segments whose box, widened by TZ, holds it (I7TRI's
-//               test) in increasing segment order and keeps those it penetrates
-//               (I7COR3 + I7DST3), so the penetrating pairs come out in (secondary
-//               node, segment) order without a global sort;
+//   search      segments binned on a grid by their box centre (per-model cell size,
+//               cells hashed with the model); each secondary node tests the segments of
+//               the 2 x 2 x 2 cells around it with I7TRI's box test widened by TZ;
+//               a second kernel filters each node's candidates with I7COR3 + I7DST3
+//               and keeps the penetrated ones in increasing segment order, so the
+//               penetrating pairs come out in (secondary node, segment) order without
+//               a global sort;
 //   forces      one 128-lane workgroup per batch of NVSIZ = 128 penetrating pairs:
 //               I7FOR3 and I7ASS2's skyline layout (main node 1 of every pair, then
 //               nodes 2, 3, 4 where H /= 0, then the secondary nodes) in the batch's
@@ -48,7 +50,7 @@ struct Contact7 {
   double gap = 0.0, fric = 0.0, visc = 0.0, viscf = 0.0, tz = 0.0;
   double em10 = 0.0, em20 = 0.0, em30 = 0.0, zep4 = 0.0;   // constant_mod.F values
   // search grid over the segments' box centres, cells hashed with the model into
-  // cell_cap buckets
+  // cell_cap buckets (a power of two)
   int cell_cap = 0;
   unsigned long long *bounds = nullptr;  // [7] ordered bits: min xyz, max xyz, max extent
   unsigned long long *bpart = nullptr;   // [7 * nbound_blocks] the workgroups' bounds
@@ -59,7 +61,9 @@ struct Contact7 {
   int32_t *seg_cell = nullptr;           // [nrtm]
   int32_t *cell_count = nullptr, *cell_fill = nullptr;   // [cell_cap]
   int32_t *cell_start = nullptr;         // [cell_cap + 1]
-  int32_t *cell_seg = nullptr;           // [nrtm]
+  // the cells' segment boxes in cell order (a cell's in any order); cell_lo[j].w holds
+  // the segment's index as int bits.  Segments with STF = 0 are left out.
+  float4 *cell_lo = nullptr, *cell_hi = nullptr;   // [nrtm]
   // penetrating pairs: per secondary node its penetrated segments in increasing order;
   // pair r of the pass is node i's (r - node_kstart[i])-th, node_kstart[i] <= r
   int pair_cap = 0, nbatch_cap = 0;
@@ -72,6 +76,8 @@ struct Contact7 {
   int32_t *nsv_off = nullptr;            // [nmodel + 1]
   int32_t *model_bstart = nullptr;       // [nmodel + 1] first batch of each model
   unsigned long long *model_ext = nullptr;   // [nmodel] ordered bits: largest segment edge
+  int32_t *node_cand = nullptr;          // [nsn * I7_NODE_CAP] candidates, found order
+  int32_t *node_ncand = nullptr;         // [nsn]
   int32_t *node_keep = nullptr;          // [nsn * I7_KEEP_CAP]
   int32_t *node_nkeep = nullptr;         // [nsn]
   int32_t *node_kstart = nullptr;        // [nsn + 1]
