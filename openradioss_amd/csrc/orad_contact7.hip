@@ -144,6 +144,7 @@ __global__ __launch_bounds__(256) void i7_reset(Contact7 c) {
     c.cell_fill[i] = 0;
   }
   if (i < c.numnod) c.node_head[i] = -1;
+  if (i < c.nsn) c.node_ncand[i] = 0;
   if (i < c.nmodel) c.model_ext[i] = 0ull;
   if (i < 3) {
     c.bounds[i] = ~0ull;
@@ -440,57 +441,50 @@ __device__ __forceinline__ bool i7_dist(const Contact7 &c, const double4 *X4, co
 // widened by TZ) in increasing segment order, and among them the ones it penetrates
 // (I7COR3 + I7DST3), kept in that order.
 // Each secondary node's candidates: the segments of the 2 x 2 x 2 cells around it whose
-// box widened by TZ holds it, but its own segments and other models' (found order).
+// box widened by TZ holds it, but its own segments and other models'.  Eight lanes per
+// node, one cell each; a lane appends to the node's row (node_ncand cleared by i7_reset),
+// so a row is in no particular order -- the filter sorts what it keeps.
 __global__ __launch_bounds__(256) void i7_node_search(Contact7 c, const double4 *X4) {
-  const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
-  if (i >= c.nsn) return;
-  int ncand = 0;
-  if (c.stfns[i] != 0.0) {
-    const int nn = c.nsv[i];
-    const double4 x = X4[nn];
-    const int model = c.node_model ? c.node_model[nn] : 0;
-    const double inv = 1.0 / model_cell(c, model);
-    const long long kx = cell_coord(x.x, c.grid[0], inv);
-    const long long ky = cell_coord(x.y, c.grid[1], inv);
-    const long long kz = cell_coord(x.z, c.grid[2], inv);
-    // the neighbour on the side of the node's half of its cell, per axis
-    const long long nx = ((x.x - c.grid[0]) * inv - (double)kx < 0.5) ? kx - 1 : kx + 1;
-    const long long ny = ((x.y - c.grid[1]) * inv - (double)ky < 0.5) ? ky - 1 : ky + 1;
-    const long long nz = ((x.z - c.grid[2]) * inv - (double)kz < 0.5) ? kz - 1 : kz + 1;
-    int32_t *cand = c.node_cand + (size_t)i * I7_NODE_CAP;
-    int seen[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const long long xx = (q & 1) ? nx : kx, y = (q & 2) ? ny : ky, z = (q & 4) ? nz : kz;
-      const int cell = cell_hash(xx, y, z, model, c.cell_cap);
-      // two of the cells in one bucket: the bucket is visited once, so every segment
-      // is met once
-      bool dup = false;
-#pragma unroll
-      for (int p = 0; p < q; ++p) dup |= seen[p] == cell;
-      seen[q] = cell;
-      if (dup) continue;
-      for (int j = c.cell_start[cell]; j < c.cell_start[cell + 1]; ++j) {
-        const float4 lo = c.cell_lo[j], hi = c.cell_hi[j];
-        if (!(x.x > (double)lo.x - c.tz && x.x < (double)hi.x + c.tz &&
-              x.y > (double)lo.y - c.tz && x.y < (double)hi.y + c.tz &&
-              x.z > (double)lo.z - c.tz && x.z < (double)hi.z + c.tz))
-          continue;
-        const int l = __float_as_int(lo.w);
-        const int4 r = c.irect[l];
-        if (nn == r.x || nn == r.y || nn == r.z || nn == r.w) continue;
-        if (c.node_model && model != c.seg_model[l]) continue;
-        if (ncand < I7_NODE_CAP) cand[ncand] = l;
-        ++ncand;
-      }
-    }
-    if (ncand > I7_NODE_CAP) {
-      atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_NODE);
-      ncand = I7_NODE_CAP;
-    }
-    if (ncand) atomicAdd(&c.counters[I7C_NCAND], ncand);
+  const long t = (long)blockIdx.x * 256 + (long)threadIdx.x;
+  const int i = (int)(t >> 3), q = (int)(t & 7);
+  if (i >= c.nsn || c.stfns[i] == 0.0) return;
+  const int nn = c.nsv[i];
+  const double4 x = X4[nn];
+  const int model = c.node_model ? c.node_model[nn] : 0;
+  const double inv = 1.0 / model_cell(c, model);
+  const long long kx = cell_coord(x.x, c.grid[0], inv);
+  const long long ky = cell_coord(x.y, c.grid[1], inv);
+  const long long kz = cell_coord(x.z, c.grid[2], inv);
+  // the neighbour on the side of the node's half of its cell, per axis
+  const long long nx = ((x.x - c.grid[0]) * inv - (double)kx < 0.5) ? kx - 1 : kx + 1;
+  const long long ny = ((x.y - c.grid[1]) * inv - (double)ky < 0.5) ? ky - 1 : ky + 1;
+  const long long nz = ((x.z - c.grid[2]) * inv - (double)kz < 0.5) ? kz - 1 : kz + 1;
+  const int cell = cell_hash((q & 1) ? nx : kx, (q & 2) ? ny : ky, (q & 4) ? nz : kz, model,
+                             c.cell_cap);
+  // two of the cells in one bucket: the lowest such lane visits it, so every segment
+  // is met once
+  for (int p = 0; p < q; ++p)
+    if (cell_hash((p & 1) ? nx : kx, (p & 2) ? ny : ky, (p & 4) ? nz : kz, model,
+                  c.cell_cap) == cell)
+      return;
+  int32_t *cand = c.node_cand + (size_t)i * I7_NODE_CAP;
+  int found = 0;
+  for (int j = c.cell_start[cell]; j < c.cell_start[cell + 1]; ++j) {
+    const float4 lo = c.cell_lo[j], hi = c.cell_hi[j];
+    if (!(x.x > (double)lo.x - c.tz && x.x < (double)hi.x + c.tz &&
+          x.y > (double)lo.y - c.tz && x.y < (double)hi.y + c.tz &&
+          x.z > (double)lo.z - c.tz && x.z < (double)hi.z + c.tz))
+      continue;
+    const int l = __float_as_int(lo.w);
+    const int4 r = c.irect[l];
+    if (nn == r.x || nn == r.y || nn == r.z || nn == r.w) continue;
+    if (c.node_model && model != c.seg_model[l]) continue;
+    const int k = atomicAdd(&c.node_ncand[i], 1);
+    if (k < I7_NODE_CAP) cand[k] = l;
+    else atomicOr(&c.counters[I7C_OVERFLOW], I7_OVF_NODE);
+    ++found;
   }
-  c.node_ncand[i] = ncand;
+  if (found) atomicAdd(&c.counters[I7C_NCAND], found);
 }
 
 // I7COR3 + I7DST3 over each node's candidates; the penetrated segments in increasing
@@ -499,7 +493,7 @@ __global__ __launch_bounds__(128) void i7_node_filter(Contact7 c, const double4 
                                                      const double4 *V4) {
   const int i = (int)blockIdx.x * 128 + (int)threadIdx.x;
   if (i >= c.nsn) return;
-  const int ncand = c.node_ncand[i];
+  const int ncand = min(c.node_ncand[i], I7_NODE_CAP);
   const int32_t *cand = c.node_cand + (size_t)i * I7_NODE_CAP;
   int32_t *keep = c.node_keep + (size_t)i * I7_KEEP_CAP;
   int nkeep = 0;
@@ -780,6 +774,7 @@ hipError_t contact7_run(const Contact7 &c, const double4 *X4, const double4 *V4,
   long nreset = c.cell_cap;
   if (c.numnod > nreset) nreset = c.numnod;
   if (c.nmodel > nreset) nreset = c.nmodel;
+  if (c.nsn > nreset) nreset = c.nsn;
   if (nreset < 8) nreset = 8;
   hipLaunchKernelGGL(i7_reset, g(nreset, 256), dim3(256), 0, s, c);
   hipLaunchKernelGGL(i7_seg_bounds, dim3(c.nbound_blocks), dim3(256), 0, s, c, X4);
@@ -789,7 +784,7 @@ hipError_t contact7_run(const Contact7 &c, const double4 *X4, const double4 *V4,
                            c.scan_tmp, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(i7_seg_place, g(c.nrtm, 256), dim3(256), 0, s, c);
-  hipLaunchKernelGGL(i7_node_search, g(c.nsn, 256), dim3(256), 0, s, c, X4);
+  hipLaunchKernelGGL(i7_node_search, g(8L * c.nsn, 256), dim3(256), 0, s, c, X4);
   hipLaunchKernelGGL(i7_node_filter, g(c.nsn, 128), dim3(128), 0, s, c, X4, V4);
   e = scan_excl(c.node_nkeep, c.node_kstart, nullptr, c.nsn, c.nsn, c.scan_tmp, s);
   if (e != hipSuccess) return e;
