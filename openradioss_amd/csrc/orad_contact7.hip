@@ -197,16 +197,20 @@ __device__ __forceinline__ long long cell_coord(double x, double o, double inv) 
   return (long long)f;
 }
 
+// Bucket of cell (kx, ky, kz) of a model: blocks of 4 x 4 x 4 cells are hashed with the
+// model, the cells of a block are consecutive buckets, so neighbouring cells (and the
+// segments binned in them) lie close in memory.  cap: a power of two, >= 64.
 __device__ __forceinline__ int cell_hash(long long kx, long long ky, long long kz, int model,
                                          int cap) {
-  unsigned long long h = (unsigned long long)kx * 0x9E3779B97F4A7C15ull;
-  h ^= (unsigned long long)ky * 0xC2B2AE3D27D4EB4Full;
-  h ^= (unsigned long long)kz * 0x165667B19E3779F9ull;
+  unsigned long long h = (unsigned long long)(kx >> 2) * 0x9E3779B97F4A7C15ull;
+  h ^= (unsigned long long)(ky >> 2) * 0xC2B2AE3D27D4EB4Full;
+  h ^= (unsigned long long)(kz >> 2) * 0x165667B19E3779F9ull;
   h ^= (unsigned long long)(unsigned)model * 0x27D4EB2F165667C5ull;
   h ^= h >> 29;
   h *= 0xBF58476D1CE4E5B9ull;
   h ^= h >> 32;
-  return (int)(h & (unsigned long long)(cap - 1));   // cap: a power of two
+  const int in_block = (int)(((kz & 3) << 4) | ((ky & 3) << 2) | (kx & 3));
+  return (int)((h & (unsigned long long)((cap >> 6) - 1)) << 6) | in_block;
 }
 
 // Segment boxes of the pass, and the bounds of their centres and their largest edge:
