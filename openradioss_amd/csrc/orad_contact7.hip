@@ -465,12 +465,9 @@ __global__ __launch_bounds__(256) void i7_node_search(Contact7 c, const double4 
   const long long nz = ((x.z - c.grid[2]) * inv - (double)kz < 0.5) ? kz - 1 : kz + 1;
   const int cell = cell_hash((q & 1) ? nx : kx, (q & 2) ? ny : ky, (q & 4) ? nz : kz, model,
                              c.cell_cap);
-  // two of the cells in one bucket: the lowest such lane visits it, so every segment
-  // is met once
-  for (int p = 0; p < q; ++p)
-    if (cell_hash((p & 1) ? nx : kx, (p & 2) ? ny : ky, (p & 4) ? nz : kz, model,
-                  c.cell_cap) == cell)
-      return;
+  // The eight cells lie in eight different buckets (on each axis the two coordinates
+  // differ by one, so the cells' places in their 4 x 4 x 4 blocks differ), so every
+  // segment is met once.
   int32_t *cand = c.node_cand + (size_t)i * I7_NODE_CAP;
   int found = 0;
   for (int j = c.cell_start[cell]; j < c.cell_start[cell + 1]; ++j) {
