@@ -303,8 +303,9 @@ int orad_autotune(orad_ctx *ctx, int cycles, orad_launch_t *best, double *best_m
  * (as the starter writes INTBUF_TAB%STFM/STFNS), params = GAP, FRIC, VISs (critical
  * damping, the reference default 0.05), VISf (friction damping); cand_cap bounds the
  * penetrating pairs of one pass (<= 0: 8 per secondary node).  Every cycle of
- * orad_run_cycles then searches the candidates on the device (I7TRI's box test widened
- * by GAP), computes I7COR3/I7DST3/I7FOR3/I7ASS2 for the penetrating pairs in batches
+ * orad_run_cycles then takes the candidates on the device (I7TRI's box test widened by
+ * 1.5 GAP, searched again once a node has moved GAP / 4 since the last search, as the
+ * reference re-sorts when its margin TZINF - GAP is used up), computes I7COR3/I7DST3/I7FOR3/I7ASS2 for the penetrating pairs in batches
  * of 128, lowers DT2T with the contact time step and adds each node's skyline entries
  * after its element slots, in skyline order.  The entries are the reference routines'
  * bit for bit on the same candidate batches; the batches are the penetrating pairs in

@@ -279,7 +279,8 @@ static void free_contact(orad_ctx *c) {
                   (void *)k->sky_node, (void *)k->sky_next, (void *)k->batch_cnt,
                   (void *)k->batch_e, (void *)k->energy, (void *)k->node_head,
                   (void *)k->scan_tmp, (void *)k->seg_model, (void *)k->nsv_off,
-                  (void *)k->model_bstart, (void *)k->model_ext})
+                  (void *)k->model_bstart, (void *)k->model_ext,
+                  (void *)k->x_search, (void *)k->dpart})
     dfree(p);
   delete k;
   c->ci = nullptr;
@@ -1812,6 +1813,10 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
   k->visc = visc;
   k->viscf = viscf;
   k->tz = gap * 1.000001;
+  // candidates searched with half a gap of margin and kept until a node has moved a
+  // quarter gap (inter_check_sort.F:103-106 re-sorts when the margin TZINF - GAP is used up)
+  k->margin = 0.5 * gap;
+  k->tzs = k->tz + k->margin;
   contact_constants(*k);
   k->cell_cap = (int)cells;
   k->pair_cap = (int)cap;
@@ -1857,6 +1862,9 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
   al(&k->nsv_off, (size_t)(nmodel + 1) * 4);
   al(&k->model_bstart, (size_t)(nmodel + 1) * 4);
   al(&k->model_ext, (size_t)nmodel * 8);
+  k->ndisp_blocks = std::max(1, (nn + 255) / 256);
+  al(&k->x_search, (size_t)nn * sizeof(double4));
+  al(&k->dpart, (size_t)k->ndisp_blocks * 8);
   if (c->batched) al(&k->seg_model, (size_t)nrtm * 4);
   if (e != hipSuccess) {
     free_contact(c);
@@ -1885,6 +1893,8 @@ int orad_set_contact7(orad_ctx *c, int nsn, const int *nsv, int nrtm, const int 
   HIPCHK(c, hipMemset(k->counters, 0, I7C_N * 4));
   HIPCHK(c, hipMemset(k->energy, 0, 3 * 8));
   HIPCHK(c, hipMemset(k->node_head, 0xFF, (size_t)nn * 4));      // empty lists (-1)
+  HIPCHK(c, hipMemset(k->x_search, 0xFF, (size_t)nn * sizeof(double4)));   // NaN: no search
+  HIPCHK(c, hipMemset(k->node_ncand, 0, (size_t)nsn * 4));
   HIPCHK(c, hipDeviceSynchronize());
   return ORAD_OK;
 }

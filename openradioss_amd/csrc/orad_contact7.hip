@@ -137,13 +137,67 @@ __device__ __forceinline__ void seg_box(const Contact7 &c, const double4 *X4, in
 
 
 // ------------------------------------------------------------ search grid --
-__global__ __launch_bounds__(256) void i7_reset(Contact7 c) {
+// Every pass: list heads, counters, and each workgroup's largest coordinate change of a
+// node since the last search.
+__global__ __launch_bounds__(256) void i7_reset(Contact7 c, const double4 *X4) {
+  __shared__ unsigned long long s_d[4];
+  const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  unsigned long long d = 0ull;
+  if (i < c.numnod) {
+    c.node_head[i] = -1;
+    const double4 x = X4[i], o = c.x_search[i];
+    // NaN (no search yet, or non-finite coordinates) orders above every number
+    const double m = fmax(fmax(fabs(x.x - o.x), fabs(x.y - o.y)), fabs(x.z - o.z));
+    d = okey(m == m ? m : __longlong_as_double(0x7FF8000000000000ll));
+  }
+  if (i < I7C_N && i != I7C_OVERFLOW && i != I7C_NCAND && i != I7C_SEARCH && i != I7C_NCELL)
+    c.counters[i] = 0;   // the overflow flag is sticky; NCAND is the last search's
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long b = __shfl_xor(d, o, 64);
+    d = b > d ? b : d;
+  }
+  if ((threadIdx.x & 63) == 0) s_d[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0 && (int)blockIdx.x < c.ndisp_blocks) {
+    unsigned long long v = s_d[0];
+    for (int w = 1; w < 4; ++w) v = s_d[w] > v ? s_d[w] : v;
+    c.dpart[blockIdx.x] = v;
+  }
+}
+
+// One workgroup: does this pass search?
+__global__ __launch_bounds__(1024) void i7_decide(Contact7 c) {
+  __shared__ unsigned long long s_d[16];
+  unsigned long long d = 0ull;
+  for (int b = threadIdx.x; b < c.ndisp_blocks; b += 1024) d = c.dpart[b] > d ? c.dpart[b] : d;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long b = __shfl_xor(d, o, 64);
+    d = b > d ? b : d;
+  }
+  if ((threadIdx.x & 63) == 0) s_d[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 16; ++w) d = s_d[w] > d ? s_d[w] : d;
+    const double m = odec(d);
+    // with every node within margin / 2 of its place at the last search, a node and a
+    // segment box have closed in by less than the margin on every axis
+    const int search = !(2.0 * m * (1.0 + 1.0e-6) < c.margin);
+    c.counters[I7C_SEARCH] = search;
+    c.counters[I7C_NCELL] = search ? c.cell_cap : 0;
+  }
+}
+
+// A searching pass: cell counts, candidate rows, bounds, and the positions it searches.
+__global__ __launch_bounds__(256) void i7_search_reset(Contact7 c, const double4 *X4) {
+  if (!c.counters[I7C_SEARCH]) return;
   const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
   if (i < c.cell_cap) {
     c.cell_count[i] = 0;
     c.cell_fill[i] = 0;
   }
-  if (i < c.numnod) c.node_head[i] = -1;
+  if (i < c.numnod) c.x_search[i] = X4[i];
   if (i < c.nsn) c.node_ncand[i] = 0;
   if (i < c.nmodel) c.model_ext[i] = 0ull;
   if (i < 3) {
@@ -151,7 +205,7 @@ __global__ __launch_bounds__(256) void i7_reset(Contact7 c) {
     c.bounds[3 + i] = 0ull;
   }
   if (i == 3) c.bounds[6] = 0ull;
-  if (i < I7C_N && i != I7C_OVERFLOW) c.counters[i] = 0;   // the overflow flag is sticky
+  if (i == 4) c.counters[I7C_NCAND] = 0;
 }
 
 // Cell size (per model): the model's largest segment box edge plus 2 TZ, so the centre
@@ -169,7 +223,7 @@ __device__ __forceinline__ I7Grid i7_grid(const Contact7 &c) {
   for (int d = 0; d < 3; ++d) g.o[d] = c.nrtm > 0 ? odec(c.bounds[d]) : 0.0;
   // padded by 1e-6 so a candidate's centre lies strictly within half a cell of the
   // node on every axis, with room for the rounding of the cell coordinates
-  double s = ((c.nrtm > 0 ? odec(c.bounds[6]) : 0.0) + 2.0 * c.tz) * (1.0 + 1.0e-6);
+  double s = ((c.nrtm > 0 ? odec(c.bounds[6]) : 0.0) + 2.0 * c.tzs) * (1.0 + 1.0e-6);
   if (!(s > 0.0)) s = 1.0;
   if (!(s < 1.0e300) || !(g.o[0] == g.o[0] && g.o[1] == g.o[1] && g.o[2] == g.o[2])) {
     // non-finite coordinates: one cell per model
@@ -184,7 +238,7 @@ __device__ __forceinline__ I7Grid i7_grid(const Contact7 &c) {
 // Model m's cell size: its largest segment edge plus 2 TZ, padded as above.
 __device__ __forceinline__ double model_cell(const Contact7 &c, int m) {
   const unsigned long long e = c.seg_model ? c.model_ext[m] : c.bounds[6];
-  double s = (odec(e) + 2.0 * c.tz) * (1.0 + 1.0e-6);
+  double s = (odec(e) + 2.0 * c.tzs) * (1.0 + 1.0e-6);
   if (!(s > 0.0)) s = 1.0;
   if (!(s < 1.0e300) || !(c.grid[3] < 1.0e300)) s = 1.0e300;
   return s;
@@ -217,6 +271,7 @@ __device__ __forceinline__ int cell_hash(long long kx, long long ky, long long k
 // reduced over each wave by shuffles, then over the workgroup, then one atomic per bound
 // and workgroup.
 __global__ __launch_bounds__(256) void i7_seg_bounds(Contact7 c, const double4 *X4) {
+  if (!c.counters[I7C_SEARCH]) return;
   __shared__ unsigned long long s_b[7][4];
   const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
   unsigned long long k[7] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull, 0ull};
@@ -282,6 +337,7 @@ __global__ __launch_bounds__(256) void i7_seg_bounds(Contact7 c, const double4 *
 
 // The workgroups' bounds reduced by one workgroup, then the grid of the pass.
 __global__ __launch_bounds__(1024) void i7_bounds_grid(Contact7 c) {
+  if (!c.counters[I7C_SEARCH]) return;
   __shared__ unsigned long long s_b[7][16];
   unsigned long long k[7] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull, 0ull};
   for (int b = threadIdx.x; b < c.nbound_blocks; b += 1024)
@@ -314,6 +370,7 @@ __global__ __launch_bounds__(1024) void i7_bounds_grid(Contact7 c) {
 }
 
 __global__ __launch_bounds__(256) void i7_seg_count(Contact7 c) {
+  if (!c.counters[I7C_SEARCH]) return;
   const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
   if (l >= c.nrtm) return;
   if (c.stfm[l] == 0.0) {   // never a candidate
@@ -332,6 +389,7 @@ __global__ __launch_bounds__(256) void i7_seg_count(Contact7 c) {
 }
 
 __global__ __launch_bounds__(256) void i7_seg_place(Contact7 c) {
+  if (!c.counters[I7C_SEARCH]) return;
   const int l = (int)blockIdx.x * 256 + (int)threadIdx.x;
   if (l >= c.nrtm) return;
   const int cell = c.seg_cell[l];
@@ -446,9 +504,10 @@ __device__ __forceinline__ bool i7_dist(const Contact7 &c, const double4 *X4, co
 // (I7COR3 + I7DST3), kept in that order.
 // Each secondary node's candidates: the segments of the 2 x 2 x 2 cells around it whose
 // box widened by TZ holds it, but its own segments and other models'.  Eight lanes per
-// node, one cell each; a lane appends to the node's row (node_ncand cleared by i7_reset),
+// node, one cell each; a lane appends to the node's row (node_ncand cleared by i7_search_reset),
 // so a row is in no particular order -- the filter sorts what it keeps.
 __global__ __launch_bounds__(256) void i7_node_search(Contact7 c, const double4 *X4) {
+  if (!c.counters[I7C_SEARCH]) return;
   const long t = (long)blockIdx.x * 256 + (long)threadIdx.x;
   const int i = (int)(t >> 3), q = (int)(t & 7);
   if (i >= c.nsn || c.stfns[i] == 0.0) return;
@@ -472,9 +531,9 @@ __global__ __launch_bounds__(256) void i7_node_search(Contact7 c, const double4 
   int found = 0;
   for (int j = c.cell_start[cell]; j < c.cell_start[cell + 1]; ++j) {
     const float4 lo = c.cell_lo[j], hi = c.cell_hi[j];
-    if (!(x.x > (double)lo.x - c.tz && x.x < (double)hi.x + c.tz &&
-          x.y > (double)lo.y - c.tz && x.y < (double)hi.y + c.tz &&
-          x.z > (double)lo.z - c.tz && x.z < (double)hi.z + c.tz))
+    if (!(x.x > (double)lo.x - c.tzs && x.x < (double)hi.x + c.tzs &&
+          x.y > (double)lo.y - c.tzs && x.y < (double)hi.y + c.tzs &&
+          x.z > (double)lo.z - c.tzs && x.z < (double)hi.z + c.tzs))
       continue;
     const int l = __float_as_int(lo.w);
     const int4 r = c.irect[l];
@@ -772,16 +831,18 @@ hipError_t contact7_run(const Contact7 &c, const double4 *X4, const double4 *V4,
     const long b = (n + blk - 1) / blk;
     return dim3((unsigned)(b > 0 ? b : 1));
   };
-  long nreset = c.cell_cap;
-  if (c.numnod > nreset) nreset = c.numnod;
-  if (c.nmodel > nreset) nreset = c.nmodel;
-  if (c.nsn > nreset) nreset = c.nsn;
-  if (nreset < 8) nreset = 8;
-  hipLaunchKernelGGL(i7_reset, g(nreset, 256), dim3(256), 0, s, c);
+  long nsr = c.cell_cap;
+  if (c.numnod > nsr) nsr = c.numnod;
+  if (c.nmodel > nsr) nsr = c.nmodel;
+  if (c.nsn > nsr) nsr = c.nsn;
+  if (nsr < 8) nsr = 8;
+  hipLaunchKernelGGL(i7_reset, g(c.numnod > 8 ? c.numnod : 8, 256), dim3(256), 0, s, c, X4);
+  hipLaunchKernelGGL(i7_decide, dim3(1), dim3(1024), 0, s, c);
+  hipLaunchKernelGGL(i7_search_reset, g(nsr, 256), dim3(256), 0, s, c, X4);
   hipLaunchKernelGGL(i7_seg_bounds, dim3(c.nbound_blocks), dim3(256), 0, s, c, X4);
   hipLaunchKernelGGL(i7_bounds_grid, dim3(1), dim3(1024), 0, s, c);
   hipLaunchKernelGGL(i7_seg_count, g(c.nrtm, 256), dim3(256), 0, s, c);
-  hipError_t e = scan_excl(c.cell_count, c.cell_start, nullptr, c.cell_cap, c.cell_cap,
+  hipError_t e = scan_excl(c.cell_count, c.cell_start, c.counters + I7C_NCELL, 0, c.cell_cap,
                            c.scan_tmp, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(i7_seg_place, g(c.nrtm, 256), dim3(256), 0, s, c);

@@ -5,6 +5,9 @@
 // contact, IVIS2=0 damping, Coulomb friction in the total viscous formulation without
 // filtering (IFQ=0, MFROT=0, isotropic).  Every cycle, on the context's stream and in
 // its captured graph:
+//   check       the largest coordinate change of any node since the last search; a
+//               new search when twice it reaches the margin (so every pair within TZ
+//               now was within TZ + margin then), otherwise the last candidates stand;
 //   search      segments binned on a grid by their box centre (per-model cell size,
 //               cells hashed with the model); each secondary node tests the segments of
 //               the 2 x 2 x 2 cells around it with I7TRI's box test widened by TZ;
@@ -36,7 +39,9 @@ constexpr int I7_KEEP_CAP = 16;           // penetrated segments of one secondar
 // counters[] of a pass
 enum {
   I7C_NCAND = 0, I7C_NPEN = 1, I7C_OVERFLOW = 2, I7C_NSKY = 3, I7C_DONE = 4, I7C_NBATCH = 5,
-  I7C_N = 6
+  I7C_SEARCH = 6,   // 1: this pass searches
+  I7C_NCELL = 7,    // cells to scan this pass (cell_cap or 0)
+  I7C_N = 8
 };
 // overflow bits (sticky until the interface is set again)
 enum { I7_OVF_NODE = 1, I7_OVF_PAIRS = 2, I7_OVF_KEEP = 4 };
@@ -49,6 +54,12 @@ struct Contact7 {
   double *stfns = nullptr;      // [nsn] secondary factor STFN, negated on deactivation
   double gap = 0.0, fric = 0.0, visc = 0.0, viscf = 0.0, tz = 0.0;
   double em10 = 0.0, em20 = 0.0, em30 = 0.0, zep4 = 0.0;   // constant_mod.F values
+  // search margin: the candidates are searched with the reach TZS = TZ + margin and kept
+  // until a node has moved by margin / 2 (largest coordinate change) since that search
+  double margin = 0.0, tzs = 0.0;
+  double4 *x_search = nullptr;           // [numnod] positions at the last search (NaN: none)
+  unsigned long long *dpart = nullptr;   // [ndisp_blocks] the workgroups' largest move
+  int ndisp_blocks = 0;
   // search grid over the segments' box centres, cells hashed with the model into
   // cell_cap buckets (a power of two)
   int cell_cap = 0;
