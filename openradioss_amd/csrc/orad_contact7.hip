@@ -547,20 +547,52 @@ __global__ __launch_bounds__(256) void i7_node_search(Contact7 c, const double4 
   if (found) atomicAdd(&c.counters[I7C_NCAND], found);
 }
 
-// I7COR3 + I7DST3 over each node's candidates; the penetrated segments in increasing
-// order.
-__global__ __launch_bounds__(128) void i7_node_filter(Contact7 c, const double4 *X4,
+// I7COR3 + I7DST3 over the candidates: one lane per node reads its count, the wave's
+// candidates are dealt out over its 64 lanes (so a wave with one busy node and one with
+// many cost alike), the penetrated ones marked in a 64-bit mask per node in LDS, then
+// each node's lane lists them in increasing segment order.
+__global__ __launch_bounds__(256) void i7_node_filter(Contact7 c, const double4 *X4,
                                                      const double4 *V4) {
-  const int i = (int)blockIdx.x * 128 + (int)threadIdx.x;
-  if (i >= c.nsn) return;
-  const int ncand = min(c.node_ncand[i], I7_NODE_CAP);
+  __shared__ unsigned long long s_mask[256];
+  __shared__ int s_start[256];
+  const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  const int lane = threadIdx.x & 63, w0 = threadIdx.x & ~63;
+  const bool live = i < c.nsn;
+  const int n = live ? min(c.node_ncand[i], I7_NODE_CAP) : 0;
+  int x = n;   // inclusive scan of the counts over the wave
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  const int total = __shfl(x, 63, 64);
+  s_start[threadIdx.x] = x - n;
+  s_mask[threadIdx.x] = 0ull;
+  __syncthreads();
+  for (int k = lane; k < total; k += 64) {
+    // the lane whose range holds k: the last one starting at or before k
+    int lo = 0, hi = 63;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_start[w0 + mid] <= k) lo = mid;
+      else hi = mid - 1;
+    }
+    const int j = k - s_start[w0 + lo];
+    const int ni = (int)blockIdx.x * 256 + w0 + lo;
+    I7Pair p;
+    if (i7_dist(c, X4, V4, ni, c.node_cand[(size_t)ni * I7_NODE_CAP + j], p))
+      atomicOr(&s_mask[w0 + lo], 1ull << j);
+  }
+  __syncthreads();
+  if (!live) return;
+  unsigned long long mask = s_mask[threadIdx.x];
   const int32_t *cand = c.node_cand + (size_t)i * I7_NODE_CAP;
   int32_t *keep = c.node_keep + (size_t)i * I7_KEEP_CAP;
   int nkeep = 0;
-  for (int k = 0; k < ncand; ++k) {
-    const int l = cand[k];
-    I7Pair q;
-    if (!i7_dist(c, X4, V4, i, l, q)) continue;
+  while (mask) {
+    const int j = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    const int l = cand[j];
     if (nkeep < I7_KEEP_CAP) {
       int p = nkeep;
       for (; p > 0 && keep[p - 1] > l; --p) keep[p] = keep[p - 1];
@@ -847,7 +879,7 @@ hipError_t contact7_run(const Contact7 &c, const double4 *X4, const double4 *V4,
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(i7_seg_place, g(c.nrtm, 256), dim3(256), 0, s, c);
   hipLaunchKernelGGL(i7_node_search, g(8L * c.nsn, 256), dim3(256), 0, s, c, X4);
-  hipLaunchKernelGGL(i7_node_filter, g(c.nsn, 128), dim3(128), 0, s, c, X4, V4);
+  hipLaunchKernelGGL(i7_node_filter, g(c.nsn, 256), dim3(256), 0, s, c, X4, V4);
   e = scan_excl(c.node_nkeep, c.node_kstart, nullptr, c.nsn, c.nsn, c.scan_tmp, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(i7_model_batches, dim3(1), dim3(1024), 0, s, c);
