@@ -178,6 +178,36 @@ def test_contact_group_path_adds_the_entries_once():
     assert np.array_equal(a2, a0)            # the entries are added once
 
 
+def test_contact_moved_nodes_are_searched_again():
+    """The last search's candidates stand while every node stays within GAP/4 of its place
+    then; a larger move (the engine's new X through orad_update_xv) searches again, so the
+    pass finds the pairs an every-cycle search finds (the oracle's on the same X)."""
+    d = dk.shell_plates_contact(n=12, m=6)
+    c = d["contact"]
+    nsv = np.asarray(c["nsv"])
+    low = np.unique(np.asarray(c["irect"]))
+    k = po.Contact7(dict(nsv=nsv, irect=c["irect"], stfm=c["stfm"], stfns=c["stfns"],
+                         gap=c["gap"], fric=c.get("fric", 0.0), visc=c.get("visc", 0.05),
+                         viscf=c.get("viscf", 0.0)))
+    x0 = np.asarray(d["x"], float).copy()
+    v = np.zeros_like(x0)
+    s = en.ShellSolver(d, en.FP64)
+    try:
+        got = []
+        for dz in (0.0, x0[nsv, 2].min() - x0[low, 2].max() - 0.5 * c["gap"], 0.1 * c["gap"]):
+            x0[nsv, 2] -= dz
+            s.update_xv(x0, v, np.zeros_like(x0))
+            s.contact7(1.0e-7)
+            cn, ce = k.candidates(x0)
+            exp = k.forces(x0, v, d["ms"], cn, ce, 1.0e-7)["npen"]
+            got.append((s.contact7_state()[1]["npen"], exp))
+    finally:
+        s.close()
+    assert got[0][0] == got[0][1]
+    assert got[1][1] > 0 and got[1][0] == got[1][1]      # moved by more than GAP/4
+    assert got[2][1] > 0 and got[2][0] == got[2][1]      # moved by less: the kept candidates
+
+
 def test_contact_capacity_overflow_is_reported():
     d = dk.shell_plates_contact(n=12, m=6)
     d["contact"]["cand_cap"] = 8
