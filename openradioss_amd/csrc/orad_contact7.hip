@@ -208,9 +208,9 @@ __global__ __launch_bounds__(256) void i7_search_reset(Contact7 c, const double4
   if (i == 4) c.counters[I7C_NCAND] = 0;
 }
 
-// Cell size (per model): the model's largest segment box edge plus 2 TZ, so the centre
-// of every segment whose widened box holds a node lies within half a cell of the node on
-// each axis: in the node's cell or the neighbour on the side of the node's half,
+// Cell size (per model): the model's largest segment box edge plus 2 TZS, so the centre
+// of every segment whose box widened by TZS holds a node lies within half a cell of the
+// node on each axis: in the node's cell or the neighbour on the side of the node's half,
 // 2 x 2 x 2 cells.  Cells are
 // hashed with the model into cell_cap buckets: models that overlap in space (batched
 // independent models) share no bucket but by collision, and the grid needs no extent.
@@ -235,7 +235,7 @@ __device__ __forceinline__ I7Grid i7_grid(const Contact7 &c) {
   return g;
 }
 
-// Model m's cell size: its largest segment edge plus 2 TZ, padded as above.
+// Model m's cell size: its largest segment edge plus 2 TZS, padded as above.
 __device__ __forceinline__ double model_cell(const Contact7 &c, int m) {
   const unsigned long long e = c.seg_model ? c.model_ext[m] : c.bounds[6];
   double s = (odec(e) + 2.0 * c.tzs) * (1.0 + 1.0e-6);
@@ -499,13 +499,11 @@ __device__ __forceinline__ bool i7_dist(const Contact7 &c, const double4 *X4, co
   return pene2 != 0.0 && q.stif != 0.0;
 }
 
-// Each secondary node: its candidate segments (I7TRI's box test, i7tri.F:530-566,
-// widened by TZ) in increasing segment order, and among them the ones it penetrates
-// (I7COR3 + I7DST3), kept in that order.
-// Each secondary node's candidates: the segments of the 2 x 2 x 2 cells around it whose
-// box widened by TZ holds it, but its own segments and other models'.  Eight lanes per
-// node, one cell each; a lane appends to the node's row (node_ncand cleared by i7_search_reset),
-// so a row is in no particular order -- the filter sorts what it keeps.
+// Each secondary node's candidates (I7TRI's box test, i7tri.F:530-566, widened by
+// TZS = TZ + margin): the segments of the 2 x 2 x 2 cells around it whose box widened by
+// TZS holds it, but its own segments and other models'.  Eight lanes per node, one cell
+// each; a lane appends to the node's row (node_ncand cleared by i7_search_reset), so a row
+// is in no particular order -- the filter keeps the penetrated ones in segment order.
 __global__ __launch_bounds__(256) void i7_node_search(Contact7 c, const double4 *X4) {
   if (!c.counters[I7C_SEARCH]) return;
   const long t = (long)blockIdx.x * 256 + (long)threadIdx.x;

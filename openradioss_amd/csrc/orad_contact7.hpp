@@ -10,7 +10,7 @@
 //               now was within TZ + margin then), otherwise the last candidates stand;
 //   search      segments binned on a grid by their box centre (per-model cell size,
 //               cells hashed with the model); each secondary node tests the segments of
-//               the 2 x 2 x 2 cells around it with I7TRI's box test widened by TZ;
+//               the 2 x 2 x 2 cells around it with I7TRI's box test widened by TZS;
 //               a second kernel filters each node's candidates with I7COR3 + I7DST3
 //               and keeps the penetrated ones in increasing segment order, so the
 //               penetrating pairs come out in (secondary node, segment) order without
