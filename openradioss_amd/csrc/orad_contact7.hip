@@ -666,192 +666,197 @@ __global__ __launch_bounds__(I7_NVSIZ) void i7_batch(Contact7 c, const double4 *
   __shared__ double s_rd[I7_NVSIZ], s_ec[I7_NVSIZ], s_ev[I7_NVSIZ];
   __shared__ int s_cnt[5][2];
   const int nbt = c.model_bstart[c.nmodel];
-  const int b = (int)blockIdx.x;
-  if (b >= nbt) return;
-  // the batch's model: the last m with model_bstart[m] <= b
-  int mlo = 0, mhi = c.nmodel - 1;
-  while (mlo < mhi) {
-    const int mid = (mlo + mhi + 1) >> 1;
-    if (c.model_bstart[mid] <= b) mlo = mid;
-    else mhi = mid - 1;
-  }
-  const int model = mlo;
-  const int p0 = i7_pairs_before(c, c.nsv_off[model]) + (b - c.model_bstart[model]) * I7_NVSIZ;
-  const int p1 = i7_pairs_before(c, c.nsv_off[model + 1]);
-  const int t = (int)threadIdx.x;
-  const int r = p0 + t;
-  const bool live = r < p1;
-  const int nlive = p1 - p0 < I7_NVSIZ ? p1 - p0 : I7_NVSIZ;
-  const double dt1 = *(const double *)((const char *)dt1b + (size_t)model * stride);
-  unsigned long long *dt2t =
-      (unsigned long long *)((char *)dt2tb + (size_t)model * stride);
-  const double dt1inv = dt1 > 0.0 ? 1.0 / dt1 : 0.0;
-  double h[4] = {0.0, 0.0, 0.0, 0.0}, fxi[3] = {0.0, 0.0, 0.0}, stif = 0.0;
-  double rdist = 1.0e300, econtt = 0.0, econvt = 0.0;
-  int ix[4] = {0, 0, 0, 0}, nsvg = 0;
-  if (live) {
-    // the secondary node of pair r: the last i with node_kstart[i] <= r
-    int lo = 0, hi = c.nsn - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (c.node_kstart[mid] <= r) lo = mid;
-      else hi = mid - 1;
+  // a bounded grid strides over the batches (most of a large interface's capacity is
+  // unused); the workgroup's shared arrays are reused batch after batch
+  for (int b = (int)blockIdx.x; b < nbt; b += (int)gridDim.x) {
+    // the batch's model: the last m with model_bstart[m] <= b
+    int mlo = 0, mhi = c.nmodel - 1;
+    while (mlo < mhi) {
+      const int mid = (mlo + mhi + 1) >> 1;
+      if (c.model_bstart[mid] <= b) mlo = mid;
+      else mhi = mid - 1;
     }
-    const int ni = lo;
-    const int l = c.node_keep[(size_t)ni * I7_KEEP_CAP + (r - c.node_kstart[ni])];
-    I7Pair q;
-    i7_dist(c, X4, V4, ni, l, q);
-    for (int j = 0; j < 4; ++j) ix[j] = q.ix[j];
-    nsvg = q.nsvg;
-    stif = q.stif;
-    const double g = q.gapv;
-    double pp[4], a[4], n[3], pene;
-    for (int j = 0; j < 4; ++j) {
-      const double dk = sqrt(q.p[j]);
-      pp[j] = dmax(0.0, g - dk);
-      a[j] = pp[j] / dmax(c.em20, dk);
-    }
-    for (int d = 0; d < 3; ++d)
-      n[d] = a[0] * q.nx[0][d] + a[1] * q.nx[1][d] + a[2] * q.nx[2][d] + a[3] * q.nx[3][d];
-    if (q.ix[2] != q.ix[3]) {
-      pene = dmax(dmax(dmax(pp[0], pp[1]), pp[2]), pp[3]);
-      double la[4];
-      for (int j = 0; j < 4; ++j) la[j] = 1.0 - q.lb[j] - q.lc[j];
-      const double h0 = 0.25 * (pp[0] * la[0] + pp[1] * la[1] + pp[2] * la[2] + pp[3] * la[3]);
-      h[0] = h0 + pp[0] * q.lb[0] + pp[3] * q.lc[3];
-      h[1] = h0 + pp[1] * q.lb[1] + pp[0] * q.lc[0];
-      h[2] = h0 + pp[2] * q.lb[2] + pp[1] * q.lc[1];
-      h[3] = h0 + pp[3] * q.lb[3] + pp[2] * q.lc[2];
-      const double h00 = 1.0 / dmax(c.em20, h[0] + h[1] + h[2] + h[3]);
-      for (int j = 0; j < 4; ++j) h[j] = h[j] * h00;
-    } else {
-      pene = pp[0];
-      for (int d = 0; d < 3; ++d) n[d] = q.nx[0][d];
-      h[0] = q.lb[0];
-      h[1] = q.lc[0];
-      h[2] = 1.0 - q.lb[0] - q.lc[0];
-      h[3] = 0.0;
-    }
-    const double s2 = 1.0 / dmax(c.em30, sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]));
-    for (int d = 0; d < 3; ++d) n[d] = n[d] * s2;
-    double vx[3];
-    for (int d = 0; d < 3; ++d) {
-      const double v0 = d == 0 ? V4[ix[0]].x : (d == 1 ? V4[ix[0]].y : V4[ix[0]].z);
-      const double v1 = d == 0 ? V4[ix[1]].x : (d == 1 ? V4[ix[1]].y : V4[ix[1]].z);
-      const double v2 = d == 0 ? V4[ix[2]].x : (d == 1 ? V4[ix[2]].y : V4[ix[2]].z);
-      const double v3 = d == 0 ? V4[ix[3]].x : (d == 1 ? V4[ix[3]].y : V4[ix[3]].z);
-      vx[d] = q.vi[d] - h[0] * v0 - h[1] * v1 - h[2] * v2 - h[3] * v3;
-    }
-    const double vn = n[0] * vx[0] + n[1] * vx[1] + n[2] * vx[2];
-    double hc = -0.25 * (h[0] - h[1] + h[2] - h[3]);
-    hc = dmin(dmin(hc, h[1]), h[3]);
-    hc = dmax(dmax(hc, -h[0]), -h[2]);
-    if (q.ix[2] == q.ix[3]) hc = 0.0;
-    h[0] = h[0] + hc;
-    h[1] = h[1] - hc;
-    h[2] = h[2] + hc;
-    h[3] = h[3] - hc;
-    // contact time step of the pair
-    rdist = 0.5 * (g - pene) / dmax(c.em30, -vn);
-    // nonlinear penalty
-    const double fac = g / dmax(c.em10, g - pene);
-    const double facm1 = 1.0 / fac;
-    if ((g - pene) / g < c.em10 && stif > 0.0) {
-      stif = 0.0;
-      c.stfns[ni] = -fabs(c.stfns[ni]);
-    }
-    econtt = 0.5 * stif * (g * g) * (facm1 - 1.0 - log(dmax(2.2250738585072014e-308, facm1)));
-    stif = 0.5 * stif * fac;
-    double fni = -stif * pene;
-    double vis2 = 0.0;
-    if (c.visc != 0.0 || c.viscf != 0.0) {
-      vis2 = 2.0 * stif * q.msi;
-      if (vn < 0.0) vis2 = vis2 / dmax(c.em10, (g - pene) / g);
-      const double fac2 = stif / dmax(c.em30, stif);
-      const double vis = sqrt(vis2);
-      double ff = fac2 * (c.visc * vis + c.zep4 * c.zep4 * 2.0 * q.msi * dmax(0.0, -vn) /
-                                              dmax(g - pene, c.em10));
-      stif = stif * g / dmax(g - pene, c.em10);
-      stif = stif + ff * dt1inv;
-      stif = dmax(stif, fac2 * sqrt(c.viscf) * vis * dt1inv);
-      ff = ff * vn;
-      fni = fni + ff;
-    } else {
-      stif = stif * g / dmax(g - pene, c.em10);
-    }
-    for (int d = 0; d < 3; ++d) fxi[d] = n[d] * fni;
-    // Coulomb friction, total viscous formulation
-    double tv[3];
-    for (int d = 0; d < 3; ++d) tv[d] = vx[d] - n[d] * vn;
-    const double v2 = tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2];
-    const double vis2f = c.viscf * vis2;
-    const double fm2 = (c.fric * fni) * (c.fric * fni);
-    const double f2 = vis2f * v2;
-    const double a2 = dmin(f2, fm2) / dmax(c.em30, f2);
-    const double aa = sqrt(a2 * vis2f);
-    for (int d = 0; d < 3; ++d) fxi[d] = fxi[d] + aa * tv[d];
-    econvt = aa * v2 * dt1;
-  }
-  s_rd[t] = rdist;
-  s_ec[t] = econtt;
-  s_ev[t] = econvt;
-  // role counts per wave (skyline layout of I7ASS2)
-  const int lane = t & 63, w = t >> 6;
-  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  unsigned long long m[4];
-  for (int j = 0; j < 4; ++j) {
-    m[j] = __ballot(live && h[j] != 0.0);
-    if (lane == 0) s_cnt[j][w] = __popcll(m[j]);
-  }
-  __syncthreads();
-  if (t == 0) {
-    // batch contact time step (i7for3.F:772-870) and energies, in pair order
-    double dti = (double)1.E20f, ec = 0.0, ev = 0.0;
-    for (int i = 0; i < nlive; ++i) dti = dmin(s_rd[i], dti);
-    if (dti <= 0.0) dti = (double)1.E20f;
-    atomicMin(dt2t, (unsigned long long)__double_as_longlong(dti));
-    for (int i = 0; i < nlive; ++i) ec = ec + s_ec[i];
-    for (int i = 0; i < nlive; ++i) ev = ev + s_ev[i];
-    c.batch_e[2 * b] = ec;
-    c.batch_e[2 * b + 1] = ev;
-    int tot = 0;
-    for (int j = 0; j < 4; ++j) tot += s_cnt[j][0] + s_cnt[j][1];
-    c.batch_cnt[b] = tot + nlive;
-    // the last batch to finish adds the batches' energies in batch order
-    // (ECONT = ECONT + ECONTT per I7FOR3 call)
-    __threadfence();
-    const int nb = nbt;
-    if (atomicAdd(&c.counters[I7C_DONE], 1) == nb - 1) {
-      __threadfence();
-      double et = c.energy[0], vt = c.energy[1];
-      int nsky = 0;
-      for (int q = 0; q < nb; ++q) {
-        et = et + __hip_atomic_load(&c.batch_e[2 * q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        vt = vt + __hip_atomic_load(&c.batch_e[2 * q + 1], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-        nsky += __hip_atomic_load(&c.batch_cnt[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int model = mlo;
+    const int p0 = i7_pairs_before(c, c.nsv_off[model]) + (b - c.model_bstart[model]) * I7_NVSIZ;
+    const int p1 = i7_pairs_before(c, c.nsv_off[model + 1]);
+    const int t = (int)threadIdx.x;
+    const int r = p0 + t;
+    const bool live = r < p1;
+    const int nlive = p1 - p0 < I7_NVSIZ ? p1 - p0 : I7_NVSIZ;
+    const double dt1 = *(const double *)((const char *)dt1b + (size_t)model * stride);
+    unsigned long long *dt2t =
+        (unsigned long long *)((char *)dt2tb + (size_t)model * stride);
+    const double dt1inv = dt1 > 0.0 ? 1.0 / dt1 : 0.0;
+    double h[4] = {0.0, 0.0, 0.0, 0.0}, fxi[3] = {0.0, 0.0, 0.0}, stif = 0.0;
+    double rdist = 1.0e300, econtt = 0.0, econvt = 0.0;
+    int ix[4] = {0, 0, 0, 0}, nsvg = 0;
+    if (live) {
+      // the secondary node of pair r: the last i with node_kstart[i] <= r
+      int lo = 0, hi = c.nsn - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (c.node_kstart[mid] <= r) lo = mid;
+        else hi = mid - 1;
       }
-      c.energy[0] = et;
-      c.energy[1] = vt;
-      c.counters[I7C_NSKY] = nsky;
+      const int ni = lo;
+      const int l = c.node_keep[(size_t)ni * I7_KEEP_CAP + (r - c.node_kstart[ni])];
+      I7Pair q;
+      i7_dist(c, X4, V4, ni, l, q);
+      for (int j = 0; j < 4; ++j) ix[j] = q.ix[j];
+      nsvg = q.nsvg;
+      stif = q.stif;
+      const double g = q.gapv;
+      double pp[4], a[4], n[3], pene;
+      for (int j = 0; j < 4; ++j) {
+        const double dk = sqrt(q.p[j]);
+        pp[j] = dmax(0.0, g - dk);
+        a[j] = pp[j] / dmax(c.em20, dk);
+      }
+      for (int d = 0; d < 3; ++d)
+        n[d] = a[0] * q.nx[0][d] + a[1] * q.nx[1][d] + a[2] * q.nx[2][d] + a[3] * q.nx[3][d];
+      if (q.ix[2] != q.ix[3]) {
+        pene = dmax(dmax(dmax(pp[0], pp[1]), pp[2]), pp[3]);
+        double la[4];
+        for (int j = 0; j < 4; ++j) la[j] = 1.0 - q.lb[j] - q.lc[j];
+        const double h0 = 0.25 * (pp[0] * la[0] + pp[1] * la[1] + pp[2] * la[2] + pp[3] * la[3]);
+        h[0] = h0 + pp[0] * q.lb[0] + pp[3] * q.lc[3];
+        h[1] = h0 + pp[1] * q.lb[1] + pp[0] * q.lc[0];
+        h[2] = h0 + pp[2] * q.lb[2] + pp[1] * q.lc[1];
+        h[3] = h0 + pp[3] * q.lb[3] + pp[2] * q.lc[2];
+        const double h00 = 1.0 / dmax(c.em20, h[0] + h[1] + h[2] + h[3]);
+        for (int j = 0; j < 4; ++j) h[j] = h[j] * h00;
+      } else {
+        pene = pp[0];
+        for (int d = 0; d < 3; ++d) n[d] = q.nx[0][d];
+        h[0] = q.lb[0];
+        h[1] = q.lc[0];
+        h[2] = 1.0 - q.lb[0] - q.lc[0];
+        h[3] = 0.0;
+      }
+      const double s2 = 1.0 / dmax(c.em30, sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]));
+      for (int d = 0; d < 3; ++d) n[d] = n[d] * s2;
+      double vx[3];
+      for (int d = 0; d < 3; ++d) {
+        const double v0 = d == 0 ? V4[ix[0]].x : (d == 1 ? V4[ix[0]].y : V4[ix[0]].z);
+        const double v1 = d == 0 ? V4[ix[1]].x : (d == 1 ? V4[ix[1]].y : V4[ix[1]].z);
+        const double v2 = d == 0 ? V4[ix[2]].x : (d == 1 ? V4[ix[2]].y : V4[ix[2]].z);
+        const double v3 = d == 0 ? V4[ix[3]].x : (d == 1 ? V4[ix[3]].y : V4[ix[3]].z);
+        vx[d] = q.vi[d] - h[0] * v0 - h[1] * v1 - h[2] * v2 - h[3] * v3;
+      }
+      const double vn = n[0] * vx[0] + n[1] * vx[1] + n[2] * vx[2];
+      double hc = -0.25 * (h[0] - h[1] + h[2] - h[3]);
+      hc = dmin(dmin(hc, h[1]), h[3]);
+      hc = dmax(dmax(hc, -h[0]), -h[2]);
+      if (q.ix[2] == q.ix[3]) hc = 0.0;
+      h[0] = h[0] + hc;
+      h[1] = h[1] - hc;
+      h[2] = h[2] + hc;
+      h[3] = h[3] - hc;
+      // contact time step of the pair
+      rdist = 0.5 * (g - pene) / dmax(c.em30, -vn);
+      // nonlinear penalty
+      const double fac = g / dmax(c.em10, g - pene);
+      const double facm1 = 1.0 / fac;
+      if ((g - pene) / g < c.em10 && stif > 0.0) {
+        stif = 0.0;
+        c.stfns[ni] = -fabs(c.stfns[ni]);
+      }
+      econtt = 0.5 * stif * (g * g) * (facm1 - 1.0 - log(dmax(2.2250738585072014e-308, facm1)));
+      stif = 0.5 * stif * fac;
+      double fni = -stif * pene;
+      double vis2 = 0.0;
+      if (c.visc != 0.0 || c.viscf != 0.0) {
+        vis2 = 2.0 * stif * q.msi;
+        if (vn < 0.0) vis2 = vis2 / dmax(c.em10, (g - pene) / g);
+        const double fac2 = stif / dmax(c.em30, stif);
+        const double vis = sqrt(vis2);
+        double ff = fac2 * (c.visc * vis + c.zep4 * c.zep4 * 2.0 * q.msi * dmax(0.0, -vn) /
+                                                dmax(g - pene, c.em10));
+        stif = stif * g / dmax(g - pene, c.em10);
+        stif = stif + ff * dt1inv;
+        stif = dmax(stif, fac2 * sqrt(c.viscf) * vis * dt1inv);
+        ff = ff * vn;
+        fni = fni + ff;
+      } else {
+        stif = stif * g / dmax(g - pene, c.em10);
+      }
+      for (int d = 0; d < 3; ++d) fxi[d] = n[d] * fni;
+      // Coulomb friction, total viscous formulation
+      double tv[3];
+      for (int d = 0; d < 3; ++d) tv[d] = vx[d] - n[d] * vn;
+      const double v2 = tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2];
+      const double vis2f = c.viscf * vis2;
+      const double fm2 = (c.fric * fni) * (c.fric * fni);
+      const double f2 = vis2f * v2;
+      const double a2 = dmin(f2, fm2) / dmax(c.em30, f2);
+      const double aa = sqrt(a2 * vis2f);
+      for (int d = 0; d < 3; ++d) fxi[d] = fxi[d] + aa * tv[d];
+      econvt = aa * v2 * dt1;
     }
-  }
-  if (!live) return;
-  const int e0 = b * I7_SLOTS;
-  int base = 0;
-  for (int j = 0; j < 4; ++j) {
-    if (h[j] != 0.0) {
-      const int e = e0 + base + (w ? s_cnt[j][0] : 0) + __popcll(m[j] & lt);
-      c.sky[e] = make_double4(fxi[0] * h[j], fxi[1] * h[j], fxi[2] * h[j], stif * fabs(h[j]));
-      c.sky_node[e] = ix[j];
-      c.sky_next[e] = atomicExch(&c.node_head[ix[j]], e);
+    s_rd[t] = rdist;
+    s_ec[t] = econtt;
+    s_ev[t] = econvt;
+    // role counts per wave (skyline layout of I7ASS2)
+    const int lane = t & 63, w = t >> 6;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long m[4];
+    for (int j = 0; j < 4; ++j) {
+      m[j] = __ballot(live && h[j] != 0.0);
+      if (lane == 0) s_cnt[j][w] = __popcll(m[j]);
     }
-    base += s_cnt[j][0] + s_cnt[j][1];
+    __syncthreads();
+    if (t == 0) {
+      // batch contact time step (i7for3.F:772-870) and energies, in pair order
+      double dti = (double)1.E20f, ec = 0.0, ev = 0.0;
+      for (int i = 0; i < nlive; ++i) dti = dmin(s_rd[i], dti);
+      if (dti <= 0.0) dti = (double)1.E20f;
+      atomicMin(dt2t, (unsigned long long)__double_as_longlong(dti));
+      for (int i = 0; i < nlive; ++i) ec = ec + s_ec[i];
+      for (int i = 0; i < nlive; ++i) ev = ev + s_ev[i];
+      c.batch_e[2 * b] = ec;
+      c.batch_e[2 * b + 1] = ev;
+      int tot = 0;
+      for (int j = 0; j < 4; ++j) tot += s_cnt[j][0] + s_cnt[j][1];
+      c.batch_cnt[b] = tot + nlive;
+      // the last batch to finish adds the batches' energies in batch order
+      // (ECONT = ECONT + ECONTT per I7FOR3 call)
+      __threadfence();
+      const int nb = nbt;
+      if (atomicAdd(&c.counters[I7C_DONE], 1) == nb - 1) {
+        __threadfence();
+        double et = c.energy[0], vt = c.energy[1];
+        int nsky = 0;
+        for (int q = 0; q < nb; ++q) {
+          et = et + __hip_atomic_load(&c.batch_e[2 * q], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+          vt = vt + __hip_atomic_load(&c.batch_e[2 * q + 1], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+          nsky += __hip_atomic_load(&c.batch_cnt[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        c.energy[0] = et;
+        c.energy[1] = vt;
+        c.counters[I7C_NSKY] = nsky;
+      }
+    }
+    if (live) {
+      const int e0 = b * I7_SLOTS;
+      int base = 0;
+      for (int j = 0; j < 4; ++j) {
+        if (h[j] != 0.0) {
+          const int e = e0 + base + (w ? s_cnt[j][0] : 0) + __popcll(m[j] & lt);
+          c.sky[e] = make_double4(fxi[0] * h[j], fxi[1] * h[j], fxi[2] * h[j], stif * fabs(h[j]));
+          c.sky_node[e] = ix[j];
+          c.sky_next[e] = atomicExch(&c.node_head[ix[j]], e);
+        }
+        base += s_cnt[j][0] + s_cnt[j][1];
+      }
+      const int e = e0 + base + t;
+      c.sky[e] = make_double4(-fxi[0], -fxi[1], -fxi[2], stif);
+      c.sky_node[e] = nsvg;
+      c.sky_next[e] = atomicExch(&c.node_head[nsvg], e);
+    }
+    __syncthreads();
   }
-  const int e = e0 + base + t;
-  c.sky[e] = make_double4(-fxi[0], -fxi[1], -fxi[2], stif);
-  c.sky_node[e] = nsvg;
-  c.sky_next[e] = atomicExch(&c.node_head[nsvg], e);
 }
 
 hipError_t contact7_run(const Contact7 &c, const double4 *X4, const double4 *V4,
@@ -881,7 +886,9 @@ hipError_t contact7_run(const Contact7 &c, const double4 *X4, const double4 *V4,
   e = scan_excl(c.node_nkeep, c.node_kstart, nullptr, c.nsn, c.nsn, c.scan_tmp, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(i7_model_batches, dim3(1), dim3(1024), 0, s, c);
-  hipLaunchKernelGGL(i7_batch, dim3(c.nbatch_cap > 0 ? c.nbatch_cap : 1), dim3(I7_NVSIZ), 0, s,
+  hipLaunchKernelGGL(i7_batch,
+                     dim3(c.nbatch_cap < 1 ? 1 : c.nbatch_cap > 2048 ? 2048 : c.nbatch_cap),
+                     dim3(I7_NVSIZ), 0, s,
                      c, X4, V4, dt1, dt2t, stride);
   return hipGetLastError();
 }
